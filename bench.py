@@ -1,0 +1,188 @@
+"""Benchmark: matrix-free FP64 apply of 3D Poisson, 64^3 cells, p = 4.
+
+Workload (the headline metric "MF apply GDOF/s"): [0,1]^3 built as six
+uniform all-axis refinements of one root cell (64^3 = 262,144 cells),
+FullTensorBasis degree 4, Gauss 5^3 points per cell, kappa = 1 + 0.5
+sin(2 pi x) sin(2 pi y) sin(2 pi z), zero Dirichlet on all faces:
+16,974,593 full-layout dofs. One step = one full-layout apply (cell pass +
+constraint rows) of a seeded N(0,1) vector resident in HBM.
+
+Prints one JSON line. `roofline` is the dominant kernel (the cell pass)
+against MI355X's 8 TB/s HBM peak using the algorithmic bytes
+8 N_full (read x) + 8 N_full (write y) + 8 n_cells n_q (per-point
+coefficients); its duration is measured with HIP events on the launch
+stream. `cpu_baseline` times the oracle (the reference algorithm restated
+in C, OpenMP over DOF-disjoint colors) on a bounded 16^3-cell sample.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "grico.jl_amd"), os.path.join(ROOT, "oracle")]
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(depth, degree=4):
+    import numpy as np
+
+    import gricoamd as g
+
+    class Diffusion:
+        def cell_accumulate(self, q, trial, test_component):
+            x, y, z = q.point
+            k = 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
+            return g.TestChannels(0.0 * trial.value, tuple(k * c for c in trial.gradient))
+
+    domain = g.Domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (1, 1, 1))
+    g.uniform_refine(domain, depth)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Diffusion())
+    for axis in range(3):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return problem
+
+
+def cpu_baseline(budget_s=12.0):
+    """Oracle apply on a 16^3-cell p=4 sample, all host threads, GDOF/s."""
+    import numpy as np
+
+    import gricoamd as g
+    import pyoracle
+
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(int(threads), 64))
+    plan = g.compile(build_problem(4))
+    mats = [pyoracle.action_matrices(plan, bk) for bk in plan.buckets]
+    cols = [pyoracle.color_batches(plan, pyoracle._Bucket(bk)) for bk in plan.buckets]
+    x = np.random.default_rng(1).standard_normal(plan.full_dof_count)
+    pyoracle.apply_lowered(plan, x, threads=threads, matrices=mats, colors=cols)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        pyoracle.apply_lowered(plan, x, threads=threads, matrices=mats, colors=cols)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    per = el / reps
+    return {
+        "value": plan.full_dof_count / per / 1e9,
+        "unit": "GDOF/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle full-layout apply, 16^3 cells p=4 ({plan.full_dof_count} dofs), "
+                  f"{reps} reps in {el:.1f}s, {threads} OpenMP threads over DOF-disjoint colors",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--depth", type=int, default=6, help="uniform refinements of the root (6 -> 64^3 cells)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (cells would shard by Morton "
+              "range with an interface-dof exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import gricoamd as g
+    from gricoamd import _devabi, _native, ops
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    t0 = time.perf_counter()
+    plan = g.compile(build_problem(args.depth))
+    log(f"host compile: {time.perf_counter() - t0:.1f}s, dofs {plan.full_dof_count}, "
+        f"cells {sum(b.cell_count for b in plan.buckets)}")
+    t0 = time.perf_counter()
+    dev = ops._ws(plan)
+    log(f"device upload: {time.perf_counter() - t0:.1f}s, {dev.device_bytes() / 2**20:.0f} MiB")
+
+    n = plan.full_dof_count
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(20241203)
+    x = torch.randn(n, dtype=torch.float64, device="cuda", generator=gen)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    dev.set_stream(stream.cuda_stream)
+    lib = _native.device()
+    for _ in range(args.warmup):
+        _devabi.check(lib.gmf_apply_async(dev.ws, x.data_ptr(), y.data_ptr()))
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        _devabi.check(lib.gmf_apply_async(dev.ws, x.data_ptr(), y.data_ptr()))
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+
+    cell_ms = ctypes.c_double()
+    total_ms = ctypes.c_double()
+    _devabi.check(lib.gmf_apply_timed(dev.ws, x.data_ptr(), y.data_ptr(), max(5, args.steps // 2),
+                                      ctypes.byref(cell_ms), ctypes.byref(total_ms)))
+    cells = sum(b.cell_count for b in plan.buckets)
+    nq = plan.buckets[0].n_points
+    k = 1 if plan.buckets[0].form == 0 else 16
+    alg_bytes = 8 * n + 8 * n + 8 * cells * nq * k
+    achieved = alg_bytes / (cell_ms.value * 1e-3) / 1e9
+    log(f"apply {ms * 1e3:.1f} us/step; cell pass {cell_ms.value * 1e3:.1f} us, timed apply "
+        f"{total_ms.value * 1e3:.1f} us; algorithmic bytes {alg_bytes / 1e6:.1f} MB")
+
+    result = {
+        "metric": "MF apply GDOF/s (%HBM peak)",
+        "value": n / (ms * 1e-3) / 1e9,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) input vector)",
+        "config": {
+            "workload": f"3D Poisson [0,1]^3, {2 ** args.depth}^3 cells (uniform refinement of one root), "
+                        f"p=4 FullTensorBasis, 5^3 Gauss points, kappa=1+0.5 sin sin sin, zero Dirichlet; "
+                        f"full-layout matrix-free apply",
+            "full_dofs": n,
+            "cells": cells,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,266 @@
+// Generic tensor-product cell kernels: any dimension <= 3, any per-axis mode
+// and point counts <= 16, both lowered action forms. One workgroup (one
+// wavefront) per cell, every intermediate staged in LDS.
+//
+// The algorithm is the reference's interpolate -> per-point action -> project
+// pipeline (src/accumulators.jl:1955-2013 with the sum-factorized helpers of
+// src/integration.jl:1140-1399): D passes per channel forward, D transposed
+// passes per channel back, gather / scatter through the term maps
+// (src/assembly.jl:1751-1803). The diagonal kernel evaluates the reference's
+// cell_diagonal! (src/accumulators.jl:1056-1083) by sum factorization with
+// products of 1D tables instead of one basis probe per mode.
+#include "common.hpp"
+
+namespace gmf {
+namespace {
+
+struct GenericArgs {
+  int D;
+  int N[kMaxDim], Q[kMaxDim];
+  int Ntot, Qtot, M;
+  int form, pstride;
+  int toff[kMaxDim];
+  const double* values;
+  const double* derivs;
+  const double* jinv;
+  const double* pdata;
+  const int32_t* idx;
+  const double* lcoef;
+  const int64_t* moff;
+  const int32_t* midx;
+  const double* mcoef;
+};
+
+// One sum-factorization pass over the block. `shape` holds the input extents
+// (axis 0 fastest); the pass replaces the extent of `axis` by `out_ext`.
+// Forward (modes -> points): out[.., q, ..] = sum_k T[k][q] in[.., k, ..].
+// Backward (points -> modes): out[.., i, ..] = sum_q T[i][q] in[.., q, ..].
+// T2, when given, multiplies T elementwise (products of tables).
+__device__ void contract(const double* __restrict__ in, double* __restrict__ out, int D, const int* shape,
+                         int axis, int out_ext, const double* T, const double* T2, int tq, bool backward) {
+  const int in_ext = shape[axis];
+  int stride = 1, outer = 1;
+  for (int a = 0; a < D; ++a) {
+    if (a < axis) stride *= shape[a];
+    if (a > axis) outer *= shape[a];
+  }
+  const int total = stride * out_ext * outer;
+  for (int o = threadIdx.x; o < total; o += blockDim.x) {
+    const int lo = o % stride;
+    const int t = o / stride;
+    const int j = t % out_ext;
+    const int hi = t / out_ext;
+    const double* src = in + lo + hi * stride * in_ext;
+    double acc = 0.0;
+    for (int k = 0; k < in_ext; ++k) {
+      const int ti = backward ? j * tq + k : k * tq + j;
+      double w = T[ti];
+      if (T2) w *= T2[ti];
+      acc = fma(w, src[k * stride], acc);
+    }
+    out[o] = acc;
+  }
+}
+
+// Evaluate the coefficient box `u` at the points with derivative along
+// `daxis` (-1: value). Result in `dst` (Qtot entries). bufA/bufB: scratch.
+__device__ void interpolate(const GenericArgs& a, const double* u, double* dst, double* bufA, double* bufB,
+                            int daxis) {
+  int shape[kMaxDim] = {1, 1, 1};
+  for (int d = 0; d < a.D; ++d) shape[d] = a.N[d];
+  const double* cur = u;
+  for (int d = 0; d < a.D; ++d) {
+    double* nxt = (d == a.D - 1) ? dst : ((d & 1) ? bufB : bufA);
+    const double* T = (d == daxis ? a.derivs : a.values) + a.toff[d];
+    contract(cur, nxt, a.D, shape, d, a.Q[d], T, nullptr, a.Q[d], false);
+    shape[d] = a.Q[d];
+    __syncthreads();
+    cur = nxt;
+  }
+}
+
+// Accumulate the projection of point data `f` (Qtot) into r (Ntot):
+// r[i] += sum_q f[q] prod_d T1_d[i_d][q_d] (T2_d...). T1 uses the derivative
+// table on axis `daxis1`, T2 (if use_t2) on `daxis2`.
+__device__ void project_add(const GenericArgs& a, const double* f, double* r, double* bufA, double* bufB,
+                            int daxis1, bool use_t2, int daxis2) {
+  int shape[kMaxDim] = {1, 1, 1};
+  for (int d = 0; d < a.D; ++d) shape[d] = a.Q[d];
+  const double* cur = f;
+  for (int d = 0; d < a.D; ++d) {
+    double* nxt = (d & 1) ? bufB : bufA;
+    const double* T1 = (d == daxis1 ? a.derivs : a.values) + a.toff[d];
+    const double* T2 = use_t2 ? (d == daxis2 ? a.derivs : a.values) + a.toff[d] : nullptr;
+    contract(cur, nxt, a.D, shape, d, a.N[d], T1, T2, a.Q[d], true);
+    shape[d] = a.N[d];
+    __syncthreads();
+    cur = nxt;
+  }
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] += cur[i];
+  __syncthreads();
+}
+
+__device__ double gather_one(const GenericArgs& a, int64_t cell, int i, const double* __restrict__ x) {
+  const int32_t g = a.idx[cell * a.Ntot + i];
+  if (g >= 0) return a.lcoef ? a.lcoef[cell * a.Ntot + i] * x[g] : x[g];
+  if (g == -1) return 0.0;
+  const int64_t m = -int64_t(g) - 2;
+  double v = 0.0;
+  for (int64_t t = a.moff[m]; t < a.moff[m + 1]; ++t) {
+    const int32_t h = a.midx[t];
+    if (h >= 0) v = fma(a.mcoef[t], x[h], v);
+  }
+  return v;
+}
+
+__device__ void scatter_one(const GenericArgs& a, int64_t cell, int i, double v, double* __restrict__ y,
+                            bool squared) {
+  const int32_t g = a.idx[cell * a.Ntot + i];
+  if (g >= 0) {
+    double c = a.lcoef ? a.lcoef[cell * a.Ntot + i] : 1.0;
+    unsafeAtomicAdd(&y[g], (squared ? c * c : c) * v);
+    return;
+  }
+  if (g == -1) return;
+  const int64_t m = -int64_t(g) - 2;
+  for (int64_t t = a.moff[m]; t < a.moff[m + 1]; ++t) {
+    const int32_t h = a.midx[t];
+    if (h < 0) continue;
+    const double c = a.mcoef[t];
+    unsafeAtomicAdd(&y[h], (squared ? c * c : c) * v);
+  }
+}
+
+__global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* u = smem;
+  double* bufA = u + a.M;
+  double* bufB = bufA + a.M;
+  double* fields = bufB + a.M;          // (D+1) x Qtot
+  double* r = fields + (a.D + 1) * a.Qtot;  // Ntot
+
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+    u[i] = gather_one(a, cell, i, x);
+    r[i] = 0.0;
+  }
+  __syncthreads();
+  const bool general = a.form == GMF_FORM_GENERAL;
+  if (general) interpolate(a, u, fields, bufA, bufB, -1);
+  for (int d = 0; d < a.D; ++d) interpolate(a, u, fields + (d + 1) * a.Qtot, bufA, bufB, d);
+
+  const double* jinv = a.jinv + cell * a.D;
+  if (!general) {
+    for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) {
+      const double t = a.pdata[cell * a.Qtot + q];
+      for (int d = 0; d < a.D; ++d) {
+        double* f = fields + (d + 1) * a.Qtot;
+        f[q] = t * jinv[d] * (jinv[d] * f[q]);
+      }
+    }
+  } else {
+    const int C = a.D + 1;
+    for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) {
+      double ch[kMaxDim + 1];
+      ch[0] = fields[q];
+      for (int d = 0; d < a.D; ++d) ch[d + 1] = jinv[d] * fields[(d + 1) * a.Qtot + q];
+      const double* Mq = a.pdata + (cell * a.Qtot + q) * a.pstride;
+      for (int c = 0; c < C; ++c) {
+        double acc = 0.0;
+        for (int k = 0; k < C; ++k) acc = fma(Mq[c * C + k], ch[k], acc);
+        fields[c * a.Qtot + q] = (c == 0) ? acc : acc * jinv[c - 1];
+      }
+    }
+  }
+  __syncthreads();
+  if (general) project_add(a, fields, r, bufA, bufB, -1, false, -1);
+  for (int d = 0; d < a.D; ++d) project_add(a, fields + (d + 1) * a.Qtot, r, bufA, bufB, d, false, -1);
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
+}
+
+__global__ void __launch_bounds__(64) cell_diagonal_generic(GenericArgs a, double* __restrict__ diag) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* bufA = smem;
+  double* bufB = bufA + a.M;
+  double* w = bufB + a.M;   // Qtot
+  double* r = w + a.Qtot;   // Ntot
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] = 0.0;
+  const double* jinv = a.jinv + cell * a.D;
+  __syncthreads();
+  if (a.form != GMF_FORM_GENERAL) {
+    for (int d = 0; d < a.D; ++d) {
+      for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x)
+        w[q] = a.pdata[cell * a.Qtot + q] * jinv[d] * jinv[d];
+      __syncthreads();
+      project_add(a, w, r, bufA, bufB, d, true, d);
+    }
+  } else {
+    const int C = a.D + 1;
+    for (int c = 0; c < C; ++c)
+      for (int k = 0; k < C; ++k) {
+        for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) {
+          double s = a.pdata[(cell * a.Qtot + q) * a.pstride + c * C + k];
+          if (c > 0) s *= jinv[c - 1];
+          if (k > 0) s *= jinv[k - 1];
+          w[q] = s;
+        }
+        __syncthreads();
+        project_add(a, w, r, bufA, bufB, c - 1, true, k - 1);
+      }
+  }
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], diag, true);
+}
+
+GenericArgs make_args(const Bucket& b, bool reduced) {
+  GenericArgs a{};
+  a.D = b.dim;
+  a.M = 1;
+  for (int d = 0; d < kMaxDim; ++d) {
+    a.N[d] = b.N[d];
+    a.Q[d] = b.Q[d];
+    a.toff[d] = b.table_offset[d];
+  }
+  for (int d = 0; d < b.dim; ++d) a.M *= (b.N[d] > b.Q[d] ? b.N[d] : b.Q[d]);
+  a.Ntot = b.Ntot;
+  a.Qtot = b.Qtot;
+  a.form = b.form;
+  a.pstride = b.point_stride;
+  a.values = b.values;
+  a.derivs = b.derivs;
+  a.jinv = b.jinv;
+  a.pdata = b.pdata;
+  a.idx = reduced ? b.ridx : b.lidx;
+  a.lcoef = b.lcoef;
+  a.moff = b.moff;
+  a.midx = reduced ? b.mridx : b.midx;
+  a.mcoef = b.mcoef;
+  return a;
+}
+
+}  // namespace
+
+void launch_cell_generic(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,
+                         hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, reduced);
+  require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  size_t lds = sizeof(double) * size_t(3 * a.M + (a.D + 1) * a.Qtot + a.Ntot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_apply_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, x, y);
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cell_diagonal_generic(const gmf_plan&, const Bucket& b, double* diag, bool reduced,
+                                  hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, reduced);
+  require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  size_t lds = sizeof(double) * size_t(2 * a.M + a.Qtot + a.Ntot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_diagonal_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, diag);
+  GMF_HIP(hipGetLastError());
+}
+
+}  // namespace gmf

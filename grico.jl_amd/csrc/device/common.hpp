@@ -1,0 +1,123 @@
+// Shared internals of the device library: plan / workspace layout, error
+// plumbing and launch helpers. All device code targets gfx950 (wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "grico_mf.h"
+
+namespace gmf {
+
+struct Error : std::runtime_error {
+  gmf_status status;
+  Error(gmf_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define GMF_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t _e = (call);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      throw ::gmf::Error(GMF_EDEVICE, std::string(#call) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+inline void require(bool ok, gmf_status s, const std::string& msg) {
+  if (!ok) throw Error(s, msg);
+}
+
+constexpr int kMaxDim = 3;
+constexpr int kMaxExtent = 16;  // per-axis modes / points handled by the generic kernel
+
+// Device-resident cell batch.
+struct Bucket {
+  int dim = 0;
+  int N[kMaxDim] = {1, 1, 1};
+  int Q[kMaxDim] = {1, 1, 1};
+  int form = GMF_FORM_DIFFUSION;
+  int64_t cells = 0;
+  int Ntot = 1, Qtot = 1;
+  int point_stride = 1;  // coefficients per point (1 or (D+1)^2)
+  double* values = nullptr;   // per-axis tables, concatenated
+  double* derivs = nullptr;
+  double* jinv = nullptr;     // cells x dim
+  double* pdata = nullptr;    // cells x Qtot x point_stride
+  int32_t* lidx = nullptr;    // cells x Ntot (full layout)
+  int32_t* ridx = nullptr;    // cells x Ntot (reduced layout; only for selection maps)
+  double* lcoef = nullptr;    // cells x Ntot or null
+  int64_t multi_count = 0;
+  int64_t* moff = nullptr;
+  int32_t* midx = nullptr;    // full layout
+  int32_t* mridx = nullptr;   // reduced layout (-1 = fixed)
+  double* mcoef = nullptr;
+  int table_offset[kMaxDim] = {0, 0, 0};  // offset of axis a in values/derivs
+  std::vector<double> h_values, h_derivs;
+  // Fast-path data (3D, N == Q, diffusion): collocation derivative matrix.
+  double* colloc = nullptr;  // Q x Q, colloc[q][s] = l_s'(xi_q)
+  bool fast = false;
+};
+
+struct Csr {
+  int64_t rows = 0;
+  int64_t* offsets = nullptr;
+  int64_t* indices = nullptr;
+  double* coefficients = nullptr;
+};
+
+}  // namespace gmf
+
+struct gmf_plan {
+  int device = 0;
+  int dim = 0;
+  int64_t n_full = 0;
+  int64_t n_red = 0;
+  std::vector<gmf::Bucket> buckets;
+  int64_t fixed_count = 0;
+  int64_t* fixed = nullptr;
+  int64_t affine_count = 0;
+  int64_t* affine_pivots = nullptr;
+  gmf::Csr affine;
+  int64_t mean_count = 0;
+  int64_t* mean_pivots = nullptr;
+  gmf::Csr mean;
+  gmf::Csr map;      // full rows -> reduced
+  gmf::Csr map_t;    // reduced rows -> full (transpose, for P^T)
+  double* shift = nullptr;
+  bool selection_map = false;  // P is a 0/1 selection without shift terms in apply
+  bool multi_terms = false;
+  int64_t device_bytes = 0;
+  std::vector<void*> allocations;
+};
+
+struct gmf_workspace {
+  gmf_plan* plan = nullptr;
+  hipStream_t stream = nullptr;
+  double* full_in = nullptr;
+  double* full_out = nullptr;
+  double* partials = nullptr;   // reduction partial sums
+  double* scalars = nullptr;    // device scalars for solvers
+  double* host_scalars = nullptr;  // pinned mirror
+  int64_t partial_capacity = 0;
+};
+
+namespace gmf {
+
+// Kernel launchers implemented in the .hip files.
+void launch_cell_generic(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
+                         hipStream_t stream);
+void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced,
+                                  hipStream_t stream);
+bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
+                      hipStream_t stream);
+void prepare_fast_path(gmf_plan& plan, Bucket& b);
+
+// Vector kernels (blas.hip).
+void fill(double* x, double v, int64_t n, hipStream_t s);
+void copy(double* dst, const double* src, int64_t n, hipStream_t s);
+// partial sums of x.y into ws->partials then final reduce into scalars[slot].
+void dot(gmf_workspace* ws, const double* x, const double* y, int64_t n, int slot);
+void read_scalars(gmf_workspace* ws, int count);
+
+}  // namespace gmf

@@ -1,0 +1,841 @@
+// C ABI of the device library (include/grico_mf.h): plan upload, workspaces,
+// operator application, reduced maps, diagonal, Jacobi, CG and FGMRES.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "blas.hpp"
+#include "common.hpp"
+
+struct gmf_preconditioner {
+  gmf_workspace* ws = nullptr;
+  virtual ~gmf_preconditioner() = default;
+  // z = M^-1 r on ws->stream.
+  virtual void apply(const double* r, double* z) = 0;
+  // z = M^-1 r and scalars[slot] = r.z (fused where the preconditioner allows).
+  virtual void apply_dot(const double* r, double* z, int64_t n, int slot) {
+    apply(r, z);
+    gmf::dot(ws, r, z, n, slot);
+  }
+};
+
+namespace gmf {
+namespace {
+
+thread_local std::string g_error;
+
+gmf_status fail(gmf_status s, const std::string& m) {
+  g_error = m;
+  return s;
+}
+
+template <class F>
+gmf_status guarded(F&& f) {
+  try {
+    f();
+    return GMF_OK;
+  } catch (const Error& e) {
+    return fail(e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(GMF_EDEVICE, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(GMF_EINVAL, e.what());
+  }
+}
+
+template <class T>
+T* upload(gmf_plan& p, const T* host, size_t n) {
+  if (n == 0 || host == nullptr) return nullptr;
+  void* d = nullptr;
+  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
+  p.allocations.push_back(d);
+  p.device_bytes += int64_t(n * sizeof(T));
+  GMF_HIP(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return static_cast<T*>(d);
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  void* d = nullptr;
+  if (n == 0) n = 1;
+  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
+  return static_cast<T*>(d);
+}
+
+void check_csr(int64_t rows, const int64_t* off, const int64_t* idx, int64_t bound, const char* what) {
+  require(rows == 0 || off != nullptr, GMF_EINVAL, std::string(what) + ": offsets missing");
+  if (rows == 0) return;
+  require(off[0] == 0, GMF_EINVAL, std::string(what) + ": offsets must start at 0");
+  for (int64_t r = 0; r < rows; ++r)
+    require(off[r + 1] >= off[r], GMF_EINVAL, std::string(what) + ": offsets must be non-decreasing");
+  for (int64_t t = 0; t < off[rows]; ++t)
+    require(idx[t] >= 0 && idx[t] < bound, GMF_EINVAL, std::string(what) + ": index out of range");
+}
+
+Csr upload_csr(gmf_plan& p, int64_t rows, const int64_t* off, const int64_t* idx, const double* coef) {
+  Csr c;
+  c.rows = rows;
+  if (rows == 0) return c;
+  c.offsets = upload(p, off, size_t(rows + 1));
+  c.indices = upload(p, idx, size_t(off[rows]));
+  c.coefficients = upload(p, coef, size_t(off[rows]));
+  return c;
+}
+
+// ---- small kernels ---------------------------------------------------------
+
+__global__ void k_fixed_rows(double* __restrict__ y, const double* __restrict__ x, const int64_t* __restrict__ rows,
+                             int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t d = rows[i];
+    y[d] = x[d];
+  }
+}
+
+// affine: y[p] = x[p] - sum c x[j]; mean: y[p] = sum c x[j].
+__global__ void k_linear_rows(double* __restrict__ y, const double* __restrict__ x, const int64_t* __restrict__ pivots,
+                              const int64_t* __restrict__ off, const int64_t* __restrict__ idx,
+                              const double* __restrict__ coef, int64_t n, int affine) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = pivots[i];
+    double v = affine ? x[p] : 0.0;
+    for (int64_t t = off[i]; t < off[i + 1]; ++t) v = affine ? fma(-coef[t], x[idx[t]], v) : fma(coef[t], x[idx[t]], v);
+    y[p] = v;
+  }
+}
+
+__global__ void k_csr_gather(double* __restrict__ out, const double* __restrict__ in, const int64_t* __restrict__ off,
+                             const int64_t* __restrict__ idx, const double* __restrict__ coef,
+                             const double* __restrict__ shift, int64_t rows) {
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < rows; r += int64_t(gridDim.x) * blockDim.x) {
+    double v = shift ? shift[r] : 0.0;
+    for (int64_t t = off[r]; t < off[r + 1]; ++t) v = fma(coef[t], in[idx[t]], v);
+    out[r] = v;
+  }
+}
+
+__global__ void k_invert_diagonal(const double* __restrict__ d, double* __restrict__ inv, int64_t n,
+                                  int* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double v = d[i];
+    if (!(v != 0.0) || !isfinite(v)) {
+      atomicOr(bad, 1);
+      inv[i] = 0.0;
+    } else {
+      inv[i] = 1.0 / v;
+    }
+  }
+}
+
+__global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x, const double* __restrict__ s, int slot,
+                            double sign, int64_t n) {
+  const double a = sign * s[slot];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = fma(a, x[i], y[i]);
+}
+
+inline unsigned grid1(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 4096)));
+}
+
+// ---- operator application ---------------------------------------------------
+
+void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
+  const gmf_plan& p = *ws->plan;
+  for (const Bucket& b : p.buckets) {
+    if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->stream)) continue;
+    launch_cell_generic(p, b, x, y, reduced, ws->stream);
+  }
+}
+
+void constraint_rows(gmf_workspace* ws, const double* x, double* y) {
+  const gmf_plan& p = *ws->plan;
+  if (p.fixed_count)
+    hipLaunchKernelGGL(k_fixed_rows, dim3(grid1(p.fixed_count)), dim3(256), 0, ws->stream, y, x, p.fixed,
+                       p.fixed_count);
+  if (p.affine_count)
+    hipLaunchKernelGGL(k_linear_rows, dim3(grid1(p.affine_count)), dim3(256), 0, ws->stream, y, x, p.affine_pivots,
+                       p.affine.offsets, p.affine.indices, p.affine.coefficients, p.affine_count, 1);
+  if (p.mean_count)
+    hipLaunchKernelGGL(k_linear_rows, dim3(grid1(p.mean_count)), dim3(256), 0, ws->stream, y, x, p.mean_pivots,
+                       p.mean.offsets, p.mean.indices, p.mean.coefficients, p.mean_count, 0);
+  GMF_HIP(hipGetLastError());
+}
+
+void apply_full(gmf_workspace* ws, const double* x, double* y) {
+  require(x != y, GMF_EALIAS, "result and coefficient vector must not alias");
+  fill(y, 0.0, ws->plan->n_full, ws->stream);
+  cell_pass(ws, x, y, false);
+  constraint_rows(ws, x, y);
+}
+
+void expand(gmf_workspace* ws, const double* xr, double* xf, bool shift) {
+  const gmf_plan& p = *ws->plan;
+  hipLaunchKernelGGL(k_csr_gather, dim3(grid1(p.n_full)), dim3(256), 0, ws->stream, xf, xr, p.map.offsets,
+                     p.map.indices, p.map.coefficients, shift ? p.shift : nullptr, p.n_full);
+  GMF_HIP(hipGetLastError());
+}
+
+void project(gmf_workspace* ws, const double* yf, double* yr) {
+  const gmf_plan& p = *ws->plan;
+  if (p.n_red == 0) return;
+  hipLaunchKernelGGL(k_csr_gather, dim3(grid1(p.n_red)), dim3(256), 0, ws->stream, yr, yf, p.map_t.offsets,
+                     p.map_t.indices, p.map_t.coefficients, nullptr, p.n_red);
+  GMF_HIP(hipGetLastError());
+}
+
+void reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
+  gmf_plan& p = *ws->plan;
+  require(xr != yr, GMF_EALIAS, "result and reduced coefficient vector must not alias");
+  if (p.selection_map) {
+    // P is a 0/1 selection: the cell kernels gather from and scatter to the
+    // reduced vector directly through reduced index maps.
+    fill(yr, 0.0, p.n_red, ws->stream);
+    cell_pass(ws, xr, yr, true);
+    return;
+  }
+  expand(ws, xr, ws->full_in, false);
+  fill(ws->full_out, 0.0, p.n_full, ws->stream);
+  cell_pass(ws, ws->full_in, ws->full_out, false);
+  project(ws, ws->full_out, yr);
+}
+
+void sync(gmf_workspace* ws) { GMF_HIP(hipStreamSynchronize(ws->stream)); }
+
+// ---- preconditioners ---------------------------------------------------------
+
+struct JacobiPrecond : gmf_preconditioner {
+  double* invdiag = nullptr;
+  int64_t n = 0;
+  ~JacobiPrecond() override {
+    if (invdiag) (void)hipFree(invdiag);
+  }
+  void apply(const double* r, double* z) override { jacobi_dot(ws, z, r, invdiag, n, kTmp); }
+  void apply_dot(const double* r, double* z, int64_t nn, int slot) override { jacobi_dot(ws, z, r, invdiag, nn, slot); }
+};
+
+void precond_apply_dot(gmf_workspace* ws, gmf_preconditioner* M, const double* r, double* z, int64_t n, int slot) {
+  if (M == nullptr) {
+    jacobi_dot(ws, z, r, nullptr, n, slot);
+  } else {
+    M->apply_dot(r, z, n, slot);
+  }
+}
+
+void precond_apply(gmf_workspace* ws, gmf_preconditioner* M, const double* r, double* z, int64_t n) {
+  if (M == nullptr)
+    copy(z, r, n, ws->stream);
+  else
+    M->apply(r, z);
+}
+
+// ---- workspace vectors for solvers ------------------------------------------
+
+struct SolverVectors {
+  double* r = nullptr;
+  double* z = nullptr;
+  double* p = nullptr;
+  double* q = nullptr;
+  double* V = nullptr;
+  double* Z = nullptr;
+  double* ydev = nullptr;
+  ~SolverVectors() {
+    for (double* v : {r, z, p, q, V, Z, ydev})
+      if (v) (void)hipFree(v);
+  }
+};
+
+}  // namespace
+}  // namespace gmf
+
+using namespace gmf;
+
+extern "C" {
+
+const char* gmf_last_error(void) { return g_error.c_str(); }
+
+gmf_status gmf_device_info(char* name, int32_t* cus, int64_t* mem) {
+  return guarded([&] {
+    int dev = 0;
+    GMF_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    GMF_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name) {
+      std::strncpy(name, prop.gcnArchName, 63);
+      name[63] = 0;
+    }
+    if (cus) *cus = prop.multiProcessorCount;
+    if (mem) *mem = int64_t(prop.totalGlobalMem);
+  });
+}
+
+gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
+  return guarded([&] {
+    require(desc != nullptr && out != nullptr, GMF_EINVAL, "null descriptor");
+    require(desc->dim >= 1 && desc->dim <= 3, GMF_EINVAL, "dimension must be 1, 2 or 3");
+    require(desc->full_dof_count >= 0 && desc->full_dof_count < (int64_t(1) << 31), GMF_EINVAL,
+            "full dof count must fit int32 indices");
+    auto plan = std::make_unique<gmf_plan>();
+    gmf_plan& p = *plan;
+    GMF_HIP(hipGetDevice(&p.device));
+    p.dim = desc->dim;
+    p.n_full = desc->full_dof_count;
+    p.n_red = desc->reduced_dof_count;
+    require(p.n_red >= 0 && p.n_red <= p.n_full, GMF_EINVAL, "reduced dof count out of range");
+    try {
+      // Reduced map (validated), selection detection, transpose.
+      check_csr(p.n_full, desc->map_offsets, desc->map_indices, p.n_red, "reduced map");
+      std::vector<int64_t> red_of_full(size_t(p.n_full), -1);
+      bool selection = true;
+      std::vector<int64_t> seen(size_t(p.n_red), 0);
+      for (int64_t d = 0; d < p.n_full; ++d) {
+        int64_t a = desc->map_offsets[d], e = desc->map_offsets[d + 1];
+        if (e - a > 1) selection = false;
+        if (e - a == 1) {
+          if (desc->map_coefficients[a] != 1.0) selection = false;
+          red_of_full[size_t(d)] = desc->map_indices[a];
+          seen[size_t(desc->map_indices[a])]++;
+        }
+      }
+      for (int64_t r = 0; r < p.n_red && selection; ++r) selection = seen[size_t(r)] == 1;
+      p.selection_map = selection;
+      p.map = upload_csr(p, p.n_full, desc->map_offsets, desc->map_indices, desc->map_coefficients);
+      p.shift = upload(p, desc->shift, size_t(p.n_full));
+      {
+        std::vector<int64_t> toff(size_t(p.n_red) + 1, 0), tidx(size_t(p.n_full ? desc->map_offsets[p.n_full] : 0));
+        std::vector<double> tcoef(tidx.size());
+        for (int64_t d = 0; d < p.n_full; ++d)
+          for (int64_t t = desc->map_offsets[d]; t < desc->map_offsets[d + 1]; ++t) toff[size_t(desc->map_indices[t]) + 1]++;
+        for (int64_t r = 0; r < p.n_red; ++r) toff[size_t(r) + 1] += toff[size_t(r)];
+        std::vector<int64_t> fillp(toff.begin(), toff.end() - 1);
+        for (int64_t d = 0; d < p.n_full; ++d)
+          for (int64_t t = desc->map_offsets[d]; t < desc->map_offsets[d + 1]; ++t) {
+            int64_t r = desc->map_indices[t];
+            tidx[size_t(fillp[size_t(r)])] = d;
+            tcoef[size_t(fillp[size_t(r)]++)] = desc->map_coefficients[t];
+          }
+        p.map_t = upload_csr(p, p.n_red, toff.data(), tidx.data(), tcoef.data());
+      }
+      // Constraint rows.
+      p.fixed_count = desc->fixed_count;
+      for (int64_t i = 0; i < p.fixed_count; ++i)
+        require(desc->fixed_dofs[i] >= 0 && desc->fixed_dofs[i] < p.n_full, GMF_EINVAL, "fixed dof out of range");
+      p.fixed = upload(p, desc->fixed_dofs, size_t(p.fixed_count));
+      p.affine_count = desc->affine_count;
+      if (p.affine_count) {
+        check_csr(p.affine_count, desc->affine_offsets, desc->affine_indices, p.n_full, "affine rows");
+        for (int64_t i = 0; i < p.affine_count; ++i)
+          require(desc->affine_pivots[i] >= 0 && desc->affine_pivots[i] < p.n_full, GMF_EINVAL, "affine pivot out of range");
+        p.affine_pivots = upload(p, desc->affine_pivots, size_t(p.affine_count));
+        p.affine = upload_csr(p, p.affine_count, desc->affine_offsets, desc->affine_indices, desc->affine_coefficients);
+      }
+      p.mean_count = desc->mean_count;
+      if (p.mean_count) {
+        check_csr(p.mean_count, desc->mean_offsets, desc->mean_indices, p.n_full, "mean rows");
+        for (int64_t i = 0; i < p.mean_count; ++i)
+          require(desc->mean_pivots[i] >= 0 && desc->mean_pivots[i] < p.n_full, GMF_EINVAL, "mean pivot out of range");
+        p.mean_pivots = upload(p, desc->mean_pivots, size_t(p.mean_count));
+        p.mean = upload_csr(p, p.mean_count, desc->mean_offsets, desc->mean_indices, desc->mean_coefficients);
+      }
+      // Cell batches.
+      require(desc->bucket_count >= 0 && (desc->bucket_count == 0 || desc->buckets), GMF_EINVAL, "bucket array missing");
+      for (int bi = 0; bi < desc->bucket_count; ++bi) {
+        const gmf_bucket_desc& bd = desc->buckets[bi];
+        Bucket b;
+        require(bd.dim == p.dim, GMF_EINVAL, "bucket dimension mismatch");
+        b.dim = bd.dim;
+        b.form = bd.form;
+        require(b.form == GMF_FORM_DIFFUSION || b.form == GMF_FORM_GENERAL, GMF_EINVAL, "unknown action form");
+        b.cells = bd.cell_count;
+        require(b.cells >= 0 && b.cells < (int64_t(1) << 31), GMF_EINVAL, "cell count out of range");
+        int off = 0;
+        for (int a = 0; a < b.dim; ++a) {
+          b.N[a] = bd.modes[a];
+          b.Q[a] = bd.points[a];
+          require(b.N[a] >= 1 && b.N[a] <= kMaxExtent && b.Q[a] >= 1 && b.Q[a] <= kMaxExtent, GMF_EUNSUPPORTED,
+                  "per-axis mode and point counts must lie in 1..16");
+          b.Ntot *= b.N[a];
+          b.Qtot *= b.Q[a];
+          b.table_offset[a] = off;
+          off += b.N[a] * b.Q[a];
+        }
+        b.point_stride = b.form == GMF_FORM_DIFFUSION ? 1 : (b.dim + 1) * (b.dim + 1);
+        b.h_values.assign(bd.values, bd.values + off);
+        b.h_derivs.assign(bd.derivatives, bd.derivatives + off);
+        b.values = upload(p, bd.values, size_t(off));
+        b.derivs = upload(p, bd.derivatives, size_t(off));
+        if (b.cells == 0) {
+          p.buckets.push_back(std::move(b));
+          continue;
+        }
+        const size_t nmode = size_t(b.cells) * size_t(b.Ntot);
+        for (size_t i = 0; i < size_t(b.cells) * size_t(b.dim); ++i)
+          require(std::isfinite(bd.inverse_jacobian[i]), GMF_EINVAL, "non-finite inverse Jacobian");
+        b.jinv = upload(p, bd.inverse_jacobian, size_t(b.cells) * size_t(b.dim));
+        b.pdata = upload(p, bd.point_data, size_t(b.cells) * size_t(b.Qtot) * size_t(b.point_stride));
+        b.multi_count = bd.multi_count;
+        for (size_t i = 0; i < nmode; ++i) {
+          int32_t g = bd.local_index[i];
+          require(g >= -1 ? g < p.n_full : (-int64_t(g) - 2) < b.multi_count, GMF_EINVAL, "local index out of range");
+        }
+        if (b.multi_count) {
+          std::vector<int64_t> tmp(bd.multi_indices, bd.multi_indices + bd.multi_offsets[b.multi_count]);
+          check_csr(b.multi_count, bd.multi_offsets, tmp.data(), p.n_full, "multi-term map");
+          p.multi_terms = true;
+        }
+        b.lidx = upload(p, bd.local_index, nmode);
+        b.lcoef = upload(p, bd.local_coefficient, nmode);
+        if (b.multi_count) {
+          b.moff = upload(p, bd.multi_offsets, size_t(b.multi_count) + 1);
+          b.midx = upload(p, bd.multi_indices, size_t(bd.multi_offsets[b.multi_count]));
+          b.mcoef = upload(p, bd.multi_coefficients, size_t(bd.multi_offsets[b.multi_count]));
+        }
+        if (p.selection_map) {
+          std::vector<int32_t> r(nmode);
+          for (size_t i = 0; i < nmode; ++i) {
+            int32_t g = bd.local_index[i];
+            r[i] = g >= 0 ? int32_t(red_of_full[size_t(g)]) : g;
+          }
+          b.ridx = upload(p, r.data(), nmode);
+          if (b.multi_count) {
+            int64_t nt = bd.multi_offsets[b.multi_count];
+            std::vector<int32_t> mr(static_cast<size_t>(nt));
+            for (int64_t t = 0; t < nt; ++t) mr[size_t(t)] = int32_t(red_of_full[size_t(bd.multi_indices[t])]);
+            b.mridx = upload(p, mr.data(), size_t(nt));
+          }
+        }
+        prepare_fast_path(p, b);
+        p.buckets.push_back(std::move(b));
+      }
+      GMF_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      for (void* a : p.allocations) (void)hipFree(a);
+      throw;
+    }
+    *out = plan.release();
+  });
+}
+
+gmf_status gmf_plan_destroy(gmf_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    for (void* a : plan->allocations) (void)hipFree(a);
+    delete plan;
+  });
+}
+
+gmf_status gmf_plan_sizes(const gmf_plan* plan, int64_t* nf, int64_t* nr, int64_t* bytes) {
+  return guarded([&] {
+    require(plan != nullptr, GMF_EINVAL, "null plan");
+    if (nf) *nf = plan->n_full;
+    if (nr) *nr = plan->n_red;
+    if (bytes) *bytes = plan->device_bytes;
+  });
+}
+
+gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** out) {
+  return guarded([&] {
+    require(plan != nullptr && out != nullptr, GMF_EINVAL, "null plan");
+    auto ws = std::make_unique<gmf_workspace>();
+    ws->plan = plan;
+    ws->full_in = dalloc<double>(size_t(plan->n_full));
+    ws->full_out = dalloc<double>(size_t(plan->n_full));
+    ws->partial_capacity = 64 * 256 + 4096;
+    ws->partials = dalloc<double>(size_t(ws->partial_capacity));
+    ws->scalars = dalloc<double>(size_t(kScalarCount));
+    GMF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_scalars), sizeof(double) * kScalarCount));
+    *out = ws.release();
+  });
+}
+
+gmf_status gmf_workspace_destroy(gmf_workspace* ws) {
+  return guarded([&] {
+    if (!ws) return;
+    for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars})
+      if (v) (void)hipFree(v);
+    if (ws->host_scalars) (void)hipHostFree(ws->host_scalars);
+    delete ws;
+  });
+}
+
+gmf_status gmf_workspace_set_stream(gmf_workspace* ws, void* stream) {
+  return guarded([&] {
+    require(ws != nullptr, GMF_EINVAL, "null workspace");
+    ws->stream = static_cast<hipStream_t>(stream);
+  });
+}
+
+gmf_status gmf_apply_async(gmf_workspace* ws, const double* x, double* y) {
+  return guarded([&] {
+    require(ws && x && y, GMF_EINVAL, "null argument");
+    apply_full(ws, x, y);
+  });
+}
+
+gmf_status gmf_apply(gmf_workspace* ws, const double* x, double* y) {
+  return guarded([&] {
+    require(ws && x && y, GMF_EINVAL, "null argument");
+    apply_full(ws, x, y);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y) {
+  return guarded([&] {
+    require(ws && x && y, GMF_EINVAL, "null argument");
+    require(x != y, GMF_EALIAS, "result and coefficient vector must not alias");
+    fill(y, 0.0, ws->plan->n_full, ws->stream);
+    cell_pass(ws, x, y, false);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_t reps, double* cell_ms,
+                           double* total_ms) {
+  return guarded([&] {
+    require(ws && x && y && reps >= 1, GMF_EINVAL, "invalid argument");
+    require(x != y, GMF_EALIAS, "result and coefficient vector must not alias");
+    hipEvent_t e0, e1, e2, e3;
+    GMF_HIP(hipEventCreate(&e0));
+    GMF_HIP(hipEventCreate(&e1));
+    GMF_HIP(hipEventCreate(&e2));
+    GMF_HIP(hipEventCreate(&e3));
+    double cell = 0.0, total = 0.0;
+    for (int r = 0; r < reps; ++r) {
+      GMF_HIP(hipEventRecord(e0, ws->stream));
+      fill(y, 0.0, ws->plan->n_full, ws->stream);
+      GMF_HIP(hipEventRecord(e1, ws->stream));
+      cell_pass(ws, x, y, false);
+      GMF_HIP(hipEventRecord(e2, ws->stream));
+      constraint_rows(ws, x, y);
+      GMF_HIP(hipEventRecord(e3, ws->stream));
+      GMF_HIP(hipEventSynchronize(e3));
+      float a = 0.f, b = 0.f;
+      GMF_HIP(hipEventElapsedTime(&a, e1, e2));
+      GMF_HIP(hipEventElapsedTime(&b, e0, e3));
+      cell += a;
+      total += b;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(e3);
+    if (cell_ms) *cell_ms = cell / reps;
+    if (total_ms) *total_ms = total / reps;
+  });
+}
+
+gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
+  return guarded([&] {
+    require(ws && xr && yr, GMF_EINVAL, "null argument");
+    reduced_apply(ws, xr, yr);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_expand_reduced(gmf_workspace* ws, const double* xr, double* xf, int32_t include_shift) {
+  return guarded([&] {
+    require(ws && xr && xf, GMF_EINVAL, "null argument");
+    expand(ws, xr, xf, include_shift != 0);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_project_reduced(gmf_workspace* ws, const double* yf, double* yr) {
+  return guarded([&] {
+    require(ws && yf && yr, GMF_EINVAL, "null argument");
+    project(ws, yf, yr);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_diagonal_physical(gmf_workspace* ws, double* diag) {
+  return guarded([&] {
+    require(ws && diag, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    fill(diag, 0.0, p.n_full, ws->stream);
+    for (const Bucket& b : p.buckets) launch_cell_diagonal_generic(p, b, diag, false, ws->stream);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_reduced_diagonal(gmf_workspace* ws, double* diag) {
+  return guarded([&] {
+    require(ws && diag, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    require(p.selection_map && !p.multi_terms, GMF_EUNSUPPORTED,
+            "exact reduced diagonal with multi-term constraint expansions is not implemented");
+    fill(diag, 0.0, p.n_red, ws->stream);
+    for (const Bucket& b : p.buckets) launch_cell_diagonal_generic(p, b, diag, true, ws->stream);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n) {
+  return guarded([&] {
+    require(ws && d && inv && n >= 0, GMF_EINVAL, "null argument");
+    int* bad = nullptr;
+    GMF_HIP(hipMalloc(&bad, sizeof(int)));
+    GMF_HIP(hipMemsetAsync(bad, 0, sizeof(int), ws->stream));
+    hipLaunchKernelGGL(k_invert_diagonal, dim3(grid1(n)), dim3(256), 0, ws->stream, d, inv, n, bad);
+    int h = 0;
+    GMF_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ws->stream));
+    GMF_HIP(hipStreamSynchronize(ws->stream));
+    (void)hipFree(bad);
+    require(h == 0, GMF_EDIAGONAL, "Jacobi preconditioner requires a nonzero finite diagonal");
+  });
+}
+
+gmf_status gmf_precond_create_jacobi(gmf_workspace* ws, const double* invdiag, gmf_preconditioner** out) {
+  return guarded([&] {
+    require(ws && invdiag && out, GMF_EINVAL, "null argument");
+    auto j = std::make_unique<JacobiPrecond>();
+    j->ws = ws;
+    j->n = ws->plan->n_red;
+    j->invdiag = dalloc<double>(size_t(j->n));
+    copy(j->invdiag, invdiag, j->n, ws->stream);
+    sync(ws);
+    *out = j.release();
+  });
+}
+
+gmf_status gmf_precond_destroy(gmf_preconditioner* M) {
+  return guarded([&] { delete M; });
+}
+
+gmf_status gmf_precond_apply(gmf_preconditioner* M, const double* r, double* z) {
+  return guarded([&] {
+    require(M && r && z, GMF_EINVAL, "null argument");
+    M->apply(r, z);
+    sync(M->ws);
+  });
+}
+
+gmf_status gmf_cg_solve(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, int32_t use_initial,
+                        double rtol, double atol, int64_t maxiter, gmf_solve_info* info) {
+  gmf_solve_info local{};
+  gmf_solve_info& inf = info ? *info : local;
+  inf = gmf_solve_info{};
+  return guarded([&] {
+    require(ws && b && x, GMF_EINVAL, "null argument");
+    require(maxiter >= 1, GMF_EINVAL, "maxiter must be positive");
+    require(std::isfinite(rtol) && rtol >= 0 && std::isfinite(atol) && atol >= 0, GMF_EINVAL,
+            "tolerances must be finite and non-negative");
+    require(M == nullptr || M->ws == ws, GMF_EINVAL, "preconditioner belongs to a different workspace");
+    const int64_t n = ws->plan->n_red;
+    if (n == 0) {
+      inf.converged = 1;
+      return;
+    }
+    SolverVectors v;
+    v.r = dalloc<double>(size_t(n));
+    v.z = dalloc<double>(size_t(n));
+    v.p = dalloc<double>(size_t(n));
+    v.q = dalloc<double>(size_t(n));
+    copy(v.r, b, n, ws->stream);
+    if (use_initial) {
+      reduced_apply(ws, x, v.q);
+      inf.operator_applications++;
+      axpby(ws, v.r, -1.0, v.q, 1.0, n);
+    } else {
+      fill(x, 0.0, n, ws->stream);
+    }
+    precond_apply_dot(ws, M, v.r, v.z, n, kRz);
+    inf.preconditioner_applications++;
+    copy(v.p, v.z, n, ws->stream);
+    dot(ws, b, b, n, kBb);
+    dot(ws, v.r, v.r, n, kRr);
+    read_scalars(ws, kTmp);
+    const double* hs = ws->host_scalars;
+    inf.rhs_norm = std::sqrt(hs[kBb]);
+    const double tol = std::max(atol, rtol * inf.rhs_norm);
+    inf.initial_residual_norm = std::sqrt(hs[kRr]);
+    inf.final_residual_norm = inf.initial_residual_norm;
+    if (hs[kRr] <= tol * tol) {
+      inf.converged = 1;
+      return;
+    }
+    for (int64_t it = 1; it <= maxiter; ++it) {
+      reduced_apply(ws, v.p, v.q);
+      inf.operator_applications++;
+      dot(ws, v.p, v.q, n, kPq);
+      cg_update(ws, x, v.r, v.p, v.q, n, kRz, kPq, kRr);
+      read_scalars(ws, kTmp);
+      inf.iterations = it;
+      require(hs[kPq] > 0.0, GMF_EBREAKDOWN,
+              "CG encountered a non-positive operator direction; provide a different linear_solve for this problem");
+      inf.final_residual_norm = std::sqrt(hs[kRr]);
+      if (hs[kRr] <= tol * tol) {
+        inf.converged = 1;
+        return;
+      }
+      precond_apply_dot(ws, M, v.r, v.z, n, kRzNew);
+      inf.preconditioner_applications++;
+      direction(ws, v.p, v.z, n, kRzNew, kRz);
+    }
+    throw Error(GMF_ENOCONV, "CG did not converge in " + std::to_string(maxiter) + " iterations");
+  });
+}
+
+gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, int32_t use_initial,
+                            int32_t restart, double rtol, double atol, int64_t maxiter, gmf_solve_info* info) {
+  gmf_solve_info local{};
+  gmf_solve_info& inf = info ? *info : local;
+  inf = gmf_solve_info{};
+  return guarded([&] {
+    require(ws && b && x, GMF_EINVAL, "null argument");
+    require(restart >= 1, GMF_EINVAL, "restart must be positive");
+    require(maxiter >= 1, GMF_EINVAL, "maxiter must be positive");
+    require(M == nullptr || M->ws == ws, GMF_EINVAL, "preconditioner belongs to a different workspace");
+    const int64_t n = ws->plan->n_red;
+    if (n == 0) {
+      inf.converged = 1;
+      return;
+    }
+    const int m = int(std::min<int64_t>(restart, n));
+    require(m + 1 < kScalarCount - kHessian, GMF_EINVAL, "restart too large");
+    SolverVectors v;
+    v.r = dalloc<double>(size_t(n));
+    v.q = dalloc<double>(size_t(n));
+    v.V = dalloc<double>(size_t(n) * size_t(m + 1));
+    v.Z = dalloc<double>(size_t(n) * size_t(m));
+    v.ydev = dalloc<double>(size_t(m));
+    copy(v.r, b, n, ws->stream);
+    if (use_initial) {
+      reduced_apply(ws, x, v.q);
+      inf.operator_applications++;
+      axpby(ws, v.r, -1.0, v.q, 1.0, n);
+    } else {
+      fill(x, 0.0, n, ws->stream);
+    }
+    dot(ws, b, b, n, kBb);
+    dot(ws, v.r, v.r, n, kRr);
+    read_scalars(ws, kTmp);
+    const double* hs = ws->host_scalars;
+    inf.rhs_norm = std::sqrt(hs[kBb]);
+    const double tol = std::max(atol, rtol * inf.rhs_norm);
+    inf.initial_residual_norm = std::sqrt(hs[kRr]);
+    inf.final_residual_norm = inf.initial_residual_norm;
+    if (inf.initial_residual_norm <= tol) {
+      inf.converged = 1;
+      return;
+    }
+    const size_t sm = static_cast<size_t>(m);
+    std::vector<double> H((sm + 1) * sm), g(sm + 1), cs(sm), sn(sm), y(sm);
+    auto Hm = [&](int r, int c) -> double& { return H[size_t(r * m + c)]; };
+    auto residual_norm = [&]() {
+      dot(ws, v.r, v.r, n, kRr);
+      read_scalars(ws, kTmp);
+      return std::sqrt(hs[kRr]);
+    };
+    auto finish = [&](int count) {
+      for (int row = count - 1; row >= 0; --row) {
+        double val = g[size_t(row)];
+        for (int c = row + 1; c < count; ++c) val -= Hm(row, c) * y[size_t(c)];
+        double dg = Hm(row, row);
+        require(std::isfinite(dg), GMF_EINVAL, "FGMRES encountered a non-finite Krylov least-squares diagonal");
+        require(dg != 0.0, GMF_EINVAL, "FGMRES encountered a singular Krylov least-squares problem");
+        y[size_t(row)] = val / dg;
+      }
+      GMF_HIP(hipMemcpyAsync(v.ydev, y.data(), sizeof(double) * size_t(count), hipMemcpyHostToDevice, ws->stream));
+      multi_axpy(ws, x, v.Z, n, v.ydev, count, n, 1.0);
+      reduced_apply(ws, x, v.q);
+      inf.operator_applications++;
+      copy(v.r, b, n, ws->stream);
+      axpby(ws, v.r, -1.0, v.q, 1.0, n);
+      sync(ws);
+    };
+    int64_t total = 0;
+    while (total < maxiter) {
+      std::fill(H.begin(), H.end(), 0.0);
+      std::fill(g.begin(), g.end(), 0.0);
+      std::fill(cs.begin(), cs.end(), 0.0);
+      std::fill(sn.begin(), sn.end(), 0.0);
+      double rn = residual_norm();
+      inf.final_residual_norm = rn;
+      if (rn <= tol) {
+        inf.converged = 1;
+        return;
+      }
+      fill(v.V, 0.0, n, ws->stream);
+      axpby(ws, v.V, 1.0 / rn, v.r, 0.0, n);
+      g[0] = rn;
+      int inner_limit = int(std::min<int64_t>(m, maxiter - total));
+      int inner = 0;
+      bool updated = false;
+      for (int col = 0; col < inner_limit; ++col) {
+        total++;
+        inner = col + 1;
+        inf.iterations = total;
+        double* zc = v.Z + int64_t(col) * n;
+        double* vc = v.V + int64_t(col) * n;
+        precond_apply(ws, M, vc, zc, n);
+        inf.preconditioner_applications++;
+        reduced_apply(ws, zc, v.q);
+        inf.operator_applications++;
+        for (int bcol = 0; bcol <= col; ++bcol) {
+          const double* vb = v.V + int64_t(bcol) * n;
+          dot(ws, v.q, vb, n, kHessian + bcol);
+          hipLaunchKernelGGL(k_scale_dev, dim3(grid1(n)), dim3(256), 0, ws->stream, v.q, vb, ws->scalars,
+                             kHessian + bcol, -1.0, n);
+        }
+        dot(ws, v.q, v.q, n, kHessian + col + 1);
+        read_scalars(ws, kHessian + col + 2);
+        for (int bcol = 0; bcol <= col; ++bcol) Hm(bcol, col) = hs[kHessian + bcol];
+        const double next = std::sqrt(hs[kHessian + col + 1]);
+        Hm(col + 1, col) = next;
+        if (next > 0.0 && col + 1 < m + 1) {
+          double* vn = v.V + int64_t(col + 1) * n;
+          fill(vn, 0.0, n, ws->stream);
+          axpby(ws, vn, 1.0 / next, v.q, 0.0, n);
+        }
+        for (int row = 0; row < col; ++row) {
+          double f = Hm(row, col), s2 = Hm(row + 1, col);
+          Hm(row, col) = cs[size_t(row)] * f + sn[size_t(row)] * s2;
+          Hm(row + 1, col) = -sn[size_t(row)] * f + cs[size_t(row)] * s2;
+        }
+        double scale = std::hypot(Hm(col, col), Hm(col + 1, col));
+        double c = 1.0, s = 0.0;
+        if (scale != 0.0) {
+          c = Hm(col, col) / scale;
+          s = Hm(col + 1, col) / scale;
+        }
+        cs[size_t(col)] = c;
+        sn[size_t(col)] = s;
+        {
+          double f = Hm(col, col), s2 = Hm(col + 1, col);
+          Hm(col, col) = c * f + s * s2;
+          Hm(col + 1, col) = -s * f + c * s2;
+          double gf = g[size_t(col)], gs = g[size_t(col) + 1];
+          g[size_t(col)] = c * gf + s * gs;
+          g[size_t(col) + 1] = -s * gf + c * gs;
+        }
+        if (std::fabs(g[size_t(col) + 1]) <= tol || next == 0.0) {
+          finish(col + 1);
+          double fr = residual_norm();
+          inf.final_residual_norm = fr;
+          if (fr <= tol) {
+            inf.converged = 1;
+            return;
+          }
+          updated = true;
+          break;
+        }
+      }
+      require(inner > 0, GMF_EINVAL, "FGMRES failed to build a Krylov update");
+      if (!updated) finish(inner);
+      double fr = residual_norm();
+      inf.final_residual_norm = fr;
+      if (fr <= tol) {
+        inf.converged = 1;
+        return;
+      }
+    }
+    throw Error(GMF_ENOCONV, "FGMRES did not converge in " + std::to_string(maxiter) + " iterations");
+  });
+}
+
+}  // extern "C"

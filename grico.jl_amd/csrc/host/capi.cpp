@@ -1,0 +1,165 @@
+// C ABI of the host plan compiler (include/grico_host.h).
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+
+#include "../../../include/grico_host.h"
+#include "polys.hpp"
+#include "space.hpp"
+#include "topology.hpp"
+
+struct gh_grid {
+  gh::Grid g;
+};
+struct gh_space {
+  gh::Space s;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    g_error = e.what();
+    return -1;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return -2;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* gh_last_error(void) { return g_error.c_str(); }
+
+int gh_gauss_legendre(int point_count, double* points, double* weights) {
+  return guarded([&] { gh::gauss_legendre(point_count, points, weights); });
+}
+
+int gh_fe_basis(double x, int degree, double* values, double* derivatives) {
+  return guarded([&] {
+    if (degree < 0) throw std::invalid_argument("degree must be non-negative");
+    gh::fe_basis(x, degree, values, derivatives);
+  });
+}
+
+int gh_restriction(int source_degree, int target_degree, int delta, int64_t relative, double* out) {
+  return guarded([&] {
+    auto m = gh::affine_restriction_matrix(source_degree, target_degree, delta, relative);
+    std::memcpy(out, m.data(), m.size() * sizeof(double));
+  });
+}
+
+int gh_grid_create(int dim, const int64_t* root_counts, gh_grid** out) {
+  return guarded([&] { *out = new gh_grid{gh::Grid(dim, root_counts)}; });
+}
+
+void gh_grid_destroy(gh_grid* grid) { delete grid; }
+
+int64_t gh_grid_cell_count(const gh_grid* grid) { return grid->g.cell_count(); }
+
+int gh_grid_refine(gh_grid* grid, int64_t cell, int axis, int64_t* first_child) {
+  return guarded([&] { *first_child = grid->g.refine(cell, axis); });
+}
+
+int gh_grid_cell(const gh_grid* grid, int64_t cell, int32_t* levels, int64_t* coords, int64_t* parent,
+                 int64_t* first_child, int32_t* split_axis, int32_t* active) {
+  return guarded([&] {
+    const gh::Grid& g = grid->g;
+    if (cell < 0 || cell >= g.cell_count()) throw std::invalid_argument("cell out of range");
+    for (int a = 0; a < g.D; ++a) {
+      if (levels) levels[a] = g.level[size_t(cell)][size_t(a)];
+      if (coords) coords[a] = g.coord[size_t(cell)][size_t(a)];
+    }
+    if (parent) *parent = g.parent[size_t(cell)];
+    if (first_child) *first_child = g.first_child[size_t(cell)];
+    if (split_axis) *split_axis = g.split[size_t(cell)];
+    if (active) *active = g.active[size_t(cell)];
+  });
+}
+
+int gh_grid_export(const gh_grid* grid, int32_t* levels, int64_t* coords, int64_t* parent,
+                   int64_t* first_child, int32_t* split_axis, uint8_t* active) {
+  return guarded([&] {
+    const gh::Grid& g = grid->g;
+    for (int64_t c = 0; c < g.cell_count(); ++c) {
+      size_t i = size_t(c);
+      for (int a = 0; a < g.D; ++a) {
+        if (levels) levels[i * size_t(g.D) + size_t(a)] = g.level[i][size_t(a)];
+        if (coords) coords[i * size_t(g.D) + size_t(a)] = g.coord[i][size_t(a)];
+      }
+      if (parent) parent[i] = g.parent[i];
+      if (first_child) first_child[i] = g.first_child[i];
+      if (split_axis) split_axis[i] = g.split[i];
+      if (active) active[i] = g.active[i];
+    }
+  });
+}
+
+int gh_space_compile(const gh_grid* grid, int64_t leaf_count, const int64_t* leaves,
+                     const int32_t* degrees, int basis, int cg_mask, gh_space** out) {
+  return guarded([&] {
+    std::vector<int64_t> act(leaves, leaves + leaf_count);
+    std::vector<int32_t> deg(degrees, degrees + leaf_count * grid->g.D);
+    auto sp = std::make_unique<gh_space>();
+    sp->s = gh::compile_space(grid->g, act, deg, basis, cg_mask);
+    *out = sp.release();
+  });
+}
+
+void gh_space_destroy(gh_space* space) { delete space; }
+
+int gh_space_sizes_get(const gh_space* space, gh_space_sizes* sz) {
+  return guarded([&] {
+    const gh::Space& s = space->s;
+    sz->leaf_count = int64_t(s.leaves.size());
+    sz->dof_count = s.dof_count;
+    sz->mode_count = int64_t(s.mode_box.size());
+    sz->term_count = int64_t(s.term_indices.size());
+    sz->boundary_count = int64_t(s.boundary_leaf.size());
+    sz->interface_count = int64_t(s.iface_minus.size());
+    sz->boundary_variable_count = s.boundary_variable_count;
+    sz->constraint_row_count = s.constraint_row_count;
+  });
+}
+
+int gh_space_export(const gh_space* space, int64_t* leaves, int32_t* degrees, int64_t* mode_offsets,
+                    int32_t* mode_box, int64_t* term_offsets, int64_t* term_indices,
+                    double* term_coefficients) {
+  return guarded([&] {
+    const gh::Space& s = space->s;
+    if (leaves) std::memcpy(leaves, s.leaves.data(), s.leaves.size() * sizeof(int64_t));
+    if (degrees)
+      for (size_t i = 0; i < s.degrees.size(); ++i)
+        for (int a = 0; a < s.D; ++a) degrees[i * size_t(s.D) + size_t(a)] = s.degrees[i][size_t(a)];
+    if (mode_offsets) std::memcpy(mode_offsets, s.mode_offsets.data(), s.mode_offsets.size() * sizeof(int64_t));
+    if (mode_box) std::memcpy(mode_box, s.mode_box.data(), s.mode_box.size() * sizeof(int32_t));
+    if (term_offsets) std::memcpy(term_offsets, s.term_offsets.data(), s.term_offsets.size() * sizeof(int64_t));
+    if (term_indices) std::memcpy(term_indices, s.term_indices.data(), s.term_indices.size() * sizeof(int64_t));
+    if (term_coefficients)
+      std::memcpy(term_coefficients, s.term_coefficients.data(), s.term_coefficients.size() * sizeof(double));
+  });
+}
+
+int gh_space_faces(const gh_space* space, int64_t* boundary_leaf, int32_t* boundary_axis,
+                   int32_t* boundary_side, int64_t* interface_minus, int64_t* interface_plus,
+                   int32_t* interface_axis) {
+  return guarded([&] {
+    const gh::Space& s = space->s;
+    size_t nb = s.boundary_leaf.size(), ni = s.iface_minus.size();
+    if (boundary_leaf) std::memcpy(boundary_leaf, s.boundary_leaf.data(), nb * sizeof(int64_t));
+    if (boundary_axis) std::memcpy(boundary_axis, s.boundary_axis.data(), nb * sizeof(int32_t));
+    if (boundary_side) std::memcpy(boundary_side, s.boundary_side.data(), nb * sizeof(int32_t));
+    if (interface_minus) std::memcpy(interface_minus, s.iface_minus.data(), ni * sizeof(int64_t));
+    if (interface_plus) std::memcpy(interface_plus, s.iface_plus.data(), ni * sizeof(int64_t));
+    if (interface_axis) std::memcpy(interface_axis, s.iface_axis.data(), ni * sizeof(int32_t));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+"""MI355X matrix-free hp-FEM backend: host-side mirror of Grico.jl's API.
+
+The names follow the reference's public surface (src/Grico.jl exports):
+Domain, HpSpace, SpaceOptions, FullTensorBasis, TrunkBasis, UniformDegree,
+AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, ScalarField, AffineProblem,
+SPD, TestChannels, add_cell_accumulator, add_constraint, Dirichlet,
+BoundaryFace, LOWER, UPPER, compile, apply, solve, CGSolver, FGMRESSolver,
+JacobiPreconditioner, GeometricMultigridPreconditioner, l2_error,
+relative_l2_error. Julia's `!` mutators map to `*_into` functions.
+"""
+
+from .mesh import Domain, uniform_refine
+from .space import (AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, FullTensorBasis, HpSpace, SpaceOptions,
+                    StoredDegrees, TrunkBasis, UniformDegree, scalar_dof_count)
+from .problem import (LOWER, UPPER, AffineProblem, BoundaryFace, Dirichlet, NonsymmetricOperator, ScalarField, SPD,
+                      TestChannels, TrialChannels, add_cell_accumulator, add_constraint, gradient, value)
+from .plan import AssemblyPlan, compile
+from .ops import (apply, apply_into, apply_physical, diagonal_physical, expand_reduced, project_reduced,
+                  reduced_apply, reduced_diagonal)
+from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, JacobiPreconditioner, State, solve,
+                    solve_reduced)
+from .verification import l2_error, relative_l2_error
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,177 @@
+"""ctypes mirror of include/grico_mf.h and the device-plan wrapper."""
+
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+c_int32, c_int64, c_double, c_void_p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+P_i32 = ctypes.POINTER(ctypes.c_int32)
+P_i64 = ctypes.POINTER(ctypes.c_int64)
+P_f64 = ctypes.POINTER(ctypes.c_double)
+
+STATUS = {
+    0: "GMF_OK", -1: "GMF_EINVAL", -2: "GMF_EALIAS", -3: "GMF_ENOCONV", -4: "GMF_EBREAKDOWN",
+    -5: "GMF_EDIAGONAL", -6: "GMF_ENOTSPD", -7: "GMF_EDEVICE", -8: "GMF_EUNSUPPORTED",
+}
+
+
+class BucketDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", c_int32), ("modes", c_int32 * 3), ("points", c_int32 * 3), ("form", c_int32),
+        ("cell_count", c_int64),
+        ("values", P_f64), ("derivatives", P_f64), ("inverse_jacobian", P_f64), ("point_data", P_f64),
+        ("local_index", P_i32), ("local_coefficient", P_f64),
+        ("multi_count", c_int64), ("multi_offsets", P_i64), ("multi_indices", P_i32),
+        ("multi_coefficients", P_f64),
+    ]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("dim", c_int32), ("full_dof_count", c_int64), ("bucket_count", c_int32),
+        ("buckets", ctypes.POINTER(BucketDesc)),
+        ("fixed_count", c_int64), ("fixed_dofs", P_i64),
+        ("affine_count", c_int64), ("affine_pivots", P_i64), ("affine_offsets", P_i64),
+        ("affine_indices", P_i64), ("affine_coefficients", P_f64),
+        ("mean_count", c_int64), ("mean_pivots", P_i64), ("mean_offsets", P_i64),
+        ("mean_indices", P_i64), ("mean_coefficients", P_f64),
+        ("reduced_dof_count", c_int64), ("map_offsets", P_i64), ("map_indices", P_i64),
+        ("map_coefficients", P_f64), ("shift", P_f64),
+    ]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_int64), ("operator_applications", c_int64),
+        ("preconditioner_applications", c_int64), ("initial_residual_norm", c_double),
+        ("final_residual_norm", c_double), ("rhs_norm", c_double), ("converged", c_int32),
+    ]
+
+
+def declare(lib):
+    lib.gmf_last_error.restype = ctypes.c_char_p
+    lib.gmf_device_info.argtypes = [ctypes.c_char_p, P_i32, P_i64]
+    lib.gmf_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_void_p)]
+    lib.gmf_plan_destroy.argtypes = [c_void_p]
+    lib.gmf_plan_sizes.argtypes = [c_void_p, P_i64, P_i64, P_i64]
+    lib.gmf_workspace_create.argtypes = [c_void_p, ctypes.POINTER(c_void_p)]
+    lib.gmf_workspace_destroy.argtypes = [c_void_p]
+    lib.gmf_workspace_set_stream.argtypes = [c_void_p, c_void_p]
+    for name in ("gmf_apply", "gmf_apply_async", "gmf_apply_physical", "gmf_reduced_apply",
+                 "gmf_project_reduced"):
+        getattr(lib, name).argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.gmf_apply_timed.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, P_f64, P_f64]
+    lib.gmf_precond_create_jacobi.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
+    lib.gmf_precond_destroy.argtypes = [c_void_p]
+    lib.gmf_precond_apply.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.gmf_expand_reduced.argtypes = [c_void_p, c_void_p, c_void_p, c_int32]
+    lib.gmf_diagonal_physical.argtypes = [c_void_p, c_void_p]
+    lib.gmf_reduced_diagonal.argtypes = [c_void_p, c_void_p]
+    lib.gmf_invert_diagonal.argtypes = [c_void_p, c_void_p, c_void_p, c_int64]
+    lib.gmf_cg_solve.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_double,
+                                 c_double, c_int64, ctypes.POINTER(SolveInfo)]
+    lib.gmf_fgmres_solve.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
+                                     c_double, c_double, c_int64, ctypes.POINTER(SolveInfo)]
+    for name in ("gmf_plan_create", "gmf_plan_destroy", "gmf_plan_sizes", "gmf_workspace_create",
+                 "gmf_workspace_destroy", "gmf_workspace_set_stream", "gmf_apply", "gmf_apply_async",
+                 "gmf_apply_physical", "gmf_reduced_apply", "gmf_project_reduced", "gmf_expand_reduced",
+                 "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
+                 "gmf_fgmres_solve", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
+                 "gmf_precond_destroy", "gmf_precond_apply"):
+        getattr(lib, name).restype = ctypes.c_int
+
+
+class DeviceError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+def check(status):
+    if status != 0:
+        raise DeviceError(status, _native.device().gmf_last_error().decode())
+
+
+def _p(a, t):
+    return _native.ptr(a, t)
+
+
+class DevicePlan:
+    """Uploads an AssemblyPlan's tables once and owns a workspace."""
+
+    def __init__(self, plan):
+        lib = _native.device()
+        keep = []
+        descs = (BucketDesc * len(plan.buckets))()
+        for i, b in enumerate(plan.buckets):
+            d = descs[i]
+            d.dim = b.dim
+            for a in range(3):
+                d.modes[a] = b.modes[a] if a < b.dim else 1
+                d.points[a] = b.points[a] if a < b.dim else 1
+            d.form = b.form
+            d.cell_count = b.cell_count
+            vals = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.B]))
+            ders = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.G]))
+            pdata = np.ascontiguousarray(b.point_data, dtype=np.float64)
+            coef = None if b.unit_coefficients else b.local_coefficient
+            keep += [vals, ders, pdata, b.inverse_jacobian, b.local_index, coef, b.multi_offsets,
+                     b.multi_indices, b.multi_coefficients]
+            d.values = _p(vals, c_double)
+            d.derivatives = _p(ders, c_double)
+            d.inverse_jacobian = _p(b.inverse_jacobian, c_double)
+            d.point_data = _p(pdata, c_double)
+            d.local_index = _p(b.local_index, c_int32)
+            d.local_coefficient = _p(coef, c_double)
+            d.multi_count = len(b.multi_offsets) - 1
+            d.multi_offsets = _p(b.multi_offsets, c_int64)
+            d.multi_indices = _p(b.multi_indices, c_int32)
+            d.multi_coefficients = _p(b.multi_coefficients, c_double)
+        pd = PlanDesc()
+        pd.dim = plan.dim
+        pd.full_dof_count = plan.full_dof_count
+        pd.bucket_count = len(plan.buckets)
+        pd.buckets = descs
+        fixed = _native.as_i64(plan.fixed_dofs)
+        keep.append(fixed)
+        pd.fixed_count = len(fixed)
+        pd.fixed_dofs = _p(fixed, c_int64)
+        pd.affine_count = 0
+        pd.mean_count = 0
+        pd.reduced_dof_count = plan.reduced_dof_count
+        mo, mi, mc, sh = (_native.as_i64(plan.map_offsets), _native.as_i64(plan.map_indices),
+                          _native.as_f64(plan.map_coefficients), _native.as_f64(plan.shift))
+        keep += [mo, mi, mc, sh]
+        pd.map_offsets = _p(mo, c_int64)
+        pd.map_indices = _p(mi, c_int64)
+        pd.map_coefficients = _p(mc, c_double)
+        pd.shift = _p(sh, c_double)
+        handle = c_void_p()
+        check(lib.gmf_plan_create(ctypes.byref(pd), ctypes.byref(handle)))
+        self.handle = handle
+        ws = c_void_p()
+        check(lib.gmf_workspace_create(handle, ctypes.byref(ws)))
+        self.ws = ws
+        self.full_dof_count = plan.full_dof_count
+        self.reduced_dof_count = plan.reduced_dof_count
+
+    def __del__(self):
+        lib = _native._device
+        if lib is None:
+            return
+        if getattr(self, "ws", None) is not None and self.ws.value:
+            lib.gmf_workspace_destroy(self.ws)
+            self.ws = None
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            lib.gmf_plan_destroy(self.handle)
+            self.handle = None
+
+    def set_stream(self, stream_ptr):
+        check(_native.device().gmf_workspace_set_stream(self.ws, c_void_p(stream_ptr)))
+
+    def device_bytes(self):
+        a, b, c = c_int64(), c_int64(), c_int64()
+        check(_native.device().gmf_plan_sizes(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return c.value

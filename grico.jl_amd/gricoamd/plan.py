@@ -1,0 +1,395 @@
+"""compile(problem) -> AssemblyPlan: host lowering + one upload of the tables to HBM.
+
+Reference call chain (src/plans.jl:170-231): field layout, integration
+compile (src/integration.jl:2145-2191 _compile_cell: points, W_q = w_q*detJ,
+J^-1; 2677-2742 basis tables), Dirichlet compile (src/plans.jl:486-972),
+traversal batches (307-332) and the reduced map (src/assembly.jl:112-314).
+Here, in addition, every cell accumulator is lowered to per-point
+coefficient tables by probing `cell_accumulate` with unit trial channels
+(the contract of src/accumulators.jl:523-539 is linear in the trial
+channels), so the device kernels never call user code.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _devabi, _native
+from .problem import AffineProblem, QuadraturePoints, TestChannels, TrialChannels
+from .quadrature import basis_tables, gauss_legendre_rule
+
+FORM_DIFFUSION = 0
+FORM_GENERAL = 1
+
+
+def _expand_ranges(starts, counts):
+    """Concatenation of arange(s, s+c) over (s, c) pairs, vectorized."""
+    starts = np.asarray(starts, dtype=np.int64)
+    counts = np.asarray(counts, dtype=np.int64)
+    total = int(counts.sum())
+    if total == 0:
+        return np.zeros(0, dtype=np.int64)
+    before = np.repeat(np.cumsum(counts) - counts, counts)
+    return np.repeat(starts, counts) + (np.arange(total, dtype=np.int64) - before)
+
+
+def _channels(result, shape, dim):
+    if isinstance(result, TestChannels):
+        v = np.broadcast_to(np.asarray(result.value, dtype=np.float64), shape)
+        if len(result.gradient) != dim:
+            raise ValueError("TestChannels gradient must have one entry per axis")
+        g = [np.broadcast_to(np.asarray(c, dtype=np.float64), shape) for c in result.gradient]
+        return v, g
+    v = np.broadcast_to(np.asarray(result, dtype=np.float64), shape)
+    return v, [np.zeros(shape)] * dim
+
+
+class Bucket:
+    """Cells sharing degree tuple and quadrature shape (one device kernel batch)."""
+
+    def __init__(self, space, leaf_idx, degrees, qshape):
+        dom = space.domain
+        self.dim = D = space.dim
+        self.leaf_idx = np.asarray(leaf_idx, dtype=np.int64)  # positions in space.leaves
+        self.cells = space.leaves[self.leaf_idx]
+        self.degrees = tuple(int(p) for p in degrees)
+        self.modes = tuple(p + 1 for p in self.degrees)
+        self.points = tuple(int(q) for q in qshape)
+        self.n_modes = int(np.prod(self.modes))
+        self.n_points = int(np.prod(self.points))
+        nc = len(self.cells)
+        self.cell_count = nc
+        # 1D tables per axis.
+        self.B = [basis_tables(self.degrees[a], self.points[a])[0] for a in range(D)]
+        self.G = [basis_tables(self.degrees[a], self.points[a])[1] for a in range(D)]
+        rules = [gauss_legendre_rule(self.points[a]) for a in range(D)]
+        # Tensor reference points / weights, axis 1 fastest.
+        grids = np.meshgrid(*[r[0] for r in rules][::-1], indexing="ij")
+        self.ref_points = [g.reshape(-1) for g in grids[::-1]]
+        wgrid = np.meshgrid(*[r[1] for r in rules][::-1], indexing="ij")
+        w = np.ones_like(wgrid[0])
+        for g in wgrid:
+            w = w * g
+        self.ref_weights = w.reshape(-1)
+        # Cell geometry (src/geometry.jl).
+        sizes = dom.cell_sizes(self.cells)
+        centers, halves = dom.cell_centers_and_half_sizes(self.cells)
+        detj = np.ldexp(np.prod(sizes, axis=1), -D)
+        self.sizes = sizes
+        self.weights = self.ref_weights[None, :] * detj[:, None]
+        self.phys_points = [centers[:, a:a + 1] + self.ref_points[a][None, :] * halves[:, a:a + 1]
+                            for a in range(D)]
+        self.inverse_jacobian = np.ascontiguousarray(1.0 / halves)
+        self._gather_maps(space)
+
+    def _gather_maps(self, space):
+        """local_index / local_coefficient / multi-term CSR over the full mode box."""
+        nc, nb = self.cell_count, self.n_modes
+        lo = space.mode_offsets[self.leaf_idx]
+        hi = space.mode_offsets[self.leaf_idx + 1]
+        counts = hi - lo
+        mode_ids = _expand_ranges(lo, counts)
+        cell_of_mode = np.repeat(np.arange(nc), counts)
+        box = space.mode_box[mode_ids].astype(np.int64)
+        tcount = space.term_offsets[mode_ids + 1] - space.term_offsets[mode_ids]
+        local_index = np.full((nc, nb), -1, dtype=np.int64)
+        local_coef = np.zeros((nc, nb))
+        single = tcount == 1
+        first = space.term_offsets[mode_ids]
+        local_index[cell_of_mode[single], box[single]] = space.term_indices[first[single]]
+        local_coef[cell_of_mode[single], box[single]] = space.term_coefficients[first[single]]
+        multi = np.nonzero(tcount > 1)[0]
+        self.multi_offsets = np.zeros(len(multi) + 1, dtype=np.int64)
+        if len(multi):
+            self.multi_offsets[1:] = np.cumsum(tcount[multi])
+            sel = _expand_ranges(space.term_offsets[mode_ids[multi]], tcount[multi])
+            self.multi_indices = space.term_indices[sel].astype(np.int32)
+            self.multi_coefficients = space.term_coefficients[sel].copy()
+            local_index[cell_of_mode[multi], box[multi]] = -(np.arange(len(multi)) + 2)
+        else:
+            self.multi_indices = np.zeros(0, dtype=np.int32)
+            self.multi_coefficients = np.zeros(0)
+        if space.dof_count >= 2**31 - 1:
+            raise ValueError("dof counts above 2^31-1 are not supported by the device index maps")
+        self.local_index = np.ascontiguousarray(local_index.astype(np.int32))
+        self.local_coefficient = np.ascontiguousarray(local_coef)
+        self.unit_coefficients = bool(np.all(local_coef[local_index >= 0] == 1.0))
+        # Expanded term list (cell, box, dof, coefficient) for host-side scatters.
+        terms_per = tcount
+        self.term_cell = np.repeat(cell_of_mode, terms_per)
+        self.term_box = np.repeat(box, terms_per)
+        sel_all = _expand_ranges(space.term_offsets[mode_ids], tcount)
+        self.term_dof = space.term_indices[sel_all]
+        self.term_coef = space.term_coefficients[sel_all]
+
+    def quadrature(self, lo=0, hi=None):
+        hi = self.cell_count if hi is None else hi
+        return QuadraturePoints([p[lo:hi] for p in self.phys_points], self.weights[lo:hi],
+                                [self.sizes[lo:hi, a:a + 1] for a in range(self.dim)])
+
+    def _probe(self, operators, lo, hi):
+        """(D+1)x(D+1) action per point of cells lo..hi (unweighted)."""
+        D = self.dim
+        shape = (hi - lo, self.n_points)
+        q = self.quadrature(lo, hi)
+        zeros = np.zeros(shape)
+        ones = np.ones(shape)
+        M = np.zeros(shape + (D + 1, D + 1))
+        for op in operators:
+            base_v, base_g = _channels(op.cell_accumulate(q, TrialChannels(0, zeros, (zeros,) * D), 0),
+                                       shape, D)
+            if np.any(base_v != 0) or any(np.any(g != 0) for g in base_g):
+                raise ValueError("cell_accumulate must be linear in the trial channels")
+            for ch in range(D + 1):
+                val = ones if ch == 0 else zeros
+                grad = tuple(ones if ch == a + 1 else zeros for a in range(D))
+                v, g = _channels(op.cell_accumulate(q, TrialChannels(0, val, grad), 0), shape, D)
+                M[..., 0, ch] += v
+                for a in range(D):
+                    M[..., a + 1, ch] += g[a]
+        return M
+
+    def lower_operators(self, operators, chunk=8192):
+        """Probe cell_accumulate with unit trial channels; returns (form, point_data).
+
+        DIFFUSION when every point's action is diag(0, c I): one coefficient
+        W_q c_q per point; otherwise GENERAL with the full W_q M_q matrix."""
+        D = self.dim
+        nc = self.cell_count
+        coef = np.zeros((nc, self.n_points))
+        general = None
+        for lo in range(0, nc, chunk):
+            hi = min(nc, lo + chunk)
+            M = self._probe(operators, lo, hi)
+            W = self.weights[lo:hi]
+            if general is None:
+                grad_block = M[..., 1:, 1:]
+                c = grad_block[..., 0, 0]
+                if (np.all(M[..., 0, :] == 0) and np.all(M[..., :, 0] == 0)
+                        and np.all(grad_block == c[..., None, None] * np.eye(D))):
+                    coef[lo:hi] = W * c
+                    continue
+                general = np.zeros((nc, self.n_points * (D + 1) ** 2))
+                for lo2 in range(0, lo, chunk):
+                    hi2 = min(nc, lo2 + chunk)
+                    M2 = self._probe(operators, lo2, hi2)
+                    general[lo2:hi2] = (self.weights[lo2:hi2][..., None, None] * M2).reshape(hi2 - lo2, -1)
+            general[lo:hi] = (W[..., None, None] * M).reshape(hi - lo, -1)
+        if general is None:
+            return FORM_DIFFUSION, np.ascontiguousarray(coef)
+        return FORM_GENERAL, np.ascontiguousarray(general)
+
+    def interpolate(self, coefficients_box, derivative_axis=None):
+        """Sum-factorized evaluation on all cells: (nc, N box) -> (nc, Q); host numpy."""
+        D = self.dim
+        t = coefficients_box.reshape((self.cell_count,) + self.modes[::-1])
+        for a in range(D):
+            table = self.G[a] if a == derivative_axis else self.B[a]
+            ax = D - a  # numpy axis of mode axis a (axis 1 fastest = last)
+            t = np.moveaxis(np.tensordot(t, table, axes=([ax], [0])), -1, ax)
+        out = t.reshape(self.cell_count, -1)
+        if derivative_axis is not None:
+            out = out * self.inverse_jacobian[:, derivative_axis:derivative_axis + 1]
+        return out
+
+    def project(self, point_values, derivative_axis=None):
+        """Transpose of interpolate: (nc, Q) -> (nc, N box)."""
+        D = self.dim
+        vals = point_values
+        if derivative_axis is not None:
+            vals = vals * self.inverse_jacobian[:, derivative_axis:derivative_axis + 1]
+        t = vals.reshape((self.cell_count,) + self.points[::-1])
+        for a in range(D):
+            table = self.G[a] if a == derivative_axis else self.B[a]
+            ax = D - a
+            t = np.moveaxis(np.tensordot(t, table, axes=([ax], [1])), -1, ax)
+        return t.reshape(self.cell_count, -1)
+
+
+class AssemblyPlan:
+    """Frozen host tables plus the device plan handle."""
+
+    def __init__(self, problem: AffineProblem):
+        self.problem = problem
+        space = problem.field.space
+        self.space = space
+        self.dim = space.dim
+        self.full_dof_count = space.dof_count
+        keys = {}
+        for li in range(len(space.leaves)):
+            key = (tuple(space.degrees[li]), tuple(space.quadrature_shapes[li]))
+            keys.setdefault(key, []).append(li)
+        self.buckets = []
+        ops = [op for (_, _, op) in problem.cell_operators]
+        for (deg, qs), lis in keys.items():
+            b = Bucket(space, lis, deg, qs)
+            if ops:
+                b.form, b.point_data = b.lower_operators(ops)
+            else:
+                b.form, b.point_data = FORM_DIFFUSION, np.zeros((b.cell_count, b.n_points))
+            self.buckets.append(b)
+        self._compile_dirichlet()
+        self._compile_reduced_map()
+        self._rhs = None
+        self._device = None
+
+    # Dirichlet data (src/plans.jl:486-514, 884-972).
+    def _compile_dirichlet(self):
+        space = self.space
+        cons = self.problem.constraints
+        fixed = {}
+        if cons:
+            for con in cons:
+                sel = (space.boundary_axis == con.boundary.axis) & (space.boundary_side == con.boundary.side)
+                if not np.any(sel):
+                    raise ValueError("Dirichlet constraint does not match any boundary face")
+            dofs = self._dirichlet_trace_dofs(cons)
+            values = self._dirichlet_values(cons, dofs)
+            fixed = dict(zip(dofs.tolist(), values.tolist()))
+        self.fixed_dofs = np.asarray(sorted(fixed), dtype=np.int64)
+        self.fixed_values = np.asarray([fixed[d] for d in self.fixed_dofs.tolist()], dtype=np.float64)
+
+    def _trace_terms(self, con):
+        """(dof, coef, face_index, box) of every term of a mode with nonzero trace on the face."""
+        space = self.space
+        sel = np.nonzero((space.boundary_axis == con.boundary.axis) &
+                         (space.boundary_side == con.boundary.side))[0]
+        out = []
+        for f in sel:
+            li = int(space.boundary_leaf[f])
+            deg = space.degrees[li]
+            shape = tuple(int(p) + 1 for p in deg)
+            a = con.boundary.axis
+            for m in range(space.mode_offsets[li], space.mode_offsets[li + 1]):
+                box = int(space.mode_box[m])
+                idx = np.unravel_index(box, shape[::-1])[::-1]
+                if deg[a] >= 1 and idx[a] != con.boundary.side:
+                    continue
+                for t in range(space.term_offsets[m], space.term_offsets[m + 1]):
+                    out.append((int(space.term_indices[t]), float(space.term_coefficients[t]), li, box))
+        return out
+
+    def _dirichlet_trace_dofs(self, cons):
+        dofs = set()
+        for con in cons:
+            for dof, coef, _, _ in self._trace_terms(con):
+                if coef != 0.0:
+                    dofs.add(dof)
+        return np.asarray(sorted(dofs), dtype=np.int64)
+
+    def _dirichlet_values(self, cons, dofs):
+        if all((not callable(c.data)) and float(c.data) == 0.0 for c in cons):
+            return np.zeros(len(dofs))
+        return _boundary_l2_projection(self, cons, dofs)
+
+    # Reduced map (src/assembly.jl:112-154, 282-314) for fixed rows.
+    def _compile_reduced_map(self):
+        n = self.full_dof_count
+        constrained = np.zeros(n, dtype=bool)
+        constrained[self.fixed_dofs] = True
+        self.solve_dofs = np.nonzero(~constrained)[0].astype(np.int64)
+        self.reduced_index = np.full(n, -1, dtype=np.int64)
+        self.reduced_index[self.solve_dofs] = np.arange(len(self.solve_dofs))
+        self.reduced_dof_count = len(self.solve_dofs)
+        self.map_offsets = np.zeros(n + 1, dtype=np.int64)
+        self.map_offsets[1:] = np.cumsum(~constrained)
+        self.map_indices = np.arange(self.reduced_dof_count, dtype=np.int64)
+        self.map_coefficients = np.ones(self.reduced_dof_count)
+        self.shift = np.zeros(n)
+        self.shift[self.fixed_dofs] = self.fixed_values
+
+    # Host-side right-hand side (src/assembly.jl:488-507 rhs!).
+    def rhs_physical(self):
+        if self._rhs is None:
+            b = np.zeros(self.full_dof_count)
+            for (_, op) in self.problem.cell_rhs:
+                for bk in self.buckets:
+                    shape = (bk.cell_count, bk.n_points)
+                    v, g = _channels(op.cell_rhs_accumulate(bk.quadrature(), 0), shape, bk.dim)
+                    local = bk.project(bk.weights * v)
+                    for a in range(bk.dim):
+                        if np.any(g[a] != 0):
+                            local = local + bk.project(bk.weights * g[a], derivative_axis=a)
+                    contrib = bk.term_coef * local[bk.term_cell, bk.term_box]
+                    b += np.bincount(bk.term_dof, weights=contrib, minlength=self.full_dof_count)
+            self._rhs = b
+        return self._rhs
+
+    def rhs(self):
+        b = self.rhs_physical().copy()
+        b[self.fixed_dofs] = self.fixed_values
+        return b
+
+    def reduced_rhs(self):
+        """P^T (b - A shift) (src/assembly.jl:917-933), A shift evaluated on the device."""
+        b = self.rhs_physical().copy()
+        if np.any(self.shift != 0):
+            from . import ops
+            b -= ops.apply_physical(self, self.shift)
+        return b[self.solve_dofs]
+
+    # Device plan.
+    def device(self):
+        if self._device is None:
+            self._device = _devabi.DevicePlan(self)
+        return self._device
+
+
+def _boundary_l2_projection(plan, cons, dofs):
+    """Fixed trace values: L2 projection of g onto the active trace space."""
+    space = plan.space
+    index = {d: i for i, d in enumerate(dofs.tolist())}
+    n = len(dofs)
+    G = np.zeros((n, n))
+    r = np.zeros(n)
+    dom = space.domain
+    for con in cons:
+        terms = plan._trace_terms(con)
+        by_face = {}
+        for dof, coef, li, box in terms:
+            by_face.setdefault(li, []).append((dof, coef, box))
+        a = con.boundary.axis
+        for li, items in by_face.items():
+            deg = space.degrees[li]
+            qs = space.quadrature_shapes[li]
+            cell = space.leaves[li]
+            centers, halves = dom.cell_centers_and_half_sizes([cell])
+            tang = [ax for ax in range(space.dim) if ax != a]
+            rules = [gauss_legendre_rule(int(qs[ax])) for ax in tang]
+            grids = np.meshgrid(*[rl[0] for rl in rules], indexing="ij") if tang else []
+            wgrid = np.meshgrid(*[rl[1] for rl in rules], indexing="ij") if tang else []
+            npts = int(np.prod([int(qs[ax]) for ax in tang])) if tang else 1
+            w = np.ones(npts)
+            for wg in wgrid:
+                w = w * wg.reshape(-1)
+            w = w * float(np.prod([halves[0, ax] for ax in tang])) if tang else w
+            xi = {}
+            for k, ax in enumerate(tang):
+                xi[ax] = grids[k].reshape(-1)
+            xi[a] = np.full(npts, -1.0 if con.boundary.side == 0 else 1.0)
+            pts = tuple(centers[0, ax] + xi[ax] * halves[0, ax] for ax in range(space.dim))
+            gval = np.broadcast_to(np.asarray(con.data(pts) if callable(con.data) else con.data,
+                                              dtype=np.float64), (npts,))
+            shape = tuple(int(p) + 1 for p in deg)
+            phi = {}
+            for dof, coef, box in items:
+                idx = np.unravel_index(box, shape[::-1])[::-1]
+                val = np.ones(npts)
+                for ax in range(space.dim):
+                    from .quadrature import fe_basis_values_and_derivatives
+                    vals, _ = fe_basis_values_and_derivatives(xi[ax], int(deg[ax]))
+                    val = val * vals[:, idx[ax]]
+                phi.setdefault(dof, np.zeros(npts))
+                phi[dof] += coef * val
+            keys = list(phi)
+            for i in keys:
+                r[index[i]] += np.sum(w * gval * phi[i])
+                for j in keys:
+                    G[index[i], index[j]] += np.sum(w * phi[i] * phi[j])
+    return np.linalg.solve(G, r)
+
+
+def compile(problem):
+    if not isinstance(problem, AffineProblem):
+        raise TypeError("compile expects an AffineProblem")
+    return AssemblyPlan(problem)

@@ -1,0 +1,127 @@
+"""Problem description: fields, accumulator operators and constraints.
+
+Reference: src/problem.jl (AffineProblem 372, SPD 234, operator classes),
+src/fields.jl (ScalarField), src/accumulators.jl (TrialChannels 118-122,
+TestChannels 148-151, cell_accumulate contract 523-539, the quadrature-point
+handle 647-676). In this backend the callbacks are evaluated on the host only,
+vectorized over every quadrature point of a cell batch, to lower each
+accumulator to per-point coefficient tables; the device never calls user code.
+
+Operators are plain Python objects that implement some of:
+
+    cell_accumulate(self, q, trial, test_component) -> TestChannels or value
+    cell_rhs_accumulate(self, q, test_component)    -> TestChannels or value
+
+`q.point` is a tuple of D coordinate arrays, `q.weight` the weights,
+`q.cell_size` a tuple of D arrays; `trial.value` is an array and
+`trial.gradient` a tuple of D arrays (physical gradient). Callbacks must be
+written with numpy operations so they broadcast over points.
+"""
+
+import numpy as np
+
+LOWER = 0
+UPPER = 1
+
+
+class SPD:
+    """Declares the affine operator symmetric positive definite."""
+
+
+class NonsymmetricOperator:
+    pass
+
+
+class ScalarField:
+    def __init__(self, space, name="u"):
+        self.space = space
+        self.name = name
+        self.components = 1
+
+
+class TrialChannels:
+    def __init__(self, component, value, gradient):
+        self.component = component
+        self.value = value
+        self.gradient = gradient
+
+
+class TestChannels:
+    def __init__(self, value, gradient):
+        self.value = value
+        self.gradient = tuple(gradient)
+
+
+def value(channels):
+    return channels.value
+
+
+def gradient(channels):
+    return channels.gradient
+
+
+class QuadraturePoints:
+    """Vectorized stand-in for the reference's per-point handle."""
+
+    def __init__(self, points, weights, cell_sizes):
+        self.point = tuple(points)
+        self.weight = weights
+        self.cell_size = tuple(cell_sizes)
+
+    def coordinate(self, axis):
+        return self.point[axis]
+
+
+class BoundaryFace:
+    def __init__(self, axis, side):
+        self.axis = int(axis)
+        self.side = int(side)
+
+
+class Dirichlet:
+    """Strong Dirichlet data g on one boundary face (src/plans.jl:486-514).
+
+    `data` is a number or a callable of the point tuple (numpy arrays)."""
+
+    def __init__(self, field, boundary, data=0.0):
+        self.field = field
+        self.boundary = boundary
+        self.data = data
+
+
+class AffineProblem:
+    def __init__(self, field, operator_class=None):
+        self.field = field
+        self.operator_class = operator_class or SPD()
+        self.cell_operators = []   # (test_field, trial_field, operator)
+        self.cell_rhs = []         # (test_field, operator)
+        self.constraints = []
+
+    def add_cell_accumulator(self, *args):
+        if len(args) == 3:
+            test, trial, op = args
+            if test is not self.field or trial is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            self.cell_operators.append((test, trial, op))
+        elif len(args) == 2:
+            test, op = args
+            if test is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            self.cell_rhs.append((test, op))
+        else:
+            raise TypeError("add_cell_accumulator(test, trial, op) or (test, op)")
+        return self
+
+    def add_constraint(self, constraint):
+        if not isinstance(constraint, Dirichlet):
+            raise NotImplementedError("only Dirichlet constraints are supported")
+        self.constraints.append(constraint)
+        return self
+
+
+def add_cell_accumulator(problem, *args):
+    return problem.add_cell_accumulator(*args)
+
+
+def add_constraint(problem, constraint):
+    return problem.add_constraint(constraint)

@@ -1,0 +1,153 @@
+"""Linear solver policies and `solve` on the device.
+
+Reference: src/solve.jl (CGSolver / FGMRESSolver / JacobiPreconditioner
+policies 10-113; solve 219-243; _solve_affine 249-266 with defaults
+relative_tolerance = sqrt(eps), absolute_tolerance = 0,
+maxiter = max(1000, 2 n_red); _cg_solve 376-431; _fgmres_solve 433-542;
+Jacobi 651-774). The Krylov loops run in libgrico_mf.so; only the stopping
+test and the Givens rotations touch the host.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _devabi, _native, ops
+from .plan import AssemblyPlan, compile as compile_plan
+from .problem import AffineProblem
+
+
+class JacobiPreconditioner:
+    pass
+
+
+class CGSolver:
+    def __init__(self, preconditioner=None):
+        self.preconditioner = preconditioner
+
+
+class FGMRESSolver:
+    def __init__(self, preconditioner=None, restart=30):
+        if int(restart) < 1:
+            raise ValueError("restart must be positive")
+        self.preconditioner = preconditioner
+        self.restart = int(restart)
+
+
+class AutoLinearSolver:
+    pass
+
+
+class State:
+    """Full-layout coefficient vector of a solved plan (src/fields.jl:528-560)."""
+
+    def __init__(self, plan, coefficients, info=None):
+        self.plan = plan
+        self.coefficients = np.asarray(coefficients, dtype=np.float64)
+        self.info = info
+
+
+class ConvergenceError(RuntimeError):
+    pass
+
+
+class DevicePreconditioner:
+    """Owning wrapper of a gmf_preconditioner handle."""
+
+    def __init__(self, plan, handle, keep=()):
+        self.plan = plan
+        self.handle = handle
+        self._keep = keep
+
+    def __del__(self):
+        lib = _native._device
+        if lib is not None and getattr(self, "handle", None) is not None and self.handle.value:
+            lib.gmf_precond_destroy(self.handle)
+            self.handle = None
+
+    def apply(self, r):
+        torch = ops._torch()
+        rt = ops.to_device(r, self.plan.reduced_dof_count)
+        z = torch.empty_like(rt)
+        _devabi.check(_native.device().gmf_precond_apply(self.handle, rt.data_ptr(), z.data_ptr()))
+        return z
+
+
+def jacobi_inverse_diagonal(plan):
+    """Inverse reduced diagonal with the reference's zero / non-finite check."""
+    torch = ops._torch()
+    d = ops.reduced_diagonal(plan, as_numpy=False)
+    inv = torch.empty_like(d)
+    dev = ops._ws(plan)
+    _devabi.check(_native.device().gmf_invert_diagonal(dev.ws, d.data_ptr(), inv.data_ptr(), d.numel()))
+    return inv
+
+
+def compile_preconditioner(policy, plan):
+    if policy is None:
+        return None
+    if isinstance(policy, DevicePreconditioner):
+        return policy
+    if isinstance(policy, JacobiPreconditioner):
+        inv = jacobi_inverse_diagonal(plan)
+        handle = _native.c_void_p()
+        dev = ops._ws(plan)
+        _devabi.check(_native.device().gmf_precond_create_jacobi(dev.ws, inv.data_ptr(), ctypes.byref(handle)))
+        return DevicePreconditioner(plan, handle, keep=(inv,))
+    compile_hook = getattr(policy, "compile", None)
+    if compile_hook is not None:
+        return compile_hook(plan)
+    raise TypeError(f"unsupported preconditioner policy {type(policy).__name__}")
+
+
+def _default_solver(plan, solver):
+    if solver is None or isinstance(solver, AutoLinearSolver):
+        return CGSolver(preconditioner=JacobiPreconditioner())
+    return solver
+
+
+def solve_reduced(plan, solver, b_red, relative_tolerance=None, absolute_tolerance=0.0, maxiter=None,
+                  initial_solution=None):
+    """Solve P^T A P x = b on the device; returns (x torch tensor, info)."""
+    torch = ops._torch()
+    n = plan.reduced_dof_count
+    rtol = math.sqrt(np.finfo(np.float64).eps) if relative_tolerance is None else float(relative_tolerance)
+    atol = float(absolute_tolerance)
+    if not (math.isfinite(rtol) and rtol >= 0 and math.isfinite(atol) and atol >= 0):
+        raise ValueError("tolerances must be finite and non-negative")
+    maxiter = max(1000, 2 * n) if maxiter is None else int(maxiter)
+    if maxiter < 1:
+        raise ValueError("maxiter must be positive")
+    b = ops.to_device(b_red, n)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda") if initial_solution is None \
+        else ops.to_device(initial_solution, n).clone()
+    M = compile_preconditioner(solver.preconditioner, plan)
+    dev = ops._ws(plan)
+    info = _devabi.SolveInfo()
+    handle = M.handle if M is not None else None
+    lib = _native.device()
+    if isinstance(solver, CGSolver):
+        status = lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), int(initial_solution is not None),
+                                  rtol, atol, maxiter, ctypes.byref(info))
+    elif isinstance(solver, FGMRESSolver):
+        status = lib.gmf_fgmres_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(),
+                                      int(initial_solution is not None), solver.restart, rtol, atol, maxiter,
+                                      ctypes.byref(info))
+    else:
+        raise TypeError(f"unsupported solver {type(solver).__name__}")
+    if status == -3:
+        raise ConvergenceError(_native.device().gmf_last_error().decode())
+    _devabi.check(status)
+    return x, info
+
+
+def solve(problem_or_plan, solver=None, relative_tolerance=None, absolute_tolerance=0.0, maxiter=None,
+          initial_solution=None):
+    """solve(problem; solver=...) / solve(plan; solver=...) -> State."""
+    plan = problem_or_plan if isinstance(problem_or_plan, AssemblyPlan) else compile_plan(problem_or_plan)
+    solver = _default_solver(plan, solver)
+    b = plan.reduced_rhs()
+    x, info = solve_reduced(plan, solver, b, relative_tolerance, absolute_tolerance, maxiter, initial_solution)
+    full = ops.expand_reduced(plan, x, include_shift=True).cpu().numpy()
+    return State(plan, full, info)

@@ -1,0 +1,201 @@
+/*
+ * grico_mf.h — device C ABI of the MI355X (gfx950) matrix-free hp-FEM backend.
+ *
+ * The entry points replace the CPU runtime of Grico.jl's compiled plans. Every
+ * function cites the reference routine it stands in for (paths relative to the
+ * Grico.jl source tree). A Julia binding would `ccall` these symbols (see
+ * INTEGRATION.md); the Python package grico.jl_amd/gricoamd binds them with
+ * ctypes.
+ *
+ * Conventions
+ *  - Indices are 0-based (the Julia side subtracts 1 once, inside plan
+ *    construction).
+ *  - Descriptor arrays passed to gmf_plan_create are HOST pointers and are
+ *    copied; the caller may free them afterwards.
+ *  - Vector arguments of apply/solve entry points are DEVICE pointers
+ *    (hipMalloc'd, caller-owned, FP64), on the device the plan was created on.
+ *  - A workspace owns one HIP stream slot (default: the null stream, or the
+ *    stream set with gmf_workspace_set_stream). Calls are synchronous with
+ *    respect to the host unless stated otherwise. A plan may be shared by
+ *    several workspaces; one workspace must not be used by two threads at once.
+ *  - Every function returns a gmf_status; gmf_last_error() returns a message
+ *    for the most recent failure on the calling thread.
+ */
+#ifndef GRICO_MF_H
+#define GRICO_MF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gmf_status {
+  GMF_OK = 0,
+  GMF_EINVAL = -1,      /* wrong length / shape / argument (ArgumentError) */
+  GMF_EALIAS = -2,      /* result aliases input (src/assembly.jl:450-453) */
+  GMF_ENOCONV = -3,     /* Krylov method did not converge (src/solve.jl:430, 541) */
+  GMF_EBREAKDOWN = -4,  /* CG non-positive direction (src/solve.jl:415-416) */
+  GMF_EDIAGONAL = -5,   /* zero / non-finite Jacobi diagonal (src/solve.jl:740-751) */
+  GMF_ENOTSPD = -6,     /* coarse matrix not SPD / singular (src/multigrid.jl:387-407) */
+  GMF_EDEVICE = -7,     /* HIP runtime failure */
+  GMF_EUNSUPPORTED = -8 /* shape outside what the kernels implement */
+} gmf_status;
+
+typedef struct gmf_plan gmf_plan;
+typedef struct gmf_workspace gmf_workspace;
+
+/* Per-point action forms lowered from cell accumulators on the host. */
+enum {
+  GMF_FORM_DIFFUSION = 0, /* a_grad = c_q * grad u, a_0 = 0: one coefficient c_q*W_q per point */
+  GMF_FORM_GENERAL = 1    /* (a_0, a_grad) = M_q (u, grad u): (D+1)^2 coefficients per point */
+};
+
+/*
+ * One batch of tensor-product cells sharing degrees, quadrature shape and
+ * action form (the reference's kernel-signature batches, src/plans.jl:307-332).
+ * Boxes and point sets are ordered axis 1 fastest (src/quadrature.jl:203-205).
+ */
+typedef struct gmf_bucket_desc {
+  int32_t dim;       /* 1, 2 or 3 */
+  int32_t modes[3];  /* N_a = p_a + 1 */
+  int32_t points[3]; /* Q_a */
+  int32_t form;      /* GMF_FORM_* */
+  int64_t cell_count;
+  /* Per axis a: N_a x Q_a row-major [mode][point] tables, axes concatenated:
+   * values psi_i(xi_q) and reference-coordinate derivatives psi_i'(xi_q)
+   * (src/integration.jl:2677-2742 before the J^-1 scaling). */
+  const double* values;
+  const double* derivatives;
+  const double* inverse_jacobian; /* cell_count x dim (diagonal J^-1) */
+  /* cell_count x Q x k, Q = prod Q_a: for DIFFUSION k = 1 (W_q c_q); for
+   * GENERAL k = (D+1)^2 row-major [test channel][trial channel] of W_q M_q,
+   * channel 0 = value, 1+a = physical derivative along axis a. */
+  const double* point_data;
+  /* cell_count x N, N = prod N_a (full box): >= 0 single global dof,
+   * -1 inactive / constrained-to-zero mode, <= -2 multi-term entry -(v+2). */
+  const int32_t* local_index;
+  const double* local_coefficient; /* cell_count x N single-term coefficients, NULL = all 1 */
+  int64_t multi_count;
+  const int64_t* multi_offsets; /* multi_count + 1 */
+  const int32_t* multi_indices;
+  const double* multi_coefficients;
+} gmf_bucket_desc;
+
+/* Compiled plan: cell batches + algebraic constraint rows + reduced map
+ * (src/plans.jl:113-145 AssemblyPlan; src/assembly.jl:19-27 _ReducedOperatorMap). */
+typedef struct gmf_plan_desc {
+  int32_t dim;
+  int64_t full_dof_count;
+  int32_t bucket_count;
+  const gmf_bucket_desc* buckets;
+  /* Constraint rows written after the cell pass (src/assembly.jl:1893-1920). */
+  int64_t fixed_count;
+  const int64_t* fixed_dofs; /* result[d] = x[d] */
+  int64_t affine_count;      /* result[p] = x[p] - sum c_j x[j] */
+  const int64_t* affine_pivots;
+  const int64_t* affine_offsets;
+  const int64_t* affine_indices;
+  const double* affine_coefficients;
+  int64_t mean_count; /* result[p] = sum c_j x[j] */
+  const int64_t* mean_pivots;
+  const int64_t* mean_offsets;
+  const int64_t* mean_indices;
+  const double* mean_coefficients;
+  /* u_full = shift + P u_red, CSR over full dofs (src/assembly.jl:282-314). */
+  int64_t reduced_dof_count;
+  const int64_t* map_offsets; /* full_dof_count + 1 */
+  const int64_t* map_indices;
+  const double* map_coefficients;
+  const double* shift;      /* full_dof_count */
+} gmf_plan_desc;
+
+typedef struct gmf_solve_info {
+  int64_t iterations;
+  int64_t operator_applications;
+  int64_t preconditioner_applications;
+  double initial_residual_norm;
+  double final_residual_norm;
+  double rhs_norm;
+  int32_t converged;
+} gmf_solve_info;
+
+/* Opaque preconditioner: z = M^-1 r on reduced vectors (src/solve.jl:32
+ * AbstractPreconditioner, 641-670 _compile_preconditioner, 753-774
+ * _apply_preconditioner!). NULL means the identity. */
+typedef struct gmf_preconditioner gmf_preconditioner;
+
+const char* gmf_last_error(void);
+
+/* Device properties of the current HIP device (name buffer >= 64 bytes). */
+gmf_status gmf_device_info(char* name, int32_t* compute_units, int64_t* global_memory_bytes);
+
+/* Plan lifetime: replaces compile(problem)'s frozen AssemblyPlan
+ * (src/plans.jl:170-231); tables are uploaded to HBM once. */
+gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** plan);
+gmf_status gmf_plan_destroy(gmf_plan* plan);
+gmf_status gmf_plan_sizes(const gmf_plan* plan, int64_t* full_dof_count, int64_t* reduced_dof_count,
+                          int64_t* device_bytes);
+
+/* Workspace: per-caller scratch tied to one plan (src/assembly.jl:388-441,
+ * 566-584 OperatorWorkspace / _ReducedOperatorWorkspace). `stream` is a
+ * hipStream_t (NULL = null stream). */
+gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** ws);
+gmf_status gmf_workspace_destroy(gmf_workspace* ws);
+gmf_status gmf_workspace_set_stream(gmf_workspace* ws, void* stream);
+
+/* y = A x on the full layout with constraint rows (src/assembly.jl:540-564 apply!). */
+gmf_status gmf_apply(gmf_workspace* ws, const double* x, double* y);
+/* Same, enqueued on the workspace stream without a host synchronization. */
+gmf_status gmf_apply_async(gmf_workspace* ws, const double* x, double* y);
+/* y = A x on the full layout without constraint rows (src/assembly.jl:603-622 _apply_physical!). */
+gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y);
+/* y_red = P^T A P x_red (src/assembly.jl:905-915 _reduced_apply!). */
+gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* x_red, double* y_red);
+/* x_full = [shift] + P x_red (src/assembly.jl:851-874 _expand_reduced!). */
+gmf_status gmf_expand_reduced(gmf_workspace* ws, const double* x_red, double* x_full, int32_t include_shift);
+/* y_red = P^T y_full (src/assembly.jl:876-891 _project_reduced!). */
+gmf_status gmf_project_reduced(gmf_workspace* ws, const double* y_full, double* y_red);
+/* Diagonal of the physical operator on the full layout, scattered with c^2
+ * (src/assembly.jl:624-639 _diagonal_physical!, src/accumulators.jl:1056-1083). */
+gmf_status gmf_diagonal_physical(gmf_workspace* ws, double* diag_full);
+/* Exact diagonal of P^T A P (src/assembly.jl:662-675 _reduced_diagonal! and the
+ * probing fallback src/solve.jl:722-738, computed without probing). */
+gmf_status gmf_reduced_diagonal(gmf_workspace* ws, double* diag_red);
+
+/* Jacobi preconditioner from an inverse diagonal (device vector of length
+ * reduced_dof_count, copied) (src/solve.jl:651-656, 761-774). */
+gmf_status gmf_precond_create_jacobi(gmf_workspace* ws, const double* inverse_diagonal,
+                                     gmf_preconditioner** out);
+gmf_status gmf_precond_destroy(gmf_preconditioner* precond);
+/* z = M^-1 r on the workspace stream. */
+gmf_status gmf_precond_apply(gmf_preconditioner* precond, const double* r, double* z);
+
+/* Preconditioned CG on the reduced system from x (in: initial guess if
+ * use_initial, else zero), stopping at ||r|| <= max(atol, rtol ||b||)
+ * (src/solve.jl:376-431 _cg_solve). */
+gmf_status gmf_cg_solve(gmf_workspace* ws, gmf_preconditioner* precond, const double* b, double* x,
+                        int32_t use_initial, double rtol, double atol, int64_t maxiter,
+                        gmf_solve_info* info);
+/* Restarted right-preconditioned FGMRES with modified Gram-Schmidt and Givens
+ * rotations (src/solve.jl:433-614 _fgmres_solve). */
+gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* precond, const double* b, double* x,
+                            int32_t use_initial, int32_t restart, double rtol, double atol,
+                            int64_t maxiter, gmf_solve_info* info);
+
+/* Timed repetitions of gmf_apply on the workspace stream with HIP events
+ * around each stage: average milliseconds of the cell-kernel pass and of the
+ * whole apply (the measurement role of src/diagnostics.jl:21-59
+ * operator_diagnostics). */
+gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_t repetitions,
+                           double* cell_ms, double* total_ms);
+
+/* Jacobi inverse diagonal with validation: rejects zero / non-finite entries
+ * (src/solve.jl:740-751). */
+gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* diag, double* inverse_diagonal, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRICO_MF_H */

@@ -1,0 +1,273 @@
+"""ORACLE (test infrastructure only): Python driver of oracle/oracle.c.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module. It restates the reference's per-cell pipeline: gather ->
+interpolate (value + gradients) -> user callback on the actual trial channels
+at every point -> weighted projection -> scatter skipping fixed rows ->
+constraint rows (src/assembly.jl:540-564, 1160-1174;
+src/accumulators.jl:1955-2013). It reads the plan's host tables (term maps,
+points, weights) but none of the device lowering.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+P_f64 = ctypes.POINTER(ctypes.c_double)
+P_i64 = ctypes.POINTER(ctypes.c_int64)
+P_i32 = ctypes.POINTER(ctypes.c_int32)
+P_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+class OracleBucket(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int), ("N", ctypes.c_int * 3), ("Q", ctypes.c_int * 3), ("cells", ctypes.c_int64),
+        ("values", P_f64 * 3), ("derivatives", P_f64 * 3), ("inverse_jacobian", P_f64), ("weights", P_f64),
+        ("term_offsets", P_i64), ("term_indices", P_i64), ("term_coefficients", P_f64),
+    ]
+
+
+OPERATOR = ctypes.CFUNCTYPE(None, ctypes.c_void_p, P_f64, P_f64)
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so from oracle.c (gcc, OpenMP)."""
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11", "-Wall",
+                    "-o", LIB, os.path.join(HERE, "oracle.c"), "-lm"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        B = ctypes.POINTER(OracleBucket)
+        L.oracle_interpolate.argtypes = [B, P_f64, P_f64, P_f64]
+        L.oracle_project.argtypes = [B, P_f64, P_f64, P_u8, P_f64]
+        L.oracle_fixed_rows.argtypes = [P_i64, ctypes.c_int64, P_f64, P_f64]
+        L.oracle_apply_lowered.argtypes = [B, P_f64, P_f64, P_u8, P_f64, P_i64, P_i64, ctypes.c_int, ctypes.c_int]
+        L.oracle_apply_basis.argtypes = [B, P_f64, P_f64, P_u8, P_f64]
+        L.oracle_diagonal.argtypes = [B, P_f64, P_f64]
+        L.oracle_color_cells.argtypes = [B, ctypes.c_int64, P_i32]
+        L.oracle_color_cells.restype = ctypes.c_int
+        L.oracle_cg.argtypes = [OPERATOR, ctypes.c_void_p, ctypes.c_int64, P_f64, P_f64, P_f64,
+                                ctypes.c_double, ctypes.c_double, ctypes.c_int64]
+        L.oracle_cg.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    if a is None:
+        return ctypes.cast(None, ctypes.POINTER(t))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class _Bucket:
+    """ctypes view of one plan bucket with its arrays kept alive."""
+
+    def __init__(self, bk):
+        self.bk = bk
+        nc, nb = bk.cell_count, bk.n_modes
+        order = np.lexsort((bk.term_box, bk.term_cell))
+        entry = bk.term_cell[order] * nb + bk.term_box[order]
+        counts = np.bincount(entry, minlength=nc * nb)
+        self.offsets = np.zeros(nc * nb + 1, dtype=np.int64)
+        self.offsets[1:] = np.cumsum(counts)
+        self.indices = np.ascontiguousarray(bk.term_dof[order], dtype=np.int64)
+        self.coefs = np.ascontiguousarray(bk.term_coef[order], dtype=np.float64)
+        self.values = [np.ascontiguousarray(t) for t in bk.B]
+        self.derivs = [np.ascontiguousarray(t) for t in bk.G]
+        self.jinv = np.ascontiguousarray(bk.inverse_jacobian)
+        self.weights = np.ascontiguousarray(bk.weights)
+        s = OracleBucket()
+        s.dim = bk.dim
+        for a in range(3):
+            s.N[a] = bk.modes[a] if a < bk.dim else 1
+            s.Q[a] = bk.points[a] if a < bk.dim else 1
+            if a < bk.dim:
+                s.values[a] = _p(self.values[a], ctypes.c_double)
+                s.derivatives[a] = _p(self.derivs[a], ctypes.c_double)
+        s.cells = nc
+        s.inverse_jacobian = _p(self.jinv, ctypes.c_double)
+        s.weights = _p(self.weights, ctypes.c_double)
+        s.term_offsets = _p(self.offsets, ctypes.c_int64)
+        s.term_indices = _p(self.indices, ctypes.c_int64)
+        s.term_coefficients = _p(self.coefs, ctypes.c_double)
+        self.struct = s
+
+    def ref(self):
+        return ctypes.byref(self.struct)
+
+
+def _fixed_mask(plan):
+    m = np.zeros(plan.full_dof_count, dtype=np.uint8)
+    m[plan.fixed_dofs] = 1
+    return m
+
+
+def _trial(value, gradient):
+    from gricoamd.problem import TrialChannels
+    return TrialChannels(0, value, tuple(gradient))
+
+
+def _test_channels(result, shape, D):
+    from gricoamd.problem import TestChannels
+    if isinstance(result, TestChannels):
+        v = np.broadcast_to(np.asarray(result.value, dtype=np.float64), shape)
+        g = [np.broadcast_to(np.asarray(c, dtype=np.float64), shape) for c in result.gradient]
+        return v, g
+    return np.broadcast_to(np.asarray(result, dtype=np.float64), shape), [np.zeros(shape)] * D
+
+
+def apply(plan, x, physical=False):
+    """Reference-algorithm full-layout apply (callback evaluated on the actual
+    trial channels). physical=True skips the constraint handling."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(plan.full_dof_count)
+    fixed = None if physical else _fixed_mask(plan)
+    for bk in plan.buckets:
+        ob = _Bucket(bk)
+        nc, nq, D = bk.cell_count, bk.n_points, bk.dim
+        vals = np.zeros((nc, nq))
+        grads = np.zeros((nc, nq, D))
+        L.oracle_interpolate(ob.ref(), _p(x, ctypes.c_double), _p(vals, ctypes.c_double), _p(grads, ctypes.c_double))
+        q = bk.quadrature()
+        a0 = np.zeros((nc, nq))
+        ag = np.zeros((nc, nq, D))
+        for (_, _, op) in plan.problem.cell_operators:
+            v, g = _test_channels(op.cell_accumulate(q, _trial(vals, [grads[..., a] for a in range(D)]), 0),
+                                  (nc, nq), D)
+            a0 += bk.weights * v
+            for a in range(D):
+                ag[..., a] += bk.weights * g[a]
+        a0 = np.ascontiguousarray(a0)
+        ag = np.ascontiguousarray(ag)
+        L.oracle_project(ob.ref(), _p(a0, ctypes.c_double), _p(ag, ctypes.c_double),
+                         _p(fixed, ctypes.c_uint8), _p(y, ctypes.c_double))
+    if not physical:
+        fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
+        L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+    return y
+
+
+def action_matrices(plan, bk):
+    """Per-point (D+1)x(D+1) action of the plan's accumulators, probed with
+    unit trial channels (the accumulator contract is linear in the trial)."""
+    nc, nq, D = bk.cell_count, bk.n_points, bk.dim
+    q = bk.quadrature()
+    M = np.zeros((nc, nq, D + 1, D + 1))
+    ones, zeros = np.ones((nc, nq)), np.zeros((nc, nq))
+    for (_, _, op) in plan.problem.cell_operators:
+        for j in range(D + 1):
+            val = ones if j == 0 else zeros
+            grad = [ones if j == a + 1 else zeros for a in range(D)]
+            v, g = _test_channels(op.cell_accumulate(q, _trial(val, grad), 0), (nc, nq), D)
+            M[..., 0, j] += v
+            for a in range(D):
+                M[..., a + 1, j] += g[a]
+    return np.ascontiguousarray(M.reshape(nc * nq, (D + 1) ** 2))
+
+
+def apply_lowered(plan, x, threads=0, matrices=None, colors=None, physical=False):
+    """Whole-pipeline C apply with probed action matrices; threaded over
+    DOF-disjoint colors. Returns y."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(plan.full_dof_count)
+    fixed = None if physical else _fixed_mask(plan)
+    for i, bk in enumerate(plan.buckets):
+        ob = _Bucket(bk)
+        M = matrices[i] if matrices is not None else action_matrices(plan, bk)
+        co, cc = colors[i] if colors is not None else color_batches(plan, ob)
+        L.oracle_apply_lowered(ob.ref(), _p(M, ctypes.c_double), _p(x, ctypes.c_double), _p(fixed, ctypes.c_uint8),
+                               _p(y, ctypes.c_double), _p(co, ctypes.c_int64), _p(cc, ctypes.c_int64),
+                               len(co) - 1, int(threads))
+    if not physical:
+        fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
+        L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+    return y
+
+
+def color_batches(plan, ob):
+    L = lib()
+    col = np.zeros(ob.bk.cell_count, dtype=np.int32)
+    k = L.oracle_color_cells(ob.ref(), plan.full_dof_count, _p(col, ctypes.c_int32))
+    if k < 0:
+        raise RuntimeError("more than 64 colors")
+    order = np.argsort(col, kind="stable").astype(np.int64)
+    offsets = np.zeros(k + 1, dtype=np.int64)
+    offsets[1:] = np.cumsum(np.bincount(col, minlength=k))
+    return offsets, np.ascontiguousarray(order)
+
+
+def apply_basis(plan, x, physical=False):
+    """Brute-force full-table path (src/accumulators.jl:1022-1054)."""
+    L = lib()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros(plan.full_dof_count)
+    fixed = None if physical else _fixed_mask(plan)
+    for bk in plan.buckets:
+        ob = _Bucket(bk)
+        M = action_matrices(plan, bk)
+        L.oracle_apply_basis(ob.ref(), _p(M, ctypes.c_double), _p(x, ctypes.c_double), _p(fixed, ctypes.c_uint8),
+                             _p(y, ctypes.c_double))
+    if not physical:
+        fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
+        L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+    return y
+
+
+def diagonal_physical(plan):
+    L = lib()
+    d = np.zeros(plan.full_dof_count)
+    for bk in plan.buckets:
+        ob = _Bucket(bk)
+        M = action_matrices(plan, bk)
+        L.oracle_diagonal(ob.ref(), _p(M, ctypes.c_double), _p(d, ctypes.c_double))
+    return d
+
+
+def reduced_apply(plan, x_red):
+    """P^T A P x with the selection map of fixed-row plans."""
+    full = np.zeros(plan.full_dof_count)
+    full[plan.solve_dofs] = x_red
+    return apply(plan, full, physical=True)[plan.solve_dofs]
+
+
+def cg(plan, b_red, inv_diag=None, rtol=1e-10, atol=0.0, maxiter=10000, lowered=True, threads=0):
+    """Reference CG on the reduced system with the oracle operator."""
+    L = lib()
+    n = plan.reduced_dof_count
+    mats = [action_matrices(plan, bk) for bk in plan.buckets] if lowered else None
+    cols = [color_batches(plan, _Bucket(bk)) for bk in plan.buckets] if lowered else None
+
+    def op(_ctx, xp, yp):
+        xr = np.ctypeslib.as_array(xp, shape=(n,))
+        yr = np.ctypeslib.as_array(yp, shape=(n,))
+        full = np.zeros(plan.full_dof_count)
+        full[plan.solve_dofs] = xr
+        if lowered:
+            out = apply_lowered(plan, full, threads=threads, matrices=mats, colors=cols, physical=True)
+        else:
+            out = apply(plan, full, physical=True)
+        yr[:] = out[plan.solve_dofs]
+
+    cb = OPERATOR(op)
+    b = np.ascontiguousarray(b_red, dtype=np.float64)
+    x = np.zeros(n)
+    inv = None if inv_diag is None else np.ascontiguousarray(inv_diag, dtype=np.float64)
+    it = L.oracle_cg(cb, None, n, _p(inv, ctypes.c_double), _p(b, ctypes.c_double), _p(x, ctypes.c_double),
+                     float(rtol), float(atol), int(maxiter))
+    return x, int(it)

@@ -1,0 +1,128 @@
+"""Problem builders shared by the tests (mirroring the reference's test problems
+and the five benchmark configurations at reduced sizes)."""
+
+import numpy as np
+
+import gricoamd as g
+
+
+class Diffusion:
+    """TestChannels(0, k grad u) (examples/poisson_matrix_free_gmg/driver.jl:34-36)."""
+
+    def __init__(self, kappa=None):
+        self.kappa = kappa
+
+    def cell_accumulate(self, q, trial, test_component):
+        k = 1.0 if self.kappa is None else self.kappa(q.point)
+        return g.TestChannels(0.0 * trial.value, tuple(k * c for c in trial.gradient))
+
+
+class Mass:
+    def __init__(self, scale=1.0):
+        self.scale = scale
+
+    def cell_accumulate(self, q, trial, test_component):
+        return self.scale * trial.value
+
+
+class MassDiffusion:
+    """Nonsymmetric-capable general form: a0 = m u + b.grad u, a_grad = k grad u."""
+
+    def __init__(self, m=1.0, k=1.0, b=None):
+        self.m, self.k, self.b = m, k, b
+
+    def cell_accumulate(self, q, trial, test_component):
+        v = self.m * trial.value
+        if self.b is not None:
+            for a, ba in enumerate(self.b):
+                v = v + ba * trial.gradient[a]
+        return g.TestChannels(v, tuple(self.k * c for c in trial.gradient))
+
+
+class SineLoad:
+    def __init__(self, dim):
+        self.dim = dim
+
+    def cell_rhs_accumulate(self, q, test_component):
+        out = self.dim * np.pi ** 2
+        for x in q.point:
+            out = out * np.sin(np.pi * x)
+        return out
+
+
+def sine_exact(p):
+    out = 1.0
+    for x in p:
+        out = out * np.sin(np.pi * x)
+    return out
+
+
+def kappa_c2(p):
+    """kappa = 1 + 0.5 sin(2 pi x) sin(2 pi y) sin(2 pi z)."""
+    return 1.0 + 0.5 * np.sin(2 * np.pi * p[0]) * np.sin(2 * np.pi * p[1]) * np.sin(2 * np.pi * p[2])
+
+
+def poisson(dim, cells, degree, kappa=None, dirichlet=True, basis=None, rhs=True, refine_depth=0,
+            operator=None, continuity="cg"):
+    if refine_depth:
+        domain = g.Domain((0.0,) * dim, (1.0,) * dim, (1,) * dim)
+        g.uniform_refine(domain, refine_depth)
+    else:
+        domain = g.Domain((0.0,) * dim, (1.0,) * dim, (cells,) * dim if np.ndim(cells) == 0 else cells)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=basis or g.FullTensorBasis(), degree=g.UniformDegree(degree),
+                                             continuity=continuity))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, operator or Diffusion(kappa))
+    if rhs:
+        problem.add_cell_accumulator(u, SineLoad(dim))
+    if dirichlet:
+        for axis in range(dim):
+            for side in (g.LOWER, g.UPPER):
+                problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return problem
+
+
+def hanging_problem(degree=2, operator=None, dim=2):
+    """Two root cells, the second bisected: one hanging face (test/space.jl:304-316)."""
+    if dim == 2:
+        domain = g.Domain((0.0, 0.0), (2.0, 1.0), (2, 1))
+        domain.refine(1, 1)
+    else:
+        domain = g.Domain((0.0, 0.0, 0.0), (2.0, 1.0, 1.0), (2, 1, 1))
+        first = domain.refine(1, 1)
+        domain.refine(first, 2)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, operator or MassDiffusion(1.0, 1.0))
+    return problem
+
+
+def corner_refined(dim=2, base=4, levels=2, degree=2, operator=None, dirichlet=True):
+    """Uniform base grid with repeated all-axis refinement of the cell at the origin."""
+    domain = g.Domain((0.0,) * dim, (1.0,) * dim, (base,) * dim)
+    target = 0
+    for _ in range(levels):
+        leaves = [target]
+        for axis in range(dim):
+            nxt = []
+            for c in leaves:
+                f = domain.refine(c, axis)
+                nxt += [f, f + 1]
+            leaves = nxt
+        target = leaves[0]
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, operator or Diffusion())
+    problem.add_cell_accumulator(u, SineLoad(dim))
+    if dirichlet:
+        for axis in range(dim):
+            for side in (g.LOWER, g.UPPER):
+                problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return problem
+
+
+def rng_vector(n, seed=0):
+    return np.random.default_rng(seed).standard_normal(n)
