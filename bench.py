@@ -116,6 +116,7 @@ def main():
     t0 = time.perf_counter()
     dev = ops._ws(plan)
     log(f"device upload: {time.perf_counter() - t0:.1f}s, {dev.device_bytes() / 2**20:.0f} MiB")
+    log(dev.describe().strip())
 
     n = plan.full_dof_count
     gen = torch.Generator(device="cuda")

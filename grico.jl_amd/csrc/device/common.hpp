@@ -31,6 +31,18 @@ inline void require(bool ok, gmf_status s, const std::string& msg) {
 constexpr int kMaxDim = 3;
 constexpr int kMaxExtent = 16;  // per-axis modes / points handled by the generic kernel
 
+// 2x2x2-brick decomposition of a fast-path bucket (cell_fast.hip).
+struct BrickPath {
+  bool enabled = false;
+  int64_t bricks = 0;
+  int colors = 0;
+  std::vector<int64_t> color_offsets;  // host: bricks of color c are order[off[c]..off[c+1])
+  int64_t* order = nullptr;            // brick ids sorted by color
+  int32_t* map_full = nullptr;         // bricks x (2N-1)^3 lattice -> full dof
+  int32_t* map_red = nullptr;          // same, reduced dofs (-1 = fixed)
+  uint8_t* first = nullptr;            // 1 where the brick is the first writer of the dof
+};
+
 // Device-resident cell batch.
 struct Bucket {
   int dim = 0;
@@ -54,9 +66,11 @@ struct Bucket {
   double* mcoef = nullptr;
   int table_offset[kMaxDim] = {0, 0, 0};  // offset of axis a in values/derivs
   std::vector<double> h_values, h_derivs;
-  // Fast-path data (3D, N == Q, diffusion): collocation derivative matrix.
-  double* colloc = nullptr;  // Q x Q, colloc[q][s] = l_s'(xi_q)
+  // Fast-path data (3D, N == Q, diffusion): collocation derivative matrix
+  // Dc[q*N+s] = l_s'(xi_q), passed to the kernel by value.
+  std::vector<double> h_colloc;
   bool fast = false;
+  BrickPath bricks;
 };
 
 struct Csr {
@@ -111,7 +125,22 @@ void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double*
                                   hipStream_t stream);
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                       hipStream_t stream);
-void prepare_fast_path(gmf_plan& plan, Bucket& b);
+// lidx: host copy of the bucket's full-layout index map; red_of_full: full dof
+// -> reduced dof (empty unless the reduced map is a selection).
+void prepare_fast_path(gmf_plan& plan, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full);
+bool plan_writes_every_dof(const gmf_plan& plan);
+
+// Copy a host array to a new device allocation owned by the plan.
+template <class T>
+T* upload_array(gmf_plan& p, const T* host, size_t n) {
+  if (n == 0 || host == nullptr) return nullptr;
+  void* d = nullptr;
+  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
+  p.allocations.push_back(d);
+  p.device_bytes += int64_t(n * sizeof(T));
+  GMF_HIP(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return static_cast<T*>(d);
+}
 
 // Vector kernels (blas.hip).
 void fill(double* x, double v, int64_t n, hipStream_t s);

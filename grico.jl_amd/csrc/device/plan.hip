@@ -47,13 +47,7 @@ gmf_status guarded(F&& f) {
 
 template <class T>
 T* upload(gmf_plan& p, const T* host, size_t n) {
-  if (n == 0 || host == nullptr) return nullptr;
-  void* d = nullptr;
-  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
-  p.allocations.push_back(d);
-  p.device_bytes += int64_t(n * sizeof(T));
-  GMF_HIP(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
-  return static_cast<T*>(d);
+  return upload_array(p, host, n);
 }
 
 template <class T>
@@ -167,7 +161,7 @@ void constraint_rows(gmf_workspace* ws, const double* x, double* y) {
 
 void apply_full(gmf_workspace* ws, const double* x, double* y) {
   require(x != y, GMF_EALIAS, "result and coefficient vector must not alias");
-  fill(y, 0.0, ws->plan->n_full, ws->stream);
+  if (!plan_writes_every_dof(*ws->plan)) fill(y, 0.0, ws->plan->n_full, ws->stream);
   cell_pass(ws, x, y, false);
   constraint_rows(ws, x, y);
 }
@@ -193,7 +187,7 @@ void reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
   if (p.selection_map) {
     // P is a 0/1 selection: the cell kernels gather from and scatter to the
     // reduced vector directly through reduced index maps.
-    fill(yr, 0.0, p.n_red, ws->stream);
+    if (!plan_writes_every_dof(p)) fill(yr, 0.0, p.n_red, ws->stream);
     cell_pass(ws, xr, yr, true);
     return;
   }
@@ -407,7 +401,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
             b.mridx = upload(p, mr.data(), size_t(nt));
           }
         }
-        prepare_fast_path(p, b);
+        prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
         p.buckets.push_back(std::move(b));
       }
       GMF_HIP(hipDeviceSynchronize());
@@ -433,6 +427,33 @@ gmf_status gmf_plan_sizes(const gmf_plan* plan, int64_t* nf, int64_t* nr, int64_
     if (nf) *nf = plan->n_full;
     if (nr) *nr = plan->n_red;
     if (bytes) *bytes = plan->device_bytes;
+  });
+}
+
+gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity) {
+  return guarded([&] {
+    require(plan && buf && capacity > 0, GMF_EINVAL, "invalid argument");
+    std::string s = "full_dofs=" + std::to_string(plan->n_full) + " reduced_dofs=" + std::to_string(plan->n_red) +
+                    " selection_map=" + std::to_string(int(plan->selection_map)) +
+                    " device_bytes=" + std::to_string(plan->device_bytes) + "\n";
+    for (size_t i = 0; i < plan->buckets.size(); ++i) {
+      const Bucket& b = plan->buckets[i];
+      s += "bucket " + std::to_string(i) + ": dim=" + std::to_string(b.dim) + " N=" + std::to_string(b.N[0]) +
+           " Q=" + std::to_string(b.Q[0]) + " cells=" + std::to_string(b.cells) + " form=" + std::to_string(b.form) +
+           " multi=" + std::to_string(b.multi_count) + " path=" +
+           (b.bricks.enabled ? "brick" : (b.fast ? "fast-cell" : "generic"));
+      if (b.bricks.enabled) {
+        s += " bricks=" + std::to_string(b.bricks.bricks) + " colors=" + std::to_string(b.bricks.colors) + " [";
+        for (int c = 0; c < b.bricks.colors; ++c)
+          s += std::to_string(b.bricks.color_offsets[size_t(c) + 1] - b.bricks.color_offsets[size_t(c)]) +
+               (c + 1 < b.bricks.colors ? "," : "");
+        s += "]";
+      }
+      s += "\n";
+    }
+    size_t n = std::min(s.size(), size_t(capacity - 1));
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
   });
 }
 
@@ -487,7 +508,7 @@ gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y) {
   return guarded([&] {
     require(ws && x && y, GMF_EINVAL, "null argument");
     require(x != y, GMF_EALIAS, "result and coefficient vector must not alias");
-    fill(y, 0.0, ws->plan->n_full, ws->stream);
+    if (!plan_writes_every_dof(*ws->plan)) fill(y, 0.0, ws->plan->n_full, ws->stream);
     cell_pass(ws, x, y, false);
     sync(ws);
   });
@@ -506,7 +527,7 @@ gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_
     double cell = 0.0, total = 0.0;
     for (int r = 0; r < reps; ++r) {
       GMF_HIP(hipEventRecord(e0, ws->stream));
-      fill(y, 0.0, ws->plan->n_full, ws->stream);
+      if (!plan_writes_every_dof(*ws->plan)) fill(y, 0.0, ws->plan->n_full, ws->stream);
       GMF_HIP(hipEventRecord(e1, ws->stream));
       cell_pass(ws, x, y, false);
       GMF_HIP(hipEventRecord(e2, ws->stream));

@@ -137,6 +137,11 @@ gmf_status gmf_plan_destroy(gmf_plan* plan);
 gmf_status gmf_plan_sizes(const gmf_plan* plan, int64_t* full_dof_count, int64_t* reduced_dof_count,
                           int64_t* device_bytes);
 
+/* Human-readable summary of the device plan (batches, kernel paths, brick
+ * colors, bytes), NUL-terminated into buf (diagnostics; no reference
+ * counterpart). */
+gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity);
+
 /* Workspace: per-caller scratch tied to one plan (src/assembly.jl:388-441,
  * 566-584 OperatorWorkspace / _ReducedOperatorWorkspace). `stream` is a
  * hipStream_t (NULL = null stream). */
