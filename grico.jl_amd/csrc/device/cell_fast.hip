@@ -8,23 +8,27 @@
 // the points with the collocation matrix Dc = (B^-1 G)^T (3 passes), apply
 // the per-point coefficient, and run the transposes back (3 + 3 passes). In
 // exact arithmetic Dc applied to B^T u equals G^T u, so the result is the
-// reference's operator; the test suite holds it to 1e-12 of the oracle.
+// reference's operator; the test suite holds it to 1e-12 of the oracle. Every
+// one-dimensional pass uses the even-odd split of eo.hpp.
 //
-// Thread layout: N*N threads per cell, each owning one pencil of N values in
-// registers; passes along the owned axis run in registers, the others after
-// an LDS transpose.
+// Thread layout: one cell per LPC lanes of a wavefront (LPC = N*N rounded up
+// to a power of two), each active lane owning one pencil of N values in
+// registers. Passes along the owned axis run in registers, the others after
+// an LDS transpose. A cell never spans two wavefronts, so the transposes only
+// need wavefront-level ordering (LDS operations of one wavefront complete in
+// issue order) and the cell body contains no workgroup barrier.
 //
 // Two drivers share that cell body:
 //  * brick kernel: one workgroup per brick of 2x2x2 cells that are
 //    consecutive in Morton order. The brick's (2N-1)^3 lattice of modes is
 //    gathered into LDS once, the eight cells read their modes from it and
-//    add their results into an LDS lattice (ds_add_f64), and the lattice is
-//    written back once: plain stores where this brick is the first writer of
-//    the dof, plain read-modify-write otherwise. Bricks are launched one
+//    add their results into an LDS accumulator lattice (ds_add_f64), which
+//    is written back once: plain stores where this brick is the first writer
+//    of the dof, plain read-modify-write otherwise. Bricks are launched one
 //    color at a time (greedy DOF-disjoint coloring), so no two workgroups of
 //    a launch touch the same dof and no global atomics are needed.
-//  * cell kernel: CPB independent cells per workgroup with global atomic
-//    scatter, used when the cells do not form verified bricks.
+//  * cell kernel: independent cells with a global atomic scatter, used when
+//    the cells do not form verified bricks.
 // Both remap workgroup ids so that consecutive Morton-ordered work stays on
 // one XCD (workgroups are dealt to the 8 XCDs round-robin).
 #include <algorithm>
@@ -32,17 +36,12 @@
 #include <vector>
 
 #include "common.hpp"
+#include "eo.hpp"
 
 namespace gmf {
 namespace {
 
 constexpr int kXcds = 8;
-
-template <int N>
-struct FastTables {
-  double B[N * N];  // B[i*N + q] = psi_i(xi_q)
-  double D[N * N];  // D[q*N + s] = d/dxi of the Lagrange polynomial s at xi_q
-};
 
 __device__ inline int64_t xcd_block(int64_t b, int64_t n) {
   const int64_t q = n / kXcds, r = n % kXcds;
@@ -50,157 +49,119 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t n) {
   return x * q + (x < r ? x : r) + local;
 }
 
-// Per-cell body. In: u = z-pencil of modes (i=t0, j=t1, k), cx = x-pencil of
-// point coefficients (q, t0, t1), st = all point coefficients of the cell in
-// LDS, squared inverse Jacobians. Out: r = x-pencil of results (i, j=t0,
-// k=t1). s1/s2: two N^3 LDS scratch arrays of this cell. Contains barriers:
-// every thread of the workgroup must call it.
+// Lanes per cell: N*N rounded up to a power of two (<= 64).
 template <int N>
-__device__ inline void cell_body(const FastTables<N>& T, double (&u)[N], const double (&cx)[N], const double* st,
-                                 double jx2, double jy2, double jz2, double* s1, double* s2, int t0, int t1,
-                                 double (&r)[N]) {
+constexpr int lanes_per_cell() {
+  return N * N <= 4 ? 4 : N * N <= 16 ? 16 : N * N <= 32 ? 32 : 64;
+}
+
+// Order LDS traffic of this wavefront: no compiler motion across, and the
+// LDS unit completes one wavefront's operations in issue order.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-cell body. In: u = z-pencil of modes (i=t0, j=t1, k); cx = x-pencil
+// point coefficients (q, t0, t1); cbase = the cell's point coefficients in
+// global memory (y/z pencils are loaded here); squared inverse Jacobians.
+// Out: r = x-pencil of results (i, j=t0, k=t1). s1/s2: the cell's two N^3 LDS
+// scratch arrays. Lanes with act == false hold valid (t0, t1) = (0, 0) and
+// never write.
+template <int N>
+__device__ inline void cell_body(const EOTables<N>& T, const double (&u)[N], const double (&cx)[N],
+                                 const double* __restrict__ cbase, double jx2, double jy2, double jz2, double* s1,
+                                 double* s2, int t0, int t1, bool act, double (&r)[N]) {
   constexpr int NN = N * N;
-  // z-pass (registers), store (t0, t1, qz).
+  double a[N], b[N];
+  // z: modes -> points, stored at (t0, t1, qz).
+  eo_interp<N>(T, u, a);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (act) s1[t0 + N * t1 + NN * q] = a[q];
+  wave_sync();
+  // y: lane (i=t0, qz=t1).
+#pragma unroll
+  for (int j = 0; j < N; ++j) a[j] = s1[t0 + N * j + NN * t1];
+  eo_interp<N>(T, a, b);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (act) s2[t0 + N * q + NN * t1] = b[q];
+  wave_sync();
+  // x: lane (qy=t0, qz=t1): point values U and d/dxi.
+  double U[N], gx[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) a[i] = s2[i + N * t0 + NN * t1];
+  // The y/z-pencil coefficients of this cell (same cache lines as cx).
+  double cy[N], cz[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
-    double v = 0.0;
-#pragma unroll
-    for (int k = 0; k < N; ++k) v = fma(T.B[k * N + q], u[k], v);
-    s1[t0 + N * t1 + NN * q] = v;
+    cy[q] = cbase[t0 + N * q + NN * t1];
+    cz[q] = cbase[t0 + N * t1 + NN * q];
   }
-  __syncthreads();
-  // y-pass: thread (mode i=t0, qz=t1).
-  {
-    double in[N];
+  eo_interp<N>(T, a, U);
+  eo_deriv<N>(T, U, gx);
 #pragma unroll
-    for (int j = 0; j < N; ++j) in[j] = s1[t0 + N * j + NN * t1];
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-      double v = 0.0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) v = fma(T.B[j * N + q], in[j], v);
-      s2[t0 + N * q + NN * t1] = v;
-    }
-  }
-  __syncthreads();
-  // x-pass to point values U (x-pencil at (qy=t0, qz=t1)) and d/dx.
-  double U[N], gx[N];
-  {
-    double in[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) in[i] = s2[i + N * t0 + NN * t1];
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-      double v = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) v = fma(T.B[i * N + q], in[i], v);
-      U[q] = v;
-    }
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-      double v = 0.0;
-#pragma unroll
-      for (int s = 0; s < N; ++s) v = fma(T.D[q * N + s], U[s], v);
-      gx[q] = v;
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < N; ++q) s1[q + N * t0 + NN * t1] = U[q];
-  __syncthreads();
-  // d/dy on the y-pencil (qx=t0, qz=t1), d/dz on the z-pencil (qx=t0, qy=t1).
+  for (int q = 0; q < N; ++q)
+    if (act) s1[q + N * t0 + NN * t1] = U[q];
+  wave_sync();
+  // d/deta on the y-pencil (qx=t0, qz=t1), d/dzeta on the z-pencil (qx=t0, qy=t1).
   double gy[N], gz[N];
-  {
-    double iny[N], inz[N];
 #pragma unroll
-    for (int s = 0; s < N; ++s) {
-      iny[s] = s1[t0 + N * s + NN * t1];
-      inz[s] = s1[t0 + N * t1 + NN * s];
-    }
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-      double vy = 0.0, vz = 0.0;
-#pragma unroll
-      for (int s = 0; s < N; ++s) {
-        vy = fma(T.D[q * N + s], iny[s], vy);
-        vz = fma(T.D[q * N + s], inz[s], vz);
-      }
-      gy[q] = vy;
-      gz[q] = vz;
-    }
+  for (int s = 0; s < N; ++s) {
+    a[s] = s1[t0 + N * s + NN * t1];
+    b[s] = s1[t0 + N * t1 + NN * s];
   }
+  eo_deriv<N>(T, a, gy);
+  eo_deriv<N>(T, b, gz);
   // Per-point action f_a = W c J_a^-2 dU/dxi_a in each pencil layout.
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     gx[q] = cx[q] * jx2 * gx[q];
-    gy[q] = st[t0 + N * q + NN * t1] * jy2 * gy[q];
-    gz[q] = st[t0 + N * t1 + NN * q] * jz2 * gz[q];
+    gy[q] = cy[q] * jy2 * gy[q];
+    gz[q] = cz[q] * jz2 * gz[q];
   }
-  __syncthreads();
-  // Transposed derivatives; sum the three channels in the z layout.
-  {
-    double hx[N], hy[N];
+  // Transposed derivatives; the three channels meet in the z layout.
+  eo_deriv_t<N>(T, gx, a);
 #pragma unroll
-    for (int s = 0; s < N; ++s) {
-      double vx = 0.0, vy = 0.0, vz = 0.0;
+  for (int s = 0; s < N; ++s)
+    if (act) s1[s + N * t0 + NN * t1] = a[s];
+  eo_deriv_t<N>(T, gy, a);
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        vx = fma(T.D[q * N + s], gx[q], vx);
-        vy = fma(T.D[q * N + s], gy[q], vy);
-        vz = fma(T.D[q * N + s], gz[q], vz);
-      }
-      hx[s] = vx;
-      hy[s] = vy;
-      U[s] = vz;  // z-pencil (t0, t1, s)
-    }
-#pragma unroll
-    for (int s = 0; s < N; ++s) {
-      s1[s + N * t0 + NN * t1] = hx[s];
-      s2[t0 + N * s + NN * t1] = hy[s];
-    }
-  }
-  __syncthreads();
+  for (int s = 0; s < N; ++s)
+    if (act) s2[t0 + N * s + NN * t1] = a[s];
+  eo_deriv_t<N>(T, gz, U);
+  wave_sync();
 #pragma unroll
   for (int s = 0; s < N; ++s) U[s] += s1[t0 + N * t1 + NN * s] + s2[t0 + N * t1 + NN * s];
-  // Transposed z-pass (registers) to modes k at (qx=t0, qy=t1).
-  double P[N];
+  // Transposed z: modes k at (qx=t0, qy=t1).
+  eo_project<N>(T, U, a);
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double v = 0.0;
+  for (int k = 0; k < N; ++k)
+    if (act) s1[t0 + N * t1 + NN * k] = a[k];
+  wave_sync();
+  // Transposed y: lane (qx=t0, k=t1) -> modes j.
 #pragma unroll
-    for (int q = 0; q < N; ++q) v = fma(T.B[k * N + q], U[q], v);
-    P[k] = v;
-  }
-  __syncthreads();
+  for (int q = 0; q < N; ++q) a[q] = s1[t0 + N * q + NN * t1];
+  eo_project<N>(T, a, b);
 #pragma unroll
-  for (int k = 0; k < N; ++k) s1[t0 + N * t1 + NN * k] = P[k];
-  __syncthreads();
-  // Transposed y-pass: thread (qx=t0, k=t1) -> modes j.
-  {
-    double in[N];
+  for (int j = 0; j < N; ++j)
+    if (act) s2[t0 + N * j + NN * t1] = b[j];
+  wave_sync();
+  // Transposed x: lane (j=t0, k=t1) -> modes i.
 #pragma unroll
-    for (int q = 0; q < N; ++q) in[q] = s1[t0 + N * q + NN * t1];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double v = 0.0;
-#pragma unroll
-      for (int q = 0; q < N; ++q) v = fma(T.B[j * N + q], in[q], v);
-      s2[t0 + N * j + NN * t1] = v;
-    }
-  }
-  __syncthreads();
-  // Transposed x-pass: thread (j=t0, k=t1) -> modes i.
-  {
-    double in[N];
-#pragma unroll
-    for (int q = 0; q < N; ++q) in[q] = s2[q + N * t0 + NN * t1];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double v = 0.0;
-#pragma unroll
-      for (int q = 0; q < N; ++q) v = fma(T.B[i * N + q], in[q], v);
-      r[i] = v;
-    }
-  }
+  for (int q = 0; q < N; ++q) a[q] = s2[q + N * t0 + NN * t1];
+  eo_project<N>(T, a, r);
+}
+
+// Lane coordinates of one cell: (t0, t1) of an active lane, (0, 0) otherwise.
+template <int N>
+__device__ inline void lane_coords(int lane, int& t0, int& t1, bool& act) {
+  act = lane < N * N;
+  const int t = act ? lane : 0;
+  t0 = t % N;
+  t1 = t / N;
 }
 
 // ---- per-cell driver (global atomic scatter) --------------------------------
@@ -213,48 +174,33 @@ struct CellArgs {
   const double* lcoef;  // cells x N^3 or null
 };
 
-template <int N, int CPB, bool COEF>
-__global__ void __launch_bounds__(CPB* N* N) cell_apply_fast(CellArgs a, FastTables<N> T,
-                                                             const double* __restrict__ x, double* __restrict__ y) {
-  constexpr int NN = N * N, NNN = N * N * N;
+template <int N, bool COEF>
+__global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T, const double* __restrict__ x,
+                                                       double* __restrict__ y) {
+  constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>(), CPB = 256 / LPC;
   __shared__ double S1[CPB * NNN];
   __shared__ double S2[CPB * NNN];
-  __shared__ double ST[CPB * NNN];
-  const int tid = threadIdx.x;
-  const int lc = tid / NN;
-  const int t = tid - lc * NN;
-  const int t0 = t % N, t1 = t / N;
-  const int64_t blk = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t cell = blk * CPB + lc;
-  const bool active = cell < a.cells;
-  double* st = ST + lc * NNN;
-  const int64_t base = cell * NNN;
-
+  const int lc = threadIdx.x / LPC;
+  int t0, t1;
+  bool act;
+  lane_coords<N>(threadIdx.x % LPC, t0, t1, act);
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool live = cell < a.cells;
+  act = act && live;
+  const int64_t c = live ? cell : 0;
+  const int64_t base = c * NNN;
   double u[N], cx[N], r[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double v = 0.0;
-    if (active) {
-      const int e = t0 + N * t1 + NN * k;
-      const int32_t g = a.idx[base + e];
-      if (g >= 0) v = COEF ? a.lcoef[base + e] * x[g] : x[g];
-    }
-    u[k] = v;
-  }
-  double jx2 = 0.0, jy2 = 0.0, jz2 = 0.0;
-  if (active) {
-    const double jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
-    jx2 = jx * jx;
-    jy2 = jy * jy;
-    jz2 = jz * jz;
+    const int e = t0 + N * t1 + NN * k;
+    const int32_t g = a.idx[base + e];
+    u[k] = g >= 0 ? (COEF ? a.lcoef[base + e] * x[g] : x[g]) : 0.0;
   }
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    cx[q] = active ? a.coef[base + q + N * t0 + NN * t1] : 0.0;
-    st[q + N * t0 + NN * t1] = cx[q];
-  }
-  cell_body<N>(T, u, cx, st, jx2, jy2, jz2, S1 + lc * NNN, S2 + lc * NNN, t0, t1, r);
-  if (active) {
+  for (int q = 0; q < N; ++q) cx[q] = a.coef[base + q + N * t0 + NN * t1];
+  const double jx = a.jinv[c * 3], jy = a.jinv[c * 3 + 1], jz = a.jinv[c * 3 + 2];
+  cell_body<N>(T, u, cx, a.coef + base, jx * jx, jy * jy, jz * jz, S1 + lc * NNN, S2 + lc * NNN, t0, t1, act, r);
+  if (act) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int e = i + N * t0 + NN * t1;
@@ -284,81 +230,81 @@ __device__ inline int lat(int m) {
 }
 
 template <int N>
-__global__ void __launch_bounds__(8 * N * N) brick_apply(BrickArgs a, FastTables<N> T, const double* __restrict__ x,
-                                                         double* __restrict__ y) {
-  constexpr int NN = N * N, NNN = N * N * N;
-  constexpr int M = 2 * N - 1, L = M * M * M;
+__global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs a, EOTables<N> T,
+                                                                        const double* __restrict__ x,
+                                                                        double* __restrict__ y) {
+  constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
+  constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC;
+  constexpr int PER = (L + THREADS - 1) / THREADS;
   extern __shared__ double smem[];
-  double* xl = smem;                // L
-  double* yl = xl + L;              // L
-  double* scratch = yl + L;         // 8 x 3 x N^3
+  double* xl = smem;          // L input modes
+  double* acc = xl + L;       // L accumulated results
+  double* scratch = acc + L;  // 8 x 2 x N^3
   const int tid = threadIdx.x;
-  const int lc = tid / NN;
-  const int t = tid - lc * NN;
-  const int t0 = t % N, t1 = t / N;
+  const int lc = tid / LPC;
+  int t0, t1;
+  bool act;
+  lane_coords<N>(tid % LPC, t0, t1, act);
   const int64_t brick = a.order[xcd_block(blockIdx.x, gridDim.x)];
   const int32_t* map = a.map + brick * L;
-  for (int p = tid; p < L; p += 8 * NN) {
-    const int32_t g = map[p];
-    xl[p] = g >= 0 ? x[g] : 0.0;
-    yl[p] = 0.0;
-  }
   const int64_t cell = brick * 8 + lc;
   const int64_t base = cell * NNN;
-  double* s1 = scratch + lc * 3 * NNN;
-  double* s2 = s1 + NNN;
-  double* st = s2 + NNN;
+  int32_t g[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    g[k] = p < L ? map[p] : -1;
+  }
   double cx[N];
 #pragma unroll
-  for (int q = 0; q < N; ++q) {
-    cx[q] = a.coef[base + q + N * t0 + NN * t1];
-    st[q + N * t0 + NN * t1] = cx[q];
-  }
+  for (int q = 0; q < N; ++q) cx[q] = a.coef[base + q + N * t0 + NN * t1];
   const double jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    if (p < L) {
+      xl[p] = g[k] >= 0 ? x[g[k]] : 0.0;
+      acc[p] = 0.0;
+    }
+  }
   // Cell position inside the brick (Morton child order: x fastest).
   const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
-  const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
   __syncthreads();
   double u[N], r[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
-  cell_body<N>(T, u, cx, st, jx * jx, jy * jy, jz * jz, s1, s2, t0, t1, r);
   {
+    const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
+#pragma unroll
+    for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
+  }
+  double* s1 = scratch + lc * 2 * NNN;
+  cell_body<N>(T, u, cx, a.coef + base, jx * jx, jy * jy, jz * jz, s1, s1 + NNN, t0, t1, act, r);
+  if (act) {
     const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
 #pragma unroll
-    for (int i = 0; i < N; ++i) atomicAdd(&yl[ox + lat<N>(i) + M * (py + M * pz)], r[i]);
+    for (int i = 0; i < N; ++i) atomicAdd(&acc[ox + lat<N>(i) + M * (py + M * pz)], r[i]);
   }
   __syncthreads();
   const uint8_t* first = a.first + brick * L;
-  for (int p = tid; p < L; p += 8 * NN) {
-    const int32_t g = map[p];
-    if (g < 0) continue;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    if (g[k] < 0) continue;
     if (a.store_first && first[p])
-      y[g] = yl[p];
+      y[g[k]] = acc[p];
     else
-      y[g] += yl[p];
+      y[g[k]] += acc[p];
   }
 }
 
 template <int N>
-constexpr int cells_per_block() {
-  return (256 / (N * N)) < 1 ? 1 : 256 / (N * N);
-}
-
-template <int N>
-FastTables<N> tables_of(const Bucket& b) {
-  FastTables<N> T;
-  for (int i = 0; i < N * N; ++i) {
-    T.B[i] = b.h_values[size_t(i)];
-    T.D[i] = b.h_colloc[size_t(i)];
-  }
-  return T;
+EOTables<N> tables_of(const Bucket& b) {
+  return make_eo_tables<N>(b.h_values.data(), b.h_colloc.data());
 }
 
 template <int N>
 void launch_cells(const Bucket& b, const double* x, double* y, bool reduced, hipStream_t stream) {
-  constexpr int CPB = cells_per_block<N>();
-  FastTables<N> T = tables_of<N>(b);
+  constexpr int CPB = 256 / lanes_per_cell<N>();
+  EOTables<N> T = tables_of<N>(b);
   CellArgs a;
   a.cells = b.cells;
   a.jinv = b.jinv;
@@ -367,23 +313,21 @@ void launch_cells(const Bucket& b, const double* x, double* y, bool reduced, hip
   a.lcoef = b.lcoef;
   const int64_t blocks = (b.cells + CPB - 1) / CPB;
   if (b.lcoef)
-    hipLaunchKernelGGL((cell_apply_fast<N, CPB, true>), dim3(unsigned(blocks)), dim3(CPB * N * N), 0, stream, a, T,
-                       x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, true>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
   else
-    hipLaunchKernelGGL((cell_apply_fast<N, CPB, false>), dim3(unsigned(blocks)), dim3(CPB * N * N), 0, stream, a, T,
-                       x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, false>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
   GMF_HIP(hipGetLastError());
 }
 
 template <int N>
 size_t brick_lds_bytes() {
   constexpr int M = 2 * N - 1;
-  return sizeof(double) * size_t(2 * M * M * M + 8 * 3 * N * N * N);
+  return sizeof(double) * size_t(2 * M * M * M + 8 * 2 * N * N * N);
 }
 
 template <int N>
 void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bool store_first, hipStream_t stream) {
-  FastTables<N> T = tables_of<N>(b);
+  EOTables<N> T = tables_of<N>(b);
   const BrickPath& bp = b.bricks;
   for (int c = 0; c < bp.colors; ++c) {
     const int64_t lo = bp.color_offsets[size_t(c)], hi = bp.color_offsets[size_t(c) + 1];
@@ -396,8 +340,8 @@ void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bo
     a.jinv = b.jinv;
     a.coef = b.pdata;
     a.store_first = store_first ? 1 : 0;
-    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(hi - lo)), dim3(8 * N * N), brick_lds_bytes<N>(), stream, a, T,
-                       x, y);
+    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
+                       stream, a, T, x, y);
   }
   GMF_HIP(hipGetLastError());
 }
@@ -533,7 +477,7 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   b.fast = false;
   if (b.dim != 3 || b.form != GMF_FORM_DIFFUSION || b.multi_count != 0) return;
   const int n = b.N[0];
-  if (n < 2 || n > 9) return;
+  if (n < 2 || n > 8) return;
   for (int a = 0; a < 3; ++a)
     if (b.N[a] != n || b.Q[a] != n) return;
   // The three axes must share one table (isotropic degree and rule).
@@ -572,7 +516,6 @@ bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, doubl
     case 6: launch_cells<6>(b, x, y, reduced, stream); return true;
     case 7: launch_cells<7>(b, x, y, reduced, stream); return true;
     case 8: launch_cells<8>(b, x, y, reduced, stream); return true;
-    case 9: launch_cells<9>(b, x, y, reduced, stream); return true;
     default: return false;
   }
 }
