@@ -87,6 +87,21 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def _traffic_from_profiles(n, cells):
+    """HBM bytes per cell pass measured with rocprofv3 PMC counters (FETCH_SIZE +
+    WRITE_SIZE, scaled by a 1 GiB copy), recorded in profiles/apply_traffic.json
+    for this workload; None when absent or for another workload."""
+    path = os.path.join(ROOT, "profiles", "apply_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("full_dofs") != n or rec.get("cells") != cells:
+        return None
+    return rec.get("traffic_bytes_per_cell_pass")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +109,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--depth", type=int, default=6, help="uniform refinements of the root (6 -> 64^3 cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="after timing, copy a 1 GiB buffer three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (cells would shard by Morton "
@@ -179,6 +196,16 @@ def main():
             "traffic": None,
         },
     }
+    traffic = _traffic_from_profiles(n, cells)
+    if traffic is not None:
+        result["roofline"]["traffic"] = traffic
+    if args.calibration_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

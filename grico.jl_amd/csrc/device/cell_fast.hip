@@ -42,6 +42,8 @@ namespace gmf {
 namespace {
 
 constexpr int kXcds = 8;
+// Workgroups of one persistent brick launch: four per CU on 256 CUs.
+constexpr int64_t kBrickGrid = 1024;
 
 __device__ inline int64_t xcd_block(int64_t b, int64_t n) {
   const int64_t q = n / kXcds, r = n % kXcds;
@@ -63,95 +65,92 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-cell body. In: u = z-pencil of modes (i=t0, j=t1, k); cx = x-pencil
-// point coefficients (q, t0, t1); cbase = the cell's point coefficients in
-// global memory (y/z pencils are loaded here); squared inverse Jacobians.
-// Out: r = x-pencil of results (i, j=t0, k=t1). s1/s2: the cell's two N^3 LDS
+// Per-cell body. In: u = z-pencil of modes (i=t0, j=t1, k); cx, cy, cz =
+// point coefficients W_q c_q of the x-pencil (q, t0, t1), the y-pencil
+// (t0, q, t1) and the z-pencil (t0, t1, q); squared inverse Jacobians.
+// Out: r = x-pencil of results (i, j=t0, k=t1). A/B: the cell's two N^3 LDS
 // scratch arrays. Lanes with act == false hold valid (t0, t1) = (0, 0) and
-// never write.
+// never write. Transposes rewrite an array in place only where each lane
+// rewrites exactly the entries it read, so the in-order LDS of one wavefront
+// makes that safe.
 template <int N>
 __device__ inline void cell_body(const EOTables<N>& T, const double (&u)[N], const double (&cx)[N],
-                                 const double* __restrict__ cbase, double jx2, double jy2, double jz2, double* s1,
-                                 double* s2, int t0, int t1, bool act, double (&r)[N]) {
+                                 const double (&cy)[N], const double (&cz)[N], double jx2, double jy2, double jz2,
+                                 double* A, double* B, int t0, int t1, bool act, double (&r)[N]) {
   constexpr int NN = N * N;
   double a[N], b[N];
-  // z: modes -> points, stored at (t0, t1, qz).
+  // z: modes -> points, A = (t0, t1, qz).
   eo_interp<N>(T, u, a);
 #pragma unroll
   for (int q = 0; q < N; ++q)
-    if (act) s1[t0 + N * t1 + NN * q] = a[q];
+    if (act) A[t0 + N * t1 + NN * q] = a[q];
   wave_sync();
-  // y: lane (i=t0, qz=t1).
+  // y: lane (i=t0, qz=t1), B = (i, qy, qz).
 #pragma unroll
-  for (int j = 0; j < N; ++j) a[j] = s1[t0 + N * j + NN * t1];
+  for (int j = 0; j < N; ++j) a[j] = A[t0 + N * j + NN * t1];
   eo_interp<N>(T, a, b);
 #pragma unroll
   for (int q = 0; q < N; ++q)
-    if (act) s2[t0 + N * q + NN * t1] = b[q];
+    if (act) B[t0 + N * q + NN * t1] = b[q];
   wave_sync();
-  // x: lane (qy=t0, qz=t1): point values U and d/dxi.
-  double U[N], gx[N];
+  // x: lane (qy=t0, qz=t1): point values to A; x channel f = W c J^-2 dU/dxi
+  // projected back with D^T, kept in B (x layout).
 #pragma unroll
-  for (int i = 0; i < N; ++i) a[i] = s2[i + N * t0 + NN * t1];
-  // The y/z-pencil coefficients of this cell (same cache lines as cx).
-  double cy[N], cz[N];
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    cy[q] = cbase[t0 + N * q + NN * t1];
-    cz[q] = cbase[t0 + N * t1 + NN * q];
-  }
-  eo_interp<N>(T, a, U);
-  eo_deriv<N>(T, U, gx);
+  for (int i = 0; i < N; ++i) a[i] = B[i + N * t0 + NN * t1];
+  eo_interp<N>(T, a, b);
 #pragma unroll
   for (int q = 0; q < N; ++q)
-    if (act) s1[q + N * t0 + NN * t1] = U[q];
-  wave_sync();
-  // d/deta on the y-pencil (qx=t0, qz=t1), d/dzeta on the z-pencil (qx=t0, qy=t1).
-  double gy[N], gz[N];
+    if (act) A[q + N * t0 + NN * t1] = b[q];
+  eo_deriv<N>(T, b, a);
 #pragma unroll
-  for (int s = 0; s < N; ++s) {
-    a[s] = s1[t0 + N * s + NN * t1];
-    b[s] = s1[t0 + N * t1 + NN * s];
-  }
-  eo_deriv<N>(T, a, gy);
-  eo_deriv<N>(T, b, gz);
-  // Per-point action f_a = W c J_a^-2 dU/dxi_a in each pencil layout.
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    gx[q] = cx[q] * jx2 * gx[q];
-    gy[q] = cy[q] * jy2 * gy[q];
-    gz[q] = cz[q] * jz2 * gz[q];
-  }
-  // Transposed derivatives; the three channels meet in the z layout.
-  eo_deriv_t<N>(T, gx, a);
+  for (int q = 0; q < N; ++q) a[q] *= cx[q] * jx2;
+  eo_deriv_t<N>(T, a, b);
 #pragma unroll
   for (int s = 0; s < N; ++s)
-    if (act) s1[s + N * t0 + NN * t1] = a[s];
-  eo_deriv_t<N>(T, gy, a);
+    if (act) B[s + N * t0 + NN * t1] = b[s];
+  wave_sync();
+  // y channel on the y-pencil (qx=t0, qz=t1), added into B in place.
+  {
 #pragma unroll
-  for (int s = 0; s < N; ++s)
-    if (act) s2[t0 + N * s + NN * t1] = a[s];
-  eo_deriv_t<N>(T, gz, U);
+    for (int s = 0; s < N; ++s) a[s] = A[t0 + N * s + NN * t1];
+    eo_deriv<N>(T, a, b);
+#pragma unroll
+    for (int q = 0; q < N; ++q) b[q] *= cy[q] * jy2;
+    eo_deriv_t<N>(T, b, a);
+#pragma unroll
+    for (int s = 0; s < N; ++s)
+      if (act) B[t0 + N * s + NN * t1] += a[s];
+  }
+  // z channel on the z-pencil (qx=t0, qy=t1), kept in registers.
+  double h[N];
+  {
+#pragma unroll
+    for (int s = 0; s < N; ++s) a[s] = A[t0 + N * t1 + NN * s];
+    eo_deriv<N>(T, a, b);
+#pragma unroll
+    for (int q = 0; q < N; ++q) b[q] *= cz[q] * jz2;
+    eo_deriv_t<N>(T, b, h);
+  }
   wave_sync();
 #pragma unroll
-  for (int s = 0; s < N; ++s) U[s] += s1[t0 + N * t1 + NN * s] + s2[t0 + N * t1 + NN * s];
-  // Transposed z: modes k at (qx=t0, qy=t1).
-  eo_project<N>(T, U, a);
+  for (int s = 0; s < N; ++s) h[s] += B[t0 + N * t1 + NN * s];
+  // Transposed z: modes k at (qx=t0, qy=t1) into A.
+  eo_project<N>(T, h, a);
 #pragma unroll
   for (int k = 0; k < N; ++k)
-    if (act) s1[t0 + N * t1 + NN * k] = a[k];
+    if (act) A[t0 + N * t1 + NN * k] = a[k];
   wave_sync();
-  // Transposed y: lane (qx=t0, k=t1) -> modes j.
+  // Transposed y: lane (qx=t0, k=t1) -> modes j into B.
 #pragma unroll
-  for (int q = 0; q < N; ++q) a[q] = s1[t0 + N * q + NN * t1];
+  for (int q = 0; q < N; ++q) a[q] = A[t0 + N * q + NN * t1];
   eo_project<N>(T, a, b);
 #pragma unroll
   for (int j = 0; j < N; ++j)
-    if (act) s2[t0 + N * j + NN * t1] = b[j];
+    if (act) B[t0 + N * j + NN * t1] = b[j];
   wave_sync();
   // Transposed x: lane (j=t0, k=t1) -> modes i.
 #pragma unroll
-  for (int q = 0; q < N; ++q) a[q] = s2[q + N * t0 + NN * t1];
+  for (int q = 0; q < N; ++q) a[q] = B[q + N * t0 + NN * t1];
   eo_project<N>(T, a, r);
 }
 
@@ -196,10 +195,15 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T
     const int32_t g = a.idx[base + e];
     u[k] = g >= 0 ? (COEF ? a.lcoef[base + e] * x[g] : x[g]) : 0.0;
   }
+  double cy[N], cz[N];
 #pragma unroll
-  for (int q = 0; q < N; ++q) cx[q] = a.coef[base + q + N * t0 + NN * t1];
+  for (int q = 0; q < N; ++q) {
+    cx[q] = a.coef[base + q + N * t0 + NN * t1];
+    cy[q] = a.coef[base + t0 + N * q + NN * t1];
+    cz[q] = a.coef[base + t0 + N * t1 + NN * q];
+  }
   const double jx = a.jinv[c * 3], jy = a.jinv[c * 3 + 1], jz = a.jinv[c * 3 + 2];
-  cell_body<N>(T, u, cx, a.coef + base, jx * jx, jy * jy, jz * jz, S1 + lc * NNN, S2 + lc * NNN, t0, t1, act, r);
+  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, S1 + lc * NNN, S2 + lc * NNN, t0, t1, act, r);
   if (act) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -215,8 +219,9 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T
 struct BrickArgs {
   const int64_t* order;   // bricks of this launch (one color), Morton order
   int64_t count;          // bricks in this launch
-  const int32_t* map;     // bricks x L lattice -> dof (-1: none)
-  const uint8_t* first;   // bricks x L: 1 if this brick writes the dof first
+  // bricks x L lattice entries: (dof << 1) | first, -1 for no dof; `first`
+  // is set where this brick is the first writer of the dof.
+  const int32_t* map;
   const double* jinv;     // cells x 3
   const double* coef;     // cells x N^3
   int store_first;        // 0: always read-modify-write (y pre-zeroed)
@@ -229,70 +234,135 @@ __device__ inline int lat(int m) {
   return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1);
 }
 
+// Registers of one brick in flight: its lattice entries, input values, the
+// current result values of its read-modify-write points, and the x-pencil
+// coefficients and Jacobian of this lane's cell.
+template <int N>
+struct BrickStage {
+  static constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * lanes_per_cell<N>();
+  static constexpr int PER = (L + THREADS - 1) / THREADS;
+  int32_t g[PER];
+  double xv[PER];
+  double yold[PER];
+  double cx[N];
+  double jx, jy, jz;
+};
+
+template <int N>
+__device__ inline void brick_map(const BrickArgs& a, int64_t brick, int tid, int32_t (&m)[BrickStage<N>::PER]) {
+  using S = BrickStage<N>;
+  const int32_t* map = a.map + brick * S::L;
+#pragma unroll
+  for (int k = 0; k < S::PER; ++k) {
+    const int p = tid + k * S::THREADS;
+    m[k] = p < S::L ? map[p] : -1;
+  }
+}
+
+// Issue the dependent loads of one brick from its lattice entries. The y
+// values of a brick are stable for the whole launch: bricks of one color are
+// DOF-disjoint and earlier colors ran in earlier launches.
+template <int N>
+__device__ inline void brick_stage(const BrickArgs& a, int64_t brick, const int32_t (&m)[BrickStage<N>::PER], int lc,
+                                   int t0, int t1, const double* __restrict__ x, const double* __restrict__ y,
+                                   BrickStage<N>& s) {
+  constexpr int NN = N * N, NNN = N * N * N;
+  using S = BrickStage<N>;
+  const int64_t cell = brick * 8 + lc;
+  const int64_t base = cell * NNN;
+#pragma unroll
+  for (int q = 0; q < N; ++q) s.cx[q] = a.coef[base + q + N * t0 + NN * t1];
+  s.jx = a.jinv[cell * 3];
+  s.jy = a.jinv[cell * 3 + 1];
+  s.jz = a.jinv[cell * 3 + 2];
+#pragma unroll
+  for (int k = 0; k < S::PER; ++k) {
+    const int32_t e = m[k];
+    s.g[k] = e;
+    const int32_t d = e >> 1;
+    s.xv[k] = e >= 0 ? x[d] : 0.0;
+    s.yold[k] = (e >= 0 && !(a.store_first && (e & 1))) ? y[d] : 0.0;
+  }
+}
+
+// Persistent, software-pipelined brick kernel: each workgroup walks a
+// contiguous chunk of the color's bricks. While brick i computes, the loads
+// of brick i+1 (inputs, results to update, coefficients) and the lattice
+// entries of brick i+2 are in flight.
 template <int N>
 __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs a, EOTables<N> T,
                                                                         const double* __restrict__ x,
                                                                         double* __restrict__ y) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
-  constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC;
-  constexpr int PER = (L + THREADS - 1) / THREADS;
+  using S = BrickStage<N>;
+  constexpr int M = S::M, L = S::L, THREADS = S::THREADS, PER = S::PER;
   extern __shared__ double smem[];
-  double* xl = smem;          // L input modes
-  double* acc = xl + L;       // L accumulated results
-  double* scratch = acc + L;  // 8 x 2 x N^3
+  double* xl = smem;           // L input modes
+  double* acc = xl + L;        // L accumulated results
+  double* scratch = acc + L;   // 8 x 2 x N^3 (per cell A, B)
   const int tid = threadIdx.x;
   const int lc = tid / LPC;
   int t0, t1;
   bool act;
   lane_coords<N>(tid % LPC, t0, t1, act);
-  const int64_t brick = a.order[xcd_block(blockIdx.x, gridDim.x)];
-  const int32_t* map = a.map + brick * L;
-  const int64_t cell = brick * 8 + lc;
-  const int64_t base = cell * NNN;
-  int32_t g[PER];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int p = tid + k * THREADS;
-    g[k] = p < L ? map[p] : -1;
-  }
-  double cx[N];
-#pragma unroll
-  for (int q = 0; q < N; ++q) cx[q] = a.coef[base + q + N * t0 + NN * t1];
-  const double jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int p = tid + k * THREADS;
-    if (p < L) {
-      xl[p] = g[k] >= 0 ? x[g[k]] : 0.0;
-      acc[p] = 0.0;
-    }
-  }
+  const int64_t nblk = gridDim.x;
+  const int64_t blk = xcd_block(blockIdx.x, nblk);
+  const int64_t per_blk = (a.count + nblk - 1) / nblk;
+  const int64_t lo = blk * per_blk;
+  const int64_t hi = lo + per_blk < a.count ? lo + per_blk : a.count;
+  if (lo >= hi) return;  // uniform across the workgroup
   // Cell position inside the brick (Morton child order: x fastest).
   const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
-  __syncthreads();
-  double u[N], r[N];
-  {
-    const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
+  const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
+  const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
+  double* A = scratch + lc * 2 * NNN;
+
+  int32_t m[PER];
+  S cur, nxt;
+  brick_map<N>(a, a.order[lo], tid, m);
+  brick_stage<N>(a, a.order[lo], m, lc, t0, t1, x, y, cur);
+  if (lo + 1 < hi) brick_map<N>(a, a.order[lo + 1], tid, m);
+  for (int64_t i = lo; i < hi; ++i) {
+    const int64_t brick = a.order[i];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int p = tid + k * THREADS;
+      if (p < L) {
+        xl[p] = cur.xv[k];
+        acc[p] = 0.0;
+      }
+    }
+    __syncthreads();
+    // y/z-pencil coefficients of this brick (cache hits: the x-pencil load of
+    // the same lines was issued one brick earlier), then the prefetches.
+    double cy[N], cz[N];
+    {
+      const double* cb = a.coef + (brick * 8 + lc) * NNN;
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        cy[q] = cb[t0 + N * q + NN * t1];
+        cz[q] = cb[t0 + N * t1 + NN * q];
+      }
+    }
+    if (i + 1 < hi) brick_stage<N>(a, a.order[i + 1], m, lc, t0, t1, x, y, nxt);
+    if (i + 2 < hi) brick_map<N>(a, a.order[i + 2], tid, m);
+    double u[N], r[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
-  }
-  double* s1 = scratch + lc * 2 * NNN;
-  cell_body<N>(T, u, cx, a.coef + base, jx * jx, jy * jy, jz * jz, s1, s1 + NNN, t0, t1, act, r);
-  if (act) {
-    const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
+    cell_body<N>(T, u, cur.cx, cy, cz, cur.jx * cur.jx, cur.jy * cur.jy, cur.jz * cur.jz, A, A + NNN, t0, t1, act,
+                 r);
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) atomicAdd(&acc[ox + lat<N>(i) + M * (py + M * pz)], r[i]);
-  }
-  __syncthreads();
-  const uint8_t* first = a.first + brick * L;
+      for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[ox + lat<N>(ii) + M * (py + M * pz)], r[ii]);
+    }
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int p = tid + k * THREADS;
-    if (g[k] < 0) continue;
-    if (a.store_first && first[p])
-      y[g[k]] = acc[p];
-    else
-      y[g[k]] += acc[p];
+    for (int k = 0; k < PER; ++k) {
+      const int p = tid + k * THREADS;
+      if (cur.g[k] >= 0) y[cur.g[k] >> 1] = cur.yold[k] + acc[p];
+    }
+    __syncthreads();
+    cur = nxt;
   }
 }
 
@@ -336,11 +406,11 @@ void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bo
     a.order = bp.order + lo;
     a.count = hi - lo;
     a.map = reduced ? bp.map_red : bp.map_full;
-    a.first = bp.first;
     a.jinv = b.jinv;
     a.coef = b.pdata;
     a.store_first = store_first ? 1 : 0;
-    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
+    const int64_t grid = std::min<int64_t>(hi - lo, kBrickGrid);
+    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(grid)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
                        stream, a, T, x, y);
   }
   GMF_HIP(hipGetLastError());
@@ -460,12 +530,16 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
   for (int64_t br = 0; br < nb; ++br) order[size_t(fillp[size_t(color[size_t(br)])]++)] = br;
   bp.bricks = nb;
   bp.order = upload_array(p, order.data(), order.size());
-  bp.map_full = upload_array(p, map.data(), map.size());
-  bp.first = upload_array(p, first.data(), first.size());
+  // Lattice entries carry the first-writer flag in bit 0.
+  require(p.n_full < (int64_t(1) << 30), GMF_EUNSUPPORTED, "brick maps need dof counts below 2^30");
+  std::vector<int32_t> enc(map.size());
+  for (size_t i = 0; i < map.size(); ++i) enc[i] = map[i] >= 0 ? (map[i] << 1) | int32_t(first[i]) : -1;
+  bp.map_full = upload_array(p, enc.data(), enc.size());
   if (!red_of_full.empty()) {
-    for (auto& g : map)
-      if (g >= 0) g = int32_t(red_of_full[size_t(g)]);
-    bp.map_red = upload_array(p, map.data(), map.size());
+    for (size_t i = 0; i < map.size(); ++i)
+      enc[i] = map[i] >= 0 && red_of_full[size_t(map[i])] >= 0 ? (int32_t(red_of_full[size_t(map[i])]) << 1) | int32_t(first[i])
+                                                               : -1;
+    bp.map_red = upload_array(p, enc.data(), enc.size());
   }
   bp.enabled = true;
   return true;

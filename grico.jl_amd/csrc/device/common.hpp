@@ -38,9 +38,9 @@ struct BrickPath {
   int colors = 0;
   std::vector<int64_t> color_offsets;  // host: bricks of color c are order[off[c]..off[c+1])
   int64_t* order = nullptr;            // brick ids sorted by color
-  int32_t* map_full = nullptr;         // bricks x (2N-1)^3 lattice -> full dof
-  int32_t* map_red = nullptr;          // same, reduced dofs (-1 = fixed)
-  uint8_t* first = nullptr;            // 1 where the brick is the first writer of the dof
+  // bricks x (2N-1)^3 lattice entries (dof << 1) | first-writer flag, -1 = none
+  int32_t* map_full = nullptr;
+  int32_t* map_red = nullptr;          // same with reduced dofs (fixed dofs -> -1)
 };
 
 // Device-resident cell batch.
