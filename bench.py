@@ -203,7 +203,7 @@ def main():
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            _devabi.check(lib.gmf_vector_copy(dev.ws, dst.data_ptr(), src.data_ptr(), src.numel()))
         torch.cuda.synchronize()
         del src, dst
     if not args.no_cpu_baseline:

@@ -595,6 +595,14 @@ gmf_status gmf_reduced_diagonal(gmf_workspace* ws, double* diag) {
   });
 }
 
+gmf_status gmf_vector_copy(gmf_workspace* ws, double* dst, const double* src, int64_t n) {
+  return guarded([&] {
+    require(ws && dst && src && n >= 0, GMF_EINVAL, "invalid argument");
+    copy(dst, src, n, ws->stream);
+    sync(ws);
+  });
+}
+
 gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n) {
   return guarded([&] {
     require(ws && d && inv && n >= 0, GMF_EINVAL, "null argument");

@@ -62,6 +62,7 @@ def declare(lib):
     for name in ("gmf_apply", "gmf_apply_async", "gmf_apply_physical", "gmf_reduced_apply",
                  "gmf_project_reduced"):
         getattr(lib, name).argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.gmf_vector_copy.argtypes = [c_void_p, c_void_p, c_void_p, c_int64]
     lib.gmf_plan_describe.argtypes = [c_void_p, ctypes.c_char_p, c_int64]
     lib.gmf_apply_timed.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, P_f64, P_f64]
     lib.gmf_precond_create_jacobi.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
@@ -80,7 +81,7 @@ def declare(lib):
                  "gmf_apply_physical", "gmf_reduced_apply", "gmf_project_reduced", "gmf_expand_reduced",
                  "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
                  "gmf_fgmres_solve", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
-                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe"):
+                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy"):
         getattr(lib, name).restype = ctypes.c_int
 
 

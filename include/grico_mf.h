@@ -195,6 +195,11 @@ gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* precond, cons
 gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_t repetitions,
                            double* cell_ms, double* total_ms);
 
+/* dst = src for n doubles with the library's plain streaming copy kernel on
+ * the workspace stream (vector utility; also the known-size kernel used to
+ * calibrate HBM byte counters). No reference counterpart. */
+gmf_status gmf_vector_copy(gmf_workspace* ws, double* dst, const double* src, int64_t n);
+
 /* Jacobi inverse diagonal with validation: rejects zero / non-finite entries
  * (src/solve.jl:740-751). */
 gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* diag, double* inverse_diagonal, int64_t n);
