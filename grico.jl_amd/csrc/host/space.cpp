@@ -90,10 +90,21 @@ Pattern make_pattern(int basis, int D, int cg_mask, const std::array<int32_t, kM
 
 // Sparse row over variables or dofs, kept in insertion order like the
 // reference's _ConstraintRow.
+//
+// Cancellation test: the reference drops coefficients below 4 eps times the
+// row's current largest coefficient (src/continuity.jl:687-698, 1213-1223).
+// After substitutions that cancel a row completely, that scale is the noise
+// itself, so a residue of a few ulps of the substituted terms survives and
+// becomes a spurious pivot that over-constrains the space (two-level hanging
+// faces of p = 2 show it: one coarse edge bubble is forced to zero). `mag`
+// keeps the largest magnitude ever added to the row, and the tolerance uses it
+// as well; rows that never cancelled are cleaned exactly as in the reference.
 struct Row {
   std::vector<int64_t> var;
   std::vector<double> coef;
+  double mag = 0.0;
   void add(int64_t v, double c) {
+    mag = std::max(mag, std::fabs(c));
     for (size_t i = 0; i < var.size(); ++i)
       if (var[i] == v) {
         coef[i] += c;
@@ -110,7 +121,7 @@ struct Row {
   void cleanup() {
     double scale = 0.0;
     for (double c : coef) scale = std::max(scale, std::fabs(c));
-    double tol = 4.0 * std::numeric_limits<double>::epsilon() * std::max(scale, 1.0);
+    double tol = 4.0 * std::numeric_limits<double>::epsilon() * std::max(std::max(scale, mag), 1.0);
     size_t i = 0;
     while (i < var.size()) {
       if (std::fabs(coef[i]) <= tol)
