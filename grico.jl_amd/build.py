@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIBDIR = os.path.join(HERE, "lib")
-HOST_SRC = [os.path.join(HERE, "csrc", "host", f) for f in ("topology.cpp", "space.cpp", "capi.cpp")]
+HOST_SRC = [os.path.join(HERE, "csrc", "host", f) for f in ("topology.cpp", "space.cpp", "transfer.cpp", "capi.cpp")]
 DEVICE_DIR = os.path.join(HERE, "csrc", "device")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

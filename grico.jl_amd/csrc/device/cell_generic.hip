@@ -131,21 +131,10 @@ __device__ void scatter_one(const GenericArgs& a, int64_t cell, int i, double v,
   }
 }
 
-__global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const double* __restrict__ x,
-                                                         double* __restrict__ y) {
-  extern __shared__ double smem[];
-  const int64_t cell = blockIdx.x;
-  double* u = smem;
-  double* bufA = u + a.M;
-  double* bufB = bufA + a.M;
-  double* fields = bufB + a.M;          // (D+1) x Qtot
-  double* r = fields + (a.D + 1) * a.Qtot;  // Ntot
-
-  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
-    u[i] = gather_one(a, cell, i, x);
-    r[i] = 0.0;
-  }
-  __syncthreads();
+// r = A_c u for one cell (u, r in LDS, Ntot entries); `fields` holds (D+1) x
+// Qtot point values, bufA / bufB are contraction scratch.
+__device__ void apply_body(const GenericArgs& a, int64_t cell, const double* u, double* r, double* bufA,
+                           double* bufB, double* fields) {
   const bool general = a.form == GMF_FORM_GENERAL;
   if (general) interpolate(a, u, fields, bufA, bufB, -1);
   for (int d = 0; d < a.D; ++d) interpolate(a, u, fields + (d + 1) * a.Qtot, bufA, bufB, d);
@@ -176,12 +165,65 @@ __global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const do
   __syncthreads();
   if (general) project_add(a, fields, r, bufA, bufB, -1, false, -1);
   for (int d = 0; d < a.D; ++d) project_add(a, fields + (d + 1) * a.Qtot, r, bufA, bufB, d, false, -1);
+}
+
+__global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* u = smem;
+  double* bufA = u + a.M;
+  double* bufB = bufA + a.M;
+  double* fields = bufB + a.M;          // (D+1) x Qtot
+  double* r = fields + (a.D + 1) * a.Qtot;  // Ntot
+
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+    u[i] = gather_one(a, cell, i, x);
+    r[i] = 0.0;
+  }
+  __syncthreads();
+  apply_body(a, cell, u, r, bufA, bufB, fields);
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
 }
 
-__global__ void __launch_bounds__(64) cell_diagonal_generic(GenericArgs a, double* __restrict__ diag) {
+// One (cell, reduced dof) pair per workgroup: diag[dof] += v^T A_c v with v the
+// dof's local coefficient vector.
+__global__ void __launch_bounds__(64) cell_diagonal_pairs(GenericArgs a, const int64_t* __restrict__ pcell,
+                                                          const int32_t* __restrict__ pdof,
+                                                          const int64_t* __restrict__ poff,
+                                                          const int32_t* __restrict__ ploc,
+                                                          const double* __restrict__ pcoef,
+                                                          double* __restrict__ diag) {
+  extern __shared__ double smem[];
+  const int64_t pair = blockIdx.x;
+  const int64_t cell = pcell[pair];
+  double* u = smem;
+  double* bufA = u + a.M;
+  double* bufB = bufA + a.M;
+  double* fields = bufB + a.M;
+  double* r = fields + (a.D + 1) * a.Qtot;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+    u[i] = 0.0;
+    r[i] = 0.0;
+  }
+  __syncthreads();
+  const int64_t e0 = poff[pair], e1 = poff[pair + 1];
+  if (threadIdx.x == 0)
+    for (int64_t e = e0; e < e1; ++e) u[ploc[e]] += pcoef[e];
+  __syncthreads();
+  apply_body(a, cell, u, r, bufA, bufB, fields);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int64_t e = e0; e < e1; ++e) s = fma(pcoef[e], r[ploc[e]], s);
+    unsafeAtomicAdd(&diag[pdof[pair]], s);
+  }
+}
+
+__global__ void __launch_bounds__(64) cell_diagonal_generic(GenericArgs a, double* __restrict__ diag,
+                                                            const uint8_t* __restrict__ skip) {
   extern __shared__ double smem[];
   const int64_t cell = blockIdx.x;
+  if (skip && skip[cell]) return;
   double* bufA = smem;
   double* bufB = bufA + a.M;
   double* w = bufB + a.M;   // Qtot
@@ -259,7 +301,30 @@ void launch_cell_diagonal_generic(const gmf_plan&, const Bucket& b, double* diag
   require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
   size_t lds = sizeof(double) * size_t(2 * a.M + a.Qtot + a.Ntot);
   require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
-  hipLaunchKernelGGL(cell_diagonal_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, diag);
+  hipLaunchKernelGGL(cell_diagonal_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, diag,
+                     static_cast<const uint8_t*>(nullptr));
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cell_diagonal_generic_skip(const gmf_plan&, const Bucket& b, double* diag, hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, true);
+  require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  size_t lds = sizeof(double) * size_t(2 * a.M + a.Qtot + a.Ntot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_diagonal_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, diag,
+                     static_cast<const uint8_t*>(b.coupled));
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cell_diagonal_pairs(const gmf_plan&, const Bucket& b, double* diag, hipStream_t stream) {
+  if (b.pair_count == 0) return;
+  GenericArgs a = make_args(b, true);
+  size_t lds = sizeof(double) * size_t(3 * a.M + (a.D + 1) * a.Qtot + a.Ntot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  require(b.pair_count < (int64_t(1) << 31), GMF_EUNSUPPORTED, "too many coupled diagonal pairs");
+  hipLaunchKernelGGL(cell_diagonal_pairs, dim3(unsigned(b.pair_count)), dim3(64), lds, stream, a, b.pair_cell,
+                     b.pair_dof, b.pair_off, b.pair_loc, b.pair_coef, diag);
   GMF_HIP(hipGetLastError());
 }
 

@@ -71,6 +71,16 @@ struct Bucket {
   std::vector<double> h_colloc;
   bool fast = false;
   BrickPath bricks;
+  // Exact reduced diagonal on cells whose modes couple through multi-term
+  // expansions or repeat a dof: `coupled` flags those cells (null if none) and
+  // each (cell, reduced dof) pair lists the dof's local coefficient vector.
+  uint8_t* coupled = nullptr;
+  int64_t pair_count = 0;
+  int64_t* pair_cell = nullptr;
+  int32_t* pair_dof = nullptr;
+  int64_t* pair_off = nullptr;
+  int32_t* pair_loc = nullptr;
+  double* pair_coef = nullptr;
 };
 
 struct Csr {
@@ -123,6 +133,10 @@ void launch_cell_generic(const gmf_plan& plan, const Bucket& b, const double* x,
                          hipStream_t stream);
 void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced,
                                   hipStream_t stream);
+// Exact reduced-diagonal contributions of the bucket's coupled cells (and the
+// local diagonal of the others, which the caller launches with skip_coupled).
+void launch_cell_diagonal_pairs(const gmf_plan& plan, const Bucket& b, double* diag, hipStream_t stream);
+void launch_cell_diagonal_generic_skip(const gmf_plan& plan, const Bucket& b, double* diag, hipStream_t stream);
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                       hipStream_t stream);
 // lidx: host copy of the bucket's full-layout index map; red_of_full: full dof

@@ -1,0 +1,105 @@
+// Internal interfaces shared by the device translation units (plan.hip, mg.hip):
+// the preconditioner base, reduced operator application and the CG core.
+#pragma once
+#include <exception>
+#include <new>
+#include <string>
+
+#include "common.hpp"
+
+struct gmf_preconditioner {
+  gmf_workspace* ws = nullptr;
+  virtual ~gmf_preconditioner() = default;
+  // z = M^-1 r on ws->stream.
+  virtual void apply(const double* r, double* z) = 0;
+  // z = M^-1 r and scalars[slot] = r.z (fused where the preconditioner allows).
+  virtual void apply_dot(const double* r, double* z, int64_t n, int slot);
+};
+
+namespace gmf {
+
+// Records the thread's last error message (gmf_last_error) and returns s.
+gmf_status record_error(gmf_status s, const std::string& message);
+
+template <class F>
+gmf_status guarded_call(F&& f) {
+  try {
+    f();
+    return GMF_OK;
+  } catch (const Error& e) {
+    return record_error(e.status, e.what());
+  } catch (const std::bad_alloc&) {
+    return record_error(GMF_EDEVICE, "host allocation failed");
+  } catch (const std::exception& e) {
+    return record_error(GMF_EINVAL, e.what());
+  }
+}
+
+inline unsigned grid1(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 4096)));
+}
+
+template <class T>
+T* dalloc(size_t n) {
+  void* d = nullptr;
+  if (n == 0) n = 1;
+  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
+  return static_cast<T*>(d);
+}
+
+// Owning device buffer.
+template <class T>
+struct DeviceBuffer {
+  T* p = nullptr;
+  DeviceBuffer() = default;
+  explicit DeviceBuffer(size_t n) : p(dalloc<T>(n)) {}
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  DeviceBuffer(DeviceBuffer&& o) noexcept : p(o.p) { o.p = nullptr; }
+  DeviceBuffer& operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+      if (p) (void)hipFree(p);
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  ~DeviceBuffer() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+void sync(gmf_workspace* ws);
+// y_red = P^T A P x_red on ws->stream (src/assembly.jl:905-915).
+void reduced_apply(gmf_workspace* ws, const double* xr, double* yr);
+// Exact reduced diagonal into diag (n_red) on ws->stream.
+void reduced_diagonal(gmf_workspace* ws, double* diag);
+// inv = 1 / d with the reference's zero / non-finite rejection (GMF_EDIAGONAL).
+void invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n);
+void precond_apply(gmf_workspace* ws, gmf_preconditioner* M, const double* r, double* z, int64_t n);
+void precond_apply_dot(gmf_workspace* ws, gmf_preconditioner* M, const double* r, double* z, int64_t n, int slot);
+
+struct CgVectors {
+  DeviceBuffer<double> r, z, p, q;
+  explicit CgVectors(int64_t n)
+      : r(static_cast<size_t>(n)), z(static_cast<size_t>(n)), p(static_cast<size_t>(n)), q(static_cast<size_t>(n)) {}
+};
+
+// Preconditioned CG core (src/solve.jl:376-431); throws Error on breakdown or
+// non-convergence.
+void cg_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, bool use_initial, double rtol,
+            double atol, int64_t maxiter, gmf_solve_info& inf, CgVectors& v);
+
+struct JacobiPrecond : gmf_preconditioner {
+  double* invdiag = nullptr;
+  bool owns = true;
+  int64_t n = 0;
+  ~JacobiPrecond() override {
+    if (invdiag && owns) (void)hipFree(invdiag);
+  }
+  void apply(const double* r, double* z) override;
+  void apply_dot(const double* r, double* z, int64_t nn, int slot) override;
+};
+
+}  // namespace gmf

@@ -1,0 +1,544 @@
+// hp geometric multigrid V-cycle on the device (include/grico_mf.h
+// gmf_precond_create_mg).
+//
+// Reference: src/multigrid.jl:858-1087 (_apply_preconditioner!, _vcycle!,
+// _smooth_jacobi!, _prolongate_reduced_add!, _restrict_reduced!,
+// _coarse_solve!) and 382-408 (_compile_coarse_solver). Every level owns a
+// workspace of its rediscretized plan; transfers are CSR tables in both
+// directions so prolongation and restriction are both gathers (deterministic,
+// no atomics). The first smoothing sweep of each level starts from x = 0 and
+// skips the product A * 0.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "blas.hpp"
+#include "common.hpp"
+#include "internal.hpp"
+
+namespace gmf {
+namespace {
+
+// ---- kernels -----------------------------------------------------------------
+
+#define GMF_GRID_STRIDE(i, n) \
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < (n); i += int64_t(gridDim.x) * blockDim.x)
+
+__global__ void k_jacobi_first(double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
+                               double omega, int64_t n) {
+  GMF_GRID_STRIDE(i, n) x[i] = omega * (dinv[i] * b[i]);
+}
+
+// x += omega D^-1 (b - q), q = A x.
+__global__ void k_jacobi_update(double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ q,
+                                const double* __restrict__ dinv, double omega, int64_t n) {
+  GMF_GRID_STRIDE(i, n) x[i] = fma(omega, dinv[i] * (b[i] - q[i]), x[i]);
+}
+
+// bc[c] = sum_f P[f,c] (b[f] - q[f]) over the coarse row's CSR (q may be null).
+__global__ void k_restrict(double* __restrict__ bc, const double* __restrict__ b, const double* __restrict__ q,
+                           const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                           const double* __restrict__ val, int64_t nc) {
+  GMF_GRID_STRIDE(c, nc) {
+    double s = 0.0;
+    for (int64_t t = off[c]; t < off[c + 1]; ++t) {
+      const int32_t f = idx[t];
+      const double r = q ? b[f] - q[f] : b[f];
+      s = fma(val[t], r, s);
+    }
+    bc[c] = s;
+  }
+}
+
+// x[f] += sum_c P[f,c] xc[c].
+__global__ void k_prolong_add(double* __restrict__ x, const double* __restrict__ xc, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ idx, const double* __restrict__ val, int64_t nf) {
+  GMF_GRID_STRIDE(f, nf) {
+    double s = 0.0;
+    for (int64_t t = off[f]; t < off[f + 1]; ++t) s = fma(val[t], xc[idx[t]], s);
+    x[f] += s;
+  }
+}
+
+// Chebyshev start from x = 0: r = b, d = s D^-1 r, x = d.
+__global__ void k_cheb_start0(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
+                              const double* __restrict__ b, const double* __restrict__ dinv, double s, int64_t n) {
+  GMF_GRID_STRIDE(i, n) {
+    const double ri = b[i];
+    const double di = s * (dinv[i] * ri);
+    r[i] = ri;
+    d[i] = di;
+    x[i] = di;
+  }
+}
+
+// Chebyshev start from x with q = A x: r = b - q, d = s D^-1 r, x += d.
+__global__ void k_cheb_start(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
+                             const double* __restrict__ b, const double* __restrict__ q, const double* __restrict__ dinv,
+                             double s, int64_t n) {
+  GMF_GRID_STRIDE(i, n) {
+    const double ri = b[i] - q[i];
+    const double di = s * (dinv[i] * ri);
+    r[i] = ri;
+    d[i] = di;
+    x[i] += di;
+  }
+}
+
+// Chebyshev step with q = A d: r -= q, d = c1 d + c2 D^-1 r, x += d.
+__global__ void k_cheb_step(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
+                            const double* __restrict__ q, const double* __restrict__ dinv, double c1, double c2,
+                            int64_t n) {
+  GMF_GRID_STRIDE(i, n) {
+    const double ri = r[i] - q[i];
+    const double di = fma(c1, d[i], c2 * (dinv[i] * ri));
+    r[i] = ri;
+    d[i] = di;
+    x[i] += di;
+  }
+}
+
+// x = Ainv b for a small dense row-major inverse (one wavefront per row).
+__global__ void k_dense_matvec(double* __restrict__ x, const double* __restrict__ A, const double* __restrict__ b,
+                               int64_t n) {
+  const int64_t row = blockIdx.x * int64_t(blockDim.x / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s = fma(A[row * n + j], b[j], s);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[row] = s;
+}
+
+__global__ void k_unit(double* __restrict__ e, int64_t j, int64_t n) {
+  GMF_GRID_STRIDE(i, n) e[i] = (i == j) ? 1.0 : 0.0;
+}
+
+__global__ void k_mul(double* __restrict__ out, const double* __restrict__ a, const double* __restrict__ b, int64_t n) {
+  GMF_GRID_STRIDE(i, n) out[i] = a[i] * b[i];
+}
+
+__global__ void k_sqrt(double* __restrict__ out, const double* __restrict__ a, int64_t n) {
+  GMF_GRID_STRIDE(i, n) out[i] = sqrt(a[i]);
+}
+
+// Deterministic start vector of the Lanczos estimate.
+__global__ void k_start_vector(double* __restrict__ v, int64_t n) {
+  GMF_GRID_STRIDE(i, n) {
+    const double t = double(i + 1);
+    v[i] = 1.0 + 0.5 * sin(0.019 * t) + 0.1 * cos(0.043 * t);
+  }
+}
+
+// ---- host helpers ------------------------------------------------------------
+
+// Largest eigenvalue of the symmetric tridiagonal (alpha, beta) by Sturm
+// bisection.
+double tridiagonal_max_eigenvalue(const std::vector<double>& alpha, const std::vector<double>& beta) {
+  const size_t m = alpha.size();
+  double lo = std::numeric_limits<double>::max(), hi = -lo;
+  for (size_t i = 0; i < m; ++i) {
+    const double rad = (i > 0 ? std::fabs(beta[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(beta[i]) : 0.0);
+    lo = std::min(lo, alpha[i] - rad);
+    hi = std::max(hi, alpha[i] + rad);
+  }
+  auto count_below = [&](double x) {
+    int c = 0;
+    double q = alpha[0] - x;
+    if (q < 0) ++c;
+    for (size_t i = 1; i < m; ++i) {
+      const double qq = (q == 0.0) ? std::numeric_limits<double>::epsilon() * (std::fabs(beta[i - 1]) + 1e-300) : q;
+      q = alpha[i] - x - beta[i - 1] * beta[i - 1] / qq;
+      if (q < 0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-14 * std::max(1.0, std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= int(m))
+      hi = mid;
+    else
+      lo = mid;
+  }
+  return hi;
+}
+
+struct Level {
+  gmf_workspace* ws = nullptr;
+  int64_t n = 0;
+  DeviceBuffer<double> invdiag, x, b, q, r, d, sdiag;
+  double lambda_max = 0.0;
+};
+
+struct TransferCsr {
+  int64_t nc = 0, nf = 0;
+  DeviceBuffer<int64_t> poff, roff;
+  DeviceBuffer<int32_t> pidx, ridx;
+  DeviceBuffer<double> pval, rval;
+};
+
+template <class T>
+DeviceBuffer<T> upload_vec(const std::vector<T>& h) {
+  DeviceBuffer<T> d(h.size());
+  if (!h.empty()) GMF_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+struct MGPrecond : gmf_preconditioner {
+  std::vector<Level> levels;
+  std::vector<TransferCsr> transfers;
+  int smoother = GMF_SMOOTHER_JACOBI;
+  int pre = 2, post = 2;
+  double omega = 0.8;
+  int cheb_degree = 4;
+  double cheb_range = 20.0;
+  // coarse solver
+  bool dense = false;
+  DeviceBuffer<double> coarse_inverse;
+  std::unique_ptr<JacobiPrecond> coarse_jacobi;
+  std::unique_ptr<CgVectors> coarse_vectors;
+  double coarse_rtol = 1e-12, coarse_atol = 0.0;
+  int64_t coarse_maxiter = 10000;
+
+  void apply(const double* r, double* z) override {
+    for (Level& L : levels) L.ws->stream = ws->stream;
+    vcycle(int(levels.size()) - 1, z, r);
+  }
+
+  void smooth(int l, double* x, const double* b, int steps, bool x_zero) {
+    Level& L = levels[size_t(l)];
+    const int64_t n = L.n;
+    hipStream_t s = ws->stream;
+    if (n == 0) return;
+    if (smoother == GMF_SMOOTHER_JACOBI) {
+      for (int k = 0; k < steps; ++k) {
+        if (x_zero && k == 0) {
+          hipLaunchKernelGGL(k_jacobi_first, dim3(grid1(n)), dim3(256), 0, s, x, b, L.invdiag.p, omega, n);
+        } else {
+          reduced_apply(L.ws, x, L.q.p);
+          hipLaunchKernelGGL(k_jacobi_update, dim3(grid1(n)), dim3(256), 0, s, x, b, L.q.p, L.invdiag.p, omega, n);
+        }
+        GMF_HIP(hipGetLastError());
+      }
+      return;
+    }
+    const double lmax = L.lambda_max, lmin = L.lambda_max / cheb_range;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    for (int k = 0; k < steps; ++k) {
+      if (x_zero && k == 0) {
+        hipLaunchKernelGGL(k_cheb_start0, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
+                           1.0 / theta, n);
+      } else {
+        reduced_apply(L.ws, x, L.q.p);
+        hipLaunchKernelGGL(k_cheb_start, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p, L.invdiag.p,
+                           1.0 / theta, n);
+      }
+      double rho = 1.0 / sigma;
+      for (int i = 1; i < cheb_degree; ++i) {
+        reduced_apply(L.ws, L.d.p, L.q.p);
+        const double rho_new = 1.0 / (2.0 * sigma - rho);
+        hipLaunchKernelGGL(k_cheb_step, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
+                           rho_new * rho, 2.0 * rho_new / delta, n);
+        rho = rho_new;
+      }
+      GMF_HIP(hipGetLastError());
+    }
+  }
+
+  void coarse_solve(double* x, const double* b) {
+    Level& L = levels[0];
+    if (L.n == 0) return;
+    if (dense) {
+      const unsigned blocks = unsigned((L.n + 3) / 4);
+      hipLaunchKernelGGL(k_dense_matvec, dim3(blocks), dim3(256), 0, ws->stream, x, coarse_inverse.p, b, L.n);
+      GMF_HIP(hipGetLastError());
+      return;
+    }
+    gmf_solve_info inf{};
+    coarse_jacobi->ws = L.ws;
+    cg_run(L.ws, coarse_jacobi.get(), b, x, false, coarse_rtol, coarse_atol, coarse_maxiter, inf, *coarse_vectors);
+  }
+
+  // x is treated as zero on entry.
+  void vcycle(int l, double* x, const double* b) {
+    if (l == 0) {
+      coarse_solve(x, b);
+      return;
+    }
+    Level& L = levels[size_t(l)];
+    Level& C = levels[size_t(l - 1)];
+    TransferCsr& T = transfers[size_t(l - 1)];
+    hipStream_t s = ws->stream;
+    if (pre > 0) {
+      smooth(l, x, b, pre, true);
+      reduced_apply(L.ws, x, L.q.p);
+      hipLaunchKernelGGL(k_restrict, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, L.q.p, T.roff.p, T.ridx.p,
+                         T.rval.p, T.nc);
+    } else {
+      fill(x, 0.0, L.n, s);
+      hipLaunchKernelGGL(k_restrict, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, nullptr, T.roff.p, T.ridx.p,
+                         T.rval.p, T.nc);
+    }
+    GMF_HIP(hipGetLastError());
+    vcycle(l - 1, C.x.p, C.b.p);
+    hipLaunchKernelGGL(k_prolong_add, dim3(grid1(T.nf)), dim3(256), 0, s, x, C.x.p, T.poff.p, T.pidx.p, T.pval.p,
+                       T.nf);
+    GMF_HIP(hipGetLastError());
+    smooth(l, x, b, post, false);
+  }
+};
+
+TransferCsr build_transfer(const gmf_transfer_desc& t, int64_t nc, int64_t nf) {
+  require(t.nnz >= 0 && (t.nnz == 0 || (t.coarse_indices && t.fine_indices && t.coefficients)), GMF_EINVAL,
+          "transfer arrays missing");
+  require(nc < (int64_t(1) << 31) && nf < (int64_t(1) << 31), GMF_EUNSUPPORTED, "level too large for 32-bit indices");
+  TransferCsr c;
+  c.nc = nc;
+  c.nf = nf;
+  const size_t nnz = size_t(t.nnz);
+  std::vector<int64_t> poff(size_t(nf) + 1, 0), roff(size_t(nc) + 1, 0);
+  for (size_t k = 0; k < nnz; ++k) {
+    const int64_t ci = t.coarse_indices[k], fi = t.fine_indices[k];
+    require(ci >= 0 && ci < nc && fi >= 0 && fi < nf, GMF_EINVAL, "transfer index out of range");
+    require(std::isfinite(t.coefficients[k]), GMF_EINVAL, "transfer coefficient must be finite");
+    if (k > 0)
+      require(t.fine_indices[k - 1] < fi || (t.fine_indices[k - 1] == fi && t.coarse_indices[k - 1] < ci),
+              GMF_EINVAL, "transfer entries must be sorted by (fine, coarse) without duplicates");
+    poff[size_t(fi) + 1]++;
+    roff[size_t(ci) + 1]++;
+  }
+  std::partial_sum(poff.begin(), poff.end(), poff.begin());
+  std::partial_sum(roff.begin(), roff.end(), roff.begin());
+  std::vector<int32_t> pidx(nnz), ridx(nnz);
+  std::vector<double> pval(nnz), rval(nnz);
+  std::vector<int64_t> fill_r(roff.begin(), roff.end() - 1);
+  for (size_t k = 0; k < nnz; ++k) {
+    pidx[k] = int32_t(t.coarse_indices[k]);
+    pval[k] = t.coefficients[k];
+    const size_t slot = size_t(fill_r[size_t(t.coarse_indices[k])]++);
+    ridx[slot] = int32_t(t.fine_indices[k]);
+    rval[slot] = t.coefficients[k];
+  }
+  c.poff = upload_vec(poff);
+  c.roff = upload_vec(roff);
+  c.pidx = upload_vec(pidx);
+  c.ridx = upload_vec(ridx);
+  c.pval = upload_vec(pval);
+  c.rval = upload_vec(rval);
+  return c;
+}
+
+// Lanczos estimate of lambda_max(D^-1 A) through the symmetric D^-1/2 A D^-1/2.
+double estimate_lambda_max(Level& L, int iterations) {
+  const int64_t n = L.n;
+  gmf_workspace* ws = L.ws;
+  hipStream_t s = ws->stream;
+  const int m = int(std::min<int64_t>(iterations, n));
+  if (m <= 0) return 0.0;
+  DeviceBuffer<double> v(static_cast<size_t>(n)), vp(static_cast<size_t>(n)), u(static_cast<size_t>(n)), w(static_cast<size_t>(n));
+  hipLaunchKernelGGL(k_sqrt, dim3(grid1(n)), dim3(256), 0, s, L.sdiag.p, L.invdiag.p, n);
+  hipLaunchKernelGGL(k_start_vector, dim3(grid1(n)), dim3(256), 0, s, v.p, n);
+  fill(vp.p, 0.0, n, s);
+  dot(ws, v.p, v.p, n, kTmp);
+  read_scalars(ws, kTmp + 1);
+  double nrm = std::sqrt(ws->host_scalars[kTmp]);
+  axpby(ws, v.p, 0.0, v.p, 1.0 / nrm, n);
+  std::vector<double> alpha, beta;
+  double beta_prev = 0.0;
+  for (int j = 0; j < m; ++j) {
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, u.p, L.sdiag.p, v.p, n);
+    reduced_apply(ws, u.p, w.p);
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, w.p, L.sdiag.p, w.p, n);
+    GMF_HIP(hipGetLastError());
+    dot(ws, w.p, v.p, n, kTmp);
+    read_scalars(ws, kTmp + 1);
+    const double a = ws->host_scalars[kTmp];
+    alpha.push_back(a);
+    axpby(ws, w.p, -a, v.p, 1.0, n);
+    if (j > 0) axpby(ws, w.p, -beta_prev, vp.p, 1.0, n);
+    dot(ws, w.p, w.p, n, kTmp);
+    read_scalars(ws, kTmp + 1);
+    const double bnext = std::sqrt(std::max(0.0, ws->host_scalars[kTmp]));
+    if (j + 1 == m || !(bnext > 1e-14 * std::fabs(a))) break;
+    beta.push_back(bnext);
+    std::swap(vp.p, v.p);                       // vp = old v
+    axpby(ws, w.p, 0.0, w.p, 1.0 / bnext, n);   // w /= beta
+    std::swap(v.p, w.p);                        // v = w
+    beta_prev = bnext;
+  }
+  beta.resize(alpha.size() > 0 ? alpha.size() - 1 : 0);
+  sync(ws);
+  return tridiagonal_max_eigenvalue(alpha, beta);
+}
+
+// Dense coarse inverse from the probed SPD coarse matrix (src/multigrid.jl:382-424).
+DeviceBuffer<double> dense_coarse_inverse(Level& L) {
+  const int64_t n = L.n;
+  gmf_workspace* ws = L.ws;
+  hipStream_t s = ws->stream;
+  DeviceBuffer<double> e(static_cast<size_t>(n)), col(static_cast<size_t>(n * n));
+  for (int64_t j = 0; j < n; ++j) {
+    hipLaunchKernelGGL(k_unit, dim3(grid1(n)), dim3(256), 0, s, e.p, j, n);
+    reduced_apply(ws, e.p, col.p + j * n);  // column j of A (stored row j of A^T)
+  }
+  GMF_HIP(hipGetLastError());
+  std::vector<double> At(size_t(n * n));
+  GMF_HIP(hipMemcpyAsync(At.data(), col.p, At.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  sync(ws);
+  // A[i][j] = At[j*n + i]
+  double scale = 0.0;
+  for (double v : At) {
+    require(std::isfinite(v), GMF_ENOTSPD, "declared SPD coarse multigrid matrix is not numerically symmetric");
+    scale = std::max(scale, std::fabs(v));
+  }
+  const double allowed = std::sqrt(std::numeric_limits<double>::epsilon()) * scale;
+  std::vector<double> A(size_t(n * n));
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < n; ++j) A[size_t(i * n + j)] = At[size_t(j * n + i)];
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t r = c + 1; r < n; ++r)
+      require(std::fabs(A[size_t(r * n + c)] - A[size_t(c * n + r)]) <= allowed, GMF_ENOTSPD,
+              "declared SPD coarse multigrid matrix is not numerically symmetric");
+  // Cholesky A = L L^T on the lower triangle.
+  std::vector<double> Lf(A);
+  for (int64_t j = 0; j < n; ++j) {
+    double d = Lf[size_t(j * n + j)];
+    for (int64_t k = 0; k < j; ++k) d -= Lf[size_t(j * n + k)] * Lf[size_t(j * n + k)];
+    require(d > 0.0 && std::isfinite(d), GMF_ENOTSPD, "declared SPD coarse multigrid matrix is not positive definite");
+    const double ljj = std::sqrt(d);
+    Lf[size_t(j * n + j)] = ljj;
+    for (int64_t i = j + 1; i < n; ++i) {
+      double v = Lf[size_t(i * n + j)];
+      for (int64_t k = 0; k < j; ++k) v -= Lf[size_t(i * n + k)] * Lf[size_t(j * n + k)];
+      Lf[size_t(i * n + j)] = v / ljj;
+    }
+  }
+  // Inverse column by column: L y = e_j, L^T x = y.
+  std::vector<double> inv(size_t(n * n));
+  std::vector<double> y(static_cast<size_t>(n));
+  for (int64_t j = 0; j < n; ++j) {
+    for (int64_t i = 0; i < n; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+      for (int64_t k = 0; k < i; ++k) v -= Lf[size_t(i * n + k)] * y[size_t(k)];
+      y[size_t(i)] = v / Lf[size_t(i * n + i)];
+    }
+    for (int64_t i = n - 1; i >= 0; --i) {
+      double v = y[size_t(i)];
+      for (int64_t k = i + 1; k < n; ++k) v -= Lf[size_t(k * n + i)] * inv[size_t(k * n + j)];
+      inv[size_t(i * n + j)] = v / Lf[size_t(i * n + i)];
+    }
+  }
+  // Symmetrize (the inverse of a symmetric matrix).
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = i + 1; j < n; ++j) {
+      const double v = 0.5 * (inv[size_t(i * n + j)] + inv[size_t(j * n + i)]);
+      inv[size_t(i * n + j)] = v;
+      inv[size_t(j * n + i)] = v;
+    }
+  return upload_vec(inv);
+}
+
+}  // namespace
+}  // namespace gmf
+
+using namespace gmf;
+
+extern "C" {
+
+gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** out) {
+  return guarded_call([&] {
+    require(desc && out, GMF_EINVAL, "null argument");
+    require(desc->level_count >= 2 && desc->levels && desc->transfers, GMF_EINVAL,
+            "multigrid needs at least two levels and their transfers");
+    require(desc->smoother == GMF_SMOOTHER_JACOBI || desc->smoother == GMF_SMOOTHER_CHEBYSHEV, GMF_EINVAL,
+            "unknown smoother");
+    require(desc->pre_steps >= 0 && desc->post_steps >= 0, GMF_EINVAL, "smoothing steps must be non-negative");
+    require(std::isfinite(desc->damping) && desc->damping > 0, GMF_EINVAL, "smoother_damping must be positive");
+    require(desc->smoother != GMF_SMOOTHER_CHEBYSHEV ||
+                (desc->chebyshev_degree >= 1 && desc->chebyshev_range > 1.0 && desc->eigen_iterations >= 1),
+            GMF_EINVAL, "invalid Chebyshev parameters");
+    require(std::isfinite(desc->coarse_rtol) && desc->coarse_rtol >= 0 && std::isfinite(desc->coarse_atol) &&
+                desc->coarse_atol >= 0 && desc->coarse_maxiter >= 1,
+            GMF_EINVAL, "invalid coarse solver tolerances");
+    auto mg = std::make_unique<MGPrecond>();
+    const int L = desc->level_count;
+    mg->ws = desc->levels[L - 1];
+    require(mg->ws != nullptr, GMF_EINVAL, "null level workspace");
+    mg->smoother = desc->smoother;
+    mg->pre = desc->pre_steps;
+    mg->post = desc->post_steps;
+    mg->omega = desc->damping;
+    mg->cheb_degree = desc->chebyshev_degree;
+    mg->cheb_range = desc->chebyshev_range;
+    mg->coarse_rtol = desc->coarse_rtol;
+    mg->coarse_atol = desc->coarse_atol;
+    mg->coarse_maxiter = desc->coarse_maxiter;
+    mg->levels.resize(size_t(L));
+    const bool cheb = desc->smoother == GMF_SMOOTHER_CHEBYSHEV;
+    for (int l = 0; l < L; ++l) {
+      Level& lv = mg->levels[size_t(l)];
+      lv.ws = desc->levels[l];
+      require(lv.ws != nullptr, GMF_EINVAL, "null level workspace");
+      lv.ws->stream = mg->ws->stream;
+      lv.n = lv.ws->plan->n_red;
+      const size_t n = size_t(lv.n);
+      lv.invdiag = DeviceBuffer<double>(n);
+      {
+        DeviceBuffer<double> d(n);
+        reduced_diagonal(lv.ws, d.p);
+        invert_diagonal(lv.ws, d.p, lv.invdiag.p, lv.n);
+      }
+      if (l < L - 1) {
+        lv.x = DeviceBuffer<double>(n);
+        lv.b = DeviceBuffer<double>(n);
+      }
+      if (l > 0) {
+        lv.q = DeviceBuffer<double>(n);
+        if (cheb) {
+          lv.r = DeviceBuffer<double>(n);
+          lv.d = DeviceBuffer<double>(n);
+          lv.sdiag = DeviceBuffer<double>(n);
+          if (lv.n > 0) {
+            lv.lambda_max = 1.2 * estimate_lambda_max(lv, desc->eigen_iterations);
+            require(std::isfinite(lv.lambda_max) && lv.lambda_max > 0.0, GMF_ENOTSPD,
+                    "Chebyshev smoother needs a positive spectrum of D^-1 A");
+          }
+        }
+      }
+    }
+    for (int l = 0; l + 1 < L; ++l)
+      mg->transfers.push_back(
+          build_transfer(desc->transfers[l], mg->levels[size_t(l)].n, mg->levels[size_t(l + 1)].n));
+    Level& C = mg->levels[0];
+    if (C.n <= desc->coarse_direct_limit) {
+      mg->dense = true;
+      if (C.n > 0) mg->coarse_inverse = dense_coarse_inverse(C);
+    } else {
+      mg->coarse_jacobi = std::make_unique<JacobiPrecond>();
+      mg->coarse_jacobi->ws = C.ws;
+      mg->coarse_jacobi->n = C.n;
+      mg->coarse_jacobi->invdiag = C.invdiag.p;
+      mg->coarse_jacobi->owns = false;
+      mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
+    }
+    sync(mg->ws);
+    *out = mg.release();
+  });
+}
+
+gmf_status gmf_mg_level_info(const gmf_preconditioner* M, int32_t level, int64_t* reduced_dofs, double* lambda_max) {
+  return guarded_call([&] {
+    const auto* mg = dynamic_cast<const MGPrecond*>(M);
+    require(mg != nullptr, GMF_EINVAL, "not a multigrid preconditioner");
+    require(level >= 0 && size_t(level) < mg->levels.size(), GMF_EINVAL, "level out of range");
+    if (reduced_dofs) *reduced_dofs = mg->levels[size_t(level)].n;
+    if (lambda_max) *lambda_max = mg->levels[size_t(level)].lambda_max;
+  });
+}
+
+}  // extern "C"

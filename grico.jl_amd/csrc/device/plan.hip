@@ -8,18 +8,7 @@
 
 #include "blas.hpp"
 #include "common.hpp"
-
-struct gmf_preconditioner {
-  gmf_workspace* ws = nullptr;
-  virtual ~gmf_preconditioner() = default;
-  // z = M^-1 r on ws->stream.
-  virtual void apply(const double* r, double* z) = 0;
-  // z = M^-1 r and scalars[slot] = r.z (fused where the preconditioner allows).
-  virtual void apply_dot(const double* r, double* z, int64_t n, int slot) {
-    apply(r, z);
-    gmf::dot(ws, r, z, n, slot);
-  }
-};
+#include "internal.hpp"
 
 namespace gmf {
 namespace {
@@ -50,13 +39,6 @@ T* upload(gmf_plan& p, const T* host, size_t n) {
   return upload_array(p, host, n);
 }
 
-template <class T>
-T* dalloc(size_t n) {
-  void* d = nullptr;
-  if (n == 0) n = 1;
-  GMF_HIP(hipMalloc(&d, n * sizeof(T)));
-  return static_cast<T*>(d);
-}
 
 void check_csr(int64_t rows, const int64_t* off, const int64_t* idx, int64_t bound, const char* what) {
   require(rows == 0 || off != nullptr, GMF_EINVAL, std::string(what) + ": offsets missing");
@@ -76,6 +58,65 @@ Csr upload_csr(gmf_plan& p, int64_t rows, const int64_t* off, const int64_t* idx
   c.indices = upload(p, idx, size_t(off[rows]));
   c.coefficients = upload(p, coef, size_t(off[rows]));
   return c;
+}
+
+// Coupled cells of a bucket for the exact reduced diagonal: a cell whose local
+// modes reach one reduced dof through several modes (multi-term hanging /
+// p-mismatch expansions, repeated dofs) contributes v_j^T A_c v_j per reduced
+// dof j, with v_j the local coefficient vector of j, instead of c^2 A_c,ii.
+void build_diagonal_pairs(gmf_plan& p, Bucket& b, const std::vector<int32_t>& ridx, const gmf_bucket_desc& bd,
+                          const std::vector<int32_t>& mridx) {
+  const int64_t N = b.Ntot;
+  std::vector<uint8_t> coupled(size_t(b.cells), 0);
+  std::vector<int64_t> pcell, poff{0};
+  std::vector<int32_t> pdof, ploc;
+  std::vector<double> pcoef;
+  bool any = false;
+  std::vector<std::pair<int32_t, std::pair<int32_t, double>>> ent;  // (dof, (local, coef))
+  for (int64_t c = 0; c < b.cells; ++c) {
+    ent.clear();
+    bool multi = false;
+    for (int64_t i = 0; i < N; ++i) {
+      const int32_t g = bd.local_index[c * N + i];
+      if (g >= 0) {
+        const int32_t r = ridx[size_t(c * N + i)];
+        if (r >= 0) ent.push_back({r, {int32_t(i), bd.local_coefficient ? bd.local_coefficient[c * N + i] : 1.0}});
+      } else if (g <= -2) {
+        multi = true;
+        const int64_t m = -int64_t(g) - 2;
+        for (int64_t t = bd.multi_offsets[m]; t < bd.multi_offsets[m + 1]; ++t) {
+          const int32_t r = mridx[size_t(t)];
+          if (r >= 0) ent.push_back({r, {int32_t(i), bd.multi_coefficients[t]}});
+        }
+      }
+    }
+    std::stable_sort(ent.begin(), ent.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    bool repeated = false;
+    for (size_t k = 1; k < ent.size() && !repeated; ++k) repeated = ent[k].first == ent[k - 1].first;
+    if (!multi && !repeated) continue;
+    coupled[size_t(c)] = 1;
+    any = true;
+    for (size_t k = 0; k < ent.size();) {
+      size_t e = k;
+      pcell.push_back(c);
+      pdof.push_back(ent[k].first);
+      while (e < ent.size() && ent[e].first == ent[k].first) {
+        ploc.push_back(ent[e].second.first);
+        pcoef.push_back(ent[e].second.second);
+        ++e;
+      }
+      poff.push_back(int64_t(ploc.size()));
+      k = e;
+    }
+  }
+  if (!any) return;
+  b.coupled = upload_array(p, coupled.data(), coupled.size());
+  b.pair_count = int64_t(pcell.size());
+  b.pair_cell = upload_array(p, pcell.data(), pcell.size());
+  b.pair_dof = upload_array(p, pdof.data(), pdof.size());
+  b.pair_off = upload_array(p, poff.data(), poff.size());
+  b.pair_loc = upload_array(p, ploc.data(), ploc.size());
+  b.pair_coef = upload_array(p, pcoef.data(), pcoef.size());
 }
 
 // ---- small kernels ---------------------------------------------------------
@@ -110,6 +151,10 @@ __global__ void k_csr_gather(double* __restrict__ out, const double* __restrict_
   }
 }
 
+__global__ void k_fill_int(int* __restrict__ p, int v, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
 __global__ void k_invert_diagonal(const double* __restrict__ d, double* __restrict__ inv, int64_t n,
                                   int* __restrict__ bad) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
@@ -130,10 +175,6 @@ __global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x
     y[i] = fma(a, x[i], y[i]);
 }
 
-inline unsigned grid1(int64_t n) {
-  int64_t g = (n + 255) / 256;
-  return unsigned(std::max<int64_t>(1, std::min<int64_t>(g, 4096)));
-}
 
 // ---- operator application ---------------------------------------------------
 
@@ -181,6 +222,15 @@ void project(gmf_workspace* ws, const double* yf, double* yr) {
   GMF_HIP(hipGetLastError());
 }
 
+}  // namespace
+
+gmf_status record_error(gmf_status s, const std::string& message) { return fail(s, message); }
+
+void gmf_preconditioner_apply_dot_default(gmf_preconditioner* M, const double* r, double* z, int64_t n, int slot) {
+  M->apply(r, z);
+  dot(M->ws, r, z, n, slot);
+}
+
 void reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
   gmf_plan& p = *ws->plan;
   require(xr != yr, GMF_EALIAS, "result and reduced coefficient vector must not alias");
@@ -197,19 +247,40 @@ void reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
   project(ws, ws->full_out, yr);
 }
 
+void reduced_diagonal(gmf_workspace* ws, double* diag) {
+  const gmf_plan& p = *ws->plan;
+  require(p.selection_map, GMF_EUNSUPPORTED,
+          "exact reduced diagonal needs a selection reduced map (affine / mean rows are not implemented)");
+  fill(diag, 0.0, p.n_red, ws->stream);
+  for (const Bucket& b : p.buckets) {
+    if (b.coupled) {
+      launch_cell_diagonal_generic_skip(p, b, diag, ws->stream);
+      launch_cell_diagonal_pairs(p, b, diag, ws->stream);
+    } else {
+      launch_cell_diagonal_generic(p, b, diag, true, ws->stream);
+    }
+  }
+}
+
+void invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n) {
+  DeviceBuffer<int> bad(static_cast<size_t>(1));
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ws->stream, bad.p, 0, int64_t(1));
+  hipLaunchKernelGGL(k_invert_diagonal, dim3(grid1(n)), dim3(256), 0, ws->stream, d, inv, n, bad.p);
+  GMF_HIP(hipGetLastError());
+  int h = 0;
+  GMF_HIP(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, ws->stream));
+  GMF_HIP(hipStreamSynchronize(ws->stream));
+  require(h == 0, GMF_EDIAGONAL, "Jacobi preconditioner requires a nonzero finite diagonal");
+}
+
 void sync(gmf_workspace* ws) { GMF_HIP(hipStreamSynchronize(ws->stream)); }
 
 // ---- preconditioners ---------------------------------------------------------
 
-struct JacobiPrecond : gmf_preconditioner {
-  double* invdiag = nullptr;
-  int64_t n = 0;
-  ~JacobiPrecond() override {
-    if (invdiag) (void)hipFree(invdiag);
-  }
-  void apply(const double* r, double* z) override { jacobi_dot(ws, z, r, invdiag, n, kTmp); }
-  void apply_dot(const double* r, double* z, int64_t nn, int slot) override { jacobi_dot(ws, z, r, invdiag, nn, slot); }
-};
+void JacobiPrecond::apply(const double* r, double* z) { jacobi_dot(ws, z, r, invdiag, n, kTmp); }
+void JacobiPrecond::apply_dot(const double* r, double* z, int64_t nn, int slot) {
+  jacobi_dot(ws, z, r, invdiag, nn, slot);
+}
 
 void precond_apply_dot(gmf_workspace* ws, gmf_preconditioner* M, const double* r, double* z, int64_t n, int slot) {
   if (M == nullptr) {
@@ -224,6 +295,58 @@ void precond_apply(gmf_workspace* ws, gmf_preconditioner* M, const double* r, do
     copy(z, r, n, ws->stream);
   else
     M->apply(r, z);
+}
+
+void cg_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, bool use_initial, double rtol,
+            double atol, int64_t maxiter, gmf_solve_info& inf, CgVectors& v) {
+  const int64_t n = ws->plan->n_red;
+  inf = gmf_solve_info{};
+  if (n == 0) {
+    inf.converged = 1;
+    return;
+  }
+  copy(v.r.p, b, n, ws->stream);
+  if (use_initial) {
+    reduced_apply(ws, x, v.q.p);
+    inf.operator_applications++;
+    axpby(ws, v.r.p, -1.0, v.q.p, 1.0, n);
+  } else {
+    fill(x, 0.0, n, ws->stream);
+  }
+  precond_apply_dot(ws, M, v.r.p, v.z.p, n, kRz);
+  inf.preconditioner_applications++;
+  copy(v.p.p, v.z.p, n, ws->stream);
+  dot(ws, b, b, n, kBb);
+  dot(ws, v.r.p, v.r.p, n, kRr);
+  read_scalars(ws, kTmp);
+  const double* hs = ws->host_scalars;
+  inf.rhs_norm = std::sqrt(hs[kBb]);
+  const double tol = std::max(atol, rtol * inf.rhs_norm);
+  inf.initial_residual_norm = std::sqrt(hs[kRr]);
+  inf.final_residual_norm = inf.initial_residual_norm;
+  if (hs[kRr] <= tol * tol) {
+    inf.converged = 1;
+    return;
+  }
+  for (int64_t it = 1; it <= maxiter; ++it) {
+    reduced_apply(ws, v.p.p, v.q.p);
+    inf.operator_applications++;
+    dot(ws, v.p.p, v.q.p, n, kPq);
+    cg_update(ws, x, v.r.p, v.p.p, v.q.p, n, kRz, kPq, kRr);
+    read_scalars(ws, kTmp);
+    inf.iterations = it;
+    require(hs[kPq] > 0.0, GMF_EBREAKDOWN,
+            "CG encountered a non-positive operator direction; provide a different linear_solve for this problem");
+    inf.final_residual_norm = std::sqrt(hs[kRr]);
+    if (hs[kRr] <= tol * tol) {
+      inf.converged = 1;
+      return;
+    }
+    precond_apply_dot(ws, M, v.r.p, v.z.p, n, kRzNew);
+    inf.preconditioner_applications++;
+    direction(ws, v.p.p, v.z.p, n, kRzNew, kRz);
+  }
+  throw Error(GMF_ENOCONV, "CG did not converge in " + std::to_string(maxiter) + " iterations");
 }
 
 // ---- workspace vectors for solvers ------------------------------------------
@@ -242,8 +365,11 @@ struct SolverVectors {
   }
 };
 
-}  // namespace
 }  // namespace gmf
+
+void gmf_preconditioner::apply_dot(const double* r, double* z, int64_t n, int slot) {
+  gmf::gmf_preconditioner_apply_dot_default(this, r, z, n, slot);
+}
 
 using namespace gmf;
 
@@ -394,12 +520,14 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
             r[i] = g >= 0 ? int32_t(red_of_full[size_t(g)]) : g;
           }
           b.ridx = upload(p, r.data(), nmode);
+          std::vector<int32_t> mr;
           if (b.multi_count) {
             int64_t nt = bd.multi_offsets[b.multi_count];
-            std::vector<int32_t> mr(static_cast<size_t>(nt));
+            mr.resize(static_cast<size_t>(nt));
             for (int64_t t = 0; t < nt; ++t) mr[size_t(t)] = int32_t(red_of_full[size_t(bd.multi_indices[t])]);
             b.mridx = upload(p, mr.data(), size_t(nt));
           }
+          build_diagonal_pairs(p, b, r, bd, mr);
         }
         prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
         p.buckets.push_back(std::move(b));
@@ -586,11 +714,7 @@ gmf_status gmf_diagonal_physical(gmf_workspace* ws, double* diag) {
 gmf_status gmf_reduced_diagonal(gmf_workspace* ws, double* diag) {
   return guarded([&] {
     require(ws && diag, GMF_EINVAL, "null argument");
-    const gmf_plan& p = *ws->plan;
-    require(p.selection_map && !p.multi_terms, GMF_EUNSUPPORTED,
-            "exact reduced diagonal with multi-term constraint expansions is not implemented");
-    fill(diag, 0.0, p.n_red, ws->stream);
-    for (const Bucket& b : p.buckets) launch_cell_diagonal_generic(p, b, diag, true, ws->stream);
+    reduced_diagonal(ws, diag);
     sync(ws);
   });
 }
@@ -606,15 +730,7 @@ gmf_status gmf_vector_copy(gmf_workspace* ws, double* dst, const double* src, in
 gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n) {
   return guarded([&] {
     require(ws && d && inv && n >= 0, GMF_EINVAL, "null argument");
-    int* bad = nullptr;
-    GMF_HIP(hipMalloc(&bad, sizeof(int)));
-    GMF_HIP(hipMemsetAsync(bad, 0, sizeof(int), ws->stream));
-    hipLaunchKernelGGL(k_invert_diagonal, dim3(grid1(n)), dim3(256), 0, ws->stream, d, inv, n, bad);
-    int h = 0;
-    GMF_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, ws->stream));
-    GMF_HIP(hipStreamSynchronize(ws->stream));
-    (void)hipFree(bad);
-    require(h == 0, GMF_EDIAGONAL, "Jacobi preconditioner requires a nonzero finite diagonal");
+    invert_diagonal(ws, d, inv, n);
   });
 }
 
@@ -654,58 +770,8 @@ gmf_status gmf_cg_solve(gmf_workspace* ws, gmf_preconditioner* M, const double* 
     require(std::isfinite(rtol) && rtol >= 0 && std::isfinite(atol) && atol >= 0, GMF_EINVAL,
             "tolerances must be finite and non-negative");
     require(M == nullptr || M->ws == ws, GMF_EINVAL, "preconditioner belongs to a different workspace");
-    const int64_t n = ws->plan->n_red;
-    if (n == 0) {
-      inf.converged = 1;
-      return;
-    }
-    SolverVectors v;
-    v.r = dalloc<double>(size_t(n));
-    v.z = dalloc<double>(size_t(n));
-    v.p = dalloc<double>(size_t(n));
-    v.q = dalloc<double>(size_t(n));
-    copy(v.r, b, n, ws->stream);
-    if (use_initial) {
-      reduced_apply(ws, x, v.q);
-      inf.operator_applications++;
-      axpby(ws, v.r, -1.0, v.q, 1.0, n);
-    } else {
-      fill(x, 0.0, n, ws->stream);
-    }
-    precond_apply_dot(ws, M, v.r, v.z, n, kRz);
-    inf.preconditioner_applications++;
-    copy(v.p, v.z, n, ws->stream);
-    dot(ws, b, b, n, kBb);
-    dot(ws, v.r, v.r, n, kRr);
-    read_scalars(ws, kTmp);
-    const double* hs = ws->host_scalars;
-    inf.rhs_norm = std::sqrt(hs[kBb]);
-    const double tol = std::max(atol, rtol * inf.rhs_norm);
-    inf.initial_residual_norm = std::sqrt(hs[kRr]);
-    inf.final_residual_norm = inf.initial_residual_norm;
-    if (hs[kRr] <= tol * tol) {
-      inf.converged = 1;
-      return;
-    }
-    for (int64_t it = 1; it <= maxiter; ++it) {
-      reduced_apply(ws, v.p, v.q);
-      inf.operator_applications++;
-      dot(ws, v.p, v.q, n, kPq);
-      cg_update(ws, x, v.r, v.p, v.q, n, kRz, kPq, kRr);
-      read_scalars(ws, kTmp);
-      inf.iterations = it;
-      require(hs[kPq] > 0.0, GMF_EBREAKDOWN,
-              "CG encountered a non-positive operator direction; provide a different linear_solve for this problem");
-      inf.final_residual_norm = std::sqrt(hs[kRr]);
-      if (hs[kRr] <= tol * tol) {
-        inf.converged = 1;
-        return;
-      }
-      precond_apply_dot(ws, M, v.r, v.z, n, kRzNew);
-      inf.preconditioner_applications++;
-      direction(ws, v.p, v.z, n, kRzNew, kRz);
-    }
-    throw Error(GMF_ENOCONV, "CG did not converge in " + std::to_string(maxiter) + " iterations");
+    CgVectors v(ws->plan->n_red);
+    cg_run(ws, M, b, x, use_initial != 0, rtol, atol, maxiter, inf, v);
   });
 }
 

@@ -8,12 +8,16 @@
 #include "polys.hpp"
 #include "space.hpp"
 #include "topology.hpp"
+#include "transfer.hpp"
 
 struct gh_grid {
   gh::Grid g;
 };
 struct gh_space {
   gh::Space s;
+};
+struct gh_transfer {
+  gh::Transfer t;
 };
 
 namespace {
@@ -159,6 +163,53 @@ int gh_space_faces(const gh_space* space, int64_t* boundary_leaf, int32_t* bound
     if (interface_minus) std::memcpy(interface_minus, s.iface_minus.data(), ni * sizeof(int64_t));
     if (interface_plus) std::memcpy(interface_plus, s.iface_plus.data(), ni * sizeof(int64_t));
     if (interface_axis) std::memcpy(interface_axis, s.iface_axis.data(), ni * sizeof(int32_t));
+  });
+}
+
+
+int gh_transfer_create(const gh_grid* grid, const gh_space_view* coarse, const gh_space_view* fine,
+                       gh_transfer** out) {
+  return guarded([&] {
+    if (!grid || !coarse || !fine || !out) throw std::invalid_argument("null argument");
+    auto view = [](const gh_space_view* v) {
+      gh::SpaceView s;
+      s.D = v->dim;
+      s.leaf_count = v->leaf_count;
+      s.leaves = v->leaves;
+      s.degrees = v->degrees;
+      s.mode_offsets = v->mode_offsets;
+      s.mode_box = v->mode_box;
+      s.term_offsets = v->term_offsets;
+      s.term_indices = v->term_indices;
+      s.term_coefficients = v->term_coefficients;
+      s.full_dof_count = v->full_dof_count;
+      s.map_offsets = v->map_offsets;
+      s.map_indices = v->map_indices;
+      s.map_coefficients = v->map_coefficients;
+      s.reduced_index = v->reduced_index;
+      if (!s.leaves || !s.degrees || !s.mode_offsets || !s.mode_box || !s.term_offsets || !s.term_indices ||
+          !s.term_coefficients || !s.map_offsets || !s.map_indices || !s.map_coefficients || !s.reduced_index)
+        throw std::invalid_argument("incomplete space view");
+      return s;
+    };
+    auto t = std::make_unique<gh_transfer>();
+    t->t = gh::build_transfer(grid->g, view(coarse), view(fine));
+    *out = t.release();
+  });
+}
+
+void gh_transfer_destroy(gh_transfer* transfer) { delete transfer; }
+
+int64_t gh_transfer_nnz(const gh_transfer* transfer) { return int64_t(transfer->t.fine.size()); }
+
+int gh_transfer_export(const gh_transfer* transfer, int64_t* coarse_indices, int64_t* fine_indices,
+                       double* coefficients) {
+  return guarded([&] {
+    const gh::Transfer& t = transfer->t;
+    const size_t n = t.fine.size();
+    if (coarse_indices) std::memcpy(coarse_indices, t.coarse.data(), n * sizeof(int64_t));
+    if (fine_indices) std::memcpy(fine_indices, t.fine.data(), n * sizeof(int64_t));
+    if (coefficients) std::memcpy(coefficients, t.coefficient.data(), n * sizeof(double));
   });
 }
 

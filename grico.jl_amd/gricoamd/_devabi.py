@@ -50,6 +50,30 @@ class SolveInfo(ctypes.Structure):
     ]
 
 
+class TransferDesc(ctypes.Structure):
+    _fields_ = [("nnz", c_int64), ("coarse_indices", P_i64), ("fine_indices", P_i64), ("coefficients", P_f64)]
+
+
+class MgDesc(ctypes.Structure):
+    _fields_ = [
+        ("level_count", c_int32), ("levels", ctypes.POINTER(c_void_p)), ("transfers", ctypes.POINTER(TransferDesc)),
+        ("smoother", c_int32), ("pre_steps", c_int32), ("post_steps", c_int32), ("damping", c_double),
+        ("chebyshev_degree", c_int32), ("chebyshev_range", c_double), ("eigen_iterations", c_int32),
+        ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
+        ("coarse_maxiter", c_int64),
+    ]
+
+
+def declare_mg(lib):
+    if getattr(lib, "_mg_declared", False):
+        return
+    lib.gmf_precond_create_mg.argtypes = [ctypes.POINTER(MgDesc), ctypes.POINTER(c_void_p)]
+    lib.gmf_precond_create_mg.restype = ctypes.c_int
+    lib.gmf_mg_level_info.argtypes = [c_void_p, c_int32, P_i64, P_f64]
+    lib.gmf_mg_level_info.restype = ctypes.c_int
+    lib._mg_declared = True
+
+
 def declare(lib):
     lib.gmf_last_error.restype = ctypes.c_char_p
     lib.gmf_device_info.argtypes = [ctypes.c_char_p, P_i32, P_i64]

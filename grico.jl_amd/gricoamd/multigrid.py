@@ -1,0 +1,407 @@
+"""hp geometric multigrid: hierarchy, reduced transfers and the device V-cycle.
+
+Reference: src/multigrid.jl. The policy mirrors GeometricMultigridPreconditioner
+(51-96); the hierarchy first halves the degrees on the fine mesh
+(_hp_multigrid_spaces 443-469, _coarsened_degree 473-499) and then merges one
+binary refinement frontier per level (_h_coarsened_space 514-561,
+_multigrid_h_coarsening_candidates 568-586, parent degrees = min of the
+children 563-566). Every level is the problem rediscretized on its space
+(_problem_on_fields 596-612). Transfers are built by the host library
+(gh_transfer_create, src/multigrid.jl:614-857) and the V-cycle runs on the
+device (gmf_precond_create_mg, src/multigrid.jl:858-1087).
+
+Smoothers: "jacobi" is the reference's damped Jacobi V(pre, post) (943-956);
+"chebyshev" is a Chebyshev polynomial in D^-1 A on [lambda_max / range,
+lambda_max] with lambda_max estimated by Lanczos (no reference counterpart).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native
+from .plan import AssemblyPlan, compile as compile_plan
+from .problem import SPD, AffineProblem, Dirichlet, ScalarField
+from .space import HpSpace, SpaceOptions, StoredDegrees
+
+_DEFAULT_COARSE_DIRECT_DOF_LIMIT = 512
+
+SMOOTHERS = {"jacobi": 0, "chebyshev": 1}
+
+
+def _positive_int(v, name):
+    if int(v) != v or int(v) < 1:
+        raise ValueError(f"{name} must be a positive integer")
+    return int(v)
+
+
+def _nonnegative_int(v, name):
+    if int(v) != v or int(v) < 0:
+        raise ValueError(f"{name} must be a non-negative integer")
+    return int(v)
+
+
+def _tolerance(v, name):
+    v = float(v)
+    if not (math.isfinite(v) and v >= 0.0):
+        raise ValueError(f"{name} must be finite and non-negative")
+    return v
+
+
+class GeometricMultigridPreconditioner:
+    """hp geometric multigrid policy (src/multigrid.jl:51-96)."""
+
+    def __init__(self, min_degree=1, max_levels=16, p_sequence="bisect", pre_smoothing_steps=None,
+                 post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
+                 coarse_relative_tolerance=1.0e-12, coarse_absolute_tolerance=0.0, coarse_maxiter=10_000,
+                 coarse_direct_dof_limit=_DEFAULT_COARSE_DIRECT_DOF_LIMIT, chebyshev_degree=4,
+                 chebyshev_range=20.0, eigen_iterations=12):
+        self.min_degree = _nonnegative_int(min_degree, "min_degree")
+        self.max_levels = _positive_int(max_levels, "max_levels")
+        if p_sequence not in ("bisect", "decrease_by_one", "go_to_one"):
+            raise ValueError("p_sequence must be 'bisect', 'decrease_by_one', or 'go_to_one'")
+        self.p_sequence = p_sequence
+        if smoother not in ("auto", "jacobi", "gmres", "chebyshev"):
+            raise ValueError("smoother must be 'auto', 'jacobi', 'gmres', or 'chebyshev'")
+        self.smoother = smoother
+        default_steps = 1 if smoother == "chebyshev" else 2
+        self.pre_smoothing_steps = _nonnegative_int(
+            default_steps if pre_smoothing_steps is None else pre_smoothing_steps, "pre_smoothing_steps")
+        self.post_smoothing_steps = _nonnegative_int(
+            default_steps if post_smoothing_steps is None else post_smoothing_steps, "post_smoothing_steps")
+        damping = float(smoother_damping)
+        if not (math.isfinite(damping) and damping > 0.0):
+            raise ValueError("smoother_damping must be finite and positive")
+        self.smoother_damping = damping
+        self.smoother_restart = _positive_int(smoother_restart, "smoother_restart")
+        self.coarse_relative_tolerance = _tolerance(coarse_relative_tolerance, "coarse_relative_tolerance")
+        self.coarse_absolute_tolerance = _tolerance(coarse_absolute_tolerance, "coarse_absolute_tolerance")
+        self.coarse_maxiter = _positive_int(coarse_maxiter, "coarse_maxiter")
+        self.coarse_direct_dof_limit = _nonnegative_int(coarse_direct_dof_limit, "coarse_direct_dof_limit")
+        self.chebyshev_degree = _positive_int(chebyshev_degree, "chebyshev_degree")
+        if not (math.isfinite(float(chebyshev_range)) and float(chebyshev_range) > 1.0):
+            raise ValueError("chebyshev_range must be greater than one")
+        self.chebyshev_range = float(chebyshev_range)
+        self.eigen_iterations = _positive_int(eigen_iterations, "eigen_iterations")
+
+    def resolved_smoother(self, operator_class):
+        if self.smoother == "auto":
+            return "jacobi" if isinstance(operator_class, SPD) else "gmres"
+        return self.smoother
+
+
+# ---- hierarchy of spaces ----------------------------------------------------------
+
+def _coarsened_degree(degree, lower, sequence):
+    if degree <= lower:
+        return degree
+    if sequence == "bisect":
+        return max(degree // 2, lower)
+    if sequence == "decrease_by_one":
+        return max(degree - 1, lower)
+    return lower
+
+
+def coarsened_degrees(degrees, continuity, min_degree, sequence):
+    """Per-leaf degree tuples one p-level down (src/multigrid.jl:473-499)."""
+    out = np.array(degrees, dtype=np.int32, copy=True)
+    for a, kind in enumerate(continuity):
+        floor = max(min_degree, 1 if kind == "cg" else 0)
+        for i in range(out.shape[0]):
+            d = int(out[i, a])
+            out[i, a] = _coarsened_degree(d, min(d, floor), sequence)
+    return out
+
+
+def _space_with(space, leaves, degrees):
+    opts = space.options
+    options = SpaceOptions(basis=opts.basis, degree=StoredDegrees(leaves, degrees), quadrature=opts.quadrature,
+                           continuity=opts.continuity)
+    return HpSpace(space.domain, options, leaves=leaves, degrees=degrees)
+
+
+def h_coarsening_candidates(space):
+    """(parents, first children) of binary splits whose two children are both
+    active leaves of `space` (src/multigrid.jl:568-586)."""
+    grid = space.domain.grid
+    first_child = np.asarray(grid.first_child())
+    active = np.zeros(len(first_child), dtype=bool)
+    active[space.leaves] = True
+    cells = np.nonzero(first_child >= 0)[0]
+    fc = first_child[cells]
+    ok = active[fc] & active[fc + 1]
+    return cells[ok], fc[ok]
+
+
+def h_coarsened_space(space):
+    parents, firsts = h_coarsening_candidates(space)
+    if parents.size == 0:
+        return None
+    index = space.leaf_index
+    deg = space.degrees
+    child0 = np.fromiter((index[int(c)] for c in firsts), dtype=np.int64, count=len(firsts))
+    child1 = np.fromiter((index[int(c) + 1] for c in firsts), dtype=np.int64, count=len(firsts))
+    parent_degrees = np.minimum(deg[child0], deg[child1])
+    drop = np.zeros(len(space.leaves), dtype=bool)
+    drop[child0] = True
+    drop[child1] = True
+    leaves = np.concatenate([space.leaves[~drop], parents.astype(np.int64)])
+    degrees = np.concatenate([deg[~drop], parent_degrees]).astype(np.int32)
+    return _space_with(space, leaves, degrees)
+
+
+def multigrid_spaces(fine_space, policy):
+    """Coarse-to-fine list of level spaces (src/multigrid.jl:443-469)."""
+    spaces = [fine_space]
+    current = fine_space
+    degrees = np.asarray(fine_space.degrees, dtype=np.int32)
+    while len(spaces) < policy.max_levels:
+        nxt = coarsened_degrees(degrees, current.continuity, policy.min_degree, policy.p_sequence)
+        if np.array_equal(nxt, degrees):
+            break
+        current = _space_with(current, current.leaves, nxt)
+        spaces.append(current)
+        degrees = nxt
+    while len(spaces) < policy.max_levels:
+        nxt = h_coarsened_space(current)
+        if nxt is None:
+            break
+        current = nxt
+        spaces.append(current)
+    spaces.reverse()
+    return spaces
+
+
+def has_coarsening_level(space, policy):
+    if policy.max_levels <= 1:
+        return False
+    degrees = np.asarray(space.degrees, dtype=np.int32)
+    if not np.array_equal(coarsened_degrees(degrees, space.continuity, policy.min_degree, policy.p_sequence),
+                          degrees):
+        return True
+    return h_coarsening_candidates(space)[0].size > 0
+
+
+def problem_on_space(problem, space):
+    """The same operators and constraints on another space (src/multigrid.jl:596-612)."""
+    old = problem.field
+    field = ScalarField(space, old.name)
+    out = AffineProblem(field, problem.operator_class)
+    for (_, _, op) in problem.cell_operators:
+        out.add_cell_accumulator(field, field, op)
+    for (_, op) in problem.cell_rhs:
+        out.add_cell_accumulator(field, op)
+    for con in problem.constraints:
+        out.add_constraint(Dirichlet(field, con.boundary, con.data))
+    return out
+
+
+# ---- transfers ------------------------------------------------------------------------
+
+class _SpaceView(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("leaf_count", ctypes.c_int64),
+        ("leaves", _native.P_i64), ("degrees", _native.P_i32), ("mode_offsets", _native.P_i64),
+        ("mode_box", _native.P_i32), ("term_offsets", _native.P_i64), ("term_indices", _native.P_i64),
+        ("term_coefficients", _native.P_f64), ("full_dof_count", ctypes.c_int64),
+        ("map_offsets", _native.P_i64), ("map_indices", _native.P_i64), ("map_coefficients", _native.P_f64),
+        ("reduced_index", _native.P_i64),
+    ]
+
+
+def _declare_transfer(lib):
+    if getattr(lib, "_transfer_declared", False):
+        return
+    lib.gh_transfer_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(_SpaceView), ctypes.POINTER(_SpaceView),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+    lib.gh_transfer_destroy.argtypes = [ctypes.c_void_p]
+    lib.gh_transfer_nnz.argtypes = [ctypes.c_void_p]
+    lib.gh_transfer_nnz.restype = ctypes.c_int64
+    lib.gh_transfer_export.argtypes = [ctypes.c_void_p, _native.P_i64, _native.P_i64, _native.P_f64]
+    lib._transfer_declared = True
+
+
+def _view(plan, keep):
+    sp = plan.space
+    arrays = [np.ascontiguousarray(a) for a in (
+        sp.leaves.astype(np.int64), sp.degrees.astype(np.int32), sp.mode_offsets.astype(np.int64),
+        sp.mode_box.astype(np.int32), sp.term_offsets.astype(np.int64), sp.term_indices.astype(np.int64),
+        sp.term_coefficients.astype(np.float64), plan.map_offsets.astype(np.int64),
+        plan.map_indices.astype(np.int64), plan.map_coefficients.astype(np.float64),
+        plan.reduced_index.astype(np.int64))]
+    keep.extend(arrays)
+    v = _SpaceView()
+    v.dim = sp.dim
+    v.leaf_count = len(sp.leaves)
+    P = _native.ptr
+    v.leaves = P(arrays[0], _native.c_int64)
+    v.degrees = P(arrays[1], _native.c_int32)
+    v.mode_offsets = P(arrays[2], _native.c_int64)
+    v.mode_box = P(arrays[3], _native.c_int32)
+    v.term_offsets = P(arrays[4], _native.c_int64)
+    v.term_indices = P(arrays[5], _native.c_int64)
+    v.term_coefficients = P(arrays[6], _native.c_double)
+    v.full_dof_count = plan.full_dof_count
+    v.map_offsets = P(arrays[7], _native.c_int64)
+    v.map_indices = P(arrays[8], _native.c_int64)
+    v.map_coefficients = P(arrays[9], _native.c_double)
+    v.reduced_index = P(arrays[10], _native.c_int64)
+    return v
+
+
+class ReducedTransfer:
+    """Prolongation from `coarse_plan` to `fine_plan` as COO entries sorted by
+    (fine, coarse) (src/multigrid.jl:115-121)."""
+
+    def __init__(self, coarse_plan, fine_plan):
+        lib = _native.host()
+        _declare_transfer(lib)
+        keep = []
+        cv, fv = _view(coarse_plan, keep), _view(fine_plan, keep)
+        handle = ctypes.c_void_p()
+        _native.check_host(lib.gh_transfer_create(coarse_plan.space.domain.grid.handle, ctypes.byref(cv),
+                                                  ctypes.byref(fv), ctypes.byref(handle)))
+        try:
+            nnz = lib.gh_transfer_nnz(handle)
+            self.coarse_indices = np.zeros(nnz, dtype=np.int64)
+            self.fine_indices = np.zeros(nnz, dtype=np.int64)
+            self.coefficients = np.zeros(nnz, dtype=np.float64)
+            _native.check_host(lib.gh_transfer_export(handle, _native.ptr(self.coarse_indices, _native.c_int64),
+                                                      _native.ptr(self.fine_indices, _native.c_int64),
+                                                      _native.ptr(self.coefficients, _native.c_double)))
+        finally:
+            lib.gh_transfer_destroy(handle)
+        self.coarse_plan = coarse_plan
+        self.fine_plan = fine_plan
+
+    @property
+    def nnz(self):
+        return len(self.coefficients)
+
+    def prolongate_add(self, fine, coarse):
+        """fine += P coarse on host arrays (src/multigrid.jl:1060-1072)."""
+        np.add.at(fine, self.fine_indices, self.coefficients * coarse[self.coarse_indices])
+        return fine
+
+    def restrict(self, fine):
+        """P^T fine on host arrays (src/multigrid.jl:1074-1087)."""
+        out = np.zeros(self.coarse_plan.reduced_dof_count)
+        np.add.at(out, self.coarse_indices, self.coefficients * fine[self.fine_indices])
+        return out
+
+    def matrix(self):
+        P = np.zeros((self.fine_plan.reduced_dof_count, self.coarse_plan.reduced_dof_count))
+        P[self.fine_indices, self.coarse_indices] = self.coefficients
+        return P
+
+
+# ---- compiled hierarchy ------------------------------------------------------------
+
+class MultigridHierarchy:
+    """Level plans (coarse to fine), transfers and, once requested, the device
+    V-cycle (src/multigrid.jl:309-345)."""
+
+    def __init__(self, problem, policy, fine_plan=None):
+        if not isinstance(problem, AffineProblem):
+            raise TypeError("GeometricMultigridPreconditioner requires the original AffineProblem so every level "
+                            "can rediscretize the operators")
+        fine_space = problem.field.space
+        if not has_coarsening_level(fine_space, policy):
+            raise ValueError("GeometricMultigridPreconditioner requires at least one admissible h or p coarsening "
+                             "level; use an explicit non-GMG preconditioner for this problem")
+        self.policy = policy
+        self.problem = problem
+        self.smoother = policy.resolved_smoother(problem.operator_class)
+        if self.smoother == "gmres":
+            raise NotImplementedError("the GMRES multigrid smoother (nonsymmetric operators) is not implemented "
+                                      "on the device; use smoother='jacobi' or 'chebyshev'")
+        self.spaces = multigrid_spaces(fine_space, policy)
+        plans = []
+        for i, sp in enumerate(self.spaces):
+            if i == len(self.spaces) - 1 and fine_plan is not None:
+                plans.append(fine_plan)
+            elif i == len(self.spaces) - 1:
+                plans.append(compile_plan(problem))
+            else:
+                plans.append(compile_plan(problem_on_space(problem, sp)))
+        self.plans = plans
+        self.transfers = [ReducedTransfer(plans[i], plans[i + 1]) for i in range(len(plans) - 1)]
+        self._device = None
+
+    @property
+    def level_count(self):
+        return len(self.plans)
+
+    def reduced_dof_counts(self):
+        return [p.reduced_dof_count for p in self.plans]
+
+    def coarse_solver_kind(self):
+        n = self.plans[0].reduced_dof_count
+        return "dense_cholesky" if 0 < n <= self.policy.coarse_direct_dof_limit else "krylov"
+
+    def device_preconditioner(self):
+        """Compile the V-cycle on the device (gmf_precond_create_mg)."""
+        if self._device is None:
+            from . import _devabi, ops
+            from .solve import DevicePreconditioner
+            lib = _native.device()
+            _devabi.declare_mg(lib)
+            wss = [ops._ws(p).ws for p in self.plans]
+            levels = (ctypes.c_void_p * len(wss))(*[w.value for w in wss])
+            tds = (_devabi.TransferDesc * len(self.transfers))()
+            keep = [levels, tds]
+            for i, t in enumerate(self.transfers):
+                ci, fi, cf = (np.ascontiguousarray(t.coarse_indices), np.ascontiguousarray(t.fine_indices),
+                              np.ascontiguousarray(t.coefficients))
+                keep += [ci, fi, cf]
+                tds[i].nnz = t.nnz
+                tds[i].coarse_indices = _native.ptr(ci, _native.c_int64)
+                tds[i].fine_indices = _native.ptr(fi, _native.c_int64)
+                tds[i].coefficients = _native.ptr(cf, _native.c_double)
+            pol = self.policy
+            d = _devabi.MgDesc()
+            d.level_count = len(wss)
+            d.levels = levels
+            d.transfers = tds
+            d.smoother = SMOOTHERS[self.smoother]
+            d.pre_steps = pol.pre_smoothing_steps
+            d.post_steps = pol.post_smoothing_steps
+            d.damping = pol.smoother_damping
+            d.chebyshev_degree = pol.chebyshev_degree
+            d.chebyshev_range = pol.chebyshev_range
+            d.eigen_iterations = pol.eigen_iterations
+            d.coarse_direct_limit = pol.coarse_direct_dof_limit
+            d.coarse_rtol = pol.coarse_relative_tolerance
+            d.coarse_atol = pol.coarse_absolute_tolerance
+            d.coarse_maxiter = pol.coarse_maxiter
+            handle = ctypes.c_void_p()
+            _devabi.check(lib.gmf_precond_create_mg(ctypes.byref(d), ctypes.byref(handle)))
+            self._device = DevicePreconditioner(self.plans[-1], handle, keep=(keep, self))
+        return self._device
+
+    def level_lambda_max(self):
+        from . import _devabi
+        lib = _native.device()
+        M = self.device_preconditioner()
+        out = []
+        for lvl in range(self.level_count):
+            n, lam = ctypes.c_int64(), ctypes.c_double()
+            _devabi.check(lib.gmf_mg_level_info(M.handle, lvl, ctypes.byref(n), ctypes.byref(lam)))
+            out.append(lam.value)
+        return out
+
+
+def compile_geometric_multigrid(problem, policy=None, fine_plan=None):
+    return MultigridHierarchy(problem, policy or GeometricMultigridPreconditioner(), fine_plan)
+
+
+def check_cg_policy(problem, policy):
+    """src/multigrid.jl:247-257: CG needs an SPD problem and a symmetric V-cycle."""
+    if not isinstance(problem.operator_class, SPD):
+        raise ValueError("CG with GeometricMultigridPreconditioner requires an SPD affine problem; use FGMRESSolver")
+    if policy.pre_smoothing_steps != policy.post_smoothing_steps:
+        raise ValueError("CG with GeometricMultigridPreconditioner requires equal pre- and post-smoothing steps so "
+                         "the V-cycle remains symmetric")
+    if policy.resolved_smoother(problem.operator_class) not in ("jacobi", "chebyshev"):
+        raise ValueError("CG with GeometricMultigridPreconditioner requires a symmetric smoother")

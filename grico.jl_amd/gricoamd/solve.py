@@ -142,11 +142,43 @@ def solve_reduced(plan, solver, b_red, relative_tolerance=None, absolute_toleran
     return x, info
 
 
+def _default_affine_preconditioner(problem):
+    """GMG for SPD problems with a coarsening level, else Jacobi (src/multigrid.jl:259-266)."""
+    from .multigrid import GeometricMultigridPreconditioner, has_coarsening_level
+    from .problem import SPD
+    candidate = GeometricMultigridPreconditioner()
+    if isinstance(problem.operator_class, SPD) and has_coarsening_level(problem.field.space, candidate):
+        return candidate
+    return JacobiPreconditioner()
+
+
 def solve(problem_or_plan, solver=None, relative_tolerance=None, absolute_tolerance=0.0, maxiter=None,
           initial_solution=None):
-    """solve(problem; solver=...) / solve(plan; solver=...) -> State."""
-    plan = problem_or_plan if isinstance(problem_or_plan, AssemblyPlan) else compile_plan(problem_or_plan)
-    solver = _default_solver(plan, solver)
+    """solve(problem; solver=...) / solve(plan; solver=...) -> State.
+
+    On a problem the default (AutoLinearSolver) is CG with the hp geometric
+    multigrid preconditioner when the problem is SPD and has a coarsening level,
+    else CG with Jacobi (src/multigrid.jl:199-266); on a compiled plan it is CG
+    with Jacobi (src/solve.jl:219-243)."""
+    from .multigrid import GeometricMultigridPreconditioner, check_cg_policy, compile_geometric_multigrid
+    is_plan = isinstance(problem_or_plan, AssemblyPlan)
+    if not is_plan and (solver is None or isinstance(solver, AutoLinearSolver)):
+        solver = CGSolver(preconditioner=_default_affine_preconditioner(problem_or_plan))
+    if isinstance(getattr(solver, "preconditioner", None), GeometricMultigridPreconditioner):
+        if is_plan:
+            raise ValueError("GeometricMultigridPreconditioner requires the original AffineProblem so every level "
+                             "can rediscretize the operators; call solve(problem, solver=CGSolver("
+                             "preconditioner=GeometricMultigridPreconditioner()))")
+        if isinstance(solver, CGSolver):
+            check_cg_policy(problem_or_plan, solver.preconditioner)
+        plan = compile_plan(problem_or_plan)
+        hierarchy = compile_geometric_multigrid(problem_or_plan, solver.preconditioner, fine_plan=plan)
+        M = hierarchy.device_preconditioner()
+        solver = CGSolver(preconditioner=M) if isinstance(solver, CGSolver) else \
+            FGMRESSolver(preconditioner=M, restart=solver.restart)
+    else:
+        plan = problem_or_plan if is_plan else compile_plan(problem_or_plan)
+        solver = _default_solver(plan, solver)
     b = plan.reduced_rhs()
     x, info = solve_reduced(plan, solver, b, relative_tolerance, absolute_tolerance, maxiter, initial_solution)
     full = ops.expand_reduced(plan, x, include_shift=True).cpu().numpy()

@@ -17,6 +17,7 @@
  *   gh_gauss_legendre    src/quadrature.jl:342-365 (gauss_legendre_rule)
  *   gh_fe_basis          src/polynomials.jl:238-247, 403-434
  *   gh_restriction       src/continuity.jl:1158-1198 (_affine_restriction_matrix)
+ *   gh_transfer_*        src/multigrid.jl:614-857 (_compile_multigrid_transfer)
  *
  * Conventions: all indices are 0-based; axes are 0-based; side 0 = lower,
  * side 1 = upper. Functions return 0 on success and a negative status on
@@ -84,6 +85,40 @@ int gh_space_export(const gh_space* space, int64_t* leaves, int32_t* degrees,
 int gh_space_faces(const gh_space* space, int64_t* boundary_leaf, int32_t* boundary_axis,
                    int32_t* boundary_side, int64_t* interface_minus, int64_t* interface_plus,
                    int32_t* interface_axis);
+
+/* Borrowed view of a compiled space (arrays as returned by gh_space_export)
+ * plus the reduced map of a plan on it: full -> reduced CSR and, for each full
+ * dof, its reduced index or -1 when it is not a solve dof. */
+typedef struct gh_space_view {
+  int32_t dim;
+  int64_t leaf_count;
+  const int64_t* leaves;
+  const int32_t* degrees;
+  const int64_t* mode_offsets;
+  const int32_t* mode_box;
+  const int64_t* term_offsets;
+  const int64_t* term_indices;
+  const double* term_coefficients;
+  int64_t full_dof_count;
+  const int64_t* map_offsets;
+  const int64_t* map_indices;
+  const double* map_coefficients;
+  const int64_t* reduced_index;
+} gh_space_view;
+
+typedef struct gh_transfer gh_transfer;
+
+/* Reduced-space prolongation from `coarse` to `fine` (nested spaces of one
+ * grid: same leaves with degrees >= the coarse ones, or the two children of a
+ * coarse leaf). Entries are sorted by (fine, coarse); the V-cycle prolongs
+ * with them and restricts with their transpose (src/multigrid.jl:614-857,
+ * 1060-1087). */
+int gh_transfer_create(const gh_grid* grid, const gh_space_view* coarse, const gh_space_view* fine,
+                       gh_transfer** out);
+void gh_transfer_destroy(gh_transfer* transfer);
+int64_t gh_transfer_nnz(const gh_transfer* transfer);
+int gh_transfer_export(const gh_transfer* transfer, int64_t* coarse_indices, int64_t* fine_indices,
+                       double* coefficients);
 
 #ifdef __cplusplus
 }

@@ -188,6 +188,56 @@ gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* precond, cons
                             int32_t use_initial, int32_t restart, double rtol, double atol,
                             int64_t maxiter, gmf_solve_info* info);
 
+/* ---- hp geometric multigrid (src/multigrid.jl) --------------------------- */
+
+enum {
+  GMF_SMOOTHER_JACOBI = 0,   /* damped Jacobi sweeps, the reference's SPD smoother (src/multigrid.jl:943-956) */
+  GMF_SMOOTHER_CHEBYSHEV = 1 /* Chebyshev polynomial in D^-1 A on [lambda_max / range, lambda_max]; no reference counterpart */
+};
+
+/* Reduced-space prolongation from level i to level i+1 as COO entries sorted
+ * by (fine, coarse) (src/multigrid.jl:115-121 _ReducedTransfer); HOST arrays,
+ * copied. Restriction is the transpose (src/multigrid.jl:1060-1087). */
+typedef struct gmf_transfer_desc {
+  int64_t nnz;
+  const int64_t* coarse_indices;
+  const int64_t* fine_indices;
+  const double* coefficients;
+} gmf_transfer_desc;
+
+/* V-cycle hierarchy (src/multigrid.jl:51-96 GeometricMultigridPreconditioner,
+ * 309-345 _compile_geometric_multigrid). levels[0] is the coarsest level and
+ * levels[level_count-1] the workspace of the operator being preconditioned; the
+ * preconditioner belongs to that workspace. Each level's plan is the problem
+ * rediscretized on that level's space. */
+typedef struct gmf_mg_desc {
+  int32_t level_count; /* >= 2 */
+  gmf_workspace* const* levels;
+  const gmf_transfer_desc* transfers; /* level_count - 1; transfers[i]: level i -> i+1 */
+  int32_t smoother;                   /* GMF_SMOOTHER_* */
+  int32_t pre_steps;                  /* Jacobi sweeps / Chebyshev applications before restriction */
+  int32_t post_steps;
+  double damping;                     /* Jacobi omega (reference default 0.8) */
+  int32_t chebyshev_degree;           /* operator applications per Chebyshev application */
+  double chebyshev_range;             /* lambda_min = lambda_max / range */
+  int32_t eigen_iterations;           /* Lanczos steps estimating lambda_max(D^-1 A) per level */
+  int64_t coarse_direct_limit;        /* dense SPD solve when n_coarse <= limit (reference default 512) */
+  double coarse_rtol;                 /* else Jacobi-CG to these tolerances (src/multigrid.jl:891-924) */
+  double coarse_atol;
+  int64_t coarse_maxiter;
+} gmf_mg_desc;
+
+/* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR
+ * tables and the coarse solver (dense Cholesky-based inverse of the probed
+ * coarse matrix, rejected with GMF_ENOTSPD if not symmetric positive definite,
+ * src/multigrid.jl:382-408). z = M^-1 r runs one V-cycle from z = 0
+ * (src/multigrid.jl:858-889). */
+gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** out);
+
+/* Estimated largest eigenvalue of D^-1 A used by level `level`'s Chebyshev
+ * smoother (0 for the coarse level or the Jacobi smoother). Diagnostics. */
+gmf_status gmf_mg_level_info(const gmf_preconditioner* mg, int32_t level, int64_t* reduced_dofs, double* lambda_max);
+
 /* Timed repetitions of gmf_apply on the workspace stream with HIP events
  * around each stage: average milliseconds of the cell-kernel pass and of the
  * whole apply (the measurement role of src/diagnostics.jl:21-59

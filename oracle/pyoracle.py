@@ -271,3 +271,68 @@ def cg(plan, b_red, inv_diag=None, rtol=1e-10, atol=0.0, maxiter=10000, lowered=
     it = L.oracle_cg(cb, None, n, _p(inv, ctypes.c_double), _p(b, ctypes.c_double), _p(x, ctypes.c_double),
                      float(rtol), float(atol), int(maxiter))
     return x, int(it)
+
+
+# ---- hp geometric multigrid (src/multigrid.jl:858-1087), dense, small levels ----
+
+def reduced_matrix(plan):
+    """Dense P^T A P of a small plan from oracle applies of unit vectors."""
+    n = plan.reduced_dof_count
+    A = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = reduced_apply(plan, e)
+    return A
+
+
+def _chebyshev(A, dinv, x, b, x_zero, lam_max, degree, rng):
+    lam_min = lam_max / rng
+    theta, delta = 0.5 * (lam_max + lam_min), 0.5 * (lam_max - lam_min)
+    sigma = theta / delta
+    r = b.copy() if x_zero else b - A @ x
+    d = dinv * r / theta
+    x = d.copy() if x_zero else x + d
+    rho = 1.0 / sigma
+    for _ in range(1, degree):
+        r = r - A @ d
+        rho_new = 1.0 / (2.0 * sigma - rho)
+        d = rho_new * rho * d + (2.0 * rho_new / delta) * (dinv * r)
+        x = x + d
+        rho = rho_new
+    return x
+
+
+def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, lambdas=None,
+           cheb_degree=4, cheb_range=20.0):
+    """z = M^-1 r for the V-cycle: damped Jacobi sweeps (src/multigrid.jl:943-956)
+    or Chebyshev applications, residual restriction with P^T, recursive coarse
+    correction, exact dense coarse solve (891-899), prolongation (1060-1072).
+    mats[l]: dense reduced operator of level l (0 = coarsest); prolongations[l]:
+    dense (n_{l+1} x n_l) matrix."""
+    dinvs = [1.0 / np.diag(A) if A.size else np.zeros(0) for A in mats]
+
+    def smooth(l, x, b, steps, x_zero):
+        A, dinv = mats[l], dinvs[l]
+        if A.size == 0:
+            return x
+        for k in range(steps):
+            first = x_zero and k == 0
+            if smoother == "jacobi":
+                res = b if first else b - A @ x
+                x = (0.0 if first else x) + omega * dinv * res
+            else:
+                x = _chebyshev(A, dinv, x, b, first, lambdas[l], cheb_degree, cheb_range)
+        return x
+
+    def cycle(l, b):
+        if l == 0:
+            return np.linalg.solve(mats[0], b) if b.size else b.copy()
+        x = np.zeros_like(b)
+        x = smooth(l, x, b, pre, True)
+        res = b - mats[l] @ x
+        xc = cycle(l - 1, prolongations[l - 1].T @ res)
+        x = x + prolongations[l - 1] @ xc
+        return smooth(l, x, b, post, False)
+
+    return cycle(len(mats) - 1, np.asarray(r, dtype=np.float64))

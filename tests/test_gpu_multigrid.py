@@ -1,0 +1,150 @@
+"""Device hp-multigrid V-cycle against the oracle's dense V-cycle, and GMG-CG
+solves against Jacobi-CG and dense solves (test/multigrid.jl:91-101, 129-149)."""
+
+import numpy as np
+import pytest
+
+import gricoamd as g
+import pyoracle
+from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "p_2d": lambda: P.poisson(2, 3, 4, rhs=False),
+    "h_2d": lambda: P.poisson(2, None, 1, refine_depth=2, rhs=False),
+    "hp_3d": lambda: P.poisson(3, None, 2, refine_depth=1, kappa=P.kappa_c2, rhs=False),
+    "hp_3d_p4": lambda: P.poisson(3, None, 4, refine_depth=1, kappa=P.kappa_c2, rhs=False),
+    "corner": lambda: P.corner_refined(2, base=2, levels=1, degree=2),
+    "dg": lambda: P.poisson(2, None, 2, refine_depth=1, continuity="dg", dirichlet=False, rhs=False,
+                            operator=P.MassDiffusion(1.0, 1.0)),
+}
+
+
+def _device_vcycle(h, r):
+    M = h.device_preconditioner()
+    return M.apply(r).cpu().numpy()
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_vcycle_matches_oracle(gpu, name, smoother):
+    problem = CASES[name]()
+    pol = GeometricMultigridPreconditioner(smoother=smoother)
+    h = compile_geometric_multigrid(problem, pol)
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    lambdas = h.level_lambda_max() if smoother == "chebyshev" else None
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 11)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r, smoother=smoother, pre=pol.pre_smoothing_steps,
+                          post=pol.post_smoothing_steps, omega=pol.smoother_damping, lambdas=lambdas,
+                          cheb_degree=pol.chebyshev_degree, cheb_range=pol.chebyshev_range)
+    assert np.abs(z - ref).max() <= 1e-10 * np.abs(ref).max()
+    # a V-cycle with equal pre/post smoothing is symmetric: r1.M r2 == r2.M r1
+    r2 = P.rng_vector(len(r), 12)
+    z2 = _device_vcycle(h, r2)
+    assert float(r2 @ z) == pytest.approx(float(r @ z2), rel=1e-10)
+
+
+@pytest.mark.parametrize("name", ["p_2d", "hp_3d", "hp_3d_p4"])
+def test_chebyshev_bounds(gpu, name):
+    """1.2 x the Lanczos estimate brackets lambda_max(D^-1 A) of every smoothed level."""
+    h = compile_geometric_multigrid(CASES[name](), GeometricMultigridPreconditioner(smoother="chebyshev"))
+    lam = h.level_lambda_max()
+    for l in range(1, h.level_count):
+        A = pyoracle.reduced_matrix(h.plans[l])
+        if A.size == 0:
+            continue
+        d = np.diag(A)
+        true = np.linalg.eigvalsh(A / np.sqrt(np.outer(d, d))).max()
+        assert 0.9 * true <= lam[l] / 1.2 <= true * (1 + 1e-8)
+
+
+def test_gmg_cg_matches_jacobi_cg_mass(gpu):
+    """DG p=3 mass on two cells (test/multigrid.jl:91-101)."""
+    d = g.Domain((0.0,), (1.0,), (2,))
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(3), continuity="dg"))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+
+    class One:
+        def cell_rhs_accumulate(self, q, test_component):
+            return 1.0 + 0.0 * q.point[0]
+
+    pr.add_cell_accumulator(u, One())
+    gmg = g.solve(pr, solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner()))
+    jac = g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()),
+                  relative_tolerance=1e-12)
+    assert np.allclose(gmg.coefficients, jac.coefficients, atol=1e-8)
+    # the L2 projection of 1 is 1: coefficient 1 on the vertex modes, 0 elsewhere
+    assert g.l2_error(gmg, lambda p: 1.0 + 0.0 * p[0]) < 1e-7
+
+
+@pytest.mark.parametrize("smoother,degree", [("jacobi", 2), ("chebyshev", 4)])
+def test_gmg_cg_poisson_3d(gpu, smoother, degree):
+    """GMG-CG reaches the Jacobi-CG solution. The reference's damped Jacobi
+    smoother (omega = 0.8) is unstable where lambda_max(D^-1 A) > 2 / omega
+    (3D modal bases reach 4-5), so only the Chebyshev smoother, whose interval
+    comes from the Lanczos estimate, is held to an iteration bound."""
+    pr = P.poisson(3, None, degree, refine_depth=2, kappa=P.kappa_c2)
+    gmg = g.solve(pr, solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner(smoother=smoother)),
+                  relative_tolerance=1e-10)
+    jac = g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()),
+                  relative_tolerance=1e-12)
+    scale = np.abs(jac.coefficients).max()
+    assert np.abs(gmg.coefficients - jac.coefficients).max() <= 1e-8 * scale
+    assert gmg.info.converged == 1
+    if smoother == "chebyshev":
+        assert gmg.info.iterations < jac.info.iterations / 3
+
+
+def test_default_solver_uses_gmg(gpu):
+    """solve(problem) defaults to GMG-CG on SPD problems with a coarsening level
+    (src/multigrid.jl:199-266)."""
+
+    class Load:
+        def cell_rhs_accumulate(self, q, test_component):
+            x, y = q.point
+            return 1.0 + x + np.exp(y) * np.cos(3.0 * x)
+
+    pr = P.poisson(2, None, 2, refine_depth=3, rhs=False, kappa=lambda p: 1.0 + 0.5 * p[0])
+    pr.add_cell_accumulator(pr.field, Load())
+    st = g.solve(pr, relative_tolerance=1e-10)
+    ref = g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()),
+                  relative_tolerance=1e-12)
+    assert st.info.iterations < ref.info.iterations
+    assert np.abs(st.coefficients - ref.coefficients).max() <= 1e-8 * np.abs(ref.coefficients).max()
+
+
+def test_gmg_requires_problem(gpu):
+    pr = P.poisson(2, None, 2, refine_depth=2)
+    with pytest.raises(ValueError):
+        g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner()))
+    with pytest.raises(ValueError):
+        g.solve(pr, solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner(pre_smoothing_steps=1,
+                                                                                      post_smoothing_steps=2)))
+
+
+def test_krylov_coarse_solve(gpu):
+    """22x22 p=2 mass: the coarse level (529 dofs) is solved by Jacobi-CG."""
+    d = g.Domain((0.0, 0.0), (1.0, 1.0), (22, 22))
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(2)))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+    pr.add_cell_accumulator(u, P.SineLoad(2))
+    h = compile_geometric_multigrid(pr)
+    assert h.coarse_solver_kind() == "krylov"
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 3)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r)
+    # the coarse CG stops at 1e-12 relative: agreement to that order
+    assert np.abs(z - ref).max() <= 1e-9 * np.abs(ref).max()
+    st = g.solve(pr, solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner()))
+    assert st.info.converged == 1

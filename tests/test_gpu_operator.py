@@ -110,6 +110,17 @@ def test_reduced_diagonal_equals_probing(gpu):
     assert rel(g.reduced_diagonal(plan), probe) <= TOL
 
 
+def test_reduced_diagonal_equals_probing_all_cases(case):
+    """Exact diagonal of P^T A P on every case, hanging / p-mismatch meshes
+    included (the reference probes there, src/solve.jl:722-738)."""
+    name, plan = case
+    n = plan.reduced_dof_count
+    if n > 700:
+        pytest.skip("probing is quadratic; covered by the smaller cases")
+    probe = pyoracle.reduced_matrix(plan).diagonal()
+    assert rel(g.reduced_diagonal(plan), probe) <= TOL, name
+
+
 def test_alias_rejected(gpu):
     plan = g.compile(P.poisson(2, 2, 2))
     x = gpu.zeros(plan.full_dof_count, dtype=gpu.float64, device="cuda")

@@ -1,0 +1,149 @@
+"""hp-multigrid hierarchy and transfers on the host (CPU).
+
+Level structure goldens from the reference's own tests (test/multigrid.jl:
+98-104 two-level DG, 153-160 [2, 4, 6], 162-171 [2, 4, 8], 173-178 hanging),
+the 64^3-by-refinement hierarchy structure (p 4 -> 2 -> 1, then binary
+merges; src/multigrid.jl:443-469), transfer adjointness (test/multigrid.jl:
+49-57) and the Galerkin identity P^T A_fine P = A_coarse, which holds exactly
+because every coarse space embeds in the next finer one and the Gauss rules
+integrate both levels' products exactly: a wrong 1D embedding matrix, a lost
+hanging term or a wrong reduced index breaks it.
+"""
+
+import numpy as np
+import pytest
+
+import gricoamd as g
+import pyoracle
+from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+
+import problems as P
+
+
+def dg_mass(degree, cells, refine=None):
+    d = g.Domain((0.0,), (1.0,), cells)
+    if refine:
+        refine(d)
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(degree), continuity="dg"))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+    return pr
+
+
+def test_level_goldens():
+    def two_level_split(d):
+        first = d.refine(0, 0)
+        d.refine(first + 1, 0)
+
+    h = compile_geometric_multigrid(dg_mass(1, (1,), two_level_split))
+    assert h.reduced_dof_counts() == [2, 4, 6]
+    assert h.coarse_solver_kind() == "dense_cholesky"
+    h = compile_geometric_multigrid(dg_mass(3, (1,), lambda d: d.refine(0, 0)))
+    assert h.reduced_dof_counts() == [2, 4, 8]
+    h = compile_geometric_multigrid(dg_mass(3, (2,)))
+    assert h.level_count == 2
+
+
+def test_hanging_two_levels():
+    d = g.Domain((0.0, 0.0), (1.0, 1.0), (2, 1))
+    d.refine(0, 0)
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(1)))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+    h = compile_geometric_multigrid(pr)
+    assert h.level_count == 2
+
+
+def test_coarse_solver_threshold():
+    """22x22 p=2 mass: the p=1 coarse level has > 512 dofs -> Krylov coarse
+    solve, unless the dense limit is raised (test/multigrid.jl:119-129)."""
+    d = g.Domain((0.0, 0.0), (1.0, 1.0), (22, 22))
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(2)))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+    h = compile_geometric_multigrid(pr)
+    assert h.plans[0].reduced_dof_count > 512
+    assert h.coarse_solver_kind() == "krylov"
+    h = compile_geometric_multigrid(pr, GeometricMultigridPreconditioner(coarse_direct_dof_limit=4096))
+    assert h.coarse_solver_kind() == "dense_cholesky"
+
+
+def test_refined_cube_hierarchy_structure():
+    """Depth-3 uniform refinement of one root, p=4: three p-levels on the fine
+    mesh, then one binary merge per level down to the root cell."""
+    h = compile_geometric_multigrid(P.poisson(3, None, 4, refine_depth=3, rhs=False))
+    leaves = [len(sp.leaves) for sp in h.spaces]
+    degrees = [int(sp.degrees[0][0]) for sp in h.spaces]
+    assert leaves == [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 512, 512]
+    assert degrees == [1] * 10 + [2, 4]
+    assert h.reduced_dof_counts()[-3:] == [343, 15 ** 3, 31 ** 3]
+    # with max_levels = 16 the 64^3 construction keeps 13 merges: coarsest 4x4x2 cells
+    h16 = GeometricMultigridPreconditioner()
+    assert h16.max_levels == 16
+
+
+def test_policy_validation():
+    with pytest.raises(ValueError):
+        GeometricMultigridPreconditioner(smoother_damping=float("inf"))
+    with pytest.raises(ValueError):
+        GeometricMultigridPreconditioner(coarse_relative_tolerance=float("inf"))
+    with pytest.raises(ValueError):
+        GeometricMultigridPreconditioner(coarse_absolute_tolerance=float("nan"))
+    with pytest.raises(ValueError):
+        GeometricMultigridPreconditioner(p_sequence="halve")
+
+
+CASES = {
+    "p_2d": lambda: P.poisson(2, 3, 4, rhs=False),
+    "h_2d": lambda: P.poisson(2, None, 1, refine_depth=2, rhs=False),
+    "hp_3d": lambda: P.poisson(3, None, 2, refine_depth=1, kappa=P.kappa_c2, rhs=False),
+    "corner": lambda: P.corner_refined(2, base=2, levels=1, degree=2),
+    "dg": lambda: P.poisson(2, None, 2, refine_depth=1, continuity="dg", dirichlet=False, rhs=False,
+                            operator=P.MassDiffusion(1.0, 1.0)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_transfer_galerkin_identity(name):
+    h = compile_geometric_multigrid(CASES[name]())
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    rng = np.random.default_rng(4)
+    for l, t in enumerate(h.transfers):
+        Pm = t.matrix()
+        Ac, Af = mats[l], mats[l + 1]
+        if Ac.size == 0:
+            continue
+        G = Pm.T @ Af @ Pm
+        assert np.abs(G - Ac).max() <= 1e-11 * np.abs(Ac).max()
+        # adjointness of the host prolongate/restrict pair
+        c = np.sin(0.2 * np.arange(1, t.coarse_plan.reduced_dof_count + 1))
+        f = np.cos(0.3 * np.arange(1, t.fine_plan.reduced_dof_count + 1))
+        pf = t.prolongate_add(np.zeros_like(f), c)
+        rc = t.restrict(f)
+        assert float(pf @ f) == pytest.approx(float(c @ rc), rel=1e-13, abs=1e-13)
+        assert np.all(np.diff(t.fine_indices) >= 0)
+        x = rng.standard_normal(Pm.shape[1])
+        assert np.allclose(Pm @ x, t.prolongate_add(np.zeros(Pm.shape[0]), x), atol=1e-14)
+
+
+def test_oracle_vcycle_is_a_contraction():
+    """The oracle V-cycle (dense restatement) reduces the error of a Poisson
+    problem by a mesh-independent factor (damped Jacobi V(2,2) at p = 2:
+    about 0.62 per cycle as a stationary iteration)."""
+    rates = []
+    for depth in (2, 3):
+        h = compile_geometric_multigrid(P.poisson(2, None, 2, refine_depth=depth, rhs=False))
+        mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+        Ps = [t.matrix() for t in h.transfers]
+        A = mats[-1]
+        b = np.random.default_rng(0).standard_normal(A.shape[0])
+        x = np.zeros_like(b)
+        for _ in range(6):
+            x = x + pyoracle.vcycle(mats, Ps, b - A @ x)
+        err = np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(np.linalg.solve(A, b))
+        rates.append(err ** (1 / 6))
+    assert max(rates) < 0.7
+    assert abs(rates[0] - rates[1]) < 0.05
