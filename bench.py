@@ -42,12 +42,18 @@ def build_problem(depth, degree=4):
             k = 1.0 + 0.5 * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y) * np.sin(2 * np.pi * z)
             return g.TestChannels(0.0 * trial.value, tuple(k * c for c in trial.gradient))
 
+    class Load:
+        def cell_rhs_accumulate(self, q, test_component):
+            x, y, z = q.point
+            return 3.0 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+
     domain = g.Domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (1, 1, 1))
     g.uniform_refine(domain, depth)
     space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
     u = g.ScalarField(space, "u")
     problem = g.AffineProblem(u, g.SPD())
     problem.add_cell_accumulator(u, u, Diffusion())
+    problem.add_cell_accumulator(u, Load())
     for axis in range(3):
         for side in (g.LOWER, g.UPPER):
             problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
@@ -87,6 +93,80 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def solve_leg(problem, plan, repeats=3):
+    """GMG-CG and Jacobi-CG from x0 = 0 to ||r|| <= 1e-8 ||b|| on the device:
+    wall time of the Krylov loop with the hierarchy already compiled."""
+    import ctypes
+
+    import torch
+
+    import gricoamd as g
+    from gricoamd import _devabi, _native, ops
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+
+    lib = _native.device()
+    t0 = time.perf_counter()
+    policy = GeometricMultigridPreconditioner(smoother="chebyshev")
+    hierarchy = compile_geometric_multigrid(problem, policy, fine_plan=plan)
+    t_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    M = hierarchy.device_preconditioner()
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t0
+    log(f"GMG hierarchy: host {t_host:.1f}s, device setup {t_dev:.1f}s, levels {hierarchy.reduced_dof_counts()}")
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    x = torch.zeros_like(b)
+    dev = ops._ws(plan)
+
+    def run(handle, maxiter):
+        info = _devabi.SolveInfo()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, maxiter,
+                                       ctypes.byref(info)))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, info
+
+    run(M.handle, 1000)
+    times = []
+    for _ in range(repeats):
+        dt, info = run(M.handle, 1000)
+        times.append(dt)
+    x_mg = x.clone()
+    gmg = {"seconds": sorted(times)[len(times) // 2], "iterations": info.iterations,
+           "operator_applications": info.operator_applications,
+           "final_relative_residual": info.final_residual_norm / info.rhs_norm}
+    # one V-cycle alone
+    r = torch.randn(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(r)
+    for _ in range(2):
+        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(10):
+        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+    torch.cuda.synchronize()
+    gmg["vcycle_seconds"] = (time.perf_counter() - t) / 10
+    log(f"GMG-CG: {gmg}")
+    from gricoamd.solve import compile_preconditioner
+    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
+    dt, info = run(J.handle, 20000)
+    jac = {"seconds": dt, "iterations": info.iterations,
+           "final_relative_residual": info.final_residual_norm / info.rhs_norm}
+    diff = float(torch.linalg.norm(x - x_mg) / torch.linalg.norm(x))
+    log(f"Jacobi-CG: {jac}; relative difference of the two solutions {diff:.2e}")
+    return {
+        "metric": "time-to-1e-8 relative residual, reduced system (s)",
+        "gmg_cg": gmg,
+        "jacobi_cg": jac,
+        "solution_relative_difference": diff,
+        "levels": hierarchy.reduced_dof_counts(),
+        "smoother": f"Chebyshev-Jacobi degree {policy.chebyshev_degree}, V({policy.pre_smoothing_steps},"
+                    f"{policy.post_smoothing_steps}), FP64",
+        "load": "f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)",
+    }
+
+
 def _traffic_from_profiles(n, cells):
     """HBM bytes per cell pass measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by a 1 GiB copy), recorded in profiles/apply_traffic.json
@@ -109,6 +189,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--depth", type=int, default=6, help="uniform refinements of the root (6 -> 64^3 cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the GMG-CG / Jacobi-CG solve leg")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="after timing, copy a 1 GiB buffer three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -127,7 +208,8 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     t0 = time.perf_counter()
-    plan = g.compile(build_problem(args.depth))
+    problem = build_problem(args.depth)
+    plan = g.compile(problem)
     log(f"host compile: {time.perf_counter() - t0:.1f}s, dofs {plan.full_dof_count}, "
         f"cells {sum(b.cell_count for b in plan.buckets)}")
     t0 = time.perf_counter()
@@ -206,6 +288,8 @@ def main():
             _devabi.check(lib.gmf_vector_copy(dev.ws, dst.data_ptr(), src.data_ptr(), src.numel()))
         torch.cuda.synchronize()
         del src, dst
+    if not args.no_solve:
+        result["solve"] = solve_leg(problem, plan)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

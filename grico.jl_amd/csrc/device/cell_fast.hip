@@ -32,6 +32,7 @@
 // Both remap workgroup ids so that consecutive Morton-ordered work stays on
 // one XCD (workgroups are dealt to the 8 XCDs round-robin).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -564,7 +565,12 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   std::vector<double> G(b.h_derivs.begin(), b.h_derivs.begin() + n * n);
   b.h_colloc = collocation_derivative(B, G, n);
   b.fast = true;
-  if (b.lcoef == nullptr && n <= 6) build_bricks(p, b, lidx, red_of_full);
+  // Small batches (coarse multigrid levels) take the one-launch per-cell
+  // kernel: eight colored brick launches of a few bricks each cost more than
+  // the atomics they avoid. GMF_BRICK_MIN_CELLS overrides the threshold.
+  int64_t min_cells = 32768;
+  if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
+  if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
 }
 
 bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced,

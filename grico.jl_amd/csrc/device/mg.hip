@@ -203,10 +203,56 @@ struct MGPrecond : gmf_preconditioner {
   std::unique_ptr<CgVectors> coarse_vectors;
   double coarse_rtol = 1e-12, coarse_atol = 0.0;
   int64_t coarse_maxiter = 10000;
+  // Captured V-cycle: the graph reads top_b and writes top_x.
+  bool use_graph = false;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  DeviceBuffer<double> top_b, top_x;
+
+  ~MGPrecond() override {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+
+  void set_streams(hipStream_t s) {
+    for (Level& L : levels) L.ws->stream = s;
+  }
+
+  void capture() {
+    const int top = int(levels.size()) - 1;
+    const hipStream_t user = ws->stream;
+    if (!capture_stream) GMF_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+    set_streams(capture_stream);
+    GMF_HIP(hipStreamBeginCapture(capture_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      vcycle(top, top_x.p, top_b.p);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(capture_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      set_streams(user);
+      throw;
+    }
+    hipError_t e = hipStreamEndCapture(capture_stream, &graph);
+    set_streams(user);
+    GMF_HIP(e);
+    GMF_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  }
 
   void apply(const double* r, double* z) override {
-    for (Level& L : levels) L.ws->stream = ws->stream;
-    vcycle(int(levels.size()) - 1, z, r);
+    const int top = int(levels.size()) - 1;
+    if (!use_graph) {
+      set_streams(ws->stream);
+      vcycle(top, z, r);
+      return;
+    }
+    const int64_t n = levels[size_t(top)].n;
+    if (!graph_exec) capture();
+    copy(top_b.p, r, n, ws->stream);
+    GMF_HIP(hipGraphLaunch(graph_exec, ws->stream));
+    copy(z, top_x.p, n, ws->stream);
   }
 
   void smooth(int l, double* x, const double* b, int steps, bool x_zero) {
@@ -525,6 +571,13 @@ gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** o
       mg->coarse_jacobi->invdiag = C.invdiag.p;
       mg->coarse_jacobi->owns = false;
       mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
+    }
+    // The V-cycle has no host synchronization when the coarse solve is dense,
+    // so it can be captured once and replayed.
+    mg->use_graph = desc->graph != 0 && mg->dense;
+    if (mg->use_graph) {
+      mg->top_b = DeviceBuffer<double>(size_t(mg->levels.back().n));
+      mg->top_x = DeviceBuffer<double>(size_t(mg->levels.back().n));
     }
     sync(mg->ws);
     *out = mg.release();

@@ -65,7 +65,20 @@ Transfer build_transfer(const Grid& grid, const SpaceView& coarse, const SpaceVi
   auto matrix = [&](int sdeg, int tdeg, int delta, int64_t rel) -> const std::vector<double>& {
     auto key = std::make_tuple(sdeg, tdeg, delta, rel);
     auto it = cache.find(key);
-    if (it == cache.end()) it = cache.emplace(key, affine_restriction_matrix(sdeg, tdeg, delta, rel)).first;
+    if (it == cache.end()) {
+      // The collocation solve leaves rounding residue (~1e-16) where the
+      // embedding is exactly zero (a p-embedding of the hierarchical basis is
+      // an identity block); such residue would turn every fine mode into a
+      // sum over all coarse modes. Entries below 4 eps of the matrix scale are
+      // dropped, the tolerance the continuity rows use for cancellation.
+      std::vector<double> m = affine_restriction_matrix(sdeg, tdeg, delta, rel);
+      double scale = 1.0;
+      for (double v : m) scale = std::max(scale, std::fabs(v));
+      const double tol = 4.0 * std::numeric_limits<double>::epsilon() * scale;
+      for (double& v : m)
+        if (std::fabs(v) <= tol) v = 0.0;
+      it = cache.emplace(key, std::move(m)).first;
+    }
     return it->second;
   };
 

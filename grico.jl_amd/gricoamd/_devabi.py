@@ -60,7 +60,7 @@ class MgDesc(ctypes.Structure):
         ("smoother", c_int32), ("pre_steps", c_int32), ("post_steps", c_int32), ("damping", c_double),
         ("chebyshev_degree", c_int32), ("chebyshev_range", c_double), ("eigen_iterations", c_int32),
         ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
-        ("coarse_maxiter", c_int64),
+        ("coarse_maxiter", c_int64), ("graph", c_int32),
     ]
 
 

@@ -56,7 +56,7 @@ class GeometricMultigridPreconditioner:
                  post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
                  coarse_relative_tolerance=1.0e-12, coarse_absolute_tolerance=0.0, coarse_maxiter=10_000,
                  coarse_direct_dof_limit=_DEFAULT_COARSE_DIRECT_DOF_LIMIT, chebyshev_degree=4,
-                 chebyshev_range=20.0, eigen_iterations=12):
+                 chebyshev_range=20.0, eigen_iterations=12, graph=True):
         self.min_degree = _nonnegative_int(min_degree, "min_degree")
         self.max_levels = _positive_int(max_levels, "max_levels")
         if p_sequence not in ("bisect", "decrease_by_one", "go_to_one"):
@@ -84,6 +84,7 @@ class GeometricMultigridPreconditioner:
             raise ValueError("chebyshev_range must be greater than one")
         self.chebyshev_range = float(chebyshev_range)
         self.eigen_iterations = _positive_int(eigen_iterations, "eigen_iterations")
+        self.graph = bool(graph)
 
     def resolved_smoother(self, operator_class):
         if self.smoother == "auto":
@@ -375,6 +376,7 @@ class MultigridHierarchy:
             d.coarse_rtol = pol.coarse_relative_tolerance
             d.coarse_atol = pol.coarse_absolute_tolerance
             d.coarse_maxiter = pol.coarse_maxiter
+            d.graph = int(pol.graph)
             handle = ctypes.c_void_p()
             _devabi.check(lib.gmf_precond_create_mg(ctypes.byref(d), ctypes.byref(handle)))
             self._device = DevicePreconditioner(self.plans[-1], handle, keep=(keep, self))

@@ -225,6 +225,7 @@ typedef struct gmf_mg_desc {
   double coarse_rtol;                 /* else Jacobi-CG to these tolerances (src/multigrid.jl:891-924) */
   double coarse_atol;
   int64_t coarse_maxiter;
+  int32_t graph; /* 1: replay the V-cycle as a captured HIP graph (dense coarse solve only) */
 } gmf_mg_desc;
 
 /* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR

@@ -5,6 +5,10 @@ import sys
 import numpy as np
 import pytest
 
+# Keep the brick path under test on small meshes (the library routes batches
+# below 32768 cells to the per-cell kernel by default).
+os.environ.setdefault("GMF_BRICK_MIN_CELLS", "0")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "grico.jl_amd")
 for p in (PKG, os.path.join(ROOT, "oracle"), os.path.dirname(os.path.abspath(__file__))):

@@ -148,3 +148,19 @@ def test_krylov_coarse_solve(gpu):
     assert np.abs(z - ref).max() <= 1e-9 * np.abs(ref).max()
     st = g.solve(pr, solver=g.CGSolver(preconditioner=GeometricMultigridPreconditioner()))
     assert st.info.converged == 1
+
+
+def test_graph_replays_match_eager(gpu):
+    """The captured V-cycle replayed on several inputs, with eager device work
+    between the replays, agrees with the eager V-cycle every time."""
+    make = lambda: P.poisson(3, None, 4, refine_depth=1, kappa=P.kappa_c2, rhs=False)
+    hg = compile_geometric_multigrid(make(), GeometricMultigridPreconditioner(smoother="chebyshev", graph=True))
+    he = compile_geometric_multigrid(make(), GeometricMultigridPreconditioner(smoother="chebyshev", graph=False))
+    n = hg.plans[-1].reduced_dof_count
+    for seed in range(4):
+        r = P.rng_vector(n, 100 + seed)
+        zg = _device_vcycle(hg, r)
+        g.apply(hg.plans[-1], P.rng_vector(hg.plans[-1].full_dof_count, seed))  # eager work in between
+        ze = _device_vcycle(he, r)
+        gpu.cuda.synchronize()
+        assert np.abs(zg - ze).max() <= 1e-13 * np.abs(ze).max()

@@ -126,3 +126,13 @@ def test_alias_rejected(gpu):
     x = gpu.zeros(plan.full_dof_count, dtype=gpu.float64, device="cuda")
     with pytest.raises(ValueError):
         g.apply_into(plan, x, x)
+
+
+def test_small_batches_take_cell_kernel(gpu, monkeypatch):
+    """Below the brick threshold the fast path runs the one-launch per-cell
+    kernel; results are unchanged."""
+    monkeypatch.setenv("GMF_BRICK_MIN_CELLS", "32768")
+    plan = g.compile(P.poisson(3, 4, 4, kappa=P.kappa_c2))
+    assert "path=fast-cell" in plan.device().describe()
+    x = P.rng_vector(plan.full_dof_count, seed=21)
+    assert rel(g.apply(plan, x), pyoracle.apply(plan, x)) <= TOL
