@@ -70,6 +70,21 @@ struct DeviceBuffer {
   }
 };
 
+// Rejects pointers that are not device (or managed) memory of the current
+// device, so a host array passed by mistake fails with GMF_EINVAL instead of
+// faulting a kernel.
+inline void require_device_pointer(const void* p, const char* what) {
+  if (p == nullptr) throw Error(GMF_EINVAL, std::string(what) + ": null pointer");
+  hipPointerAttribute_t attr{};
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(GMF_EINVAL, std::string(what) + ": not a device pointer");
+  }
+  if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
+    throw Error(GMF_EINVAL, std::string(what) + ": not a device pointer");
+}
+
 void sync(gmf_workspace* ws);
 // y_red = P^T A P x_red on ws->stream (src/assembly.jl:905-915).
 void reduced_apply(gmf_workspace* ws, const double* xr, double* yr);

@@ -118,8 +118,13 @@ __global__ void k_unit(double* __restrict__ e, int64_t j, int64_t n) {
   GMF_GRID_STRIDE(i, n) e[i] = (i == j) ? 1.0 : 0.0;
 }
 
-__global__ void k_mul(double* __restrict__ out, const double* __restrict__ a, const double* __restrict__ b, int64_t n) {
+// out = a * b elementwise; out may alias b.
+__global__ void k_mul(double* out, const double* __restrict__ a, const double* b, int64_t n) {
   GMF_GRID_STRIDE(i, n) out[i] = a[i] * b[i];
+}
+
+__global__ void k_scale(double* __restrict__ x, double s, int64_t n) {
+  GMF_GRID_STRIDE(i, n) x[i] *= s;
 }
 
 __global__ void k_sqrt(double* __restrict__ out, const double* __restrict__ a, int64_t n) {
@@ -393,7 +398,7 @@ double estimate_lambda_max(Level& L, int iterations) {
   dot(ws, v.p, v.p, n, kTmp);
   read_scalars(ws, kTmp + 1);
   double nrm = std::sqrt(ws->host_scalars[kTmp]);
-  axpby(ws, v.p, 0.0, v.p, 1.0 / nrm, n);
+  hipLaunchKernelGGL(k_scale, dim3(grid1(n)), dim3(256), 0, s, v.p, 1.0 / nrm, n);
   std::vector<double> alpha, beta;
   double beta_prev = 0.0;
   for (int j = 0; j < m; ++j) {
@@ -413,7 +418,7 @@ double estimate_lambda_max(Level& L, int iterations) {
     if (j + 1 == m || !(bnext > 1e-14 * std::fabs(a))) break;
     beta.push_back(bnext);
     std::swap(vp.p, v.p);                       // vp = old v
-    axpby(ws, w.p, 0.0, w.p, 1.0 / bnext, n);   // w /= beta
+    hipLaunchKernelGGL(k_scale, dim3(grid1(n)), dim3(256), 0, s, w.p, 1.0 / bnext, n);  // w /= beta
     std::swap(v.p, w.p);                        // v = w
     beta_prev = bnext;
   }

@@ -727,6 +727,23 @@ gmf_status gmf_vector_copy(gmf_workspace* ws, double* dst, const double* src, in
   });
 }
 
+gmf_status gmf_vector_axpby(gmf_workspace* ws, double* y, double a, const double* x, double b, int64_t n) {
+  return guarded([&] {
+    require(ws && y && x && n >= 0, GMF_EINVAL, "invalid argument");
+    axpby(ws, y, a, x, b, n);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_vector_dot(gmf_workspace* ws, const double* x, const double* y, int64_t n, double* out) {
+  return guarded([&] {
+    require(ws && x && y && out && n >= 0, GMF_EINVAL, "invalid argument");
+    dot(ws, x, y, n, kTmp);
+    read_scalars(ws, kTmp + 1);
+    *out = ws->host_scalars[kTmp];
+  });
+}
+
 gmf_status gmf_invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n) {
   return guarded([&] {
     require(ws && d && inv && n >= 0, GMF_EINVAL, "null argument");

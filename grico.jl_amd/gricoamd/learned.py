@@ -1,0 +1,295 @@
+"""Learned per-cell block correction for FGMRES (no reference counterpart).
+
+A small MLP maps the normalized residual on one cell's local modes to a local
+error correction; the corrections of all cells are averaged on shared dofs and
+added after an inner preconditioner (the hp-GMG V-cycle):
+
+    z0 = V(r),  r1 = r - A z0,  z = z0 + sum_c P_c^T (w * net(r1_c / |r1_c|) |r1_c| / kappa_c)
+
+kappa_c is the cell-mean diffusion coefficient, so one network serves cells
+of different conductivity. The preconditioner varies with the data it sees, so
+it is used inside FGMRES (src/solve.jl:80-88 allows that); the seam is the
+preconditioner policy interface (src/solve.jl:32, 641-670, 753-774).
+
+Training data come from the GMG-CG solves themselves: for reference
+solutions x* and the CG iterates x_k, the pair (b - A y, x* - y) with
+y = x_k + V(b - A x_k) is the residual / error the V-cycle leaves, sampled on
+random cells. The network (BF16 weights, FP32 accumulation on MFMA) is trained
+on the device with Adam (libgrico_mf.so gmf_mlp_*).
+"""
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _devabi, _native, ops
+from .multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+
+
+def _declare(lib):
+    if getattr(lib, "_mlp_declared", False):
+        return
+    c_void_p, c_int64, c_int32, c_double, c_float = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
+                                                     ctypes.c_double, ctypes.c_float)
+    lib.gmf_mlp_create.argtypes = [ctypes.POINTER(MlpDesc), ctypes.POINTER(c_void_p)]
+    lib.gmf_mlp_destroy.argtypes = [c_void_p]
+    lib.gmf_mlp_set_stream.argtypes = [c_void_p, c_void_p]
+    lib.gmf_mlp_parameter_count.argtypes = [c_void_p, ctypes.POINTER(c_int64)]
+    lib.gmf_mlp_get_parameters.argtypes = [c_void_p, c_void_p]
+    lib.gmf_mlp_forward.argtypes = [c_void_p, c_void_p, c_void_p, c_int64]
+    lib.gmf_mlp_gradient.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_double), c_void_p]
+    lib.gmf_mlp_train.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_float, c_float, c_float,
+                                  c_float, c_void_p]
+    lib.gmf_cell_patches.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]
+    lib.gmf_precond_create_learned.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
+    lib.gmf_vector_axpby.argtypes = [c_void_p, c_void_p, c_double, c_void_p, c_double, c_int64]
+    lib.gmf_vector_dot.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_double)]
+    for name in ("gmf_mlp_create", "gmf_mlp_destroy", "gmf_mlp_set_stream", "gmf_mlp_parameter_count",
+                 "gmf_mlp_get_parameters", "gmf_mlp_forward", "gmf_mlp_gradient", "gmf_mlp_train", "gmf_cell_patches",
+                 "gmf_precond_create_learned", "gmf_vector_axpby", "gmf_vector_dot"):
+        getattr(lib, name).restype = ctypes.c_int
+    lib._mlp_declared = True
+
+
+class MlpDesc(ctypes.Structure):
+    _fields_ = [("input", ctypes.c_int32), ("hidden1", ctypes.c_int32), ("hidden2", ctypes.c_int32),
+                ("output", ctypes.c_int32), ("parameters", ctypes.POINTER(ctypes.c_float))]
+
+
+def init_parameters(widths, seed=0):
+    """He-uniform weights, zero biases, packed [W1, b1, W2, b2, W3, b3]."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for k, n in zip(widths[:-1], widths[1:]):
+        bound = math.sqrt(6.0 / k)
+        parts.append(rng.uniform(-bound, bound, size=(k, n)).astype(np.float32).reshape(-1))
+        parts.append(np.zeros(n, dtype=np.float32))
+    out = np.concatenate(parts)
+    # the last layer starts small: the untrained correction is close to zero
+    n_last = widths[-2] * widths[-1]
+    start = out.size - widths[-1] - n_last
+    out[start:start + n_last] *= 0.1
+    return out
+
+
+class CellMLP:
+    """Device MLP widths = (in, h1, h2, out) (gmf_mlp_*)."""
+
+    def __init__(self, widths=(64, 256, 256, 64), seed=0, parameters=None):
+        if len(widths) != 4:
+            raise ValueError("widths must be (input, hidden1, hidden2, output)")
+        lib = _native.device()
+        _declare(lib)
+        self.widths = tuple(int(w) for w in widths)
+        params = init_parameters(self.widths, seed) if parameters is None else \
+            np.ascontiguousarray(parameters, dtype=np.float32)
+        d = MlpDesc(*self.widths, params.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        self.handle = ctypes.c_void_p()
+        _devabi.check(lib.gmf_mlp_create(ctypes.byref(d), ctypes.byref(self.handle)))
+        self._sync_stream()
+
+    def __del__(self):
+        lib = _native._device
+        if lib is not None and getattr(self, "handle", None) is not None and self.handle.value:
+            lib.gmf_mlp_destroy(self.handle)
+            self.handle = None
+
+    def _sync_stream(self):
+        torch = ops._torch()
+        _devabi.check(_native.device().gmf_mlp_set_stream(self.handle,
+                                                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+    @property
+    def parameter_count(self):
+        n = ctypes.c_int64()
+        _devabi.check(_native.device().gmf_mlp_parameter_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def parameters(self):
+        out = np.zeros(self.parameter_count, dtype=np.float32)
+        _devabi.check(_native.device().gmf_mlp_get_parameters(self.handle, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def forward(self, x):
+        torch = ops._torch()
+        self._sync_stream()
+        x = x.to(device="cuda", dtype=torch.float32).contiguous()
+        rows = x.shape[0]
+        y = torch.empty((rows, self.widths[-1]), dtype=torch.float32, device="cuda")
+        _devabi.check(_native.device().gmf_mlp_forward(self.handle, x.data_ptr(), y.data_ptr(), rows))
+        return y
+
+    def gradient(self, x, t):
+        torch = ops._torch()
+        self._sync_stream()
+        x = x.to(device="cuda", dtype=torch.float32).contiguous()
+        t = t.to(device="cuda", dtype=torch.float32).contiguous()
+        loss = ctypes.c_double()
+        g = np.zeros(self.parameter_count, dtype=np.float32)
+        _devabi.check(_native.device().gmf_mlp_gradient(self.handle, x.data_ptr(), t.data_ptr(), x.shape[0],
+                                                        ctypes.byref(loss), g.ctypes.data_as(ctypes.c_void_p)))
+        return loss.value, g
+
+    def train(self, x, t, batch=4096, steps=2000, learning_rate=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
+              record=True):
+        torch = ops._torch()
+        self._sync_stream()
+        x = x.to(device="cuda", dtype=torch.float32).contiguous()
+        t = t.to(device="cuda", dtype=torch.float32).contiguous()
+        hist = np.zeros(steps, dtype=np.float64) if record else None
+        _devabi.check(_native.device().gmf_mlp_train(
+            self.handle, x.data_ptr(), t.data_ptr(), x.shape[0], int(batch), int(steps), float(learning_rate),
+            float(beta1), float(beta2), float(eps), hist.ctypes.data_as(ctypes.c_void_p) if record else None))
+        return hist
+
+
+def cell_weights(plan):
+    """kappa_c = sum_q W_q c_q / sum_q W_q per cell of the single diffusion batch."""
+    if len(plan.buckets) != 1 or plan.buckets[0].form != 0:
+        raise ValueError("the learned correction needs one diffusion-form cell batch")
+    b = plan.buckets[0]
+    w = np.asarray(b.weights).reshape(b.cell_count, -1)
+    return (np.asarray(b.point_data).reshape(b.cell_count, -1).sum(axis=1) / w.sum(axis=1)).astype(np.float32)
+
+
+def _ws(plan):
+    return ops._ws(plan).ws
+
+
+def _axpby(plan, y, a, x, b):
+    _devabi.check(_native.device().gmf_vector_axpby(_ws(plan), y.data_ptr(), float(a), x.data_ptr(), float(b),
+                                                    y.numel()))
+
+
+def _dot(plan, x, y):
+    out = ctypes.c_double()
+    _devabi.check(_native.device().gmf_vector_dot(_ws(plan), x.data_ptr(), y.data_ptr(), x.numel(),
+                                                  ctypes.byref(out)))
+    return out.value
+
+
+def _rapply(plan, x, y):
+    _devabi.check(_native.device().gmf_reduced_apply(_ws(plan), x.data_ptr(), y.data_ptr()))
+
+
+def collect_pairs(plan, vcycle, pairs=2000, cells_per_pair=256, iterations=None, seed=0, rtol=1e-12):
+    """(X, T) training patches from GMG-CG solves (module docstring). Returns
+    device tensors of shape (pairs * cells_per_pair, width)."""
+    torch = ops._torch()
+    lib = _native.device()
+    _declare(lib)
+    n = plan.reduced_dof_count
+    b0 = plan.buckets[0]
+    width = b0.n_modes
+    ncell = b0.cell_count
+    cw = torch.as_tensor(cell_weights(plan), device="cuda")
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    iterations = iterations or 40
+    solves = int(math.ceil(pairs / iterations))
+    X = torch.empty((pairs * cells_per_pair, width), dtype=torch.float32, device="cuda")
+    T = torch.empty_like(X)
+    f64 = dict(dtype=torch.float64, device="cuda")
+    xs, b, x, r, z, p, q, y, r1, e1 = (torch.empty(n, **f64) for _ in range(10))
+    got = 0
+    for s in range(solves):
+        if got >= pairs:
+            break
+        # reference solution: rough random field plus a smooth mode mix
+        xs.normal_(generator=gen)
+        _rapply(plan, xs, b)
+        # GMG-CG from zero, recording the V-cycle's leftover residual/error
+        x.zero_()
+        r.copy_(b)
+        z.copy_(vcycle.apply(r))
+        p.copy_(z)
+        rz = _dot(plan, r, z)
+        bnorm = math.sqrt(_dot(plan, b, b))
+        for k in range(iterations):
+            if got >= pairs:
+                break
+            # pair at the current iterate: y = x + V(r)
+            y.copy_(x)
+            _axpby(plan, y, 1.0, z, 1.0)
+            _rapply(plan, y, q)
+            r1.copy_(b)
+            _axpby(plan, r1, -1.0, q, 1.0)
+            e1.copy_(xs)
+            _axpby(plan, e1, -1.0, y, 1.0)
+            cells = torch.as_tensor(rng.choice(ncell, size=cells_per_pair, replace=cells_per_pair > ncell),
+                                    dtype=torch.int64, device="cuda")
+            rows = slice(got * cells_per_pair, (got + 1) * cells_per_pair)
+            xr, tr = X[rows], T[rows]
+            _devabi.check(lib.gmf_cell_patches(_ws(plan), r1.data_ptr(), e1.data_ptr(), cells.data_ptr(),
+                                               cells_per_pair, cw.data_ptr(), xr.data_ptr(), tr.data_ptr(), None))
+            got += 1
+            # CG step
+            _rapply(plan, p, q)
+            alpha = rz / _dot(plan, p, q)
+            _axpby(plan, x, alpha, p, 1.0)
+            _axpby(plan, r, -alpha, q, 1.0)
+            if math.sqrt(_dot(plan, r, r)) <= rtol * bnorm:
+                break
+            z.copy_(vcycle.apply(r))
+            rz_new = _dot(plan, r, z)
+            _axpby(plan, p, 1.0, z, rz_new / rz)
+            rz = rz_new
+    X, T = X[:got * cells_per_pair], T[:got * cells_per_pair]
+    perm = torch.as_tensor(rng.permutation(X.shape[0]), device="cuda")
+    return X.index_select(0, perm).contiguous(), T.index_select(0, perm).contiguous(), got
+
+
+class LearnedBlockCorrection:
+    """Preconditioner policy: the hp-GMG V-cycle followed by the learned cell
+    correction. `network` must have the cell's local mode count as width."""
+
+    def __init__(self, network, inner=None):
+        self.network = network
+        self.inner = inner if inner is not None else GeometricMultigridPreconditioner(smoother="chebyshev")
+
+    def compile(self, plan):
+        from .solve import DevicePreconditioner
+        lib = _native.device()
+        _declare(lib)
+        inner_handle, keep = None, ()
+        if self.inner is not None:
+            hierarchy = compile_geometric_multigrid(plan.problem, self.inner, fine_plan=plan)
+            M = hierarchy.device_preconditioner()
+            inner_handle, keep = M.handle, (M, hierarchy)
+        cw = cell_weights(plan)
+        h = ctypes.c_void_p()
+        _devabi.check(lib.gmf_precond_create_learned(_ws(plan), inner_handle, self.network.handle,
+                                                     cw.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h)))
+        return DevicePreconditioner(plan, h, keep=(keep, self.network, cw))
+
+
+def train_block_correction(problem, plan=None, widths=None, pairs=2000, cells_per_pair=256, steps=3000,
+                           batch=4096, learning_rate=2e-3, seed=0, inner=None, log=None):
+    """Generate GMG-CG pairs on the problem and train a CellMLP. Returns
+    (network, report)."""
+    from .plan import compile as compile_plan
+    plan = plan or compile_plan(problem)
+    inner = inner or GeometricMultigridPreconditioner(smoother="chebyshev")
+    t0 = time.perf_counter()
+    hierarchy = compile_geometric_multigrid(problem, inner, fine_plan=plan)
+    V = hierarchy.device_preconditioner()
+    width = plan.buckets[0].n_modes
+    widths = widths or (width, 256, 256, width)
+    t1 = time.perf_counter()
+    X, T, got = collect_pairs(plan, V, pairs=pairs, cells_per_pair=cells_per_pair, seed=seed)
+    ops._torch().cuda.synchronize()
+    t2 = time.perf_counter()
+    net = CellMLP(widths, seed=seed)
+    hist = net.train(X, T, batch=min(batch, X.shape[0]), steps=steps, learning_rate=learning_rate)
+    ops._torch().cuda.synchronize()
+    t3 = time.perf_counter()
+    report = {"pairs": got, "samples": int(X.shape[0]), "hierarchy_seconds": t1 - t0, "collect_seconds": t2 - t1,
+              "train_seconds": t3 - t2, "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-20:].mean()),
+              "steps": steps, "batch": int(min(batch, X.shape[0]))}
+    if log:
+        log(f"learned correction: {report}")
+    return net, report, (plan, hierarchy, V)

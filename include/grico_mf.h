@@ -239,6 +239,49 @@ gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** o
  * smoother (0 for the coarse level or the Jacobi smoother). Diagnostics. */
 gmf_status gmf_mg_level_info(const gmf_preconditioner* mg, int32_t level, int64_t* reduced_dofs, double* lambda_max);
 
+/* ---- learned per-cell block correction ---------------------------------------
+ * No reference counterpart: a small per-cell MLP (BF16 weights on MFMA, FP32
+ * accumulation) used as an additional correction inside FGMRES, which accepts
+ * variable preconditioners (src/solve.jl:80-88), through the preconditioner
+ * seam src/solve.jl:32, 641-670, 753-774. */
+typedef struct gmf_mlp gmf_mlp;
+
+typedef struct gmf_mlp_desc {
+  int32_t input, hidden1, hidden2, output; /* layer widths */
+  /* Initial FP32 parameters, packed [W1 (input x hidden1, row-major), b1,
+   * W2 (hidden1 x hidden2), b2, W3 (hidden2 x output), b3]; y = W^T x + b per layer,
+   * ReLU between layers. HOST array, copied. */
+  const float* parameters;
+} gmf_mlp_desc;
+
+gmf_status gmf_mlp_create(const gmf_mlp_desc* desc, gmf_mlp** out);
+gmf_status gmf_mlp_destroy(gmf_mlp* mlp);
+gmf_status gmf_mlp_set_stream(gmf_mlp* mlp, void* stream);
+gmf_status gmf_mlp_parameter_count(const gmf_mlp* mlp, int64_t* count);
+/* Current FP32 master parameters into a HOST array (packed as in gmf_mlp_desc). */
+gmf_status gmf_mlp_get_parameters(gmf_mlp* mlp, float* parameters);
+/* y = net(x) for `rows` FP32 rows (DEVICE arrays). */
+gmf_status gmf_mlp_forward(gmf_mlp* mlp, const float* x, float* y, int64_t rows);
+/* Mean squared error of net(x) against t and its parameter gradient (HOST,
+ * packed) for one batch (DEVICE x, t). */
+gmf_status gmf_mlp_gradient(gmf_mlp* mlp, const float* x, const float* t, int64_t rows, double* loss, float* gradient);
+/* Adam on consecutive batches of `batch` rows (cycling through
+ * floor(samples / batch) batches) for `steps` steps; loss_history (HOST,
+ * steps entries) may be NULL. */
+gmf_status gmf_mlp_train(gmf_mlp* mlp, const float* x, const float* t, int64_t samples, int64_t batch, int64_t steps,
+                         float learning_rate, float beta1, float beta2, float eps, double* loss_history);
+/* Cell patches of reduced vectors for a single-batch plan: x[p][i] =
+ * r[dof(cell_p, i)] / ||r|_cell_p||, scale[p] = ||r|_cell_p||, and with e,
+ * t[p][i] = e[dof(cell_p, i)] * cell_weight[cell_p] / scale[p]. `cells`
+ * (DEVICE, count entries) may be NULL for all cells in order. */
+gmf_status gmf_cell_patches(gmf_workspace* ws, const double* r, const double* e, const int64_t* cells, int64_t count,
+                            const float* cell_weight, float* x, float* t, float* scale);
+/* z = M(r) = z0 + P_cells(net(patches(r - A z0)) * scale / cell_weight), z0 =
+ * inner(r) (or 0 when inner is NULL), the per-cell outputs averaged on shared
+ * dofs. cell_weight: HOST array of one positive weight per cell (copied). */
+gmf_status gmf_precond_create_learned(gmf_workspace* ws, gmf_preconditioner* inner, gmf_mlp* net,
+                                      const float* cell_weight, gmf_preconditioner** out);
+
 /* Timed repetitions of gmf_apply on the workspace stream with HIP events
  * around each stage: average milliseconds of the cell-kernel pass and of the
  * whole apply (the measurement role of src/diagnostics.jl:21-59
@@ -250,6 +293,12 @@ gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_
  * the workspace stream (vector utility; also the known-size kernel used to
  * calibrate HBM byte counters). No reference counterpart. */
 gmf_status gmf_vector_copy(gmf_workspace* ws, double* dst, const double* src, int64_t n);
+
+/* y = a x + b y and *out = x . y on reduced-length device vectors (vector
+ * utilities for callers that drive their own iterations, e.g. training-data
+ * generation for the learned correction). No reference counterpart. */
+gmf_status gmf_vector_axpby(gmf_workspace* ws, double* y, double a, const double* x, double b, int64_t n);
+gmf_status gmf_vector_dot(gmf_workspace* ws, const double* x, const double* y, int64_t n, double* out);
 
 /* Jacobi inverse diagonal with validation: rejects zero / non-finite entries
  * (src/solve.jl:740-751). */

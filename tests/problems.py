@@ -126,3 +126,28 @@ def corner_refined(dim=2, base=4, levels=2, degree=2, operator=None, dirichlet=T
 
 def rng_vector(n, seed=0):
     return np.random.default_rng(seed).standard_normal(n)
+
+
+class LogNormalKappa:
+    """kappa = exp(g), g a Gaussian random field with squared-exponential
+    covariance sigma^2 exp(-|x - y|^2 / (2 l^2)), sampled by random Fourier
+    features: g(x) = sigma sqrt(2 / M) sum_m cos(w_m . x + phi_m), w_m ~ N(0, I / l^2),
+    phi_m ~ U(0, 2 pi) (seeded)."""
+
+    def __init__(self, seed=0, corr=0.1, sigma=1.0, features=256, dim=3):
+        rng = np.random.default_rng(seed)
+        self.w = rng.standard_normal((features, dim)) / corr
+        self.phi = rng.uniform(0.0, 2.0 * np.pi, size=features)
+        self.scale = sigma * np.sqrt(2.0 / features)
+
+    def __call__(self, p):
+        p = [np.asarray(c, dtype=np.float64) for c in p]
+        shape = p[0].shape
+        x = np.stack([c.reshape(-1) for c in p], axis=1)
+        g = self.scale * np.cos(x @ self.w.T + self.phi).sum(axis=1)
+        return np.exp(g).reshape(shape)
+
+
+def lognormal_poisson(depth=2, degree=3, seed=0, dim=3):
+    """The random-coefficient Poisson problem (p = 3: 64 local modes per cell)."""
+    return poisson(dim, None, degree, refine_depth=depth, kappa=LogNormalKappa(seed, dim=dim))
