@@ -167,6 +167,124 @@ def solve_leg(problem, plan, repeats=3):
     }
 
 
+def build_lognormal_problem(depth=5, degree=3, seed=2024):
+    """32^3 cells (five uniform refinements of one root), p = 3, kappa = exp(g)
+    with g a Gaussian random field (squared-exponential covariance, correlation
+    length 0.1, sigma = 1) from 256 seeded random Fourier features, unit load,
+    zero Dirichlet."""
+    import numpy as np
+
+    import gricoamd as g
+
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((256, 3)) / 0.1
+    phi = rng.uniform(0.0, 2.0 * np.pi, size=256)
+    amp = np.sqrt(2.0 / 256)
+
+    def kappa(x, y, z):
+        pts = np.stack([np.asarray(x).reshape(-1), np.asarray(y).reshape(-1), np.asarray(z).reshape(-1)], axis=1)
+        return np.exp(amp * np.cos(pts @ w.T + phi).sum(axis=1)).reshape(np.shape(x))
+
+    class Diffusion:
+        def cell_accumulate(self, q, trial, test_component):
+            k = kappa(*q.point)
+            return g.TestChannels(0.0 * trial.value, tuple(k * c for c in trial.gradient))
+
+    class Load:
+        def cell_rhs_accumulate(self, q, test_component):
+            return 1.0 + 0.0 * q.point[0]
+
+    domain = g.Domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (1, 1, 1))
+    g.uniform_refine(domain, depth)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Diffusion())
+    problem.add_cell_accumulator(u, Load())
+    for axis in range(3):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return problem
+
+
+def learned_leg(repeats=3, pairs=2000, steps=3000):
+    """Learned-correction FGMRES against GMG-CG on the log-normal p = 3
+    problem: time from x0 = 0 to ||r|| <= 1e-8 ||b||, hierarchy and network
+    already built; training reported separately."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import gricoamd as g
+    from gricoamd import _devabi, _native, ops
+    from gricoamd.learned import CellMLP, LearnedBlockCorrection, train_block_correction
+    from gricoamd.multigrid import GeometricMultigridPreconditioner
+    from gricoamd.solve import compile_preconditioner
+
+    lib = _native.device()
+    t0 = time.perf_counter()
+    problem = build_lognormal_problem()
+    plan = g.compile(problem)
+    log(f"log-normal problem: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
+    net, report, M, V = train_block_correction(problem, plan=plan, pairs=pairs, steps=steps, log=log)
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    x = torch.zeros_like(b)
+    dev = ops._ws(plan)
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return sorted(ts)[len(ts) // 2], info
+
+    def cg(handle):
+        info = _devabi.SolveInfo()
+        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 5000,
+                                       ctypes.byref(info)))
+        return info
+
+    def fgmres(handle):
+        info = _devabi.SolveInfo()
+        status = lib.gmf_fgmres_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 30, 1e-8, 0.0, 5000,
+                                      ctypes.byref(info))
+        if status == -3:
+            return None
+        _devabi.check(status)
+        return info
+
+    t_gmg, i_gmg = timed(lambda: cg(V.handle))
+    x_gmg = x.clone()
+    t_l, i_l = timed(lambda: fgmres(M.handle))
+    fallback = i_l is None
+    if fallback:
+        log("learned FGMRES did not converge: GMG-CG fallback")
+        t_l, i_l = t_gmg, i_gmg
+    x_l = x.clone()
+    t_0, i_0 = timed(lambda: fgmres(V.handle))
+    diff = float(torch.linalg.norm(x_l - x_gmg) / torch.linalg.norm(x_gmg))
+    out = {
+        "metric": "time-to-1e-8 relative residual, learned-correction FGMRES vs GMG-CG (s)",
+        "workload": "3D Poisson 32^3 cells (uniform refinement of one root), p=3, log-normal kappa "
+                    "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
+        "reduced_dofs": plan.reduced_dof_count,
+        "gmg_cg": {"seconds": t_gmg, "iterations": i_gmg.iterations},
+        "learned_fgmres": {"seconds": t_l, "iterations": i_l.iterations if i_l else None, "fallback": fallback},
+        "fgmres_vcycle_only": {"seconds": t_0, "iterations": i_0.iterations if i_0 else None},
+        "speedup_vs_gmg_cg": t_gmg / t_l,
+        "solution_relative_difference": diff,
+        "training": report,
+        "network": "per-cell MLP 64-256-256-64, BF16 weights on MFMA, FP32 accumulation",
+    }
+    log(f"learned leg: {out}")
+    return out
+
+
 def _traffic_from_profiles(n, cells):
     """HBM bytes per cell pass measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by a 1 GiB copy), recorded in profiles/apply_traffic.json
@@ -190,6 +308,9 @@ def main():
     ap.add_argument("--depth", type=int, default=6, help="uniform refinements of the root (6 -> 64^3 cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMG-CG / Jacobi-CG solve leg")
+    ap.add_argument("--learned", action="store_true",
+                    help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
+                         "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="after timing, copy a 1 GiB buffer three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -290,6 +411,8 @@ def main():
         del src, dst
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
+    if args.learned:
+        result["learned"] = learned_leg()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

@@ -20,6 +20,7 @@
 #include <memory>
 #include <vector>
 
+#include "blas.hpp"
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -256,6 +257,24 @@ __global__ void k_scatter_correction(double* __restrict__ z, const float* __rest
   }
 }
 
+// dY[c][i] = pu[idx] * g[idx] * scale[c] / cw[c] (0 where idx < 0): the output
+// gradient of the per-cell network for a global gradient g of the assembled
+// correction.
+__global__ void k_patch_gradient(float* __restrict__ dY, const double* __restrict__ g, const float* __restrict__ scale,
+                                 const float* __restrict__ cw, const double* __restrict__ pu,
+                                 const int32_t* __restrict__ idx, int64_t cells, int width) {
+  GMF_STRIDE(k, cells * int64_t(width)) {
+    const int64_t c = k / width;
+    const int32_t j = idx[k];
+    dY[k] = j >= 0 ? float(pu[j] * g[j] * double(scale[c]) / double(cw[c])) : 0.f;
+  }
+}
+
+__global__ void k_energy_gradient(double* __restrict__ g, const double* __restrict__ q, const double* __restrict__ r,
+                                  double scale, int64_t n) {
+  GMF_STRIDE(i, n) g[i] = scale * (q[i] - r[i]);
+}
+
 __global__ void k_count_cells(const int32_t* __restrict__ idx, int64_t n, double* __restrict__ count) {
   GMF_STRIDE(k, n) {
     const int32_t j = idx[k];
@@ -335,6 +354,21 @@ struct gmf_mlp {
     hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, stream, y.p, tgt, dy16.p, partial.p, ny, scale);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, stream, partial.p, int(blocks), loss.p);
     GMF_HIP(hipGetLastError());
+    backprop(rows);
+  }
+
+  // Parameter gradient for an FP32 output gradient dy (rows x out) of the last forward.
+  void backward_from(const float* dy, int64_t rows) {
+    using namespace gmf;
+    const int64_t ny = rows * out;
+    hipLaunchKernelGGL(k_to_bf16, dim3(grid1(ny)), dim3(256), 0, stream, dy16.p, dy, ny);
+    GMF_HIP(hipGetLastError());
+    backprop(rows);
+  }
+
+  // Backpropagation from dy16 through the activations of the last forward.
+  void backprop(int64_t rows) {
+    using namespace gmf;
     // layer 3: dW3 = a2^T dy, db3 = colsum(dy), dz2 = (dy W3^T) * (a2 > 0)
     GemmArgs g{};
     g = GemmArgs{h2, out, int(rows), a2.p, h2, dy16.p, out, grad.p + off_w3, nullptr, out, nullptr, nullptr};
@@ -382,6 +416,56 @@ struct LearnedPrecond : gmf_preconditioner {
   DeviceBuffer<double> pu, q, r1;
   double omega = 1.0;
 
+  // c = sum_cells P^T (w * net(patches(r1)) * scale / cw) into z (added).
+  void correction(const double* r1v, double* z) {
+    hipStream_t s = ws->stream;
+    const unsigned pb = unsigned((cells + 3) / 4);
+    hipLaunchKernelGGL(k_patches, dim3(pb), dim3(256), 0, s, r1v, static_cast<const double*>(nullptr), idx,
+                       static_cast<const int64_t*>(nullptr), cells, width, cw.p, X.p, static_cast<float*>(nullptr),
+                       scale.p);
+    GMF_HIP(hipGetLastError());
+    net->stream = s;
+    net->forward(X.p, cells);
+    hipLaunchKernelGGL(k_scatter_correction, dim3(grid1(cells * width)), dim3(256), 0, s, z, net->y.p, scale.p, cw.p,
+                       pu.p, idx, cells, width);
+    GMF_HIP(hipGetLastError());
+  }
+
+  // Energy-norm training on residuals r'_k left by the inner preconditioner:
+  // minimise ||e' - c||_A^2 = e'^T A e' - 2 c^T r' + c^T A c over the network,
+  // one residual per Adam step. history[s] = ||e' - c||_A^2 / ||e'||_A^2
+  // before the step (energy[k] = e'^T r' = ||e'||_A^2).
+  void train(const double* residuals, const double* energy, int64_t pairs, int64_t steps, float lr, float b1, float b2,
+             float eps, double* history) {
+    const int64_t n = ws->plan->n_red;
+    hipStream_t s = ws->stream;
+    DeviceBuffer<double> c(static_cast<size_t>(n)), gq(static_cast<size_t>(n));
+    DeviceBuffer<float> dY(size_t(cells) * size_t(width));
+    net->stream = s;
+    for (int64_t st = 0; st < steps; ++st) {
+      const int64_t k = st % pairs;
+      const double* rk = residuals + k * n;
+      fill(c.p, 0.0, n, s);
+      correction(rk, c.p);
+      reduced_apply(ws, c.p, q.p);
+      double cq = 0.0, cr = 0.0;
+      dot(ws, c.p, q.p, n, kTmp);
+      dot(ws, c.p, rk, n, kTmp + 1);
+      read_scalars(ws, kTmp + 2);
+      cq = ws->host_scalars[kTmp];
+      cr = ws->host_scalars[kTmp + 1];
+      const double en = energy[k] > 0.0 ? energy[k] : 1.0;
+      if (history) history[st] = 1.0 + (cq - 2.0 * cr) / en;
+      hipLaunchKernelGGL(k_energy_gradient, dim3(grid1(n)), dim3(256), 0, s, gq.p, q.p, rk, 2.0 / en, n);
+      hipLaunchKernelGGL(k_patch_gradient, dim3(grid1(cells * width)), dim3(256), 0, s, dY.p, gq.p, scale.p, cw.p,
+                         pu.p, idx, cells, width);
+      GMF_HIP(hipGetLastError());
+      net->backward_from(dY.p, cells);
+      net->adam(lr, b1, b2, eps);
+    }
+    sync(ws);
+  }
+
   void apply(const double* r, double* z) override {
     const int64_t n = ws->plan->n_red;
     hipStream_t s = ws->stream;
@@ -394,16 +478,7 @@ struct LearnedPrecond : gmf_preconditioner {
       fill(z, 0.0, n, s);
       copy(r1.p, r, n, s);
     }
-    const unsigned pb = unsigned((cells + 3) / 4);
-    hipLaunchKernelGGL(k_patches, dim3(pb), dim3(256), 0, s, r1.p, static_cast<const double*>(nullptr), idx,
-                       static_cast<const int64_t*>(nullptr), cells, width, cw.p, X.p, static_cast<float*>(nullptr),
-                       scale.p);
-    GMF_HIP(hipGetLastError());
-    net->stream = s;
-    net->forward(X.p, cells);
-    hipLaunchKernelGGL(k_scatter_correction, dim3(grid1(cells * width)), dim3(256), 0, s, z, net->y.p, scale.p, cw.p,
-                       pu.p, idx, cells, width);
-    GMF_HIP(hipGetLastError());
+    correction(r1.p, z);
   }
 };
 
@@ -608,6 +683,20 @@ gmf_status gmf_precond_create_learned(gmf_workspace* ws, gmf_preconditioner* inn
     net->ensure(b.cells);
     GMF_HIP(hipStreamSynchronize(ws->stream));
     *out = L.release();
+  });
+}
+
+gmf_status gmf_learned_train(gmf_preconditioner* learned, const double* residuals, const double* energies,
+                             int64_t pairs, int64_t steps, float lr, float beta1, float beta2, float eps,
+                             double* history) {
+  return guarded_call([&] {
+    auto* L = dynamic_cast<LearnedPrecond*>(learned);
+    require(L != nullptr, GMF_EINVAL, "not a learned-correction preconditioner");
+    require(residuals && energies && pairs > 0 && steps >= 0, GMF_EINVAL, "invalid argument");
+    require(std::isfinite(lr) && lr > 0 && beta1 >= 0 && beta1 < 1 && beta2 >= 0 && beta2 < 1 && eps > 0, GMF_EINVAL,
+            "invalid Adam parameters");
+    require_device_pointer(residuals, "residuals");
+    L->train(residuals, energies, pairs, steps, lr, beta1, beta2, eps, history);
   });
 }
 

@@ -12,10 +12,13 @@ it is used inside FGMRES (src/solve.jl:80-88 allows that); the seam is the
 preconditioner policy interface (src/solve.jl:32, 641-670, 753-774).
 
 Training data come from the GMG-CG solves themselves: for reference
-solutions x* and the CG iterates x_k, the pair (b - A y, x* - y) with
-y = x_k + V(b - A x_k) is the residual / error the V-cycle leaves, sampled on
-random cells. The network (BF16 weights, FP32 accumulation on MFMA) is trained
-on the device with Adam (libgrico_mf.so gmf_mlp_*).
+solutions x* and the CG iterates x_k, the pair (r', e') = (b - A y, x* - y)
+with y = x_k + V(b - A x_k) is the residual / error the V-cycle leaves. The
+network (BF16 weights, FP32 accumulation on MFMA) is trained on the device
+with Adam to minimise the energy norm of the error left after its correction,
+||e' - c(r')||_A^2 = e'^T A e' - 2 c^T r' + c^T A c, which is what the outer
+Krylov method sees (gmf_learned_train). A supervised per-cell fit of patches
+(gmf_mlp_train on gmf_cell_patches output) is available as well.
 """
 
 import ctypes
@@ -47,7 +50,9 @@ def _declare(lib):
     lib.gmf_precond_create_learned.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
     lib.gmf_vector_axpby.argtypes = [c_void_p, c_void_p, c_double, c_void_p, c_double, c_int64]
     lib.gmf_vector_dot.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_double)]
-    for name in ("gmf_mlp_create", "gmf_mlp_destroy", "gmf_mlp_set_stream", "gmf_mlp_parameter_count",
+    lib.gmf_learned_train.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_float, c_float, c_float,
+                                      c_float, c_void_p]
+    for name in ("gmf_learned_train", "gmf_mlp_create", "gmf_mlp_destroy", "gmf_mlp_set_stream", "gmf_mlp_parameter_count",
                  "gmf_mlp_get_parameters", "gmf_mlp_forward", "gmf_mlp_gradient", "gmf_mlp_train", "gmf_cell_patches",
                  "gmf_precond_create_learned", "gmf_vector_axpby", "gmf_vector_dot"):
         getattr(lib, name).restype = ctypes.c_int
@@ -243,9 +248,58 @@ def collect_pairs(plan, vcycle, pairs=2000, cells_per_pair=256, iterations=None,
     return X.index_select(0, perm).contiguous(), T.index_select(0, perm).contiguous(), got
 
 
+def collect_residuals(plan, vcycle, pairs=2000, iterations=40, seed=0, rtol=1e-12):
+    """Residuals r' = b - A y and energies e'^T r' from GMG-CG solves (module
+    docstring): a (pairs x n_red) float64 device tensor and a host array."""
+    torch = ops._torch()
+    n = plan.reduced_dof_count
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    f64 = dict(dtype=torch.float64, device="cuda")
+    R = torch.empty((pairs, n), **f64)
+    energy = np.zeros(pairs)
+    xs, b, x, r, z, p, q, y, e1 = (torch.empty(n, **f64) for _ in range(9))
+    got = 0
+    while got < pairs:
+        xs.normal_(generator=gen)
+        _rapply(plan, xs, b)
+        x.zero_()
+        r.copy_(b)
+        z.copy_(vcycle.apply(r))
+        p.copy_(z)
+        rz = _dot(plan, r, z)
+        bnorm = math.sqrt(_dot(plan, b, b))
+        for _ in range(iterations):
+            if got >= pairs:
+                break
+            r1 = R[got]
+            y.copy_(x)
+            _axpby(plan, y, 1.0, z, 1.0)
+            _rapply(plan, y, q)
+            r1.copy_(b)
+            _axpby(plan, r1, -1.0, q, 1.0)
+            e1.copy_(xs)
+            _axpby(plan, e1, -1.0, y, 1.0)
+            energy[got] = _dot(plan, e1, r1)
+            got += 1
+            _rapply(plan, p, q)
+            alpha = rz / _dot(plan, p, q)
+            _axpby(plan, x, alpha, p, 1.0)
+            _axpby(plan, r, -alpha, q, 1.0)
+            if math.sqrt(_dot(plan, r, r)) <= rtol * bnorm:
+                break
+            z.copy_(vcycle.apply(r))
+            rz_new = _dot(plan, r, z)
+            _axpby(plan, p, 1.0, z, rz_new / rz)
+            rz = rz_new
+    return R, energy
+
+
 class LearnedBlockCorrection:
     """Preconditioner policy: the hp-GMG V-cycle followed by the learned cell
-    correction. `network` must have the cell's local mode count as width."""
+    correction. `network` must have the cell's local mode count as width;
+    `inner` is a GeometricMultigridPreconditioner policy or an already
+    compiled DevicePreconditioner of the same plan."""
 
     def __init__(self, network, inner=None):
         self.network = network
@@ -256,7 +310,9 @@ class LearnedBlockCorrection:
         lib = _native.device()
         _declare(lib)
         inner_handle, keep = None, ()
-        if self.inner is not None:
+        if isinstance(self.inner, DevicePreconditioner):
+            inner_handle, keep = self.inner.handle, (self.inner,)
+        elif self.inner is not None:
             hierarchy = compile_geometric_multigrid(plan.problem, self.inner, fine_plan=plan)
             M = hierarchy.device_preconditioner()
             inner_handle, keep = M.handle, (M, hierarchy)
@@ -267,11 +323,13 @@ class LearnedBlockCorrection:
         return DevicePreconditioner(plan, h, keep=(keep, self.network, cw))
 
 
-def train_block_correction(problem, plan=None, widths=None, pairs=2000, cells_per_pair=256, steps=3000,
-                           batch=4096, learning_rate=2e-3, seed=0, inner=None, log=None):
-    """Generate GMG-CG pairs on the problem and train a CellMLP. Returns
-    (network, report)."""
+def train_block_correction(problem, plan=None, widths=None, pairs=2000, steps=3000, learning_rate=1e-3, seed=0,
+                           inner=None, iterations=40, log=None):
+    """Build the GMG hierarchy, collect V-cycle residuals from GMG-CG solves
+    and train a CellMLP in the energy norm. Returns (network, report,
+    compiled learned preconditioner, compiled V-cycle)."""
     from .plan import compile as compile_plan
+    torch = ops._torch()
     plan = plan or compile_plan(problem)
     inner = inner or GeometricMultigridPreconditioner(smoother="chebyshev")
     t0 = time.perf_counter()
@@ -280,16 +338,22 @@ def train_block_correction(problem, plan=None, widths=None, pairs=2000, cells_pe
     width = plan.buckets[0].n_modes
     widths = widths or (width, 256, 256, width)
     t1 = time.perf_counter()
-    X, T, got = collect_pairs(plan, V, pairs=pairs, cells_per_pair=cells_per_pair, seed=seed)
-    ops._torch().cuda.synchronize()
+    R, energy = collect_residuals(plan, V, pairs=pairs, iterations=iterations, seed=seed)
+    torch.cuda.synchronize()
     t2 = time.perf_counter()
     net = CellMLP(widths, seed=seed)
-    hist = net.train(X, T, batch=min(batch, X.shape[0]), steps=steps, learning_rate=learning_rate)
-    ops._torch().cuda.synchronize()
+    M = LearnedBlockCorrection(net, inner=V).compile(plan)
+    hist = np.zeros(steps)
+    _devabi.check(_native.device().gmf_learned_train(M.handle, R.data_ptr(), energy.ctypes.data_as(ctypes.c_void_p),
+                                                     pairs, steps, float(learning_rate), 0.9, 0.999, 1e-8,
+                                                     hist.ctypes.data_as(ctypes.c_void_p)))
+    torch.cuda.synchronize()
     t3 = time.perf_counter()
-    report = {"pairs": got, "samples": int(X.shape[0]), "hierarchy_seconds": t1 - t0, "collect_seconds": t2 - t1,
-              "train_seconds": t3 - t2, "loss_first": float(hist[:20].mean()), "loss_last": float(hist[-20:].mean()),
-              "steps": steps, "batch": int(min(batch, X.shape[0]))}
+    del R
+    k = max(1, min(50, steps // 10))
+    report = {"pairs": pairs, "hierarchy_seconds": t1 - t0, "collect_seconds": t2 - t1, "train_seconds": t3 - t2,
+              "steps": steps, "energy_ratio_first": float(hist[:k].mean()), "energy_ratio_last": float(hist[-k:].mean())}
     if log:
         log(f"learned correction: {report}")
-    return net, report, (plan, hierarchy, V)
+    M._keep = (M._keep, hierarchy)
+    return net, report, M, V

@@ -282,6 +282,15 @@ gmf_status gmf_cell_patches(gmf_workspace* ws, const double* r, const double* e,
 gmf_status gmf_precond_create_learned(gmf_workspace* ws, gmf_preconditioner* inner, gmf_mlp* net,
                                       const float* cell_weight, gmf_preconditioner** out);
 
+/* Energy-norm training of the learned preconditioner's network: for each
+ * residual r'_k (DEVICE, pairs x n_red, row k = pair k) left by the inner
+ * preconditioner, minimise ||e'_k - c(r'_k)||_A^2 = e'^T A e' - 2 c^T r' + c^T A c
+ * with one Adam step per residual (cycling). energies (HOST) = e'_k^T r'_k;
+ * history (HOST, steps entries, may be NULL) records ||e' - c||_A^2 / ||e'||_A^2. */
+gmf_status gmf_learned_train(gmf_preconditioner* learned, const double* residuals, const double* energies,
+                             int64_t pairs, int64_t steps, float learning_rate, float beta1, float beta2, float eps,
+                             double* history);
+
 /* Timed repetitions of gmf_apply on the workspace stream with HIP events
  * around each stage: average milliseconds of the cell-kernel pass and of the
  * whole apply (the measurement role of src/diagnostics.jl:21-59

@@ -110,19 +110,17 @@ def test_untrained_correction_keeps_fgmres_exact(gpu):
 
 
 def test_trained_correction_reduces_iterations(gpu):
-    """Trained on GMG-CG pairs of one random field, the correction lowers the
-    FGMRES iteration count on a new right-hand side, against FGMRES with the
-    V-cycle alone (the method without the network)."""
+    """Trained in the energy norm on V-cycle residuals of GMG-CG solves, the
+    correction leaves less energy error than it starts with and lowers the
+    FGMRES iteration count on the problem's right-hand side, against FGMRES
+    with the V-cycle alone (the same method without the network)."""
     pr = P.lognormal_poisson(depth=3, degree=3, seed=5)
     plan = g.compile(pr)
-    net, report, _ = train_block_correction(pr, plan=plan, pairs=400, cells_per_pair=128, steps=1500,
-                                            batch=2048, seed=1)
-    assert report["loss_last"] < report["loss_first"]
-    with_net = g.solve(plan, solver=g.FGMRESSolver(preconditioner=LearnedBlockCorrection(net)),
-                       relative_tolerance=1e-8)
-    zero = CellMLP((64, 256, 256, 64), parameters=np.zeros(net.parameter_count, dtype=np.float32))
-    without = g.solve(plan, solver=g.FGMRESSolver(preconditioner=LearnedBlockCorrection(zero)),
-                      relative_tolerance=1e-8)
+    net, report, M, V = train_block_correction(pr, plan=plan, pairs=200, steps=1500, seed=1)
+    assert report["energy_ratio_last"] < report["energy_ratio_first"]
+    assert report["energy_ratio_last"] < 1.0
+    with_net = g.solve(plan, solver=g.FGMRESSolver(preconditioner=M), relative_tolerance=1e-8)
+    without = g.solve(plan, solver=g.FGMRESSolver(preconditioner=V), relative_tolerance=1e-8)
     print("iterations with / without the network:", with_net.info.iterations, without.info.iterations, report)
     assert with_net.info.converged == 1 and without.info.converged == 1
     assert with_net.info.iterations < without.info.iterations
