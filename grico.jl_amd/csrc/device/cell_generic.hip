@@ -10,6 +10,7 @@
 // cell_diagonal! (src/accumulators.jl:1056-1083) by sum factorization with
 // products of 1D tables instead of one basis probe per mode.
 #include "common.hpp"
+#include "termmap.hpp"
 
 namespace gmf {
 namespace {
@@ -100,35 +101,17 @@ __device__ void project_add(const GenericArgs& a, const double* f, double* r, do
   __syncthreads();
 }
 
-__device__ double gather_one(const GenericArgs& a, int64_t cell, int i, const double* __restrict__ x) {
-  const int32_t g = a.idx[cell * a.Ntot + i];
-  if (g >= 0) return a.lcoef ? a.lcoef[cell * a.Ntot + i] * x[g] : x[g];
-  if (g == -1) return 0.0;
-  const int64_t m = -int64_t(g) - 2;
-  double v = 0.0;
-  for (int64_t t = a.moff[m]; t < a.moff[m + 1]; ++t) {
-    const int32_t h = a.midx[t];
-    if (h >= 0) v = fma(a.mcoef[t], x[h], v);
-  }
-  return v;
+__device__ inline TermMap term_map(const GenericArgs& a) {
+  return TermMap{a.idx, a.lcoef, a.moff, a.midx, a.mcoef, a.Ntot};
 }
 
-__device__ void scatter_one(const GenericArgs& a, int64_t cell, int i, double v, double* __restrict__ y,
-                            bool squared) {
-  const int32_t g = a.idx[cell * a.Ntot + i];
-  if (g >= 0) {
-    double c = a.lcoef ? a.lcoef[cell * a.Ntot + i] : 1.0;
-    unsafeAtomicAdd(&y[g], (squared ? c * c : c) * v);
-    return;
-  }
-  if (g == -1) return;
-  const int64_t m = -int64_t(g) - 2;
-  for (int64_t t = a.moff[m]; t < a.moff[m + 1]; ++t) {
-    const int32_t h = a.midx[t];
-    if (h < 0) continue;
-    const double c = a.mcoef[t];
-    unsafeAtomicAdd(&y[h], (squared ? c * c : c) * v);
-  }
+__device__ inline double gather_one(const GenericArgs& a, int64_t cell, int i, const double* __restrict__ x) {
+  return gather_term(term_map(a), cell, i, x);
+}
+
+__device__ inline void scatter_one(const GenericArgs& a, int64_t cell, int i, double v, double* __restrict__ y,
+                                   bool squared) {
+  scatter_term(term_map(a), cell, i, v, y, squared);
 }
 
 // r = A_c u for one cell (u, r in LDS, Ntot entries); `fields` holds (D+1) x

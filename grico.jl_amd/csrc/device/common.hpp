@@ -71,6 +71,13 @@ struct Bucket {
   std::vector<double> h_colloc;
   bool fast = false;
   BrickPath bricks;
+  // Point-cloud batch: item c owns points poff[c]..poff[c+1]-1 at biunit
+  // reference coordinates rpts (points x dim); pdata rows per point.
+  bool cloud = false;
+  int64_t total_points = 0;
+  int max_points = 0;
+  int64_t* poff = nullptr;
+  double* rpts = nullptr;
   // Exact reduced diagonal on cells whose modes couple through multi-term
   // expansions or repeat a dof: `coupled` flags those cells (null if none) and
   // each (cell, reduced dof) pair lists the dof's local coefficient vector.
@@ -137,6 +144,10 @@ void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double*
 // local diagonal of the others, which the caller launches with skip_coupled).
 void launch_cell_diagonal_pairs(const gmf_plan& plan, const Bucket& b, double* diag, hipStream_t stream);
 void launch_cell_diagonal_generic_skip(const gmf_plan& plan, const Bucket& b, double* diag, hipStream_t stream);
+// Point-cloud batches (cell_cloud.hip).
+void launch_cell_cloud(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
+                       hipStream_t stream);
+void launch_cell_cloud_diagonal(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced, hipStream_t stream);
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                       hipStream_t stream);
 // lidx: host copy of the bucket's full-layout index map; red_of_full: full dof

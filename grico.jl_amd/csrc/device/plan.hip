@@ -181,6 +181,10 @@ __global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x
 void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   const gmf_plan& p = *ws->plan;
   for (const Bucket& b : p.buckets) {
+    if (b.cloud) {
+      launch_cell_cloud(p, b, x, y, reduced, ws->stream);
+      continue;
+    }
     if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->stream)) continue;
     launch_cell_generic(p, b, x, y, reduced, ws->stream);
   }
@@ -253,6 +257,12 @@ void reduced_diagonal(gmf_workspace* ws, double* diag) {
           "exact reduced diagonal needs a selection reduced map (affine / mean rows are not implemented)");
   fill(diag, 0.0, p.n_red, ws->stream);
   for (const Bucket& b : p.buckets) {
+    if (b.cloud) {
+      require(!b.coupled, GMF_EUNSUPPORTED,
+              "exact reduced diagonal of point-cloud items with multi-term expansions is not implemented");
+      launch_cell_cloud_diagonal(p, b, diag, true, ws->stream);
+      continue;
+    }
     if (b.coupled) {
       launch_cell_diagonal_generic_skip(p, b, diag, ws->stream);
       launch_cell_diagonal_pairs(p, b, diag, ws->stream);
@@ -472,9 +482,10 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         b.cells = bd.cell_count;
         require(b.cells >= 0 && b.cells < (int64_t(1) << 31), GMF_EINVAL, "cell count out of range");
         int off = 0;
+        b.cloud = bd.point_offsets != nullptr;
         for (int a = 0; a < b.dim; ++a) {
           b.N[a] = bd.modes[a];
-          b.Q[a] = bd.points[a];
+          b.Q[a] = b.cloud ? 1 : bd.points[a];
           require(b.N[a] >= 1 && b.N[a] <= kMaxExtent && b.Q[a] >= 1 && b.Q[a] <= kMaxExtent, GMF_EUNSUPPORTED,
                   "per-axis mode and point counts must lie in 1..16");
           b.Ntot *= b.N[a];
@@ -483,10 +494,27 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           off += b.N[a] * b.Q[a];
         }
         b.point_stride = b.form == GMF_FORM_DIFFUSION ? 1 : (b.dim + 1) * (b.dim + 1);
-        b.h_values.assign(bd.values, bd.values + off);
-        b.h_derivs.assign(bd.derivatives, bd.derivatives + off);
-        b.values = upload(p, bd.values, size_t(off));
-        b.derivs = upload(p, bd.derivatives, size_t(off));
+        if (b.cloud) {
+          require(b.form == GMF_FORM_GENERAL, GMF_EINVAL, "point-cloud batches carry the general point action");
+          require(bd.ref_points != nullptr, GMF_EINVAL, "point-cloud batch without reference points");
+          const int64_t* po = bd.point_offsets;
+          require(po[0] == 0, GMF_EINVAL, "point offsets must start at 0");
+          for (int64_t c = 0; c < b.cells; ++c) {
+            require(po[c + 1] >= po[c], GMF_EINVAL, "point offsets must be non-decreasing");
+            b.max_points = std::max<int>(b.max_points, int(po[c + 1] - po[c]));
+          }
+          b.total_points = po[b.cells];
+          for (int64_t k = 0; k < b.total_points * b.dim; ++k)
+            require(std::isfinite(bd.ref_points[k]) && std::fabs(bd.ref_points[k]) <= 1.0 + 1e-12, GMF_EINVAL,
+                    "reference points must lie in the biunit cell");
+          b.poff = upload(p, po, size_t(b.cells) + 1);
+          b.rpts = upload(p, bd.ref_points, size_t(b.total_points) * size_t(b.dim));
+        } else {
+          b.h_values.assign(bd.values, bd.values + off);
+          b.h_derivs.assign(bd.derivatives, bd.derivatives + off);
+          b.values = upload(p, bd.values, size_t(off));
+          b.derivs = upload(p, bd.derivatives, size_t(off));
+        }
         if (b.cells == 0) {
           p.buckets.push_back(std::move(b));
           continue;
@@ -495,7 +523,8 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         for (size_t i = 0; i < size_t(b.cells) * size_t(b.dim); ++i)
           require(std::isfinite(bd.inverse_jacobian[i]), GMF_EINVAL, "non-finite inverse Jacobian");
         b.jinv = upload(p, bd.inverse_jacobian, size_t(b.cells) * size_t(b.dim));
-        b.pdata = upload(p, bd.point_data, size_t(b.cells) * size_t(b.Qtot) * size_t(b.point_stride));
+        b.pdata = upload(p, bd.point_data,
+                         (b.cloud ? size_t(b.total_points) : size_t(b.cells) * size_t(b.Qtot)) * size_t(b.point_stride));
         b.multi_count = bd.multi_count;
         for (size_t i = 0; i < nmode; ++i) {
           int32_t g = bd.local_index[i];
@@ -529,7 +558,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           }
           build_diagonal_pairs(p, b, r, bd, mr);
         }
-        prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
+        if (!b.cloud) prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
         p.buckets.push_back(std::move(b));
       }
       GMF_HIP(hipDeviceSynchronize());
@@ -569,7 +598,8 @@ gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity) 
       s += "bucket " + std::to_string(i) + ": dim=" + std::to_string(b.dim) + " N=" + std::to_string(b.N[0]) +
            " Q=" + std::to_string(b.Q[0]) + " cells=" + std::to_string(b.cells) + " form=" + std::to_string(b.form) +
            " multi=" + std::to_string(b.multi_count) + " path=" +
-           (b.bricks.enabled ? "brick" : (b.fast ? "fast-cell" : "generic"));
+           (b.cloud ? "cloud points=" + std::to_string(b.total_points)
+                    : (b.bricks.enabled ? "brick" : (b.fast ? "fast-cell" : "generic")));
       if (b.bricks.enabled) {
         s += " bricks=" + std::to_string(b.bricks.bricks) + " colors=" + std::to_string(b.bricks.colors) + " [";
         for (int c = 0; c < b.bricks.colors; ++c)
@@ -706,7 +736,12 @@ gmf_status gmf_diagonal_physical(gmf_workspace* ws, double* diag) {
     require(ws && diag, GMF_EINVAL, "null argument");
     const gmf_plan& p = *ws->plan;
     fill(diag, 0.0, p.n_full, ws->stream);
-    for (const Bucket& b : p.buckets) launch_cell_diagonal_generic(p, b, diag, false, ws->stream);
+    for (const Bucket& b : p.buckets) {
+      if (b.cloud)
+        launch_cell_cloud_diagonal(p, b, diag, false, ws->stream);
+      else
+        launch_cell_diagonal_generic(p, b, diag, false, ws->stream);
+    }
     sync(ws);
   });
 }

@@ -24,7 +24,7 @@ class BucketDesc(ctypes.Structure):
         ("values", P_f64), ("derivatives", P_f64), ("inverse_jacobian", P_f64), ("point_data", P_f64),
         ("local_index", P_i32), ("local_coefficient", P_f64),
         ("multi_count", c_int64), ("multi_offsets", P_i64), ("multi_indices", P_i32),
-        ("multi_coefficients", P_f64),
+        ("multi_coefficients", P_f64), ("point_offsets", P_i64), ("ref_points", P_f64),
     ]
 
 
@@ -130,23 +130,37 @@ class DevicePlan:
     def __init__(self, plan):
         lib = _native.device()
         keep = []
-        descs = (BucketDesc * len(plan.buckets))()
-        for i, b in enumerate(plan.buckets):
+        buckets = getattr(plan, "all_buckets", plan.buckets)
+        descs = (BucketDesc * len(buckets))()
+        for i, b in enumerate(buckets):
             d = descs[i]
             d.dim = b.dim
+            cloud = getattr(b, "cloud", False)
             for a in range(3):
                 d.modes[a] = b.modes[a] if a < b.dim else 1
-                d.points[a] = b.points[a] if a < b.dim else 1
+                d.points[a] = (1 if cloud else b.points[a]) if a < b.dim else 1
             d.form = b.form
             d.cell_count = b.cell_count
-            vals = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.B]))
-            ders = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.G]))
             pdata = np.ascontiguousarray(b.point_data, dtype=np.float64)
             coef = None if b.unit_coefficients else b.local_coefficient
-            keep += [vals, ders, pdata, b.inverse_jacobian, b.local_index, coef, b.multi_offsets,
+            keep += [pdata, b.inverse_jacobian, b.local_index, coef, b.multi_offsets,
                      b.multi_indices, b.multi_coefficients]
-            d.values = _p(vals, c_double)
-            d.derivatives = _p(ders, c_double)
+            if cloud:
+                po = np.ascontiguousarray(b.point_offsets, dtype=np.int64)
+                rp = np.ascontiguousarray(b.ref, dtype=np.float64)
+                keep += [po, rp]
+                d.point_offsets = _p(po, c_int64)
+                d.ref_points = _p(rp, c_double)
+                d.values = None
+                d.derivatives = None
+            else:
+                vals = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.B]))
+                ders = np.ascontiguousarray(np.concatenate([t.reshape(-1) for t in b.G]))
+                keep += [vals, ders]
+                d.values = _p(vals, c_double)
+                d.derivatives = _p(ders, c_double)
+                d.point_offsets = None
+                d.ref_points = None
             d.inverse_jacobian = _p(b.inverse_jacobian, c_double)
             d.point_data = _p(pdata, c_double)
             d.local_index = _p(b.local_index, c_int32)
@@ -158,7 +172,7 @@ class DevicePlan:
         pd = PlanDesc()
         pd.dim = plan.dim
         pd.full_dof_count = plan.full_dof_count
-        pd.bucket_count = len(plan.buckets)
+        pd.bucket_count = len(buckets)
         pd.buckets = descs
         fixed = _native.as_i64(plan.fixed_dofs)
         keep.append(fixed)

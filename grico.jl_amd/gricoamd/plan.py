@@ -206,6 +206,121 @@ class Bucket:
         return t.reshape(self.cell_count, -1)
 
 
+def _basis_at(x, degree):
+    """psi_0..psi_p and derivatives at points x (n,) -> (n, p+1) each."""
+    from .quadrature import fe_basis_values_and_derivatives
+    return fe_basis_values_and_derivatives(x, degree)
+
+
+class CloudBucket:
+    """Items that carry their own reference points: cells with a custom
+    quadrature (src/accumulators.jl:1022-1054) or embedded-surface pieces
+    (1133-1214, src/integration.jl:2338-2368). One device batch per degree
+    tuple; the point action is always the general (D+1) x (D+1) form."""
+
+    cloud = True
+
+    def __init__(self, space, leaf_idx, degrees, rules, normals=None):
+        dom = space.domain
+        self.dim = D = space.dim
+        self.surface = normals is not None
+        self.leaf_idx = np.asarray(leaf_idx, dtype=np.int64)
+        self.cells = space.leaves[self.leaf_idx]
+        self.degrees = tuple(int(p) for p in degrees)
+        self.modes = tuple(p + 1 for p in self.degrees)
+        self.n_modes = int(np.prod(self.modes))
+        self.cell_count = nc = len(self.cells)
+        counts = np.asarray([r.point_count() for r in rules], dtype=np.int64)
+        self.point_offsets = np.zeros(nc + 1, dtype=np.int64)
+        self.point_offsets[1:] = np.cumsum(counts)
+        self.n_points_total = int(self.point_offsets[-1])
+        self.ref = np.ascontiguousarray(np.concatenate([r.points for r in rules], axis=0).reshape(-1, D)) \
+            if nc else np.zeros((0, D))
+        ref_w = np.concatenate([r.weights for r in rules]) if nc else np.zeros(0)
+        self.item_of_point = np.repeat(np.arange(nc), counts)
+        sizes = dom.cell_sizes(self.cells)
+        centers, halves = dom.cell_centers_and_half_sizes(self.cells)
+        self.sizes = sizes
+        self.inverse_jacobian = np.ascontiguousarray(1.0 / halves)
+        h = halves[self.item_of_point]
+        detj = np.prod(h, axis=1)
+        self.phys_points = [centers[self.item_of_point, a] + self.ref[:, a] * h[:, a] for a in range(D)]
+        if self.surface:
+            n_ref = np.concatenate(normals, axis=0).reshape(-1, D)
+            # src/integration.jl:2501-2516: weight scale det J / |J n_ref|, normal J^-1 n_ref normalized
+            mapped = np.linalg.norm(h * n_ref, axis=1)
+            self.weights = ref_w * detj / mapped
+            t = n_ref / h
+            self.normals = t / np.linalg.norm(t, axis=1)[:, None]
+        else:
+            self.weights = ref_w * detj
+            self.normals = None
+        Bucket._gather_maps(self, space)
+
+    def quadrature(self):
+        return QuadraturePoints(self.phys_points, self.weights,
+                                [self.sizes[self.item_of_point, a] for a in range(self.dim)],
+                                normal=None if self.normals is None else [self.normals[:, a] for a in range(self.dim)])
+
+    def lower(self, operators, surface=False):
+        """Probe the callbacks with unit trial channels at every point:
+        point_data (P x (D+1)^2) = W_q M_q, row-major [test channel][trial channel]."""
+        D = self.dim
+        P = self.n_points_total
+        q = self.quadrature()
+        ones, zeros = np.ones(P), np.zeros(P)
+        M = np.zeros((P, D + 1, D + 1))
+        for op in operators:
+            call = op.surface_accumulate if surface else op.cell_accumulate
+            for ch in range(D + 1):
+                val = ones if ch == 0 else zeros
+                grad = tuple(ones if ch == a + 1 else zeros for a in range(D))
+                v, g = _channels(call(q, TrialChannels(0, val, grad), 0), (P,), D)
+                M[:, 0, ch] += v
+                for a in range(D):
+                    M[:, a + 1, ch] += g[a]
+        self.form = FORM_GENERAL
+        self.point_data = np.ascontiguousarray((self.weights[:, None, None] * M).reshape(P, -1))
+        return self.form, self.point_data
+
+    def tables(self):
+        """Full basis tables at the points: values (P x N box) and physical
+        gradients (D, P, N box), axis 1 fastest in the box."""
+        D = self.dim
+        vals, ders = [], []
+        for a in range(D):
+            v, d = _basis_at(self.ref[:, a], self.degrees[a])
+            vals.append(v)
+            ders.append(d)
+        P = self.n_points_total
+        phi = np.ones((P, 1))
+        for a in range(D):  # box index = i0 + N0 (i1 + N1 i2): axis 0 fastest
+            phi = (vals[a][:, :, None] * phi[:, None, :]).reshape(P, -1)
+        grads = []
+        jinv = self.inverse_jacobian[self.item_of_point]
+        for ax in range(D):
+            g = np.ones((P, 1))
+            for a in range(D):
+                f = ders[a] if a == ax else vals[a]
+                g = (f[:, :, None] * g[:, None, :]).reshape(P, -1)
+            grads.append(g * jinv[:, ax:ax + 1])
+        return phi, np.stack(grads)
+
+    def interpolate(self, coefficients_box, derivative_axis=None):
+        """(items, N box) -> (P,) values at the points (host)."""
+        phi, grads = self.tables()
+        t = phi if derivative_axis is None else grads[derivative_axis]
+        return np.einsum("pn,pn->p", t, coefficients_box[self.item_of_point])
+
+    def project(self, point_values, derivative_axis=None):
+        """(P,) -> (items, N box): sum_q f_q phi_i(q) per item (host)."""
+        phi, grads = self.tables()
+        t = phi if derivative_axis is None else grads[derivative_axis]
+        out = np.zeros((self.cell_count, self.n_modes))
+        np.add.at(out, self.item_of_point, t * point_values[:, None])
+        return out
+
+
 class AssemblyPlan:
     """Frozen host tables plus the device plan handle."""
 
@@ -216,7 +331,13 @@ class AssemblyPlan:
         self.dim = space.dim
         self.full_dof_count = space.dof_count
         keys = {}
+        custom = {}
+        cq = getattr(problem, "cell_quadratures", {})
         for li in range(len(space.leaves)):
+            leaf = int(space.leaves[li])
+            if leaf in cq:
+                custom.setdefault(tuple(space.degrees[li]), []).append(li)
+                continue
             key = (tuple(space.degrees[li]), tuple(space.quadrature_shapes[li]))
             keys.setdefault(key, []).append(li)
         self.buckets = []
@@ -228,6 +349,26 @@ class AssemblyPlan:
             else:
                 b.form, b.point_data = FORM_DIFFUSION, np.zeros((b.cell_count, b.n_points))
             self.buckets.append(b)
+        # Cells with their own rule (cut cells) and embedded-surface pieces.
+        self.cloud_buckets = []
+        for deg, lis in custom.items():
+            rules = [cq[int(space.leaves[li])] for li in lis]
+            cb = CloudBucket(space, lis, deg, rules)
+            cb.lower(ops)
+            self.cloud_buckets.append(cb)
+        sops = [op for (_, _, op) in getattr(problem, "surface_operators", [])]
+        surf = {}
+        for sq in getattr(problem, "surface_quadratures", []):
+            if sq.leaf not in space.leaf_index:
+                raise ValueError(f"surface quadrature leaf {sq.leaf} is not an active leaf of the space")
+            li = space.leaf_index[sq.leaf]
+            surf.setdefault(tuple(space.degrees[li]), []).append((li, sq))
+        self.surface_buckets = []
+        for deg, items in surf.items():
+            sb = CloudBucket(space, [li for li, _ in items], deg, [sq.quadrature for _, sq in items],
+                             normals=[sq.normals for _, sq in items])
+            sb.lower(sops, surface=True)
+            self.surface_buckets.append(sb)
         self._compile_dirichlet()
         self._compile_reduced_map()
         self._rhs = None
@@ -299,19 +440,30 @@ class AssemblyPlan:
         self.shift[self.fixed_dofs] = self.fixed_values
 
     # Host-side right-hand side (src/assembly.jl:488-507 rhs!).
+    @property
+    def all_buckets(self):
+        return list(self.buckets) + list(self.cloud_buckets) + list(self.surface_buckets)
+
     def rhs_physical(self):
+        """Host right-hand side: cell loads on tensor and cut cells, surface
+        loads on embedded pieces (src/assembly.jl:488-507, 586-601)."""
         if self._rhs is None:
             b = np.zeros(self.full_dof_count)
-            for (_, op) in self.problem.cell_rhs:
-                for bk in self.buckets:
-                    shape = (bk.cell_count, bk.n_points)
-                    v, g = _channels(op.cell_rhs_accumulate(bk.quadrature(), 0), shape, bk.dim)
-                    local = bk.project(bk.weights * v)
-                    for a in range(bk.dim):
-                        if np.any(g[a] != 0):
-                            local = local + bk.project(bk.weights * g[a], derivative_axis=a)
-                    contrib = bk.term_coef * local[bk.term_cell, bk.term_box]
-                    b += np.bincount(bk.term_dof, weights=contrib, minlength=self.full_dof_count)
+            groups = [(self.problem.cell_rhs, list(self.buckets) + list(self.cloud_buckets), False),
+                      (getattr(self.problem, "surface_rhs", []), list(self.surface_buckets), True)]
+            for ops, buckets, surface in groups:
+                for (_, op) in ops:
+                    for bk in buckets:
+                        cloud = getattr(bk, "cloud", False)
+                        shape = (bk.n_points_total,) if cloud else (bk.cell_count, bk.n_points)
+                        call = op.surface_rhs_accumulate if surface else op.cell_rhs_accumulate
+                        v, g = _channels(call(bk.quadrature(), 0), shape, bk.dim)
+                        local = bk.project(bk.weights * v)
+                        for a in range(bk.dim):
+                            if np.any(g[a] != 0):
+                                local = local + bk.project(bk.weights * g[a], derivative_axis=a)
+                        contrib = bk.term_coef * local[bk.term_cell, bk.term_box]
+                        b += np.bincount(bk.term_dof, weights=contrib, minlength=self.full_dof_count)
             self._rhs = b
         return self._rhs
 

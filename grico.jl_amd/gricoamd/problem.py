@@ -61,12 +61,14 @@ def gradient(channels):
 
 
 class QuadraturePoints:
-    """Vectorized stand-in for the reference's per-point handle."""
+    """Vectorized stand-in for the reference's per-point handle (surface
+    points also carry the physical unit `normal`)."""
 
-    def __init__(self, points, weights, cell_sizes):
+    def __init__(self, points, weights, cell_sizes, normal=None):
         self.point = tuple(points)
         self.weight = weights
         self.cell_size = tuple(cell_sizes)
+        self.normal = None if normal is None else tuple(normal)
 
     def coordinate(self, axis):
         return self.point[axis]
@@ -89,6 +91,43 @@ class Dirichlet:
         self.data = data
 
 
+class PointQuadrature:
+    """Explicit quadrature on the biunit reference cell [-1, 1]^D (src/quadrature.jl
+    PointQuadrature): points (n x D) and weights (n)."""
+
+    def __init__(self, points, weights):
+        self.points = np.ascontiguousarray(np.atleast_2d(np.asarray(points, dtype=np.float64)))
+        self.weights = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))
+        if self.points.shape[0] != self.weights.shape[0]:
+            raise ValueError("point and weight data must match")
+        if not (np.all(np.isfinite(self.points)) and np.all(np.isfinite(self.weights))):
+            raise ValueError("quadrature data must be finite")
+
+    def point_count(self):
+        return self.weights.shape[0]
+
+
+class SurfaceQuadrature:
+    """Embedded-surface piece inside one leaf (src/embedded.jl:33-48): reference
+    points and reference-surface weights plus one reference-space unit normal per
+    point."""
+
+    def __init__(self, leaf, quadrature, normals):
+        if int(leaf) < 0:
+            raise ValueError("leaf must be a cell id")
+        self.leaf = int(leaf)
+        self.quadrature = quadrature
+        n = np.atleast_2d(np.asarray(normals, dtype=np.float64))
+        if n.shape[0] == 1 and quadrature.point_count() > 1:
+            n = np.repeat(n, quadrature.point_count(), axis=0)
+        if n.shape != quadrature.points.shape:
+            raise ValueError("one normal per surface quadrature point is required")
+        norms = np.linalg.norm(n, axis=1)
+        if np.any(norms <= 0) or not np.all(np.isfinite(norms)):
+            raise ValueError("surface normals must be finite and nonzero")
+        self.normals = np.ascontiguousarray(n / norms[:, None])
+
+
 class AffineProblem:
     def __init__(self, field, operator_class=None):
         self.field = field
@@ -96,6 +135,41 @@ class AffineProblem:
         self.cell_operators = []   # (test_field, trial_field, operator)
         self.cell_rhs = []         # (test_field, operator)
         self.constraints = []
+        self.cell_quadratures = {}     # leaf -> PointQuadrature
+        self.surface_quadratures = []  # SurfaceQuadrature
+        self.surface_operators = []    # (test_field, trial_field, operator)
+        self.surface_rhs = []          # (test_field, operator)
+
+    def add_cell_quadrature(self, leaf, quadrature):
+        """Replace the tensor rule of one leaf (src/embedded.jl:203-208)."""
+        if quadrature.points.shape[1] != self.field.space.dim:
+            raise ValueError("quadrature dimension must match the space")
+        self.cell_quadratures[int(leaf)] = quadrature
+        return self
+
+    def add_surface_quadrature(self, surface):
+        """Attach one embedded-surface piece (src/embedded.jl:181-186)."""
+        if surface.quadrature.points.shape[1] != self.field.space.dim:
+            raise ValueError("surface quadrature dimension must match the space")
+        self.surface_quadratures.append(surface)
+        return self
+
+    def add_surface_accumulator(self, *args):
+        """surface_accumulate(q, trial, test_component) / surface_rhs_accumulate
+        operators (src/accumulators.jl:435-466)."""
+        if len(args) == 3:
+            test, trial, op = args
+            if test is not self.field or trial is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            self.surface_operators.append((test, trial, op))
+        elif len(args) == 2:
+            test, op = args
+            if test is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            self.surface_rhs.append((test, op))
+        else:
+            raise TypeError("add_surface_accumulator(test, trial, op) or (test, op)")
+        return self
 
     def add_cell_accumulator(self, *args):
         if len(args) == 3:
@@ -125,3 +199,15 @@ def add_cell_accumulator(problem, *args):
 
 def add_constraint(problem, constraint):
     return problem.add_constraint(constraint)
+
+
+def add_cell_quadrature(problem, leaf, quadrature):
+    return problem.add_cell_quadrature(leaf, quadrature)
+
+
+def add_surface_quadrature(problem, surface):
+    return problem.add_surface_quadrature(surface)
+
+
+def add_surface_accumulator(problem, *args):
+    return problem.add_surface_accumulator(*args)

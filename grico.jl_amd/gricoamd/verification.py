@@ -20,7 +20,7 @@ def _field_values(plan, coefficients, bucket):
 def l2_error(state, exact):
     plan = state.plan
     total = 0.0
-    for b in plan.buckets:
+    for b in list(plan.buckets) + list(getattr(plan, "cloud_buckets", [])):
         uh = _field_values(plan, state.coefficients, b)
         ue = np.broadcast_to(np.asarray(exact(tuple(b.phys_points)), dtype=np.float64), uh.shape)
         total += float(np.sum(b.weights * (uh - ue) ** 2))
@@ -31,7 +31,7 @@ def relative_l2_error(state, exact):
     plan = state.plan
     num = 0.0
     den = 0.0
-    for b in plan.buckets:
+    for b in list(plan.buckets) + list(getattr(plan, "cloud_buckets", [])):
         uh = _field_values(plan, state.coefficients, b)
         ue = np.broadcast_to(np.asarray(exact(tuple(b.phys_points)), dtype=np.float64), uh.shape)
         num += float(np.sum(b.weights * (uh - ue) ** 2))

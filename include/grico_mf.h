@@ -80,6 +80,15 @@ typedef struct gmf_bucket_desc {
   const int64_t* multi_offsets; /* multi_count + 1 */
   const int32_t* multi_indices;
   const double* multi_coefficients;
+  /* Point-cloud batches (cut cells with their own quadrature, embedded surface
+   * pieces; src/accumulators.jl:1022-1054 basis path, 1133-1214 one-sided
+   * surface apply): when point_offsets is not NULL, item c owns points
+   * point_offsets[c] .. point_offsets[c+1]-1 with biunit reference coordinates
+   * ref_points (total x dim, row-major) and point_data rows (total x (D+1)^2,
+   * GENERAL form, weights included); `points`, `values` and `derivatives` are
+   * then ignored and the basis is evaluated at the points on the device. */
+  const int64_t* point_offsets;
+  const double* ref_points;
 } gmf_bucket_desc;
 
 /* Compiled plan: cell batches + algebraic constraint rows + reduced map

@@ -130,6 +130,71 @@ def _test_channels(result, shape, D):
     return np.broadcast_to(np.asarray(result, dtype=np.float64), shape), [np.zeros(shape)] * D
 
 
+def _cloud_groups(plan):
+    """(bucket, operators, surface) for every point-cloud batch of the plan."""
+    cells = [op for (_, _, op) in plan.problem.cell_operators]
+    surf = [op for (_, _, op) in getattr(plan.problem, "surface_operators", [])]
+    return ([(b, cells, False) for b in getattr(plan, "cloud_buckets", [])] +
+            [(b, surf, True) for b in getattr(plan, "surface_buckets", [])])
+
+
+def _cloud_apply(bk, ops, surface, x, y, fixed):
+    """Basis path of the reference (src/accumulators.jl:1022-1054, one-sided
+    surface items 1133-1161): gather, full tables at the points, the callback
+    on the actual trial channels, weighted projection, scatter skipping fixed
+    rows. Host numpy."""
+    D = bk.dim
+    phi, grads = bk.tables()
+    loc = np.zeros((bk.cell_count, bk.n_modes))
+    np.add.at(loc, (bk.term_cell, bk.term_box), bk.term_coef * x[bk.term_dof])
+    up = loc[bk.item_of_point]
+    val = np.einsum("pn,pn->p", phi, up)
+    grad = [np.einsum("pn,pn->p", grads[a], up) for a in range(D)]
+    q = bk.quadrature()
+    P = bk.n_points_total
+    a0 = np.zeros(P)
+    ag = np.zeros((D, P))
+    for op in ops:
+        call = op.surface_accumulate if surface else op.cell_accumulate
+        v, g = _test_channels(call(q, _trial(val, grad), 0), (P,), D)
+        a0 += bk.weights * v
+        for a in range(D):
+            ag[a] += bk.weights * g[a]
+    r = phi * a0[:, None]
+    for a in range(D):
+        r = r + grads[a] * ag[a][:, None]
+    rloc = np.zeros((bk.cell_count, bk.n_modes))
+    np.add.at(rloc, bk.item_of_point, r)
+    contrib = bk.term_coef * rloc[bk.term_cell, bk.term_box]
+    keep = np.ones(len(contrib), dtype=bool) if fixed is None else fixed[bk.term_dof] == 0
+    np.add.at(y, bk.term_dof[keep], contrib[keep])
+
+
+def _cloud_diagonal(bk, ops, surface, d):
+    """Local diagonal sum_q sum_{c,k} W M_q[c][k] chi_c chi_k scattered with c^2."""
+    D = bk.dim
+    phi, grads = bk.tables()
+    q = bk.quadrature()
+    P = bk.n_points_total
+    ones, zeros = np.ones(P), np.zeros(P)
+    M = np.zeros((P, D + 1, D + 1))
+    for op in ops:
+        call = op.surface_accumulate if surface else op.cell_accumulate
+        for j in range(D + 1):
+            v, g = _test_channels(call(q, _trial(ones if j == 0 else zeros,
+                                                 [ones if j == a + 1 else zeros for a in range(D)]), 0), (P,), D)
+            M[:, 0, j] += v
+            for a in range(D):
+                M[:, a + 1, j] += g[a]
+    M *= bk.weights[:, None, None]
+    chi = np.concatenate([phi[None], grads], axis=0)  # (D+1, P, N)
+    dl = np.einsum("pck,cpn,kpn->pn", M, chi, chi)
+    rloc = np.zeros((bk.cell_count, bk.n_modes))
+    np.add.at(rloc, bk.item_of_point, dl)
+    contrib = bk.term_coef ** 2 * rloc[bk.term_cell, bk.term_box]
+    np.add.at(d, bk.term_dof, contrib)
+
+
 def apply(plan, x, physical=False):
     """Reference-algorithm full-layout apply (callback evaluated on the actual
     trial channels). physical=True skips the constraint handling."""
@@ -156,6 +221,8 @@ def apply(plan, x, physical=False):
         ag = np.ascontiguousarray(ag)
         L.oracle_project(ob.ref(), _p(a0, ctypes.c_double), _p(ag, ctypes.c_double),
                          _p(fixed, ctypes.c_uint8), _p(y, ctypes.c_double))
+    for bk, ops, surface in _cloud_groups(plan):
+        _cloud_apply(bk, ops, surface, x, y, fixed)
     if not physical:
         fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
         L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
@@ -194,6 +261,8 @@ def apply_lowered(plan, x, threads=0, matrices=None, colors=None, physical=False
         L.oracle_apply_lowered(ob.ref(), _p(M, ctypes.c_double), _p(x, ctypes.c_double), _p(fixed, ctypes.c_uint8),
                                _p(y, ctypes.c_double), _p(co, ctypes.c_int64), _p(cc, ctypes.c_int64),
                                len(co) - 1, int(threads))
+    for bk, ops, surface in _cloud_groups(plan):
+        _cloud_apply(bk, ops, surface, x, y, fixed)
     if not physical:
         fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
         L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
@@ -223,6 +292,8 @@ def apply_basis(plan, x, physical=False):
         M = action_matrices(plan, bk)
         L.oracle_apply_basis(ob.ref(), _p(M, ctypes.c_double), _p(x, ctypes.c_double), _p(fixed, ctypes.c_uint8),
                              _p(y, ctypes.c_double))
+    for bk, ops, surface in _cloud_groups(plan):
+        _cloud_apply(bk, ops, surface, x, y, fixed)
     if not physical:
         fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
         L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
@@ -236,6 +307,8 @@ def diagonal_physical(plan):
         ob = _Bucket(bk)
         M = action_matrices(plan, bk)
         L.oracle_diagonal(ob.ref(), _p(M, ctypes.c_double), _p(d, ctypes.c_double))
+    for bk, ops, surface in _cloud_groups(plan):
+        _cloud_diagonal(bk, ops, surface, d)
     return d
 
 
