@@ -285,7 +285,12 @@ class CloudBucket:
 
     def tables(self):
         """Full basis tables at the points: values (P x N box) and physical
-        gradients (D, P, N box), axis 1 fastest in the box."""
+        gradients (D, P, N box), axis 1 fastest in the box (cached)."""
+        if getattr(self, "_tables", None) is None:
+            self._tables = self._build_tables()
+        return self._tables
+
+    def _build_tables(self):
         D = self.dim
         vals, ders = [], []
         for a in range(D):

@@ -151,3 +151,51 @@ class LogNormalKappa:
 def lognormal_poisson(depth=2, degree=3, seed=0, dim=3):
     """The random-coefficient Poisson problem (p = 3: 64 local modes per cell)."""
     return poisson(dim, None, degree, refine_depth=depth, kappa=LogNormalKappa(seed, dim=dim))
+
+
+class BallLoad:
+    """-lap(1 - r^2) = 2 D."""
+
+    def __init__(self, dim=3):
+        self.dim = dim
+
+    def cell_rhs_accumulate(self, q, test_component):
+        return 2.0 * self.dim + 0.0 * q.point[0]
+
+
+class NitscheDirichlet:
+    """Symmetric Nitsche terms for u = 0 on the embedded surface:
+    -(du/dn) v - u (dv/dn) + gamma / h u v."""
+
+    def __init__(self, gamma):
+        self.gamma = gamma
+
+    def surface_accumulate(self, q, trial, test_component):
+        n = q.normal
+        dn = sum(nc * gc for nc, gc in zip(n, trial.gradient))
+        return g.TestChannels(-dn + self.gamma / q.cell_size[0] * trial.value,
+                              tuple(-trial.value * nc for nc in n))
+
+
+def ball_exact(p):
+    return 1.0 - (p[0] ** 2 + p[1] ** 2 + p[2] ** 2)
+
+
+def fcm_ball(n=8, degree=2, depth=2, gamma=None, n_theta=48, n_phi=96, alpha=1e-5):
+    """Finite-cell Poisson on the unit ball in [-1.2, 1.2]^3: n^3 background
+    cells, cut cells with subdivision rules, Nitsche on the sphere, exact
+    solution 1 - r^2."""
+    from gricoamd.embedded import ImplicitRegion, add_cut_cell_quadratures, physical_space, \
+        sphere_surface_quadratures
+    region = ImplicitRegion(lambda p: p[0] ** 2 + p[1] ** 2 + p[2] ** 2 - 1.0, subdivision_depth=depth)
+    dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (n, n, n))
+    space, cut = physical_space(dom, region, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, Diffusion())
+    pr.add_cell_accumulator(u, BallLoad(3))
+    add_cut_cell_quadratures(pr, region, cut, alpha=alpha)
+    for sq in sphere_surface_quadratures(space, (0.0, 0.0, 0.0), 1.0, n_theta=n_theta, n_phi=n_phi):
+        pr.add_surface_quadrature(sq)
+    pr.add_surface_accumulator(u, u, NitscheDirichlet(gamma if gamma is not None else 10.0 * (degree + 1) ** 2))
+    return pr

@@ -1,0 +1,51 @@
+"""Finite-cell Poisson on the unit ball (the reduced C5 configuration): cut
+cells with subdivision rules and the finite-cell extension, Nitsche on the
+sphere, exact solution 1 - r^2. Device apply against the oracle's basis path,
+Jacobi-CG solves against the exact solution and the oracle's dense operator."""
+
+import numpy as np
+import pytest
+
+import gricoamd as g
+import pyoracle
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ball6(gpu):
+    return g.compile(P.fcm_ball(n=6, degree=2, depth=2))
+
+
+def test_apply_matches_oracle(ball6):
+    plan = ball6
+    assert plan.cloud_buckets and plan.surface_buckets
+    x = P.rng_vector(plan.full_dof_count, 5)
+    y = g.apply(plan, x)
+    ref = pyoracle.apply(plan, x)
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_solve_exact_solution(ball6):
+    """Jacobi-CG to 1e-10 on the SPD Nitsche system: relative L2 error of the
+    quadratic exact solution within 2% (the oracle's dense solve: 1.07%)."""
+    st = g.solve(ball6, solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()), relative_tolerance=1e-10,
+                 maxiter=200000)
+    assert st.info.converged == 1
+    assert g.relative_l2_error(st, P.ball_exact) < 2e-2
+
+
+def test_batched_random_rhs_residuals(gpu):
+    """Several seeded random right-hand sides on the 4^3 ball: every device
+    CG solution has a small residual against the oracle's dense operator."""
+    plan = g.compile(P.fcm_ball(n=4, degree=2, depth=2))
+    A = pyoracle.reduced_matrix(plan)
+    for seed in range(4):
+        b = P.rng_vector(plan.reduced_dof_count, 100 + seed)
+        x, info = g.solve_reduced(plan, g.CGSolver(preconditioner=g.JacobiPreconditioner()), b,
+                                  relative_tolerance=1e-11, maxiter=200000)
+        x = x.cpu().numpy()
+        assert info.converged == 1
+        assert np.linalg.norm(b - A @ x) <= 1e-9 * np.linalg.norm(b)
