@@ -43,8 +43,6 @@ namespace gmf {
 namespace {
 
 constexpr int kXcds = 8;
-// Workgroups of one persistent brick launch: four per CU on 256 CUs.
-constexpr int64_t kBrickGrid = 1024;
 
 __device__ inline int64_t xcd_block(int64_t b, int64_t n) {
   const int64_t q = n / kXcds, r = n % kXcds;
@@ -217,86 +215,47 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T
 
 // ---- brick driver --------------------------------------------------------------
 
+// Lattice entry kinds (low two bits of a brick map entry).
+constexpr int kLatRmw = 0, kLatFirst = 1, kLatCorner = 2;
+
 struct BrickArgs {
   const int64_t* order;   // bricks of this launch (one color), Morton order
   int64_t count;          // bricks in this launch
-  // bricks x L lattice entries: (dof << 1) | first, -1 for no dof; `first`
-  // is set where this brick is the first writer of the dof.
+  // bricks x L lattice entries: (dof << 2) | kind, -1 for no dof.
   const int32_t* map;
   const double* jinv;     // cells x 3
   const double* coef;     // cells x N^3
+  double* slots;          // bricks x 8 corner results
   int store_first;        // 0: always read-modify-write (y pre-zeroed)
 };
 
 // Lattice offset of local mode m along one axis of a cell: vertex 0 at 0,
 // interior modes 2..N-1 at 1..N-2, vertex 1 at N-1.
 template <int N>
-__device__ inline int lat(int m) {
+__host__ __device__ inline int lat(int m) {
   return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1);
 }
 
-// Registers of one brick in flight: its lattice entries, input values, the
-// current result values of its read-modify-write points, and the x-pencil
-// coefficients and Jacobian of this lane's cell.
-template <int N>
-struct BrickStage {
-  static constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * lanes_per_cell<N>();
-  static constexpr int PER = (L + THREADS - 1) / THREADS;
-  int32_t g[PER];
-  double xv[PER];
-  double yold[PER];
-  double cx[N];
-  double jx, jy, jz;
-};
-
-template <int N>
-__device__ inline void brick_map(const BrickArgs& a, int64_t brick, int tid, int32_t (&m)[BrickStage<N>::PER]) {
-  using S = BrickStage<N>;
-  const int32_t* map = a.map + brick * S::L;
-#pragma unroll
-  for (int k = 0; k < S::PER; ++k) {
-    const int p = tid + k * S::THREADS;
-    m[k] = p < S::L ? map[p] : -1;
-  }
+// Corner index (x bit 0, y bit 1, z bit 2) of a lattice position of an
+// M^3 brick lattice, -1 if the position is not a brick corner.
+__host__ __device__ inline int lattice_corner(int pos, int M) {
+  const int i = pos % M, j = (pos / M) % M, k = pos / (M * M);
+  const bool c = (i == 0 || i == M - 1) && (j == 0 || j == M - 1) && (k == 0 || k == M - 1);
+  return c ? int(i != 0) | (int(j != 0) << 1) | (int(k != 0) << 2) : -1;
 }
 
-// Issue the dependent loads of one brick from its lattice entries. The y
-// values of a brick are stable for the whole launch: bricks of one color are
-// DOF-disjoint and earlier colors ran in earlier launches.
-template <int N>
-__device__ inline void brick_stage(const BrickArgs& a, int64_t brick, const int32_t (&m)[BrickStage<N>::PER], int lc,
-                                   int t0, int t1, const double* __restrict__ x, const double* __restrict__ y,
-                                   BrickStage<N>& s) {
-  constexpr int NN = N * N, NNN = N * N * N;
-  using S = BrickStage<N>;
-  const int64_t cell = brick * 8 + lc;
-  const int64_t base = cell * NNN;
-#pragma unroll
-  for (int q = 0; q < N; ++q) s.cx[q] = a.coef[base + q + N * t0 + NN * t1];
-  s.jx = a.jinv[cell * 3];
-  s.jy = a.jinv[cell * 3 + 1];
-  s.jz = a.jinv[cell * 3 + 2];
-#pragma unroll
-  for (int k = 0; k < S::PER; ++k) {
-    const int32_t e = m[k];
-    s.g[k] = e;
-    const int32_t d = e >> 1;
-    s.xv[k] = e >= 0 ? x[d] : 0.0;
-    s.yold[k] = (e >= 0 && !(a.store_first && (e & 1))) ? y[d] : 0.0;
-  }
-}
-
-// Persistent, software-pipelined brick kernel: each workgroup walks a
-// contiguous chunk of the color's bricks. While brick i computes, the loads
-// of brick i+1 (inputs, results to update, coefficients) and the lattice
-// entries of brick i+2 are in flight.
+// One workgroup per brick of one color. The coefficient pencils and the
+// lattice entries are loaded first (independent of each other), then the
+// inputs and the current results of the read-modify-write points (dependent
+// on the entries); the y values are stable for the launch because bricks of
+// one color share no dof outside the brick corners, and corners go to the
+// brick's own slots.
 template <int N>
 __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs a, EOTables<N> T,
                                                                         const double* __restrict__ x,
                                                                         double* __restrict__ y) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
-  using S = BrickStage<N>;
-  constexpr int M = S::M, L = S::L, THREADS = S::THREADS, PER = S::PER;
+  constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC, PER = (L + THREADS - 1) / THREADS;
   extern __shared__ double smem[];
   double* xl = smem;           // L input modes
   double* acc = xl + L;        // L accumulated results
@@ -306,65 +265,75 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   int t0, t1;
   bool act;
   lane_coords<N>(tid % LPC, t0, t1, act);
-  const int64_t nblk = gridDim.x;
-  const int64_t blk = xcd_block(blockIdx.x, nblk);
-  const int64_t per_blk = (a.count + nblk - 1) / nblk;
-  const int64_t lo = blk * per_blk;
-  const int64_t hi = lo + per_blk < a.count ? lo + per_blk : a.count;
-  if (lo >= hi) return;  // uniform across the workgroup
+  const int64_t brick = a.order[xcd_block(blockIdx.x, gridDim.x)];
   // Cell position inside the brick (Morton child order: x fastest).
   const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
-  const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
-  const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
-  double* A = scratch + lc * 2 * NNN;
-
-  int32_t m[PER];
-  S cur, nxt;
-  brick_map<N>(a, a.order[lo], tid, m);
-  brick_stage<N>(a, a.order[lo], m, lc, t0, t1, x, y, cur);
-  if (lo + 1 < hi) brick_map<N>(a, a.order[lo + 1], tid, m);
-  for (int64_t i = lo; i < hi; ++i) {
-    const int64_t brick = a.order[i];
+  const int64_t cell = brick * 8 + lc;
+  const double* cb = a.coef + cell * NNN;
+  double cx[N], cy[N], cz[N];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int p = tid + k * THREADS;
-      if (p < L) {
-        xl[p] = cur.xv[k];
-        acc[p] = 0.0;
-      }
-    }
-    __syncthreads();
-    // y/z-pencil coefficients of this brick (cache hits: the x-pencil load of
-    // the same lines was issued one brick earlier), then the prefetches.
-    double cy[N], cz[N];
-    {
-      const double* cb = a.coef + (brick * 8 + lc) * NNN;
+  for (int q = 0; q < N; ++q) {
+    cx[q] = cb[q + N * t0 + NN * t1];
+    cy[q] = cb[t0 + N * q + NN * t1];
+    cz[q] = cb[t0 + N * t1 + NN * q];
+  }
+  const double jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
+  const int32_t* map = a.map + brick * L;
+  int32_t e[PER];
 #pragma unroll
-      for (int q = 0; q < N; ++q) {
-        cy[q] = cb[t0 + N * q + NN * t1];
-        cz[q] = cb[t0 + N * t1 + NN * q];
-      }
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    e[k] = p < L ? map[p] : -1;
+  }
+  double yold[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    const int32_t d = e[k] >> 2, kind = e[k] & 3;
+    const bool read = e[k] >= 0 && kind != kLatCorner && !(a.store_first && kind == kLatFirst);
+    yold[k] = read ? y[d] : 0.0;
+    if (p < L) {
+      xl[p] = e[k] >= 0 ? x[d] : 0.0;
+      acc[p] = 0.0;
     }
-    if (i + 1 < hi) brick_stage<N>(a, a.order[i + 1], m, lc, t0, t1, x, y, nxt);
-    if (i + 2 < hi) brick_map<N>(a, a.order[i + 2], tid, m);
-    double u[N], r[N];
+  }
+  __syncthreads();
+  double u[N], r[N];
+  {
+    const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
 #pragma unroll
     for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
-    cell_body<N>(T, u, cur.cx, cy, cz, cur.jx * cur.jx, cur.jy * cur.jy, cur.jz * cur.jz, A, A + NNN, t0, t1, act,
-                 r);
-    if (act) {
-#pragma unroll
-      for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[ox + lat<N>(ii) + M * (py + M * pz)], r[ii]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int p = tid + k * THREADS;
-      if (cur.g[k] >= 0) y[cur.g[k] >> 1] = cur.yold[k] + acc[p];
-    }
-    __syncthreads();
-    cur = nxt;
   }
+  double* A = scratch + lc * 2 * NNN;
+  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, A + NNN, t0, t1, act, r);
+  if (act) {
+    const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
+#pragma unroll
+    for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[ox + lat<N>(ii) + M * (py + M * pz)], r[ii]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int p = tid + k * THREADS;
+    if (e[k] < 0) continue;
+    if ((e[k] & 3) == kLatCorner)
+      a.slots[brick * 8 + lattice_corner(p, M)] = acc[p];
+    else
+      y[e[k] >> 2] = yold[k] + acc[p];
+  }
+}
+
+// Brick-corner dofs: y[d] (+)= sum of the corner slots that hold d, in slot
+// order (deterministic).
+__global__ void k_brick_corners(double* __restrict__ y, const double* __restrict__ slots, const int32_t* dof,
+                                const int64_t* ptr, const int32_t* slot, int64_t count, int accumulate) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= count) return;
+  const int32_t d = dof[i];
+  if (d < 0) return;
+  double s = 0.0;
+  for (int64_t j = ptr[i]; j < ptr[i + 1]; ++j) s += slots[slot[j]];
+  y[d] = accumulate ? y[d] + s : s;
 }
 
 template <int N>
@@ -397,9 +366,12 @@ size_t brick_lds_bytes() {
 }
 
 template <int N>
-void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bool store_first, hipStream_t stream) {
+void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bool store_first, double* slots,
+                   hipStream_t stream) {
   EOTables<N> T = tables_of<N>(b);
   const BrickPath& bp = b.bricks;
+  require(bp.corner_count == 0 || slots != nullptr, GMF_EINVAL, "workspace has no brick corner slots");
+  double* bslots = slots ? slots + bp.slot_base : nullptr;
   for (int c = 0; c < bp.colors; ++c) {
     const int64_t lo = bp.color_offsets[size_t(c)], hi = bp.color_offsets[size_t(c) + 1];
     if (hi == lo) continue;
@@ -409,11 +381,15 @@ void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bo
     a.map = reduced ? bp.map_red : bp.map_full;
     a.jinv = b.jinv;
     a.coef = b.pdata;
+    a.slots = bslots;
     a.store_first = store_first ? 1 : 0;
-    const int64_t grid = std::min<int64_t>(hi - lo, kBrickGrid);
-    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(grid)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
+    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
                        stream, a, T, x, y);
   }
+  if (bp.corner_count > 0)
+    hipLaunchKernelGGL(k_brick_corners, dim3(unsigned((bp.corner_count + 255) / 256)), dim3(256), 0, stream, y, bslots,
+                       reduced ? bp.corner_red : bp.corner_full, bp.corner_ptr, bp.corner_slot, bp.corner_count,
+                       store_first ? 0 : 1);
   GMF_HIP(hipGetLastError());
 }
 
@@ -453,14 +429,21 @@ std::vector<double> collocation_derivative(const std::vector<double>& B, const s
 // consistently (cells sharing a lattice point agree on its dof) and every
 // dof may appear at most once per brick. Returns false when the cells do not
 // form such bricks.
+//
+// Dofs that sit only at brick corners (a vertex shared by up to eight
+// bricks) are summed through per-brick corner slots by k_brick_corners, so
+// two bricks conflict only when they share another dof. On a brick grid that
+// leaves four colors (each brick with the brick diagonally opposite across a
+// corner) instead of eight, i.e. four dependent launches per pass.
 bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full) {
   const int N = b.N[0];
   const int M = 2 * N - 1, L = M * M * M;
   const int NNN = N * N * N;
   if (b.cells % 8 != 0 || b.cells == 0) return false;
   const int64_t nb = b.cells / 8;
+  if (nb > (int64_t(1) << 27) || p.n_full >= (int64_t(1) << 29)) return false;
   std::vector<int32_t> map(size_t(nb) * size_t(L));
-  auto lat = [N](int m) { return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1); };
+  auto latp = [N](int m) { return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1); };
   std::vector<int32_t> lattice(static_cast<size_t>(L));
   std::vector<int64_t> seen_brick(size_t(p.n_full), -1);
   for (int64_t br = 0; br < nb; ++br) {
@@ -473,7 +456,7 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
           for (int i = 0; i < N; ++i) {
             const int32_t g = li[i + N * j + N * N * k];
             if (g < -1) return false;
-            const int pos = (ox + lat(i)) + M * ((oy + lat(j)) + M * (oz + lat(k)));
+            const int pos = (ox + latp(i)) + M * ((oy + latp(j)) + M * (oz + latp(k)));
             if (lattice[size_t(pos)] == -2)
               lattice[size_t(pos)] = g;
             else if (lattice[size_t(pos)] != g)
@@ -490,7 +473,17 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
       map[size_t(br * L + pos)] = g;
     }
   }
-  // Greedy DOF-disjoint coloring of bricks in Morton order.
+  // Corner-only dofs: every occurrence at a brick corner.
+  std::vector<int> corner_of(static_cast<size_t>(L));
+  for (int pos = 0; pos < L; ++pos) corner_of[size_t(pos)] = lattice_corner(pos, M);
+  std::vector<uint8_t> corner_only(size_t(p.n_full), 1);
+  for (int64_t br = 0; br < nb; ++br)
+    for (int pos = 0; pos < L; ++pos) {
+      const int32_t g = map[size_t(br * L + pos)];
+      if (g >= 0 && corner_of[size_t(pos)] < 0) corner_only[size_t(g)] = 0;
+    }
+  // Greedy coloring of bricks in Morton order: bricks sharing a dof that is
+  // not corner-only get different colors.
   std::vector<uint64_t> used(size_t(p.n_full), 0);
   std::vector<int> color(static_cast<size_t>(nb));
   int ncolors = 0;
@@ -498,7 +491,7 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
     uint64_t mask = 0;
     for (int pos = 0; pos < L; ++pos) {
       const int32_t g = map[size_t(br * L + pos)];
-      if (g >= 0) mask |= used[size_t(g)];
+      if (g >= 0 && !corner_only[size_t(g)]) mask |= used[size_t(g)];
     }
     int c = 0;
     while (c < 64 && ((mask >> c) & 1u)) ++c;
@@ -507,19 +500,26 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
     ncolors = std::max(ncolors, c + 1);
     for (int pos = 0; pos < L; ++pos) {
       const int32_t g = map[size_t(br * L + pos)];
-      if (g >= 0) used[size_t(g)] |= uint64_t(1) << c;
+      if (g >= 0 && !corner_only[size_t(g)]) used[size_t(g)] |= uint64_t(1) << c;
     }
   }
-  // First writer of each dof: the lowest color touching it.
-  std::vector<uint8_t> first(size_t(nb) * size_t(L), 0);
+  // Entry kinds: corner slot, first writer (the lowest color touching the
+  // dof), or read-modify-write.
+  std::vector<uint8_t> kind(size_t(nb) * size_t(L), uint8_t(kLatRmw));
+  std::vector<std::pair<int32_t, int32_t>> corners;  // (dof, slot)
   for (int64_t br = 0; br < nb; ++br)
     for (int pos = 0; pos < L; ++pos) {
       const int32_t g = map[size_t(br * L + pos)];
       if (g < 0) continue;
+      if (corner_only[size_t(g)]) {
+        kind[size_t(br * L + pos)] = uint8_t(kLatCorner);
+        corners.emplace_back(g, int32_t(br * 8 + corner_of[size_t(pos)]));
+        continue;
+      }
       const uint64_t m = used[size_t(g)];
       int lowest = 0;
       while (!((m >> lowest) & 1u)) ++lowest;
-      first[size_t(br * L + pos)] = uint8_t(color[size_t(br)] == lowest);
+      if (color[size_t(br)] == lowest) kind[size_t(br * L + pos)] = uint8_t(kLatFirst);
     }
   BrickPath& bp = b.bricks;
   bp.colors = ncolors;
@@ -531,17 +531,38 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
   for (int64_t br = 0; br < nb; ++br) order[size_t(fillp[size_t(color[size_t(br)])]++)] = br;
   bp.bricks = nb;
   bp.order = upload_array(p, order.data(), order.size());
-  // Lattice entries carry the first-writer flag in bit 0.
-  require(p.n_full < (int64_t(1) << 30), GMF_EUNSUPPORTED, "brick maps need dof counts below 2^30");
   std::vector<int32_t> enc(map.size());
-  for (size_t i = 0; i < map.size(); ++i) enc[i] = map[i] >= 0 ? (map[i] << 1) | int32_t(first[i]) : -1;
+  for (size_t i = 0; i < map.size(); ++i) enc[i] = map[i] >= 0 ? (map[i] << 2) | int32_t(kind[i]) : -1;
   bp.map_full = upload_array(p, enc.data(), enc.size());
   if (!red_of_full.empty()) {
     for (size_t i = 0; i < map.size(); ++i)
-      enc[i] = map[i] >= 0 && red_of_full[size_t(map[i])] >= 0 ? (int32_t(red_of_full[size_t(map[i])]) << 1) | int32_t(first[i])
-                                                               : -1;
+      enc[i] = map[i] >= 0 && red_of_full[size_t(map[i])] >= 0
+                   ? (int32_t(red_of_full[size_t(map[i])]) << 2) | int32_t(kind[i])
+                   : -1;
     bp.map_red = upload_array(p, enc.data(), enc.size());
   }
+  // Corner CSR, dofs ascending, slots ascending within a dof.
+  std::sort(corners.begin(), corners.end());
+  std::vector<int32_t> cfull, cred, cslot;
+  std::vector<int64_t> cptr(1, 0);
+  for (size_t i = 0; i < corners.size(); ++i) {
+    if (i == 0 || corners[i].first != corners[i - 1].first) {
+      if (i > 0) cptr.push_back(int64_t(i));
+      cfull.push_back(corners[i].first);
+      cred.push_back(red_of_full.empty() ? -1 : int32_t(red_of_full[size_t(corners[i].first)]));
+    }
+    cslot.push_back(corners[i].second);
+  }
+  if (!corners.empty()) cptr.push_back(int64_t(corners.size()));
+  bp.corner_count = int64_t(cfull.size());
+  if (bp.corner_count > 0) {
+    bp.corner_full = upload_array(p, cfull.data(), cfull.size());
+    bp.corner_red = upload_array(p, cred.data(), cred.size());
+    bp.corner_ptr = upload_array(p, cptr.data(), cptr.size());
+    bp.corner_slot = upload_array(p, cslot.data(), cslot.size());
+  }
+  bp.slot_base = p.brick_slots;
+  p.brick_slots += nb * 8;
   bp.enabled = true;
   return true;
 }
@@ -573,18 +594,18 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
 }
 
-bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced,
+bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream) {
   if (!b.fast || b.cells == 0) return false;
   if (reduced && !b.ridx) return false;
   if (b.bricks.enabled && (!reduced || b.bricks.map_red)) {
     const bool store_first = p.buckets.size() == 1;
     switch (b.N[0]) {
-      case 2: launch_bricks<2>(b, x, y, reduced, store_first, stream); return true;
-      case 3: launch_bricks<3>(b, x, y, reduced, store_first, stream); return true;
-      case 4: launch_bricks<4>(b, x, y, reduced, store_first, stream); return true;
-      case 5: launch_bricks<5>(b, x, y, reduced, store_first, stream); return true;
-      case 6: launch_bricks<6>(b, x, y, reduced, store_first, stream); return true;
+      case 2: launch_bricks<2>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 3: launch_bricks<3>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 4: launch_bricks<4>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 5: launch_bricks<5>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 6: launch_bricks<6>(b, x, y, reduced, store_first, slots, stream); return true;
       default: break;
     }
   }

@@ -38,9 +38,19 @@ struct BrickPath {
   int colors = 0;
   std::vector<int64_t> color_offsets;  // host: bricks of color c are order[off[c]..off[c+1])
   int64_t* order = nullptr;            // brick ids sorted by color
-  // bricks x (2N-1)^3 lattice entries (dof << 1) | first-writer flag, -1 = none
+  // bricks x (2N-1)^3 lattice entries (dof << 2) | kind, -1 = none; kind 0:
+  // read-modify-write, 1: first writer (plain store), 2: brick-corner dof
+  // whose result goes to the brick's corner slot.
   int32_t* map_full = nullptr;
   int32_t* map_red = nullptr;          // same with reduced dofs (fixed dofs -> -1)
+  // Corner dofs: dof i sums slots corner_slot[corner_ptr[i]..corner_ptr[i+1])
+  // (slot = brick * 8 + corner) into corner_full[i] / corner_red[i] (-1: none).
+  int64_t corner_count = 0;
+  int32_t* corner_full = nullptr;
+  int32_t* corner_red = nullptr;
+  int64_t* corner_ptr = nullptr;
+  int32_t* corner_slot = nullptr;
+  int64_t slot_base = 0;               // offset of this bucket's slots in the workspace array
 };
 
 // Device-resident cell batch.
@@ -119,6 +129,7 @@ struct gmf_plan {
   bool selection_map = false;  // P is a 0/1 selection without shift terms in apply
   bool multi_terms = false;
   int64_t device_bytes = 0;
+  int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
   std::vector<void*> allocations;
 };
 
@@ -131,6 +142,7 @@ struct gmf_workspace {
   double* scalars = nullptr;    // device scalars for solvers
   double* host_scalars = nullptr;  // pinned mirror
   int64_t partial_capacity = 0;
+  double* brick_slots = nullptr;   // plan->brick_slots corner results of the brick kernels
 };
 
 namespace gmf {
@@ -148,7 +160,8 @@ void launch_cell_diagonal_generic_skip(const gmf_plan& plan, const Bucket& b, do
 void launch_cell_cloud(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                        hipStream_t stream);
 void launch_cell_cloud_diagonal(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced, hipStream_t stream);
-bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
+// slots: the workspace's brick corner array (plan.brick_slots entries).
+bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream);
 // lidx: host copy of the bucket's full-layout index map; red_of_full: full dof
 // -> reduced dof (empty unless the reduced map is a selection).

@@ -185,7 +185,7 @@ void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
       launch_cell_cloud(p, b, x, y, reduced, ws->stream);
       continue;
     }
-    if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->stream)) continue;
+    if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->brick_slots, ws->stream)) continue;
     launch_cell_generic(p, b, x, y, reduced, ws->stream);
   }
 }
@@ -625,6 +625,7 @@ gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** out) {
     ws->partial_capacity = 64 * 256 + 4096;
     ws->partials = dalloc<double>(size_t(ws->partial_capacity));
     ws->scalars = dalloc<double>(size_t(kScalarCount));
+    if (plan->brick_slots > 0) ws->brick_slots = dalloc<double>(size_t(plan->brick_slots));
     GMF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_scalars), sizeof(double) * kScalarCount));
     *out = ws.release();
   });
@@ -633,7 +634,7 @@ gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** out) {
 gmf_status gmf_workspace_destroy(gmf_workspace* ws) {
   return guarded([&] {
     if (!ws) return;
-    for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars})
+    for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars, ws->brick_slots})
       if (v) (void)hipFree(v);
     if (ws->host_scalars) (void)hipHostFree(ws->host_scalars);
     delete ws;
