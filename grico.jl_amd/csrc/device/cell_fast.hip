@@ -24,11 +24,13 @@
 //    gathered into LDS once, the eight cells read their modes from it and
 //    add their results into an LDS accumulator lattice (ds_add_f64), which
 //    is written back once: plain stores where this brick is the first writer
-//    of the dof, plain read-modify-write otherwise. Bricks are launched one
-//    color at a time (greedy DOF-disjoint coloring), so no two workgroups of
-//    a launch touch the same dof and no global atomics are needed.
+//    of the dof, plain read-modify-write otherwise, and per-brick corner
+//    slots for dofs that occur only at brick corners (summed afterwards by
+//    k_brick_corners). Bricks are launched one color at a time; colors are
+//    disjoint in every non-corner dof, so no two workgroups of a launch touch
+//    the same dof and no global atomics are needed.
 //  * cell kernel: independent cells with a global atomic scatter, used when
-//    the cells do not form verified bricks.
+//    the cells do not form verified bricks or the batch is small.
 // Both remap workgroup ids so that consecutive Morton-ordered work stays on
 // one XCD (workgroups are dealt to the 8 XCDs round-robin).
 #include <algorithm>
