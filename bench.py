@@ -94,8 +94,9 @@ def cpu_baseline(budget_s=12.0):
 
 
 def solve_leg(problem, plan, repeats=3):
-    """GMG-CG and Jacobi-CG from x0 = 0 to ||r|| <= 1e-8 ||b|| on the device:
-    wall time of the Krylov loop with the hierarchy already compiled."""
+    """GMG-CG (FP64 and FP32 V-cycles) and Jacobi-CG from x0 = 0 to ||r|| <=
+    1e-8 ||b|| on the device: wall time of the Krylov loop with the hierarchy
+    already compiled."""
     import ctypes
 
     import torch
@@ -109,11 +110,6 @@ def solve_leg(problem, plan, repeats=3):
     policy = GeometricMultigridPreconditioner(smoother="chebyshev")
     hierarchy = compile_geometric_multigrid(problem, policy, fine_plan=plan)
     t_host = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    M = hierarchy.device_preconditioner()
-    torch.cuda.synchronize()
-    t_dev = time.perf_counter() - t0
-    log(f"GMG hierarchy: host {t_host:.1f}s, device setup {t_dev:.1f}s, levels {hierarchy.reduced_dof_counts()}")
     b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
     x = torch.zeros_like(b)
     dev = ops._ws(plan)
@@ -127,42 +123,56 @@ def solve_leg(problem, plan, repeats=3):
         torch.cuda.synchronize()
         return time.perf_counter() - t, info
 
-    run(M.handle, 1000)
-    times = []
-    for _ in range(repeats):
-        dt, info = run(M.handle, 1000)
-        times.append(dt)
-    x_mg = x.clone()
-    gmg = {"seconds": sorted(times)[len(times) // 2], "iterations": info.iterations,
-           "operator_applications": info.operator_applications,
-           "final_relative_residual": info.final_residual_norm / info.rhs_norm}
-    # one V-cycle alone
-    r = torch.randn(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
-    z = torch.empty_like(r)
-    for _ in range(2):
-        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(10):
-        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
-    torch.cuda.synchronize()
-    gmg["vcycle_seconds"] = (time.perf_counter() - t) / 10
-    log(f"GMG-CG: {gmg}")
+    def gmg(h, tag):
+        t0 = time.perf_counter()
+        M = h.device_preconditioner()
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+        log(f"GMG hierarchy ({tag}): host {t_host:.1f}s, device setup {t_dev:.1f}s, levels {h.reduced_dof_counts()}")
+        run(M.handle, 1000)
+        times = []
+        for _ in range(repeats):
+            dt, info = run(M.handle, 1000)
+            times.append(dt)
+        sol = x.clone()
+        out = {"seconds": sorted(times)[len(times) // 2], "iterations": info.iterations,
+               "operator_applications": info.operator_applications,
+               "final_relative_residual": info.final_residual_norm / info.rhs_norm}
+        r = torch.randn(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
+        z = torch.empty_like(r)
+        for _ in range(2):
+            _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(10):
+            _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+        torch.cuda.synchronize()
+        out["vcycle_seconds"] = (time.perf_counter() - t) / 10
+        log(f"GMG-CG ({tag}): {out}")
+        return out, sol
+
+    gmg64, x64 = gmg(hierarchy, "FP64 V-cycle")
+    policy32 = GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32")
+    gmg32, x32 = gmg(hierarchy.with_policy(policy32), "FP32 V-cycle")
     from gricoamd.solve import compile_preconditioner
     J = compile_preconditioner(g.JacobiPreconditioner(), plan)
     dt, info = run(J.handle, 20000)
     jac = {"seconds": dt, "iterations": info.iterations,
            "final_relative_residual": info.final_residual_norm / info.rhs_norm}
-    diff = float(torch.linalg.norm(x - x_mg) / torch.linalg.norm(x))
-    log(f"Jacobi-CG: {jac}; relative difference of the two solutions {diff:.2e}")
+    nrm = float(torch.linalg.norm(x))
+    diff64 = float(torch.linalg.norm(x - x64)) / nrm
+    diff32 = float(torch.linalg.norm(x - x32)) / nrm
+    log(f"Jacobi-CG: {jac}; relative difference to GMG-CG solutions: FP64 V-cycle {diff64:.2e}, "
+        f"FP32 V-cycle {diff32:.2e}")
     return {
         "metric": "time-to-1e-8 relative residual, reduced system (s)",
-        "gmg_cg": gmg,
+        "gmg_cg": gmg64,
+        "gmg_cg_fp32_vcycle": gmg32,
         "jacobi_cg": jac,
-        "solution_relative_difference": diff,
+        "solution_relative_difference": {"fp64_vcycle": diff64, "fp32_vcycle": diff32},
         "levels": hierarchy.reduced_dof_counts(),
         "smoother": f"Chebyshev-Jacobi degree {policy.chebyshev_degree}, V({policy.pre_smoothing_steps},"
-                    f"{policy.post_smoothing_steps}), FP64",
+                    f"{policy.post_smoothing_steps}); FP64 CG outside",
         "load": "f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)",
     }
 

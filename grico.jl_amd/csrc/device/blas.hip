@@ -16,14 +16,16 @@ inline unsigned grid_for(int64_t n) {
   return unsigned(g);
 }
 
-__global__ void k_fill(double* __restrict__ x, double v, int64_t n) {
+template <class T>
+__global__ void k_fill(T* __restrict__ x, T v, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     x[i] = v;
 }
 
-__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, int64_t n) {
+template <class T, class S>
+__global__ void k_copy(T* __restrict__ d, const S* __restrict__ s, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    d[i] = s[i];
+    d[i] = T(s[i]);
 }
 
 __device__ inline double block_sum(double v) {
@@ -58,13 +60,37 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partials(const double* __rest
 
 void fill(double* x, double v, int64_t n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
+  GMF_HIP(hipGetLastError());
+}
+
+void fill(float* x, float v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
   GMF_HIP(hipGetLastError());
 }
 
 void copy(double* dst, const double* src, int64_t n, hipStream_t s) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_copy, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  hipLaunchKernelGGL((k_copy<double, double>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  GMF_HIP(hipGetLastError());
+}
+
+void copy(float* dst, const float* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_copy<float, float>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  GMF_HIP(hipGetLastError());
+}
+
+void narrow(float* dst, const double* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_copy<float, double>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+  GMF_HIP(hipGetLastError());
+}
+
+void widen(double* dst, const float* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_copy<double, float>), dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
   GMF_HIP(hipGetLastError());
 }
 

@@ -35,6 +35,7 @@
 // one XCD (workgroups are dealt to the 8 XCDs round-robin).
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
@@ -74,12 +75,12 @@ __device__ inline void wave_sync() {
 // never write. Transposes rewrite an array in place only where each lane
 // rewrites exactly the entries it read, so the in-order LDS of one wavefront
 // makes that safe.
-template <int N>
-__device__ inline void cell_body(const EOTables<N>& T, const double (&u)[N], const double (&cx)[N],
-                                 const double (&cy)[N], const double (&cz)[N], double jx2, double jy2, double jz2,
-                                 double* A, double* B, int t0, int t1, bool act, double (&r)[N]) {
+template <int N, class R>
+__device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const R (&cx)[N], const R (&cy)[N],
+                                 const R (&cz)[N], R jx2, R jy2, R jz2, R* A, R* B, int t0, int t1, bool act,
+                                 R (&r)[N]) {
   constexpr int NN = N * N;
-  double a[N], b[N];
+  R a[N], b[N];
   // z: modes -> points, A = (t0, t1, qz).
   eo_interp<N>(T, u, a);
 #pragma unroll
@@ -123,7 +124,7 @@ __device__ inline void cell_body(const EOTables<N>& T, const double (&u)[N], con
       if (act) B[t0 + N * s + NN * t1] += a[s];
   }
   // z channel on the z-pencil (qx=t0, qy=t1), kept in registers.
-  double h[N];
+  R h[N];
   {
 #pragma unroll
     for (int s = 0; s < N; ++s) a[s] = A[t0 + N * t1 + NN * s];
@@ -166,20 +167,21 @@ __device__ inline void lane_coords(int lane, int& t0, int& t1, bool& act) {
 
 // ---- per-cell driver (global atomic scatter) --------------------------------
 
+template <class R>
 struct CellArgs {
   int64_t cells;
-  const double* jinv;   // cells x 3
-  const double* coef;   // cells x N^3 (W_q c_q)
+  const R* jinv;        // cells x 3
+  const R* coef;        // cells x N^3 (W_q c_q)
   const int32_t* idx;   // cells x N^3
-  const double* lcoef;  // cells x N^3 or null
+  const R* lcoef;       // cells x N^3 or null
 };
 
-template <int N, bool COEF>
-__global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T, const double* __restrict__ x,
-                                                       double* __restrict__ y) {
+template <int N, bool COEF, class R>
+__global__ void __launch_bounds__(256) cell_apply_fast(CellArgs<R> a, EOTables<N, R> T, const R* __restrict__ x,
+                                                       R* __restrict__ y) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>(), CPB = 256 / LPC;
-  __shared__ double S1[CPB * NNN];
-  __shared__ double S2[CPB * NNN];
+  __shared__ R S1[CPB * NNN];
+  __shared__ R S2[CPB * NNN];
   const int lc = threadIdx.x / LPC;
   int t0, t1;
   bool act;
@@ -189,21 +191,21 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T
   act = act && live;
   const int64_t c = live ? cell : 0;
   const int64_t base = c * NNN;
-  double u[N], cx[N], r[N];
+  R u[N], cx[N], r[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int e = t0 + N * t1 + NN * k;
     const int32_t g = a.idx[base + e];
-    u[k] = g >= 0 ? (COEF ? a.lcoef[base + e] * x[g] : x[g]) : 0.0;
+    u[k] = g >= 0 ? (COEF ? a.lcoef[base + e] * x[g] : x[g]) : R(0);
   }
-  double cy[N], cz[N];
+  R cy[N], cz[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     cx[q] = a.coef[base + q + N * t0 + NN * t1];
     cy[q] = a.coef[base + t0 + N * q + NN * t1];
     cz[q] = a.coef[base + t0 + N * t1 + NN * q];
   }
-  const double jx = a.jinv[c * 3], jy = a.jinv[c * 3 + 1], jz = a.jinv[c * 3 + 2];
+  const R jx = a.jinv[c * 3], jy = a.jinv[c * 3 + 1], jz = a.jinv[c * 3 + 2];
   cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, S1 + lc * NNN, S2 + lc * NNN, t0, t1, act, r);
   if (act) {
 #pragma unroll
@@ -220,14 +222,15 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs a, EOTables<N> T
 // Lattice entry kinds (low two bits of a brick map entry).
 constexpr int kLatRmw = 0, kLatFirst = 1, kLatCorner = 2;
 
+template <class R>
 struct BrickArgs {
   const int64_t* order;   // bricks of this launch (one color), Morton order
   int64_t count;          // bricks in this launch
   // bricks x L lattice entries: (dof << 2) | kind, -1 for no dof.
   const int32_t* map;
-  const double* jinv;     // cells x 3
-  const double* coef;     // cells x N^3
-  double* slots;          // bricks x 8 corner results
+  const R* jinv;          // cells x 3
+  const R* coef;          // cells x N^3
+  R* slots;               // bricks x 8 corner results
   int store_first;        // 0: always read-modify-write (y pre-zeroed)
 };
 
@@ -252,16 +255,15 @@ __host__ __device__ inline int lattice_corner(int pos, int M) {
 // on the entries); the y values are stable for the launch because bricks of
 // one color share no dof outside the brick corners, and corners go to the
 // brick's own slots.
-template <int N>
-__global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs a, EOTables<N> T,
-                                                                        const double* __restrict__ x,
-                                                                        double* __restrict__ y) {
+template <int N, class R>
+__global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs<R> a, EOTables<N, R> T,
+                                                                        const R* __restrict__ x, R* __restrict__ y) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
   constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC, PER = (L + THREADS - 1) / THREADS;
-  extern __shared__ double smem[];
-  double* xl = smem;           // L input modes
-  double* acc = xl + L;        // L accumulated results
-  double* scratch = acc + L;   // 8 x 2 x N^3 (per cell A, B)
+  extern __shared__ double smem_words[];
+  R* xl = reinterpret_cast<R*>(smem_words);  // L input modes
+  R* acc = xl + L;                            // L accumulated results
+  R* scratch = acc + L;                       // 8 x 2 x N^3 (per cell A, B)
   const int tid = threadIdx.x;
   const int lc = tid / LPC;
   int t0, t1;
@@ -271,15 +273,15 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   // Cell position inside the brick (Morton child order: x fastest).
   const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
   const int64_t cell = brick * 8 + lc;
-  const double* cb = a.coef + cell * NNN;
-  double cx[N], cy[N], cz[N];
+  const R* cb = a.coef + cell * NNN;
+  R cx[N], cy[N], cz[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
     cx[q] = cb[q + N * t0 + NN * t1];
     cy[q] = cb[t0 + N * q + NN * t1];
     cz[q] = cb[t0 + N * t1 + NN * q];
   }
-  const double jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
+  const R jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
   const int32_t* map = a.map + brick * L;
   int32_t e[PER];
 #pragma unroll
@@ -287,26 +289,26 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int p = tid + k * THREADS;
     e[k] = p < L ? map[p] : -1;
   }
-  double yold[PER];
+  R yold[PER];
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int p = tid + k * THREADS;
     const int32_t d = e[k] >> 2, kind = e[k] & 3;
     const bool read = e[k] >= 0 && kind != kLatCorner && !(a.store_first && kind == kLatFirst);
-    yold[k] = read ? y[d] : 0.0;
+    yold[k] = read ? y[d] : R(0);
     if (p < L) {
-      xl[p] = e[k] >= 0 ? x[d] : 0.0;
-      acc[p] = 0.0;
+      xl[p] = e[k] >= 0 ? x[d] : R(0);
+      acc[p] = R(0);
     }
   }
   __syncthreads();
-  double u[N], r[N];
+  R u[N], r[N];
   {
     const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
 #pragma unroll
     for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
   }
-  double* A = scratch + lc * 2 * NNN;
+  R* A = scratch + lc * 2 * NNN;
   cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, A + NNN, t0, t1, act, r);
   if (act) {
     const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
@@ -327,71 +329,86 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
 
 // Brick-corner dofs: y[d] (+)= sum of the corner slots that hold d, in slot
 // order (deterministic).
-__global__ void k_brick_corners(double* __restrict__ y, const double* __restrict__ slots, const int32_t* dof,
+template <class R>
+__global__ void k_brick_corners(R* __restrict__ y, const R* __restrict__ slots, const int32_t* dof,
                                 const int64_t* ptr, const int32_t* slot, int64_t count, int accumulate) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= count) return;
   const int32_t d = dof[i];
   if (d < 0) return;
-  double s = 0.0;
+  R s = R(0);
   for (int64_t j = ptr[i]; j < ptr[i + 1]; ++j) s += slots[slot[j]];
   y[d] = accumulate ? y[d] + s : s;
 }
 
-template <int N>
-EOTables<N> tables_of(const Bucket& b) {
-  return make_eo_tables<N>(b.h_values.data(), b.h_colloc.data());
+template <int N, class R>
+EOTables<N, R> tables_of(const Bucket& b) {
+  return make_eo_tables<N, R>(b.h_values.data(), b.h_colloc.data());
 }
 
-template <int N>
-void launch_cells(const Bucket& b, const double* x, double* y, bool reduced, hipStream_t stream) {
+// Per-cell arrays of a bucket in arithmetic type R (FP64 originals or their
+// FP32 copies made by prepare_fp32).
+struct CellArrays64 {
+  static const double* jinv(const Bucket& b) { return b.jinv; }
+  static const double* coef(const Bucket& b) { return b.pdata; }
+  static const double* lcoef(const Bucket& b) { return b.lcoef; }
+};
+struct CellArrays32 {
+  static const float* jinv(const Bucket& b) { return b.jinv32; }
+  static const float* coef(const Bucket& b) { return b.pdata32; }
+  static const float* lcoef(const Bucket& b) { return b.lcoef32; }
+};
+template <class R>
+using CellArrays = typename std::conditional<std::is_same<R, float>::value, CellArrays32, CellArrays64>::type;
+
+template <int N, class R>
+void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, hipStream_t stream) {
   constexpr int CPB = 256 / lanes_per_cell<N>();
-  EOTables<N> T = tables_of<N>(b);
-  CellArgs a;
+  EOTables<N, R> T = tables_of<N, R>(b);
+  CellArgs<R> a;
   a.cells = b.cells;
-  a.jinv = b.jinv;
-  a.coef = b.pdata;
+  a.jinv = CellArrays<R>::jinv(b);
+  a.coef = CellArrays<R>::coef(b);
   a.idx = reduced ? b.ridx : b.lidx;
-  a.lcoef = b.lcoef;
+  a.lcoef = CellArrays<R>::lcoef(b);
   const int64_t blocks = (b.cells + CPB - 1) / CPB;
   if (b.lcoef)
-    hipLaunchKernelGGL((cell_apply_fast<N, true>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, true, R>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
   else
-    hipLaunchKernelGGL((cell_apply_fast<N, false>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, false, R>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
   GMF_HIP(hipGetLastError());
 }
 
-template <int N>
+template <int N, class R>
 size_t brick_lds_bytes() {
   constexpr int M = 2 * N - 1;
-  return sizeof(double) * size_t(2 * M * M * M + 8 * 2 * N * N * N);
+  return sizeof(R) * size_t(2 * M * M * M + 8 * 2 * N * N * N);
 }
 
-template <int N>
-void launch_bricks(const Bucket& b, const double* x, double* y, bool reduced, bool store_first, double* slots,
-                   hipStream_t stream) {
-  EOTables<N> T = tables_of<N>(b);
+template <int N, class R>
+void launch_bricks(const Bucket& b, const R* x, R* y, bool reduced, bool store_first, R* slots, hipStream_t stream) {
+  EOTables<N, R> T = tables_of<N, R>(b);
   const BrickPath& bp = b.bricks;
   require(bp.corner_count == 0 || slots != nullptr, GMF_EINVAL, "workspace has no brick corner slots");
-  double* bslots = slots ? slots + bp.slot_base : nullptr;
+  R* bslots = slots ? slots + bp.slot_base : nullptr;
   for (int c = 0; c < bp.colors; ++c) {
     const int64_t lo = bp.color_offsets[size_t(c)], hi = bp.color_offsets[size_t(c) + 1];
     if (hi == lo) continue;
-    BrickArgs a;
+    BrickArgs<R> a;
     a.order = bp.order + lo;
     a.count = hi - lo;
     a.map = reduced ? bp.map_red : bp.map_full;
-    a.jinv = b.jinv;
-    a.coef = b.pdata;
+    a.jinv = CellArrays<R>::jinv(b);
+    a.coef = CellArrays<R>::coef(b);
     a.slots = bslots;
     a.store_first = store_first ? 1 : 0;
-    hipLaunchKernelGGL((brick_apply<N>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()), brick_lds_bytes<N>(),
-                       stream, a, T, x, y);
+    hipLaunchKernelGGL((brick_apply<N, R>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()),
+                       (brick_lds_bytes<N, R>()), stream, a, T, x, y);
   }
   if (bp.corner_count > 0)
-    hipLaunchKernelGGL(k_brick_corners, dim3(unsigned((bp.corner_count + 255) / 256)), dim3(256), 0, stream, y, bslots,
-                       reduced ? bp.corner_red : bp.corner_full, bp.corner_ptr, bp.corner_slot, bp.corner_count,
-                       store_first ? 0 : 1);
+    hipLaunchKernelGGL(k_brick_corners<R>, dim3(unsigned((bp.corner_count + 255) / 256)), dim3(256), 0, stream, y,
+                       bslots, reduced ? bp.corner_red : bp.corner_full, bp.corner_ptr, bp.corner_slot,
+                       bp.corner_count, store_first ? 0 : 1);
   GMF_HIP(hipGetLastError());
 }
 
@@ -596,31 +613,83 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
 }
 
-bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
-                      hipStream_t stream) {
+namespace {
+
+template <class R>
+bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool reduced, R* slots, hipStream_t stream) {
   if (!b.fast || b.cells == 0) return false;
   if (reduced && !b.ridx) return false;
   if (b.bricks.enabled && (!reduced || b.bricks.map_red)) {
     const bool store_first = p.buckets.size() == 1;
     switch (b.N[0]) {
-      case 2: launch_bricks<2>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 3: launch_bricks<3>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 4: launch_bricks<4>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 5: launch_bricks<5>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 6: launch_bricks<6>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 2: launch_bricks<2, R>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 3: launch_bricks<3, R>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 4: launch_bricks<4, R>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 5: launch_bricks<5, R>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 6: launch_bricks<6, R>(b, x, y, reduced, store_first, slots, stream); return true;
       default: break;
     }
   }
   switch (b.N[0]) {
-    case 2: launch_cells<2>(b, x, y, reduced, stream); return true;
-    case 3: launch_cells<3>(b, x, y, reduced, stream); return true;
-    case 4: launch_cells<4>(b, x, y, reduced, stream); return true;
-    case 5: launch_cells<5>(b, x, y, reduced, stream); return true;
-    case 6: launch_cells<6>(b, x, y, reduced, stream); return true;
-    case 7: launch_cells<7>(b, x, y, reduced, stream); return true;
-    case 8: launch_cells<8>(b, x, y, reduced, stream); return true;
+    case 2: launch_cells<2, R>(b, x, y, reduced, stream); return true;
+    case 3: launch_cells<3, R>(b, x, y, reduced, stream); return true;
+    case 4: launch_cells<4, R>(b, x, y, reduced, stream); return true;
+    case 5: launch_cells<5, R>(b, x, y, reduced, stream); return true;
+    case 6: launch_cells<6, R>(b, x, y, reduced, stream); return true;
+    case 7: launch_cells<7, R>(b, x, y, reduced, stream); return true;
+    case 8: launch_cells<8, R>(b, x, y, reduced, stream); return true;
     default: return false;
   }
+}
+
+__global__ void k_round_to_float(float* __restrict__ out, const double* __restrict__ in, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    out[i] = float(in[i]);
+}
+
+float* float_copy(gmf_plan& p, const double* d, size_t n) {
+  if (d == nullptr || n == 0) return nullptr;
+  void* f = nullptr;
+  GMF_HIP(hipMalloc(&f, n * sizeof(float)));
+  p.allocations.push_back(f);
+  p.device_bytes += int64_t(n * sizeof(float));
+  const int64_t blocks = std::min<int64_t>((int64_t(n) + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_round_to_float, dim3(unsigned(blocks)), dim3(256), 0, nullptr, static_cast<float*>(f), d,
+                     int64_t(n));
+  GMF_HIP(hipGetLastError());
+  return static_cast<float*>(f);
+}
+
+}  // namespace
+
+bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
+                      hipStream_t stream) {
+  return launch_fast<double>(p, b, x, y, reduced, slots, stream);
+}
+
+bool launch_cell_fast32(const gmf_plan& p, const Bucket& b, const float* x, float* y, bool reduced, float* slots,
+                        hipStream_t stream) {
+  if (!b.fp32) return false;
+  return launch_fast<float>(p, b, x, y, reduced, slots, stream);
+}
+
+bool prepare_fp32(gmf_plan& p) {
+  if (p.fp32_state != 0) return p.fp32_state > 0;
+  bool all = p.selection_map;
+  for (Bucket& b : p.buckets) {
+    if (!b.fast || b.cloud) {
+      all = false;
+      continue;
+    }
+    const size_t cells = size_t(b.cells);
+    b.jinv32 = float_copy(p, b.jinv, cells * 3);
+    b.pdata32 = float_copy(p, b.pdata, cells * size_t(b.Qtot) * size_t(b.point_stride));
+    b.lcoef32 = float_copy(p, b.lcoef, cells * size_t(b.Ntot));
+    b.fp32 = true;
+  }
+  GMF_HIP(hipDeviceSynchronize());
+  p.fp32_state = all ? 1 : -1;
+  return all;
 }
 
 bool plan_writes_every_dof(const gmf_plan& p) {

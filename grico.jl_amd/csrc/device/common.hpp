@@ -81,6 +81,11 @@ struct Bucket {
   std::vector<double> h_colloc;
   bool fast = false;
   BrickPath bricks;
+  // FP32 copies of the per-cell arrays for FP32 multigrid levels (prepare_fp32).
+  bool fp32 = false;
+  float* jinv32 = nullptr;
+  float* pdata32 = nullptr;
+  float* lcoef32 = nullptr;
   // Point-cloud batch: item c owns points poff[c]..poff[c+1]-1 at biunit
   // reference coordinates rpts (points x dim); pdata rows per point.
   bool cloud = false;
@@ -130,6 +135,7 @@ struct gmf_plan {
   bool multi_terms = false;
   int64_t device_bytes = 0;
   int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
+  int fp32_state = 0;       // 0: not prepared, 1: every bucket has FP32 kernels, -1: FP64 with conversion
   std::vector<void*> allocations;
 };
 
@@ -143,6 +149,8 @@ struct gmf_workspace {
   double* host_scalars = nullptr;  // pinned mirror
   int64_t partial_capacity = 0;
   double* brick_slots = nullptr;   // plan->brick_slots corner results of the brick kernels
+  double* red64_in = nullptr;      // FP64 staging of FP32 reduced applies on plans without FP32 kernels
+  double* red64_out = nullptr;
 };
 
 namespace gmf {
@@ -167,6 +175,12 @@ bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, do
 // -> reduced dof (empty unless the reduced map is a selection).
 void prepare_fast_path(gmf_plan& plan, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full);
 bool plan_writes_every_dof(const gmf_plan& plan);
+// FP32 variant of launch_cell_fast for buckets prepared by prepare_fp32.
+bool launch_cell_fast32(const gmf_plan& plan, const Bucket& b, const float* x, float* y, bool reduced, float* slots,
+                        hipStream_t stream);
+// Make FP32 copies of the fast buckets' per-cell arrays; true when every
+// bucket of the plan then has FP32 kernels (and the reduced map is a selection).
+bool prepare_fp32(gmf_plan& plan);
 
 // Copy a host array to a new device allocation owned by the plan.
 template <class T>
@@ -182,7 +196,12 @@ T* upload_array(gmf_plan& p, const T* host, size_t n) {
 
 // Vector kernels (blas.hip).
 void fill(double* x, double v, int64_t n, hipStream_t s);
+void fill(float* x, float v, int64_t n, hipStream_t s);
 void copy(double* dst, const double* src, int64_t n, hipStream_t s);
+void copy(float* dst, const float* src, int64_t n, hipStream_t s);
+// Precision conversions (round to nearest).
+void narrow(float* dst, const double* src, int64_t n, hipStream_t s);
+void widen(double* dst, const float* src, int64_t n, hipStream_t s);
 // partial sums of x.y into ws->partials then final reduce into scalars[slot].
 void dot(gmf_workspace* ws, const double* x, const double* y, int64_t n, int slot);
 void read_scalars(gmf_workspace* ws, int count);

@@ -88,6 +88,12 @@ inline void require_device_pointer(const void* p, const char* what) {
 void sync(gmf_workspace* ws);
 // y_red = P^T A P x_red on ws->stream (src/assembly.jl:905-915).
 void reduced_apply(gmf_workspace* ws, const double* xr, double* yr);
+// The same operator on FP32 vectors: FP32 kernels on plans whose buckets all
+// have them, otherwise the FP64 apply of the widened input, rounded back.
+// The workspace must have been prepared with prepare_fp32_workspace (which
+// allocates; the apply itself does not).
+void reduced_apply_f32(gmf_workspace* ws, const float* xr, float* yr);
+void prepare_fp32_workspace(gmf_workspace* ws);
 // Exact reduced diagonal into diag (n_red) on ws->stream.
 void reduced_diagonal(gmf_workspace* ws, double* diag);
 // inv = 1 / d with the reference's zero / non-finite rejection (GMF_EDIAGONAL).

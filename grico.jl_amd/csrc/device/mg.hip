@@ -7,13 +7,16 @@
 // workspace of its rediscretized plan; transfers are CSR tables in both
 // directions so prolongation and restriction are both gathers (deterministic,
 // no atomics). The first smoothing sweep of each level starts from x = 0 and
-// skips the product A * 0.
+// skips the product A * 0. The level vectors, smoothers, transfers and level
+// operators run in the level type R (double, or float for the FP32 V-cycle);
+// the coarse solve, the eigenvalue estimates and the diagonals stay FP64.
 #include <algorithm>
 #include <cmath>
 #include <limits>
 #include <memory>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "blas.hpp"
@@ -28,26 +31,29 @@ namespace {
 #define GMF_GRID_STRIDE(i, n) \
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < (n); i += int64_t(gridDim.x) * blockDim.x)
 
-__global__ void k_jacobi_first(double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
-                               double omega, int64_t n) {
+template <class R>
+__global__ void k_jacobi_first(R* __restrict__ x, const R* __restrict__ b, const R* __restrict__ dinv, R omega,
+                               int64_t n) {
   GMF_GRID_STRIDE(i, n) x[i] = omega * (dinv[i] * b[i]);
 }
 
 // x += omega D^-1 (b - q), q = A x.
-__global__ void k_jacobi_update(double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ q,
-                                const double* __restrict__ dinv, double omega, int64_t n) {
+template <class R>
+__global__ void k_jacobi_update(R* __restrict__ x, const R* __restrict__ b, const R* __restrict__ q,
+                                const R* __restrict__ dinv, R omega, int64_t n) {
   GMF_GRID_STRIDE(i, n) x[i] = fma(omega, dinv[i] * (b[i] - q[i]), x[i]);
 }
 
 // bc[c] = sum_f P[f,c] (b[f] - q[f]) over the coarse row's CSR (q may be null).
-__global__ void k_restrict(double* __restrict__ bc, const double* __restrict__ b, const double* __restrict__ q,
+template <class R>
+__global__ void k_restrict(R* __restrict__ bc, const R* __restrict__ b, const R* __restrict__ q,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
-                           const double* __restrict__ val, int64_t nc) {
+                           const R* __restrict__ val, int64_t nc) {
   GMF_GRID_STRIDE(c, nc) {
-    double s = 0.0;
+    R s = R(0);
     for (int64_t t = off[c]; t < off[c + 1]; ++t) {
       const int32_t f = idx[t];
-      const double r = q ? b[f] - q[f] : b[f];
+      const R r = q ? b[f] - q[f] : b[f];
       s = fma(val[t], r, s);
     }
     bc[c] = s;
@@ -55,21 +61,23 @@ __global__ void k_restrict(double* __restrict__ bc, const double* __restrict__ b
 }
 
 // x[f] += sum_c P[f,c] xc[c].
-__global__ void k_prolong_add(double* __restrict__ x, const double* __restrict__ xc, const int64_t* __restrict__ off,
-                              const int32_t* __restrict__ idx, const double* __restrict__ val, int64_t nf) {
+template <class R>
+__global__ void k_prolong_add(R* __restrict__ x, const R* __restrict__ xc, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ idx, const R* __restrict__ val, int64_t nf) {
   GMF_GRID_STRIDE(f, nf) {
-    double s = 0.0;
+    R s = R(0);
     for (int64_t t = off[f]; t < off[f + 1]; ++t) s = fma(val[t], xc[idx[t]], s);
     x[f] += s;
   }
 }
 
 // Chebyshev start from x = 0: r = b, d = s D^-1 r, x = d.
-__global__ void k_cheb_start0(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
-                              const double* __restrict__ b, const double* __restrict__ dinv, double s, int64_t n) {
+template <class R>
+__global__ void k_cheb_start0(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
+                              const R* __restrict__ dinv, R s, int64_t n) {
   GMF_GRID_STRIDE(i, n) {
-    const double ri = b[i];
-    const double di = s * (dinv[i] * ri);
+    const R ri = b[i];
+    const R di = s * (dinv[i] * ri);
     r[i] = ri;
     d[i] = di;
     x[i] = di;
@@ -77,12 +85,12 @@ __global__ void k_cheb_start0(double* __restrict__ x, double* __restrict__ d, do
 }
 
 // Chebyshev start from x with q = A x: r = b - q, d = s D^-1 r, x += d.
-__global__ void k_cheb_start(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
-                             const double* __restrict__ b, const double* __restrict__ q, const double* __restrict__ dinv,
-                             double s, int64_t n) {
+template <class R>
+__global__ void k_cheb_start(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
+                             const R* __restrict__ q, const R* __restrict__ dinv, R s, int64_t n) {
   GMF_GRID_STRIDE(i, n) {
-    const double ri = b[i] - q[i];
-    const double di = s * (dinv[i] * ri);
+    const R ri = b[i] - q[i];
+    const R di = s * (dinv[i] * ri);
     r[i] = ri;
     d[i] = di;
     x[i] += di;
@@ -90,12 +98,12 @@ __global__ void k_cheb_start(double* __restrict__ x, double* __restrict__ d, dou
 }
 
 // Chebyshev step with q = A d: r -= q, d = c1 d + c2 D^-1 r, x += d.
-__global__ void k_cheb_step(double* __restrict__ x, double* __restrict__ d, double* __restrict__ r,
-                            const double* __restrict__ q, const double* __restrict__ dinv, double c1, double c2,
-                            int64_t n) {
+template <class R>
+__global__ void k_cheb_step(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ q,
+                            const R* __restrict__ dinv, R c1, R c2, int64_t n) {
   GMF_GRID_STRIDE(i, n) {
-    const double ri = r[i] - q[i];
-    const double di = fma(c1, d[i], c2 * (dinv[i] * ri));
+    const R ri = r[i] - q[i];
+    const R di = fma(c1, d[i], c2 * (dinv[i] * ri));
     r[i] = ri;
     d[i] = di;
     x[i] += di;
@@ -139,6 +147,7 @@ __global__ void k_start_vector(double* __restrict__ v, int64_t n) {
   }
 }
 
+
 // ---- host helpers ------------------------------------------------------------
 
 // Largest eigenvalue of the symmetric tridiagonal (alpha, beta) by Sturm
@@ -172,18 +181,20 @@ double tridiagonal_max_eigenvalue(const std::vector<double>& alpha, const std::v
   return hi;
 }
 
+template <class R>
 struct Level {
   gmf_workspace* ws = nullptr;
   int64_t n = 0;
-  DeviceBuffer<double> invdiag, x, b, q, r, d, sdiag;
+  DeviceBuffer<R> invdiag, x, b, q, r, d;
   double lambda_max = 0.0;
 };
 
+template <class R>
 struct TransferCsr {
   int64_t nc = 0, nf = 0;
   DeviceBuffer<int64_t> poff, roff;
   DeviceBuffer<int32_t> pidx, ridx;
-  DeviceBuffer<double> pval, rval;
+  DeviceBuffer<R> pval, rval;
 };
 
 template <class T>
@@ -193,27 +204,48 @@ DeviceBuffer<T> upload_vec(const std::vector<T>& h) {
   return d;
 }
 
-struct MGPrecond : gmf_preconditioner {
-  std::vector<Level> levels;
-  std::vector<TransferCsr> transfers;
+inline void level_apply(gmf_workspace* ws, const double* x, double* y) { reduced_apply(ws, x, y); }
+inline void level_apply(gmf_workspace* ws, const float* x, float* y) { reduced_apply_f32(ws, x, y); }
+
+// dst (type R) = src (FP64), and back.
+inline void to_level(double* dst, const double* src, int64_t n, hipStream_t s) { copy(dst, src, n, s); }
+inline void to_level(float* dst, const double* src, int64_t n, hipStream_t s) { narrow(dst, src, n, s); }
+inline void from_level(double* dst, const double* src, int64_t n, hipStream_t s) { copy(dst, src, n, s); }
+inline void from_level(double* dst, const float* src, int64_t n, hipStream_t s) { widen(dst, src, n, s); }
+
+// Common part of the FP64 and FP32 V-cycles (diagnostics).
+struct MGBase : gmf_preconditioner {
+  std::vector<int64_t> level_n;
+  std::vector<double> level_lambda;
+};
+
+template <class R>
+struct MGPrecond : MGBase {
+  std::vector<Level<R>> levels;
+  std::vector<TransferCsr<R>> transfers;
   int smoother = GMF_SMOOTHER_JACOBI;
   int pre = 2, post = 2;
   double omega = 0.8;
   int cheb_degree = 4;
   double cheb_range = 20.0;
-  // coarse solver
+  // coarse solver (FP64)
   bool dense = false;
   DeviceBuffer<double> coarse_inverse;
+  DeviceBuffer<double> coarse_invdiag;
+  DeviceBuffer<double> coarse_b, coarse_x;  // FP64 coarse vectors when R is float
   std::unique_ptr<JacobiPrecond> coarse_jacobi;
   std::unique_ptr<CgVectors> coarse_vectors;
   double coarse_rtol = 1e-12, coarse_atol = 0.0;
   int64_t coarse_maxiter = 10000;
-  // Captured V-cycle: the graph reads top_b and writes top_x.
+  // Top-level vectors in R (FP32 conversion, or the buffers a captured graph
+  // reads and writes) and the captured V-cycle.
   bool use_graph = false;
   hipStream_t capture_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  DeviceBuffer<double> top_b, top_x;
+  DeviceBuffer<R> top_b, top_x;
+
+  static constexpr bool kDouble = std::is_same<R, double>::value;
 
   ~MGPrecond() override {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -222,7 +254,7 @@ struct MGPrecond : gmf_preconditioner {
   }
 
   void set_streams(hipStream_t s) {
-    for (Level& L : levels) L.ws->stream = s;
+    for (Level<R>& L : levels) L.ws->stream = s;
   }
 
   void capture() {
@@ -248,30 +280,38 @@ struct MGPrecond : gmf_preconditioner {
 
   void apply(const double* r, double* z) override {
     const int top = int(levels.size()) - 1;
-    if (!use_graph) {
-      set_streams(ws->stream);
-      vcycle(top, z, r);
-      return;
-    }
     const int64_t n = levels[size_t(top)].n;
-    if (!graph_exec) capture();
-    copy(top_b.p, r, n, ws->stream);
-    GMF_HIP(hipGraphLaunch(graph_exec, ws->stream));
-    copy(z, top_x.p, n, ws->stream);
+    if constexpr (kDouble) {
+      if (!use_graph) {
+        set_streams(ws->stream);
+        vcycle(top, z, r);
+        return;
+      }
+    }
+    if (use_graph && !graph_exec) capture();
+    to_level(top_b.p, r, n, ws->stream);
+    if (use_graph) {
+      GMF_HIP(hipGraphLaunch(graph_exec, ws->stream));
+    } else {
+      set_streams(ws->stream);
+      vcycle(top, top_x.p, top_b.p);
+    }
+    from_level(z, top_x.p, n, ws->stream);
   }
 
-  void smooth(int l, double* x, const double* b, int steps, bool x_zero) {
-    Level& L = levels[size_t(l)];
+  void smooth(int l, R* x, const R* b, int steps, bool x_zero) {
+    Level<R>& L = levels[size_t(l)];
     const int64_t n = L.n;
     hipStream_t s = ws->stream;
     if (n == 0) return;
     if (smoother == GMF_SMOOTHER_JACOBI) {
       for (int k = 0; k < steps; ++k) {
         if (x_zero && k == 0) {
-          hipLaunchKernelGGL(k_jacobi_first, dim3(grid1(n)), dim3(256), 0, s, x, b, L.invdiag.p, omega, n);
+          hipLaunchKernelGGL(k_jacobi_first<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.invdiag.p, R(omega), n);
         } else {
-          reduced_apply(L.ws, x, L.q.p);
-          hipLaunchKernelGGL(k_jacobi_update, dim3(grid1(n)), dim3(256), 0, s, x, b, L.q.p, L.invdiag.p, omega, n);
+          level_apply(L.ws, x, L.q.p);
+          hipLaunchKernelGGL(k_jacobi_update<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.q.p, L.invdiag.p,
+                             R(omega), n);
         }
         GMF_HIP(hipGetLastError());
       }
@@ -282,28 +322,27 @@ struct MGPrecond : gmf_preconditioner {
     const double sigma = theta / delta;
     for (int k = 0; k < steps; ++k) {
       if (x_zero && k == 0) {
-        hipLaunchKernelGGL(k_cheb_start0, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
-                           1.0 / theta, n);
+        hipLaunchKernelGGL(k_cheb_start0<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
+                           R(1.0 / theta), n);
       } else {
-        reduced_apply(L.ws, x, L.q.p);
-        hipLaunchKernelGGL(k_cheb_start, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p, L.invdiag.p,
-                           1.0 / theta, n);
+        level_apply(L.ws, x, L.q.p);
+        hipLaunchKernelGGL(k_cheb_start<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p,
+                           L.invdiag.p, R(1.0 / theta), n);
       }
       double rho = 1.0 / sigma;
       for (int i = 1; i < cheb_degree; ++i) {
-        reduced_apply(L.ws, L.d.p, L.q.p);
+        level_apply(L.ws, L.d.p, L.q.p);
         const double rho_new = 1.0 / (2.0 * sigma - rho);
-        hipLaunchKernelGGL(k_cheb_step, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
-                           rho_new * rho, 2.0 * rho_new / delta, n);
+        hipLaunchKernelGGL(k_cheb_step<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
+                           R(rho_new * rho), R(2.0 * rho_new / delta), n);
         rho = rho_new;
       }
       GMF_HIP(hipGetLastError());
     }
   }
 
-  void coarse_solve(double* x, const double* b) {
-    Level& L = levels[0];
-    if (L.n == 0) return;
+  void coarse_solve64(double* x, const double* b) {
+    Level<R>& L = levels[0];
     if (dense) {
       const unsigned blocks = unsigned((L.n + 3) / 4);
       hipLaunchKernelGGL(k_dense_matvec, dim3(blocks), dim3(256), 0, ws->stream, x, coarse_inverse.p, b, L.n);
@@ -315,40 +354,53 @@ struct MGPrecond : gmf_preconditioner {
     cg_run(L.ws, coarse_jacobi.get(), b, x, false, coarse_rtol, coarse_atol, coarse_maxiter, inf, *coarse_vectors);
   }
 
+  void coarse_solve(R* x, const R* b) {
+    Level<R>& L = levels[0];
+    if (L.n == 0) return;
+    if constexpr (kDouble) {
+      coarse_solve64(x, b);
+    } else {
+      widen(coarse_b.p, b, L.n, ws->stream);
+      coarse_solve64(coarse_x.p, coarse_b.p);
+      narrow(x, coarse_x.p, L.n, ws->stream);
+    }
+  }
+
   // x is treated as zero on entry.
-  void vcycle(int l, double* x, const double* b) {
+  void vcycle(int l, R* x, const R* b) {
     if (l == 0) {
       coarse_solve(x, b);
       return;
     }
-    Level& L = levels[size_t(l)];
-    Level& C = levels[size_t(l - 1)];
-    TransferCsr& T = transfers[size_t(l - 1)];
+    Level<R>& L = levels[size_t(l)];
+    Level<R>& C = levels[size_t(l - 1)];
+    TransferCsr<R>& T = transfers[size_t(l - 1)];
     hipStream_t s = ws->stream;
     if (pre > 0) {
       smooth(l, x, b, pre, true);
-      reduced_apply(L.ws, x, L.q.p);
-      hipLaunchKernelGGL(k_restrict, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, L.q.p, T.roff.p, T.ridx.p,
+      level_apply(L.ws, x, L.q.p);
+      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, L.q.p, T.roff.p, T.ridx.p,
                          T.rval.p, T.nc);
     } else {
-      fill(x, 0.0, L.n, s);
-      hipLaunchKernelGGL(k_restrict, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, nullptr, T.roff.p, T.ridx.p,
+      fill(x, R(0), L.n, s);
+      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, nullptr, T.roff.p, T.ridx.p,
                          T.rval.p, T.nc);
     }
     GMF_HIP(hipGetLastError());
     vcycle(l - 1, C.x.p, C.b.p);
-    hipLaunchKernelGGL(k_prolong_add, dim3(grid1(T.nf)), dim3(256), 0, s, x, C.x.p, T.poff.p, T.pidx.p, T.pval.p,
-                       T.nf);
+    hipLaunchKernelGGL(k_prolong_add<R>, dim3(grid1(T.nf)), dim3(256), 0, s, x, C.x.p, T.poff.p, T.pidx.p,
+                       T.pval.p, T.nf);
     GMF_HIP(hipGetLastError());
     smooth(l, x, b, post, false);
   }
 };
 
-TransferCsr build_transfer(const gmf_transfer_desc& t, int64_t nc, int64_t nf) {
+template <class R>
+TransferCsr<R> build_transfer(const gmf_transfer_desc& t, int64_t nc, int64_t nf) {
   require(t.nnz >= 0 && (t.nnz == 0 || (t.coarse_indices && t.fine_indices && t.coefficients)), GMF_EINVAL,
           "transfer arrays missing");
   require(nc < (int64_t(1) << 31) && nf < (int64_t(1) << 31), GMF_EUNSUPPORTED, "level too large for 32-bit indices");
-  TransferCsr c;
+  TransferCsr<R> c;
   c.nc = nc;
   c.nf = nf;
   const size_t nnz = size_t(t.nnz);
@@ -366,14 +418,14 @@ TransferCsr build_transfer(const gmf_transfer_desc& t, int64_t nc, int64_t nf) {
   std::partial_sum(poff.begin(), poff.end(), poff.begin());
   std::partial_sum(roff.begin(), roff.end(), roff.begin());
   std::vector<int32_t> pidx(nnz), ridx(nnz);
-  std::vector<double> pval(nnz), rval(nnz);
+  std::vector<R> pval(nnz), rval(nnz);
   std::vector<int64_t> fill_r(roff.begin(), roff.end() - 1);
   for (size_t k = 0; k < nnz; ++k) {
     pidx[k] = int32_t(t.coarse_indices[k]);
-    pval[k] = t.coefficients[k];
+    pval[k] = R(t.coefficients[k]);
     const size_t slot = size_t(fill_r[size_t(t.coarse_indices[k])]++);
     ridx[slot] = int32_t(t.fine_indices[k]);
-    rval[slot] = t.coefficients[k];
+    rval[slot] = R(t.coefficients[k]);
   }
   c.poff = upload_vec(poff);
   c.roff = upload_vec(roff);
@@ -385,14 +437,14 @@ TransferCsr build_transfer(const gmf_transfer_desc& t, int64_t nc, int64_t nf) {
 }
 
 // Lanczos estimate of lambda_max(D^-1 A) through the symmetric D^-1/2 A D^-1/2.
-double estimate_lambda_max(Level& L, int iterations) {
-  const int64_t n = L.n;
-  gmf_workspace* ws = L.ws;
+// FP64 throughout (invdiag: the level's FP64 inverse diagonal).
+double estimate_lambda_max(gmf_workspace* ws, int64_t n, const double* invdiag, int iterations) {
   hipStream_t s = ws->stream;
   const int m = int(std::min<int64_t>(iterations, n));
   if (m <= 0) return 0.0;
-  DeviceBuffer<double> v(static_cast<size_t>(n)), vp(static_cast<size_t>(n)), u(static_cast<size_t>(n)), w(static_cast<size_t>(n));
-  hipLaunchKernelGGL(k_sqrt, dim3(grid1(n)), dim3(256), 0, s, L.sdiag.p, L.invdiag.p, n);
+  DeviceBuffer<double> v(static_cast<size_t>(n)), vp(static_cast<size_t>(n)), u(static_cast<size_t>(n)),
+      w(static_cast<size_t>(n)), sdiag(static_cast<size_t>(n));
+  hipLaunchKernelGGL(k_sqrt, dim3(grid1(n)), dim3(256), 0, s, sdiag.p, invdiag, n);
   hipLaunchKernelGGL(k_start_vector, dim3(grid1(n)), dim3(256), 0, s, v.p, n);
   fill(vp.p, 0.0, n, s);
   dot(ws, v.p, v.p, n, kTmp);
@@ -402,9 +454,9 @@ double estimate_lambda_max(Level& L, int iterations) {
   std::vector<double> alpha, beta;
   double beta_prev = 0.0;
   for (int j = 0; j < m; ++j) {
-    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, u.p, L.sdiag.p, v.p, n);
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, u.p, sdiag.p, v.p, n);
     reduced_apply(ws, u.p, w.p);
-    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, w.p, L.sdiag.p, w.p, n);
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, w.p, sdiag.p, w.p, n);
     GMF_HIP(hipGetLastError());
     dot(ws, w.p, v.p, n, kTmp);
     read_scalars(ws, kTmp + 1);
@@ -428,9 +480,7 @@ double estimate_lambda_max(Level& L, int iterations) {
 }
 
 // Dense coarse inverse from the probed SPD coarse matrix (src/multigrid.jl:382-424).
-DeviceBuffer<double> dense_coarse_inverse(Level& L) {
-  const int64_t n = L.n;
-  gmf_workspace* ws = L.ws;
+DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n) {
   hipStream_t s = ws->stream;
   DeviceBuffer<double> e(static_cast<size_t>(n)), col(static_cast<size_t>(n * n));
   for (int64_t j = 0; j < n; ++j) {
@@ -494,6 +544,94 @@ DeviceBuffer<double> dense_coarse_inverse(Level& L) {
   return upload_vec(inv);
 }
 
+
+template <class R>
+std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
+  auto mg = std::make_unique<MGPrecond<R>>();
+  const int L = desc->level_count;
+  mg->ws = desc->levels[L - 1];
+  require(mg->ws != nullptr, GMF_EINVAL, "null level workspace");
+  mg->smoother = desc->smoother;
+  mg->pre = desc->pre_steps;
+  mg->post = desc->post_steps;
+  mg->omega = desc->damping;
+  mg->cheb_degree = desc->chebyshev_degree;
+  mg->cheb_range = desc->chebyshev_range;
+  mg->coarse_rtol = desc->coarse_rtol;
+  mg->coarse_atol = desc->coarse_atol;
+  mg->coarse_maxiter = desc->coarse_maxiter;
+  mg->levels.resize(size_t(L));
+  const bool cheb = desc->smoother == GMF_SMOOTHER_CHEBYSHEV;
+  for (int l = 0; l < L; ++l) {
+    Level<R>& lv = mg->levels[size_t(l)];
+    lv.ws = desc->levels[l];
+    require(lv.ws != nullptr, GMF_EINVAL, "null level workspace");
+    lv.ws->stream = mg->ws->stream;
+    lv.n = lv.ws->plan->n_red;
+    const size_t n = size_t(lv.n);
+    if (l > 0 && !MGPrecond<R>::kDouble) prepare_fp32_workspace(lv.ws);
+    DeviceBuffer<double> inv64(n);
+    {
+      DeviceBuffer<double> d(n);
+      reduced_diagonal(lv.ws, d.p);
+      invert_diagonal(lv.ws, d.p, inv64.p, lv.n);
+    }
+    if (l > 0 && cheb && lv.n > 0) {
+      lv.lambda_max = 1.2 * estimate_lambda_max(lv.ws, lv.n, inv64.p, desc->eigen_iterations);
+      require(std::isfinite(lv.lambda_max) && lv.lambda_max > 0.0, GMF_ENOTSPD,
+              "Chebyshev smoother needs a positive spectrum of D^-1 A");
+    }
+    if (l > 0) {
+      lv.invdiag = DeviceBuffer<R>(n);
+      if constexpr (MGPrecond<R>::kDouble)
+        copy(lv.invdiag.p, inv64.p, lv.n, lv.ws->stream);
+      else
+        narrow(lv.invdiag.p, inv64.p, lv.n, lv.ws->stream);
+      lv.q = DeviceBuffer<R>(n);
+      if (cheb) {
+        lv.r = DeviceBuffer<R>(n);
+        lv.d = DeviceBuffer<R>(n);
+      }
+    } else {
+      mg->coarse_invdiag = std::move(inv64);
+    }
+    if (l < L - 1) {
+      lv.x = DeviceBuffer<R>(n);
+      lv.b = DeviceBuffer<R>(n);
+    }
+    mg->level_n.push_back(lv.n);
+    mg->level_lambda.push_back(lv.lambda_max);
+  }
+  for (int l = 0; l + 1 < L; ++l)
+    mg->transfers.push_back(
+        build_transfer<R>(desc->transfers[l], mg->levels[size_t(l)].n, mg->levels[size_t(l + 1)].n));
+  Level<R>& C = mg->levels[0];
+  if (C.n <= desc->coarse_direct_limit) {
+    mg->dense = true;
+    if (C.n > 0) mg->coarse_inverse = dense_coarse_inverse(C.ws, C.n);
+  } else {
+    mg->coarse_jacobi = std::make_unique<JacobiPrecond>();
+    mg->coarse_jacobi->ws = C.ws;
+    mg->coarse_jacobi->n = C.n;
+    mg->coarse_jacobi->invdiag = mg->coarse_invdiag.p;
+    mg->coarse_jacobi->owns = false;
+    mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
+  }
+  if (!MGPrecond<R>::kDouble) {
+    mg->coarse_b = DeviceBuffer<double>(size_t(C.n));
+    mg->coarse_x = DeviceBuffer<double>(size_t(C.n));
+  }
+  // The V-cycle has no host synchronization when the coarse solve is dense,
+  // so it can be captured once and replayed.
+  mg->use_graph = desc->graph != 0 && mg->dense;
+  if (mg->use_graph || !MGPrecond<R>::kDouble) {
+    mg->top_b = DeviceBuffer<R>(size_t(mg->levels.back().n));
+    mg->top_x = DeviceBuffer<R>(size_t(mg->levels.back().n));
+  }
+  sync(mg->ws);
+  return mg;
+}
+
 }  // namespace
 }  // namespace gmf
 
@@ -516,86 +654,20 @@ gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** o
     require(std::isfinite(desc->coarse_rtol) && desc->coarse_rtol >= 0 && std::isfinite(desc->coarse_atol) &&
                 desc->coarse_atol >= 0 && desc->coarse_maxiter >= 1,
             GMF_EINVAL, "invalid coarse solver tolerances");
-    auto mg = std::make_unique<MGPrecond>();
-    const int L = desc->level_count;
-    mg->ws = desc->levels[L - 1];
-    require(mg->ws != nullptr, GMF_EINVAL, "null level workspace");
-    mg->smoother = desc->smoother;
-    mg->pre = desc->pre_steps;
-    mg->post = desc->post_steps;
-    mg->omega = desc->damping;
-    mg->cheb_degree = desc->chebyshev_degree;
-    mg->cheb_range = desc->chebyshev_range;
-    mg->coarse_rtol = desc->coarse_rtol;
-    mg->coarse_atol = desc->coarse_atol;
-    mg->coarse_maxiter = desc->coarse_maxiter;
-    mg->levels.resize(size_t(L));
-    const bool cheb = desc->smoother == GMF_SMOOTHER_CHEBYSHEV;
-    for (int l = 0; l < L; ++l) {
-      Level& lv = mg->levels[size_t(l)];
-      lv.ws = desc->levels[l];
-      require(lv.ws != nullptr, GMF_EINVAL, "null level workspace");
-      lv.ws->stream = mg->ws->stream;
-      lv.n = lv.ws->plan->n_red;
-      const size_t n = size_t(lv.n);
-      lv.invdiag = DeviceBuffer<double>(n);
-      {
-        DeviceBuffer<double> d(n);
-        reduced_diagonal(lv.ws, d.p);
-        invert_diagonal(lv.ws, d.p, lv.invdiag.p, lv.n);
-      }
-      if (l < L - 1) {
-        lv.x = DeviceBuffer<double>(n);
-        lv.b = DeviceBuffer<double>(n);
-      }
-      if (l > 0) {
-        lv.q = DeviceBuffer<double>(n);
-        if (cheb) {
-          lv.r = DeviceBuffer<double>(n);
-          lv.d = DeviceBuffer<double>(n);
-          lv.sdiag = DeviceBuffer<double>(n);
-          if (lv.n > 0) {
-            lv.lambda_max = 1.2 * estimate_lambda_max(lv, desc->eigen_iterations);
-            require(std::isfinite(lv.lambda_max) && lv.lambda_max > 0.0, GMF_ENOTSPD,
-                    "Chebyshev smoother needs a positive spectrum of D^-1 A");
-          }
-        }
-      }
-    }
-    for (int l = 0; l + 1 < L; ++l)
-      mg->transfers.push_back(
-          build_transfer(desc->transfers[l], mg->levels[size_t(l)].n, mg->levels[size_t(l + 1)].n));
-    Level& C = mg->levels[0];
-    if (C.n <= desc->coarse_direct_limit) {
-      mg->dense = true;
-      if (C.n > 0) mg->coarse_inverse = dense_coarse_inverse(C);
-    } else {
-      mg->coarse_jacobi = std::make_unique<JacobiPrecond>();
-      mg->coarse_jacobi->ws = C.ws;
-      mg->coarse_jacobi->n = C.n;
-      mg->coarse_jacobi->invdiag = C.invdiag.p;
-      mg->coarse_jacobi->owns = false;
-      mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
-    }
-    // The V-cycle has no host synchronization when the coarse solve is dense,
-    // so it can be captured once and replayed.
-    mg->use_graph = desc->graph != 0 && mg->dense;
-    if (mg->use_graph) {
-      mg->top_b = DeviceBuffer<double>(size_t(mg->levels.back().n));
-      mg->top_x = DeviceBuffer<double>(size_t(mg->levels.back().n));
-    }
-    sync(mg->ws);
+    require(desc->precision == 0 || desc->precision == 32 || desc->precision == 64, GMF_EINVAL,
+            "multigrid precision must be 32 or 64");
+    std::unique_ptr<MGBase> mg = desc->precision == 32 ? build_mg<float>(desc) : build_mg<double>(desc);
     *out = mg.release();
   });
 }
 
 gmf_status gmf_mg_level_info(const gmf_preconditioner* M, int32_t level, int64_t* reduced_dofs, double* lambda_max) {
   return guarded_call([&] {
-    const auto* mg = dynamic_cast<const MGPrecond*>(M);
+    const auto* mg = dynamic_cast<const MGBase*>(M);
     require(mg != nullptr, GMF_EINVAL, "not a multigrid preconditioner");
-    require(level >= 0 && size_t(level) < mg->levels.size(), GMF_EINVAL, "level out of range");
-    if (reduced_dofs) *reduced_dofs = mg->levels[size_t(level)].n;
-    if (lambda_max) *lambda_max = mg->levels[size_t(level)].lambda_max;
+    require(level >= 0 && size_t(level) < mg->level_n.size(), GMF_EINVAL, "level out of range");
+    if (reduced_dofs) *reduced_dofs = mg->level_n[size_t(level)];
+    if (lambda_max) *lambda_max = mg->level_lambda[size_t(level)];
   });
 }
 

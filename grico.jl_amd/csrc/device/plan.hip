@@ -251,6 +251,31 @@ void reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
   project(ws, ws->full_out, yr);
 }
 
+void prepare_fp32_workspace(gmf_workspace* ws) {
+  gmf_plan& p = *ws->plan;
+  if (prepare_fp32(p)) return;
+  if (!ws->red64_in) ws->red64_in = dalloc<double>(size_t(std::max<int64_t>(p.n_red, 1)));
+  if (!ws->red64_out) ws->red64_out = dalloc<double>(size_t(std::max<int64_t>(p.n_red, 1)));
+}
+
+void reduced_apply_f32(gmf_workspace* ws, const float* xr, float* yr) {
+  gmf_plan& p = *ws->plan;
+  require(xr != yr, GMF_EALIAS, "result and reduced coefficient vector must not alias");
+  require(p.fp32_state != 0, GMF_EINVAL, "workspace is not prepared for FP32 applies");
+  if (p.fp32_state > 0) {
+    if (!plan_writes_every_dof(p)) fill(yr, 0.0f, p.n_red, ws->stream);
+    for (const Bucket& b : p.buckets)
+      require(launch_cell_fast32(p, b, xr, yr, true, reinterpret_cast<float*>(ws->brick_slots), ws->stream),
+              GMF_EINVAL, "bucket without FP32 kernel");
+    return;
+  }
+  // Buckets without FP32 kernels: FP64 apply of the widened input, rounded.
+  require(ws->red64_in && ws->red64_out, GMF_EINVAL, "workspace is not prepared for FP32 applies");
+  widen(ws->red64_in, xr, p.n_red, ws->stream);
+  reduced_apply(ws, ws->red64_in, ws->red64_out);
+  narrow(yr, ws->red64_out, p.n_red, ws->stream);
+}
+
 void reduced_diagonal(gmf_workspace* ws, double* diag) {
   const gmf_plan& p = *ws->plan;
   require(p.selection_map, GMF_EUNSUPPORTED,
@@ -634,7 +659,7 @@ gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** out) {
 gmf_status gmf_workspace_destroy(gmf_workspace* ws) {
   return guarded([&] {
     if (!ws) return;
-    for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars, ws->brick_slots})
+    for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars, ws->brick_slots, ws->red64_in, ws->red64_out})
       if (v) (void)hipFree(v);
     if (ws->host_scalars) (void)hipHostFree(ws->host_scalars);
     delete ws;
@@ -712,6 +737,17 @@ gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* xr, double* yr) {
   return guarded([&] {
     require(ws && xr && yr, GMF_EINVAL, "null argument");
     reduced_apply(ws, xr, yr);
+    sync(ws);
+  });
+}
+
+gmf_status gmf_reduced_apply_f32(gmf_workspace* ws, const float* xr, float* yr) {
+  return guarded([&] {
+    require(ws && xr && yr, GMF_EINVAL, "null argument");
+    require_device_pointer(xr, "x_red");
+    require_device_pointer(yr, "y_red");
+    prepare_fp32_workspace(ws);
+    reduced_apply_f32(ws, xr, yr);
     sync(ws);
   });
 }

@@ -60,7 +60,7 @@ class MgDesc(ctypes.Structure):
         ("smoother", c_int32), ("pre_steps", c_int32), ("post_steps", c_int32), ("damping", c_double),
         ("chebyshev_degree", c_int32), ("chebyshev_range", c_double), ("eigen_iterations", c_int32),
         ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
-        ("coarse_maxiter", c_int64), ("graph", c_int32),
+        ("coarse_maxiter", c_int64), ("graph", c_int32), ("precision", c_int32),
     ]
 
 
@@ -84,7 +84,7 @@ def declare(lib):
     lib.gmf_workspace_destroy.argtypes = [c_void_p]
     lib.gmf_workspace_set_stream.argtypes = [c_void_p, c_void_p]
     for name in ("gmf_apply", "gmf_apply_async", "gmf_apply_physical", "gmf_reduced_apply",
-                 "gmf_project_reduced"):
+                 "gmf_reduced_apply_f32", "gmf_project_reduced"):
         getattr(lib, name).argtypes = [c_void_p, c_void_p, c_void_p]
     lib.gmf_vector_copy.argtypes = [c_void_p, c_void_p, c_void_p, c_int64]
     lib.gmf_plan_describe.argtypes = [c_void_p, ctypes.c_char_p, c_int64]
@@ -102,7 +102,8 @@ def declare(lib):
                                      c_double, c_double, c_int64, ctypes.POINTER(SolveInfo)]
     for name in ("gmf_plan_create", "gmf_plan_destroy", "gmf_plan_sizes", "gmf_workspace_create",
                  "gmf_workspace_destroy", "gmf_workspace_set_stream", "gmf_apply", "gmf_apply_async",
-                 "gmf_apply_physical", "gmf_reduced_apply", "gmf_project_reduced", "gmf_expand_reduced",
+                 "gmf_apply_physical", "gmf_reduced_apply", "gmf_reduced_apply_f32", "gmf_project_reduced",
+                 "gmf_expand_reduced",
                  "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
                  "gmf_fgmres_solve", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
                  "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy"):

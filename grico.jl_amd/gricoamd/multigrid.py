@@ -50,13 +50,20 @@ def _tolerance(v, name):
 
 
 class GeometricMultigridPreconditioner:
-    """hp geometric multigrid policy (src/multigrid.jl:51-96)."""
+    """hp geometric multigrid policy (src/multigrid.jl:51-96).
+
+    Additions without a reference counterpart: the Chebyshev-Jacobi smoother
+    (`chebyshev_degree`, `chebyshev_range`, `eigen_iterations`), `graph`
+    (replay the V-cycle as a captured HIP graph) and `precision`: "fp64", or
+    "fp32" for smoothers, transfers and level operators in FP32 (coarse solve,
+    diagonals and eigenvalue estimates in FP64) inside the FP64 outer Krylov
+    method."""
 
     def __init__(self, min_degree=1, max_levels=16, p_sequence="bisect", pre_smoothing_steps=None,
                  post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
                  coarse_relative_tolerance=1.0e-12, coarse_absolute_tolerance=0.0, coarse_maxiter=10_000,
                  coarse_direct_dof_limit=_DEFAULT_COARSE_DIRECT_DOF_LIMIT, chebyshev_degree=4,
-                 chebyshev_range=20.0, eigen_iterations=12, graph=True):
+                 chebyshev_range=20.0, eigen_iterations=12, graph=True, precision="fp64"):
         self.min_degree = _nonnegative_int(min_degree, "min_degree")
         self.max_levels = _positive_int(max_levels, "max_levels")
         if p_sequence not in ("bisect", "decrease_by_one", "go_to_one"):
@@ -85,6 +92,9 @@ class GeometricMultigridPreconditioner:
         self.chebyshev_range = float(chebyshev_range)
         self.eigen_iterations = _positive_int(eigen_iterations, "eigen_iterations")
         self.graph = bool(graph)
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
 
     def resolved_smoother(self, operator_class):
         if self.smoother == "auto":
@@ -330,6 +340,23 @@ class MultigridHierarchy:
         self.transfers = [ReducedTransfer(plans[i], plans[i + 1]) for i in range(len(plans) - 1)]
         self._device = None
 
+    def with_policy(self, policy):
+        """The same level plans and transfers under another smoother, step
+        count or precision (the coarsening parameters must agree)."""
+        for name in ("min_degree", "max_levels", "p_sequence"):
+            if getattr(policy, name) != getattr(self.policy, name):
+                raise ValueError(f"with_policy keeps the hierarchy: {name} must agree")
+        smoother = policy.resolved_smoother(self.problem.operator_class)
+        if smoother == "gmres":
+            raise NotImplementedError("the GMRES multigrid smoother (nonsymmetric operators) is not implemented "
+                                      "on the device; use smoother='jacobi' or 'chebyshev'")
+        other = object.__new__(MultigridHierarchy)
+        other.__dict__.update(self.__dict__)
+        other.policy = policy
+        other.smoother = smoother
+        other._device = None
+        return other
+
     @property
     def level_count(self):
         return len(self.plans)
@@ -377,6 +404,7 @@ class MultigridHierarchy:
             d.coarse_atol = pol.coarse_absolute_tolerance
             d.coarse_maxiter = pol.coarse_maxiter
             d.graph = int(pol.graph)
+            d.precision = 32 if pol.precision == "fp32" else 64
             handle = ctypes.c_void_p()
             _devabi.check(lib.gmf_precond_create_mg(ctypes.byref(d), ctypes.byref(handle)))
             self._device = DevicePreconditioner(self.plans[-1], handle, keep=(keep, self))

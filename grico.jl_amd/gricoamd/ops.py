@@ -91,6 +91,17 @@ def reduced_apply(plan, x):
     return _out_like(x, y)
 
 
+def reduced_apply_f32(plan, x):
+    """FP32 reduced action (the FP32 multigrid level operator); returns FP64
+    values of the FP32 result."""
+    torch = _torch()
+    dev = _ws(plan)
+    xt = to_device(x, plan.reduced_dof_count).to(torch.float32)
+    y = torch.empty_like(xt)
+    _call("gmf_reduced_apply_f32", dev.ws, xt.data_ptr(), y.data_ptr())
+    return _out_like(x, y.to(torch.float64))
+
+
 def reduced_apply_into(plan, y, x):
     dev = _ws(plan)
     _call("gmf_reduced_apply", dev.ws, x.data_ptr(), y.data_ptr())

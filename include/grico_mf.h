@@ -166,6 +166,14 @@ gmf_status gmf_apply_async(gmf_workspace* ws, const double* x, double* y);
 gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y);
 /* y_red = P^T A P x_red (src/assembly.jl:905-915 _reduced_apply!). */
 gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* x_red, double* y_red);
+/* The same operator on FP32 vectors, the level operator of the FP32 V-cycle:
+ * FP32 cell kernels where every bucket of the plan has them (3D tensor cells
+ * with N = Q <= 8 and the diffusion form, selection reduced maps), otherwise
+ * the FP64 action of the widened input rounded to FP32. No reference
+ * counterpart (the reference computes in the vector element type,
+ * src/assembly.jl:905). The first call on a workspace allocates its FP32
+ * copies of the per-cell arrays. */
+gmf_status gmf_reduced_apply_f32(gmf_workspace* ws, const float* x_red, float* y_red);
 /* x_full = [shift] + P x_red (src/assembly.jl:851-874 _expand_reduced!). */
 gmf_status gmf_expand_reduced(gmf_workspace* ws, const double* x_red, double* x_full, int32_t include_shift);
 /* y_red = P^T y_full (src/assembly.jl:876-891 _project_reduced!). */
@@ -235,6 +243,14 @@ typedef struct gmf_mg_desc {
   double coarse_atol;
   int64_t coarse_maxiter;
   int32_t graph; /* 1: replay the V-cycle as a captured HIP graph (dense coarse solve only) */
+  /* Level arithmetic: 64 (or 0) FP64 throughout; 32: smoothers, transfers and
+   * the level operators in FP32 (FP32 cell kernels where the level's buckets
+   * have them, else the FP64 apply of the widened vector, rounded), while the
+   * coarse solve, the Chebyshev eigenvalue estimates and the Jacobi diagonals
+   * are computed in FP64. The outer Krylov method stays FP64: z = M^-1 r
+   * rounds r to FP32 and widens the FP32 V-cycle result. No reference
+   * counterpart (the reference V-cycle runs in the vector element type). */
+  int32_t precision;
 } gmf_mg_desc;
 
 /* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR

@@ -164,3 +164,89 @@ def test_graph_replays_match_eager(gpu):
         ze = _device_vcycle(he, r)
         gpu.cuda.synchronize()
         assert np.abs(zg - ze).max() <= 1e-13 * np.abs(ze).max()
+
+
+# ---- FP32 V-cycle inside the FP64 Krylov method ---------------------------------
+
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_fp32_vcycle_matches_oracle(gpu, name, smoother):
+    """The FP32 V-cycle (FP32 level vectors, smoothers, transfers and level
+    operators; FP64 coarse solve) against the oracle's FP64 V-cycle with the
+    same eigenvalue bounds. Tolerance: 5e-5 of the largest output entry, a
+    few hundred FP32 roundings (unit 6e-8) through the smoothing steps of all
+    levels; the error must also exceed 1e-12 of it, so that the FP32 path is
+    the one that ran."""
+    problem = CASES[name]()
+    pol = GeometricMultigridPreconditioner(smoother=smoother, precision="fp32")
+    h = compile_geometric_multigrid(problem, pol)
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    lambdas = h.level_lambda_max() if smoother == "chebyshev" else None
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 11)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r, smoother=smoother, pre=pol.pre_smoothing_steps,
+                          post=pol.post_smoothing_steps, omega=pol.smoother_damping, lambdas=lambdas,
+                          cheb_degree=pol.chebyshev_degree, cheb_range=pol.chebyshev_range)
+    err = np.abs(z - ref).max()
+    scale = np.abs(ref).max()
+    assert err <= 5e-5 * scale
+    assert err > 1e-12 * scale
+
+
+@pytest.mark.parametrize("name", ["hp_3d", "hp_3d_p4"])
+def test_fp32_levels_use_fp32_kernels(gpu, name):
+    """Uniform 3D levels run FP32 cell kernels: the FP32 level operator agrees
+    with the FP64 one to FP32 accuracy, not to FP64 accuracy."""
+    h = compile_geometric_multigrid(CASES[name](), GeometricMultigridPreconditioner(smoother="chebyshev",
+                                                                                    precision="fp32"))
+    h.device_preconditioner()
+    plan = h.plans[-1]
+    x = P.rng_vector(plan.reduced_dof_count, 5)
+    y64 = g.reduced_apply(plan, x)
+    y32 = g.ops.reduced_apply_f32(plan, x)
+    scale = np.abs(y64).max()
+    assert np.abs(y32 - y64).max() <= 2e-6 * scale
+    assert np.abs(y32 - y64).max() > 1e-12 * scale
+
+
+@pytest.mark.parametrize("degree", [2, 4])
+def test_fp32_gmg_cg_poisson_3d(gpu, degree):
+    """FP64 CG preconditioned by the FP32 Chebyshev V-cycle reaches 1e-10 and
+    the Jacobi-CG solution; it needs at most two iterations more than the FP64
+    V-cycle."""
+    pr = P.poisson(3, None, degree, refine_depth=2, kappa=P.kappa_c2)
+    pol64 = GeometricMultigridPreconditioner(smoother="chebyshev")
+    pol32 = GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32")
+    s64 = g.solve(pr, solver=g.CGSolver(preconditioner=pol64), relative_tolerance=1e-10)
+    s32 = g.solve(pr, solver=g.CGSolver(preconditioner=pol32), relative_tolerance=1e-10)
+    jac = g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()),
+                  relative_tolerance=1e-12)
+    scale = np.abs(jac.coefficients).max()
+    assert s32.info.converged == 1
+    assert np.abs(s32.coefficients - jac.coefficients).max() <= 1e-8 * scale
+    assert s32.info.iterations <= s64.info.iterations + 2
+
+
+def test_fp32_graph_replays_match_eager(gpu):
+    """Replays of the captured FP32 V-cycle agree with the eager one to FP32
+    rounding (the brick kernels add shared lattice points with LDS atomics,
+    whose order varies from run to run)."""
+    make = lambda: P.poisson(3, None, 4, refine_depth=1, kappa=P.kappa_c2, rhs=False)
+    hg = compile_geometric_multigrid(make(), GeometricMultigridPreconditioner(smoother="chebyshev", graph=True,
+                                                                              precision="fp32"))
+    he = compile_geometric_multigrid(make(), GeometricMultigridPreconditioner(smoother="chebyshev", graph=False,
+                                                                              precision="fp32"))
+    n = hg.plans[-1].reduced_dof_count
+    for seed in range(4):
+        r = P.rng_vector(n, 200 + seed)
+        zg = _device_vcycle(hg, r)
+        g.apply(hg.plans[-1], P.rng_vector(hg.plans[-1].full_dof_count, seed))
+        ze = _device_vcycle(he, r)
+        gpu.cuda.synchronize()
+        assert np.abs(zg - ze).max() <= 1e-5 * np.abs(ze).max()
+
+
+def test_precision_policy_errors():
+    with pytest.raises(ValueError):
+        GeometricMultigridPreconditioner(precision="fp16")
