@@ -199,3 +199,61 @@ def fcm_ball(n=8, degree=2, depth=2, gamma=None, n_theta=48, n_phi=96, alpha=1e-
         pr.add_surface_quadrature(sq)
     pr.add_surface_accumulator(u, u, NitscheDirichlet(gamma if gamma is not None else 10.0 * (degree + 1) ** 2))
     return pr
+
+
+# ---- adaptive hp corner problem ---------------------------------------------------
+
+def corner_exact(p):
+    """u = r^(2/3), r = |x| (the corner singularity at the origin)."""
+    r2 = 0.0
+    for x in p:
+        r2 = r2 + x * x
+    return np.sqrt(r2) ** (2.0 / 3.0)
+
+
+class CornerLoad:
+    """-div grad r^a = -a (a + 1) r^(a - 2) in 3D; a = 2/3 gives -(10/9) r^(-4/3)."""
+
+    def cell_rhs_accumulate(self, q, test_component):
+        r2 = 0.0
+        for x in q.point:
+            r2 = r2 + x * x
+        return -(10.0 / 9.0) * r2 ** (-2.0 / 3.0)
+
+
+def adaptive_hp(base=4, levels=3, pmax=8):
+    """3D Poisson on [0,1]^3 shaped like the adaptive hp workload: base^3 cells,
+    `levels` rounds of dyadic refinement of the leaf at the corner (0,0,0)
+    (all three axes on even rounds, x and y only on odd rounds, so cells are
+    anisotropic and faces carry hanging nodes), per-leaf degree 1..pmax rising
+    with the distance of the leaf center from the corner, u = r^(2/3)
+    manufactured (load CornerLoad, Dirichlet data u on every face)."""
+    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (base,) * 3)
+    for lev in range(levels):
+        axes = (0, 1, 2) if lev % 2 == 0 else (0, 1)
+        lowers = domain.cell_lowers(domain.active_leaves())
+        corner = [int(l) for l, lo in zip(domain.active_leaves(), lowers) if np.all(lo == 0.0)]
+        for leaf in corner:
+            cur = [leaf]
+            for a in axes:
+                nxt = []
+                for c in cur:
+                    f = domain.refine(c, a)
+                    nxt += [f, f + 1]
+                cur = nxt
+
+    def degree(dom, leaf):
+        lo = dom.cell_lowers([leaf])[0]
+        h = dom.cell_sizes([leaf])[0]
+        d = float(np.linalg.norm(lo + 0.5 * h)) / np.sqrt(3.0)
+        return int(min(pmax, 1 + np.floor(pmax * d)))
+
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.ByLeafDegrees(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Diffusion())
+    problem.add_cell_accumulator(u, CornerLoad())
+    for axis in range(3):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), corner_exact))
+    return problem
