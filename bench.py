@@ -177,6 +177,139 @@ def solve_leg(problem, plan, repeats=3):
     }
 
 
+def build_adaptive_problem(base=16, levels=5, pmax=8):
+    """3D adaptive hp Poisson: base^3 cells from uniform refinements of one root,
+    `levels` rounds of dyadic refinement of the leaf at (0,0,0) (all axes on
+    even rounds, x and y on odd rounds: anisotropic cells, hanging nodes),
+    per-leaf degree 1..pmax rising with the distance of the leaf center from
+    the corner, u = r^(2/3) manufactured (-div grad u = -(10/9) r^(-4/3)),
+    Dirichlet data u on every face."""
+    import numpy as np
+
+    import gricoamd as g
+
+    def exact(p):
+        return np.sqrt(p[0] ** 2 + p[1] ** 2 + p[2] ** 2) ** (2.0 / 3.0)
+
+    class Diffusion:
+        def cell_accumulate(self, q, trial, test_component):
+            return g.TestChannels(0.0 * trial.value, tuple(trial.gradient))
+
+    class Load:
+        def cell_rhs_accumulate(self, q, test_component):
+            x, y, z = q.point
+            return -(10.0 / 9.0) * (x * x + y * y + z * z) ** (-2.0 / 3.0)
+
+    domain = g.Domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (1, 1, 1))
+    g.uniform_refine(domain, int(round(np.log2(base))))
+    for lev in range(levels):
+        axes = (0, 1, 2) if lev % 2 == 0 else (0, 1)
+        leaves = domain.active_leaves()
+        for leaf in [int(l) for l, lo in zip(leaves, domain.cell_lowers(leaves)) if np.all(lo == 0.0)]:
+            cur = [leaf]
+            for a in axes:
+                nxt = []
+                for c in cur:
+                    f = domain.refine(c, a)
+                    nxt += [f, f + 1]
+                cur = nxt
+
+    def degree(dom, leaf):
+        lo = dom.cell_lowers([leaf])[0]
+        h = dom.cell_sizes([leaf])[0]
+        return int(min(pmax, 1 + np.floor(pmax * float(np.linalg.norm(lo + 0.5 * h)) / np.sqrt(3.0))))
+
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.ByLeafDegrees(degree)))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Diffusion())
+    problem.add_cell_accumulator(u, Load())
+    for axis in range(3):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), exact))
+    return problem, exact
+
+
+def adaptive_leg(repeats=3, steps=20):
+    """Adaptive hp workload: full-layout apply throughput, GMG-CG (FP32 and FP64
+    Chebyshev V-cycles) and Jacobi-CG to 1e-8 from x0 = 0, relative L2 error
+    of the GMG solution against u = r^(2/3)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import gricoamd as g
+    from gricoamd import _devabi, _native, ops
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+    from gricoamd.solve import compile_preconditioner
+
+    lib = _native.device()
+    t0 = time.perf_counter()
+    problem, exact = build_adaptive_problem()
+    plan = g.compile(problem)
+    t_plan = time.perf_counter() - t0
+    dev = ops._ws(plan)
+    degrees = sorted({b.degrees[0] for b in plan.buckets})
+    log(f"adaptive: host compile {t_plan:.1f}s, full dofs {plan.full_dof_count}, reduced {plan.reduced_dof_count}, "
+        f"cells {sum(b.cell_count for b in plan.buckets)}, degrees {degrees}")
+    n = plan.full_dof_count
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        _devabi.check(lib.gmf_apply_async(dev.ws, x.data_ptr(), y.data_ptr()))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        _devabi.check(lib.gmf_apply_async(dev.ws, x.data_ptr(), y.data_ptr()))
+    torch.cuda.synchronize()
+    apply_s = (time.perf_counter() - t) / steps
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    xs = torch.zeros_like(b)
+
+    def run(handle, maxiter):
+        info = _devabi.SolveInfo()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), xs.data_ptr(), 0, 1e-8, 0.0, maxiter,
+                                       ctypes.byref(info)))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, info
+
+    t0 = time.perf_counter()
+    h64 = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"), fine_plan=plan)
+    t_h = time.perf_counter() - t0
+    out = {"workload": "adaptive hp Poisson, 16^3 base cells (4 uniform refinements of one root), 5 rounds of "
+                       "anisotropic dyadic refinement at (0,0,0), degree 1..8 rising with distance, u = r^(2/3)",
+           "full_dofs": n, "reduced_dofs": plan.reduced_dof_count,
+           "buckets": [list(bk.degrees) + [bk.cell_count] for bk in plan.buckets],
+           "apply_gdofs": n / apply_s / 1e9, "apply_seconds": apply_s,
+           "levels": h64.reduced_dof_counts(), "host_compile_seconds": t_plan, "hierarchy_seconds": t_h}
+    for tag, h in (("gmg_cg_fp32_vcycle", h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev",
+                                                                                            precision="fp32"))),
+                   ("gmg_cg", h64)):
+        M = h.device_preconditioner()
+        run(M.handle, 2000)
+        times = []
+        for _ in range(repeats):
+            dt, info = run(M.handle, 2000)
+            times.append(dt)
+        full = ops.expand_reduced(plan, xs, include_shift=True).cpu().numpy()
+        err = g.relative_l2_error(g.State(plan, full), exact)
+        out[tag] = {"seconds": sorted(times)[len(times) // 2], "iterations": info.iterations,
+                    "final_relative_residual": info.final_residual_norm / info.rhs_norm,
+                    "relative_l2_error": err}
+        log(f"adaptive {tag}: {out[tag]}")
+    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
+    dt, info = run(J.handle, 50000)
+    full = ops.expand_reduced(plan, xs, include_shift=True).cpu().numpy()
+    out["jacobi_cg"] = {"seconds": dt, "iterations": info.iterations,
+                        "final_relative_residual": info.final_residual_norm / info.rhs_norm,
+                        "relative_l2_error": g.relative_l2_error(g.State(plan, full), exact)}
+    log(f"adaptive jacobi_cg: {out['jacobi_cg']}")
+    return out
+
+
 def build_lognormal_problem(depth=5, degree=3, seed=2024):
     """32^3 cells (five uniform refinements of one root), p = 3, kappa = exp(g)
     with g a Gaussian random field (squared-exponential covariance, correlation
@@ -318,6 +451,7 @@ def main():
     ap.add_argument("--depth", type=int, default=6, help="uniform refinements of the root (6 -> 64^3 cells)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMG-CG / Jacobi-CG solve leg")
+    ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive hp leg")
     ap.add_argument("--learned", action="store_true",
                     help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
                          "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
@@ -421,6 +555,12 @@ def main():
         del src, dst
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
+    if not args.no_adaptive:
+        try:
+            result["adaptive"] = adaptive_leg()
+        except Exception as exc:  # keep the headline line
+            log(f"adaptive leg failed: {exc!r}")
+            result["adaptive"] = {"error": repr(exc)}
     if args.learned:
         result["learned"] = learned_leg()
     if not args.no_cpu_baseline:

@@ -9,6 +9,9 @@
 // (src/assembly.jl:1751-1803). The diagonal kernel evaluates the reference's
 // cell_diagonal! (src/accumulators.jl:1056-1083) by sum factorization with
 // products of 1D tables instead of one basis probe per mode.
+#include <algorithm>
+#include <vector>
+
 #include "common.hpp"
 #include "termmap.hpp"
 
@@ -169,6 +172,33 @@ __global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const do
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
 }
 
+// Cells of several buckets in one launch: block b of the grid belongs to the
+// bucket whose cumulative cell range holds it (buckets ordered by decreasing
+// work so the longest cells start first).
+__global__ void __launch_bounds__(64) cell_apply_generic_batched(const GenericArgs* __restrict__ args,
+                                                                 const int64_t* __restrict__ start, int count,
+                                                                 const double* __restrict__ x,
+                                                                 double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const int64_t blk = blockIdx.x;
+  int k = 0;
+  while (k + 1 < count && blk >= start[k + 1]) ++k;
+  const GenericArgs a = args[k];
+  const int64_t cell = blk - start[k];
+  double* u = smem;
+  double* bufA = u + a.M;
+  double* bufB = bufA + a.M;
+  double* fields = bufB + a.M;
+  double* r = fields + (a.D + 1) * a.Qtot;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+    u[i] = gather_one(a, cell, i, x);
+    r[i] = 0.0;
+  }
+  __syncthreads();
+  apply_body(a, cell, u, r, bufA, bufB, fields);
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
+}
+
 // One (cell, reduced dof) pair per workgroup: diag[dof] += v^T A_c v with v the
 // dof's local coefficient vector.
 __global__ void __launch_bounds__(64) cell_diagonal_pairs(GenericArgs a, const int64_t* __restrict__ pcell,
@@ -275,6 +305,58 @@ void launch_cell_generic(const gmf_plan&, const Bucket& b, const double* x, doub
   require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
   hipLaunchKernelGGL(cell_apply_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, x, y);
   GMF_HIP(hipGetLastError());
+}
+
+void prepare_generic_batch(gmf_plan& p) {
+  GenericBatch& g = p.generic_batch;
+  g = GenericBatch{};
+  std::vector<int> ids;
+  for (size_t i = 0; i < p.buckets.size(); ++i) {
+    const Bucket& b = p.buckets[i];
+    if (!b.cloud && !b.fast && b.cells > 0) ids.push_back(int(i));
+  }
+  if (ids.size() < 2) return;
+  // heaviest cells first
+  std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+    const Bucket &A = p.buckets[size_t(a)], &B = p.buckets[size_t(b)];
+    return int64_t(A.Ntot) * A.Qtot > int64_t(B.Ntot) * B.Qtot;
+  });
+  std::vector<GenericArgs> full, red;
+  std::vector<int64_t> start(1, 0);
+  bool have_red = true;
+  size_t lds = 0;
+  for (int i : ids) {
+    const Bucket& b = p.buckets[size_t(i)];
+    GenericArgs a = make_args(b, false);
+    if (a.idx == nullptr) return;
+    full.push_back(a);
+    GenericArgs ar = make_args(b, true);
+    if (ar.idx == nullptr) have_red = false;
+    red.push_back(ar);
+    start.push_back(start.back() + b.cells);
+    const size_t l = sizeof(double) * size_t(3 * a.M + (a.D + 1) * a.Qtot + a.Ntot);
+    if (l > 64 * 1024) return;
+    lds = std::max(lds, l);
+  }
+  if (start.back() >= (int64_t(1) << 31)) return;
+  g.count = int(ids.size());
+  g.buckets = ids;
+  g.cells = start.back();
+  g.lds = lds;
+  g.args_full = upload_array(p, full.data(), full.size());
+  if (have_red) g.args_red = upload_array(p, red.data(), red.size());
+  g.start = upload_array(p, start.data(), start.size());
+}
+
+bool launch_generic_batch(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream) {
+  const GenericBatch& g = p.generic_batch;
+  if (g.count < 2) return false;
+  const void* args = reduced ? g.args_red : g.args_full;
+  if (args == nullptr) return false;
+  hipLaunchKernelGGL(cell_apply_generic_batched, dim3(unsigned(g.cells)), dim3(64), g.lds, stream,
+                     static_cast<const GenericArgs*>(args), g.start, g.count, x, y);
+  GMF_HIP(hipGetLastError());
+  return true;
 }
 
 void launch_cell_diagonal_generic(const gmf_plan&, const Bucket& b, double* diag, bool reduced,

@@ -105,6 +105,19 @@ struct Bucket {
   double* pair_coef = nullptr;
 };
 
+// Generic-kernel buckets of a plan applied in one launch (cell_generic.hip):
+// per-bucket kernel arguments on the device (full and reduced index maps),
+// cumulative cell offsets, and the bucket ids covered.
+struct GenericBatch {
+  int count = 0;
+  void* args_full = nullptr;
+  void* args_red = nullptr;  // null when a bucket has no reduced index map
+  int64_t* start = nullptr;  // count + 1 cumulative cell offsets (device)
+  int64_t cells = 0;
+  size_t lds = 0;
+  std::vector<int> buckets;
+};
+
 struct Csr {
   int64_t rows = 0;
   int64_t* offsets = nullptr;
@@ -136,6 +149,7 @@ struct gmf_plan {
   int64_t device_bytes = 0;
   int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
   int fp32_state = 0;       // 0: not prepared, 1: every bucket has FP32 kernels, -1: FP64 with conversion
+  gmf::GenericBatch generic_batch;
   std::vector<void*> allocations;
 };
 
@@ -158,6 +172,11 @@ namespace gmf {
 // Kernel launchers implemented in the .hip files.
 void launch_cell_generic(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                          hipStream_t stream);
+// Batch the plan's generic-kernel buckets (two or more) into one launch.
+void prepare_generic_batch(gmf_plan& plan);
+// One launch over every bucket of plan.generic_batch; false if the batch
+// cannot serve this call (no batch, or no reduced maps for a reduced call).
+bool launch_generic_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream);
 void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced,
                                   hipStream_t stream);
 // Exact reduced-diagonal contributions of the bucket's coupled cells (and the

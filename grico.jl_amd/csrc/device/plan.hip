@@ -180,12 +180,18 @@ __global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x
 
 void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   const gmf_plan& p = *ws->plan;
-  for (const Bucket& b : p.buckets) {
+  // Generic-kernel buckets of uneven degree share one launch when batched.
+  const bool batched = launch_generic_batch(p, x, y, reduced, ws->stream);
+  for (size_t i = 0; i < p.buckets.size(); ++i) {
+    const Bucket& b = p.buckets[i];
     if (b.cloud) {
       launch_cell_cloud(p, b, x, y, reduced, ws->stream);
       continue;
     }
     if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->brick_slots, ws->stream)) continue;
+    if (batched && std::find(p.generic_batch.buckets.begin(), p.generic_batch.buckets.end(), int(i)) !=
+                       p.generic_batch.buckets.end())
+      continue;
     launch_cell_generic(p, b, x, y, reduced, ws->stream);
   }
 }
@@ -586,6 +592,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         if (!b.cloud) prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
         p.buckets.push_back(std::move(b));
       }
+      prepare_generic_batch(p);
       GMF_HIP(hipDeviceSynchronize());
     } catch (...) {
       for (void* a : p.allocations) (void)hipFree(a);

@@ -493,19 +493,31 @@ class AssemblyPlan:
 
 
 def _boundary_l2_projection(plan, cons, dofs):
-    """Fixed trace values: L2 projection of g onto the active trace space."""
+    """Fixed trace values: L2 projection of g onto the active trace space
+    (src/plans.jl:884-972). Per boundary face the basis values are tensor
+    products of per-axis tables at the face's Gauss points, the face's Gram
+    block is Phi W Phi^T; the global Gram matrix is sparse (face blocks) and
+    solved with SciPy's sparse LU."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+
+    from .quadrature import fe_basis_values_and_derivatives
+
     space = plan.space
-    index = {d: i for i, d in enumerate(dofs.tolist())}
     n = len(dofs)
-    G = np.zeros((n, n))
+    pos = np.full(plan.full_dof_count, -1, dtype=np.int64)
+    pos[dofs] = np.arange(n)
+    rows, cols, vals = [], [], []
     r = np.zeros(n)
     dom = space.domain
+    tables = {}
     for con in cons:
         terms = plan._trace_terms(con)
         by_face = {}
         for dof, coef, li, box in terms:
             by_face.setdefault(li, []).append((dof, coef, box))
         a = con.boundary.axis
+        side = -1.0 if con.boundary.side == 0 else 1.0
         for li, items in by_face.items():
             deg = space.degrees[li]
             qs = space.quadrature_shapes[li]
@@ -523,27 +535,39 @@ def _boundary_l2_projection(plan, cons, dofs):
             xi = {}
             for k, ax in enumerate(tang):
                 xi[ax] = grids[k].reshape(-1)
-            xi[a] = np.full(npts, -1.0 if con.boundary.side == 0 else 1.0)
+            xi[a] = np.full(npts, side)
             pts = tuple(centers[0, ax] + xi[ax] * halves[0, ax] for ax in range(space.dim))
             gval = np.broadcast_to(np.asarray(con.data(pts) if callable(con.data) else con.data,
                                               dtype=np.float64), (npts,))
+            # per-axis basis tables (npts x (p+1)), shared by faces of equal shape
+            tab = []
+            for ax in range(space.dim):
+                key = (a, side, tuple(int(q) for q in qs), ax, int(deg[ax]))
+                if key not in tables:
+                    tables[key] = fe_basis_values_and_derivatives(xi[ax], int(deg[ax]))[0]
+                tab.append(tables[key])
             shape = tuple(int(p) + 1 for p in deg)
-            phi = {}
-            for dof, coef, box in items:
-                idx = np.unravel_index(box, shape[::-1])[::-1]
-                val = np.ones(npts)
-                for ax in range(space.dim):
-                    from .quadrature import fe_basis_values_and_derivatives
-                    vals, _ = fe_basis_values_and_derivatives(xi[ax], int(deg[ax]))
-                    val = val * vals[:, idx[ax]]
-                phi.setdefault(dof, np.zeros(npts))
-                phi[dof] += coef * val
-            keys = list(phi)
-            for i in keys:
-                r[index[i]] += np.sum(w * gval * phi[i])
-                for j in keys:
-                    G[index[i], index[j]] += np.sum(w * phi[i] * phi[j])
-    return np.linalg.solve(G, r)
+            dof = np.array([it[0] for it in items], dtype=np.int64)
+            coef = np.array([it[1] for it in items])
+            idx = np.unravel_index(np.array([it[2] for it in items], dtype=np.int64), shape[::-1])[::-1]
+            phi_t = np.ones((npts, len(items)))
+            for ax in range(space.dim):
+                phi_t = phi_t * tab[ax][:, idx[ax]]
+            phi_t = phi_t * coef[None, :]
+            u, inv = np.unique(dof, return_inverse=True)
+            phi = np.zeros((len(u), npts))
+            np.add.at(phi, inv, phi_t.T)
+            gl = (phi * w[None, :]) @ phi.T
+            rl = phi @ (w * gval)
+            ii = pos[u]
+            rows.append(np.repeat(ii, len(u)))
+            cols.append(np.tile(ii, len(u)))
+            vals.append(gl.reshape(-1))
+            np.add.at(r, ii, rl)
+    if n == 0:
+        return np.zeros(0)
+    G = sps.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    return np.asarray(spla.spsolve(G, r), dtype=np.float64)
 
 
 def compile(problem):

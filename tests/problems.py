@@ -222,13 +222,19 @@ class CornerLoad:
 
 
 def adaptive_hp(base=4, levels=3, pmax=8):
-    """3D Poisson on [0,1]^3 shaped like the adaptive hp workload: base^3 cells,
-    `levels` rounds of dyadic refinement of the leaf at the corner (0,0,0)
+    """3D Poisson on [0,1]^3 shaped like the adaptive hp workload: base^3 cells
+    (uniform refinements of one root cell, so that multigrid can coarsen the
+    base grid too; base a power of two), `levels` rounds of dyadic refinement
+    of the leaf at the corner (0,0,0)
     (all three axes on even rounds, x and y only on odd rounds, so cells are
     anisotropic and faces carry hanging nodes), per-leaf degree 1..pmax rising
     with the distance of the leaf center from the corner, u = r^(2/3)
     manufactured (load CornerLoad, Dirichlet data u on every face)."""
-    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (base,) * 3)
+    depth = int(round(np.log2(base)))
+    if 2 ** depth != base:
+        raise ValueError("base must be a power of two")
+    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (1, 1, 1))
+    g.uniform_refine(domain, depth)
     for lev in range(levels):
         axes = (0, 1, 2) if lev % 2 == 0 else (0, 1)
         lowers = domain.cell_lowers(domain.active_leaves())
