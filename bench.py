@@ -250,6 +250,7 @@ def adaptive_leg(repeats=3, steps=20):
     plan = g.compile(problem)
     t_plan = time.perf_counter() - t0
     dev = ops._ws(plan)
+    log(dev.describe().strip())
     degrees = sorted({b.degrees[0] for b in plan.buckets})
     log(f"adaptive: host compile {t_plan:.1f}s, full dofs {plan.full_dof_count}, reduced {plan.reduced_dof_count}, "
         f"cells {sum(b.cell_count for b in plan.buckets)}, degrees {degrees}")

@@ -1,6 +1,7 @@
 // Generic tensor-product cell kernels: any dimension <= 3, any per-axis mode
-// and point counts <= 16, both lowered action forms. One workgroup (one
-// wavefront) per cell, every intermediate staged in LDS.
+// and point counts <= 16, both lowered action forms. One workgroup per cell
+// (64, 128 or 256 lanes, picked per bucket by tune_generic), every
+// intermediate and the one-dimensional tables staged in LDS.
 //
 // The algorithm is the reference's interpolate -> per-point action -> project
 // pipeline (src/accumulators.jl:1955-2013 with the sum-factorized helpers of
@@ -10,9 +11,11 @@
 // cell_diagonal! (src/accumulators.jl:1056-1083) by sum factorization with
 // products of 1D tables instead of one basis probe per mode.
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "common.hpp"
+#include "internal.hpp"
 #include "termmap.hpp"
 
 namespace gmf {
@@ -33,6 +36,7 @@ struct GenericArgs {
   const int64_t* moff;
   const int32_t* midx;
   const double* mcoef;
+  int lds_tables;  // 1: the apply kernels copy the tables to LDS (when they fit)
 };
 
 // One sum-factorization pass over the block. `shape` holds the input extents
@@ -153,50 +157,64 @@ __device__ void apply_body(const GenericArgs& a, int64_t cell, const double* u, 
   for (int d = 0; d < a.D; ++d) project_add(a, fields + (d + 1) * a.Qtot, r, bufA, bufB, d, false, -1);
 }
 
-__global__ void __launch_bounds__(64) cell_apply_generic(GenericArgs a, const double* __restrict__ x,
-                                                         double* __restrict__ y) {
-  extern __shared__ double smem[];
-  const int64_t cell = blockIdx.x;
+// Length of the concatenated per-axis tables (values or derivatives).
+__host__ __device__ inline int table_length(const GenericArgs& a) {
+  return a.toff[a.D - 1] + a.N[a.D - 1] * a.Q[a.D - 1];
+}
+
+// LDS doubles of the apply kernels: u, bufA, bufB, (D+1) fields, r and,
+// when staged, the tables.
+__host__ __device__ inline int apply_lds_doubles(const GenericArgs& a) {
+  return 3 * a.M + (a.D + 1) * a.Qtot + a.Ntot + (a.lds_tables ? 2 * table_length(a) : 0);
+}
+
+// One cell: gather, the body with the tables in LDS, scatter.
+__device__ inline void apply_cell(const GenericArgs& a, int64_t cell, double* smem, const double* __restrict__ x,
+                                  double* __restrict__ y) {
   double* u = smem;
   double* bufA = u + a.M;
   double* bufB = bufA + a.M;
-  double* fields = bufB + a.M;          // (D+1) x Qtot
+  double* fields = bufB + a.M;              // (D+1) x Qtot
   double* r = fields + (a.D + 1) * a.Qtot;  // Ntot
-
+  double* tabs = r + a.Ntot;                // values, derivatives (if staged)
+  const int tl = table_length(a);
+  if (a.lds_tables)
+    for (int i = threadIdx.x; i < tl; i += blockDim.x) {
+      tabs[i] = a.values[i];
+      tabs[tl + i] = a.derivs[i];
+    }
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
     u[i] = gather_one(a, cell, i, x);
     r[i] = 0.0;
   }
   __syncthreads();
-  apply_body(a, cell, u, r, bufA, bufB, fields);
+  GenericArgs b = a;
+  if (a.lds_tables) {
+    b.values = tabs;
+    b.derivs = tabs + tl;
+  }
+  apply_body(b, cell, u, r, bufA, bufB, fields);
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
+}
+
+__global__ void __launch_bounds__(256) cell_apply_generic(GenericArgs a, const double* __restrict__ x,
+                                                          double* __restrict__ y) {
+  extern __shared__ double smem[];
+  apply_cell(a, blockIdx.x, smem, x, y);
 }
 
 // Cells of several buckets in one launch: block b of the grid belongs to the
 // bucket whose cumulative cell range holds it (buckets ordered by decreasing
 // work so the longest cells start first).
-__global__ void __launch_bounds__(64) cell_apply_generic_batched(const GenericArgs* __restrict__ args,
-                                                                 const int64_t* __restrict__ start, int count,
-                                                                 const double* __restrict__ x,
-                                                                 double* __restrict__ y) {
+__global__ void __launch_bounds__(256) cell_apply_generic_batched(const GenericArgs* __restrict__ args,
+                                                                  const int64_t* __restrict__ start, int count,
+                                                                  const double* __restrict__ x,
+                                                                  double* __restrict__ y) {
   extern __shared__ double smem[];
   const int64_t blk = blockIdx.x;
   int k = 0;
   while (k + 1 < count && blk >= start[k + 1]) ++k;
-  const GenericArgs a = args[k];
-  const int64_t cell = blk - start[k];
-  double* u = smem;
-  double* bufA = u + a.M;
-  double* bufB = bufA + a.M;
-  double* fields = bufB + a.M;
-  double* r = fields + (a.D + 1) * a.Qtot;
-  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
-    u[i] = gather_one(a, cell, i, x);
-    r[i] = 0.0;
-  }
-  __syncthreads();
-  apply_body(a, cell, u, r, bufA, bufB, fields);
-  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
+  apply_cell(args[k], blk - start[k], smem, x, y);
 }
 
 // One (cell, reduced dof) pair per workgroup: diag[dof] += v^T A_c v with v the
@@ -291,7 +309,21 @@ GenericArgs make_args(const Bucket& b, bool reduced) {
   a.moff = b.moff;
   a.midx = reduced ? b.mridx : b.midx;
   a.mcoef = b.mcoef;
+  a.lds_tables = 1;
+  if (sizeof(double) * size_t(apply_lds_doubles(a)) > 64 * 1024) a.lds_tables = 0;
   return a;
+}
+
+size_t generic_apply_lds(const GenericArgs& a) { return sizeof(double) * size_t(apply_lds_doubles(a)); }
+
+void launch_generic_threads(const Bucket& b, const double* x, double* y, bool reduced, int threads,
+                            hipStream_t stream) {
+  GenericArgs a = make_args(b, reduced);
+  require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  const size_t lds = generic_apply_lds(a);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_apply_generic, dim3(unsigned(b.cells)), dim3(unsigned(threads)), lds, stream, a, x, y);
+  GMF_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -299,64 +331,101 @@ GenericArgs make_args(const Bucket& b, bool reduced) {
 void launch_cell_generic(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,
                          hipStream_t stream) {
   if (b.cells == 0) return;
-  GenericArgs a = make_args(b, reduced);
-  require(a.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
-  size_t lds = sizeof(double) * size_t(3 * a.M + (a.D + 1) * a.Qtot + a.Ntot);
-  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
-  hipLaunchKernelGGL(cell_apply_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, x, y);
-  GMF_HIP(hipGetLastError());
+  launch_generic_threads(b, x, y, reduced, b.generic_threads, stream);
+}
+
+void tune_generic(gmf_plan& p) {
+  int forced = 0;
+  if (const char* env = std::getenv("GMF_GENERIC_THREADS")) forced = std::atoi(env);
+  std::vector<Bucket*> todo;
+  for (Bucket& b : p.buckets)
+    if (!b.cloud && !b.fast && b.cells > 0) todo.push_back(&b);
+  if (todo.empty()) return;
+  if (forced == 64 || forced == 128 || forced == 256) {
+    for (Bucket* b : todo) b->generic_threads = forced;
+    return;
+  }
+  DeviceBuffer<double> x(size_t(std::max<int64_t>(p.n_full, 1))), y(size_t(std::max<int64_t>(p.n_full, 1)));
+  fill(x.p, 1.0, p.n_full, nullptr);
+  fill(y.p, 0.0, p.n_full, nullptr);
+  hipEvent_t e0, e1;
+  GMF_HIP(hipEventCreate(&e0));
+  GMF_HIP(hipEventCreate(&e1));
+  for (Bucket* b : todo) {
+    float best = -1.0f;
+    for (int threads : {64, 128, 256}) {
+      launch_generic_threads(*b, x.p, y.p, false, threads, nullptr);  // warm-up
+      GMF_HIP(hipEventRecord(e0, nullptr));
+      for (int rep = 0; rep < 3; ++rep) launch_generic_threads(*b, x.p, y.p, false, threads, nullptr);
+      GMF_HIP(hipEventRecord(e1, nullptr));
+      GMF_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      GMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (best < 0.0f || ms < best) {
+        best = ms;
+        b->generic_threads = threads;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
 }
 
 void prepare_generic_batch(gmf_plan& p) {
-  GenericBatch& g = p.generic_batch;
-  g = GenericBatch{};
-  std::vector<int> ids;
-  for (size_t i = 0; i < p.buckets.size(); ++i) {
-    const Bucket& b = p.buckets[i];
-    if (!b.cloud && !b.fast && b.cells > 0) ids.push_back(int(i));
+  p.generic_batches.clear();
+  for (int threads : {64, 128, 256}) {
+    std::vector<int> ids;
+    for (size_t i = 0; i < p.buckets.size(); ++i) {
+      const Bucket& b = p.buckets[i];
+      if (!b.cloud && !b.fast && b.cells > 0 && b.generic_threads == threads) ids.push_back(int(i));
+    }
+    if (ids.size() < 2) continue;
+    // heaviest cells first
+    std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+      const Bucket &A = p.buckets[size_t(a)], &B = p.buckets[size_t(b)];
+      return int64_t(A.Ntot) * A.Qtot > int64_t(B.Ntot) * B.Qtot;
+    });
+    std::vector<GenericArgs> full, red;
+    std::vector<int64_t> start(1, 0);
+    bool have_red = true, ok = true;
+    size_t lds = 0;
+    for (int i : ids) {
+      const Bucket& b = p.buckets[size_t(i)];
+      GenericArgs a = make_args(b, false);
+      GenericArgs ar = make_args(b, true);
+      if (ar.idx == nullptr) have_red = false;
+      full.push_back(a);
+      red.push_back(ar);
+      start.push_back(start.back() + b.cells);
+      const size_t l = generic_apply_lds(a);
+      if (a.idx == nullptr || l > 64 * 1024) ok = false;
+      lds = std::max(lds, l);
+    }
+    if (!ok || start.back() >= (int64_t(1) << 31)) continue;
+    GenericBatch g;
+    g.threads = threads;
+    g.count = int(ids.size());
+    g.buckets = ids;
+    g.cells = start.back();
+    g.lds = lds;
+    g.args_full = upload_array(p, full.data(), full.size());
+    if (have_red) g.args_red = upload_array(p, red.data(), red.size());
+    g.start = upload_array(p, start.data(), start.size());
+    p.generic_batches.push_back(std::move(g));
   }
-  if (ids.size() < 2) return;
-  // heaviest cells first
-  std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
-    const Bucket &A = p.buckets[size_t(a)], &B = p.buckets[size_t(b)];
-    return int64_t(A.Ntot) * A.Qtot > int64_t(B.Ntot) * B.Qtot;
-  });
-  std::vector<GenericArgs> full, red;
-  std::vector<int64_t> start(1, 0);
-  bool have_red = true;
-  size_t lds = 0;
-  for (int i : ids) {
-    const Bucket& b = p.buckets[size_t(i)];
-    GenericArgs a = make_args(b, false);
-    if (a.idx == nullptr) return;
-    full.push_back(a);
-    GenericArgs ar = make_args(b, true);
-    if (ar.idx == nullptr) have_red = false;
-    red.push_back(ar);
-    start.push_back(start.back() + b.cells);
-    const size_t l = sizeof(double) * size_t(3 * a.M + (a.D + 1) * a.Qtot + a.Ntot);
-    if (l > 64 * 1024) return;
-    lds = std::max(lds, l);
-  }
-  if (start.back() >= (int64_t(1) << 31)) return;
-  g.count = int(ids.size());
-  g.buckets = ids;
-  g.cells = start.back();
-  g.lds = lds;
-  g.args_full = upload_array(p, full.data(), full.size());
-  if (have_red) g.args_red = upload_array(p, red.data(), red.size());
-  g.start = upload_array(p, start.data(), start.size());
 }
 
-bool launch_generic_batch(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream) {
-  const GenericBatch& g = p.generic_batch;
-  if (g.count < 2) return false;
-  const void* args = reduced ? g.args_red : g.args_full;
-  if (args == nullptr) return false;
-  hipLaunchKernelGGL(cell_apply_generic_batched, dim3(unsigned(g.cells)), dim3(64), g.lds, stream,
-                     static_cast<const GenericArgs*>(args), g.start, g.count, x, y);
-  GMF_HIP(hipGetLastError());
-  return true;
+void launch_generic_batches(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream,
+                            std::vector<char>& done) {
+  done.assign(p.buckets.size(), 0);
+  for (const GenericBatch& g : p.generic_batches) {
+    const void* args = reduced ? g.args_red : g.args_full;
+    if (args == nullptr) continue;
+    hipLaunchKernelGGL(cell_apply_generic_batched, dim3(unsigned(g.cells)), dim3(unsigned(g.threads)), g.lds, stream,
+                       static_cast<const GenericArgs*>(args), g.start, g.count, x, y);
+    GMF_HIP(hipGetLastError());
+    for (int i : g.buckets) done[size_t(i)] = 1;
+  }
 }
 
 void launch_cell_diagonal_generic(const gmf_plan&, const Bucket& b, double* diag, bool reduced,

@@ -81,6 +81,8 @@ struct Bucket {
   std::vector<double> h_colloc;
   bool fast = false;
   BrickPath bricks;
+  // Lanes per cell of the generic kernel, picked by tune_generic at plan creation.
+  int generic_threads = 64;
   // FP32 copies of the per-cell arrays for FP32 multigrid levels (prepare_fp32).
   bool fp32 = false;
   float* jinv32 = nullptr;
@@ -109,6 +111,7 @@ struct Bucket {
 // per-bucket kernel arguments on the device (full and reduced index maps),
 // cumulative cell offsets, and the bucket ids covered.
 struct GenericBatch {
+  int threads = 64;          // lanes per cell of this launch
   int count = 0;
   void* args_full = nullptr;
   void* args_red = nullptr;  // null when a bucket has no reduced index map
@@ -149,7 +152,7 @@ struct gmf_plan {
   int64_t device_bytes = 0;
   int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
   int fp32_state = 0;       // 0: not prepared, 1: every bucket has FP32 kernels, -1: FP64 with conversion
-  gmf::GenericBatch generic_batch;
+  std::vector<gmf::GenericBatch> generic_batches;  // one launch per lanes-per-cell tier
   std::vector<void*> allocations;
 };
 
@@ -172,11 +175,17 @@ namespace gmf {
 // Kernel launchers implemented in the .hip files.
 void launch_cell_generic(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                          hipStream_t stream);
-// Batch the plan's generic-kernel buckets (two or more) into one launch.
+// Autotune the generic kernel: for every generic-kernel bucket time 64, 128
+// and 256 lanes per cell on scratch vectors and keep the fastest
+// (GMF_GENERIC_THREADS=64|128|256 forces one width).
+void tune_generic(gmf_plan& plan);
+// Batch the plan's generic-kernel buckets into one launch per lanes-per-cell
+// tier (tiers holding at least two buckets).
 void prepare_generic_batch(gmf_plan& plan);
-// One launch over every bucket of plan.generic_batch; false if the batch
-// cannot serve this call (no batch, or no reduced maps for a reduced call).
-bool launch_generic_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream);
+// Launch every batch; `done[i]` is set for the buckets they covered. False
+// entries must be launched per bucket (no batch, or no reduced maps).
+void launch_generic_batches(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
+                            std::vector<char>& done);
 void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced,
                                   hipStream_t stream);
 // Exact reduced-diagonal contributions of the bucket's coupled cells (and the

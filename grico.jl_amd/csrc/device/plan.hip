@@ -180,8 +180,9 @@ __global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x
 
 void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   const gmf_plan& p = *ws->plan;
-  // Generic-kernel buckets of uneven degree share one launch when batched.
-  const bool batched = launch_generic_batch(p, x, y, reduced, ws->stream);
+  // Generic-kernel buckets of uneven degree share one launch per tier.
+  std::vector<char> batched;
+  launch_generic_batches(p, x, y, reduced, ws->stream, batched);
   for (size_t i = 0; i < p.buckets.size(); ++i) {
     const Bucket& b = p.buckets[i];
     if (b.cloud) {
@@ -189,9 +190,7 @@ void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
       continue;
     }
     if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->brick_slots, ws->stream)) continue;
-    if (batched && std::find(p.generic_batch.buckets.begin(), p.generic_batch.buckets.end(), int(i)) !=
-                       p.generic_batch.buckets.end())
-      continue;
+    if (batched[i]) continue;
     launch_cell_generic(p, b, x, y, reduced, ws->stream);
   }
 }
@@ -592,6 +591,8 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         if (!b.cloud) prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
         p.buckets.push_back(std::move(b));
       }
+      GMF_HIP(hipDeviceSynchronize());
+      tune_generic(p);
       prepare_generic_batch(p);
       GMF_HIP(hipDeviceSynchronize());
     } catch (...) {
@@ -639,8 +640,12 @@ gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity) 
                (c + 1 < b.bricks.colors ? "," : "");
         s += "]";
       }
+      if (!b.cloud && !b.fast) s += " lanes=" + std::to_string(b.generic_threads);
       s += "\n";
     }
+    for (const GenericBatch& g : plan->generic_batches)
+      s += "generic batch: lanes=" + std::to_string(g.threads) + " buckets=" + std::to_string(g.count) +
+           " cells=" + std::to_string(g.cells) + "\n";
     size_t n = std::min(s.size(), size_t(capacity - 1));
     std::memcpy(buf, s.data(), n);
     buf[n] = 0;
