@@ -112,6 +112,30 @@ struct CgVectors {
 void cg_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, bool use_initial, double rtol,
             double atol, int64_t maxiter, gmf_solve_info& inf, CgVectors& v);
 
+struct FgmresVectors {
+  int64_t n = 0;
+  int m = 0;
+  DeviceBuffer<double> r, q, V, Z, ydev;
+  FgmresVectors(int64_t nn, int mm)
+      : n(nn),
+        m(mm),
+        r(static_cast<size_t>(nn)),
+        q(static_cast<size_t>(nn)),
+        V(static_cast<size_t>(nn) * static_cast<size_t>(mm + 1)),
+        Z(static_cast<size_t>(nn) * static_cast<size_t>(mm)),
+        ydev(static_cast<size_t>(mm)) {}
+};
+
+// Restarted right-preconditioned FGMRES core (src/solve.jl:433-614), restart
+// m <= v.m; throws Error on breakdown or non-convergence.
+void fgmres_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, bool use_initial, int m,
+                double rtol, double atol, int64_t maxiter, gmf_solve_info& inf, FgmresVectors& v);
+
+// One modified Gram-Schmidt step of q against the columns 0..col of V (column
+// stride n): h_i = q.v_i, q -= h_i v_i in turn, then |q|^2. The col + 2
+// values are read back into ws->host_scalars[kHessian ..].
+void arnoldi_mgs(gmf_workspace* ws, double* q, const double* V, int64_t n, int col);
+
 struct JacobiPrecond : gmf_preconditioner {
   double* invdiag = nullptr;
   bool owns = true;

@@ -186,6 +186,7 @@ struct Level {
   gmf_workspace* ws = nullptr;
   int64_t n = 0;
   DeviceBuffer<R> invdiag, x, b, q, r, d;
+  DeviceBuffer<double> gV, gZ, gy;  // GMRES smoother Krylov basis, preconditioned basis, coefficients
   double lambda_max = 0.0;
 };
 
@@ -228,6 +229,8 @@ struct MGPrecond : MGBase {
   double omega = 0.8;
   int cheb_degree = 4;
   double cheb_range = 20.0;
+  int restart = 8;
+  bool spd = true;
   // coarse solver (FP64)
   bool dense = false;
   DeviceBuffer<double> coarse_inverse;
@@ -235,6 +238,7 @@ struct MGPrecond : MGBase {
   DeviceBuffer<double> coarse_b, coarse_x;  // FP64 coarse vectors when R is float
   std::unique_ptr<JacobiPrecond> coarse_jacobi;
   std::unique_ptr<CgVectors> coarse_vectors;
+  std::unique_ptr<FgmresVectors> coarse_fgmres;
   double coarse_rtol = 1e-12, coarse_atol = 0.0;
   int64_t coarse_maxiter = 10000;
   // Top-level vectors in R (FP32 conversion, or the buffers a captured graph
@@ -299,11 +303,100 @@ struct MGPrecond : MGBase {
     from_level(z, top_x.p, n, ws->stream);
   }
 
+  // One fixed GMRES cycle (src/multigrid.jl:963-1035): count = min(steps,
+  // restart, n) Arnoldi steps on A D^-1 from the residual, Givens rotations,
+  // x += omega sum_j y_j D^-1 v_j. Host reads per column (not captured).
+  void smooth_gmres(int l, double* x, const double* b, int steps, bool x_zero) {
+    Level<R>& L = levels[size_t(l)];
+    const int64_t n = L.n;
+    gmf_workspace* lw = L.ws;
+    hipStream_t s = ws->stream;
+    const int count = int(std::min<int64_t>(std::min(steps, restart), n));
+    // x is updated in place below; on the first sweep of a level its memory
+    // holds no meaningful value yet and must be cleared.
+    if (x_zero) fill(x, 0.0, n, s);
+    if (count <= 0) return;
+    if (x_zero) {
+      copy(L.r.p, b, n, s);
+    } else {
+      level_apply(lw, x, L.q.p);
+      copy(L.r.p, b, n, s);
+      axpby(lw, L.r.p, -1.0, L.q.p, 1.0, n);
+    }
+    dot(lw, L.r.p, L.r.p, n, kTmp);
+    read_scalars(lw, kTmp + 1);
+    const double* hs = lw->host_scalars;
+    const double beta = std::sqrt(hs[kTmp]);
+    if (beta == 0.0) return;
+    double* V = L.gV.p;
+    double* Z = L.gZ.p;
+    fill(V, 0.0, n, s);
+    axpby(lw, V, 1.0 / beta, L.r.p, 0.0, n);
+    const size_t sc = size_t(count);
+    std::vector<double> H((sc + 1) * sc, 0.0), g(sc + 1, 0.0), cs(sc, 0.0), sn(sc, 0.0), y(sc, 0.0);
+    auto Hm = [&](int rr, int cc) -> double& { return H[size_t(rr * count + cc)]; };
+    g[0] = beta;
+    int inner = 0;
+    for (int col = 0; col < count; ++col) {
+      inner = col + 1;
+      double* zc = Z + int64_t(col) * n;
+      const double* vc = V + int64_t(col) * n;
+      hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, zc, reinterpret_cast<const double*>(L.invdiag.p),
+                         vc, n);
+      GMF_HIP(hipGetLastError());
+      level_apply(lw, zc, L.q.p);
+      arnoldi_mgs(lw, L.q.p, V, n, col);
+      for (int bb = 0; bb <= col; ++bb) Hm(bb, col) = hs[kHessian + bb];
+      const double next = std::sqrt(std::max(0.0, hs[kHessian + col + 1]));
+      Hm(col + 1, col) = next;
+      if (next > 0.0) {
+        double* vn = V + int64_t(col + 1) * n;
+        fill(vn, 0.0, n, s);
+        axpby(lw, vn, 1.0 / next, L.q.p, 0.0, n);
+      }
+      for (int row = 0; row < col; ++row) {
+        const double f = Hm(row, col), t = Hm(row + 1, col);
+        Hm(row, col) = cs[size_t(row)] * f + sn[size_t(row)] * t;
+        Hm(row + 1, col) = -sn[size_t(row)] * f + cs[size_t(row)] * t;
+      }
+      const double scale = std::hypot(Hm(col, col), Hm(col + 1, col));
+      const double c = scale == 0.0 ? 1.0 : Hm(col, col) / scale;
+      const double sv = scale == 0.0 ? 0.0 : Hm(col + 1, col) / scale;
+      cs[size_t(col)] = c;
+      sn[size_t(col)] = sv;
+      {
+        const double f = Hm(col, col), t = Hm(col + 1, col);
+        Hm(col, col) = c * f + sv * t;
+        Hm(col + 1, col) = -sv * f + c * t;
+        const double gf = g[size_t(col)], gt = g[size_t(col) + 1];
+        g[size_t(col)] = c * gf + sv * gt;
+        g[size_t(col) + 1] = -sv * gf + c * gt;
+      }
+      if (next == 0.0) break;
+    }
+    for (int row = inner - 1; row >= 0; --row) {
+      double v = g[size_t(row)];
+      for (int c = row + 1; c < inner; ++c) v -= Hm(row, c) * y[size_t(c)];
+      const double dg = Hm(row, row);
+      require(std::isfinite(dg), GMF_EINVAL, "GMRES smoother encountered a non-finite Krylov least-squares diagonal");
+      require(dg != 0.0, GMF_EINVAL, "GMRES smoother encountered a singular Krylov least-squares problem");
+      y[size_t(row)] = v / dg;
+    }
+    for (int c = 0; c < inner; ++c) y[size_t(c)] *= omega;
+    GMF_HIP(hipMemcpyAsync(L.gy.p, y.data(), sizeof(double) * size_t(inner), hipMemcpyHostToDevice, s));
+    multi_axpy(lw, x, Z, n, L.gy.p, inner, n, 1.0);
+    sync(lw);
+  }
+
   void smooth(int l, R* x, const R* b, int steps, bool x_zero) {
     Level<R>& L = levels[size_t(l)];
     const int64_t n = L.n;
     hipStream_t s = ws->stream;
     if (n == 0) return;
+    if (smoother == GMF_SMOOTHER_GMRES) {
+      if constexpr (kDouble) smooth_gmres(l, x, b, steps, x_zero);
+      return;
+    }
     if (smoother == GMF_SMOOTHER_JACOBI) {
       for (int k = 0; k < steps; ++k) {
         if (x_zero && k == 0) {
@@ -351,7 +444,11 @@ struct MGPrecond : MGBase {
     }
     gmf_solve_info inf{};
     coarse_jacobi->ws = L.ws;
-    cg_run(L.ws, coarse_jacobi.get(), b, x, false, coarse_rtol, coarse_atol, coarse_maxiter, inf, *coarse_vectors);
+    if (spd)
+      cg_run(L.ws, coarse_jacobi.get(), b, x, false, coarse_rtol, coarse_atol, coarse_maxiter, inf, *coarse_vectors);
+    else
+      fgmres_run(L.ws, coarse_jacobi.get(), b, x, false, coarse_fgmres->m, coarse_rtol, coarse_atol, coarse_maxiter,
+                 inf, *coarse_fgmres);
   }
 
   void coarse_solve(R* x, const R* b) {
@@ -479,8 +576,43 @@ double estimate_lambda_max(gmf_workspace* ws, int64_t n, const double* invdiag, 
   return tridiagonal_max_eigenvalue(alpha, beta);
 }
 
+// Inverse of a general dense matrix by LU with partial pivoting (the
+// reference's non-SPD coarse solve, src/multigrid.jl:382-408, 899-907).
+std::vector<double> lu_inverse(std::vector<double> A, int64_t n) {
+  std::vector<int64_t> piv(static_cast<size_t>(n));
+  for (int64_t k = 0; k < n; ++k) {
+    int64_t p = k;
+    for (int64_t i = k + 1; i < n; ++i)
+      if (std::fabs(A[size_t(i * n + k)]) > std::fabs(A[size_t(p * n + k)])) p = i;
+    require(A[size_t(p * n + k)] != 0.0 && std::isfinite(A[size_t(p * n + k)]), GMF_EINVAL,
+            "coarse multigrid matrix is singular");
+    piv[size_t(k)] = p;
+    if (p != k)
+      for (int64_t j = 0; j < n; ++j) std::swap(A[size_t(k * n + j)], A[size_t(p * n + j)]);
+    for (int64_t i = k + 1; i < n; ++i) {
+      const double f = A[size_t(i * n + k)] / A[size_t(k * n + k)];
+      A[size_t(i * n + k)] = f;
+      for (int64_t j = k + 1; j < n; ++j) A[size_t(i * n + j)] -= f * A[size_t(k * n + j)];
+    }
+  }
+  std::vector<double> inv(size_t(n * n)), col(static_cast<size_t>(n));
+  for (int64_t j = 0; j < n; ++j) {
+    std::fill(col.begin(), col.end(), 0.0);
+    col[size_t(j)] = 1.0;
+    for (int64_t k = 0; k < n; ++k) std::swap(col[size_t(k)], col[size_t(piv[size_t(k)])]);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t k = 0; k < i; ++k) col[size_t(i)] -= A[size_t(i * n + k)] * col[size_t(k)];
+    for (int64_t i = n - 1; i >= 0; --i) {
+      for (int64_t k = i + 1; k < n; ++k) col[size_t(i)] -= A[size_t(i * n + k)] * col[size_t(k)];
+      col[size_t(i)] /= A[size_t(i * n + i)];
+    }
+    for (int64_t i = 0; i < n; ++i) inv[size_t(i * n + j)] = col[size_t(i)];
+  }
+  return inv;
+}
+
 // Dense coarse inverse from the probed SPD coarse matrix (src/multigrid.jl:382-424).
-DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n) {
+DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n, bool spd) {
   hipStream_t s = ws->stream;
   DeviceBuffer<double> e(static_cast<size_t>(n)), col(static_cast<size_t>(n * n));
   for (int64_t j = 0; j < n; ++j) {
@@ -494,13 +626,14 @@ DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n) {
   // A[i][j] = At[j*n + i]
   double scale = 0.0;
   for (double v : At) {
-    require(std::isfinite(v), GMF_ENOTSPD, "declared SPD coarse multigrid matrix is not numerically symmetric");
+    require(std::isfinite(v), spd ? GMF_ENOTSPD : GMF_EINVAL, "coarse multigrid matrix is not finite");
     scale = std::max(scale, std::fabs(v));
   }
-  const double allowed = std::sqrt(std::numeric_limits<double>::epsilon()) * scale;
   std::vector<double> A(size_t(n * n));
   for (int64_t i = 0; i < n; ++i)
     for (int64_t j = 0; j < n; ++j) A[size_t(i * n + j)] = At[size_t(j * n + i)];
+  if (!spd) return upload_vec(lu_inverse(A, n));
+  const double allowed = std::sqrt(std::numeric_limits<double>::epsilon()) * scale;
   for (int64_t c = 0; c < n; ++c)
     for (int64_t r = c + 1; r < n; ++r)
       require(std::fabs(A[size_t(r * n + c)] - A[size_t(c * n + r)]) <= allowed, GMF_ENOTSPD,
@@ -560,8 +693,12 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
   mg->coarse_rtol = desc->coarse_rtol;
   mg->coarse_atol = desc->coarse_atol;
   mg->coarse_maxiter = desc->coarse_maxiter;
+  mg->restart = desc->smoother_restart;
+  mg->spd = desc->spd != 0;
   mg->levels.resize(size_t(L));
   const bool cheb = desc->smoother == GMF_SMOOTHER_CHEBYSHEV;
+  const bool gmres = desc->smoother == GMF_SMOOTHER_GMRES;
+  require(!gmres || MGPrecond<R>::kDouble, GMF_EUNSUPPORTED, "the GMRES smoother runs on FP64 levels only");
   for (int l = 0; l < L; ++l) {
     Level<R>& lv = mg->levels[size_t(l)];
     lv.ws = desc->levels[l];
@@ -588,9 +725,13 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
       else
         narrow(lv.invdiag.p, inv64.p, lv.n, lv.ws->stream);
       lv.q = DeviceBuffer<R>(n);
-      if (cheb) {
-        lv.r = DeviceBuffer<R>(n);
-        lv.d = DeviceBuffer<R>(n);
+      if (cheb || gmres) lv.r = DeviceBuffer<R>(n);
+      if (cheb) lv.d = DeviceBuffer<R>(n);
+      if (gmres) {
+        const int64_t cnt = std::min<int64_t>(std::min(std::max(desc->pre_steps, desc->post_steps), mg->restart), lv.n);
+        lv.gV = DeviceBuffer<double>(n * size_t(cnt + 1));
+        lv.gZ = DeviceBuffer<double>(n * size_t(std::max<int64_t>(cnt, 1)));
+        lv.gy = DeviceBuffer<double>(size_t(std::max<int64_t>(cnt, 1)));
       }
     } else {
       mg->coarse_invdiag = std::move(inv64);
@@ -608,14 +749,17 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
   Level<R>& C = mg->levels[0];
   if (C.n <= desc->coarse_direct_limit) {
     mg->dense = true;
-    if (C.n > 0) mg->coarse_inverse = dense_coarse_inverse(C.ws, C.n);
+    if (C.n > 0) mg->coarse_inverse = dense_coarse_inverse(C.ws, C.n, mg->spd);
   } else {
     mg->coarse_jacobi = std::make_unique<JacobiPrecond>();
     mg->coarse_jacobi->ws = C.ws;
     mg->coarse_jacobi->n = C.n;
     mg->coarse_jacobi->invdiag = mg->coarse_invdiag.p;
     mg->coarse_jacobi->owns = false;
-    mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
+    if (mg->spd)
+      mg->coarse_vectors = std::make_unique<CgVectors>(C.n);
+    else
+      mg->coarse_fgmres = std::make_unique<FgmresVectors>(C.n, int(std::max<int64_t>(1, std::min<int64_t>(30, C.n))));
   }
   if (!MGPrecond<R>::kDouble) {
     mg->coarse_b = DeviceBuffer<double>(size_t(C.n));
@@ -623,7 +767,7 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
   }
   // The V-cycle has no host synchronization when the coarse solve is dense,
   // so it can be captured once and replayed.
-  mg->use_graph = desc->graph != 0 && mg->dense;
+  mg->use_graph = desc->graph != 0 && mg->dense && !gmres;
   if (mg->use_graph || !MGPrecond<R>::kDouble) {
     mg->top_b = DeviceBuffer<R>(size_t(mg->levels.back().n));
     mg->top_x = DeviceBuffer<R>(size_t(mg->levels.back().n));
@@ -644,8 +788,14 @@ gmf_status gmf_precond_create_mg(const gmf_mg_desc* desc, gmf_preconditioner** o
     require(desc && out, GMF_EINVAL, "null argument");
     require(desc->level_count >= 2 && desc->levels && desc->transfers, GMF_EINVAL,
             "multigrid needs at least two levels and their transfers");
-    require(desc->smoother == GMF_SMOOTHER_JACOBI || desc->smoother == GMF_SMOOTHER_CHEBYSHEV, GMF_EINVAL,
-            "unknown smoother");
+    require(desc->smoother == GMF_SMOOTHER_JACOBI || desc->smoother == GMF_SMOOTHER_CHEBYSHEV ||
+                desc->smoother == GMF_SMOOTHER_GMRES,
+            GMF_EINVAL, "unknown smoother");
+    require(desc->smoother != GMF_SMOOTHER_GMRES ||
+                (desc->smoother_restart >= 1 && desc->smoother_restart >= std::max(desc->pre_steps, desc->post_steps)),
+            GMF_EINVAL,
+            "GMRES multigrid smoother requires smoother_restart to be at least max(pre_smoothing_steps, "
+            "post_smoothing_steps)");
     require(desc->pre_steps >= 0 && desc->post_steps >= 0, GMF_EINVAL, "smoothing steps must be non-negative");
     require(std::isfinite(desc->damping) && desc->damping > 0, GMF_EINVAL, "smoother_damping must be positive");
     require(desc->smoother != GMF_SMOOTHER_CHEBYSHEV ||

@@ -413,6 +413,158 @@ void gmf_preconditioner::apply_dot(const double* r, double* z, int64_t n, int sl
 
 using namespace gmf;
 
+namespace gmf {
+
+void arnoldi_mgs(gmf_workspace* ws, double* q, const double* V, int64_t n, int col) {
+  for (int bcol = 0; bcol <= col; ++bcol) {
+    const double* vb = V + int64_t(bcol) * n;
+    dot(ws, q, vb, n, kHessian + bcol);
+    hipLaunchKernelGGL(k_scale_dev, dim3(grid1(n)), dim3(256), 0, ws->stream, q, vb, ws->scalars, kHessian + bcol,
+                       -1.0, n);
+  }
+  dot(ws, q, q, n, kHessian + col + 1);
+  read_scalars(ws, kHessian + col + 2);
+}
+
+void fgmres_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, bool use_initial, int m,
+                double rtol, double atol, int64_t maxiter, gmf_solve_info& inf, FgmresVectors& v) {
+  const int64_t n = ws->plan->n_red;
+  inf = gmf_solve_info{};
+  if (n == 0) {
+    inf.converged = 1;
+    return;
+  }
+  require(m >= 1 && m <= v.m && n == v.n, GMF_EINVAL, "FGMRES vectors do not match the restart length");
+  require(m + 1 < kScalarCount - kHessian, GMF_EINVAL, "restart too large");
+  copy(v.r.p, b, n, ws->stream);
+  if (use_initial) {
+    reduced_apply(ws, x, v.q.p);
+    inf.operator_applications++;
+    axpby(ws, v.r.p, -1.0, v.q.p, 1.0, n);
+  } else {
+    fill(x, 0.0, n, ws->stream);
+  }
+  dot(ws, b, b, n, kBb);
+  dot(ws, v.r.p, v.r.p, n, kRr);
+  read_scalars(ws, kTmp);
+  const double* hs = ws->host_scalars;
+  inf.rhs_norm = std::sqrt(hs[kBb]);
+  const double tol = std::max(atol, rtol * inf.rhs_norm);
+  inf.initial_residual_norm = std::sqrt(hs[kRr]);
+  inf.final_residual_norm = inf.initial_residual_norm;
+  if (inf.initial_residual_norm <= tol) {
+    inf.converged = 1;
+    return;
+  }
+  const size_t sm = static_cast<size_t>(m);
+  std::vector<double> H((sm + 1) * sm), g(sm + 1), cs(sm), sn(sm), y(sm);
+  auto Hm = [&](int r, int c) -> double& { return H[size_t(r * m + c)]; };
+  auto residual_norm = [&]() {
+    dot(ws, v.r.p, v.r.p, n, kRr);
+    read_scalars(ws, kTmp);
+    return std::sqrt(hs[kRr]);
+  };
+  auto finish = [&](int count) {
+    for (int row = count - 1; row >= 0; --row) {
+      double val = g[size_t(row)];
+      for (int c = row + 1; c < count; ++c) val -= Hm(row, c) * y[size_t(c)];
+      double dg = Hm(row, row);
+      require(std::isfinite(dg), GMF_EINVAL, "FGMRES encountered a non-finite Krylov least-squares diagonal");
+      require(dg != 0.0, GMF_EINVAL, "FGMRES encountered a singular Krylov least-squares problem");
+      y[size_t(row)] = val / dg;
+    }
+    GMF_HIP(hipMemcpyAsync(v.ydev.p, y.data(), sizeof(double) * size_t(count), hipMemcpyHostToDevice, ws->stream));
+    multi_axpy(ws, x, v.Z.p, n, v.ydev.p, count, n, 1.0);
+    reduced_apply(ws, x, v.q.p);
+    inf.operator_applications++;
+    copy(v.r.p, b, n, ws->stream);
+    axpby(ws, v.r.p, -1.0, v.q.p, 1.0, n);
+    sync(ws);
+  };
+  int64_t total = 0;
+  while (total < maxiter) {
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    std::fill(cs.begin(), cs.end(), 0.0);
+    std::fill(sn.begin(), sn.end(), 0.0);
+    double rn = residual_norm();
+    inf.final_residual_norm = rn;
+    if (rn <= tol) {
+      inf.converged = 1;
+      return;
+    }
+    fill(v.V.p, 0.0, n, ws->stream);
+    axpby(ws, v.V.p, 1.0 / rn, v.r.p, 0.0, n);
+    g[0] = rn;
+    int inner_limit = int(std::min<int64_t>(m, maxiter - total));
+    int inner = 0;
+    bool updated = false;
+    for (int col = 0; col < inner_limit; ++col) {
+      total++;
+      inner = col + 1;
+      inf.iterations = total;
+      double* zc = v.Z.p + int64_t(col) * n;
+      double* vc = v.V.p + int64_t(col) * n;
+      precond_apply(ws, M, vc, zc, n);
+      inf.preconditioner_applications++;
+      reduced_apply(ws, zc, v.q.p);
+      inf.operator_applications++;
+      arnoldi_mgs(ws, v.q.p, v.V.p, n, col);
+      for (int bcol = 0; bcol <= col; ++bcol) Hm(bcol, col) = hs[kHessian + bcol];
+      const double next = std::sqrt(hs[kHessian + col + 1]);
+      Hm(col + 1, col) = next;
+      if (next > 0.0 && col + 1 < m + 1) {
+        double* vn = v.V.p + int64_t(col + 1) * n;
+        fill(vn, 0.0, n, ws->stream);
+        axpby(ws, vn, 1.0 / next, v.q.p, 0.0, n);
+      }
+      for (int row = 0; row < col; ++row) {
+        double f = Hm(row, col), s2 = Hm(row + 1, col);
+        Hm(row, col) = cs[size_t(row)] * f + sn[size_t(row)] * s2;
+        Hm(row + 1, col) = -sn[size_t(row)] * f + cs[size_t(row)] * s2;
+      }
+      double scale = std::hypot(Hm(col, col), Hm(col + 1, col));
+      double c = 1.0, s = 0.0;
+      if (scale != 0.0) {
+        c = Hm(col, col) / scale;
+        s = Hm(col + 1, col) / scale;
+      }
+      cs[size_t(col)] = c;
+      sn[size_t(col)] = s;
+      {
+        double f = Hm(col, col), s2 = Hm(col + 1, col);
+        Hm(col, col) = c * f + s * s2;
+        Hm(col + 1, col) = -s * f + c * s2;
+        double gf = g[size_t(col)], gs = g[size_t(col) + 1];
+        g[size_t(col)] = c * gf + s * gs;
+        g[size_t(col) + 1] = -s * gf + c * gs;
+      }
+      if (std::fabs(g[size_t(col) + 1]) <= tol || next == 0.0) {
+        finish(col + 1);
+        double fr = residual_norm();
+        inf.final_residual_norm = fr;
+        if (fr <= tol) {
+          inf.converged = 1;
+          return;
+        }
+        updated = true;
+        break;
+      }
+    }
+    require(inner > 0, GMF_EINVAL, "FGMRES failed to build a Krylov update");
+    if (!updated) finish(inner);
+    double fr = residual_norm();
+    inf.final_residual_norm = fr;
+    if (fr <= tol) {
+      inf.converged = 1;
+      return;
+    }
+  }
+  throw Error(GMF_ENOCONV, "FGMRES did not converge in " + std::to_string(maxiter) + " iterations");
+}
+
+}  // namespace gmf
+
 extern "C" {
 
 const char* gmf_last_error(void) { return g_error.c_str(); }
@@ -893,144 +1045,8 @@ gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* M, const doub
     }
     const int m = int(std::min<int64_t>(restart, n));
     require(m + 1 < kScalarCount - kHessian, GMF_EINVAL, "restart too large");
-    SolverVectors v;
-    v.r = dalloc<double>(size_t(n));
-    v.q = dalloc<double>(size_t(n));
-    v.V = dalloc<double>(size_t(n) * size_t(m + 1));
-    v.Z = dalloc<double>(size_t(n) * size_t(m));
-    v.ydev = dalloc<double>(size_t(m));
-    copy(v.r, b, n, ws->stream);
-    if (use_initial) {
-      reduced_apply(ws, x, v.q);
-      inf.operator_applications++;
-      axpby(ws, v.r, -1.0, v.q, 1.0, n);
-    } else {
-      fill(x, 0.0, n, ws->stream);
-    }
-    dot(ws, b, b, n, kBb);
-    dot(ws, v.r, v.r, n, kRr);
-    read_scalars(ws, kTmp);
-    const double* hs = ws->host_scalars;
-    inf.rhs_norm = std::sqrt(hs[kBb]);
-    const double tol = std::max(atol, rtol * inf.rhs_norm);
-    inf.initial_residual_norm = std::sqrt(hs[kRr]);
-    inf.final_residual_norm = inf.initial_residual_norm;
-    if (inf.initial_residual_norm <= tol) {
-      inf.converged = 1;
-      return;
-    }
-    const size_t sm = static_cast<size_t>(m);
-    std::vector<double> H((sm + 1) * sm), g(sm + 1), cs(sm), sn(sm), y(sm);
-    auto Hm = [&](int r, int c) -> double& { return H[size_t(r * m + c)]; };
-    auto residual_norm = [&]() {
-      dot(ws, v.r, v.r, n, kRr);
-      read_scalars(ws, kTmp);
-      return std::sqrt(hs[kRr]);
-    };
-    auto finish = [&](int count) {
-      for (int row = count - 1; row >= 0; --row) {
-        double val = g[size_t(row)];
-        for (int c = row + 1; c < count; ++c) val -= Hm(row, c) * y[size_t(c)];
-        double dg = Hm(row, row);
-        require(std::isfinite(dg), GMF_EINVAL, "FGMRES encountered a non-finite Krylov least-squares diagonal");
-        require(dg != 0.0, GMF_EINVAL, "FGMRES encountered a singular Krylov least-squares problem");
-        y[size_t(row)] = val / dg;
-      }
-      GMF_HIP(hipMemcpyAsync(v.ydev, y.data(), sizeof(double) * size_t(count), hipMemcpyHostToDevice, ws->stream));
-      multi_axpy(ws, x, v.Z, n, v.ydev, count, n, 1.0);
-      reduced_apply(ws, x, v.q);
-      inf.operator_applications++;
-      copy(v.r, b, n, ws->stream);
-      axpby(ws, v.r, -1.0, v.q, 1.0, n);
-      sync(ws);
-    };
-    int64_t total = 0;
-    while (total < maxiter) {
-      std::fill(H.begin(), H.end(), 0.0);
-      std::fill(g.begin(), g.end(), 0.0);
-      std::fill(cs.begin(), cs.end(), 0.0);
-      std::fill(sn.begin(), sn.end(), 0.0);
-      double rn = residual_norm();
-      inf.final_residual_norm = rn;
-      if (rn <= tol) {
-        inf.converged = 1;
-        return;
-      }
-      fill(v.V, 0.0, n, ws->stream);
-      axpby(ws, v.V, 1.0 / rn, v.r, 0.0, n);
-      g[0] = rn;
-      int inner_limit = int(std::min<int64_t>(m, maxiter - total));
-      int inner = 0;
-      bool updated = false;
-      for (int col = 0; col < inner_limit; ++col) {
-        total++;
-        inner = col + 1;
-        inf.iterations = total;
-        double* zc = v.Z + int64_t(col) * n;
-        double* vc = v.V + int64_t(col) * n;
-        precond_apply(ws, M, vc, zc, n);
-        inf.preconditioner_applications++;
-        reduced_apply(ws, zc, v.q);
-        inf.operator_applications++;
-        for (int bcol = 0; bcol <= col; ++bcol) {
-          const double* vb = v.V + int64_t(bcol) * n;
-          dot(ws, v.q, vb, n, kHessian + bcol);
-          hipLaunchKernelGGL(k_scale_dev, dim3(grid1(n)), dim3(256), 0, ws->stream, v.q, vb, ws->scalars,
-                             kHessian + bcol, -1.0, n);
-        }
-        dot(ws, v.q, v.q, n, kHessian + col + 1);
-        read_scalars(ws, kHessian + col + 2);
-        for (int bcol = 0; bcol <= col; ++bcol) Hm(bcol, col) = hs[kHessian + bcol];
-        const double next = std::sqrt(hs[kHessian + col + 1]);
-        Hm(col + 1, col) = next;
-        if (next > 0.0 && col + 1 < m + 1) {
-          double* vn = v.V + int64_t(col + 1) * n;
-          fill(vn, 0.0, n, ws->stream);
-          axpby(ws, vn, 1.0 / next, v.q, 0.0, n);
-        }
-        for (int row = 0; row < col; ++row) {
-          double f = Hm(row, col), s2 = Hm(row + 1, col);
-          Hm(row, col) = cs[size_t(row)] * f + sn[size_t(row)] * s2;
-          Hm(row + 1, col) = -sn[size_t(row)] * f + cs[size_t(row)] * s2;
-        }
-        double scale = std::hypot(Hm(col, col), Hm(col + 1, col));
-        double c = 1.0, s = 0.0;
-        if (scale != 0.0) {
-          c = Hm(col, col) / scale;
-          s = Hm(col + 1, col) / scale;
-        }
-        cs[size_t(col)] = c;
-        sn[size_t(col)] = s;
-        {
-          double f = Hm(col, col), s2 = Hm(col + 1, col);
-          Hm(col, col) = c * f + s * s2;
-          Hm(col + 1, col) = -s * f + c * s2;
-          double gf = g[size_t(col)], gs = g[size_t(col) + 1];
-          g[size_t(col)] = c * gf + s * gs;
-          g[size_t(col) + 1] = -s * gf + c * gs;
-        }
-        if (std::fabs(g[size_t(col) + 1]) <= tol || next == 0.0) {
-          finish(col + 1);
-          double fr = residual_norm();
-          inf.final_residual_norm = fr;
-          if (fr <= tol) {
-            inf.converged = 1;
-            return;
-          }
-          updated = true;
-          break;
-        }
-      }
-      require(inner > 0, GMF_EINVAL, "FGMRES failed to build a Krylov update");
-      if (!updated) finish(inner);
-      double fr = residual_norm();
-      inf.final_residual_norm = fr;
-      if (fr <= tol) {
-        inf.converged = 1;
-        return;
-      }
-    }
-    throw Error(GMF_ENOCONV, "FGMRES did not converge in " + std::to_string(maxiter) + " iterations");
+    FgmresVectors v(n, m);
+    fgmres_run(ws, M, b, x, use_initial != 0, m, rtol, atol, maxiter, inf, v);
   });
 }
 

@@ -61,6 +61,7 @@ class MgDesc(ctypes.Structure):
         ("chebyshev_degree", c_int32), ("chebyshev_range", c_double), ("eigen_iterations", c_int32),
         ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
         ("coarse_maxiter", c_int64), ("graph", c_int32), ("precision", c_int32),
+        ("smoother_restart", c_int32), ("spd", c_int32),
     ]
 
 

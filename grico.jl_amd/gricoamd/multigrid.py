@@ -27,7 +27,7 @@ from .space import HpSpace, SpaceOptions, StoredDegrees
 
 _DEFAULT_COARSE_DIRECT_DOF_LIMIT = 512
 
-SMOOTHERS = {"jacobi": 0, "chebyshev": 1}
+SMOOTHERS = {"jacobi": 0, "chebyshev": 1, "gmres": 2}
 
 
 def _positive_int(v, name):
@@ -309,6 +309,17 @@ class ReducedTransfer:
 
 # ---- compiled hierarchy ------------------------------------------------------------
 
+def _check_smoother(policy, smoother):
+    """src/multigrid.jl:350-357: the GMRES smoother's Krylov length covers the
+    larger smoothing count. It runs on FP64 levels only."""
+    if smoother == "gmres":
+        if policy.smoother_restart < max(policy.pre_smoothing_steps, policy.post_smoothing_steps):
+            raise ValueError("GMRES multigrid smoother requires smoother_restart to be at least "
+                             "max(pre_smoothing_steps, post_smoothing_steps)")
+        if policy.precision != "fp64":
+            raise ValueError("the GMRES multigrid smoother runs on FP64 levels only")
+
+
 class MultigridHierarchy:
     """Level plans (coarse to fine), transfers and, once requested, the device
     V-cycle (src/multigrid.jl:309-345)."""
@@ -324,9 +335,7 @@ class MultigridHierarchy:
         self.policy = policy
         self.problem = problem
         self.smoother = policy.resolved_smoother(problem.operator_class)
-        if self.smoother == "gmres":
-            raise NotImplementedError("the GMRES multigrid smoother (nonsymmetric operators) is not implemented "
-                                      "on the device; use smoother='jacobi' or 'chebyshev'")
+        _check_smoother(policy, self.smoother)
         self.spaces = multigrid_spaces(fine_space, policy)
         plans = []
         for i, sp in enumerate(self.spaces):
@@ -347,9 +356,7 @@ class MultigridHierarchy:
             if getattr(policy, name) != getattr(self.policy, name):
                 raise ValueError(f"with_policy keeps the hierarchy: {name} must agree")
         smoother = policy.resolved_smoother(self.problem.operator_class)
-        if smoother == "gmres":
-            raise NotImplementedError("the GMRES multigrid smoother (nonsymmetric operators) is not implemented "
-                                      "on the device; use smoother='jacobi' or 'chebyshev'")
+        _check_smoother(policy, smoother)
         other = object.__new__(MultigridHierarchy)
         other.__dict__.update(self.__dict__)
         other.policy = policy
@@ -405,6 +412,8 @@ class MultigridHierarchy:
             d.coarse_maxiter = pol.coarse_maxiter
             d.graph = int(pol.graph)
             d.precision = 32 if pol.precision == "fp32" else 64
+            d.smoother_restart = pol.smoother_restart
+            d.spd = int(isinstance(self.problem.operator_class, SPD))
             handle = ctypes.c_void_p()
             _devabi.check(lib.gmf_precond_create_mg(ctypes.byref(d), ctypes.byref(handle)))
             self._device = DevicePreconditioner(self.plans[-1], handle, keep=(keep, self))

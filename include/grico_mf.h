@@ -209,7 +209,10 @@ gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* precond, cons
 
 enum {
   GMF_SMOOTHER_JACOBI = 0,   /* damped Jacobi sweeps, the reference's SPD smoother (src/multigrid.jl:943-956) */
-  GMF_SMOOTHER_CHEBYSHEV = 1 /* Chebyshev polynomial in D^-1 A on [lambda_max / range, lambda_max]; no reference counterpart */
+  GMF_SMOOTHER_CHEBYSHEV = 1, /* Chebyshev polynomial in D^-1 A on [lambda_max / range, lambda_max]; no reference counterpart */
+  GMF_SMOOTHER_GMRES = 2      /* one fixed cycle of min(steps, restart) Jacobi right-preconditioned Arnoldi steps,
+                                 damped update: the reference's smoother for general operators
+                                 (src/multigrid.jl:963-1035); FP64 levels only */
 };
 
 /* Reduced-space prolongation from level i to level i+1 as COO entries sorted
@@ -251,6 +254,11 @@ typedef struct gmf_mg_desc {
    * rounds r to FP32 and widens the FP32 V-cycle result. No reference
    * counterpart (the reference V-cycle runs in the vector element type). */
   int32_t precision;
+  int32_t smoother_restart; /* GMRES smoother Krylov length (reference default 8); >= max(pre, post) */
+  /* 1: SPD operator (Cholesky dense coarse inverse, CG coarse solve); 0: general
+   * (LU dense coarse inverse, FGMRES coarse solve with restart min(30, n))
+   * (src/multigrid.jl:382-408, 891-924). */
+  int32_t spd;
 } gmf_mg_desc;
 
 /* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR

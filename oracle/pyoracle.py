@@ -376,8 +376,42 @@ def _chebyshev(A, dinv, x, b, x_zero, lam_max, degree, rng):
     return x
 
 
+def gmres_smooth(A, dinv, x, b, steps, restart, omega):
+    """One fixed GMRES smoothing cycle (src/multigrid.jl:963-1035): m =
+    min(steps, restart, n) Arnoldi steps on A D^-1 from r = b - A x (modified
+    Gram-Schmidt), then x += omega Z y with y the least-squares minimizer of
+    |beta e1 - H y| (solved here with numpy's lstsq instead of the
+    reference's Givens rotations)."""
+    n = len(b)
+    r = b - A @ x
+    beta = float(np.linalg.norm(r))
+    m = min(int(steps), int(restart), n)
+    if beta == 0.0 or m == 0:
+        return x
+    V = np.zeros((n, m + 1))
+    Z = np.zeros((n, m))
+    H = np.zeros((m + 1, m))
+    V[:, 0] = r / beta
+    k = 0
+    for j in range(m):
+        k = j + 1
+        Z[:, j] = dinv * V[:, j]
+        w = A @ Z[:, j]
+        for i in range(j + 1):
+            H[i, j] = w @ V[:, i]
+            w = w - H[i, j] * V[:, i]
+        H[j + 1, j] = np.linalg.norm(w)
+        if H[j + 1, j] == 0.0:
+            break
+        V[:, j + 1] = w / H[j + 1, j]
+    e = np.zeros(k + 1)
+    e[0] = beta
+    y = np.linalg.lstsq(H[:k + 1, :k], e, rcond=None)[0]
+    return x + omega * (Z[:, :k] @ y)
+
+
 def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, lambdas=None,
-           cheb_degree=4, cheb_range=20.0):
+           cheb_degree=4, cheb_range=20.0, restart=8):
     """z = M^-1 r for the V-cycle: damped Jacobi sweeps (src/multigrid.jl:943-956)
     or Chebyshev applications, residual restriction with P^T, recursive coarse
     correction, exact dense coarse solve (891-899), prolongation (1060-1072).
@@ -389,6 +423,8 @@ def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, 
         A, dinv = mats[l], dinvs[l]
         if A.size == 0:
             return x
+        if smoother == "gmres":
+            return gmres_smooth(A, dinv, x, b, steps, restart, omega) if steps else x
         for k in range(steps):
             first = x_zero and k == 0
             if smoother == "jacobi":

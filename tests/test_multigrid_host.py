@@ -147,3 +147,59 @@ def test_oracle_vcycle_is_a_contraction():
         rates.append(err ** (1 / 6))
     assert max(rates) < 0.7
     assert abs(rates[0] - rates[1]) < 0.05
+
+
+# ---- GMRES smoother (nonsymmetric operators) --------------------------------------
+
+def _nonsym_matrix(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n)) * 0.3 + np.diag(4.0 + rng.random(n))
+    return A
+
+
+def test_oracle_gmres_smoother_full_krylov_space_solves():
+    """With n Arnoldi steps and damping 1 the GMRES smoother minimizes the
+    residual over the whole space: x = A^-1 b."""
+    A = _nonsym_matrix(12, 1)
+    b = np.random.default_rng(2).standard_normal(12)
+    x0 = np.random.default_rng(3).standard_normal(12)
+    x = pyoracle.gmres_smooth(A, 1.0 / np.diag(A), x0, b, 12, 12, 1.0)
+    assert np.allclose(x, np.linalg.solve(A, b), atol=1e-10)
+
+
+def test_oracle_gmres_smoother_one_step_is_minimal_residual():
+    """One Arnoldi step: x += omega (r . Az) / (Az . Az) z with z = D^-1 r."""
+    A = _nonsym_matrix(9, 4)
+    b = np.random.default_rng(5).standard_normal(9)
+    x0 = np.random.default_rng(6).standard_normal(9)
+    dinv = 1.0 / np.diag(A)
+    r = b - A @ x0
+    z = dinv * r
+    w = A @ z
+    expect = x0 + 0.8 * (r @ w) / (w @ w) * z
+    assert np.allclose(pyoracle.gmres_smooth(A, dinv, x0, b, 1, 8, 0.8), expect, atol=1e-13)
+
+
+def _advection_problem(cells=4, degree=2, b=(3.0, 1.0)):
+    d = g.Domain((0.0, 0.0), (1.0, 1.0), (1, 1))
+    g.uniform_refine(d, int(round(np.log2(cells))))
+    sp = g.HpSpace(d, g.SpaceOptions(degree=g.UniformDegree(degree)))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.NonsymmetricOperator())
+    pr.add_cell_accumulator(u, u, P.MassDiffusion(1.0, 1.0, b=b))
+    pr.add_cell_accumulator(u, P.SineLoad(2))
+    for axis in range(2):
+        for side in (g.LOWER, g.UPPER):
+            pr.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return pr
+
+
+def test_gmres_smoother_policy_checks():
+    pr = _advection_problem()
+    with pytest.raises(ValueError):
+        compile_geometric_multigrid(pr, GeometricMultigridPreconditioner(pre_smoothing_steps=9,
+                                                                          post_smoothing_steps=9))
+    with pytest.raises(ValueError):
+        compile_geometric_multigrid(pr, GeometricMultigridPreconditioner(precision="fp32"))
+    h = compile_geometric_multigrid(pr, GeometricMultigridPreconditioner())
+    assert h.smoother == "gmres"
