@@ -83,6 +83,10 @@ def cpu_baseline(budget_s=12.0):
         if el >= budget_s:
             break
     per = el / reps
+    # The device apply on the same sample and input, against the oracle.
+    y_ref = pyoracle.apply_lowered(plan, x, threads=threads, matrices=mats, colors=cols)
+    y_dev = g.apply(plan, x)
+    err = float(np.abs(y_dev - y_ref).max() / np.abs(y_ref).max())
     return {
         "value": plan.full_dof_count / per / 1e9,
         "unit": "GDOF/s",
@@ -90,6 +94,7 @@ def cpu_baseline(budget_s=12.0):
         "kind": "port",
         "sample": f"oracle full-layout apply, 16^3 cells p=4 ({plan.full_dof_count} dofs), "
                   f"{reps} reps in {el:.1f}s, {threads} OpenMP threads over DOF-disjoint colors",
+        "device_max_relative_error_vs_oracle": err,
     }
 
 
@@ -351,6 +356,69 @@ def build_lognormal_problem(depth=5, degree=3, seed=2024):
     return problem
 
 
+def lognormal_leg(repeats=3):
+    """GMG-CG (FP64 and FP32 Chebyshev V-cycles) and FGMRES with the FP64
+    V-cycle alone on the log-normal p = 3 problem, from x0 = 0 to
+    ||r|| <= 1e-8 ||b||, hierarchy already built: the baseline side of the
+    learned-correction comparison."""
+    import ctypes
+
+    import torch
+
+    import gricoamd as g
+    from gricoamd import _devabi, _native, ops
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+
+    lib = _native.device()
+    t0 = time.perf_counter()
+    problem = build_lognormal_problem()
+    plan = g.compile(problem)
+    log(f"log-normal problem: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
+    h64 = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"), fine_plan=plan)
+    h32 = h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32"))
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    x = torch.zeros_like(b)
+    dev = ops._ws(plan)
+
+    def timed(fn):
+        fn()
+        ts = []
+        info = None
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return {"seconds": sorted(ts)[len(ts) // 2], "iterations": info.iterations,
+                "final_relative_residual": info.final_residual_norm / info.rhs_norm}
+
+    def cg(handle):
+        info = _devabi.SolveInfo()
+        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 5000,
+                                       ctypes.byref(info)))
+        return info
+
+    def fgmres(handle):
+        info = _devabi.SolveInfo()
+        _devabi.check(lib.gmf_fgmres_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 30, 1e-8, 0.0, 5000,
+                                           ctypes.byref(info)))
+        return info
+
+    M64 = h64.device_preconditioner()
+    out = {"workload": "3D Poisson 32^3 cells (uniform refinement of one root), p=3, log-normal kappa "
+                       "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
+           "reduced_dofs": plan.reduced_dof_count, "levels": h64.reduced_dof_counts()}
+    out["gmg_cg"] = timed(lambda: cg(M64.handle))
+    x64 = x.clone()
+    out["gmg_cg_fp32_vcycle"] = timed(lambda: cg(h32.device_preconditioner().handle))
+    x32 = x.clone()
+    out["fgmres_vcycle"] = timed(lambda: fgmres(M64.handle))
+    out["fp32_vs_fp64_solution_relative_difference"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
+    log(f"log-normal leg: {out}")
+    return out
+
+
 def learned_leg(repeats=3, pairs=2000, steps=3000):
     """Learned-correction FGMRES against GMG-CG on the log-normal p = 3
     problem: time from x0 = 0 to ||r|| <= 1e-8 ||b||, hierarchy and network
@@ -453,6 +521,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMG-CG / Jacobi-CG solve leg")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive hp leg")
+    ap.add_argument("--no-lognormal", action="store_true", help="skip the log-normal GMG-CG leg")
     ap.add_argument("--learned", action="store_true",
                     help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
                          "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
@@ -562,6 +631,12 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"adaptive leg failed: {exc!r}")
             result["adaptive"] = {"error": repr(exc)}
+    if not args.no_lognormal:
+        try:
+            result["lognormal"] = lognormal_leg()
+        except Exception as exc:  # keep the headline line
+            log(f"log-normal leg failed: {exc!r}")
+            result["lognormal"] = {"error": repr(exc)}
     if args.learned:
         result["learned"] = learned_leg()
     if not args.no_cpu_baseline:
