@@ -395,7 +395,7 @@ def lognormal_leg(repeats=3):
 
     def cg(handle):
         info = _devabi.SolveInfo()
-        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 5000,
+        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 50000,
                                        ctypes.byref(info)))
         return info
 
@@ -414,6 +414,9 @@ def lognormal_leg(repeats=3):
     out["gmg_cg_fp32_vcycle"] = timed(lambda: cg(h32.device_preconditioner().handle))
     x32 = x.clone()
     out["fgmres_vcycle"] = timed(lambda: fgmres(M64.handle))
+    from gricoamd.solve import compile_preconditioner
+    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
+    out["jacobi_cg"] = timed(lambda: cg(J.handle))
     out["fp32_vs_fp64_solution_relative_difference"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
     log(f"log-normal leg: {out}")
     return out

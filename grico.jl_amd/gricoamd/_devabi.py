@@ -200,6 +200,8 @@ class DevicePlan:
         self.reduced_dof_count = plan.reduced_dof_count
 
     def __del__(self):
+        if _native is None:  # interpreter shutdown
+            return
         lib = _native._device
         if lib is None:
             return

@@ -97,6 +97,8 @@ class CellMLP:
         self._sync_stream()
 
     def __del__(self):
+        if _native is None:  # interpreter shutdown
+            return
         lib = _native._device
         if lib is not None and getattr(self, "handle", None) is not None and self.handle.value:
             lib.gmf_mlp_destroy(self.handle)

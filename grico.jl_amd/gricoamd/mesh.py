@@ -32,6 +32,8 @@ class CartesianGrid:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
+        if _native is None:  # interpreter shutdown
+            return
         if h is not None and h.value:
             _native.host().gh_grid_destroy(h)
             self._handle = None

@@ -61,6 +61,8 @@ class DevicePreconditioner:
         self._keep = keep
 
     def __del__(self):
+        if _native is None:  # interpreter shutdown
+            return
         lib = _native._device
         if lib is not None and getattr(self, "handle", None) is not None and self.handle.value:
             lib.gmf_precond_destroy(self.handle)
