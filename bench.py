@@ -316,6 +316,68 @@ def adaptive_leg(repeats=3, steps=20):
     return out
 
 
+def rhs_l2_leg(plan, reps=5):
+    """Device right-hand side, reduced right-hand side and verification sums on
+    the apply workload with inputs resident in HBM (loads lowered on the host
+    once); roofline against the algorithmic bytes; the host sum-factorized
+    right-hand side and the oracle (dense basis, CPU) beside it."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import pyoracle
+    from gricoamd import _devabi, _native, ops
+
+    lib = _native.device()
+    dev = ops._ws(plan)
+    t0 = time.perf_counter()
+    loads, channels = plan.lowered_loads()
+    t_lower = time.perf_counter() - t0
+    lt = ops.to_device(loads)
+    y = torch.empty(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    b = torch.empty(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
+    w = ops.to_device(plan.point_weights())
+    e = ops.to_device(plan.point_values(lambda p: np.sin(np.pi * p[0]) * np.sin(np.pi * p[1]) * np.sin(np.pi * p[2])))
+    u = torch.randn(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    sums = (ctypes.c_double * 2)()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_rhs = timed(lambda: _devabi.check(lib.gmf_rhs_physical(dev.ws, lt.data_ptr(), channels, y.data_ptr())))
+    t_red = timed(lambda: _devabi.check(lib.gmf_reduced_rhs(dev.ws, y.data_ptr(), b.data_ptr())))
+    t_l2 = timed(lambda: _devabi.check(lib.gmf_l2_error(dev.ws, u.data_ptr(), w.data_ptr(), e.data_ptr(), sums)))
+    npts = int(lt.numel() // channels)
+    rhs_bytes = 8 * npts * channels + 8 * plan.full_dof_count
+    l2_bytes = 8 * plan.full_dof_count + 16 * npts
+    plan._rhs = None  # time the host path from scratch
+    t0 = time.perf_counter()
+    host_rhs = plan.rhs_physical()
+    t_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle_rhs = pyoracle.rhs_from_loads(plan, loads, channels)
+    t_oracle = time.perf_counter() - t0
+    y_np = y.cpu().numpy()
+    err = float(np.abs(y_np - oracle_rhs).max() / np.abs(oracle_rhs).max())
+    out = {
+        "rhs_seconds": t_rhs, "rhs_roofline_frac": rhs_bytes / t_rhs / (HBM_PEAK_GBS * 1e9),
+        "reduced_rhs_seconds": t_red, "l2_seconds": t_l2,
+        "l2_roofline_frac": l2_bytes / t_l2 / (HBM_PEAK_GBS * 1e9),
+        "host_lowering_seconds": t_lower, "host_sum_factorized_rhs_seconds": t_host,
+        "oracle_dense_rhs_seconds": t_oracle, "rhs_max_relative_error_vs_oracle": err,
+        "host_rhs_max_relative_difference": float(np.abs(y_np - host_rhs).max() / np.abs(host_rhs).max()),
+    }
+    log(f"rhs/l2: {out}")
+    return out
+
+
 def build_lognormal_problem(depth=5, degree=3, seed=2024):
     """32^3 cells (five uniform refinements of one root), p = 3, kappa = exp(g)
     with g a Gaussian random field (squared-exponential covariance, correlation
@@ -626,6 +688,11 @@ def main():
             _devabi.check(lib.gmf_vector_copy(dev.ws, dst.data_ptr(), src.data_ptr(), src.numel()))
         torch.cuda.synchronize()
         del src, dst
+    try:
+        result["rhs_l2"] = rhs_l2_leg(plan)
+    except Exception as exc:  # keep the headline line
+        log(f"rhs/l2 leg failed: {exc!r}")
+        result["rhs_l2"] = {"error": repr(exc)}
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
     if not args.no_adaptive:

@@ -217,6 +217,95 @@ __global__ void __launch_bounds__(256) cell_apply_generic_batched(const GenericA
   apply_cell(args[k], blk - start[k], smem, x, y);
 }
 
+// Right-hand side from lowered point loads: one cell per workgroup, loads
+// (Qtot x channels per cell: W v, then W g_d) projected with the value table
+// (channel 0) and the derivative table on axis d scaled by J^-1_d (channel
+// d + 1), scattered through the term maps (src/assembly.jl:488-507).
+__global__ void __launch_bounds__(256) cell_rhs_generic(GenericArgs a, const double* __restrict__ loads, int channels,
+                                                        double* __restrict__ y) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* bufA = smem;
+  double* bufB = bufA + a.M;
+  double* f = bufB + a.M;  // Qtot
+  double* r = f + a.Qtot;  // Ntot
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] = 0.0;
+  const double* L = loads + cell * int64_t(a.Qtot) * channels;
+  const double* jinv = a.jinv + cell * a.D;
+  for (int c = 0; c < channels; ++c) {
+    const double scale = c == 0 ? 1.0 : jinv[c - 1];
+    for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) f[q] = L[int64_t(q) * channels + c] * scale;
+    __syncthreads();
+    project_add(a, f, r, bufA, bufB, c - 1, false, -1);
+  }
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_one(a, cell, i, r[i], y, false);
+}
+
+// Verification sums of one cell: partial[2 c] = sum_q w_q (u_h(x_q) - e_q)^2,
+// partial[2 c + 1] = sum_q w_q e_q^2, u_h interpolated from the full-layout
+// coefficients (src/verification.jl:201-230). Fixed-order tree reduction.
+__global__ void __launch_bounds__(256) cell_l2_generic(GenericArgs a, const double* __restrict__ u,
+                                                       const double* __restrict__ w, const double* __restrict__ e,
+                                                       double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* uc = smem;
+  double* bufA = uc + a.M;
+  double* bufB = bufA + a.M;
+  double* f = bufB + a.M;             // Qtot
+  double* red = f + a.Qtot;           // 2 x blockDim.x
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) uc[i] = gather_one(a, cell, i, u);
+  __syncthreads();
+  interpolate(a, uc, f, bufA, bufB, -1);
+  double s0 = 0.0, s1 = 0.0;
+  const int64_t base = cell * int64_t(a.Qtot);
+  for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) {
+    const double ev = e[base + q], wq = w[base + q];
+    const double d = f[q] - ev;
+    s0 = fma(wq * d, d, s0);
+    s1 = fma(wq * ev, ev, s1);
+  }
+  red[threadIdx.x] = s0;
+  red[blockDim.x + threadIdx.x] = s1;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) {
+      red[threadIdx.x] += red[threadIdx.x + off];
+      red[blockDim.x + threadIdx.x] += red[blockDim.x + threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * cell] = red[0];
+    partial[2 * cell + 1] = red[blockDim.x];
+  }
+}
+
+// out[k] = sum_c partial[2 c + k] in a fixed order (one workgroup).
+__global__ void __launch_bounds__(256) k_sum_pairs(const double* __restrict__ partial, int64_t cells,
+                                                   double* __restrict__ out) {
+  __shared__ double red[2 * 256];
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t c = threadIdx.x; c < cells; c += blockDim.x) {
+    s0 += partial[2 * c];
+    s1 += partial[2 * c + 1];
+  }
+  red[threadIdx.x] = s0;
+  red[256 + threadIdx.x] = s1;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) {
+      red[threadIdx.x] += red[threadIdx.x + off];
+      red[256 + threadIdx.x] += red[256 + threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = red[0];
+    out[1] = red[256];
+  }
+}
+
 // One (cell, reduced dof) pair per workgroup: diag[dof] += v^T A_c v with v the
 // dof's local coefficient vector.
 __global__ void __launch_bounds__(64) cell_diagonal_pairs(GenericArgs a, const int64_t* __restrict__ pcell,
@@ -426,6 +515,32 @@ void launch_generic_batches(const gmf_plan& p, const double* x, double* y, bool 
     GMF_HIP(hipGetLastError());
     for (int i : g.buckets) done[size_t(i)] = 1;
   }
+}
+
+void launch_cell_rhs(const Bucket& b, const double* loads, int channels, double* y, hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, false);
+  require(channels == 1 || channels == a.D + 1, GMF_EINVAL, "load channels must be 1 or 1 + dimension");
+  const size_t lds = sizeof(double) * size_t(2 * a.M + a.Qtot + a.Ntot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_rhs_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, loads, channels, y);
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cell_l2(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
+                    hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, false);
+  const int threads = 64;
+  const size_t lds = sizeof(double) * size_t(3 * a.M + a.Qtot + 2 * threads);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_l2_generic, dim3(unsigned(b.cells)), dim3(threads), lds, stream, a, u, w, e, partial);
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, stream, partial, cells, out);
+  GMF_HIP(hipGetLastError());
 }
 
 void launch_cell_diagonal_generic(const gmf_plan&, const Bucket& b, double* diag, bool reduced,

@@ -175,6 +175,14 @@ namespace gmf {
 // Kernel launchers implemented in the .hip files.
 void launch_cell_generic(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                          hipStream_t stream);
+// Right-hand side of one tensor bucket from lowered point loads (cells x Q x
+// channels, channels 1 or 1 + D, weights included), added into y (full layout).
+void launch_cell_rhs(const Bucket& b, const double* loads, int channels, double* y, hipStream_t stream);
+// Per-cell verification sums (w (u_h - e)^2, w e^2) into partial (2 per cell).
+void launch_cell_l2(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
+                    hipStream_t stream);
+// out[0..1] = column sums of partial (cells x 2) in a fixed order.
+void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream);
 // Autotune the generic kernel: for every generic-kernel bucket time 64, 128
 // and 256 lanes per cell on scratch vectors and keep the fastest
 // (GMF_GENERIC_THREADS=64|128|256 forces one width).

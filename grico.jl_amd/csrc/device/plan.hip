@@ -862,6 +862,70 @@ gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y) {
   });
 }
 
+gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t channels, double* y) {
+  return guarded([&] {
+    require(ws && loads && y, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    for (const Bucket& b : p.buckets)
+      require(!b.cloud, GMF_EUNSUPPORTED, "the device right-hand side covers tensor-product cells only");
+    require_device_pointer(loads, "loads");
+    require_device_pointer(y, "y");
+    fill(y, 0.0, p.n_full, ws->stream);
+    int64_t off = 0;
+    for (const Bucket& b : p.buckets) {
+      launch_cell_rhs(b, loads + off, channels, y, ws->stream);
+      off += b.cells * int64_t(b.Qtot) * channels;
+    }
+    sync(ws);
+  });
+}
+
+gmf_status gmf_reduced_rhs(gmf_workspace* ws, const double* rhs_full, double* b_red) {
+  return guarded([&] {
+    require(ws && rhs_full && b_red, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    require(rhs_full != ws->full_in && rhs_full != ws->full_out, GMF_EALIAS, "right-hand side aliases workspace");
+    if (p.shift) {
+      // b - A_phys shift on the full layout, then P^T.
+      if (!plan_writes_every_dof(p)) fill(ws->full_out, 0.0, p.n_full, ws->stream);
+      cell_pass(ws, p.shift, ws->full_out, false);
+      copy(ws->full_in, rhs_full, p.n_full, ws->stream);
+      axpby(ws, ws->full_in, -1.0, ws->full_out, 1.0, p.n_full);
+      project(ws, ws->full_in, b_red);
+    } else {
+      project(ws, rhs_full, b_red);
+    }
+    sync(ws);
+  });
+}
+
+gmf_status gmf_l2_error(gmf_workspace* ws, const double* u_full, const double* weights, const double* exact,
+                        double* sums) {
+  return guarded([&] {
+    require(ws && u_full && weights && exact && sums, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    int64_t cells = 0;
+    for (const Bucket& b : p.buckets) {
+      require(!b.cloud, GMF_EUNSUPPORTED, "the device L2 error covers tensor-product cells only");
+      cells += b.cells;
+    }
+    require_device_pointer(u_full, "u_full");
+    require_device_pointer(weights, "weights");
+    require_device_pointer(exact, "exact");
+    DeviceBuffer<double> partial(size_t(std::max<int64_t>(2 * cells, 2))), out(2);
+    fill(partial.p, 0.0, 2 * std::max<int64_t>(cells, 1), ws->stream);
+    int64_t off = 0, coff = 0;
+    for (const Bucket& b : p.buckets) {
+      launch_cell_l2(b, u_full, weights + off, exact + off, partial.p + 2 * coff, ws->stream);
+      off += b.cells * int64_t(b.Qtot);
+      coff += b.cells;
+    }
+    launch_sum_pairs(partial.p, cells, out.p, ws->stream);
+    GMF_HIP(hipMemcpyAsync(sums, out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    sync(ws);
+  });
+}
+
 gmf_status gmf_apply_timed(gmf_workspace* ws, const double* x, double* y, int32_t reps, double* cell_ms,
                            double* total_ms) {
   return guarded([&] {

@@ -19,7 +19,7 @@ from .ops import (apply, apply_into, apply_physical, diagonal_physical, expand_r
                   reduced_apply, reduced_diagonal)
 from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, JacobiPreconditioner, State, solve,
                     solve_reduced)
-from .verification import l2_error, relative_l2_error
+from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

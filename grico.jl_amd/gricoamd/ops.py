@@ -10,6 +10,8 @@ _project_reduced! (851-891), _diagonal_physical! / _reduced_diagonal!
 (624-675).
 """
 
+import ctypes
+
 import numpy as np
 
 from . import _devabi, _native
@@ -100,6 +102,40 @@ def reduced_apply_f32(plan, x):
     y = torch.empty_like(xt)
     _call("gmf_reduced_apply_f32", dev.ws, xt.data_ptr(), y.data_ptr())
     return _out_like(x, y.to(torch.float64))
+
+
+def rhs_physical(plan, as_numpy=True):
+    """Physical right-hand side on the device from the plan's lowered point
+    loads (tensor-product cells; src/assembly.jl:488-507)."""
+    torch = _torch()
+    dev = _ws(plan)
+    loads, channels = plan.lowered_loads()
+    lt = to_device(loads)
+    y = torch.empty(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    _call("gmf_rhs_physical", dev.ws, lt.data_ptr(), int(channels), y.data_ptr())
+    return y.cpu().numpy() if as_numpy else y
+
+
+def reduced_rhs(plan, as_numpy=True):
+    """b_red = P^T (rhs - A_phys shift) on the device (src/assembly.jl:917-933)."""
+    torch = _torch()
+    dev = _ws(plan)
+    rhs = rhs_physical(plan, as_numpy=False)
+    b = torch.empty(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
+    _call("gmf_reduced_rhs", dev.ws, rhs.data_ptr(), b.data_ptr())
+    return b.cpu().numpy() if as_numpy else b
+
+
+def l2_sums(plan, u_full, exact):
+    """(sum w (u_h - u)^2, sum w u^2) over the tensor-product cells' points on
+    the device (src/verification.jl:201-230)."""
+    dev = _ws(plan)
+    ut = to_device(u_full, plan.full_dof_count)
+    w = to_device(plan.point_weights())
+    e = to_device(plan.point_values(exact))
+    out = (ctypes.c_double * 2)()
+    _call("gmf_l2_error", dev.ws, ut.data_ptr(), w.data_ptr(), e.data_ptr(), out)
+    return float(out[0]), float(out[1])
 
 
 def reduced_apply_into(plan, y, x):

@@ -472,6 +472,47 @@ class AssemblyPlan:
             self._rhs = b
         return self._rhs
 
+    def lowered_loads(self):
+        """Point loads of the tensor-product buckets for the device right-hand
+        side: (flat array, channels), per cell and point (W v) or
+        (W v, W g_1, .., W g_D), cells in bucket order; the load callbacks
+        evaluated as rhs_physical does."""
+        if getattr(self, "_loads", None) is None:
+            D = self.dim
+            per = []
+            grad = False
+            for bk in self.buckets:
+                shape = (bk.cell_count, bk.n_points)
+                v = np.zeros(shape)
+                g = [np.zeros(shape) for _ in range(D)]
+                for (_, op) in self.problem.cell_rhs:
+                    vv, gg = _channels(op.cell_rhs_accumulate(bk.quadrature(), 0), shape, D)
+                    v = v + bk.weights * vv
+                    for a in range(D):
+                        g[a] = g[a] + bk.weights * gg[a]
+                grad = grad or any(np.any(ga != 0) for ga in g)
+                per.append((v, g))
+            channels = 1 + D if grad else 1
+            flat = []
+            for v, g in per:
+                arr = np.stack([v] + (g if grad else []), axis=-1)
+                flat.append(arr.reshape(-1))
+            self._loads = (np.concatenate(flat) if flat else np.zeros(0), channels)
+        return self._loads
+
+    def point_weights(self):
+        """Quadrature weights of the tensor-product cells' points (bucket order)."""
+        return np.concatenate([bk.weights.reshape(-1) for bk in self.buckets])
+
+    def point_values(self, fn):
+        """fn(point tuple) at the tensor-product cells' points (bucket order)."""
+        return np.concatenate([np.broadcast_to(np.asarray(fn(tuple(bk.phys_points)), dtype=np.float64),
+                                               (bk.cell_count, bk.n_points)).reshape(-1) for bk in self.buckets])
+
+    @property
+    def tensor_only(self):
+        return not self.cloud_buckets and not self.surface_buckets
+
     def rhs(self):
         b = self.rhs_physical().copy()
         b[self.fixed_dofs] = self.fixed_values

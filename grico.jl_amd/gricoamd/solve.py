@@ -181,7 +181,9 @@ def solve(problem_or_plan, solver=None, relative_tolerance=None, absolute_tolera
     else:
         plan = problem_or_plan if is_plan else compile_plan(problem_or_plan)
         solver = _default_solver(plan, solver)
-    b = plan.reduced_rhs()
+    # Device-resident right-hand side for tensor-product plans (point loads
+    # lowered on the host); the host path covers point-cloud items.
+    b = ops.reduced_rhs(plan, as_numpy=False) if plan.tensor_only else plan.reduced_rhs()
     x, info = solve_reduced(plan, solver, b, relative_tolerance, absolute_tolerance, maxiter, initial_solution)
     full = ops.expand_reduced(plan, x, include_shift=True).cpu().numpy()
     return State(plan, full, info)

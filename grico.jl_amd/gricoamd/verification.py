@@ -37,3 +37,17 @@ def relative_l2_error(state, exact):
         num += float(np.sum(b.weights * (uh - ue) ** 2))
         den += float(np.sum(b.weights * ue ** 2))
     return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))
+
+
+def device_l2_error(state, exact):
+    """l2_error with the interpolation and the weighted sum on the device
+    (tensor-product plans)."""
+    from . import ops
+    err, _ = ops.l2_sums(state.plan, state.coefficients, exact)
+    return float(np.sqrt(err))
+
+
+def device_relative_l2_error(state, exact):
+    from . import ops
+    err, ref = ops.l2_sums(state.plan, state.coefficients, exact)
+    return float(np.sqrt(err / ref)) if ref > 0 else float(np.sqrt(err))

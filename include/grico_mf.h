@@ -174,6 +174,23 @@ gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* x_red, double* y_r
  * src/assembly.jl:905). The first call on a workspace allocates its FP32
  * copies of the per-cell arrays. */
 gmf_status gmf_reduced_apply_f32(gmf_workspace* ws, const float* x_red, float* y_red);
+/* Physical right-hand side from lowered point loads (src/assembly.jl:488-507
+ * rhs!, 586-601 _rhs_physical!): y = sum over cells of the projections
+ * sum_q [phi_i(x_q) v_q + grad phi_i(x_q) . g_q]. `loads` holds, for the plan's
+ * tensor-product cells in bucket order and each cell's points, the channels
+ * (W v) or (W v, W g_1, .., W g_D) (channels = 1 or 1 + D), W the quadrature
+ * weight: the load callbacks evaluated on the host, as the operator action is
+ * lowered to point tables. Point-cloud items are not covered (GMF_EUNSUPPORTED).
+ * Device pointers; synchronous. */
+gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t channels, double* y);
+/* b_red = P^T (rhs_full - A_phys shift) (src/assembly.jl:917-933 _reduced_rhs!). */
+gmf_status gmf_reduced_rhs(gmf_workspace* ws, const double* rhs_full, double* b_red);
+/* Verification sums over the tensor-product cells' points (src/verification.jl:201-230
+ * _l2_error_components): sums[0] = sum_q w_q (u_h(x_q) - e_q)^2, sums[1] = sum_q w_q e_q^2,
+ * u_h from the full-layout coefficients u_full; weights and exact values per point in the
+ * `loads` order. sums is a host array of two doubles. */
+gmf_status gmf_l2_error(gmf_workspace* ws, const double* u_full, const double* weights, const double* exact,
+                        double* sums);
 /* x_full = [shift] + P x_red (src/assembly.jl:851-874 _expand_reduced!). */
 gmf_status gmf_expand_reduced(gmf_workspace* ws, const double* x_red, double* x_full, int32_t include_shift);
 /* y_red = P^T y_full (src/assembly.jl:876-891 _project_reduced!). */

@@ -445,3 +445,62 @@ def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, 
         return smooth(l, x, b, post, False)
 
     return cycle(len(mats) - 1, np.asarray(r, dtype=np.float64))
+
+
+# ---- right-hand side and verification sums (dense per-cell basis matrices) ----
+
+def _dense_tables(bk):
+    """Phi (Q x N) and reference-derivative tables dPhi[d] of one tensor bucket:
+    Kronecker products of the 1D tables B[a] / G[a] (modes x points), axis 0
+    fastest for both modes and points (src/integration.jl:2677-2742 full tables)."""
+    D = bk.dim
+    Phi = np.ones((1, 1))
+    dPhi = [np.ones((1, 1)) for _ in range(D)]
+    for a in range(D):
+        Ba, Ga = bk.B[a].T, bk.G[a].T  # (Q_a x N_a)
+        Phi = np.kron(Ba, Phi)
+        for d in range(D):
+            dPhi[d] = np.kron(Ga if d == a else Ba, dPhi[d])
+    return Phi, dPhi
+
+
+def rhs_from_loads(plan, loads, channels):
+    """Physical right-hand side from lowered point loads (per tensor cell and
+    point: W v, or W v and W g_d): r_i = sum_q phi_i(x_q) (W v)_q + sum_d
+    J^-1_d dphi_i/dxi_d (W g_d)_q, scattered through the term maps
+    (src/assembly.jl:488-507; src/accumulators.jl:1230-1274 for the one-sided
+    form), with dense basis matrices instead of sum factorization."""
+    y = np.zeros(plan.full_dof_count)
+    off = 0
+    for bk in plan.buckets:
+        nc, Q = bk.cell_count, bk.n_points
+        L = np.asarray(loads[off:off + nc * Q * channels]).reshape(nc, Q, channels)
+        off += nc * Q * channels
+        Phi, dPhi = _dense_tables(bk)
+        local = L[:, :, 0] @ Phi
+        for d in range(bk.dim if channels > 1 else 0):
+            local = local + (L[:, :, 1 + d] * bk.inverse_jacobian[:, d:d + 1]) @ dPhi[d]
+        contrib = bk.term_coef * local[bk.term_cell, bk.term_box]
+        y += np.bincount(bk.term_dof, weights=contrib, minlength=plan.full_dof_count)
+    return y
+
+
+def l2_sums(plan, u_full, weights, exact):
+    """(sum_q w_q (u_h(x_q) - e_q)^2, sum_q w_q e_q^2) over the tensor cells
+    (src/verification.jl:201-230), u_h from the full-layout coefficients
+    through the term maps and the dense value table."""
+    s0 = s1 = 0.0
+    off = 0
+    u = np.asarray(u_full, dtype=np.float64)
+    for bk in plan.buckets:
+        nc, Q = bk.cell_count, bk.n_points
+        Phi, _ = _dense_tables(bk)
+        box = np.zeros((nc, bk.n_modes))
+        np.add.at(box, (bk.term_cell, bk.term_box), bk.term_coef * u[bk.term_dof])
+        uh = box @ Phi.T
+        w = np.asarray(weights[off:off + nc * Q]).reshape(nc, Q)
+        e = np.asarray(exact[off:off + nc * Q]).reshape(nc, Q)
+        off += nc * Q
+        s0 += float(np.sum(w * (uh - e) ** 2))
+        s1 += float(np.sum(w * e ** 2))
+    return s0, s1

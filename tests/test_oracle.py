@@ -296,3 +296,65 @@ def test_l2_error_constant_goldens():
     state = g.State(plan, np.zeros(plan.full_dof_count))
     assert g.l2_error(state, lambda pt: 1.0 + 0.0 * pt[0]) == pytest.approx(1.0, abs=1e-14)
     assert g.l2_error(state, lambda pt: pt[0]) == pytest.approx(np.sqrt(1.0 / 3.0), abs=1e-14)
+
+
+# ---- oracle right-hand side and verification sums, pinned by closed forms ----
+
+class _Load:
+    def __init__(self, fn):
+        self.fn = fn
+
+    def cell_rhs_accumulate(self, q, test_component):
+        return self.fn(q.point)
+
+
+def _mass_problem(load, degree=2):
+    d = g.Domain((0.0,) * 3, (1.0,) * 3, (2, 1, 2))
+    sp = g.HpSpace(d, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(sp, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.Mass())
+    pr.add_cell_accumulator(u, _Load(load))
+    return pr
+
+
+def _mass_projection(load):
+    plan = g.compile(_mass_problem(load))
+    r = pyoracle.rhs_from_loads(plan, *plan.lowered_loads())
+    M = pyoracle.reduced_matrix(plan)  # no constraints: reduced = full
+    return plan, r, np.linalg.solve(M, r)
+
+
+def test_oracle_rhs_and_l2_constant():
+    """Load 1: the mass projection c of 1 is exact, c.r = volume, u_h - 1 = 0."""
+    plan, r, c = _mass_projection(lambda p: 1.0 + 0.0 * p[0])
+    assert float(c @ r) == pytest.approx(1.0, rel=1e-12)
+    w = plan.point_weights()
+    s0, s1 = pyoracle.l2_sums(plan, c, w, plan.point_values(lambda p: 1.0 + 0.0 * p[0]))
+    assert s0 < 1e-24 and s1 == pytest.approx(1.0, rel=1e-12)
+
+
+def test_oracle_rhs_gradient_channel_and_l2_moment():
+    """Gradient load (1, 0, 0) against the exact interpolant of x:
+    integral of d x / d x = 1; integral of x^2 = 1/3."""
+    plan_x, _, cx = _mass_projection(lambda p: p[0])
+
+    class GradLoad:
+        def cell_rhs_accumulate(self, q, test_component):
+            z = 0.0 * q.point[0]
+            return g.TestChannels(z, (1.0 + z, z, z))
+
+    d = plan_x.problem
+    pr = g.AffineProblem(d.field, g.SPD())
+    pr.add_cell_accumulator(d.field, d.field, P.Mass())
+    pr.add_cell_accumulator(d.field, GradLoad())
+    plan_g = g.compile(pr)
+    loads, channels = plan_g.lowered_loads()
+    assert channels == 4
+    rg = pyoracle.rhs_from_loads(plan_g, loads, channels)
+    assert float(cx @ rg) == pytest.approx(1.0, rel=1e-12)
+    w = plan_x.point_weights()
+    s0, _ = pyoracle.l2_sums(plan_x, cx, w, plan_x.point_values(lambda p: p[0]))
+    assert s0 < 1e-24
+    s0, _ = pyoracle.l2_sums(plan_x, cx, w, plan_x.point_values(lambda p: 0.0 * p[0]))
+    assert s0 == pytest.approx(1.0 / 3.0, rel=1e-12)
