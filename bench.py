@@ -378,6 +378,149 @@ def rhs_l2_leg(plan, reps=5):
     return out
 
 
+class _SIPGCells:
+    def cell_accumulate(self, q, trial, test_component):
+        import gricoamd as g
+        return g.TestChannels(0.0 * trial.value, tuple(trial.gradient))
+
+    def cell_rhs_accumulate(self, q, test_component):
+        return -8.0 + 0.0 * q.point[0]
+
+
+class _SIPGFaces:
+    """Symmetric interior penalty (benchmark/sine_poisson_sipg.jl:54-150):
+    interfaces -{d_n u}[v] - {d_n v}[u] + sigma [u][v], boundary faces the
+    one-sided analogue with Dirichlet data g = x^2 + 2y^2 - xy + z^2 - yz,
+    sigma = 10 (p+1)^2 / h."""
+
+    def __init__(self, degree):
+        self.degree = degree
+
+    def _sigma(self, q):
+        return 10.0 * (self.degree + 1) ** 2 / q.cell_size[q.face_axis]
+
+    @staticmethod
+    def exact(p):
+        return p[0] ** 2 + 2.0 * p[1] ** 2 - p[0] * p[1] + p[2] ** 2 - p[1] * p[2]
+
+    def interface_accumulate(self, q, trial, test_component):
+        import gricoamd as g
+        jmp = g.jump(trial.value)
+        avg = g.average(g.normal_component(trial.gradient, q.normal))
+        sig = self._sigma(q)
+        shared = tuple(0.5 * jmp * c for c in q.normal)
+        return g.TraceTestChannels(g.TestChannels(-avg - sig * jmp, shared), g.TestChannels(avg + sig * jmp, shared))
+
+    def boundary_accumulate(self, q, trial, test_component):
+        import gricoamd as g
+        dn = g.normal_component(trial.gradient, q.normal)
+        return g.TestChannels(-dn + self._sigma(q) * trial.value, tuple(-trial.value * c for c in q.normal))
+
+    def boundary_rhs_accumulate(self, q, test_component):
+        import gricoamd as g
+        gv = self.exact(q.point)
+        return g.TestChannels(self._sigma(q) * gv, tuple(-gv * c for c in q.normal))
+
+
+def build_sipg_problem(cells, degree, interfaces_only=False):
+    import gricoamd as g
+    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (cells,) * 3)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree),
+                                             continuity="dg"))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    faces = _SIPGFaces(degree)
+    problem.add_interface_accumulator(u, u, faces)
+    if interfaces_only:
+        return problem
+    problem.add_cell_accumulator(u, u, _SIPGCells())
+    problem.add_cell_accumulator(u, _SIPGCells())
+    for axis in range(3):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_boundary_accumulator(g.BoundaryFace(axis, side), u, u, faces)
+            problem.add_boundary_accumulator(g.BoundaryFace(axis, side), u, faces)
+    return problem
+
+
+def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
+    """Symmetric interior penalty DG Poisson (3D, DG p=3): device apply of the
+    whole operator and of the interface items alone (two-sided point-cloud
+    kernel), roofline of the interface kernel against its algorithmic bytes,
+    and a device CG + Jacobi solve to 1e-8 that must reproduce the quadratic
+    exact solution; the oracle apply on a small mesh timed beside it."""
+    import numpy as np
+    import torch
+
+    import gricoamd as g
+    import pyoracle
+    from gricoamd import _devabi, _native, ops
+
+    lib = _native.device()
+    out = {"workload": f"3D SIPG DG Poisson, {cells}^3 cells, p={degree}, FullTensorBasis"}
+    t0 = time.perf_counter()
+    plan = g.compile(build_sipg_problem(cells, degree))
+    iplan = g.compile(build_sipg_problem(cells, degree, interfaces_only=True))
+    out["host_compile_seconds"] = time.perf_counter() - t0
+    n = plan.full_dof_count
+    x = torch.randn(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+
+    def timed(pl):
+        dev = ops._ws(pl)
+        _devabi.check(lib.gmf_apply_physical(dev.ws, x.data_ptr(), y.data_ptr()))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            _devabi.check(lib.gmf_apply_physical(dev.ws, x.data_ptr(), y.data_ptr()))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_all = timed(plan)
+    t_if = timed(iplan)
+    # The same interface items on the point-cloud kernel (no tensor detection).
+    os.environ["GMF_FACE_TENSOR"] = "0"
+    try:
+        cplan = g.compile(build_sipg_problem(cells, degree, interfaces_only=True))
+        ops._ws(cplan)
+    finally:
+        del os.environ["GMF_FACE_TENSOR"]
+    reps_saved = reps
+    reps = 3
+    t_cloud = timed(cplan)
+    reps = reps_saved
+    del cplan
+    items = sum(b.cell_count for b in iplan.interface_buckets)
+    pts = sum(b.n_points_total for b in iplan.interface_buckets)
+    ntot = iplan.interface_buckets[0].n_modes
+    # Algorithmic bytes of the interface items: gather x and read-modify-write y
+    # on both sides' modes, both sides' J^-1, and the lowered point action
+    # restricted to its nonzero channel pairs (value and normal derivative of
+    # both sides for this form: 16 of 64).
+    nnz = max(int(np.count_nonzero(np.any(b.point_data != 0, axis=0))) for b in iplan.interface_buckets)
+    if_bytes = items * (24 * ntot + 8 * 6) + pts * 8 * nnz
+    out.update({"full_dofs": n, "interface_items": items, "interface_points": pts, "action_nonzeros": nnz,
+                "apply_seconds": t_all, "apply_gdofs": n / t_all / 1e9,
+                "interface_apply_seconds": t_if, "interface_algorithmic_bytes": if_bytes,
+                "interface_apply_seconds_point_cloud_kernel": t_cloud,
+                "interface_path": "face-tensor" if "face-tensor" in ops._ws(iplan).describe() else "point-cloud",
+                "interface_roofline_frac": if_bytes / t_if / (HBM_PEAK_GBS * 1e9)})
+    t0 = time.perf_counter()
+    state = g.solve(plan, solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()), relative_tolerance=1e-8,
+                    maxiter=50000)
+    out.update({"solve_seconds": time.perf_counter() - t0, "solve_iterations": int(state.info.iterations),
+                "solve_relative_l2_error": g.relative_l2_error(state, _SIPGFaces.exact)})
+    small = g.compile(build_sipg_problem(cpu_cells, degree))
+    xv = np.random.default_rng(5).standard_normal(small.full_dof_count)
+    t0 = time.perf_counter()
+    yo = pyoracle.apply(small, xv, physical=True)
+    t_cpu = time.perf_counter() - t0
+    yd = g.apply_physical(small, xv)
+    out.update({"oracle_apply_seconds_small": t_cpu, "oracle_small_cells": cpu_cells ** 3,
+                "device_vs_oracle_max_relative_error_small": float(np.abs(yd - yo).max() / np.abs(yo).max())})
+    log(f"sipg: {out}")
+    return out
+
+
 def build_lognormal_problem(depth=5, degree=3, seed=2024):
     """32^3 cells (five uniform refinements of one root), p = 3, kappa = exp(g)
     with g a Gaussian random field (squared-exponential covariance, correlation
@@ -587,6 +730,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the GMG-CG / Jacobi-CG solve leg")
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive hp leg")
     ap.add_argument("--no-lognormal", action="store_true", help="skip the log-normal GMG-CG leg")
+    ap.add_argument("--no-sipg", action="store_true", help="skip the interior penalty DG (face kernels) leg")
     ap.add_argument("--learned", action="store_true",
                     help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
                          "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
@@ -693,6 +837,12 @@ def main():
     except Exception as exc:  # keep the headline line
         log(f"rhs/l2 leg failed: {exc!r}")
         result["rhs_l2"] = {"error": repr(exc)}
+    if not args.no_sipg:
+        try:
+            result["sipg"] = sipg_leg()
+        except Exception as exc:  # keep the headline line
+            log(f"sipg leg failed: {exc!r}")
+            result["sipg"] = {"error": repr(exc)}
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
     if not args.no_adaptive:

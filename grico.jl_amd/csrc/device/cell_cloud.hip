@@ -1,16 +1,25 @@
 // Point-cloud cell kernels: cells or embedded-surface pieces that carry their
-// own quadrature points (cut-cell rules, surface rules) instead of a tensor
-// rule. The reference evaluates these through full basis tables
-// (src/accumulators.jl:1022-1054 _accumulate_cell_accumulator_basis!,
-// 1133-1161 _accumulate_one_sided_apply_basis!); here the integrated Legendre
-// factors psi_k(xi_a), psi_k'(xi_a) are recomputed at each point on the device
-// (src/polynomials.jl:403-434), so only the reference coordinates and the
-// lowered per-point action travel from HBM.
+// own quadrature points (cut-cell rules, surface rules, boundary faces)
+// instead of a tensor rule, and two-sided interface items. The reference
+// evaluates these through full basis tables (src/accumulators.jl:1022-1054
+// _accumulate_cell_accumulator_basis!, 1133-1161
+// _accumulate_one_sided_apply_basis!, 1576-1616 interface_apply!); here the
+// integrated Legendre factors psi_k(xi_a), psi_k'(xi_a) are recomputed at each
+// point on the device (src/polynomials.jl:403-434), so only the reference
+// coordinates and the lowered per-point action travel from HBM.
 //
-// One 64-lane workgroup per item. Points are processed in chunks of 64: lanes
-// over points evaluate the 1D factors into LDS, interpolate value and
-// gradient, apply the (D+1) x (D+1) point action; then lanes over modes project
-// the chunk's channels back.
+// An item has S sides (S = 1: one leaf; S = 2: the minus and plus leaves of
+// an interface, src/integration.jl:187-207 InterfaceValues). Its local box is
+// the concatenation of the sides' mode boxes, every point carries reference
+// coordinates in each side's leaf, and the point action is the
+// S(D+1) x S(D+1) matrix acting on the channels (value, physical gradient) of
+// all sides.
+//
+// One 64-lane workgroup per item. Points are processed in chunks of at most 64
+// (fewer when the LDS tables of large boxes would exceed 64 KB): lanes over
+// points evaluate the 1D factors into LDS, interpolate value and gradient on
+// each side, apply the point action; then lanes over the local modes project
+// the chunk's channels back onto their side.
 #include <cmath>
 
 #include "common.hpp"
@@ -20,20 +29,23 @@ namespace gmf {
 namespace {
 
 constexpr int kCloudLanes = 64;
+constexpr int kMaxSides = 2;
 
 struct CloudArgs {
-  int D;
-  int N[kMaxDim];
-  int Ntot, Nmax;
-  const double* jinv;   // items x D
-  const double* pdata;  // points x (D+1)^2
-  const int64_t* poff;  // items + 1
-  const double* rpts;   // points x D
-  TermMap map;
+  int D, S;                    // dimension; sides per item
+  int N[kMaxSides][kMaxDim];   // modes per axis of each side
+  int Ns[kMaxSides];           // box size of each side
+  int Ntot, Nmax;              // sum of the box sizes; largest N over sides and axes
+  int chunk;                   // points per chunk = LDS stride of the per-point arrays
+  const double* jinv;          // items x S*D
+  const double* pdata;         // points x (S(D+1))^2
+  const int64_t* poff;         // items + 1
+  const double* rpts;          // points x S*D
+  TermMap map;                 // items x Ntot
 };
 
-// psi_0..psi_{n-1} and derivatives at x into v[k * 64], d[k * 64] (stride 64).
-__device__ inline void integrated_legendre(double x, int n, double* v, double* d) {
+// psi_0..psi_{n-1} and derivatives at x into v[k * stride], d[k * stride].
+__device__ inline void integrated_legendre(double x, int n, int stride, double* v, double* d) {
   if (n == 1) {
     v[0] = 1.0;
     d[0] = 0.0;
@@ -41,70 +53,94 @@ __device__ inline void integrated_legendre(double x, int n, double* v, double* d
   }
   v[0] = 0.5 * (1.0 - x);
   d[0] = -0.5;
-  v[kCloudLanes] = 0.5 * (1.0 + x);
-  d[kCloudLanes] = 0.5;
+  v[stride] = 0.5 * (1.0 + x);
+  d[stride] = 0.5;
   double pm2 = 1.0, pm1 = x;  // P_{k-2}, P_{k-1}
   for (int k = 2; k < n; ++k) {
     const double pk = ((2.0 * k - 1.0) * x * pm1 - (k - 1.0) * pm2) / double(k);
-    v[k * kCloudLanes] = (pk - pm2) / sqrt(4.0 * k - 2.0);
-    d[k * kCloudLanes] = sqrt((2.0 * k - 1.0) / 2.0) * pm1;
+    v[k * stride] = (pk - pm2) / sqrt(4.0 * k - 2.0);
+    d[k * stride] = sqrt((2.0 * k - 1.0) / 2.0) * pm1;
     pm2 = pm1;
     pm1 = pk;
   }
 }
 
-// Mode multi-index of flat box index i (axis 0 fastest).
-__device__ inline void unflatten(const CloudArgs& a, int i, int (&m)[kMaxDim]) {
-  m[0] = i % a.N[0];
-  const int t = i / a.N[0];
-  m[1] = a.D > 1 ? t % a.N[1] : 0;
-  m[2] = a.D > 2 ? t / a.N[1] : 0;
+// Side of local mode i and its mode multi-index (axis 0 fastest).
+__device__ inline int unflatten(const CloudArgs& a, int i, int (&m)[kMaxDim]) {
+  const int s = (a.S > 1 && i >= a.Ns[0]) ? 1 : 0;
+  const int il = s ? i - a.Ns[0] : i;
+  m[0] = il % a.N[s][0];
+  const int t = il / a.N[s][0];
+  m[1] = a.D > 1 ? t % a.N[s][1] : 0;
+  m[2] = a.D > 2 ? t / a.N[s][1] : 0;
+  return s;
 }
 
-// Phase 1 for one chunk: 1D tables of each point into tv / td
-// ([axis][k][lane]), value and physical gradient of u, point action into
-// ch[c][lane] (channel c = 0 value, 1 + a gradient along a). Returns nothing;
-// lanes >= cnt write zeros.
+// Offset of side s's tables in tv / td ([side][axis][k][lane]).
+__device__ inline int table_base(const CloudArgs& a, int s) { return s * a.D * a.Nmax * a.chunk; }
+
+// Basis value phi and reference derivatives dphi[ax] of multi-index m on
+// side s at chunk lane q.
+__device__ inline void basis_at(const CloudArgs& a, const double* tv, const double* td, int s, const int (&m)[kMaxDim],
+                                int q, double& phi, double (&dphi)[kMaxDim]) {
+  double f[kMaxDim], fd[kMaxDim];
+  const int base = table_base(a, s);
+  for (int ax = 0; ax < a.D; ++ax) {
+    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+  }
+  phi = 1.0;
+  for (int ax = 0; ax < a.D; ++ax) phi *= f[ax];
+  for (int ax = 0; ax < a.D; ++ax) {
+    double v = fd[ax];
+    for (int bx = 0; bx < a.D; ++bx)
+      if (bx != ax) v *= f[bx];
+    dphi[ax] = v;
+  }
+}
+
+// Lane q < cnt of a chunk: 1D factors of point p at every side into tv / td.
+__device__ inline void point_tables(const CloudArgs& a, int64_t p, int q, double* tv, double* td) {
+  for (int s = 0; s < a.S; ++s)
+    for (int ax = 0; ax < a.D; ++ax) {
+      const int off = table_base(a, s) + ax * a.Nmax * a.chunk + q;
+      integrated_legendre(a.rpts[p * a.S * a.D + s * a.D + ax], a.N[s][ax], a.chunk, tv + off, td + off);
+    }
+}
+
+// Phase 1 for one chunk (lanes q < cnt): value and physical gradient of u on
+// every side, point action into ch[r][lane] (channel r = s(D+1) + c; c = 0
+// value, 1 + a gradient along a), gradient channels already scaled by J^-1.
 __device__ void chunk_forward(const CloudArgs& a, int64_t item, int64_t p0, int cnt, const double* u, double* tv,
                               double* td, double* ch) {
   const int q = threadIdx.x;
+  if (q >= cnt) return;
   const int C = a.D + 1;
-  const double* jinv = a.jinv + item * a.D;
-  if (q < cnt) {
-    for (int ax = 0; ax < a.D; ++ax)
-      integrated_legendre(a.rpts[(p0 + q) * a.D + ax], a.N[ax], tv + ax * a.Nmax * kCloudLanes + q,
-                          td + ax * a.Nmax * kCloudLanes + q);
+  const int SC = a.S * C;
+  const double* jinv = a.jinv + item * a.S * a.D;
+  point_tables(a, p0 + q, q, tv, td);
+  double chn[kMaxSides * (kMaxDim + 1)];
+  for (int s = 0; s < a.S; ++s) {
     double val = 0.0, g[kMaxDim] = {0.0, 0.0, 0.0};
-    for (int i = 0; i < a.Ntot; ++i) {
+    const int first = s ? a.Ns[0] : 0;
+    for (int i = 0; i < a.Ns[s]; ++i) {
       int m[kMaxDim];
-      unflatten(a, i, m);
-      double f[kMaxDim], fd[kMaxDim];
-      for (int ax = 0; ax < a.D; ++ax) {
-        f[ax] = tv[(ax * a.Nmax + m[ax]) * kCloudLanes + q];
-        fd[ax] = td[(ax * a.Nmax + m[ax]) * kCloudLanes + q];
-      }
-      double phi = 1.0;
-      for (int ax = 0; ax < a.D; ++ax) phi *= f[ax];
-      const double ui = u[i];
+      unflatten(a, first + i, m);
+      double phi, dphi[kMaxDim];
+      basis_at(a, tv, td, s, m, q, phi, dphi);
+      const double ui = u[first + i];
       val = fma(ui, phi, val);
-      for (int ax = 0; ax < a.D; ++ax) {
-        double dphi = fd[ax];
-        for (int bx = 0; bx < a.D; ++bx)
-          if (bx != ax) dphi *= f[bx];
-        g[ax] = fma(ui, dphi, g[ax]);
-      }
+      for (int ax = 0; ax < a.D; ++ax) g[ax] = fma(ui, dphi[ax], g[ax]);
     }
-    double chn[kMaxDim + 1];
-    chn[0] = val;
-    for (int ax = 0; ax < a.D; ++ax) chn[ax + 1] = jinv[ax] * g[ax];
-    const double* M = a.pdata + (p0 + q) * C * C;
-    for (int c = 0; c < C; ++c) {
-      double acc = 0.0;
-      for (int k = 0; k < C; ++k) acc = fma(M[c * C + k], chn[k], acc);
-      ch[c * kCloudLanes + q] = c == 0 ? acc : acc * jinv[c - 1];
-    }
-  } else {
-    for (int c = 0; c < C; ++c) ch[c * kCloudLanes + q] = 0.0;
+    chn[s * C] = val;
+    for (int ax = 0; ax < a.D; ++ax) chn[s * C + 1 + ax] = jinv[s * a.D + ax] * g[ax];
+  }
+  const double* M = a.pdata + (p0 + q) * SC * SC;
+  for (int r = 0; r < SC; ++r) {
+    double acc = 0.0;
+    for (int k = 0; k < SC; ++k) acc = fma(M[r * SC + k], chn[k], acc);
+    const int s = r / C, c = r % C;
+    ch[r * a.chunk + q] = c == 0 ? acc : acc * jinv[s * a.D + c - 1];
   }
 }
 
@@ -112,40 +148,32 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
                                                                 double* __restrict__ y) {
   extern __shared__ double smem[];
   const int64_t item = blockIdx.x;
+  const int C = a.D + 1;
   double* u = smem;
   double* r = u + a.Ntot;
   double* tv = r + a.Ntot;
-  double* td = tv + a.D * a.Nmax * kCloudLanes;
-  double* ch = td + a.D * a.Nmax * kCloudLanes;
+  double* td = tv + a.S * a.D * a.Nmax * a.chunk;
+  double* ch = td + a.S * a.D * a.Nmax * a.chunk;
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
     u[i] = gather_term(a.map, item, i, x);
     r[i] = 0.0;
   }
   __syncthreads();
   const int64_t pb = a.poff[item], pe = a.poff[item + 1];
-  for (int64_t p0 = pb; p0 < pe; p0 += kCloudLanes) {
-    const int cnt = int(pe - p0 < kCloudLanes ? pe - p0 : kCloudLanes);
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
     chunk_forward(a, item, p0, cnt, u, tv, td, ch);
     __syncthreads();
     for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
       int m[kMaxDim];
-      unflatten(a, i, m);
+      const int s = unflatten(a, i, m);
+      const double* chs = ch + s * C * a.chunk;
       double acc = 0.0;
       for (int q = 0; q < cnt; ++q) {
-        double f[kMaxDim], fd[kMaxDim];
-        for (int ax = 0; ax < a.D; ++ax) {
-          f[ax] = tv[(ax * a.Nmax + m[ax]) * kCloudLanes + q];
-          fd[ax] = td[(ax * a.Nmax + m[ax]) * kCloudLanes + q];
-        }
-        double phi = 1.0;
-        for (int ax = 0; ax < a.D; ++ax) phi *= f[ax];
-        acc = fma(ch[q], phi, acc);
-        for (int ax = 0; ax < a.D; ++ax) {
-          double dphi = fd[ax];
-          for (int bx = 0; bx < a.D; ++bx)
-            if (bx != ax) dphi *= f[bx];
-          acc = fma(ch[(ax + 1) * kCloudLanes + q], dphi, acc);
-        }
+        double phi, dphi[kMaxDim];
+        basis_at(a, tv, td, s, m, q, phi, dphi);
+        acc = fma(chs[q], phi, acc);
+        for (int ax = 0; ax < a.D; ++ax) acc = fma(chs[(ax + 1) * a.chunk + q], dphi[ax], acc);
       }
       r[i] += acc;
     }
@@ -154,51 +182,46 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], y, false);
 }
 
-// Local diagonal d_i = sum_q sum_{c,k} M_q[c][k] chi_c chi_k with chi_0 = phi_i,
-// chi_{1+a} = J^-1_a d_a phi_i, scattered with squared coefficients.
+// Local diagonal d_i = sum_q sum_{c,k} M_q[(s,c)][(s,k)] chi_c chi_k with
+// chi_0 = phi_i, chi_{1+a} = J^-1_a d_a phi_i on the side s of mode i (the
+// other side's channels of a trace basis function are zero,
+// src/accumulators.jl:723-736), scattered with squared coefficients.
 __global__ void __launch_bounds__(kCloudLanes) cell_diagonal_cloud(CloudArgs a, double* __restrict__ diag) {
   extern __shared__ double smem[];
   const int64_t item = blockIdx.x;
   const int C = a.D + 1;
+  const int SC = a.S * C;
   double* r = smem;
   double* tv = r + a.Ntot;
-  double* td = tv + a.D * a.Nmax * kCloudLanes;
-  double* Ms = td + a.D * a.Nmax * kCloudLanes;  // C*C x 64
-  const double* jinv = a.jinv + item * a.D;
+  double* td = tv + a.S * a.D * a.Nmax * a.chunk;
+  double* Ms = td + a.S * a.D * a.Nmax * a.chunk;  // S x C x C x chunk: diagonal blocks
+  const double* jinv = a.jinv + item * a.S * a.D;
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] = 0.0;
   __syncthreads();
   const int64_t pb = a.poff[item], pe = a.poff[item + 1];
-  for (int64_t p0 = pb; p0 < pe; p0 += kCloudLanes) {
-    const int cnt = int(pe - p0 < kCloudLanes ? pe - p0 : kCloudLanes);
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
     const int q = threadIdx.x;
     if (q < cnt) {
-      for (int ax = 0; ax < a.D; ++ax)
-        integrated_legendre(a.rpts[(p0 + q) * a.D + ax], a.N[ax], tv + ax * a.Nmax * kCloudLanes + q,
-                            td + ax * a.Nmax * kCloudLanes + q);
-      for (int e = 0; e < C * C; ++e) Ms[e * kCloudLanes + q] = a.pdata[(p0 + q) * C * C + e];
+      point_tables(a, p0 + q, q, tv, td);
+      const double* M = a.pdata + (p0 + q) * SC * SC;
+      for (int s = 0; s < a.S; ++s)
+        for (int c = 0; c < C; ++c)
+          for (int k = 0; k < C; ++k)
+            Ms[((s * C + c) * C + k) * a.chunk + q] = M[(s * C + c) * SC + s * C + k];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
       int m[kMaxDim];
-      unflatten(a, i, m);
+      const int s = unflatten(a, i, m);
       double acc = 0.0;
       for (int qq = 0; qq < cnt; ++qq) {
-        double f[kMaxDim], fd[kMaxDim], chi[kMaxDim + 1];
-        for (int ax = 0; ax < a.D; ++ax) {
-          f[ax] = tv[(ax * a.Nmax + m[ax]) * kCloudLanes + qq];
-          fd[ax] = td[(ax * a.Nmax + m[ax]) * kCloudLanes + qq];
-        }
-        double phi = 1.0;
-        for (int ax = 0; ax < a.D; ++ax) phi *= f[ax];
+        double phi, dphi[kMaxDim], chi[kMaxDim + 1];
+        basis_at(a, tv, td, s, m, qq, phi, dphi);
         chi[0] = phi;
-        for (int ax = 0; ax < a.D; ++ax) {
-          double dphi = fd[ax];
-          for (int bx = 0; bx < a.D; ++bx)
-            if (bx != ax) dphi *= f[bx];
-          chi[ax + 1] = jinv[ax] * dphi;
-        }
+        for (int ax = 0; ax < a.D; ++ax) chi[ax + 1] = jinv[s * a.D + ax] * dphi[ax];
         for (int c = 0; c < C; ++c)
-          for (int k = 0; k < C; ++k) acc = fma(Ms[(c * C + k) * kCloudLanes + qq] * chi[c], chi[k], acc);
+          for (int k = 0; k < C; ++k) acc = fma(Ms[((s * C + c) * C + k) * a.chunk + qq] * chi[c], chi[k], acc);
       }
       r[i] += acc;
     }
@@ -207,13 +230,39 @@ __global__ void __launch_bounds__(kCloudLanes) cell_diagonal_cloud(CloudArgs a, 
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], diag, true);
 }
 
+size_t apply_lds(const CloudArgs& a) {
+  return sizeof(double) * (size_t(2 * a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.chunk) +
+                           size_t(a.S * (a.D + 1) * a.chunk));
+}
+
+size_t diagonal_lds(const CloudArgs& a) {
+  return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.chunk) +
+                           size_t(a.S * (a.D + 1) * (a.D + 1) * a.chunk));
+}
+
+// Largest chunk (64, 32, .., 8 points) whose LDS footprint fits 64 KB.
+template <class F>
+void pick_chunk(CloudArgs& a, F lds_of) {
+  a.chunk = kCloudLanes;
+  while (a.chunk > 8 && lds_of(a) > 64 * 1024) a.chunk /= 2;
+  require(lds_of(a) <= 64 * 1024, GMF_EUNSUPPORTED, "point-cloud batch exceeds the LDS budget");
+}
+
 CloudArgs make_cloud_args(const Bucket& b, bool reduced) {
   CloudArgs a{};
   a.D = b.dim;
+  a.S = b.sides;
   a.Nmax = 1;
-  for (int d = 0; d < kMaxDim; ++d) a.N[d] = b.N[d];
-  for (int d = 0; d < b.dim; ++d) a.Nmax = b.N[d] > a.Nmax ? b.N[d] : a.Nmax;
   a.Ntot = b.Ntot;
+  for (int s = 0; s < kMaxSides; ++s) {
+    a.Ns[s] = 1;
+    for (int d = 0; d < kMaxDim; ++d) {
+      a.N[s][d] = s == 0 ? b.N[d] : b.Np[d];
+      if (d < b.dim) a.Ns[s] *= a.N[s][d];
+      if (s < a.S && d < b.dim) a.Nmax = a.N[s][d] > a.Nmax ? a.N[s][d] : a.Nmax;
+    }
+  }
+  require(a.S == 1 ? a.Ns[0] == a.Ntot : a.Ns[0] + a.Ns[1] == a.Ntot, GMF_EINVAL, "interface box sizes do not add up");
   a.jinv = b.jinv;
   a.pdata = b.pdata;
   a.poff = b.poff;
@@ -229,9 +278,8 @@ void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double
   if (b.cells == 0) return;
   CloudArgs a = make_cloud_args(b, reduced);
   require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
-  const size_t lds = sizeof(double) * size_t(2 * a.Ntot + 2 * a.D * a.Nmax * kCloudLanes + (a.D + 1) * kCloudLanes);
-  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "point-cloud batch exceeds the LDS budget");
-  hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.cells)), dim3(kCloudLanes), lds, stream, a, x, y);
+  pick_chunk(a, apply_lds);
+  hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.cells)), dim3(kCloudLanes), apply_lds(a), stream, a, x, y);
   GMF_HIP(hipGetLastError());
 }
 
@@ -239,10 +287,9 @@ void launch_cell_cloud_diagonal(const gmf_plan&, const Bucket& b, double* diag, 
   if (b.cells == 0) return;
   CloudArgs a = make_cloud_args(b, reduced);
   require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
-  const int C = a.D + 1;
-  const size_t lds = sizeof(double) * size_t(a.Ntot + 2 * a.D * a.Nmax * kCloudLanes + C * C * kCloudLanes);
-  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "point-cloud batch exceeds the LDS budget");
-  hipLaunchKernelGGL(cell_diagonal_cloud, dim3(unsigned(b.cells)), dim3(kCloudLanes), lds, stream, a, diag);
+  pick_chunk(a, diagonal_lds);
+  hipLaunchKernelGGL(cell_diagonal_cloud, dim3(unsigned(b.cells)), dim3(kCloudLanes), diagonal_lds(a), stream, a,
+                     diag);
   GMF_HIP(hipGetLastError());
 }
 

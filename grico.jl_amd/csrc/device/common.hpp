@@ -53,6 +53,25 @@ struct BrickPath {
   int64_t slot_base = 0;               // offset of this bucket's slots in the workspace array
 };
 
+// Sum-factorized path of a point-cloud batch of conforming faces
+// (face_tensor.hip): every item's points are the same tensor grid of
+// tangential Gauss coordinates on both sides, at a fixed normal coordinate
+// +-1 per side. The point action is stored compressed to the nonzero
+// (test channel, trial channel) pattern shared by all points.
+struct FacePath {
+  bool enabled = false;
+  int axis = 0;                 // normal axis
+  int T[2] = {-1, -1};          // tangential axes (-1: absent)
+  int Qt[2] = {1, 1};           // tangential points per slot
+  int toff[2][6] = {};          // per side: offsets of psi(+-1), psi'(+-1), tv0, td0, tv1, td1 in tables
+  int table_doubles = 0;
+  double* tables = nullptr;
+  int nnz = 0;
+  unsigned char pr[64] = {}, pc[64] = {};  // pattern: test channel, trial channel
+  double* pdata = nullptr;      // points x nnz
+  size_t lds = 0;
+};
+
 // Device-resident cell batch.
 struct Bucket {
   int dim = 0;
@@ -91,6 +110,12 @@ struct Bucket {
   // Point-cloud batch: item c owns points poff[c]..poff[c+1]-1 at biunit
   // reference coordinates rpts (points x dim); pdata rows per point.
   bool cloud = false;
+  // Interface items (sides = 2): the local box is the minus box (N) followed
+  // by the plus box (Np); rpts and jinv hold both sides per point / item and
+  // pdata rows are (2(D+1))^2.
+  int sides = 1;
+  int Np[kMaxDim] = {1, 1, 1};
+  FacePath face;
   int64_t total_points = 0;
   int max_points = 0;
   int64_t* poff = nullptr;
@@ -204,6 +229,11 @@ void launch_cell_diagonal_generic_skip(const gmf_plan& plan, const Bucket& b, do
 void launch_cell_cloud(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                        hipStream_t stream);
 void launch_cell_cloud_diagonal(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced, hipStream_t stream);
+// Detect conforming-face batches and prepare their sum-factorized path
+// (face_tensor.hip; GMF_FACE_TENSOR=0 keeps every batch on the point-cloud kernel).
+void prepare_face_path(gmf_plan& plan, Bucket& b, const gmf_bucket_desc& bd);
+bool launch_face_tensor(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
+                        hipStream_t stream);
 // slots: the workspace's brick corner array (plan.brick_slots entries).
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream);

@@ -186,7 +186,7 @@ void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   for (size_t i = 0; i < p.buckets.size(); ++i) {
     const Bucket& b = p.buckets[i];
     if (b.cloud) {
-      launch_cell_cloud(p, b, x, y, reduced, ws->stream);
+      if (!launch_face_tensor(p, b, x, y, reduced, ws->stream)) launch_cell_cloud(p, b, x, y, reduced, ws->stream);
       continue;
     }
     if (b.fast && launch_cell_fast(p, b, x, y, reduced, ws->brick_slots, ws->stream)) continue;
@@ -665,6 +665,9 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         require(b.cells >= 0 && b.cells < (int64_t(1) << 31), GMF_EINVAL, "cell count out of range");
         int off = 0;
         b.cloud = bd.point_offsets != nullptr;
+        require(bd.sides >= 0 && bd.sides <= 2, GMF_EINVAL, "sides must be 0, 1 or 2");
+        b.sides = bd.sides == 2 ? 2 : 1;
+        require(b.sides == 1 || b.cloud, GMF_EINVAL, "interface items must be point-cloud batches");
         for (int a = 0; a < b.dim; ++a) {
           b.N[a] = bd.modes[a];
           b.Q[a] = b.cloud ? 1 : bd.points[a];
@@ -675,7 +678,17 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           b.table_offset[a] = off;
           off += b.N[a] * b.Q[a];
         }
-        b.point_stride = b.form == GMF_FORM_DIFFUSION ? 1 : (b.dim + 1) * (b.dim + 1);
+        if (b.sides == 2) {
+          int nplus = 1;
+          for (int a = 0; a < b.dim; ++a) {
+            b.Np[a] = bd.plus_modes[a];
+            require(b.Np[a] >= 1 && b.Np[a] <= kMaxExtent, GMF_EUNSUPPORTED,
+                    "per-axis mode counts must lie in 1..16");
+            nplus *= b.Np[a];
+          }
+          b.Ntot += nplus;
+        }
+        b.point_stride = b.form == GMF_FORM_DIFFUSION ? 1 : (b.sides * (b.dim + 1)) * (b.sides * (b.dim + 1));
         if (b.cloud) {
           require(b.form == GMF_FORM_GENERAL, GMF_EINVAL, "point-cloud batches carry the general point action");
           require(bd.ref_points != nullptr, GMF_EINVAL, "point-cloud batch without reference points");
@@ -686,11 +699,11 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
             b.max_points = std::max<int>(b.max_points, int(po[c + 1] - po[c]));
           }
           b.total_points = po[b.cells];
-          for (int64_t k = 0; k < b.total_points * b.dim; ++k)
+          for (int64_t k = 0; k < b.total_points * b.sides * b.dim; ++k)
             require(std::isfinite(bd.ref_points[k]) && std::fabs(bd.ref_points[k]) <= 1.0 + 1e-12, GMF_EINVAL,
                     "reference points must lie in the biunit cell");
           b.poff = upload(p, po, size_t(b.cells) + 1);
-          b.rpts = upload(p, bd.ref_points, size_t(b.total_points) * size_t(b.dim));
+          b.rpts = upload(p, bd.ref_points, size_t(b.total_points) * size_t(b.sides) * size_t(b.dim));
         } else {
           b.h_values.assign(bd.values, bd.values + off);
           b.h_derivs.assign(bd.derivatives, bd.derivatives + off);
@@ -702,9 +715,10 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           continue;
         }
         const size_t nmode = size_t(b.cells) * size_t(b.Ntot);
-        for (size_t i = 0; i < size_t(b.cells) * size_t(b.dim); ++i)
+        const size_t njinv = size_t(b.cells) * size_t(b.sides) * size_t(b.dim);
+        for (size_t i = 0; i < njinv; ++i)
           require(std::isfinite(bd.inverse_jacobian[i]), GMF_EINVAL, "non-finite inverse Jacobian");
-        b.jinv = upload(p, bd.inverse_jacobian, size_t(b.cells) * size_t(b.dim));
+        b.jinv = upload(p, bd.inverse_jacobian, njinv);
         b.pdata = upload(p, bd.point_data,
                          (b.cloud ? size_t(b.total_points) : size_t(b.cells) * size_t(b.Qtot)) * size_t(b.point_stride));
         b.multi_count = bd.multi_count;
@@ -741,6 +755,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           build_diagonal_pairs(p, b, r, bd, mr);
         }
         if (!b.cloud) prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
+        else prepare_face_path(p, b, bd);
         p.buckets.push_back(std::move(b));
       }
       GMF_HIP(hipDeviceSynchronize());
@@ -783,7 +798,8 @@ gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity) 
       s += "bucket " + std::to_string(i) + ": dim=" + std::to_string(b.dim) + " N=" + std::to_string(b.N[0]) +
            " Q=" + std::to_string(b.Q[0]) + " cells=" + std::to_string(b.cells) + " form=" + std::to_string(b.form) +
            " multi=" + std::to_string(b.multi_count) + " path=" +
-           (b.cloud ? "cloud points=" + std::to_string(b.total_points)
+           (b.cloud ? (b.sides == 2 ? "interface points=" : "cloud points=") + std::to_string(b.total_points) +
+                          (b.face.enabled ? " face-tensor nnz=" + std::to_string(b.face.nnz) : std::string())
                     : (b.bricks.enabled ? "brick" : (b.fast ? "fast-cell" : "generic")));
       if (b.bricks.enabled) {
         s += " bricks=" + std::to_string(b.bricks.bricks) + " colors=" + std::to_string(b.bricks.colors) + " [";

@@ -3,7 +3,8 @@
 The names follow the reference's public surface (src/Grico.jl exports):
 Domain, HpSpace, SpaceOptions, FullTensorBasis, TrunkBasis, UniformDegree,
 AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, ScalarField, AffineProblem,
-SPD, TestChannels, add_cell_accumulator, add_constraint, Dirichlet,
+SPD, TestChannels, TraceTestChannels, add_cell_accumulator,
+add_boundary_accumulator, add_interface_accumulator, jump, average, add_constraint, Dirichlet,
 BoundaryFace, LOWER, UPPER, compile, apply, solve, CGSolver, FGMRESSolver,
 JacobiPreconditioner, GeometricMultigridPreconditioner, l2_error,
 relative_l2_error. Julia's `!` mutators map to `*_into` functions.
@@ -13,7 +14,10 @@ from .mesh import Domain, uniform_refine
 from .space import (AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, FullTensorBasis, HpSpace, SpaceOptions,
                     StoredDegrees, TrunkBasis, UniformDegree, scalar_dof_count)
 from .problem import (LOWER, UPPER, AffineProblem, BoundaryFace, Dirichlet, NonsymmetricOperator, ScalarField, SPD,
-                      TestChannels, TrialChannels, add_cell_accumulator, add_constraint, gradient, value)
+                      TestChannels, TraceTestChannels, TraceTrialChannels, TracePair, TrialChannels,
+                      add_boundary_accumulator, add_cell_accumulator, add_constraint, add_interface_accumulator,
+                      average, cell_size, component, gradient, inner, jump, minus, normal, normal_component, plus,
+                      value, weight)
 from .plan import AssemblyPlan, compile
 from .ops import (apply, apply_into, apply_physical, diagonal_physical, expand_reduced, project_reduced,
                   reduced_apply, reduced_diagonal)

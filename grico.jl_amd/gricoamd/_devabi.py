@@ -25,6 +25,7 @@ class BucketDesc(ctypes.Structure):
         ("local_index", P_i32), ("local_coefficient", P_f64),
         ("multi_count", c_int64), ("multi_offsets", P_i64), ("multi_indices", P_i32),
         ("multi_coefficients", P_f64), ("point_offsets", P_i64), ("ref_points", P_f64),
+        ("sides", c_int32), ("plus_modes", c_int32 * 3),
     ]
 
 
@@ -146,6 +147,10 @@ class DevicePlan:
                 d.points[a] = (1 if cloud else b.points[a]) if a < b.dim else 1
             d.form = b.form
             d.cell_count = b.cell_count
+            d.sides = getattr(b, "sides", 1)
+            if d.sides == 2:
+                for a in range(3):
+                    d.plus_modes[a] = b.plus_modes[a] if a < b.dim else 1
             pdata = np.ascontiguousarray(b.point_data, dtype=np.float64)
             coef = None if b.unit_coefficients else b.local_coefficient
             keep += [pdata, b.inverse_jacobian, b.local_index, coef, b.multi_offsets,

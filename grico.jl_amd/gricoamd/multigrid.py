@@ -195,7 +195,8 @@ def has_coarsening_level(space, policy):
 
 
 def problem_on_space(problem, space):
-    """The same operators and constraints on another space (src/multigrid.jl:596-612)."""
+    """The same operators and constraints on another space (src/multigrid.jl:596-612):
+    cell, boundary-face and interface operators carry over."""
     old = problem.field
     field = ScalarField(space, old.name)
     out = AffineProblem(field, problem.operator_class)
@@ -203,6 +204,14 @@ def problem_on_space(problem, space):
         out.add_cell_accumulator(field, field, op)
     for (_, op) in problem.cell_rhs:
         out.add_cell_accumulator(field, op)
+    for (bf, _, _, op) in getattr(problem, "boundary_operators", []):
+        out.add_boundary_accumulator(bf, field, field, op)
+    for (bf, _, op) in getattr(problem, "boundary_rhs", []):
+        out.add_boundary_accumulator(bf, field, op)
+    for (_, _, op) in getattr(problem, "interface_operators", []):
+        out.add_interface_accumulator(field, field, op)
+    for (_, op) in getattr(problem, "interface_rhs", []):
+        out.add_interface_accumulator(field, op)
     for con in problem.constraints:
         out.add_constraint(Dirichlet(field, con.boundary, con.data))
     return out

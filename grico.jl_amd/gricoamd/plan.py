@@ -15,7 +15,8 @@ import ctypes
 import numpy as np
 
 from . import _devabi, _native
-from .problem import AffineProblem, QuadraturePoints, TestChannels, TrialChannels
+from .problem import (AffineProblem, PointQuadrature, QuadraturePoints, TestChannels, TraceTestChannels,
+                      TraceTrialChannels, TrialChannels)
 from .quadrature import basis_tables, gauss_legendre_rule
 
 FORM_DIFFUSION = 0
@@ -257,12 +258,16 @@ class CloudBucket:
             self.normals = None
         Bucket._gather_maps(self, space)
 
+    face_axis = None
+    face_side = None
+
     def quadrature(self):
         return QuadraturePoints(self.phys_points, self.weights,
                                 [self.sizes[self.item_of_point, a] for a in range(self.dim)],
-                                normal=None if self.normals is None else [self.normals[:, a] for a in range(self.dim)])
+                                normal=None if self.normals is None else [self.normals[:, a] for a in range(self.dim)],
+                                face_axis=self.face_axis, face_side=self.face_side)
 
-    def lower(self, operators, surface=False):
+    def lower(self, operators, surface=False, callback=None):
         """Probe the callbacks with unit trial channels at every point:
         point_data (P x (D+1)^2) = W_q M_q, row-major [test channel][trial channel]."""
         D = self.dim
@@ -270,8 +275,9 @@ class CloudBucket:
         q = self.quadrature()
         ones, zeros = np.ones(P), np.zeros(P)
         M = np.zeros((P, D + 1, D + 1))
+        callback = callback or ("surface_accumulate" if surface else "cell_accumulate")
         for op in operators:
-            call = op.surface_accumulate if surface else op.cell_accumulate
+            call = getattr(op, callback)
             for ch in range(D + 1):
                 val = ones if ch == 0 else zeros
                 grad = tuple(ones if ch == a + 1 else zeros for a in range(D))
@@ -326,6 +332,199 @@ class CloudBucket:
         return out
 
 
+def _tangential_rule(shape, axis):
+    """Gauss tensor rule over the tangential axes of a face (first tangential
+    axis fastest, src/integration.jl:2866-2884): (points (n x D-1), weights)."""
+    D = len(shape)
+    tang = [b for b in range(D) if b != axis]
+    if not tang:
+        return np.zeros((1, 0)), np.ones(1)
+    rules = [gauss_legendre_rule(int(shape[b])) for b in tang]
+    grids = np.meshgrid(*[r[0] for r in rules[::-1]], indexing="ij")
+    wgrids = np.meshgrid(*[r[1] for r in rules[::-1]], indexing="ij")
+    pts = np.stack([g.reshape(-1) for g in grids[::-1]], axis=1)
+    w = np.ones(pts.shape[0])
+    for g in wgrids:
+        w = w * g.reshape(-1)
+    return pts, w
+
+
+def boundary_face_rule(shape, axis, side):
+    """Reference points and weights on one face of the biunit cell
+    (src/integration.jl:2846-2862); the face axis of `shape` is ignored."""
+    D = len(shape)
+    t, w = _tangential_rule(shape, axis)
+    pts = np.zeros((t.shape[0], D))
+    tang = [b for b in range(D) if b != axis]
+    for k, b in enumerate(tang):
+        pts[:, b] = t[:, k]
+    pts[:, axis] = -1.0 if side == 0 else 1.0
+    return PointQuadrature(pts, w)
+
+
+class InterfaceBucket:
+    """Interior interfaces between two leaves (src/integration.jl:2262-2316
+    _compile_interface, 2800-2841 _interface_face_points): items share the
+    face axis, both sides' degree tuples and the tangential point shape. The
+    points are Gauss points of the shared face patch; each side evaluates its
+    own basis at its reference coordinates. The point action is lowered by
+    probing `interface_accumulate` with the 2(D+1) unit trace channels:
+    point_data (P x (2(D+1))^2) = W_q M_q over channels s(D+1) + c."""
+
+    cloud = True
+    sides = 2
+    form = FORM_GENERAL
+
+    def __init__(self, space, minus_idx, plus_idx, axis, degrees_minus, degrees_plus, tshape):
+        dom = space.domain
+        self.dim = D = space.dim
+        self.axis = axis = int(axis)
+        self.minus_idx = np.asarray(minus_idx, dtype=np.int64)
+        self.plus_idx = np.asarray(plus_idx, dtype=np.int64)
+        self.degrees = tuple(int(p) for p in degrees_minus)
+        self.plus_degrees = tuple(int(p) for p in degrees_plus)
+        self.modes = tuple(p + 1 for p in self.degrees)
+        self.plus_modes = tuple(p + 1 for p in self.plus_degrees)
+        self.n_minus = int(np.prod(self.modes))
+        self.n_plus = int(np.prod(self.plus_modes))
+        self.n_modes = self.n_minus + self.n_plus
+        self.cell_count = nc = len(self.minus_idx)
+        cm, hm = dom.cell_centers_and_half_sizes(space.leaves[self.minus_idx])
+        cp, hp = dom.cell_centers_and_half_sizes(space.leaves[self.plus_idx])
+        tang = [b for b in range(D) if b != axis]
+        lo = np.maximum(cm - hm, cp - hp)
+        up = np.minimum(cm + hm, cp + hp)
+        half = 0.5 * (up - lo)
+        if tang and np.any(half[:, tang] <= 0):
+            raise ValueError("interface leaves do not share a face patch")
+        t, w = _tangential_rule(tshape, axis)
+        n = t.shape[0]
+        self.point_offsets = np.arange(nc + 1, dtype=np.int64) * n
+        self.n_points_total = P = nc * n
+        self.item_of_point = np.repeat(np.arange(nc), n)
+        phys = np.zeros((P, D))
+        for k, b in enumerate(tang):
+            phys[:, b] = (lo[:, b:b + 1] + half[:, b:b + 1] * (1.0 + t[None, :, k])).reshape(-1)
+        phys[:, axis] = np.repeat((cm + hm)[:, axis], n)
+        scale = np.prod(half[:, tang], axis=1) if tang else np.ones(nc)
+        self.weights = np.repeat(scale, n) * np.tile(w, nc)
+        ref = np.zeros((P, 2 * D))
+        ip = self.item_of_point
+        for b in range(D):
+            ref[:, b] = (phys[:, b] - cm[ip, b]) / hm[ip, b]
+            ref[:, D + b] = (phys[:, b] - cp[ip, b]) / hp[ip, b]
+        ref[:, axis] = 1.0
+        ref[:, D + axis] = -1.0
+        self.ref = np.ascontiguousarray(np.clip(ref, -1.0, 1.0))
+        self.phys_points = [phys[:, b] for b in range(D)]
+        self.inverse_jacobian = np.ascontiguousarray(np.concatenate([1.0 / hm, 1.0 / hp], axis=1))
+        self.sizes = np.minimum(2.0 * hm, 2.0 * hp)
+        self.normals = np.zeros((P, D))
+        self.normals[:, axis] = 1.0
+        self._gather_maps(space)
+
+    def _gather_maps(self, space):
+        class _Side:
+            pass
+
+        sides = []
+        for idx, modes in ((self.minus_idx, self.modes), (self.plus_idx, self.plus_modes)):
+            sd = _Side()
+            sd.leaf_idx, sd.cell_count, sd.n_modes = idx, len(idx), int(np.prod(modes))
+            Bucket._gather_maps(sd, space)
+            sides.append(sd)
+        m, p = sides
+        nm = len(m.multi_offsets) - 1
+        pl = p.local_index.astype(np.int64)
+        pl = np.where(pl <= -2, pl - nm, pl)
+        self.local_index = np.ascontiguousarray(np.concatenate([m.local_index, pl.astype(np.int32)], axis=1))
+        self.local_coefficient = np.ascontiguousarray(np.concatenate([m.local_coefficient, p.local_coefficient], 1))
+        self.unit_coefficients = m.unit_coefficients and p.unit_coefficients
+        self.multi_offsets = np.concatenate([m.multi_offsets, p.multi_offsets[1:] + m.multi_offsets[-1]])
+        self.multi_indices = np.concatenate([m.multi_indices, p.multi_indices]).astype(np.int32)
+        self.multi_coefficients = np.concatenate([m.multi_coefficients, p.multi_coefficients])
+        self.term_cell = np.concatenate([m.term_cell, p.term_cell])
+        self.term_box = np.concatenate([m.term_box, p.term_box + self.n_minus])
+        self.term_dof = np.concatenate([m.term_dof, p.term_dof])
+        self.term_coef = np.concatenate([m.term_coef, p.term_coef])
+
+    def quadrature(self):
+        return QuadraturePoints(self.phys_points, self.weights,
+                                [self.sizes[self.item_of_point, a] for a in range(self.dim)],
+                                normal=[self.normals[:, a] for a in range(self.dim)], face_axis=self.axis)
+
+    def trace_channels(self, result):
+        """(value, gradient) arrays of the minus and plus test channels of one callback result."""
+        if not isinstance(result, TraceTestChannels):
+            raise TypeError("interface callbacks must return TraceTestChannels")
+        shape = (self.n_points_total,)
+        return [_channels(side, shape, self.dim) for side in (result.minus, result.plus)]
+
+    def lower(self, operators):
+        D = self.dim
+        C = D + 1
+        P = self.n_points_total
+        q = self.quadrature()
+        ones, zeros = np.ones(P), np.zeros(P)
+        M = np.zeros((P, 2 * C, 2 * C))
+        for op in operators:
+            for ch in range(2 * C):
+                sides = []
+                for s in range(2):
+                    k = ch - s * C
+                    val = ones if k == 0 else zeros
+                    grad = tuple(ones if k == a + 1 else zeros for a in range(D))
+                    sides.append(TrialChannels(0, val, grad))
+                res = op.interface_accumulate(q, TraceTrialChannels(0, sides[0], sides[1]), 0)
+                for s, (v, g) in enumerate(self.trace_channels(res)):
+                    M[:, s * C, ch] += v
+                    for a in range(D):
+                        M[:, s * C + a + 1, ch] += g[a]
+        self.point_data = np.ascontiguousarray((self.weights[:, None, None] * M).reshape(P, -1))
+        return self.form, self.point_data
+
+    def tables(self):
+        """Per side: full basis values (P x N_s) and physical gradients (D, P, N_s) (cached)."""
+        if getattr(self, "_tables", None) is None:
+            D = self.dim
+            P = self.n_points_total
+            out = []
+            for s, degs in enumerate((self.degrees, self.plus_degrees)):
+                vals, ders = [], []
+                for a in range(D):
+                    v, d = _basis_at(self.ref[:, s * D + a], degs[a])
+                    vals.append(v)
+                    ders.append(d)
+                phi = np.ones((P, 1))
+                for a in range(D):
+                    phi = (vals[a][:, :, None] * phi[:, None, :]).reshape(P, -1)
+                jinv = self.inverse_jacobian[self.item_of_point, s * D:(s + 1) * D]
+                grads = []
+                for ax in range(D):
+                    g = np.ones((P, 1))
+                    for a in range(D):
+                        f = ders[a] if a == ax else vals[a]
+                        g = (f[:, :, None] * g[:, None, :]).reshape(P, -1)
+                    grads.append(g * jinv[:, ax:ax + 1])
+                out.append((phi, np.stack(grads)))
+            self._tables = out
+        return self._tables
+
+    def project_sides(self, channels):
+        """[(v, g) minus, (v, g) plus] point channels (weights included) ->
+        (items, N_minus + N_plus) local vector (host)."""
+        out = np.zeros((self.cell_count, self.n_modes))
+        for s, ((phi, grads), (v, g)) in enumerate(zip(self.tables(), channels)):
+            r = phi * np.asarray(v)[:, None]
+            for a in range(self.dim):
+                r = r + grads[a] * np.asarray(g[a])[:, None]
+            lo = 0 if s == 0 else self.n_minus
+            block = np.zeros((self.cell_count, r.shape[1]))
+            np.add.at(block, self.item_of_point, r)
+            out[:, lo:lo + r.shape[1]] += block
+        return out
+
+
 class AssemblyPlan:
     """Frozen host tables plus the device plan handle."""
 
@@ -374,10 +573,63 @@ class AssemblyPlan:
                              normals=[sq.normals for _, sq in items])
             sb.lower(sops, surface=True)
             self.surface_buckets.append(sb)
+        self._compile_faces(problem)
         self._compile_dirichlet()
         self._compile_reduced_map()
         self._rhs = None
         self._device = None
+
+    # Boundary-face and interface items (src/integration.jl:2195-2316).
+    def _compile_faces(self, problem):
+        space = self.space
+        D = self.dim
+        self.boundary_buckets = []  # (bucket, bilinear operators, rhs operators)
+        sides = {}
+        for (bf, _, _, op) in getattr(problem, "boundary_operators", []):
+            sides.setdefault((bf.axis, bf.side), ([], []))[0].append(op)
+        for (bf, _, op) in getattr(problem, "boundary_rhs", []):
+            sides.setdefault((bf.axis, bf.side), ([], []))[1].append(op)
+        for (axis, side), (ops, rops) in sorted(sides.items()):
+            sel = np.nonzero((space.boundary_axis == axis) & (space.boundary_side == side))[0]
+            groups = {}
+            for f in sel:
+                li = int(space.boundary_leaf[f])
+                key = (tuple(space.degrees[li]), tuple(space.quadrature_shapes[li]))
+                groups.setdefault(key, []).append(li)
+            for (deg, qs), lis in groups.items():
+                rule = boundary_face_rule(qs, axis, side)
+                normal = np.zeros((rule.point_count(), D))
+                normal[:, axis] = -1.0 if side == 0 else 1.0
+                bk = CloudBucket(space, lis, deg, [rule] * len(lis), normals=[normal] * len(lis))
+                bk.face_axis, bk.face_side = axis, side
+                if ops:
+                    bk.lower(ops, callback="boundary_accumulate")
+                self.boundary_buckets.append((bk, ops, rops))
+        self.interface_buckets = []
+        iops = [op for (_, _, op) in getattr(problem, "interface_operators", [])]
+        irops = [op for (_, op) in getattr(problem, "interface_rhs", [])]
+        if (iops or irops) and len(space.interface_minus):
+            mi, pi, ax = space.interface_minus, space.interface_plus, space.interface_axis
+            qs = space.quadrature_shapes
+            # Conforming interfaces (both sides span the same face) form their
+            # own batches: their points are one tensor grid on both sides.
+            cm, hm = space.domain.cell_centers_and_half_sizes(space.leaves[mi])
+            cp, hp = space.domain.cell_centers_and_half_sizes(space.leaves[pi])
+            same = np.isclose(cm - hm, cp - hp, rtol=0, atol=1e-14) & np.isclose(cm + hm, cp + hp, rtol=0, atol=1e-14)
+            same[np.arange(len(ax)), ax] = True
+            conforming = np.all(same, axis=1).astype(np.int64)
+            key = np.concatenate([ax[:, None], space.degrees[mi], space.degrees[pi],
+                                  np.maximum(qs[mi], qs[pi]), conforming[:, None]], axis=1)
+            uniq, inv = np.unique(key, axis=0, return_inverse=True)
+            for g, k in enumerate(uniq):
+                sel = np.nonzero(inv.reshape(-1) == g)[0]
+                axis = int(k[0])
+                tshape = tuple(int(v) for v in k[1 + 2 * D:1 + 3 * D])
+                ib = InterfaceBucket(space, mi[sel], pi[sel], axis, k[1:1 + D], k[1 + D:1 + 2 * D], tshape)
+                if iops:
+                    ib.lower(iops)
+                self.interface_buckets.append(ib)
+        self.interface_ops, self.interface_rhs_ops = iops, irops
 
     # Dirichlet data (src/plans.jl:486-514, 884-972).
     def _compile_dirichlet(self):
@@ -447,7 +699,11 @@ class AssemblyPlan:
     # Host-side right-hand side (src/assembly.jl:488-507 rhs!).
     @property
     def all_buckets(self):
-        return list(self.buckets) + list(self.cloud_buckets) + list(self.surface_buckets)
+        """Every batch the device applies: tensor cells, point-cloud cells,
+        embedded surfaces, boundary faces and interfaces with an operator."""
+        faces = [bk for (bk, ops, _) in self.boundary_buckets if ops]
+        faces += list(self.interface_buckets) if self.interface_ops else []
+        return list(self.buckets) + list(self.cloud_buckets) + list(self.surface_buckets) + faces
 
     def rhs_physical(self):
         """Host right-hand side: cell loads on tensor and cut cells, surface
@@ -469,6 +725,24 @@ class AssemblyPlan:
                                 local = local + bk.project(bk.weights * g[a], derivative_axis=a)
                         contrib = bk.term_coef * local[bk.term_cell, bk.term_box]
                         b += np.bincount(bk.term_dof, weights=contrib, minlength=self.full_dof_count)
+            # Boundary-face loads (src/accumulators.jl:1230-1274) and interface
+            # loads (1666-1698): both traces projected.
+            for bk, _, rops in self.boundary_buckets:
+                for op in rops:
+                    v, g = _channels(op.boundary_rhs_accumulate(bk.quadrature(), 0), (bk.n_points_total,), bk.dim)
+                    local = bk.project(bk.weights * v)
+                    for a in range(bk.dim):
+                        if np.any(g[a] != 0):
+                            local = local + bk.project(bk.weights * g[a], derivative_axis=a)
+                    b += np.bincount(bk.term_dof, weights=bk.term_coef * local[bk.term_cell, bk.term_box],
+                                     minlength=self.full_dof_count)
+            for bk in self.interface_buckets:
+                for op in self.interface_rhs_ops:
+                    ch = bk.trace_channels(op.interface_rhs_accumulate(bk.quadrature(), 0))
+                    ch = [(bk.weights * v, [bk.weights * ga for ga in g]) for v, g in ch]
+                    local = bk.project_sides(ch)
+                    b += np.bincount(bk.term_dof, weights=bk.term_coef * local[bk.term_cell, bk.term_box],
+                                     minlength=self.full_dof_count)
             self._rhs = b
         return self._rhs
 
@@ -511,7 +785,9 @@ class AssemblyPlan:
 
     @property
     def tensor_only(self):
-        return not self.cloud_buckets and not self.surface_buckets
+        """True when every operator and load lives on tensor-product cells
+        (the device right-hand side covers exactly those)."""
+        return not (self.cloud_buckets or self.surface_buckets or self.boundary_buckets or self.interface_buckets)
 
     def rhs(self):
         b = self.rhs_physical().copy()

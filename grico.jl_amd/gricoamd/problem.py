@@ -11,6 +11,12 @@ Operators are plain Python objects that implement some of:
 
     cell_accumulate(self, q, trial, test_component) -> TestChannels or value
     cell_rhs_accumulate(self, q, test_component)    -> TestChannels or value
+    boundary_accumulate(self, q, trial, test_component) / boundary_rhs_accumulate
+    surface_accumulate(self, q, trial, test_component) / surface_rhs_accumulate
+    interface_accumulate(self, q, trace_trial, test_component) -> TraceTestChannels
+    interface_rhs_accumulate(self, q, test_component)           -> TraceTestChannels
+
+(src/accumulators.jl:396-520 registration, 559-641 callback contracts).
 
 `q.point` is a tuple of D coordinate arrays, `q.weight` the weights,
 `q.cell_size` a tuple of D arrays; `trial.value` is an array and
@@ -52,6 +58,40 @@ class TestChannels:
         self.gradient = tuple(gradient)
 
 
+class TracePair:
+    """Oriented (minus, plus) pair (src/accumulators.jl:153-166): `jump` is
+    plus - minus, `average` the arithmetic mean."""
+
+    def __init__(self, minus, plus):
+        self.minus = minus
+        self.plus = plus
+
+
+class TraceTrialChannels:
+    """Two-sided trial data of an interface callback (src/accumulators.jl:168-180)."""
+
+    def __init__(self, component, minus, plus):
+        self.component = component
+        self.minus = minus
+        self.plus = plus
+
+    @property
+    def value(self):
+        return TracePair(self.minus.value, self.plus.value)
+
+    @property
+    def gradient(self):
+        return TracePair(tuple(self.minus.gradient), tuple(self.plus.gradient))
+
+
+class TraceTestChannels:
+    """Coefficients of the minus and plus test traces (src/accumulators.jl:196-209)."""
+
+    def __init__(self, minus, plus):
+        self.minus = minus
+        self.plus = plus
+
+
 def value(channels):
     return channels.value
 
@@ -60,15 +100,77 @@ def gradient(channels):
     return channels.gradient
 
 
+def component(channels):
+    return channels.component
+
+
+def minus(pair):
+    return pair.minus
+
+
+def plus(pair):
+    return pair.plus
+
+
+def jump(a, b=None):
+    """plus - minus of a TracePair, or jump(minus, plus) (src/integration.jl:375-380)."""
+    if b is None:
+        a, b = a.minus, a.plus
+    if isinstance(a, tuple):
+        return tuple(y - x for x, y in zip(a, b))
+    return b - a
+
+
+def average(a, b=None):
+    if b is None:
+        a, b = a.minus, a.plus
+    if isinstance(a, tuple):
+        return tuple(0.5 * (x + y) for x, y in zip(a, b))
+    return 0.5 * (a + b)
+
+
+def inner(a, b):
+    if isinstance(a, TracePair):
+        return TracePair(inner(a.minus, b.minus if isinstance(b, TracePair) else b),
+                         inner(a.plus, b.plus if isinstance(b, TracePair) else b))
+    if isinstance(a, tuple):
+        out = a[0] * b[0]
+        for x, y in zip(a[1:], b[1:]):
+            out = out + x * y
+        return out
+    return a * b
+
+
+def normal_component(vector, normal):
+    """Contraction with the normal, side by side for a TracePair (src/accumulators.jl:295-302)."""
+    if isinstance(vector, TracePair):
+        return TracePair(normal_component(vector.minus, normal), normal_component(vector.plus, normal))
+    return inner(tuple(vector), tuple(normal))
+
+
+def normal(q):
+    return q.normal
+
+
+def weight(q):
+    return q.weight
+
+
+def cell_size(q, *_field):
+    return q.cell_size
+
+
 class QuadraturePoints:
     """Vectorized stand-in for the reference's per-point handle (surface
     points also carry the physical unit `normal`)."""
 
-    def __init__(self, points, weights, cell_sizes, normal=None):
+    def __init__(self, points, weights, cell_sizes, normal=None, face_axis=None, face_side=None):
         self.point = tuple(points)
         self.weight = weights
         self.cell_size = tuple(cell_sizes)
         self.normal = None if normal is None else tuple(normal)
+        self.face_axis = face_axis
+        self.face_side = face_side
 
     def coordinate(self, axis):
         return self.point[axis]
@@ -139,6 +241,41 @@ class AffineProblem:
         self.surface_quadratures = []  # SurfaceQuadrature
         self.surface_operators = []    # (test_field, trial_field, operator)
         self.surface_rhs = []          # (test_field, operator)
+        self.boundary_operators = []   # (BoundaryFace, test_field, trial_field, operator)
+        self.boundary_rhs = []         # (BoundaryFace, test_field, operator)
+        self.interface_operators = []  # (test_field, trial_field, operator)
+        self.interface_rhs = []        # (test_field, operator)
+
+    def _fields(self, args, kind):
+        if len(args) == 3:
+            test, trial, op = args
+            if test is not self.field or trial is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            return (test, trial, op), True
+        if len(args) == 2:
+            test, op = args
+            if test is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            return (test, op), False
+        raise TypeError(f"add_{kind}_accumulator(..., test, trial, op) or (..., test, op)")
+
+    def add_boundary_accumulator(self, boundary, *args):
+        """boundary_accumulate / boundary_rhs_accumulate operators on one
+        boundary side (src/accumulators.jl:396-421)."""
+        if not isinstance(boundary, BoundaryFace):
+            raise TypeError("add_boundary_accumulator expects a BoundaryFace")
+        if not 0 <= boundary.axis < self.field.space.dim or boundary.side not in (LOWER, UPPER):
+            raise ValueError("boundary face axis or side out of range")
+        entry, bilinear = self._fields(args, "boundary")
+        (self.boundary_operators if bilinear else self.boundary_rhs).append((boundary,) + entry)
+        return self
+
+    def add_interface_accumulator(self, *args):
+        """interface_accumulate / interface_rhs_accumulate operators on every
+        interior interface (src/accumulators.jl:488-520)."""
+        entry, bilinear = self._fields(args, "interface")
+        (self.interface_operators if bilinear else self.interface_rhs).append(entry)
+        return self
 
     def add_cell_quadrature(self, leaf, quadrature):
         """Replace the tensor rule of one leaf (src/embedded.jl:203-208)."""
@@ -211,3 +348,11 @@ def add_surface_quadrature(problem, surface):
 
 def add_surface_accumulator(problem, *args):
     return problem.add_surface_accumulator(*args)
+
+
+def add_boundary_accumulator(problem, boundary, *args):
+    return problem.add_boundary_accumulator(boundary, *args)
+
+
+def add_interface_accumulator(problem, *args):
+    return problem.add_interface_accumulator(*args)

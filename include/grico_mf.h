@@ -89,6 +89,18 @@ typedef struct gmf_bucket_desc {
    * then ignored and the basis is evaluated at the points on the device. */
   const int64_t* point_offsets;
   const double* ref_points;
+  /* Two-sided interface items (src/integration.jl:187-207 InterfaceValues,
+   * src/accumulators.jl:1576-1616 interface_apply!): sides = 2 on a
+   * point-cloud batch. Item c couples a minus leaf (modes[]) and a plus leaf
+   * (plus_modes[]); its local_index row holds the minus box followed by the
+   * plus box (N = prod modes + prod plus_modes), ref_points rows hold the
+   * minus then the plus reference coordinates (total x 2 dim),
+   * inverse_jacobian rows the minus then the plus diagonal (cell_count x
+   * 2 dim), and point_data rows are (2(D+1))^2 row-major over the channels
+   * s(D+1) + c (s = 0 minus, 1 plus; c = 0 value, 1 + a physical derivative).
+   * sides = 0 or 1: one-sided items (plus_modes ignored). */
+  int32_t sides;
+  int32_t plus_modes[3];
 } gmf_bucket_desc;
 
 /* Compiled plan: cell batches + algebraic constraint rows + reduced map

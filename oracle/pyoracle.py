@@ -130,19 +130,94 @@ def _test_channels(result, shape, D):
     return np.broadcast_to(np.asarray(result, dtype=np.float64), shape), [np.zeros(shape)] * D
 
 
+_CALLBACK = {False: "cell_accumulate", True: "surface_accumulate", "boundary": "boundary_accumulate"}
+
+
 def _cloud_groups(plan):
-    """(bucket, operators, surface) for every point-cloud batch of the plan."""
+    """(bucket, operators, kind) for every point-cloud batch of the plan; kind
+    False = cells, True = embedded surface, "boundary" = boundary faces,
+    "interface" = interfaces."""
     cells = [op for (_, _, op) in plan.problem.cell_operators]
     surf = [op for (_, _, op) in getattr(plan.problem, "surface_operators", [])]
+    iops = [op for (_, _, op) in getattr(plan.problem, "interface_operators", [])]
     return ([(b, cells, False) for b in getattr(plan, "cloud_buckets", [])] +
-            [(b, surf, True) for b in getattr(plan, "surface_buckets", [])])
+            [(b, surf, True) for b in getattr(plan, "surface_buckets", [])] +
+            [(b, ops, "boundary") for (b, ops, _) in getattr(plan, "boundary_buckets", []) if ops] +
+            ([(b, iops, "interface") for b in getattr(plan, "interface_buckets", [])] if iops else []))
+
+
+def _gather_local(bk, x):
+    loc = np.zeros((bk.cell_count, bk.n_modes))
+    np.add.at(loc, (bk.term_cell, bk.term_box), bk.term_coef * x[bk.term_dof])
+    return loc
+
+
+def _scatter_local(bk, rloc, y, fixed, squared=False):
+    c = bk.term_coef ** 2 if squared else bk.term_coef
+    contrib = c * rloc[bk.term_cell, bk.term_box]
+    keep = np.ones(len(contrib), dtype=bool) if fixed is None else fixed[bk.term_dof] == 0
+    np.add.at(y, bk.term_dof[keep], contrib[keep])
+
+
+def _interface_apply(bk, ops, x, y, fixed):
+    """Interface items (src/accumulators.jl:1576-1616 interface_apply!): the
+    two-sided trial trace from the local coefficients (minus box, then plus
+    box), the callback on the actual trace channels, each returned side
+    projected onto that side's modes, scatter skipping fixed rows. Host numpy."""
+    from gricoamd.problem import TraceTrialChannels
+    D = bk.dim
+    loc = _gather_local(bk, x)
+    up = loc[bk.item_of_point]
+    trial = []
+    for s, (phi, grads) in enumerate(bk.tables()):
+        lo = 0 if s == 0 else bk.n_minus
+        u = up[:, lo:lo + phi.shape[1]]
+        trial.append(_trial(np.einsum("pn,pn->p", phi, u), [np.einsum("pn,pn->p", grads[a], u) for a in range(D)]))
+    q = bk.quadrature()
+    acc = [(np.zeros(bk.n_points_total), [np.zeros(bk.n_points_total) for _ in range(D)]) for _ in range(2)]
+    for op in ops:
+        res = op.interface_accumulate(q, TraceTrialChannels(0, trial[0], trial[1]), 0)
+        for s, (v, g) in enumerate(bk.trace_channels(res)):
+            acc[s][0][:] += bk.weights * v
+            for a in range(D):
+                acc[s][1][a][:] += bk.weights * g[a]
+    _scatter_local(bk, bk.project_sides(acc), y, fixed)
+
+
+def _interface_diagonal(bk, ops, d):
+    """src/accumulators.jl:1618-1664 interface_diagonal!: each trace basis
+    function as the trial (the other side zero, 723-736), its own side's test
+    contribution, scattered with squared coefficients."""
+    from gricoamd.problem import TraceTrialChannels
+    D = bk.dim
+    P = bk.n_points_total
+    q = bk.quadrature()
+    zeros = np.zeros(P)
+    rloc = np.zeros((bk.cell_count, bk.n_modes))
+    for s, (phi, grads) in enumerate(bk.tables()):
+        lo = 0 if s == 0 else bk.n_minus
+        for i in range(phi.shape[1]):
+            active = _trial(phi[:, i], [grads[a][:, i] for a in range(D)])
+            inactive = _trial(zeros, [zeros] * D)
+            tr = TraceTrialChannels(0, active, inactive) if s == 0 else TraceTrialChannels(0, inactive, active)
+            contrib = np.zeros(P)
+            for op in ops:
+                v, g = bk.trace_channels(op.interface_accumulate(q, tr, 0))[s]
+                c = v * phi[:, i]
+                for a in range(D):
+                    c = c + g[a] * grads[a][:, i]
+                contrib += bk.weights * c
+            np.add.at(rloc[:, lo + i], bk.item_of_point, contrib)
+    _scatter_local(bk, rloc, d, None, squared=True)
 
 
 def _cloud_apply(bk, ops, surface, x, y, fixed):
     """Basis path of the reference (src/accumulators.jl:1022-1054, one-sided
-    surface items 1133-1161): gather, full tables at the points, the callback
-    on the actual trial channels, weighted projection, scatter skipping fixed
-    rows. Host numpy."""
+    surface and boundary-face items 1133-1161, 1202-1214): gather, full tables
+    at the points, the callback on the actual trial channels, weighted
+    projection, scatter skipping fixed rows. Host numpy."""
+    if surface == "interface":
+        return _interface_apply(bk, ops, x, y, fixed)
     D = bk.dim
     phi, grads = bk.tables()
     loc = np.zeros((bk.cell_count, bk.n_modes))
@@ -155,7 +230,7 @@ def _cloud_apply(bk, ops, surface, x, y, fixed):
     a0 = np.zeros(P)
     ag = np.zeros((D, P))
     for op in ops:
-        call = op.surface_accumulate if surface else op.cell_accumulate
+        call = getattr(op, _CALLBACK[surface])
         v, g = _test_channels(call(q, _trial(val, grad), 0), (P,), D)
         a0 += bk.weights * v
         for a in range(D):
@@ -172,6 +247,8 @@ def _cloud_apply(bk, ops, surface, x, y, fixed):
 
 def _cloud_diagonal(bk, ops, surface, d):
     """Local diagonal sum_q sum_{c,k} W M_q[c][k] chi_c chi_k scattered with c^2."""
+    if surface == "interface":
+        return _interface_diagonal(bk, ops, d)
     D = bk.dim
     phi, grads = bk.tables()
     q = bk.quadrature()
@@ -179,7 +256,7 @@ def _cloud_diagonal(bk, ops, surface, d):
     ones, zeros = np.ones(P), np.zeros(P)
     M = np.zeros((P, D + 1, D + 1))
     for op in ops:
-        call = op.surface_accumulate if surface else op.cell_accumulate
+        call = getattr(op, _CALLBACK[surface])
         for j in range(D + 1):
             v, g = _test_channels(call(q, _trial(ones if j == 0 else zeros,
                                                  [ones if j == a + 1 else zeros for a in range(D)]), 0), (P,), D)

@@ -263,3 +263,151 @@ def adaptive_hp(base=4, levels=3, pmax=8):
         for side in (g.LOWER, g.UPPER):
             problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), corner_exact))
     return problem
+
+
+# ---- symmetric interior penalty DG (benchmark/sine_poisson_sipg.jl) ----------------
+
+def _penalty(q, degree, eta):
+    """sigma = eta (p + 1)^2 / h with h the cell size across the face (the
+    interface's smaller cell, src/integration.jl:2311-2313)."""
+    return eta * (degree + 1) ** 2 / q.cell_size[q.face_axis]
+
+
+class SIPGInterface:
+    """-{d_n u}[v] - {d_n v}[u] + sigma [u][v] with [w] = w+ - w- (the reference's
+    jump orientation, src/integration.jl:375); the form is invariant to the
+    orientation of [.]."""
+
+    def __init__(self, degree, eta=10.0):
+        self.degree, self.eta = degree, eta
+
+    def interface_accumulate(self, q, trial, test_component):
+        n = q.normal
+        jmp = g.jump(trial.value)
+        avg = g.average(g.normal_component(trial.gradient, n))
+        sigma = _penalty(q, self.degree, self.eta)
+        shared = tuple(0.5 * jmp * c for c in n)
+        return g.TraceTestChannels(g.TestChannels(-avg - sigma * jmp, shared),
+                                   g.TestChannels(avg + sigma * jmp, shared))
+
+
+class SIPGBoundary:
+    """Weak Dirichlet (Nitsche) on a boundary face: -d_n u v - d_n v u + sigma u v."""
+
+    def __init__(self, degree, eta=10.0):
+        self.degree, self.eta = degree, eta
+
+    def boundary_accumulate(self, q, trial, test_component):
+        n = q.normal
+        dn = g.normal_component(trial.gradient, n)
+        sigma = _penalty(q, self.degree, self.eta)
+        return g.TestChannels(-dn + sigma * trial.value, tuple(-trial.value * c for c in n))
+
+
+class SIPGBoundaryLoad:
+    """g (sigma v - d_n v) for Dirichlet data g."""
+
+    def __init__(self, degree, data, eta=10.0):
+        self.degree, self.data, self.eta = degree, data, eta
+
+    def boundary_rhs_accumulate(self, q, test_component):
+        gv = self.data(q.point)
+        sigma = _penalty(q, self.degree, self.eta)
+        return g.TestChannels(sigma * gv, tuple(-gv * c for c in q.normal))
+
+
+class ConstantLoad:
+    def __init__(self, value):
+        self.value = value
+
+    def cell_rhs_accumulate(self, q, test_component):
+        return self.value + 0.0 * q.point[0]
+
+
+def quadratic_exact(p):
+    """u = x^2 + 2 y^2 - x y (+ z^2 - y z in 3D); -Laplace u = -6 (2D) / -8 (3D)."""
+    out = p[0] ** 2 + 2.0 * p[1] ** 2 - p[0] * p[1]
+    if len(p) > 2:
+        out = out + p[2] ** 2 - p[1] * p[2]
+    return out
+
+
+def _refined_grid(dim, cells, refine):
+    """[0,1]^dim with `cells` roots per axis and (cell, axis) bisections; cell -1
+    is the first child of the previous bisection."""
+    domain = g.Domain((0.0,) * dim, (1.0,) * dim, (cells,) * dim)
+    last = None
+    for cell, axis in refine:
+        last = domain.refine(last if cell == -1 else cell, axis)
+    return domain
+
+
+def sipg_poisson(dim=2, cells=3, degree=2, eta=10.0, exact=None, source=None, refine=(), degrees=None,
+                 rhs=True):
+    """DG Poisson with SIPG interfaces and Nitsche boundaries on [0,1]^dim.
+    `refine` lists (cell, axis) bisections applied to the root grid (hanging
+    interfaces); `degrees(domain, leaves)` overrides the uniform degree."""
+    domain = _refined_grid(dim, cells, refine)
+    deg = g.UniformDegree(degree)
+    if degrees is not None:
+        deg = g.ByLeafDegrees(degrees)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=deg, continuity="dg"))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    pmax = int(np.max(space.degrees))
+    problem.add_cell_accumulator(u, u, Diffusion())
+    problem.add_interface_accumulator(u, u, SIPGInterface(pmax, eta))
+    exact = exact or quadratic_exact
+    for axis in range(dim):
+        for side in (g.LOWER, g.UPPER):
+            bf = g.BoundaryFace(axis, side)
+            problem.add_boundary_accumulator(bf, u, u, SIPGBoundary(pmax, eta))
+            if rhs:
+                problem.add_boundary_accumulator(bf, u, SIPGBoundaryLoad(pmax, exact, eta))
+    if rhs:
+        problem.add_cell_accumulator(u, ConstantLoad(-6.0 if dim == 2 else -8.0) if source is None else source)
+    return problem
+
+
+class OneSidedAffine:
+    """Nonsymmetric one-sided form of the reference's accumulator tests
+    (test/accumulators.jl:70-87): a0 = m u + t d_n u, a_grad = s u n + k grad u."""
+
+    def __init__(self, m, t, s, k):
+        self.m, self.t, self.s, self.k = m, t, s, k
+
+    def boundary_accumulate(self, q, trial, test_component):
+        n = q.normal
+        dn = g.normal_component(trial.gradient, n)
+        return g.TestChannels(self.m * trial.value + self.t * dn,
+                              tuple(self.s * trial.value * c + self.k * gc for c, gc in zip(n, trial.gradient)))
+
+
+class InterfaceAffine:
+    """Nonsymmetric interface form (test/accumulators.jl:105-113)."""
+
+    def __init__(self, penalty, trial_normal, test_normal):
+        self.penalty, self.trial_normal, self.test_normal = penalty, trial_normal, test_normal
+
+    def interface_accumulate(self, q, trial, test_component):
+        n = q.normal
+        jmp = g.jump(trial.value)
+        avg = g.average(g.normal_component(trial.gradient, n))
+        tn = tuple(0.5 * self.test_normal * jmp * c for c in n)
+        v = self.penalty * jmp + self.trial_normal * avg
+        return g.TraceTestChannels(g.TestChannels(-v, tn), g.TestChannels(v, tn))
+
+
+class GradientJumpPenalty:
+    """Continuous interior penalty gamma h^2 [d_n u][d_n v] (acts on CG spaces too)."""
+
+    def __init__(self, gamma=0.1):
+        self.gamma = gamma
+
+    def interface_accumulate(self, q, trial, test_component):
+        n = q.normal
+        h = q.cell_size[q.face_axis]
+        jn = g.jump(g.normal_component(trial.gradient, n))
+        c = self.gamma * h * h * jn
+        return g.TraceTestChannels(g.TestChannels(0.0 * jn, tuple(-c * a for a in n)),
+                                   g.TestChannels(0.0 * jn, tuple(c * a for a in n)))
