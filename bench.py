@@ -731,6 +731,8 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive hp leg")
     ap.add_argument("--no-lognormal", action="store_true", help="skip the log-normal GMG-CG leg")
     ap.add_argument("--no-sipg", action="store_true", help="skip the interior penalty DG (face kernels) leg")
+    ap.add_argument("--sipg-only", action="store_true",
+                    help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
     ap.add_argument("--learned", action="store_true",
                     help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
                          "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
@@ -751,6 +753,9 @@ def main():
     from gricoamd import _devabi, _native, ops
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.sipg_only:
+        print(json.dumps({"sipg": sipg_leg()}), flush=True)
+        return
     t0 = time.perf_counter()
     problem = build_problem(args.depth)
     plan = g.compile(problem)

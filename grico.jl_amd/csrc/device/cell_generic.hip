@@ -281,6 +281,30 @@ __global__ void __launch_bounds__(256) cell_l2_generic(GenericArgs a, const doub
   }
 }
 
+// Value and physical gradient of u at every point of a cell (the state
+// channels of src/accumulators.jl:752-787 _tensor_state_channels):
+// out[(cell Q + q) channels + c], c = 0 the value, 1 + a the derivative along a
+// (J^-1 scaled); channels = 1 or 1 + D.
+__global__ void __launch_bounds__(256) cell_interp_generic(GenericArgs a, const double* __restrict__ u, int channels,
+                                                           double* __restrict__ out) {
+  extern __shared__ double smem[];
+  const int64_t cell = blockIdx.x;
+  double* uc = smem;
+  double* bufA = uc + a.M;
+  double* bufB = bufA + a.M;
+  double* f = bufB + a.M;  // Qtot
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) uc[i] = gather_one(a, cell, i, u);
+  __syncthreads();
+  double* O = out + cell * int64_t(a.Qtot) * channels;
+  const double* jinv = a.jinv + cell * a.D;
+  for (int c = 0; c < channels; ++c) {
+    interpolate(a, uc, f, bufA, bufB, c - 1);
+    const double scale = c == 0 ? 1.0 : jinv[c - 1];
+    for (int q = threadIdx.x; q < a.Qtot; q += blockDim.x) O[int64_t(q) * channels + c] = f[q] * scale;
+    __syncthreads();
+  }
+}
+
 // out[k] = sum_c partial[2 c + k] in a fixed order (one workgroup).
 __global__ void __launch_bounds__(256) k_sum_pairs(const double* __restrict__ partial, int64_t cells,
                                                    double* __restrict__ out) {
@@ -535,6 +559,16 @@ void launch_cell_l2(const Bucket& b, const double* u, const double* w, const dou
   const size_t lds = sizeof(double) * size_t(3 * a.M + a.Qtot + 2 * threads);
   require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
   hipLaunchKernelGGL(cell_l2_generic, dim3(unsigned(b.cells)), dim3(threads), lds, stream, a, u, w, e, partial);
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cell_interp(const Bucket& b, const double* u, int channels, double* out, hipStream_t stream) {
+  if (b.cells == 0) return;
+  GenericArgs a = make_args(b, false);
+  require(channels == 1 || channels == a.D + 1, GMF_EINVAL, "state channels must be 1 or 1 + dimension");
+  const size_t lds = sizeof(double) * size_t(3 * a.M + a.Qtot);
+  require(lds <= 64 * 1024, GMF_EUNSUPPORTED, "cell batch exceeds the generic kernel's LDS budget");
+  hipLaunchKernelGGL(cell_interp_generic, dim3(unsigned(b.cells)), dim3(64), lds, stream, a, u, channels, out);
   GMF_HIP(hipGetLastError());
 }
 

@@ -206,6 +206,9 @@ void launch_cell_rhs(const Bucket& b, const double* loads, int channels, double*
 // Per-cell verification sums (w (u_h - e)^2, w e^2) into partial (2 per cell).
 void launch_cell_l2(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
                     hipStream_t stream);
+// Value (+ physical gradient) of full-layout u at every point of one tensor
+// bucket: cells x Q x channels (channels 1 or 1 + D).
+void launch_cell_interp(const Bucket& b, const double* u, int channels, double* out, hipStream_t stream);
 // out[0..1] = column sums of partial (cells x 2) in a fixed order.
 void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream);
 // Autotune the generic kernel: for every generic-kernel bucket time 64, 128

@@ -896,6 +896,41 @@ gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t chan
   });
 }
 
+gmf_status gmf_interpolate_channels(gmf_workspace* ws, const double* u_full, int32_t channels, double* out) {
+  return guarded([&] {
+    require(ws && u_full && out, GMF_EINVAL, "null argument");
+    const gmf_plan& p = *ws->plan;
+    for (const Bucket& b : p.buckets)
+      require(!b.cloud, GMF_EUNSUPPORTED, "point-channel interpolation covers tensor-product cells only");
+    require_device_pointer(u_full, "u_full");
+    require_device_pointer(out, "out");
+    int64_t off = 0;
+    for (const Bucket& b : p.buckets) {
+      launch_cell_interp(b, u_full, channels, out + off, ws->stream);
+      off += b.cells * int64_t(b.Qtot) * channels;
+    }
+    sync(ws);
+  });
+}
+
+gmf_status gmf_plan_set_point_data(gmf_plan* plan, int32_t bucket, const double* point_data) {
+  return guarded([&] {
+    require(plan && point_data, GMF_EINVAL, "null argument");
+    require(bucket >= 0 && size_t(bucket) < plan->buckets.size(), GMF_EINVAL, "bucket index out of range");
+    Bucket& b = plan->buckets[size_t(bucket)];
+    require(!b.cloud, GMF_EUNSUPPORTED, "point data updates cover tensor-product cells only");
+    if (b.cells == 0) return;
+    const size_t n = size_t(b.cells) * size_t(b.Qtot) * size_t(b.point_stride);
+    for (size_t i = 0; i < n; ++i)
+      require(std::isfinite(point_data[i]), GMF_EINVAL, "non-finite point data");
+    GMF_HIP(hipMemcpy(b.pdata, point_data, n * sizeof(double), hipMemcpyHostToDevice));
+    if (b.pdata32) {
+      narrow(b.pdata32, b.pdata, int64_t(n), nullptr);
+      GMF_HIP(hipDeviceSynchronize());
+    }
+  });
+}
+
 gmf_status gmf_reduced_rhs(gmf_workspace* ws, const double* rhs_full, double* b_red) {
   return guarded([&] {
     require(ws && rhs_full && b_red, GMF_EINVAL, "null argument");

@@ -137,6 +137,9 @@ class Bucket:
         ones = np.ones(shape)
         M = np.zeros(shape + (D + 1, D + 1))
         for op in operators:
+            window = getattr(op, "set_window", None)
+            if window is not None:  # operators holding per-point data of the whole bucket
+                window(lo, hi)
             base_v, base_g = _channels(op.cell_accumulate(q, TrialChannels(0, zeros, (zeros,) * D), 0),
                                        shape, D)
             if np.any(base_v != 0) or any(np.any(g != 0) for g in base_g):

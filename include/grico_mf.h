@@ -195,6 +195,21 @@ gmf_status gmf_reduced_apply_f32(gmf_workspace* ws, const float* x_red, float* y
  * lowered to point tables. Point-cloud items are not covered (GMF_EUNSUPPORTED).
  * Device pointers; synchronous. */
 gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t channels, double* y);
+/* State channels at the points of the plan's tensor-product cells (bucket
+ * order, cells x Q x channels): the value of the full-layout u and, for
+ * channels = 1 + D, its physical gradient - the StateChannels handed to
+ * cell_residual_accumulate / cell_tangent_accumulate
+ * (src/accumulators.jl:752-787 _tensor_state_channels, 2015-2148 residual and
+ * tangent tensor kernels). With gmf_rhs_physical (residual loads) and
+ * gmf_plan_set_point_data (tangent re-lowered at the state) it carries the
+ * matrix-free residual! / tangent_apply! (src/assembly.jl:997-1080).
+ * Point-cloud items are not covered (GMF_EUNSUPPORTED). Device pointers. */
+gmf_status gmf_interpolate_channels(gmf_workspace* ws, const double* u_full, int32_t channels, double* out);
+/* Replace the lowered point action of tensor bucket `bucket` (host array of
+ * the plan descriptor's point_data shape and form): the tangent of a residual
+ * problem lowered at a new state (src/assembly.jl:1050-1080 tangent_apply!).
+ * Not concurrent with calls that use the plan. */
+gmf_status gmf_plan_set_point_data(gmf_plan* plan, int32_t bucket, const double* point_data);
 /* b_red = P^T (rhs_full - A_phys shift) (src/assembly.jl:917-933 _reduced_rhs!). */
 gmf_status gmf_reduced_rhs(gmf_workspace* ws, const double* rhs_full, double* b_red);
 /* Verification sums over the tensor-product cells' points (src/verification.jl:201-230

@@ -389,6 +389,94 @@ def diagonal_physical(plan):
     return d
 
 
+def _state_channel_arrays(L, bk, u):
+    ob = _Bucket(bk)
+    nc, nq, D = bk.cell_count, bk.n_points, bk.dim
+    vals = np.zeros((nc, nq))
+    grads = np.zeros((nc, nq, D))
+    L.oracle_interpolate(ob.ref(), _p(np.ascontiguousarray(u, dtype=np.float64), ctypes.c_double),
+                         _p(vals, ctypes.c_double), _p(grads, ctypes.c_double))
+    return ob, vals, [grads[..., a] for a in range(D)]
+
+
+def _project_channels(L, ob, bk, results, fixed, y):
+    D = bk.dim
+    shape = (bk.cell_count, bk.n_points)
+    a0 = np.zeros(shape)
+    ag = np.zeros(shape + (D,))
+    for res in results:
+        v, g = _test_channels(res, shape, D)
+        a0 += bk.weights * v
+        for a in range(D):
+            ag[..., a] += bk.weights * g[a]
+    L.oracle_project(ob.ref(), _p(np.ascontiguousarray(a0), ctypes.c_double),
+                     _p(np.ascontiguousarray(ag), ctypes.c_double), _p(fixed, ctypes.c_uint8),
+                     _p(y, ctypes.c_double))
+
+
+def residual(rplan, u_full):
+    """Reference residual! (src/assembly.jl:997-1026 with the tensor residual
+    kernel src/accumulators.jl:2015-2060): state channels interpolated from the
+    gathered coefficients, the residual callback on them, weighted projection,
+    scatter skipping fixed rows, then the constraint rows u[d] - g_d
+    (1922-1940). Full layout."""
+    from gricoamd.nonlinear import StateChannels
+    L = lib()
+    plan = rplan.plan
+    y = np.zeros(plan.full_dof_count)
+    fixed = _fixed_mask(plan)
+    for bk in plan.buckets:
+        ob, v, g = _state_channel_arrays(L, bk, u_full)
+        q = bk.quadrature()
+        st = StateChannels(v, g)
+        _project_channels(L, ob, bk, [op.cell_residual_accumulate(q, st, 0)
+                                      for (_, _, op) in rplan.problem.cell_operators], fixed, y)
+    y[plan.fixed_dofs] = np.asarray(u_full)[plan.fixed_dofs] - plan.fixed_values
+    return y
+
+
+def tangent_apply(rplan, u_full, increment):
+    """Reference tangent_apply! (src/assembly.jl:1050-1080, tensor tangent
+    kernel src/accumulators.jl:2062-2148): state and increment channels at the
+    points, the tangent callback on the actual increment channels, weighted
+    projection, scatter skipping fixed rows; fixed rows copy the increment."""
+    from gricoamd.nonlinear import StateChannels
+    L = lib()
+    plan = rplan.plan
+    y = np.zeros(plan.full_dof_count)
+    fixed = _fixed_mask(plan)
+    for bk in plan.buckets:
+        ob, v, g = _state_channel_arrays(L, bk, u_full)
+        _, dv, dg = _state_channel_arrays(L, bk, increment)
+        q = bk.quadrature()
+        st = StateChannels(v, g)
+        _project_channels(L, ob, bk, [op.cell_tangent_accumulate(q, st, _trial(dv, dg), 0)
+                                      for (_, _, op) in rplan.problem.cell_operators], fixed, y)
+    y[plan.fixed_dofs] = np.asarray(increment)[plan.fixed_dofs]
+    return y
+
+
+def newton_dense(rplan, u0=None, rtol=1e-10, atol=1e-10, maxiter=20):
+    """Newton on the free dofs with the dense tangent from oracle tangent
+    applies of unit vectors (small plans): u_free -= J^-1 R(u)."""
+    plan = rplan.plan
+    n = plan.reduced_dof_count
+    u = plan.shift.copy() if u0 is None else np.array(u0, dtype=np.float64)
+    norms = []
+    for _ in range(maxiter + 1):
+        r = residual(rplan, u)[plan.solve_dofs]
+        norms.append(float(np.linalg.norm(r)))
+        if norms[-1] <= max(atol, rtol * norms[0]):
+            return u, norms
+        J = np.zeros((n, n))
+        for j in range(n):
+            e = np.zeros(plan.full_dof_count)
+            e[plan.solve_dofs[j]] = 1.0
+            J[:, j] = tangent_apply(rplan, u, e)[plan.solve_dofs]
+        u[plan.solve_dofs] += np.linalg.solve(J, -r)
+    raise RuntimeError("oracle Newton did not converge")
+
+
 def reduced_apply(plan, x_red):
     """P^T A P x with the selection map of fixed-row plans."""
     full = np.zeros(plan.full_dof_count)

@@ -411,3 +411,55 @@ class GradientJumpPenalty:
         c = self.gamma * h * h * jn
         return g.TraceTestChannels(g.TestChannels(0.0 * jn, tuple(-c * a for a in n)),
                                    g.TestChannels(0.0 * jn, tuple(c * a for a in n)))
+
+
+# ---- nonlinear residual problems (examples/nonlinear_bratu/driver.jl) -------------------
+
+class Bratu:
+    """-Laplace u + lam exp(u) = f: residual (lam e^u - f, grad u), tangent
+    (lam e^u du, grad du) (examples/nonlinear_bratu/driver.jl:38-52)."""
+
+    def __init__(self, lam=1.0, source=None):
+        self.lam, self.source = lam, source
+
+    def cell_residual_accumulate(self, q, state, test_component):
+        f = 0.0 if self.source is None else self.source(q.point)
+        return g.TestChannels(self.lam * np.exp(state.value) - f, tuple(state.gradient))
+
+    def cell_tangent_accumulate(self, q, state, increment, test_component):
+        return g.TestChannels(self.lam * np.exp(state.value) * increment.value, tuple(increment.gradient))
+
+
+def bratu_exact(amplitude):
+    def exact(p):
+        out = amplitude
+        for x in p:
+            out = out * np.sin(np.pi * x)
+        return out
+    return exact
+
+
+def bratu_source(lam, amplitude, dim=2):
+    exact = bratu_exact(amplitude)
+
+    def source(p):
+        e = exact(p)
+        return dim * np.pi ** 2 * e + lam * np.exp(e)
+    return source
+
+
+def bratu_problem(root_counts=(4, 4), degree=3, lam=1.0, amplitude=0.2, refine=()):
+    from gricoamd.nonlinear import ResidualProblem
+    dim = len(root_counts)
+    domain = g.Domain((0.0,) * dim, (1.0,) * dim, root_counts)
+    last = None
+    for cell, axis in refine:
+        last = domain.refine(last if cell == -1 else cell, axis)
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = ResidualProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Bratu(lam, bratu_source(lam, amplitude, dim)))
+    for axis in range(dim):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    return problem
