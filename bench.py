@@ -477,6 +477,12 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
 
     t_all = timed(plan)
     t_if = timed(iplan)
+    # Face kernels alone (HIP events around the interface-only plan's cell pass).
+    import ctypes
+    cell_ms, total_ms = ctypes.c_double(), ctypes.c_double()
+    _devabi.check(lib.gmf_apply_timed(ops._ws(iplan).ws, x.data_ptr(), y.data_ptr(), reps,
+                                      ctypes.byref(cell_ms), ctypes.byref(total_ms)))
+    t_if_kernels = cell_ms.value * 1e-3
     # The same interface items on the point-cloud kernel (no tensor detection).
     os.environ["GMF_FACE_TENSOR"] = "0"
     try:
@@ -500,10 +506,11 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
     if_bytes = items * (24 * ntot + 8 * 6) + pts * 8 * nnz
     out.update({"full_dofs": n, "interface_items": items, "interface_points": pts, "action_nonzeros": nnz,
                 "apply_seconds": t_all, "apply_gdofs": n / t_all / 1e9,
-                "interface_apply_seconds": t_if, "interface_algorithmic_bytes": if_bytes,
+                "interface_apply_seconds": t_if, "interface_kernel_seconds": t_if_kernels,
+                "interface_algorithmic_bytes": if_bytes,
                 "interface_apply_seconds_point_cloud_kernel": t_cloud,
                 "interface_path": "face-tensor" if "face-tensor" in ops._ws(iplan).describe() else "point-cloud",
-                "interface_roofline_frac": if_bytes / t_if / (HBM_PEAK_GBS * 1e9)})
+                "interface_roofline_frac": if_bytes / t_if_kernels / (HBM_PEAK_GBS * 1e9)})
     t0 = time.perf_counter()
     state = g.solve(plan, solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()), relative_tolerance=1e-8,
                     maxiter=50000)
@@ -518,6 +525,80 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
     out.update({"oracle_apply_seconds_small": t_cpu, "oracle_small_cells": cpu_cells ** 3,
                 "device_vs_oracle_max_relative_error_small": float(np.abs(yd - yo).max() / np.abs(yo).max())})
     log(f"sipg: {out}")
+    return out
+
+
+class _Bratu:
+    """examples/nonlinear_bratu/driver.jl: residual (lam e^u - f, grad u),
+    tangent (lam e^u du, grad du), u_e = A sin(pi x) sin(pi y)."""
+
+    def __init__(self, lam=1.0, amplitude=0.2):
+        self.lam, self.amplitude = lam, amplitude
+
+    def exact(self, p):
+        import numpy as np
+        return self.amplitude * np.sin(np.pi * p[0]) * np.sin(np.pi * p[1])
+
+    def cell_residual_accumulate(self, q, state, test_component):
+        import numpy as np
+
+        import gricoamd as g
+        e = self.exact(q.point)
+        f = 2.0 * np.pi ** 2 * e + self.lam * np.exp(e)
+        return g.TestChannels(self.lam * np.exp(state.value) - f, tuple(state.gradient))
+
+    def cell_tangent_accumulate(self, q, state, increment, test_component):
+        import numpy as np
+
+        import gricoamd as g
+        return g.TestChannels(self.lam * np.exp(state.value) * increment.value, tuple(increment.gradient))
+
+
+def newton_leg(cells=256, degree=3):
+    """Nonlinear Bratu example (2D, CG p=3, zero Dirichlet) at 256^2 cells:
+    device Newton (residual from host-evaluated point loads projected on the
+    device, tangent re-lowered per step, unpreconditioned device CG as the
+    reference's default tangent solve); the oracle residual on the same mesh
+    timed beside it."""
+    import numpy as np
+
+    import gricoamd as g
+    import pyoracle
+    from gricoamd import nonlinear as NL
+
+    op = _Bratu()
+    domain = g.Domain((0.0, 0.0), (1.0, 1.0), (cells, cells))
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    problem = NL.ResidualProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, op)
+    for axis in range(2):
+        for side in (g.LOWER, g.UPPER):
+            problem.add_constraint(g.Dirichlet(u, g.BoundaryFace(axis, side), 0.0))
+    t0 = time.perf_counter()
+    rplan = NL.ResidualPlan(problem)
+    t_compile = time.perf_counter() - t0
+    NL.newton_solve(NL.ResidualPlan(problem), maxiter=1, relative_tolerance=1.0, absolute_tolerance=1e300)  # warm-up
+    t0 = time.perf_counter()
+    state = NL.newton_solve(rplan, maxiter=10, relative_tolerance=1e-10, absolute_tolerance=1e-10,
+                            linear_relative_tolerance=1e-10)
+    t_newton = time.perf_counter() - t0
+    x = 0.5 * state.coefficients  # away from the solution, where the residual is not round-off
+    t0 = time.perf_counter()
+    r_dev = NL.residual(rplan, x)
+    t_res = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r_or = pyoracle.residual(rplan, x)
+    t_or = time.perf_counter() - t0
+    scale = max(np.abs(r_or).max(), 1e-300)
+    out = {"workload": f"2D Bratu (lambda=1, A=0.2), {cells}^2 cells, p={degree}, zero Dirichlet",
+           "full_dofs": rplan.full_dof_count, "host_compile_seconds": t_compile,
+           "newton_seconds": t_newton, "newton_iterations": state.info["iterations"],
+           "residual_norms": state.info["residual_norms"],
+           "l2_error": g.l2_error(state, op.exact),
+           "device_residual_seconds": t_res, "oracle_residual_seconds": t_or,
+           "residual_max_difference_vs_oracle_scaled": float(np.abs(r_dev - r_or).max() / scale)}
+    log(f"newton: {out}")
     return out
 
 
@@ -731,6 +812,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true", help="skip the adaptive hp leg")
     ap.add_argument("--no-lognormal", action="store_true", help="skip the log-normal GMG-CG leg")
     ap.add_argument("--no-sipg", action="store_true", help="skip the interior penalty DG (face kernels) leg")
+    ap.add_argument("--no-newton", action="store_true", help="skip the nonlinear Bratu Newton leg")
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
     ap.add_argument("--learned", action="store_true",
@@ -848,6 +930,12 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"sipg leg failed: {exc!r}")
             result["sipg"] = {"error": repr(exc)}
+    if not args.no_newton:
+        try:
+            result["newton"] = newton_leg()
+        except Exception as exc:  # keep the headline line
+            log(f"newton leg failed: {exc!r}")
+            result["newton"] = {"error": repr(exc)}
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
     if not args.no_adaptive:

@@ -178,6 +178,7 @@ struct gmf_plan {
   int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
   int fp32_state = 0;       // 0: not prepared, 1: every bucket has FP32 kernels, -1: FP64 with conversion
   std::vector<gmf::GenericBatch> generic_batches;  // one launch per lanes-per-cell tier
+  gmf::GenericBatch face_batch;                    // every face-tensor batch in one launch
   std::vector<void*> allocations;
 };
 
@@ -237,6 +238,11 @@ void launch_cell_cloud_diagonal(const gmf_plan& plan, const Bucket& b, double* d
 void prepare_face_path(gmf_plan& plan, Bucket& b, const gmf_bucket_desc& bd);
 bool launch_face_tensor(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced,
                         hipStream_t stream);
+// All face-tensor batches of the plan in one launch (after every bucket is
+// prepared); launch_face_batch marks the buckets it covered in `done`.
+void prepare_face_batch(gmf_plan& plan);
+void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
+                       std::vector<char>& done);
 // slots: the workspace's brick corner array (plan.brick_slots entries).
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream);

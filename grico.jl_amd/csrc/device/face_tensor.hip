@@ -74,19 +74,16 @@ __device__ inline int face_box(const FaceArgs& a, int s, int k, int t1, int t2) 
   return m[0] + a.N[s][0] * (m[1] + a.N[s][1] * m[2]);
 }
 
-__global__ void __launch_bounds__(kFaceLanes) face_apply_tensor(FaceArgs a, const double* __restrict__ x,
-                                                                double* __restrict__ y) {
-  extern __shared__ double sm[];
-  const int64_t item = blockIdx.x;
+// One item: the whole pipeline on one 64-lane workgroup.
+__device__ void face_item(const FaceArgs& a, int64_t item, double* sm, const double* __restrict__ x,
+                          double* __restrict__ y) {
   const int tid = threadIdx.x;
   const int C = a.C, SC = a.S * a.C, Q = a.Q, NtM = a.NtMax;
   double* tab = sm;
-  double* u = tab + a.table_doubles;
-  double* r = u + a.Ntot;
-  double* U = r + a.Ntot;           // [S][2][NtMax]: psi(+-1) and psi'(+-1) contractions
-  double* R = U + a.S * 2 * NtM;    // same shape, transposed results
-  double* ch = R + a.S * 2 * NtM;   // [SC][Q] point channels
-  double* ob = ch + SC * Q;         // [SC][Q] action output
+  double* u = tab + a.table_doubles;  // gathered coefficients
+  double* U = u + a.Ntot;             // [S][2][NtMax]: psi(+-1) / psi'(+-1) contractions, later their transposes
+  double* ch = U + a.S * 2 * NtM;     // [SC][Q] point channels
+  double* R = U;
   for (int i = tid; i < a.table_doubles; i += kFaceLanes) tab[i] = a.tables[i];
   for (int i = tid; i < a.Ntot; i += kFaceLanes) u[i] = gather_term(a.map, item, i, x);
   __syncthreads();
@@ -138,14 +135,25 @@ __global__ void __launch_bounds__(kFaceLanes) face_apply_tensor(FaceArgs a, cons
   }
   __syncthreads();
 
-  // Point action on the compressed pattern; gradient test channels scaled by J^-1.
+  // Point action on the compressed pattern (rows selected without dynamic
+  // register indexing); gradient test channels scaled by J^-1.
+  constexpr int kMaxChannels = 2 * (kMaxDim + 1);
   for (int q = tid; q < Q; q += kFaceLanes) {
-    for (int c = 0; c < SC; ++c) ob[c * Q + q] = 0.0;
+    double out[kMaxChannels];
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c) out[c] = 0.0;
     const double* M = a.pdata + (item * Q + q) * a.nnz;
-    for (int e = 0; e < a.nnz; ++e) ob[a.pr[e] * Q + q] = fma(M[e], ch[a.pc[e] * Q + q], ob[a.pr[e] * Q + q]);
-    for (int c = 0; c < SC; ++c) {
+    for (int e = 0; e < a.nnz; ++e) {
+      const double v = M[e] * ch[a.pc[e] * Q + q];
+      const int row = a.pr[e];
+#pragma unroll
+      for (int c = 0; c < kMaxChannels; ++c) out[c] += row == c ? v : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kMaxChannels; ++c) {
+      if (c >= SC) break;
       const int s = c / C, cc = c % C;
-      ch[c * Q + q] = cc == 0 ? ob[c * Q + q] : ob[c * Q + q] * jinv[s * a.D + cc - 1];
+      ch[c * Q + q] = cc == 0 ? out[c] : out[c] * jinv[s * a.D + cc - 1];
     }
   }
   __syncthreads();
@@ -194,6 +202,25 @@ __global__ void __launch_bounds__(kFaceLanes) face_apply_tensor(FaceArgs a, cons
   }
 }
 
+__global__ void __launch_bounds__(kFaceLanes) face_apply_tensor(FaceArgs a, const double* __restrict__ x,
+                                                                double* __restrict__ y) {
+  extern __shared__ double sm[];
+  face_item(a, blockIdx.x, sm, x, y);
+}
+
+// Every face-tensor batch of a plan in one launch: workgroup w belongs to
+// batch k with start[k] <= w < start[k + 1].
+__global__ void __launch_bounds__(kFaceLanes) face_apply_tensor_batched(const FaceArgs* __restrict__ args,
+                                                                        const int64_t* __restrict__ start, int count,
+                                                                        const double* __restrict__ x,
+                                                                        double* __restrict__ y) {
+  extern __shared__ double sm[];
+  const int64_t blk = blockIdx.x;
+  int k = 0;
+  while (k + 1 < count && blk >= start[k + 1]) ++k;
+  face_item(args[k], blk - start[k], sm, x, y);
+}
+
 // psi_0..psi_{n-1} and derivatives at x (src/polynomials.jl:403-434; p = 0 -> 1).
 void psi_host(double x, int n, double* v, double* d) {
   if (n == 1) {
@@ -216,9 +243,12 @@ void psi_host(double x, int n, double* v, double* d) {
 }
 
 size_t face_lds(const FaceArgs& a) {
-  return sizeof(double) * (size_t(a.table_doubles) + size_t(2 * a.Ntot) + size_t(4 * a.S * a.NtMax) +
-                           size_t(2 * a.S * a.C * a.Q));
+  return sizeof(double) * (size_t(a.table_doubles) + size_t(a.Ntot) + size_t(2 * a.S * a.NtMax) +
+                           size_t(a.S * a.C * a.Q));
 }
+
+// Batches below this many items share one launch; larger ones fill the chip alone.
+constexpr int64_t kFaceBatchMaxItems = 16384;
 
 FaceArgs make_face_args(const Bucket& b, bool reduced) {
   const FacePath& f = b.face;
@@ -379,6 +409,49 @@ void prepare_face_path(gmf_plan& p, Bucket& b, const gmf_bucket_desc& bd) {
   if (f.lds > 64 * 1024) return;
   f.enabled = true;
   b.face = f;
+}
+
+void prepare_face_batch(gmf_plan& p) {
+  std::vector<int> ids;
+  for (size_t i = 0; i < p.buckets.size(); ++i)
+    if (p.buckets[i].face.enabled && p.buckets[i].cells > 0 && p.buckets[i].cells < kFaceBatchMaxItems)
+      ids.push_back(int(i));
+  p.face_batch = GenericBatch{};
+  if (ids.size() < 2) return;
+  std::vector<FaceArgs> full, red;
+  std::vector<int64_t> start(1, 0);
+  bool have_red = true;
+  size_t lds = 0;
+  for (int i : ids) {
+    const Bucket& b = p.buckets[size_t(i)];
+    full.push_back(make_face_args(b, false));
+    red.push_back(make_face_args(b, true));
+    if (red.back().map.idx == nullptr) have_red = false;
+    start.push_back(start.back() + b.cells);
+    lds = std::max(lds, b.face.lds);
+  }
+  if (start.back() >= (int64_t(1) << 31)) return;
+  GenericBatch g;
+  g.threads = kFaceLanes;
+  g.count = int(ids.size());
+  g.buckets = ids;
+  g.cells = start.back();
+  g.lds = lds;
+  g.args_full = upload_array(p, full.data(), full.size());
+  if (have_red) g.args_red = upload_array(p, red.data(), red.size());
+  g.start = upload_array(p, start.data(), start.size());
+  p.face_batch = std::move(g);
+}
+
+void launch_face_batch(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream,
+                       std::vector<char>& done) {
+  const GenericBatch& g = p.face_batch;
+  const void* args = reduced ? g.args_red : g.args_full;
+  if (g.count == 0 || args == nullptr) return;
+  hipLaunchKernelGGL(face_apply_tensor_batched, dim3(unsigned(g.cells)), dim3(kFaceLanes), g.lds, stream,
+                     static_cast<const FaceArgs*>(args), g.start, g.count, x, y);
+  GMF_HIP(hipGetLastError());
+  for (int i : g.buckets) done[size_t(i)] = 1;
 }
 
 bool launch_face_tensor(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,

@@ -183,9 +183,11 @@ void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   // Generic-kernel buckets of uneven degree share one launch per tier.
   std::vector<char> batched;
   launch_generic_batches(p, x, y, reduced, ws->stream, batched);
+  launch_face_batch(p, x, y, reduced, ws->stream, batched);
   for (size_t i = 0; i < p.buckets.size(); ++i) {
     const Bucket& b = p.buckets[i];
     if (b.cloud) {
+      if (batched[i]) continue;
       if (!launch_face_tensor(p, b, x, y, reduced, ws->stream)) launch_cell_cloud(p, b, x, y, reduced, ws->stream);
       continue;
     }
@@ -761,6 +763,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
       GMF_HIP(hipDeviceSynchronize());
       tune_generic(p);
       prepare_generic_batch(p);
+      prepare_face_batch(p);
       GMF_HIP(hipDeviceSynchronize());
     } catch (...) {
       for (void* a : p.allocations) (void)hipFree(a);

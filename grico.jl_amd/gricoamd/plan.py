@@ -706,7 +706,10 @@ class AssemblyPlan:
         embedded surfaces, boundary faces and interfaces with an operator."""
         faces = [bk for (bk, ops, _) in self.boundary_buckets if ops]
         faces += list(self.interface_buckets) if self.interface_ops else []
-        return list(self.buckets) + list(self.cloud_buckets) + list(self.surface_buckets) + faces
+        cells = list(self.buckets) + list(self.cloud_buckets)
+        if not self.problem.cell_operators and (faces or self.surface_buckets):
+            cells = []  # no cell action: the cell batches would apply zeros
+        return cells + list(self.surface_buckets) + faces
 
     def rhs_physical(self):
         """Host right-hand side: cell loads on tensor and cut cells, surface
