@@ -45,7 +45,25 @@ def _channels(result, shape, dim):
     return v, [np.zeros(shape)] * dim
 
 
-class Bucket:
+class _TermLists:
+    """term_cell / term_box / term_dof / term_coef (one entry per term of every
+    active mode) built on first access from the sources _gather_maps keeps."""
+
+    _TERM_NAMES = ("term_cell", "term_box", "term_dof", "term_coef")
+
+    def __getattr__(self, name):
+        if name in _TermLists._TERM_NAMES and "_terms_src" in self.__dict__:
+            cell_of_mode, box, first, tcount, indices, coefficients = self.__dict__.pop("_terms_src")
+            self.term_cell = np.repeat(cell_of_mode, tcount)
+            self.term_box = np.repeat(box, tcount)
+            sel = _expand_ranges(first, tcount)
+            self.term_dof = indices[sel]
+            self.term_coef = coefficients[sel]
+            return self.__dict__[name]
+        raise AttributeError(name)
+
+
+class Bucket(_TermLists):
     """Cells sharing degree tuple and quadrature shape (one device kernel batch)."""
 
     def __init__(self, space, leaf_idx, degrees, qshape):
@@ -115,69 +133,100 @@ class Bucket:
         self.local_index = np.ascontiguousarray(local_index.astype(np.int32))
         self.local_coefficient = np.ascontiguousarray(local_coef)
         self.unit_coefficients = bool(np.all(local_coef[local_index >= 0] == 1.0))
-        # Expanded term list (cell, box, dof, coefficient) for host-side scatters.
-        terms_per = tcount
-        self.term_cell = np.repeat(cell_of_mode, terms_per)
-        self.term_box = np.repeat(box, terms_per)
-        sel_all = _expand_ranges(space.term_offsets[mode_ids], tcount)
-        self.term_dof = space.term_indices[sel_all]
-        self.term_coef = space.term_coefficients[sel_all]
+        # Expanded term list (cell, box, dof, coefficient) for host-side
+        # scatters, built on first use (_TermLists).
+        self._terms_src = (cell_of_mode, box, space.term_offsets[mode_ids], tcount, space.term_indices,
+                           space.term_coefficients)
 
     def quadrature(self, lo=0, hi=None):
         hi = self.cell_count if hi is None else hi
         return QuadraturePoints([p[lo:hi] for p in self.phys_points], self.weights[lo:hi],
                                 [self.sizes[lo:hi, a:a + 1] for a in range(self.dim)])
 
-    def _probe(self, operators, lo, hi):
-        """(D+1)x(D+1) action per point of cells lo..hi (unweighted)."""
+    def _responses(self, operators, lo, hi, check_linear):
+        """Callback responses to the D+1 unit trial channels on cells lo..hi,
+        summed over the operators: [(v, [g_1..g_D]) per channel] (unweighted)."""
         D = self.dim
         shape = (hi - lo, self.n_points)
         q = self.quadrature(lo, hi)
         zeros = np.zeros(shape)
         ones = np.ones(shape)
-        M = np.zeros(shape + (D + 1, D + 1))
+        out = [(np.zeros(shape), [np.zeros(shape) for _ in range(D)]) for _ in range(D + 1)]
         for op in operators:
             window = getattr(op, "set_window", None)
             if window is not None:  # operators holding per-point data of the whole bucket
                 window(lo, hi)
-            base_v, base_g = _channels(op.cell_accumulate(q, TrialChannels(0, zeros, (zeros,) * D), 0),
-                                       shape, D)
-            if np.any(base_v != 0) or any(np.any(g != 0) for g in base_g):
-                raise ValueError("cell_accumulate must be linear in the trial channels")
+            if check_linear:
+                base_v, base_g = _channels(op.cell_accumulate(q, TrialChannels(0, zeros, (zeros,) * D), 0),
+                                           shape, D)
+                if np.any(base_v != 0) or any(np.any(g != 0) for g in base_g):
+                    raise ValueError("cell_accumulate must be linear in the trial channels")
             for ch in range(D + 1):
                 val = ones if ch == 0 else zeros
                 grad = tuple(ones if ch == a + 1 else zeros for a in range(D))
                 v, g = _channels(op.cell_accumulate(q, TrialChannels(0, val, grad), 0), shape, D)
-                M[..., 0, ch] += v
+                out[ch][0][...] += v
                 for a in range(D):
-                    M[..., a + 1, ch] += g[a]
+                    out[ch][1][a][...] += g[a]
+        return out
+
+    def _probe(self, operators, lo, hi, check_linear=True, resp=None):
+        """(D+1)x(D+1) action per point of cells lo..hi (unweighted)."""
+        D = self.dim
+        resp = resp if resp is not None else self._responses(operators, lo, hi, check_linear)
+        M = np.empty((hi - lo, self.n_points, D + 1, D + 1))
+        for ch, (v, g) in enumerate(resp):
+            M[..., 0, ch] = v
+            for a in range(D):
+                M[..., a + 1, ch] = g[a]
         return M
+
+    @staticmethod
+    def _diffusion_coefficient(resp, D):
+        """c when every point's action is diag(0, c I), else None."""
+        v0, g0 = resp[0]
+        if np.any(v0 != 0) or any(np.any(g != 0) for g in g0):
+            return None
+        c = resp[1][1][0]
+        for a in range(D):
+            v, g = resp[a + 1]
+            if np.any(v != 0):
+                return None
+            for b in range(D):
+                if b == a:
+                    if not np.array_equal(g[b], c):
+                        return None
+                elif np.any(g[b] != 0):
+                    return None
+        return c
 
     def lower_operators(self, operators, chunk=8192):
         """Probe cell_accumulate with unit trial channels; returns (form, point_data).
 
         DIFFUSION when every point's action is diag(0, c I): one coefficient
-        W_q c_q per point; otherwise GENERAL with the full W_q M_q matrix."""
+        W_q c_q per point; otherwise GENERAL with the full W_q M_q matrix. The
+        action matrix is only materialized for the general form; linearity in
+        the trial channels is checked on the first chunk."""
         D = self.dim
         nc = self.cell_count
         coef = np.zeros((nc, self.n_points))
         general = None
         for lo in range(0, nc, chunk):
             hi = min(nc, lo + chunk)
-            M = self._probe(operators, lo, hi)
             W = self.weights[lo:hi]
+            resp = None
             if general is None:
-                grad_block = M[..., 1:, 1:]
-                c = grad_block[..., 0, 0]
-                if (np.all(M[..., 0, :] == 0) and np.all(M[..., :, 0] == 0)
-                        and np.all(grad_block == c[..., None, None] * np.eye(D))):
+                resp = self._responses(operators, lo, hi, check_linear=lo == 0)
+                c = self._diffusion_coefficient(resp, D)
+                if c is not None:
                     coef[lo:hi] = W * c
                     continue
                 general = np.zeros((nc, self.n_points * (D + 1) ** 2))
                 for lo2 in range(0, lo, chunk):
                     hi2 = min(nc, lo2 + chunk)
-                    M2 = self._probe(operators, lo2, hi2)
+                    M2 = self._probe(operators, lo2, hi2, check_linear=False)
                     general[lo2:hi2] = (self.weights[lo2:hi2][..., None, None] * M2).reshape(hi2 - lo2, -1)
+            M = self._probe(operators, lo, hi, check_linear=lo == 0, resp=resp)
             general[lo:hi] = (W[..., None, None] * M).reshape(hi - lo, -1)
         if general is None:
             return FORM_DIFFUSION, np.ascontiguousarray(coef)
@@ -216,7 +265,7 @@ def _basis_at(x, degree):
     return fe_basis_values_and_derivatives(x, degree)
 
 
-class CloudBucket:
+class CloudBucket(_TermLists):
     """Items that carry their own reference points: cells with a custom
     quadrature (src/accumulators.jl:1022-1054) or embedded-surface pieces
     (1133-1214, src/integration.jl:2338-2368). One device batch per degree
@@ -427,7 +476,7 @@ class InterfaceBucket:
         self._gather_maps(space)
 
     def _gather_maps(self, space):
-        class _Side:
+        class _Side(_TermLists):
             pass
 
         sides = []
@@ -651,32 +700,35 @@ class AssemblyPlan:
         self.fixed_values = np.asarray([fixed[d] for d in self.fixed_dofs.tolist()], dtype=np.float64)
 
     def _trace_terms(self, con):
-        """(dof, coef, face_index, box) of every term of a mode with nonzero trace on the face."""
+        """Terms of every mode with nonzero trace on the constraint's boundary
+        side, as arrays (dof, coef, leaf index, box), vectorized over faces."""
         space = self.space
-        sel = np.nonzero((space.boundary_axis == con.boundary.axis) &
-                         (space.boundary_side == con.boundary.side))[0]
-        out = []
-        for f in sel:
-            li = int(space.boundary_leaf[f])
-            deg = space.degrees[li]
-            shape = tuple(int(p) + 1 for p in deg)
-            a = con.boundary.axis
-            for m in range(space.mode_offsets[li], space.mode_offsets[li + 1]):
-                box = int(space.mode_box[m])
-                idx = np.unravel_index(box, shape[::-1])[::-1]
-                if deg[a] >= 1 and idx[a] != con.boundary.side:
-                    continue
-                for t in range(space.term_offsets[m], space.term_offsets[m + 1]):
-                    out.append((int(space.term_indices[t]), float(space.term_coefficients[t]), li, box))
-        return out
+        a, side = con.boundary.axis, con.boundary.side
+        sel = np.nonzero((space.boundary_axis == a) & (space.boundary_side == side))[0]
+        li = space.boundary_leaf[sel].astype(np.int64)
+        lo = space.mode_offsets[li]
+        counts = space.mode_offsets[li + 1] - lo
+        modes = _expand_ranges(lo, counts)
+        leaf = np.repeat(li, counts)
+        box = space.mode_box[modes].astype(np.int64)
+        shape = space.degrees[leaf].astype(np.int64) + 1
+        stride = np.ones(len(modes), dtype=np.int64)
+        for b in range(a):
+            stride = stride * shape[:, b]
+        idx_a = (box // stride) % shape[:, a]
+        keep = (shape[:, a] < 2) | (idx_a == side)
+        modes, leaf, box = modes[keep], leaf[keep], box[keep]
+        tcount = space.term_offsets[modes + 1] - space.term_offsets[modes]
+        terms = _expand_ranges(space.term_offsets[modes], tcount)
+        return (space.term_indices[terms].astype(np.int64), space.term_coefficients[terms],
+                np.repeat(leaf, tcount), np.repeat(box, tcount))
 
     def _dirichlet_trace_dofs(self, cons):
-        dofs = set()
+        dofs = []
         for con in cons:
-            for dof, coef, _, _ in self._trace_terms(con):
-                if coef != 0.0:
-                    dofs.add(dof)
-        return np.asarray(sorted(dofs), dtype=np.int64)
+            dof, coef, _, _ = self._trace_terms(con)
+            dofs.append(dof[coef != 0.0])
+        return np.unique(np.concatenate(dofs)).astype(np.int64) if dofs else np.zeros(0, dtype=np.int64)
 
     def _dirichlet_values(self, cons, dofs):
         if all((not callable(c.data)) and float(c.data) == 0.0 for c in cons):
@@ -835,9 +887,9 @@ def _boundary_l2_projection(plan, cons, dofs):
     dom = space.domain
     tables = {}
     for con in cons:
-        terms = plan._trace_terms(con)
+        t_dof, t_coef, t_leaf, t_box = plan._trace_terms(con)
         by_face = {}
-        for dof, coef, li, box in terms:
+        for dof, coef, li, box in zip(t_dof.tolist(), t_coef.tolist(), t_leaf.tolist(), t_box.tolist()):
             by_face.setdefault(li, []).append((dof, coef, box))
         a = con.boundary.axis
         side = -1.0 if con.boundary.side == 0 else 1.0
