@@ -17,6 +17,7 @@ in C, OpenMP over DOF-disjoint colors) on a bounded 16^3-cell sample.
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -423,8 +424,14 @@ class _SIPGFaces:
 
 
 def build_sipg_problem(cells, degree, interfaces_only=False):
+    """cells = 2^k per axis, built by k uniform refinements of one root (so the
+    multigrid hierarchy has h-levels)."""
     import gricoamd as g
-    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (cells,) * 3)
+    depth = int(round(math.log2(cells)))
+    if 2 ** depth != cells:
+        raise ValueError("the SIPG workload needs a power-of-two cell count per axis")
+    domain = g.Domain((0.0,) * 3, (1.0,) * 3, (1, 1, 1))
+    g.uniform_refine(domain, depth)
     space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree),
                                              continuity="dg"))
     u = g.ScalarField(space, "u")
@@ -442,7 +449,7 @@ def build_sipg_problem(cells, degree, interfaces_only=False):
     return problem
 
 
-def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
+def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
     """Symmetric interior penalty DG Poisson (3D, DG p=3): device apply of the
     whole operator and of the interface items alone (two-sided point-cloud
     kernel), roofline of the interface kernel against its algorithmic bytes,
@@ -516,6 +523,25 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=6):
                     maxiter=50000)
     out.update({"solve_seconds": time.perf_counter() - t0, "solve_iterations": int(state.info.iterations),
                 "solve_relative_l2_error": g.relative_l2_error(state, _SIPGFaces.exact)})
+    # hp-multigrid on the DG space, faces rediscretized on every level.
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+    from gricoamd.solve import solve_reduced
+    t0 = time.perf_counter()
+    policy = GeometricMultigridPreconditioner(smoother="chebyshev")  # damped Jacobi diverges under the penalty
+    h = compile_geometric_multigrid(build_sipg_problem(cells, degree), policy, fine_plan=plan)
+    M = h.device_preconditioner()
+    t_setup = time.perf_counter() - t0
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    solve_reduced(plan, g.CGSolver(preconditioner=M), b, 1e-8, 0.0, 2000)  # warm-up (graph capture)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xg, info = solve_reduced(plan, g.CGSolver(preconditioner=M), b, 1e-8, 0.0, 2000)
+    torch.cuda.synchronize()
+    t_gmg = time.perf_counter() - t0
+    full = ops.expand_reduced(plan, xg, include_shift=True).cpu().numpy()
+    out.update({"gmg_cg_seconds": t_gmg, "gmg_cg_iterations": int(info.iterations),
+                "gmg_setup_seconds": t_setup, "gmg_levels": h.reduced_dof_counts(),
+                "gmg_cg_relative_l2_error": g.relative_l2_error(g.State(plan, full), _SIPGFaces.exact)})
     small = g.compile(build_sipg_problem(cpu_cells, degree))
     xv = np.random.default_rng(5).standard_normal(small.full_dof_count)
     t0 = time.perf_counter()

@@ -52,6 +52,9 @@ class _TermLists:
     _TERM_NAMES = ("term_cell", "term_box", "term_dof", "term_coef")
 
     def __getattr__(self, name):
+        if name in _TermLists._TERM_NAMES and "_terms_fn" in self.__dict__:
+            self.__dict__.pop("_terms_fn")(self)
+            return self.__dict__[name]
         if name in _TermLists._TERM_NAMES and "_terms_src" in self.__dict__:
             cell_of_mode, box, first, tcount, indices, coefficients = self.__dict__.pop("_terms_src")
             self.term_cell = np.repeat(cell_of_mode, tcount)
@@ -414,7 +417,7 @@ def boundary_face_rule(shape, axis, side):
     return PointQuadrature(pts, w)
 
 
-class InterfaceBucket:
+class InterfaceBucket(_TermLists):
     """Interior interfaces between two leaves (src/integration.jl:2262-2316
     _compile_interface, 2800-2841 _interface_face_points): items share the
     face axis, both sides' degree tuples and the tangential point shape. The
@@ -495,10 +498,14 @@ class InterfaceBucket:
         self.multi_offsets = np.concatenate([m.multi_offsets, p.multi_offsets[1:] + m.multi_offsets[-1]])
         self.multi_indices = np.concatenate([m.multi_indices, p.multi_indices]).astype(np.int32)
         self.multi_coefficients = np.concatenate([m.multi_coefficients, p.multi_coefficients])
-        self.term_cell = np.concatenate([m.term_cell, p.term_cell])
-        self.term_box = np.concatenate([m.term_box, p.term_box + self.n_minus])
-        self.term_dof = np.concatenate([m.term_dof, p.term_dof])
-        self.term_coef = np.concatenate([m.term_coef, p.term_coef])
+
+        def terms(obj):
+            obj.term_cell = np.concatenate([m.term_cell, p.term_cell])
+            obj.term_box = np.concatenate([m.term_box, p.term_box + obj.n_minus])
+            obj.term_dof = np.concatenate([m.term_dof, p.term_dof])
+            obj.term_coef = np.concatenate([m.term_coef, p.term_coef])
+
+        self._terms_fn = terms
 
     def quadrature(self):
         return QuadraturePoints(self.phys_points, self.weights,
@@ -518,7 +525,7 @@ class InterfaceBucket:
         P = self.n_points_total
         q = self.quadrature()
         ones, zeros = np.ones(P), np.zeros(P)
-        M = np.zeros((P, 2 * C, 2 * C))
+        Mt = np.zeros((2 * C, 2 * C, P))  # [test channel][trial channel][point]: contiguous fills
         for op in operators:
             for ch in range(2 * C):
                 sides = []
@@ -529,10 +536,11 @@ class InterfaceBucket:
                     sides.append(TrialChannels(0, val, grad))
                 res = op.interface_accumulate(q, TraceTrialChannels(0, sides[0], sides[1]), 0)
                 for s, (v, g) in enumerate(self.trace_channels(res)):
-                    M[:, s * C, ch] += v
+                    Mt[s * C, ch] += v
                     for a in range(D):
-                        M[:, s * C + a + 1, ch] += g[a]
-        self.point_data = np.ascontiguousarray((self.weights[:, None, None] * M).reshape(P, -1))
+                        Mt[s * C + a + 1, ch] += g[a]
+        Mt *= self.weights[None, None, :]
+        self.point_data = np.ascontiguousarray(Mt.transpose(2, 0, 1).reshape(P, -1))
         return self.form, self.point_data
 
     def tables(self):

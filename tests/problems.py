@@ -343,11 +343,17 @@ def _refined_grid(dim, cells, refine):
 
 
 def sipg_poisson(dim=2, cells=3, degree=2, eta=10.0, exact=None, source=None, refine=(), degrees=None,
-                 rhs=True):
+                 rhs=True, refine_depth=0):
     """DG Poisson with SIPG interfaces and Nitsche boundaries on [0,1]^dim.
     `refine` lists (cell, axis) bisections applied to the root grid (hanging
-    interfaces); `degrees(domain, leaves)` overrides the uniform degree."""
-    domain = _refined_grid(dim, cells, refine)
+    interfaces); `degrees(domain, leaves)` overrides the uniform degree;
+    refine_depth > 0 builds the mesh by uniform refinement of one root (h-levels
+    for multigrid)."""
+    if refine_depth:
+        domain = g.Domain((0.0,) * dim, (1.0,) * dim, (1,) * dim)
+        g.uniform_refine(domain, refine_depth)
+    else:
+        domain = _refined_grid(dim, cells, refine)
     deg = g.UniformDegree(degree)
     if degrees is not None:
         deg = g.ByLeafDegrees(degrees)
