@@ -51,7 +51,7 @@ def device_sources():
 def build_device():
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libgrico_mf.so")
-    _run([HIPCC] + HIP_FLAGS + ["-o", out] + device_sources())
+    _run([HIPCC] + HIP_FLAGS + ["-o", out] + device_sources() + ["-lrocsolver", "-lrocblas"])
     return out
 
 

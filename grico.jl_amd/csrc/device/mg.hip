@@ -20,6 +20,8 @@
 #include <vector>
 
 #include "blas.hpp"
+#include <rocsolver/rocsolver.h>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -611,6 +613,70 @@ std::vector<double> lu_inverse(std::vector<double> A, int64_t n) {
   return inv;
 }
 
+// Lower triangle -> upper triangle of a column-major n x n matrix (symmetric result).
+__global__ void k_mirror_lower(double* __restrict__ A, int64_t n) {
+  const int64_t total = n * n;
+  GMF_GRID_STRIDE(k, total) {
+    const int64_t i = k % n, j = k / n;  // entry (i, j) at A[i + j n]
+    if (i < j) A[i + j * n] = A[j + i * n];
+  }
+}
+
+// out[i n + j] = A[i + j n] (column-major -> row-major).
+__global__ void k_transpose(double* __restrict__ out, const double* __restrict__ A, int64_t n) {
+  const int64_t total = n * n;
+  GMF_GRID_STRIDE(k, total) out[(k % n) * n + k / n] = A[k];
+}
+
+// Dense inverse on the device for coarse levels above the reference's direct
+// limit: Cholesky factorization and inverse (rocSOLVER potrf / potri) for SPD
+// operators, LU with partial pivoting (getrf / getri) otherwise. `col` holds
+// A column-major (column j = A e_j) and is overwritten; the result is
+// row-major as k_dense_matvec reads it.
+DeviceBuffer<double> device_dense_inverse(gmf_workspace* ws, int64_t n, bool spd, DeviceBuffer<double>& col) {
+  hipStream_t s = ws->stream;
+  rocblas_handle h = nullptr;
+  require(rocblas_create_handle(&h) == rocblas_status_success, GMF_EDEVICE, "rocblas_create_handle failed");
+  struct Guard {
+    rocblas_handle h;
+    ~Guard() { (void)rocblas_destroy_handle(h); }
+  } guard{h};
+  require(rocblas_set_stream(h, s) == rocblas_status_success, GMF_EDEVICE, "rocblas_set_stream failed");
+  DeviceBuffer<rocblas_int> info(1);
+  rocblas_int hinfo = 0;
+  const rocblas_int m = rocblas_int(n);
+  if (spd) {
+    require(rocsolver_dpotrf(h, rocblas_fill_lower, m, col.p, m, info.p) == rocblas_status_success, GMF_EDEVICE,
+            "rocsolver_dpotrf failed");
+    GMF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, s));
+    sync(ws);
+    require(hinfo == 0, GMF_ENOTSPD, "declared SPD coarse multigrid matrix is not positive definite");
+    require(rocsolver_dpotri(h, rocblas_fill_lower, m, col.p, m, info.p) == rocblas_status_success, GMF_EDEVICE,
+            "rocsolver_dpotri failed");
+    hipLaunchKernelGGL(k_mirror_lower, dim3(grid1(n * n)), dim3(256), 0, s, col.p, n);
+    GMF_HIP(hipGetLastError());
+    sync(ws);
+    return std::move(col);
+  }
+  DeviceBuffer<rocblas_int> piv(static_cast<size_t>(n));
+  require(rocsolver_dgetrf(h, m, m, col.p, m, piv.p, info.p) == rocblas_status_success, GMF_EDEVICE,
+          "rocsolver_dgetrf failed");
+  GMF_HIP(hipMemcpyAsync(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost, s));
+  sync(ws);
+  require(hinfo == 0, GMF_EINVAL, "coarse multigrid matrix is singular");
+  require(rocsolver_dgetri(h, m, col.p, m, piv.p, info.p) == rocblas_status_success, GMF_EDEVICE,
+          "rocsolver_dgetri failed");
+  DeviceBuffer<double> inv(static_cast<size_t>(n * n));
+  hipLaunchKernelGGL(k_transpose, dim3(grid1(n * n)), dim3(256), 0, s, inv.p, col.p, n);
+  GMF_HIP(hipGetLastError());
+  sync(ws);
+  return inv;
+}
+
+// The reference's dense direct-solve limit (src/multigrid.jl:19
+// _DEFAULT_COARSE_DIRECT_DOF_LIMIT); larger coarse levels are inverted on the device.
+constexpr int64_t kHostInverseLimit = 512;
+
 // Dense coarse inverse from the probed SPD coarse matrix (src/multigrid.jl:382-424).
 DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n, bool spd) {
   hipStream_t s = ws->stream;
@@ -629,15 +695,18 @@ DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n, bool spd
     require(std::isfinite(v), spd ? GMF_ENOTSPD : GMF_EINVAL, "coarse multigrid matrix is not finite");
     scale = std::max(scale, std::fabs(v));
   }
+  if (spd) {
+    const double allowed = std::sqrt(std::numeric_limits<double>::epsilon()) * scale;
+    for (int64_t c = 0; c < n; ++c)
+      for (int64_t r = c + 1; r < n; ++r)
+        require(std::fabs(At[size_t(r * n + c)] - At[size_t(c * n + r)]) <= allowed, GMF_ENOTSPD,
+                "declared SPD coarse multigrid matrix is not numerically symmetric");
+  }
+  if (n > kHostInverseLimit) return device_dense_inverse(ws, n, spd, col);
   std::vector<double> A(size_t(n * n));
   for (int64_t i = 0; i < n; ++i)
     for (int64_t j = 0; j < n; ++j) A[size_t(i * n + j)] = At[size_t(j * n + i)];
   if (!spd) return upload_vec(lu_inverse(A, n));
-  const double allowed = std::sqrt(std::numeric_limits<double>::epsilon()) * scale;
-  for (int64_t c = 0; c < n; ++c)
-    for (int64_t r = c + 1; r < n; ++r)
-      require(std::fabs(A[size_t(r * n + c)] - A[size_t(c * n + r)]) <= allowed, GMF_ENOTSPD,
-              "declared SPD coarse multigrid matrix is not numerically symmetric");
   // Cholesky A = L L^T on the lower triangle.
   std::vector<double> Lf(A);
   for (int64_t j = 0; j < n; ++j) {

@@ -360,14 +360,24 @@ class MultigridHierarchy:
 
     def with_policy(self, policy):
         """The same level plans and transfers under another smoother, step
-        count or precision (the coarsening parameters must agree)."""
-        for name in ("min_degree", "max_levels", "p_sequence"):
+        count, precision or coarse-solve limit. A smaller max_levels keeps the
+        finest levels (the hierarchy is generated from the fine end, so this is
+        the hierarchy that max_levels builds); the other coarsening parameters
+        must agree."""
+        for name in ("min_degree", "p_sequence"):
             if getattr(policy, name) != getattr(self.policy, name):
                 raise ValueError(f"with_policy keeps the hierarchy: {name} must agree")
+        if policy.max_levels > self.policy.max_levels and len(self.plans) == self.policy.max_levels:
+            raise ValueError("with_policy cannot add levels: max_levels must not grow")
         smoother = policy.resolved_smoother(self.problem.operator_class)
         _check_smoother(policy, smoother)
         other = object.__new__(MultigridHierarchy)
         other.__dict__.update(self.__dict__)
+        keep = min(len(self.plans), policy.max_levels)
+        if keep < len(self.plans):
+            other.spaces = self.spaces[-keep:]
+            other.plans = self.plans[-keep:]
+            other.transfers = self.transfers[len(self.plans) - keep:]
         other.policy = policy
         other.smoother = smoother
         other._device = None

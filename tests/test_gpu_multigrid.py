@@ -301,3 +301,35 @@ def test_fgmres_gmg_nonsymmetric_solve(gpu):
     jac = g.solve(g.compile(pr), solver=g.FGMRESSolver(preconditioner=g.JacobiPreconditioner()),
                   relative_tolerance=1e-10)
     assert st.info.iterations < jac.info.iterations
+
+
+def test_device_dense_coarse_inverse_spd(gpu):
+    """A coarse level above the reference's 512-dof direct limit is inverted on
+    the device (Cholesky factorization and inverse): V-cycle against the
+    oracle's dense V-cycle (exact coarse solve)."""
+    problem = P.poisson(2, None, 2, refine_depth=5, rhs=False)
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", max_levels=2, coarse_direct_dof_limit=2000)
+    h = compile_geometric_multigrid(problem, pol)
+    assert h.reduced_dof_counts()[0] > 512 and h.coarse_solver_kind() == "dense_cholesky"
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 31)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r, smoother="chebyshev", pre=pol.pre_smoothing_steps,
+                          post=pol.post_smoothing_steps, lambdas=h.level_lambda_max(),
+                          cheb_degree=pol.chebyshev_degree, cheb_range=pol.chebyshev_range)
+    assert np.abs(z - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_device_dense_coarse_inverse_lu(gpu):
+    """Nonsymmetric coarse level above 512 dofs: device LU inverse."""
+    pol = GeometricMultigridPreconditioner(max_levels=2, coarse_direct_dof_limit=2000)
+    h = compile_geometric_multigrid(_advection(32, 2), pol)
+    assert h.reduced_dof_counts()[0] > 512
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 32)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r, smoother="gmres", pre=pol.pre_smoothing_steps,
+                          post=pol.post_smoothing_steps, omega=pol.smoother_damping, restart=pol.smoother_restart)
+    assert np.abs(z - ref).max() <= 1e-10 * np.abs(ref).max()
