@@ -737,6 +737,15 @@ def lognormal_leg(repeats=3):
     out["gmg_cg_fp32_vcycle"] = timed(lambda: cg(h32.device_preconditioner().handle))
     x32 = x.clone()
     out["fgmres_vcycle"] = timed(lambda: fgmres(M64.handle))
+    # Hierarchy cut at the first level with at most 4096 dofs, solved there by
+    # a device-computed dense inverse (the smaller levels are launch-latency bound).
+    counts = h64.reduced_dof_counts()
+    keep = len(counts) - max(i for i, c in enumerate(counts) if c <= 4096)
+    for tag, prec in (("gmg_cg_dense_coarse", "fp64"), ("gmg_cg_fp32_vcycle_dense_coarse", "fp32")):
+        ht = h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", precision=prec, max_levels=keep,
+                                                              coarse_direct_dof_limit=4096))
+        Mt = ht.device_preconditioner()
+        out[tag] = dict(timed(lambda: cg(Mt.handle)), levels=ht.reduced_dof_counts())
     from gricoamd.solve import compile_preconditioner
     J = compile_preconditioner(g.JacobiPreconditioner(), plan)
     out["jacobi_cg"] = timed(lambda: cg(J.handle))
