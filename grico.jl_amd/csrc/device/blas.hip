@@ -108,6 +108,14 @@ void read_scalars(gmf_workspace* ws, int count) {
   GMF_HIP(hipStreamSynchronize(ws->stream));
 }
 
+void read_scalars_async(gmf_workspace* ws, int count) {
+  GMF_HIP(hipMemcpyAsync(ws->pipelined_scalars, ws->scalars, sizeof(double) * size_t(count), hipMemcpyDeviceToHost,
+                         ws->stream));
+  GMF_HIP(hipEventRecord(ws->scalar_event, ws->stream));
+}
+
+void wait_scalars(gmf_workspace* ws) { GMF_HIP(hipEventSynchronize(ws->scalar_event)); }
+
 // ---- Krylov building blocks ------------------------------------------------
 
 // x += alpha p; r -= alpha q; partial sums of r.r. alpha = s[num] / s[den].

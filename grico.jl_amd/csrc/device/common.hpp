@@ -190,6 +190,8 @@ struct gmf_workspace {
   double* partials = nullptr;   // reduction partial sums
   double* scalars = nullptr;    // device scalars for solvers
   double* host_scalars = nullptr;  // pinned mirror
+  double* pipelined_scalars = nullptr;  // pinned mirror of the CG loop's overlapped reads
+  hipEvent_t scalar_event = nullptr;    // marks the arrival of pipelined_scalars
   int64_t partial_capacity = 0;
   double* brick_slots = nullptr;   // plan->brick_slots corner results of the brick kernels
   double* red64_in = nullptr;      // FP64 staging of FP32 reduced applies on plans without FP32 kernels
@@ -280,5 +282,9 @@ void widen(double* dst, const float* src, int64_t n, hipStream_t s);
 // partial sums of x.y into ws->partials then final reduce into scalars[slot].
 void dot(gmf_workspace* ws, const double* x, const double* y, int64_t n, int slot);
 void read_scalars(gmf_workspace* ws, int count);
+// Queue a copy of the first `count` device scalars into ws->pipelined_scalars
+// (a buffer no other reader uses) and mark it; wait_scalars blocks until it arrived.
+void read_scalars_async(gmf_workspace* ws, int count);
+void wait_scalars(gmf_workspace* ws);
 
 }  // namespace gmf

@@ -370,24 +370,33 @@ void cg_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x
     inf.converged = 1;
     return;
   }
+  const double* ps = ws->pipelined_scalars;
   for (int64_t it = 1; it <= maxiter; ++it) {
     reduced_apply(ws, v.p.p, v.q.p);
     inf.operator_applications++;
     dot(ws, v.p.p, v.q.p, n, kPq);
     cg_update(ws, x, v.r.p, v.p.p, v.q.p, n, kRz, kPq, kRr);
-    read_scalars(ws, kTmp);
+    read_scalars_async(ws, kTmp);
+    // The next preconditioner application and search direction write only z,
+    // p and their scalars, so they are queued before the host looks at this
+    // iteration's residual: the device keeps working during the check. x and
+    // r, hence the result and the iteration count, are those of the
+    // unpipelined loop.
+    precond_apply_dot(ws, M, v.r.p, v.z.p, n, kRzNew);
+    direction(ws, v.p.p, v.z.p, n, kRzNew, kRz);
+    wait_scalars(ws);
     inf.iterations = it;
-    require(hs[kPq] > 0.0, GMF_EBREAKDOWN,
+    require(ps[kPq] > 0.0, GMF_EBREAKDOWN,
             "CG encountered a non-positive operator direction; provide a different linear_solve for this problem");
-    inf.final_residual_norm = std::sqrt(hs[kRr]);
-    if (hs[kRr] <= tol * tol) {
+    inf.final_residual_norm = std::sqrt(ps[kRr]);
+    if (ps[kRr] <= tol * tol) {
       inf.converged = 1;
+      sync(ws);
       return;
     }
-    precond_apply_dot(ws, M, v.r.p, v.z.p, n, kRzNew);
     inf.preconditioner_applications++;
-    direction(ws, v.p.p, v.z.p, n, kRzNew, kRz);
   }
+  sync(ws);
   throw Error(GMF_ENOCONV, "CG did not converge in " + std::to_string(maxiter) + " iterations");
 }
 
@@ -835,6 +844,8 @@ gmf_status gmf_workspace_create(gmf_plan* plan, gmf_workspace** out) {
     ws->scalars = dalloc<double>(size_t(kScalarCount));
     if (plan->brick_slots > 0) ws->brick_slots = dalloc<double>(size_t(plan->brick_slots));
     GMF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_scalars), sizeof(double) * kScalarCount));
+    GMF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->pipelined_scalars), sizeof(double) * kScalarCount));
+    GMF_HIP(hipEventCreateWithFlags(&ws->scalar_event, hipEventDisableTiming));
     *out = ws.release();
   });
 }
@@ -845,6 +856,8 @@ gmf_status gmf_workspace_destroy(gmf_workspace* ws) {
     for (double* v : {ws->full_in, ws->full_out, ws->partials, ws->scalars, ws->brick_slots, ws->red64_in, ws->red64_out})
       if (v) (void)hipFree(v);
     if (ws->host_scalars) (void)hipHostFree(ws->host_scalars);
+    if (ws->pipelined_scalars) (void)hipHostFree(ws->pipelined_scalars);
+    if (ws->scalar_event) (void)hipEventDestroy(ws->scalar_event);
     delete ws;
   });
 }
