@@ -606,10 +606,15 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   b.h_colloc = collocation_derivative(B, G, n);
   b.fast = true;
   // Small batches (coarse multigrid levels) take the one-launch per-cell
-  // kernel: eight colored brick launches of a few bricks each cost more than
-  // the atomics they avoid. GMF_BRICK_MIN_CELLS overrides the threshold.
-  int64_t min_cells = 32768;
+  // kernel: four colored brick launches and the corner pass cost more than the
+  // atomics they avoid. For p = 1 (8 vertex dofs per cell) that holds up to a
+  // few million cells (measured on 262,144-cell levels: V-cycles 3-10% faster).
+  // GMF_BRICK_MIN_CELLS overrides the threshold (GMF_BRICK_MIN_CELLS_N2 for
+  // p = 1 batches only).
+  int64_t min_cells = n == 2 ? (int64_t(1) << 22) : 32768;
   if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
+  if (n == 2)
+    if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS_N2")) min_cells = std::strtoll(env, nullptr, 10);
   if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
 }
 
