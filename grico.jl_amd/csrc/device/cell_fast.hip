@@ -35,6 +35,7 @@
 // one XCD (workgroups are dealt to the 8 XCDs round-robin).
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 #include <type_traits>
 #include <cmath>
 #include <vector>
@@ -609,10 +610,12 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   // kernel: four colored brick launches and the corner pass cost more than the
   // atomics they avoid. For p = 1 (8 vertex dofs per cell) that holds up to a
   // few million cells (measured on 262,144-cell levels: V-cycles 3-10% faster).
-  // GMF_BRICK_MIN_CELLS overrides the threshold (GMF_BRICK_MIN_CELLS_N2 for
-  // p = 1 batches only).
+  // GMF_BRICK_MIN_CELLS overrides the threshold, GMF_BRICK_MIN_CELLS_N<n> for
+  // batches of n modes per axis only.
   int64_t min_cells = n == 2 ? (int64_t(1) << 22) : 32768;
   if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
+  const std::string per_n = "GMF_BRICK_MIN_CELLS_N" + std::to_string(n);
+  if (const char* env = std::getenv(per_n.c_str())) min_cells = std::strtoll(env, nullptr, 10);
   if (n == 2)
     if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS_N2")) min_cells = std::strtoll(env, nullptr, 10);
   if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
