@@ -303,9 +303,14 @@ def adaptive_leg(repeats=3, steps=20):
            "buckets": [list(bk.degrees) + [bk.cell_count] for bk in plan.buckets],
            "apply_gdofs": n / apply_s / 1e9, "apply_seconds": apply_s,
            "levels": h64.reduced_dof_counts(), "host_compile_seconds": t_plan, "hierarchy_seconds": t_h}
+    counts = h64.reduced_dof_counts()
+    keep = len(counts) - max(i for i, c in enumerate(counts) if c <= 4096)
+    dense = h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32", max_levels=keep,
+                                                             coarse_direct_dof_limit=4096))
+    out["dense_coarse_levels"] = dense.reduced_dof_counts()
     for tag, h in (("gmg_cg_fp32_vcycle", h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev",
                                                                                             precision="fp32"))),
-                   ("gmg_cg", h64)):
+                   ("gmg_cg", h64), ("gmg_cg_fp32_vcycle_dense_coarse", dense)):
         M = h.device_preconditioner()
         run(M.handle, 2000)
         times = []

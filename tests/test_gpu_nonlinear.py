@@ -91,3 +91,37 @@ def test_newton_rejects_preconditioner(gpu):
     rplan = NL.ResidualPlan(P.bratu_problem((2, 2), 2))
     with pytest.raises(ValueError, match="preconditioning"):
         NL.newton_solve(rplan, preconditioner=g.JacobiPreconditioner())
+
+
+def test_entry_point_errors(gpu):
+    """Invalid arguments of the residual-support entry points come back as
+    GMF_EINVAL / GMF_EUNSUPPORTED (the reference's ArgumentError class)."""
+    import ctypes
+
+    import torch
+    from gricoamd import _devabi, _native, ops
+
+    lib = _native.device()
+    rplan = NL.ResidualPlan(P.bratu_problem((2, 2), 2))
+    dev = ops._ws(rplan.plan)
+    u = torch.zeros(rplan.full_dof_count, dtype=torch.float64, device="cuda")
+    npts = sum(b.cell_count * b.n_points for b in rplan.plan.buckets)
+    out = torch.empty(npts * 3, dtype=torch.float64, device="cuda")
+    assert lib.gmf_interpolate_channels(dev.ws, u.data_ptr(), 2, out.data_ptr()) == -1  # GMF_EINVAL
+    data = np.ascontiguousarray(rplan.plan.buckets[0].point_data)
+    ptr = data.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert lib.gmf_plan_set_point_data(dev.handle, 7, ptr) == -1  # GMF_EINVAL
+    bad = data.copy()
+    bad.reshape(-1)[3] = np.nan
+    assert lib.gmf_plan_set_point_data(dev.handle, 0, bad.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) \
+        == -1
+    assert lib.gmf_plan_set_point_data(dev.handle, 0, ptr) == 0
+    # point-cloud items are outside the state interpolation
+    from test_faces_host import _BoundaryMass, _dg
+    f = _dg(1, 2, 1)
+    pr = g.AffineProblem(f, g.SPD())
+    pr.add_boundary_accumulator(g.BoundaryFace(0, g.UPPER), f, f, _BoundaryMass())
+    plan = g.compile(pr)
+    fd = ops._ws(plan)
+    v = torch.zeros(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    assert lib.gmf_interpolate_channels(fd.ws, v.data_ptr(), 1, out.data_ptr()) == -8  # GMF_EUNSUPPORTED
