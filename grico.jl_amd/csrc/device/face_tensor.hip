@@ -1,9 +1,12 @@
-// Sum-factorized kernel for conforming boundary faces and interfaces.
+// Sum-factorized kernel for boundary faces and interfaces whose points form
+// one tangential tensor grid per side.
 //
-// A boundary face or an interface between two leaves of equal extent carries
-// the Gauss points of its tangential rule at the normal coordinate +-1 of each
-// side (src/integration.jl:2230-2260 _compile_boundary_face, 2279-2316
-// _compile_interface). Since psi_k(+-1) = 0 for k >= 2 (integrated Legendre,
+// A boundary face or an interface carries the Gauss points of its tangential
+// rule on the shared face patch at the normal coordinate +-1 of each side
+// (src/integration.jl:2230-2260 _compile_boundary_face, 2279-2316
+// _compile_interface, 2800-2841 _interface_face_points); on a hanging
+// interface the larger side sees them mapped into its sub-face, so batches of
+// equal sub-face position share one grid per side. Since psi_k(+-1) = 0 for k >= 2 (integrated Legendre,
 // src/polynomials.jl:403-434), the trace of a side is a contraction of its
 // mode box along the normal axis with psi(+-1) (value) and psi'(+-1) (normal
 // derivative), followed by tensor interpolation over the tangential axes; the
@@ -317,8 +320,10 @@ void prepare_face_path(gmf_plan& p, Bucket& b, const gmf_bucket_desc& bd) {
   f.axis = axis;
   for (int a = 0, j = 0; a < D; ++a)
     if (a != axis) f.T[j++] = a;
-  // Tangential tensor grid (first tangential axis fastest), identical for every
-  // item and both sides up to rounding of the mapped coordinates.
+  // Tangential tensor grid (first tangential axis fastest): each side's
+  // coordinates are the same for every item up to rounding of the mapped
+  // coordinates (conforming faces: both sides the Gauss points; hanging
+  // interfaces batched by sub-face position: the larger side's are mapped).
   auto coord = [&](int64_t q, int s, int a) { return rp[q * SD + s * D + a]; };
   constexpr double tol = 1e-13;
   int Q0 = 1;
@@ -334,14 +339,18 @@ void prepare_face_path(gmf_plan& p, Bucket& b, const gmf_bucket_desc& bd) {
   const int Q1 = int(n / Q0);
   if (f.T[1] < 0 && Q1 != 1) return;
   if (Q0 > kMaxExtent || Q1 > kMaxExtent) return;
-  std::vector<double> t0(static_cast<size_t>(Q0), 0.0), t1(static_cast<size_t>(Q1), 0.0);
-  for (int k = 0; k < Q0; ++k) t0[size_t(k)] = f.T[0] >= 0 ? coord(k, 0, f.T[0]) : 0.0;
-  for (int k = 0; k < Q1; ++k) t1[size_t(k)] = f.T[1] >= 0 ? coord(int64_t(k) * Q0, 0, f.T[1]) : 0.0;
+  std::vector<double> t0s[2], t1s[2];
+  for (int s = 0; s < S; ++s) {
+    t0s[s].assign(static_cast<size_t>(Q0), 0.0);
+    t1s[s].assign(static_cast<size_t>(Q1), 0.0);
+    for (int k = 0; k < Q0; ++k) t0s[s][size_t(k)] = f.T[0] >= 0 ? coord(k, s, f.T[0]) : 0.0;
+    for (int k = 0; k < Q1; ++k) t1s[s][size_t(k)] = f.T[1] >= 0 ? coord(int64_t(k) * Q0, s, f.T[1]) : 0.0;
+  }
   for (int64_t q = 0; q < P; ++q) {
     const int64_t k = q % n;
     for (int s = 0; s < S; ++s) {
-      if (f.T[0] >= 0 && std::fabs(coord(q, s, f.T[0]) - t0[size_t(k % Q0)]) > tol) return;
-      if (f.T[1] >= 0 && std::fabs(coord(q, s, f.T[1]) - t1[size_t(k / Q0)]) > tol) return;
+      if (f.T[0] >= 0 && std::fabs(coord(q, s, f.T[0]) - t0s[s][size_t(k % Q0)]) > tol) return;
+      if (f.T[1] >= 0 && std::fabs(coord(q, s, f.T[1]) - t1s[s][size_t(k / Q0)]) > tol) return;
     }
   }
   f.Qt[0] = Q0;
@@ -378,7 +387,7 @@ void prepare_face_path(gmf_plan& p, Bucket& b, const gmf_bucket_desc& bd) {
     tab.insert(tab.end(), d.begin(), d.end());
     for (int j = 0; j < 2; ++j) {
       const int nt = f.T[j] >= 0 ? Ns[f.T[j]] : 1;
-      const std::vector<double>& pts = j == 0 ? t0 : t1;
+      const std::vector<double>& pts = j == 0 ? t0s[s] : t1s[s];
       const int qn = f.Qt[j];
       std::vector<double> tv(static_cast<size_t>(nt * qn)), td(static_cast<size_t>(nt * qn));
       std::vector<double> pv(static_cast<size_t>(nt)), pd(static_cast<size_t>(nt));

@@ -671,15 +671,23 @@ class AssemblyPlan:
         if (iops or irops) and len(space.interface_minus):
             mi, pi, ax = space.interface_minus, space.interface_plus, space.interface_axis
             qs = space.quadrature_shapes
-            # Conforming interfaces (both sides span the same face) form their
-            # own batches: their points are one tensor grid on both sides.
+            # Interfaces of equal sub-face position on both sides (position
+            # and extent of the shared patch within each leaf's face, per
+            # tangential axis; conforming: 0 and 1) form their own batches: in
+            # each, every side's points are one fixed tangential tensor grid.
             cm, hm = space.domain.cell_centers_and_half_sizes(space.leaves[mi])
             cp, hp = space.domain.cell_centers_and_half_sizes(space.leaves[pi])
-            same = np.isclose(cm - hm, cp - hp, rtol=0, atol=1e-14) & np.isclose(cm + hm, cp + hp, rtol=0, atol=1e-14)
-            same[np.arange(len(ax)), ax] = True
-            conforming = np.all(same, axis=1).astype(np.int64)
+            lo = np.maximum(cm - hm, cp - hp)
+            half = 0.5 * (np.minimum(cm + hm, cp + hp) - lo)
+            sub = []
+            for c, h in ((cm, hm), (cp, hp)):
+                pos = (lo - (c - h)) / (2.0 * h)
+                scl = half / h
+                pos[np.arange(len(ax)), ax] = 0.0
+                scl[np.arange(len(ax)), ax] = 0.0
+                sub += [np.rint(pos * 2 ** 24).astype(np.int64), np.rint(scl * 2 ** 24).astype(np.int64)]
             key = np.concatenate([ax[:, None], space.degrees[mi], space.degrees[pi],
-                                  np.maximum(qs[mi], qs[pi]), conforming[:, None]], axis=1)
+                                  np.maximum(qs[mi], qs[pi])] + sub, axis=1)
             uniq, inv = np.unique(key, axis=0, return_inverse=True)
             for g, k in enumerate(uniq):
                 sel = np.nonzero(inv.reshape(-1) == g)[0]

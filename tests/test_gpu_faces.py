@@ -106,13 +106,16 @@ def _describe(plan):
     return ops._ws(plan).describe()
 
 
-@pytest.mark.parametrize("name", ["2d_sipg", "3d_sipg", "3d_sipg_p7_small_chunks", "2d_sipg_p_mismatch"])
-def test_conforming_faces_take_the_tensor_path(gpu, name):
+@pytest.mark.parametrize("name", ["2d_sipg", "3d_sipg", "3d_sipg_p7_small_chunks", "2d_sipg_p_mismatch",
+                                  "2d_sipg_hanging", "2d_nonsymmetric_forms"])
+def test_faces_take_the_tensor_path(gpu, name):
+    """Every interface batch, hanging ones included (batched by sub-face
+    position), and every boundary-face batch runs the sum-factorized kernel."""
     global CASES
     if CASES is None:
         CASES = {k: g.compile(v) for k, v in cases().items()}
-    text = _describe(CASES[name])
-    assert "face-tensor" in text, text
+    lines = [l for l in _describe(CASES[name]).splitlines() if "interface points=" in l or "cloud points=" in l]
+    assert lines and all("face-tensor" in l for l in lines), "\n".join(lines)
 
 
 @pytest.mark.parametrize("name", ["2d_sipg_hanging", "3d_sipg", "2d_nonsymmetric_forms"])
