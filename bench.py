@@ -465,6 +465,17 @@ def build_sipg_problem(cells, degree, interfaces_only=False):
     return problem
 
 
+def _rebuild_sipg_on(domain, degree):
+    """The interface-only SIPG problem on an existing (refined) domain."""
+    import gricoamd as g
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree),
+                                             continuity="dg"))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_interface_accumulator(u, u, _SIPGFaces(degree))
+    return problem
+
+
 def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
     """Symmetric interior penalty DG Poisson (3D, DG p=3): device apply of the
     whole operator and of the interface items alone (two-sided point-cloud
@@ -558,6 +569,48 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
     out.update({"gmg_cg_seconds": t_gmg, "gmg_cg_iterations": int(info.iterations),
                 "gmg_setup_seconds": t_setup, "gmg_levels": h.reduced_dof_counts(),
                 "gmg_cg_relative_l2_error": g.relative_l2_error(g.State(plan, full), _SIPGFaces.exact)})
+    # Hanging interfaces: the same operator on 16^3 cells with the cells of the
+    # octant [0, 1/2]^3 refined once in every axis.
+    hang = build_sipg_problem(16, degree, interfaces_only=True)
+    dom = hang.field.space.domain
+    cells = [int(c) for c in dom.active_leaves()]
+    centers, _ = dom.cell_centers_and_half_sizes(cells)
+    for c, ctr in zip(cells, centers):
+        if np.all(ctr < 0.5):
+            kids = [c]
+            for axis in range(3):
+                kids = [k2 for k in kids for k2 in (lambda f: (f, f + 1))(dom.refine(k, axis))]
+    hang = _rebuild_sipg_on(hang.field.space.domain, degree)
+    t_h = {}
+    xh = None
+    for tag, env in (("face_tensor", None), ("point_cloud_kernel", "0")):
+        if env is not None:
+            os.environ["GMF_FACE_TENSOR"] = env
+        try:
+            hp = g.compile(hang)
+            ops._ws(hp)
+        finally:
+            os.environ.pop("GMF_FACE_TENSOR", None)
+        cm, tm = ctypes.c_double(), ctypes.c_double()
+        if xh is None:  # one input for both paths
+            gen = torch.Generator(device="cuda")
+            gen.manual_seed(7)
+            xh = torch.randn(hp.full_dof_count, dtype=torch.float64, device="cuda", generator=gen)
+        yh = torch.empty_like(xh)
+        _devabi.check(lib.gmf_apply_timed(ops._ws(hp).ws, xh.data_ptr(), yh.data_ptr(), 5, ctypes.byref(cm),
+                                          ctypes.byref(tm)))
+        t_h[tag] = cm.value * 1e-3
+        if tag == "face_tensor":
+            hang_items = sum(b.cell_count for b in hp.interface_buckets)
+            hang_batches = len(hp.interface_buckets)
+            yt = yh.cpu().numpy()
+        else:
+            hang_diff = float(np.abs(yh.cpu().numpy() - yt).max() / np.abs(yt).max())
+        del hp
+    out.update({"hanging_interface_items": hang_items, "hanging_interface_batches": hang_batches,
+                "hanging_interface_kernel_seconds": t_h["face_tensor"],
+                "hanging_interface_kernel_seconds_point_cloud": t_h["point_cloud_kernel"],
+                "hanging_tensor_vs_point_cloud_max_relative_difference": hang_diff})
     small = g.compile(build_sipg_problem(cpu_cells, degree))
     xv = np.random.default_rng(5).standard_normal(small.full_dof_count)
     t0 = time.perf_counter()
