@@ -545,20 +545,32 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
                 "interface_apply_seconds_point_cloud_kernel": t_cloud,
                 "interface_path": "face-tensor" if "face-tensor" in ops._ws(iplan).describe() else "point-cloud",
                 "interface_roofline_frac": if_bytes / t_if_kernels / (HBM_PEAK_GBS * 1e9)})
-    t0 = time.perf_counter()
-    state = g.solve(plan, solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()), relative_tolerance=1e-8,
-                    maxiter=50000)
-    out.update({"solve_seconds": time.perf_counter() - t0, "solve_iterations": int(state.info.iterations),
-                "solve_relative_l2_error": g.relative_l2_error(state, _SIPGFaces.exact)})
-    # hp-multigrid on the DG space, faces rediscretized on every level.
+    # Right-hand side on the host (face loads are not on the device path), then
+    # Jacobi-CG and GMG-CG timed on the Krylov loop alone.
     from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
     from gricoamd.solve import solve_reduced
+    t0 = time.perf_counter()
+    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
+    out["host_rhs_seconds"] = time.perf_counter() - t0
+    plan._loads = None
+    t0 = time.perf_counter()
+    bd = ops.reduced_rhs(plan, as_numpy=False)  # loads lowered on the host, projected on the device
+    torch.cuda.synchronize()
+    out["device_rhs_seconds_including_host_lowering"] = time.perf_counter() - t0
+    out["device_vs_host_rhs_max_relative_difference"] = float(torch.max(torch.abs(bd - b)) / torch.max(torch.abs(b)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xj, info = solve_reduced(plan, g.CGSolver(preconditioner=g.JacobiPreconditioner()), b, 1e-8, 0.0, 50000)
+    torch.cuda.synchronize()
+    full = ops.expand_reduced(plan, xj, include_shift=True).cpu().numpy()
+    out.update({"solve_seconds": time.perf_counter() - t0, "solve_iterations": int(info.iterations),
+                "solve_relative_l2_error": g.relative_l2_error(g.State(plan, full), _SIPGFaces.exact)})
+    # hp-multigrid on the DG space, faces rediscretized on every level.
     t0 = time.perf_counter()
     policy = GeometricMultigridPreconditioner(smoother="chebyshev")  # damped Jacobi diverges under the penalty
     h = compile_geometric_multigrid(build_sipg_problem(cells, degree), policy, fine_plan=plan)
     M = h.device_preconditioner()
     t_setup = time.perf_counter() - t0
-    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
     solve_reduced(plan, g.CGSolver(preconditioner=M), b, 1e-8, 0.0, 2000)  # warm-up (graph capture)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -569,6 +581,19 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
     out.update({"gmg_cg_seconds": t_gmg, "gmg_cg_iterations": int(info.iterations),
                 "gmg_setup_seconds": t_setup, "gmg_levels": h.reduced_dof_counts(),
                 "gmg_cg_relative_l2_error": g.relative_l2_error(g.State(plan, full), _SIPGFaces.exact)})
+    counts = h.reduced_dof_counts()
+    keep = len(counts) - max(i for i, c in enumerate(counts) if c <= 4096)
+    ht = h.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", max_levels=keep,
+                                                        coarse_direct_dof_limit=4096))
+    Mt = ht.device_preconditioner()
+    solve_reduced(plan, g.CGSolver(preconditioner=Mt), b, 1e-8, 0.0, 2000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, info = solve_reduced(plan, g.CGSolver(preconditioner=Mt), b, 1e-8, 0.0, 2000)
+    torch.cuda.synchronize()
+    out.update({"gmg_cg_dense_coarse_seconds": time.perf_counter() - t0,
+                "gmg_cg_dense_coarse_iterations": int(info.iterations),
+                "gmg_dense_coarse_levels": ht.reduced_dof_counts()})
     # Hanging interfaces: the same operator on 16^3 cells with the cells of the
     # octant [0, 1/2]^3 refined once in every axis.
     hang = build_sipg_problem(16, degree, interfaces_only=True)

@@ -243,6 +243,8 @@ bool launch_face_tensor(const gmf_plan& plan, const Bucket& b, const double* x, 
 // All face-tensor batches of the plan in one launch (after every bucket is
 // prepared); launch_face_batch marks the buckets it covered in `done`.
 void prepare_face_batch(gmf_plan& plan);
+// Face loads (items x points x sides x (1 + D)) of one face-tensor batch added into y.
+void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
 void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
                        std::vector<char>& done);
 // slots: the workspace's brick corner array (plan.brick_slots entries).

@@ -77,6 +77,9 @@ __device__ inline int face_box(const FaceArgs& a, int s, int k, int t1, int t2) 
   return m[0] + a.N[s][0] * (m[1] + a.N[s][1] * m[2]);
 }
 
+__device__ void face_project(const FaceArgs& a, int64_t item, const double* tab, const double* ch, double* R,
+                             double* __restrict__ y);
+
 // One item: the whole pipeline on one 64-lane workgroup.
 __device__ void face_item(const FaceArgs& a, int64_t item, double* sm, const double* __restrict__ x,
                           double* __restrict__ y) {
@@ -161,6 +164,16 @@ __device__ void face_item(const FaceArgs& a, int64_t item, double* sm, const dou
   }
   __syncthreads();
 
+  face_project(a, item, tab, ch, R, y);
+}
+
+// Transposed passes of one item: the test channels ch[SC][Q] (gradient
+// channels already scaled by J^-1) projected onto both sides' modes and
+// scattered into y.
+__device__ void face_project(const FaceArgs& a, int64_t item, const double* tab, const double* ch, double* R,
+                             double* __restrict__ y) {
+  const int tid = threadIdx.x;
+  const int C = a.C, Q = a.Q, NtM = a.NtMax;
   // Transposed tangential pass: R[s][0] collects value and tangential
   // derivative channels, R[s][1] the normal derivative channel.
   for (int t = tid; t < a.S * 2 * NtM; t += kFaceLanes) {
@@ -222,6 +235,30 @@ __global__ void __launch_bounds__(kFaceLanes) face_apply_tensor_batched(const Fa
   int k = 0;
   while (k + 1 < count && blk >= start[k + 1]) ++k;
   face_item(args[k], blk - start[k], sm, x, y);
+}
+
+// Face loads of one batch (src/accumulators.jl:1230-1274 one-sided loads,
+// 1666-1698 interface loads): per item, point and side the channels (W v,
+// W g_1, .., W g_D), projected like the test channels of the operator.
+__global__ void __launch_bounds__(kFaceLanes) face_rhs_tensor(FaceArgs a, const double* __restrict__ loads,
+                                                              double* __restrict__ y) {
+  extern __shared__ double sm[];
+  const int64_t item = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int C = a.C, SC = a.S * a.C, Q = a.Q;
+  double* tab = sm;
+  double* U = tab + a.table_doubles + a.Ntot;  // same layout as face_item
+  double* ch = U + a.S * 2 * a.NtMax;
+  for (int i = tid; i < a.table_doubles; i += kFaceLanes) tab[i] = a.tables[i];
+  const double* jinv = a.jinv + item * a.S * a.D;
+  const double* L = loads + item * int64_t(Q) * SC;
+  for (int t = tid; t < SC * Q; t += kFaceLanes) {
+    const int q = t / SC, r = t % SC, s = r / C, c = r % C;
+    const double v = L[t];
+    ch[r * Q + q] = c == 0 ? v : v * jinv[s * a.D + c - 1];
+  }
+  __syncthreads();
+  face_project(a, item, tab, ch, U, y);
 }
 
 // psi_0..psi_{n-1} and derivatives at x (src/polynomials.jl:403-434; p = 0 -> 1).
@@ -461,6 +498,14 @@ void launch_face_batch(const gmf_plan& p, const double* x, double* y, bool reduc
                      static_cast<const FaceArgs*>(args), g.start, g.count, x, y);
   GMF_HIP(hipGetLastError());
   for (int i : g.buckets) done[size_t(i)] = 1;
+}
+
+void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream) {
+  require(b.face.enabled, GMF_EUNSUPPORTED, "device face loads cover batches on the sum-factorized face path");
+  if (b.cells == 0) return;
+  FaceArgs a = make_face_args(b, false);
+  hipLaunchKernelGGL(face_rhs_tensor, dim3(unsigned(b.cells)), dim3(kFaceLanes), b.face.lds, stream, a, loads, y);
+  GMF_HIP(hipGetLastError());
 }
 
 bool launch_face_tensor(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,

@@ -99,6 +99,7 @@ def declare(lib):
     lib.gmf_reduced_rhs.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.gmf_l2_error.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, P_f64]
     lib.gmf_interpolate_channels.argtypes = [c_void_p, c_void_p, c_int32, c_void_p]
+    lib.gmf_rhs_faces.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.gmf_plan_set_point_data.argtypes = [c_void_p, c_int32, P_f64]
     lib.gmf_diagonal_physical.argtypes = [c_void_p, c_void_p]
     lib.gmf_reduced_diagonal.argtypes = [c_void_p, c_void_p]
@@ -113,7 +114,7 @@ def declare(lib):
                  "gmf_expand_reduced",
                  "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
                  "gmf_fgmres_solve", "gmf_rhs_physical", "gmf_reduced_rhs", "gmf_l2_error",
-                 "gmf_interpolate_channels", "gmf_plan_set_point_data", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
+                 "gmf_interpolate_channels", "gmf_plan_set_point_data", "gmf_rhs_faces", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
                  "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy"):
         getattr(lib, name).restype = ctypes.c_int
 

@@ -858,6 +858,51 @@ class AssemblyPlan:
                                                (bk.cell_count, bk.n_points)).reshape(-1) for bk in self.buckets])
 
     @property
+    def device_rhs(self):
+        """True when every load lives where the device right-hand side reaches:
+        tensor-product cells (with the cell batches on the device) and
+        boundary faces / interfaces of batches the device plan holds."""
+        if self.cloud_buckets or self.surface_buckets:
+            return False
+        if self.problem.cell_rhs and not self.problem.cell_operators and (self.boundary_buckets or
+                                                                          self.interface_buckets):
+            return False  # the device plan then holds no cell batches
+        if any(rops and not ops for (_, ops, rops) in self.boundary_buckets):
+            return False
+        if self.interface_rhs_ops and not self.interface_ops:
+            return False
+        return True
+
+    def face_loads(self):
+        """Point loads of the boundary-face and interface batches of the device
+        plan, in its bucket order: per item, point and side (W v, W g_1, ..,
+        W g_D) (gmf_rhs_faces)."""
+        D = self.dim
+        rops_of = {id(bk): rops for (bk, _, rops) in self.boundary_buckets}
+        out = []
+        for bk in self.all_buckets:
+            if not getattr(bk, "cloud", False):
+                continue
+            P = bk.n_points_total
+            q = bk.quadrature()
+            if getattr(bk, "sides", 1) == 2:
+                arr = np.zeros((P, 2, D + 1))
+                for op in self.interface_rhs_ops:
+                    for s, (v, g) in enumerate(bk.trace_channels(op.interface_rhs_accumulate(q, 0))):
+                        arr[:, s, 0] += bk.weights * v
+                        for a in range(D):
+                            arr[:, s, a + 1] += bk.weights * g[a]
+            else:
+                arr = np.zeros((P, 1, D + 1))
+                for op in rops_of.get(id(bk), []):
+                    v, g = _channels(op.boundary_rhs_accumulate(q, 0), (P,), D)
+                    arr[:, 0, 0] += bk.weights * v
+                    for a in range(D):
+                        arr[:, 0, a + 1] += bk.weights * g[a]
+            out.append(arr.reshape(-1))
+        return np.concatenate(out) if out else np.zeros(0)
+
+    @property
     def tensor_only(self):
         """True when every operator and load lives on tensor-product cells
         (the device right-hand side covers exactly those)."""

@@ -181,9 +181,18 @@ def solve(problem_or_plan, solver=None, relative_tolerance=None, absolute_tolera
     else:
         plan = problem_or_plan if is_plan else compile_plan(problem_or_plan)
         solver = _default_solver(plan, solver)
-    # Device-resident right-hand side for tensor-product plans (point loads
-    # lowered on the host); the host path covers point-cloud items.
-    b = ops.reduced_rhs(plan, as_numpy=False) if plan.tensor_only else plan.reduced_rhs()
+    # Device-resident right-hand side (point loads lowered on the host) for
+    # tensor cells, boundary faces and interfaces; the host path covers
+    # embedded surfaces and cut cells.
+    b = None
+    if plan.device_rhs:
+        try:
+            b = ops.reduced_rhs(plan, as_numpy=False)
+        except _devabi.DeviceError as exc:
+            if exc.status != -8:  # GMF_EUNSUPPORTED: a face batch outside the face path
+                raise
+    if b is None:
+        b = plan.reduced_rhs()
     x, info = solve_reduced(plan, solver, b, relative_tolerance, absolute_tolerance, maxiter, initial_solution)
     full = ops.expand_reduced(plan, x, include_shift=True).cpu().numpy()
     return State(plan, full, info)

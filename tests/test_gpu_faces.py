@@ -170,3 +170,48 @@ def test_sipg_gmg_cg_reproduces_quadratic(gpu):
                   relative_tolerance=1e-12)
     assert g.relative_l2_error(gmg, P.quadratic_exact) < 1e-9
     assert gmg.info.iterations < jac.info.iterations / 3
+
+
+def _rhs_cases():
+    from test_faces_host import _PlusLoad
+    out = {
+        "2d_sipg": P.sipg_poisson(dim=2, cells=4, degree=2),
+        "2d_sipg_hanging": P.sipg_poisson(dim=2, cells=3, degree=3, refine=((4, 0), (-1, 1), (0, 1))),
+        "3d_sipg_hanging": P.sipg_poisson(dim=3, cells=3, degree=2, refine=((13, 0),)),
+    }
+    pr = P.sipg_poisson(dim=2, cells=3, degree=2, refine=((4, 0),))
+    pr.add_interface_accumulator(pr.field, _PlusLoad(0.7))
+    out["2d_interface_loads"] = pr
+    return out
+
+
+@pytest.mark.parametrize("name", ["2d_sipg", "2d_sipg_hanging", "3d_sipg_hanging", "2d_interface_loads"])
+def test_device_face_rhs_matches_host(gpu, name):
+    """gmf_rhs_physical + gmf_rhs_faces against the host projection of the same
+    loads (itself pinned by the SIPG exactness and interface-load checks)."""
+    from gricoamd import ops
+    plan = g.compile(_rhs_cases()[name])
+    assert plan.device_rhs
+    dev = ops.rhs_physical(plan)
+    host = plan.rhs_physical()
+    assert rel(dev, host) <= 1e-12, name
+
+
+def test_device_rhs_falls_back_for_load_only_faces(gpu):
+    """A boundary side with a load but no operator is not in the device plan:
+    the host path takes over, and solve() still reproduces the solution."""
+    u = _dg(1, 3, 2)
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, P.MassDiffusion(1.0, 1.0))
+
+    class _Load:
+        def boundary_rhs_accumulate(self, q, test_component):
+            return 2.0 + 0.0 * q.point[0]
+
+    pr.add_boundary_accumulator(g.BoundaryFace(0, g.UPPER), u, _Load())
+    plan = g.compile(pr)
+    assert not plan.device_rhs
+    st = g.solve(plan, solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()), relative_tolerance=1e-12)
+    A = pyoracle.reduced_matrix(plan)
+    x = np.linalg.solve(A, plan.reduced_rhs())
+    assert np.abs(st.coefficients[plan.solve_dofs] - x).max() <= 1e-9 * np.abs(x).max()
