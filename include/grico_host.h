@@ -14,6 +14,7 @@
  * Reference counterparts:
  *   gh_grid_*            src/topology.jl:96-130 (CartesianGrid), src/refinement.jl:47 (refine!)
  *   gh_space_compile     src/space.jl:300-326 (HpSpace) -> src/continuity.jl:153-422
+ *   gh_space_faces       src/topology.jl:1502-1545 (boundary faces, interfaces of the snapshot)
  *   gh_gauss_legendre    src/quadrature.jl:342-365 (gauss_legendre_rule)
  *   gh_fe_basis          src/polynomials.jl:238-247, 403-434
  *   gh_restriction       src/continuity.jl:1158-1198 (_affine_restriction_matrix)
