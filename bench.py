@@ -930,6 +930,165 @@ def _traffic_from_profiles(n, cells):
     return rec.get("traffic_bytes_per_cell_pass")
 
 
+def build_fcm_problem(cells, degree, depth, alpha=1e-5):
+    """Finite-cell Poisson on the unit ball embedded in [-1.2, 1.2]^3: cells^3
+    background cells, cut cells with subdivision rules to `depth` and the
+    finite-cell extension alpha, Nitsche (symmetric, gamma = 10 (p + 1)^2 / h)
+    on explicit sphere surface pieces, load 6 (exact solution 1 - r^2)."""
+    import gricoamd as g
+    from gricoamd.embedded import ImplicitRegion, add_cut_cell_quadratures, physical_space, \
+        sphere_surface_quadratures
+
+    class Diffusion:
+        def cell_accumulate(self, q, trial, test_component):
+            return g.TestChannels(0.0 * trial.value, tuple(trial.gradient))
+
+    class Load:
+        def cell_rhs_accumulate(self, q, test_component):
+            return 6.0 + 0.0 * q.point[0]
+
+    class Nitsche:
+        def __init__(self, gamma):
+            self.gamma = gamma
+
+        def surface_accumulate(self, q, trial, test_component):
+            n = q.normal
+            dn = sum(nc * gc for nc, gc in zip(n, trial.gradient))
+            return g.TestChannels(-dn + self.gamma / q.cell_size[0] * trial.value,
+                                  tuple(-trial.value * nc for nc in n))
+
+    region = ImplicitRegion(lambda p: p[0] ** 2 + p[1] ** 2 + p[2] ** 2 - 1.0, subdivision_depth=depth)
+    dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (cells, cells, cells))
+    space, cut = physical_space(dom, region, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, Diffusion())
+    pr.add_cell_accumulator(u, Load())
+    add_cut_cell_quadratures(pr, region, cut, alpha=alpha)
+    for sq in sphere_surface_quadratures(space, (0.0, 0.0, 0.0), 1.0, n_theta=4 * cells, n_phi=8 * cells):
+        pr.add_surface_quadrature(sq)
+    pr.add_surface_accumulator(u, u, Nitsche(10.0 * (degree + 1) ** 2))
+    return pr
+
+
+def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, oracle_cells=4):
+    """Finite-cell sphere workload with nrhs seeded N(0,1) reduced right-hand
+    sides: the block apply (point-cloud items take every column in one launch)
+    against the one-column apply, and CG with the FP32 Jacobi preconditioner
+    (FP64 Krylov) run for a fixed number of iterations on the whole block and
+    on `sequential` columns one at a time (time per iteration and right-hand
+    side; Jacobi-CG needs tens of thousands of iterations to reach 1e-8 on this
+    operator, so convergence is checked by the small-ball tests); the block
+    apply against the oracle on a small ball."""
+    import numpy as np
+    import torch
+
+    import gricoamd as g
+    import pyoracle
+    from gricoamd import _devabi, _native, ops
+    from gricoamd.solve import compile_preconditioner
+
+    lib = _native.device()
+    out = {"workload": f"finite-cell Poisson, unit ball in [-1.2,1.2]^3, {cells}^3 background cells, p={degree}, "
+                       f"cut cells subdivided to depth {depth} (alpha 1e-5), Nitsche on explicit sphere pieces; "
+                       f"{nrhs} seeded N(0,1) right-hand sides, CG + FP32 Jacobi (FP64 Krylov)"}
+    t0 = time.perf_counter()
+    problem = build_fcm_problem(cells, degree, depth)
+    plan = g.compile(problem)
+    dev = ops._ws(plan)
+    torch.cuda.synchronize()
+    out["host_compile_seconds"] = time.perf_counter() - t0
+    n = plan.reduced_dof_count
+    cut = plan.cloud_buckets
+    out.update({"reduced_dofs": n, "tensor_cells": sum(b.cell_count for b in plan.buckets),
+                "cut_cells": sum(b.cell_count for b in cut),
+                "cut_cell_points": int(sum(b.n_points_total for b in cut)),
+                "surface_pieces": sum(b.cell_count for b in plan.surface_buckets),
+                "surface_points": int(sum(b.n_points_total for b in plan.surface_buckets))})
+    log(f"fcm: {out}")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(5)
+    X = torch.randn((nrhs, n), dtype=torch.float64, device="cuda", generator=gen)  # column j = X[j]
+    Y = torch.empty_like(X)
+    y1 = torch.empty(n, dtype=torch.float64, device="cuda")
+
+    def clock(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps
+
+    t_one = clock(lambda: _devabi.check(lib.gmf_reduced_apply(dev.ws, X[0].data_ptr(), y1.data_ptr())), 5)
+    t_block = clock(lambda: _devabi.check(lib.gmf_reduced_apply_block(dev.ws, X.data_ptr(), Y.data_ptr(), nrhs)), 2)
+    _devabi.check(lib.gmf_reduced_apply(dev.ws, X[nrhs - 1].data_ptr(), y1.data_ptr()))
+    torch.cuda.synchronize()
+    out.update({"apply_one_column_seconds": t_one, "apply_block_seconds": t_block,
+                "apply_block_seconds_per_column": t_block / nrhs, "apply_block_speedup_per_column": t_one / (t_block / nrhs),
+                "block_vs_one_column_max_relative_difference":
+                    float(torch.max(torch.abs(Y[nrhs - 1] - y1)) / torch.max(torch.abs(y1)))})
+    log(f"fcm apply: one column {t_one * 1e3:.2f} ms, block {t_block / nrhs * 1e3:.2f} ms per column")
+    del Y
+    # CG with a fixed iteration budget: the whole block, then columns one at a time.
+    solver = g.CGSolver(preconditioner=g.JacobiPreconditioner(precision="fp32"))
+    M = compile_preconditioner(solver.preconditioner, plan)
+    B = X
+    Xs = torch.zeros_like(B)
+    infos = (_devabi.SolveInfo * nrhs)()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    status = lib.gmf_cg_solve_block(dev.ws, M.handle, B.data_ptr(), Xs.data_ptr(), nrhs, 0, 1e-8, 0.0, iterations,
+                                    infos)
+    torch.cuda.synchronize()
+    t_bsolve = time.perf_counter() - t0
+    # -3: the budget ended before 1e-8 (expected); -4: a column met a
+    # non-positive direction (Nitsche on small cuts can make the operator
+    # indefinite), reported with the iterations done.
+    if status not in (0, -3, -4):
+        _devabi.check(status)
+    its = [int(i.iterations) for i in infos]
+    out.update({"cg_iterations": iterations, "block_cg_status": int(status), "block_cg_seconds": t_bsolve,
+                "block_cg_seconds_per_iteration_and_rhs": t_bsolve / (max(1, max(its)) * nrhs),
+                "block_relative_residual_after_budget_max": max(i.final_residual_norm / i.rhs_norm for i in infos)})
+    log(f"fcm block cg: {t_bsolve:.2f} s for {max(its)} iterations on {nrhs} rhs")
+    seq = []
+    for j in range(sequential):
+        info = _devabi.SolveInfo()
+        x1 = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        status = lib.gmf_cg_solve(dev.ws, M.handle, B[j].data_ptr(), x1.data_ptr(), 0, 1e-8, 0.0, iterations,
+                                  ctypes_byref(info))
+        torch.cuda.synchronize()
+        if status not in (0, -3, -4):
+            _devabi.check(status)
+        seq.append((time.perf_counter() - t0, max(1, int(info.iterations)), info.final_residual_norm / info.rhs_norm,
+                    infos[j].final_residual_norm / infos[j].rhs_norm,
+                    float(torch.max(torch.abs(x1 - Xs[j])) / torch.max(torch.abs(x1)))))
+    t_seq = sum(s_[0] / s_[1] for s_ in seq) / len(seq)
+    out.update({"one_column_cg_seconds_per_iteration": t_seq,
+                "block_speedup_per_iteration_and_rhs": t_seq / out["block_cg_seconds_per_iteration_and_rhs"],
+                "one_column_vs_block_relative_residual": [[s_[2], s_[3]] for s_ in seq],
+                "one_column_vs_block_iterate_max_relative_difference": max(s_[4] for s_ in seq)})
+    log(f"fcm one-column cg: {t_seq * 1e3:.2f} ms per iteration")
+    del Xs, X
+    # The block apply against the oracle on a small ball of the same kind.
+    small = g.compile(build_fcm_problem(oracle_cells, degree, depth))
+    Xo = np.random.default_rng(9).standard_normal((small.reduced_dof_count, 3))
+    Yo = g.reduced_apply_block(small, Xo).cpu().numpy()
+    ref = np.stack([pyoracle.reduced_apply(small, Xo[:, j]) for j in range(3)], axis=1)
+    out["oracle_max_relative_error"] = float(np.abs(Yo - ref).max() / np.abs(ref).max())
+    out["oracle_sample"] = f"{oracle_cells}^3 background cells, same degree and depth, 3 columns"
+    return out
+
+
+def ctypes_byref(obj):
+    import ctypes
+    return ctypes.byref(obj)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -944,6 +1103,12 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the nonlinear Bratu Newton leg")
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
+    ap.add_argument("--fcm", action="store_true",
+                    help="run the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG)")
+    ap.add_argument("--fcm-only", action="store_true", help="run only the finite-cell sphere leg and print its JSON")
+    ap.add_argument("--fcm-cells", type=int, default=48, help="background cells per axis of the finite-cell leg")
+    ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
+    ap.add_argument("--fcm-iterations", type=int, default=30, help="CG iterations timed in the finite-cell leg")
     ap.add_argument("--learned", action="store_true",
                     help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
                          "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
@@ -966,6 +1131,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.sipg_only:
         print(json.dumps({"sipg": sipg_leg()}), flush=True)
+        return
+    if args.fcm_only:
+        print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)}), flush=True)
         return
     t0 = time.perf_counter()
     problem = build_problem(args.depth)
@@ -1079,6 +1247,12 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"log-normal leg failed: {exc!r}")
             result["lognormal"] = {"error": repr(exc)}
+    if args.fcm:
+        try:
+            result["fcm"] = fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)
+        except Exception as exc:  # keep the headline line
+            log(f"fcm leg failed: {exc!r}")
+            result["fcm"] = {"error": repr(exc)}
     if args.learned:
         result["learned"] = learned_leg()
     if not args.no_cpu_baseline:

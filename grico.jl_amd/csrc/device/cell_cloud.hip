@@ -21,6 +21,7 @@
 // each side, apply the point action; then lanes over the local modes project
 // the chunk's channels back onto their side.
 #include <cmath>
+#include <cstdlib>
 
 #include "common.hpp"
 #include "termmap.hpp"
@@ -42,7 +43,18 @@ struct CloudArgs {
   const int64_t* poff;         // items + 1
   const double* rpts;          // points x S*D
   TermMap map;                 // items x Ntot
+  const int32_t* unit_item;    // work unit -> item
+  const int64_t* unit_start;   // work unit -> first point
 };
+
+// Item and point range [pb, pe) of the apply kernels' work unit `u`.
+__device__ inline int64_t unit_range(const CloudArgs& a, int64_t u, int64_t& pb, int64_t& pe) {
+  const int64_t item = a.unit_item[u];
+  pb = a.unit_start[u];
+  const int64_t end = a.poff[item + 1];
+  pe = end - pb < kCloudUnitPoints ? end : pb + kCloudUnitPoints;
+  return item;
+}
 
 // psi_0..psi_{n-1} and derivatives at x into v[k * stride], d[k * stride].
 __device__ inline void integrated_legendre(double x, int n, int stride, double* v, double* d) {
@@ -147,7 +159,8 @@ __device__ void chunk_forward(const CloudArgs& a, int64_t item, int64_t p0, int 
 __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, const double* __restrict__ x,
                                                                 double* __restrict__ y) {
   extern __shared__ double smem[];
-  const int64_t item = blockIdx.x;
+  int64_t pb, pe;
+  const int64_t item = unit_range(a, blockIdx.x, pb, pe);
   const int C = a.D + 1;
   double* u = smem;
   double* r = u + a.Ntot;
@@ -159,7 +172,6 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
     r[i] = 0.0;
   }
   __syncthreads();
-  const int64_t pb = a.poff[item], pe = a.poff[item + 1];
   for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
     const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
     chunk_forward(a, item, p0, cnt, u, tv, td, ch);
@@ -180,6 +192,145 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
     __syncthreads();
   }
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], y, false);
+}
+
+// Basis value and reference derivatives with the dimension known at compile
+// time (the products of basis_at in the same order, kept in registers).
+template <int D>
+__device__ inline void basis_static(const CloudArgs& a, const double* tv, const double* td, int s,
+                                    const int (&m)[kMaxDim], int q, double& phi, double (&dphi)[kMaxDim]) {
+  double f[D], fd[D];
+  const int base = table_base(a, s);
+#pragma unroll
+  for (int ax = 0; ax < D; ++ax) {
+    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+  }
+  phi = 1.0;
+#pragma unroll
+  for (int ax = 0; ax < D; ++ax) phi *= f[ax];
+#pragma unroll
+  for (int ax = 0; ax < D; ++ax) {
+    double v = fd[ax];
+#pragma unroll
+    for (int bx = 0; bx < D; ++bx)
+      if (bx != ax) v *= f[bx];
+    dphi[ax] = v;
+  }
+}
+
+// Several columns per workgroup (block applies over right-hand sides stored
+// column after column, leading dimension ld): the 1D factors of a chunk's
+// points and the point action serve KC columns, while every column runs the
+// arithmetic of cell_apply_cloud in the same order, so each item's local
+// result per column equals the one-column kernel's.
+template <int KC, int D>
+__global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud_block(CloudArgs a, const double* __restrict__ X,
+                                                                      double* __restrict__ Y, int64_t ld, int k) {
+  extern __shared__ double smem[];
+  constexpr int C = D + 1;
+  constexpr int kMaxSC = kMaxSides * C;
+  int64_t pb, pe;
+  const int64_t item = unit_range(a, blockIdx.x, pb, pe);
+  const int j0 = int(blockIdx.y) * KC;
+  const int kc = k - j0 < KC ? k - j0 : KC;
+  const int SC = a.S * C;
+  double* u = smem;                                  // KC x Ntot
+  double* r = u + KC * a.Ntot;                       // KC x Ntot
+  double* tv = r + KC * a.Ntot;                      // S x D x Nmax x chunk
+  double* td = tv + a.S * D * a.Nmax * a.chunk;      // S x D x Nmax x chunk
+  double* ch = td + a.S * D * a.Nmax * a.chunk;      // KC x SC x chunk
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x)
+    for (int j = 0; j < kc; ++j) {
+      u[j * a.Ntot + i] = gather_term(a.map, item, i, X + int64_t(j0 + j) * ld);
+      r[j * a.Ntot + i] = 0.0;
+    }
+  __syncthreads();
+  const double* jinv = a.jinv + item * a.S * D;
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
+    const int q = threadIdx.x;
+    if (q < cnt) {
+      // Each lane writes and reads only its own point's column of tv / td.
+      point_tables(a, p0 + q, q, tv, td);
+      for (int s = 0; s < a.S; ++s) {
+        double val[KC], g[KC][D];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+          val[j] = 0.0;
+#pragma unroll
+          for (int ax = 0; ax < D; ++ax) g[j][ax] = 0.0;
+        }
+        const int first = s ? a.Ns[0] : 0;
+        for (int i = 0; i < a.Ns[s]; ++i) {
+          int m[kMaxDim];
+          unflatten(a, first + i, m);
+          double phi, dphi[kMaxDim];
+          basis_static<D>(a, tv, td, s, m, q, phi, dphi);
+#pragma unroll
+          for (int j = 0; j < KC; ++j) {
+            if (j < kc) {
+              const double ui = u[j * a.Ntot + first + i];
+              val[j] = fma(ui, phi, val[j]);
+#pragma unroll
+              for (int ax = 0; ax < D; ++ax) g[j][ax] = fma(ui, dphi[ax], g[j][ax]);
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+          if (j < kc) {
+            double* cj = ch + (j * SC + s * C) * a.chunk + q;
+            cj[0] = val[j];
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) cj[(1 + ax) * a.chunk] = jinv[s * D + ax] * g[j][ax];
+          }
+        }
+      }
+      const double* M = a.pdata + (p0 + q) * SC * SC;
+      for (int j = 0; j < kc; ++j) {
+        double chn[kMaxSC];
+        double* cj = ch + j * SC * a.chunk + q;
+#pragma unroll
+        for (int kk = 0; kk < kMaxSC; ++kk) chn[kk] = kk < SC ? cj[kk * a.chunk] : 0.0;
+        for (int rr = 0; rr < SC; ++rr) {
+          double acc = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < kMaxSC; ++kk)
+            if (kk < SC) acc = fma(M[rr * SC + kk], chn[kk], acc);
+          const int s = rr / C, c = rr % C;
+          cj[rr * a.chunk] = c == 0 ? acc : acc * jinv[s * D + c - 1];
+        }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+      int m[kMaxDim];
+      const int s = unflatten(a, i, m);
+      double acc[KC];
+#pragma unroll
+      for (int j = 0; j < KC; ++j) acc[j] = 0.0;
+      for (int qq = 0; qq < cnt; ++qq) {
+        double phi, dphi[kMaxDim];
+        basis_static<D>(a, tv, td, s, m, qq, phi, dphi);
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+          if (j < kc) {
+            const double* chs = ch + (j * SC + s * C) * a.chunk;
+            acc[j] = fma(chs[qq], phi, acc[j]);
+#pragma unroll
+            for (int ax = 0; ax < D; ++ax) acc[j] = fma(chs[(ax + 1) * a.chunk + qq], dphi[ax], acc[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < KC; ++j)
+        if (j < kc) r[j * a.Ntot + i] += acc[j];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x)
+    for (int j = 0; j < kc; ++j) scatter_term(a.map, item, i, r[j * a.Ntot + i], Y + int64_t(j0 + j) * ld, false);
 }
 
 // Local diagonal d_i = sum_q sum_{c,k} M_q[(s,c)][(s,k)] chi_c chi_k with
@@ -240,6 +391,12 @@ size_t diagonal_lds(const CloudArgs& a) {
                            size_t(a.S * (a.D + 1) * (a.D + 1) * a.chunk));
 }
 
+template <int KC>
+size_t block_lds(const CloudArgs& a) {
+  return sizeof(double) * (size_t(KC) * size_t(2 * a.Ntot + a.S * (a.D + 1) * a.chunk) +
+                           size_t(2 * a.S * a.D * a.Nmax * a.chunk));
+}
+
 // Largest chunk (64, 32, .., 8 points) whose LDS footprint fits 64 KB.
 template <class F>
 void pick_chunk(CloudArgs& a, F lds_of) {
@@ -268,10 +425,46 @@ CloudArgs make_cloud_args(const Bucket& b, bool reduced) {
   a.poff = b.poff;
   a.rpts = b.rpts;
   a.map = TermMap{reduced ? b.ridx : b.lidx, b.lcoef, b.moff, reduced ? b.mridx : b.midx, b.mcoef, b.Ntot};
+  a.unit_item = b.unit_item;
+  a.unit_start = b.unit_start;
   return a;
 }
 
+// Columns per workgroup of the block kernel (GMF_CLOUD_COLUMNS=4 selects 4).
+int cloud_block_columns() {
+  static const int cols = [] {
+    const char* e = std::getenv("GMF_CLOUD_COLUMNS");
+    return (e && std::atoi(e) == 4) ? 4 : 8;
+  }();
+  return cols;
+}
+
+template <int KC>
+void launch_block(const Bucket& b, const double* X, double* Y, int64_t ld, int k, bool reduced, hipStream_t stream) {
+  CloudArgs a = make_cloud_args(b, reduced);
+  require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  pick_chunk(a, block_lds<KC>);
+  const dim3 grid(unsigned(b.units), unsigned((k + KC - 1) / KC));
+  const size_t lds = block_lds<KC>(a);
+  switch (a.D) {
+    case 1: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 1>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 2: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 2>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    default: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k);
+  }
+  GMF_HIP(hipGetLastError());
+}
+
 }  // namespace
+
+void launch_cell_cloud_block(const gmf_plan&, const Bucket& b, const double* X, double* Y, int64_t ld, int k,
+                             bool reduced, hipStream_t stream) {
+  if (b.cells == 0 || k <= 0) return;
+  require(k <= 65535 * 4, GMF_EINVAL, "too many columns for one block launch");
+  if (cloud_block_columns() == 4)
+    launch_block<4>(b, X, Y, ld, k, reduced, stream);
+  else
+    launch_block<8>(b, X, Y, ld, k, reduced, stream);
+}
 
 void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,
                        hipStream_t stream) {
@@ -279,7 +472,7 @@ void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double
   CloudArgs a = make_cloud_args(b, reduced);
   require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
   pick_chunk(a, apply_lds);
-  hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.cells)), dim3(kCloudLanes), apply_lds(a), stream, a, x, y);
+  hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), apply_lds(a), stream, a, x, y);
   GMF_HIP(hipGetLastError());
 }
 

@@ -73,6 +73,8 @@ struct FacePath {
 };
 
 // Device-resident cell batch.
+constexpr int kCloudUnitPoints = 512;
+
 struct Bucket {
   int dim = 0;
   int N[kMaxDim] = {1, 1, 1};
@@ -120,6 +122,13 @@ struct Bucket {
   int max_points = 0;
   int64_t* poff = nullptr;
   double* rpts = nullptr;
+  // Work units of the apply kernels: items with more points than
+  // kCloudUnitPoints are split into point ranges (unit u covers item
+  // unit_item[u], points unit_start[u] .. min(unit_start[u] + kCloudUnitPoints,
+  // poff[item + 1]) - 1), so heavy cut cells spread over several workgroups.
+  int64_t units = 0;
+  int32_t* unit_item = nullptr;
+  int64_t* unit_start = nullptr;
   // Exact reduced diagonal on cells whose modes couple through multi-term
   // expansions or repeat a dof: `coupled` flags those cells (null if none) and
   // each (cell, reduced dof) pair lists the dof's local coefficient vector.

@@ -86,6 +86,9 @@ inline void require_device_pointer(const void* p, const char* what) {
 }
 
 void sync(gmf_workspace* ws);
+// GMF_PROGRESS=N (N > 0): the Krylov loops print one line to stderr every N
+// iterations (long solves); 0 when unset.
+int progress_interval();
 // y_red = P^T A P x_red on ws->stream (src/assembly.jl:905-915).
 void reduced_apply(gmf_workspace* ws, const double* xr, double* yr);
 // The same operator on FP32 vectors: FP32 kernels on plans whose buckets all

@@ -2,6 +2,8 @@
 // operator application, reduced maps, diagonal, Jacobi, CG and FGMRES.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -317,6 +319,15 @@ void invert_diagonal(gmf_workspace* ws, const double* d, double* inv, int64_t n)
 
 void sync(gmf_workspace* ws) { GMF_HIP(hipStreamSynchronize(ws->stream)); }
 
+int progress_interval() {
+  static const int every = [] {
+    const char* e = std::getenv("GMF_PROGRESS");
+    const int v = e ? std::atoi(e) : 0;
+    return v > 0 ? v : 0;
+  }();
+  return every;
+}
+
 // ---- preconditioners ---------------------------------------------------------
 
 void JacobiPrecond::apply(const double* r, double* z) { jacobi_dot(ws, z, r, invdiag, n, kTmp); }
@@ -389,6 +400,9 @@ void cg_run(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x
     require(ps[kPq] > 0.0, GMF_EBREAKDOWN,
             "CG encountered a non-positive operator direction; provide a different linear_solve for this problem");
     inf.final_residual_norm = std::sqrt(ps[kRr]);
+    if (progress_interval() && it % progress_interval() == 0)
+      std::fprintf(stderr, "gmf cg: iteration %lld, residual %.3e of tolerance %.3e\n", (long long)it,
+                   inf.final_residual_norm, tol);
     if (ps[kRr] <= tol * tol) {
       inf.converged = 1;
       sync(ws);
@@ -715,6 +729,17 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
                     "reference points must lie in the biunit cell");
           b.poff = upload(p, po, size_t(b.cells) + 1);
           b.rpts = upload(p, bd.ref_points, size_t(b.total_points) * size_t(b.sides) * size_t(b.dim));
+          std::vector<int32_t> uitem;
+          std::vector<int64_t> ustart;
+          for (int64_t c = 0; c < b.cells; ++c)
+            for (int64_t s0 = po[c]; s0 == po[c] || s0 < po[c + 1]; s0 += kCloudUnitPoints) {
+              uitem.push_back(int32_t(c));
+              ustart.push_back(s0);
+            }
+          require(b.cells <= INT32_MAX, GMF_EINVAL, "too many point-cloud items in one batch");
+          b.units = int64_t(uitem.size());
+          b.unit_item = upload(p, uitem.data(), uitem.size());
+          b.unit_start = upload(p, ustart.data(), ustart.size());
         } else {
           b.h_values.assign(bd.values, bd.values + off);
           b.h_derivs.assign(bd.derivatives, bd.derivatives + off);

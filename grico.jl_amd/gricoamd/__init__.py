@@ -22,6 +22,7 @@ from .plan import AssemblyPlan, compile
 from .ops import (apply, apply_into, apply_physical, diagonal_physical, expand_reduced, project_reduced,
                   reduced_apply, reduced_diagonal)
 from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, JacobiPreconditioner, State, solve,
+                    solve_block, reduced_apply_block,
                     solve_reduced)
 from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid

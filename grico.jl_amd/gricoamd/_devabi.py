@@ -108,6 +108,10 @@ def declare(lib):
                                  c_double, c_int64, ctypes.POINTER(SolveInfo)]
     lib.gmf_fgmres_solve.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32,
                                      c_double, c_double, c_int64, ctypes.POINTER(SolveInfo)]
+    lib.gmf_reduced_apply_block.argtypes = [c_void_p, c_void_p, c_void_p, c_int32]
+    lib.gmf_cg_solve_block.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double,
+                                       c_double, c_int64, ctypes.POINTER(SolveInfo)]
+    lib.gmf_precond_create_jacobi_f32.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
     for name in ("gmf_plan_create", "gmf_plan_destroy", "gmf_plan_sizes", "gmf_workspace_create",
                  "gmf_workspace_destroy", "gmf_workspace_set_stream", "gmf_apply", "gmf_apply_async",
                  "gmf_apply_physical", "gmf_reduced_apply", "gmf_reduced_apply_f32", "gmf_project_reduced",
@@ -115,7 +119,8 @@ def declare(lib):
                  "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
                  "gmf_fgmres_solve", "gmf_rhs_physical", "gmf_reduced_rhs", "gmf_l2_error",
                  "gmf_interpolate_channels", "gmf_plan_set_point_data", "gmf_rhs_faces", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
-                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy"):
+                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy",
+                 "gmf_reduced_apply_block", "gmf_cg_solve_block", "gmf_precond_create_jacobi_f32"):
         getattr(lib, name).restype = ctypes.c_int
 
 

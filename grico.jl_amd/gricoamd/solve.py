@@ -19,7 +19,13 @@ from .problem import AffineProblem
 
 
 class JacobiPreconditioner:
-    pass
+    """Jacobi policy (src/solve.jl:651-656). precision "fp32" applies the
+    inverse diagonal in single precision inside the FP64 Krylov method."""
+
+    def __init__(self, precision="fp64"):
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
 
 
 class CGSolver:
@@ -95,7 +101,9 @@ def compile_preconditioner(policy, plan):
         inv = jacobi_inverse_diagonal(plan)
         handle = _native.c_void_p()
         dev = ops._ws(plan)
-        _devabi.check(_native.device().gmf_precond_create_jacobi(dev.ws, inv.data_ptr(), ctypes.byref(handle)))
+        create = _native.device().gmf_precond_create_jacobi_f32 if getattr(policy, "precision", "fp64") == "fp32" \
+            else _native.device().gmf_precond_create_jacobi
+        _devabi.check(create(dev.ws, inv.data_ptr(), ctypes.byref(handle)))
         return DevicePreconditioner(plan, handle, keep=(inv,))
     compile_hook = getattr(policy, "compile", None)
     if compile_hook is not None:
@@ -142,6 +150,61 @@ def solve_reduced(plan, solver, b_red, relative_tolerance=None, absolute_toleran
         raise ConvergenceError(_native.device().gmf_last_error().decode())
     _devabi.check(status)
     return x, info
+
+
+def reduced_apply_block(plan, X):
+    """Y[:, j] = P^T A P X[:, j] for a (n_red, k) block (gmf_reduced_apply_block);
+    point-cloud batches take every column in one launch."""
+    torch = ops._torch()
+    Xc = _block(plan, X)
+    Y = torch.empty_like(Xc)
+    _devabi.check(_native.device().gmf_reduced_apply_block(ops._ws(plan).ws, Xc.data_ptr(), Y.data_ptr(),
+                                                           Xc.shape[0]))
+    return Y.T
+
+
+def _block(plan, B):
+    """Device copy of an (n_red, k) block, stored column after column (k x n_red)."""
+    torch = ops._torch()
+    n = plan.reduced_dof_count
+    Bt = torch.as_tensor(B, dtype=torch.float64)
+    if Bt.dim() != 2 or Bt.shape[0] != n:
+        raise ValueError(f"block must have shape ({n}, k)")
+    if not 1 <= Bt.shape[1] <= 4096:
+        raise ValueError("column count must be in 1..4096")
+    return Bt.T.to("cuda").contiguous()
+
+
+def solve_block(plan, B, solver=None, relative_tolerance=None, absolute_tolerance=0.0, maxiter=None,
+                initial_solution=None):
+    """CG on every column of an (n_red, k) block of reduced right-hand sides in
+    lockstep (gmf_cg_solve_block): the k solves of solve_reduced with a shared
+    operator, each column with its own scalars and stopping test. Returns
+    (X as an (n_red, k) device view, list of per-column info)."""
+    torch = ops._torch()
+    solver = solver or CGSolver(preconditioner=JacobiPreconditioner())
+    if not isinstance(solver, CGSolver):
+        raise TypeError("block solves support CGSolver")
+    n = plan.reduced_dof_count
+    rtol = math.sqrt(np.finfo(np.float64).eps) if relative_tolerance is None else float(relative_tolerance)
+    atol = float(absolute_tolerance)
+    if not (math.isfinite(rtol) and rtol >= 0 and math.isfinite(atol) and atol >= 0):
+        raise ValueError("tolerances must be finite and non-negative")
+    maxiter = max(1000, 2 * n) if maxiter is None else int(maxiter)
+    if maxiter < 1:
+        raise ValueError("maxiter must be positive")
+    Bd = _block(plan, B)
+    k = Bd.shape[0]
+    X = torch.zeros_like(Bd) if initial_solution is None else _block(plan, initial_solution).clone()
+    M = compile_preconditioner(solver.preconditioner, plan)
+    infos = (_devabi.SolveInfo * k)()
+    status = _native.device().gmf_cg_solve_block(ops._ws(plan).ws, M.handle if M is not None else None,
+                                                 Bd.data_ptr(), X.data_ptr(), k, int(initial_solution is not None),
+                                                 rtol, atol, maxiter, infos)
+    if status == -3:
+        raise ConvergenceError(_native.device().gmf_last_error().decode())
+    _devabi.check(status)
+    return X.T, list(infos)
 
 
 def _default_affine_preconditioner(problem):

@@ -242,6 +242,12 @@ gmf_status gmf_reduced_diagonal(gmf_workspace* ws, double* diag_red);
  * reduced_dof_count, copied) (src/solve.jl:651-656, 761-774). */
 gmf_status gmf_precond_create_jacobi(gmf_workspace* ws, const double* inverse_diagonal,
                                      gmf_preconditioner** out);
+/* Jacobi preconditioner applied in FP32: the inverse diagonal (device FP64,
+ * reduced_dof_count) is rounded to FP32 and z = D^-1 r is formed in FP32, then
+ * widened (the single-precision preconditioner of an FP64 Krylov method; the
+ * reference's Jacobi, src/solve.jl:651-656, 761-774, in FP64 only). */
+gmf_status gmf_precond_create_jacobi_f32(gmf_workspace* ws, const double* inverse_diagonal,
+                                         gmf_preconditioner** out);
 gmf_status gmf_precond_destroy(gmf_preconditioner* precond);
 /* z = M^-1 r on the workspace stream. */
 gmf_status gmf_precond_apply(gmf_preconditioner* precond, const double* r, double* z);
@@ -252,6 +258,19 @@ gmf_status gmf_precond_apply(gmf_preconditioner* precond, const double* r, doubl
 gmf_status gmf_cg_solve(gmf_workspace* ws, gmf_preconditioner* precond, const double* b, double* x,
                         int32_t use_initial, double rtol, double atol, int64_t maxiter,
                         gmf_solve_info* info);
+/* Several right-hand sides at once (no single reference counterpart: the
+ * batched form of k calls of src/solve.jl:376-431 _cg_solve on one operator).
+ * Blocks are reduced_dof_count x k, column j at offset j * reduced_dof_count,
+ * 1 <= k <= 4096. gmf_reduced_apply_block: Y_j = P^T A P X_j (src/assembly.jl:
+ * 905-915 per column; point-cloud batches take every column in one launch).
+ * gmf_cg_solve_block: CG on every column in lockstep with its own scalars, the
+ * iteration of gmf_cg_solve per column; a converged column is frozen at the
+ * iterate where it converged; info holds k entries. GMF_ENOCONV when a column
+ * has not converged after maxiter iterations (info still filled). */
+gmf_status gmf_reduced_apply_block(gmf_workspace* ws, const double* X, double* Y, int32_t k);
+gmf_status gmf_cg_solve_block(gmf_workspace* ws, gmf_preconditioner* precond, const double* B, double* X,
+                              int32_t k, int32_t use_initial, double rtol, double atol, int64_t maxiter,
+                              gmf_solve_info* info);
 /* Restarted right-preconditioned FGMRES with modified Gram-Schmidt and Givens
  * rotations (src/solve.jl:433-614 _fgmres_solve). */
 gmf_status gmf_fgmres_solve(gmf_workspace* ws, gmf_preconditioner* precond, const double* b, double* x,

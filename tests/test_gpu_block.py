@@ -1,0 +1,136 @@
+"""Several right-hand sides at once (gmf_reduced_apply_block, gmf_cg_solve_block,
+gmf_precond_create_jacobi_f32): the block apply against the oracle's dense
+operator and the one-column device apply; block CG against the oracle's dense
+solves and the one-column CG. The reference solves one right-hand side per call
+(src/solve.jl:376-431), so every column is checked against that iteration."""
+
+import numpy as np
+import pytest
+
+import gricoamd as g
+import pyoracle
+
+import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _block(n, k, seed):
+    return np.random.default_rng(seed).standard_normal((n, k))
+
+
+CASES = {
+    # cut cells and sphere pieces: the multi-column point-cloud kernel
+    "fcm_ball_p2": lambda: P.fcm_ball(n=4, degree=2, depth=2),
+    "fcm_ball_p3": lambda: P.fcm_ball(n=3, degree=3, depth=2, n_theta=12, n_phi=24),
+    # hanging nodes (multi-term maps: one column at a time through the full layout)
+    "adaptive_hp": lambda: P.adaptive_hp(base=4, levels=2, pmax=3),
+    # interfaces and boundary faces (sum-factorized face kernels per column)
+    "sipg_2d": lambda: P.sipg_poisson(dim=2, cells=3, degree=2),
+}
+
+
+class _Operator:
+    """The oracle's reduced operator: a dense matrix for small plans, column
+    by column oracle applies otherwise."""
+
+    def __init__(self, plan):
+        self.plan = plan
+        self.A = pyoracle.reduced_matrix(plan) if plan.reduced_dof_count <= 700 else None
+
+    def __matmul__(self, X):
+        if self.A is not None:
+            return self.A @ X
+        if X.ndim == 1:
+            return pyoracle.reduced_apply(self.plan, X)
+        return np.stack([pyoracle.reduced_apply(self.plan, X[:, j]) for j in range(X.shape[1])], axis=1)
+
+
+@pytest.fixture(scope="module", params=sorted(CASES))
+def case(request, gpu):
+    print("case", request.param, flush=True)
+    plan = g.compile(CASES[request.param]())
+    return request.param, plan, _Operator(plan)
+
+
+# Attainable true residual of CG in FP64 after its recursive residual met
+# 1e-10: the gap between the two grows with the condition number (about 1e3 -
+# 1e4 for the conforming and DG cases, 5e8 for the p = 2 ball, 9e9 for the
+# p = 3 ball, whose true residual stops near 2e-7 after ~67k iterations).
+TRUE_RESIDUAL = {"adaptive_hp": 1e-9, "sipg_2d": 1e-9, "fcm_ball_p2": 1e-8, "fcm_ball_p3": 2e-6}
+
+
+@pytest.mark.parametrize("k", [1, 5, 8, 19])
+def test_block_apply(case, k):
+    name, plan, A = case
+    X = _block(plan.reduced_dof_count, k, 10 + k)
+    Y = g.reduced_apply_block(plan, X).cpu().numpy()
+    ref = A @ X
+    assert np.abs(Y - ref).max() <= 1e-12 * np.abs(ref).max(), name
+    for j in (0, k - 1):
+        yj = g.reduced_apply(plan, X[:, j])
+        assert np.abs(Y[:, j] - yj).max() <= 1e-13 * np.abs(yj).max(), name
+
+
+def test_block_cg_residuals(case):
+    """Every column reaches the stopping test of the one-column CG with a true
+    residual against the oracle's dense operator at the same level; a zero
+    column converges at iteration 0 with x = 0."""
+    name, plan, A = case
+    n = plan.reduced_dof_count
+    B = _block(n, 6, 3)
+    B[:, 2] = 0.0
+    X, infos = g.solve_block(plan, B, relative_tolerance=1e-10, maxiter=100000)
+    X = X.cpu().numpy()
+    for j in range(6):
+        assert infos[j].converged == 1, (name, j)
+        if j == 2:
+            assert infos[j].iterations == 0 and not X[:, j].any()
+            continue
+        res = np.linalg.norm(B[:, j] - A @ X[:, j])
+        assert res <= TRUE_RESIDUAL[name] * np.linalg.norm(B[:, j]), (name, j, res)
+        assert infos[j].final_residual_norm <= 1e-10 * infos[j].rhs_norm
+
+
+def test_block_cg_follows_single_column_cg(gpu):
+    """Away from the attainable-accuracy floor (rtol 1e-8 on a Jacobi-scaled
+    condition number near 1e4) every column takes the one-column CG's
+    iteration count, within one from summation order, and reaches its
+    solution."""
+    plan = g.compile(P.poisson(3, (4, 4, 4), 3))
+    n = plan.reduced_dof_count
+    B = _block(n, 9, 21)
+    solver = g.CGSolver(preconditioner=g.JacobiPreconditioner())
+    X, infos = g.solve_block(plan, B, solver=solver, relative_tolerance=1e-8, maxiter=5000)
+    X = X.cpu().numpy()
+    for j in (0, 4, 8):
+        x1, info1 = g.solve_reduced(plan, solver, B[:, j], relative_tolerance=1e-8, maxiter=5000)
+        x1 = x1.cpu().numpy()
+        assert abs(int(info1.iterations) - int(infos[j].iterations)) <= 1
+        assert np.abs(X[:, j] - x1).max() <= 1e-6 * np.abs(x1).max()
+        assert infos[j].operator_applications == infos[j].iterations
+
+
+def test_fp32_jacobi(case):
+    """The FP32 Jacobi preconditioner inside FP64 CG (one column and a block)
+    still meets the FP64 residual test."""
+    name, plan, A = case
+    n = plan.reduced_dof_count
+    B = _block(n, 3, 7)
+    solver = g.CGSolver(preconditioner=g.JacobiPreconditioner(precision="fp32"))
+    X, infos = g.solve_block(plan, B, solver=solver, relative_tolerance=1e-9, maxiter=100000)
+    X = X.cpu().numpy()
+    x1, info1 = g.solve_reduced(plan, solver, B[:, 0], relative_tolerance=1e-9, maxiter=100000)
+    for x, b in ((X[:, 0], B[:, 0]), (X[:, 2], B[:, 2]), (x1.cpu().numpy(), B[:, 0])):
+        assert np.linalg.norm(b - A @ x) <= (1e-8 + TRUE_RESIDUAL[name]) * np.linalg.norm(b), name
+
+
+def test_block_errors(gpu):
+    plan = g.compile(P.fcm_ball(n=4, degree=2, depth=2))
+    n = plan.reduced_dof_count
+    with pytest.raises(g.ConvergenceError):
+        g.solve_block(plan, _block(n, 2, 1), relative_tolerance=1e-12, maxiter=2)
+    with pytest.raises(ValueError):
+        g.solve_block(plan, np.zeros((n + 1, 2)))
+    with pytest.raises(ValueError):
+        g.JacobiPreconditioner(precision="bf16")
