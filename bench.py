@@ -1103,8 +1103,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the nonlinear Bratu Newton leg")
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
-    ap.add_argument("--fcm", action="store_true",
-                    help="run the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG)")
+    ap.add_argument("--no-fcm", action="store_true",
+                    help="skip the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG)")
     ap.add_argument("--fcm-only", action="store_true", help="run only the finite-cell sphere leg and print its JSON")
     ap.add_argument("--fcm-cells", type=int, default=48, help="background cells per axis of the finite-cell leg")
     ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
@@ -1247,7 +1247,7 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"log-normal leg failed: {exc!r}")
             result["lognormal"] = {"error": repr(exc)}
-    if args.fcm:
+    if not args.no_fcm:
         try:
             result["fcm"] = fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)
         except Exception as exc:  # keep the headline line

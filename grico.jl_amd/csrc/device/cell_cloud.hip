@@ -22,6 +22,7 @@
 // the chunk's channels back onto their side.
 #include <cmath>
 #include <cstdlib>
+#include <string>
 
 #include "common.hpp"
 #include "termmap.hpp"
@@ -37,7 +38,9 @@ struct CloudArgs {
   int N[kMaxSides][kMaxDim];   // modes per axis of each side
   int Ns[kMaxSides];           // box size of each side
   int Ntot, Nmax;              // sum of the box sizes; largest N over sides and axes
-  int chunk;                   // points per chunk = LDS stride of the per-point arrays
+  int chunk;                   // points per chunk = LDS stride of the per-point channel arrays
+  int ts;                      // row stride of the 1D factor tables: chunk + 1, so rows of one
+                               // point column fall in different LDS banks
   const double* jinv;          // items x S*D
   const double* pdata;         // points x (S(D+1))^2
   const int64_t* poff;         // items + 1
@@ -89,7 +92,7 @@ __device__ inline int unflatten(const CloudArgs& a, int i, int (&m)[kMaxDim]) {
 }
 
 // Offset of side s's tables in tv / td ([side][axis][k][lane]).
-__device__ inline int table_base(const CloudArgs& a, int s) { return s * a.D * a.Nmax * a.chunk; }
+__device__ inline int table_base(const CloudArgs& a, int s) { return s * a.D * a.Nmax * a.ts; }
 
 // Basis value phi and reference derivatives dphi[ax] of multi-index m on
 // side s at chunk lane q.
@@ -98,8 +101,8 @@ __device__ inline void basis_at(const CloudArgs& a, const double* tv, const doub
   double f[kMaxDim], fd[kMaxDim];
   const int base = table_base(a, s);
   for (int ax = 0; ax < a.D; ++ax) {
-    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
-    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.ts + q];
+    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.ts + q];
   }
   phi = 1.0;
   for (int ax = 0; ax < a.D; ++ax) phi *= f[ax];
@@ -115,8 +118,8 @@ __device__ inline void basis_at(const CloudArgs& a, const double* tv, const doub
 __device__ inline void point_tables(const CloudArgs& a, int64_t p, int q, double* tv, double* td) {
   for (int s = 0; s < a.S; ++s)
     for (int ax = 0; ax < a.D; ++ax) {
-      const int off = table_base(a, s) + ax * a.Nmax * a.chunk + q;
-      integrated_legendre(a.rpts[p * a.S * a.D + s * a.D + ax], a.N[s][ax], a.chunk, tv + off, td + off);
+      const int off = table_base(a, s) + ax * a.Nmax * a.ts + q;
+      integrated_legendre(a.rpts[p * a.S * a.D + s * a.D + ax], a.N[s][ax], a.ts, tv + off, td + off);
     }
 }
 
@@ -165,8 +168,8 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
   double* u = smem;
   double* r = u + a.Ntot;
   double* tv = r + a.Ntot;
-  double* td = tv + a.S * a.D * a.Nmax * a.chunk;
-  double* ch = td + a.S * a.D * a.Nmax * a.chunk;
+  double* td = tv + a.S * a.D * a.Nmax * a.ts;
+  double* ch = td + a.S * a.D * a.Nmax * a.ts;
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
     u[i] = gather_term(a.map, item, i, x);
     r[i] = 0.0;
@@ -203,8 +206,8 @@ __device__ inline void basis_static(const CloudArgs& a, const double* tv, const 
   const int base = table_base(a, s);
 #pragma unroll
   for (int ax = 0; ax < D; ++ax) {
-    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
-    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.chunk + q];
+    f[ax] = tv[base + (ax * a.Nmax + m[ax]) * a.ts + q];
+    fd[ax] = td[base + (ax * a.Nmax + m[ax]) * a.ts + q];
   }
   phi = 1.0;
 #pragma unroll
@@ -238,8 +241,8 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud_block(CloudArgs 
   double* u = smem;                                  // KC x Ntot
   double* r = u + KC * a.Ntot;                       // KC x Ntot
   double* tv = r + KC * a.Ntot;                      // S x D x Nmax x chunk
-  double* td = tv + a.S * D * a.Nmax * a.chunk;      // S x D x Nmax x chunk
-  double* ch = td + a.S * D * a.Nmax * a.chunk;      // KC x SC x chunk
+  double* td = tv + a.S * D * a.Nmax * a.ts;         // S x D x Nmax x ts
+  double* ch = td + a.S * D * a.Nmax * a.ts;         // KC x SC x chunk
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x)
     for (int j = 0; j < kc; ++j) {
       u[j * a.Ntot + i] = gather_term(a.map, item, i, X + int64_t(j0 + j) * ld);
@@ -344,8 +347,8 @@ __global__ void __launch_bounds__(kCloudLanes) cell_diagonal_cloud(CloudArgs a, 
   const int SC = a.S * C;
   double* r = smem;
   double* tv = r + a.Ntot;
-  double* td = tv + a.S * a.D * a.Nmax * a.chunk;
-  double* Ms = td + a.S * a.D * a.Nmax * a.chunk;  // S x C x C x chunk: diagonal blocks
+  double* td = tv + a.S * a.D * a.Nmax * a.ts;
+  double* Ms = td + a.S * a.D * a.Nmax * a.ts;  // S x C x C x chunk: diagonal blocks
   const double* jinv = a.jinv + item * a.S * a.D;
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] = 0.0;
   __syncthreads();
@@ -382,26 +385,45 @@ __global__ void __launch_bounds__(kCloudLanes) cell_diagonal_cloud(CloudArgs a, 
 }
 
 size_t apply_lds(const CloudArgs& a) {
-  return sizeof(double) * (size_t(2 * a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.chunk) +
+  return sizeof(double) * (size_t(2 * a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.ts) +
                            size_t(a.S * (a.D + 1) * a.chunk));
 }
 
 size_t diagonal_lds(const CloudArgs& a) {
-  return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.chunk) +
+  return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.ts) +
                            size_t(a.S * (a.D + 1) * (a.D + 1) * a.chunk));
 }
 
 template <int KC>
 size_t block_lds(const CloudArgs& a) {
   return sizeof(double) * (size_t(KC) * size_t(2 * a.Ntot + a.S * (a.D + 1) * a.chunk) +
-                           size_t(2 * a.S * a.D * a.Nmax * a.chunk));
+                           size_t(2 * a.S * a.D * a.Nmax * a.ts));
 }
 
-// Largest chunk (64, 32, .., 8 points) whose LDS footprint fits 64 KB.
+// LDS budget per workgroup in KB (GMF_CLOUD_LDS_KB, 8..64; default 64).
+size_t cloud_lds_budget() {
+  static const size_t kb = [] {
+    const char* e = std::getenv("GMF_CLOUD_LDS_KB");
+    const int v = e ? std::atoi(e) : 64;
+    return size_t(v >= 8 && v <= 64 ? v : 64);
+  }();
+  return kb * 1024;
+}
+
+// Largest chunk (64, 32, .., 8 points) whose LDS footprint fits the budget
+// (64 KB at most).
 template <class F>
 void pick_chunk(CloudArgs& a, F lds_of) {
   a.chunk = kCloudLanes;
-  while (a.chunk > 8 && lds_of(a) > 64 * 1024) a.chunk /= 2;
+  a.ts = a.chunk + 1;
+  while (a.chunk > 8 && lds_of(a) > cloud_lds_budget()) {
+    a.chunk /= 2;
+    a.ts = a.chunk + 1;
+  }
+  while (a.chunk > 8 && lds_of(a) > 64 * 1024) {
+    a.chunk /= 2;
+    a.ts = a.chunk + 1;
+  }
   require(lds_of(a) <= 64 * 1024, GMF_EUNSUPPORTED, "point-cloud batch exceeds the LDS budget");
 }
 
@@ -430,13 +452,25 @@ CloudArgs make_cloud_args(const Bucket& b, bool reduced) {
   return a;
 }
 
-// Columns per workgroup of the block kernel (GMF_CLOUD_COLUMNS=4 selects 4).
+// Columns per workgroup of the block kernel: 4 (higher occupancy, measured
+// faster on the finite-cell sphere) or 8 (GMF_CLOUD_COLUMNS=8).
 int cloud_block_columns() {
   static const int cols = [] {
     const char* e = std::getenv("GMF_CLOUD_COLUMNS");
-    return (e && std::atoi(e) == 4) ? 4 : 8;
+    return (e && std::atoi(e) == 8) ? 8 : 4;
   }();
   return cols;
+}
+
+// One-column applies run the dimension-specialized kernel with one column
+// (registers instead of scratch for the per-point arrays);
+// GMF_CLOUD_KERNEL=item selects the dimension-generic cell_apply_cloud.
+bool cloud_generic_kernel() {
+  static const bool generic = [] {
+    const char* e = std::getenv("GMF_CLOUD_KERNEL");
+    return e && std::string(e) == "item";
+  }();
+  return generic;
 }
 
 template <int KC>
@@ -471,6 +505,10 @@ void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double
   if (b.cells == 0) return;
   CloudArgs a = make_cloud_args(b, reduced);
   require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  if (!cloud_generic_kernel()) {
+    launch_block<1>(b, x, y, 0, 1, reduced, stream);
+    return;
+  }
   pick_chunk(a, apply_lds);
   hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), apply_lds(a), stream, a, x, y);
   GMF_HIP(hipGetLastError());
