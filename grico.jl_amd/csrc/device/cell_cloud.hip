@@ -227,7 +227,7 @@ __device__ inline void basis_static(const CloudArgs& a, const double* tv, const 
 // points and the point action serve KC columns, while every column runs the
 // arithmetic of cell_apply_cloud in the same order, so each item's local
 // result per column equals the one-column kernel's.
-template <int KC, int D>
+template <int KC, int D, int NF>
 __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud_block(CloudArgs a, const double* __restrict__ X,
                                                                       double* __restrict__ Y, int64_t ld, int k) {
   extern __shared__ double smem[];
@@ -256,6 +256,57 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud_block(CloudArgs 
     if (q < cnt) {
       // Each lane writes and reads only its own point's column of tv / td.
       point_tables(a, p0 + q, q, tv, td);
+      if constexpr (NF > 0) {
+        // One side, 3D, NF modes per axis: the point's 1D factors stay in
+        // registers; the products and the summation order are those of
+        // basis_static over the modes in unflatten order.
+        static_assert(D == 3, "register forward is 3D");
+        double f[3][NF], fd[3][NF];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+          for (int kk = 0; kk < NF; ++kk) {
+            f[ax][kk] = tv[(ax * a.Nmax + kk) * a.ts + q];
+            fd[ax][kk] = td[(ax * a.Nmax + kk) * a.ts + q];
+          }
+        double val[KC], g[KC][3];
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+          val[j] = 0.0;
+          g[j][0] = g[j][1] = g[j][2] = 0.0;
+        }
+#pragma unroll
+        for (int i2 = 0; i2 < NF; ++i2)
+#pragma unroll
+          for (int i1 = 0; i1 < NF; ++i1)
+#pragma unroll
+            for (int i0 = 0; i0 < NF; ++i0) {
+              const double phi = f[0][i0] * f[1][i1] * f[2][i2];
+              const double d0 = fd[0][i0] * f[1][i1] * f[2][i2];
+              const double d1 = fd[1][i1] * f[0][i0] * f[2][i2];
+              const double d2 = fd[2][i2] * f[0][i0] * f[1][i1];
+              const int i = i0 + NF * (i1 + NF * i2);
+#pragma unroll
+              for (int j = 0; j < KC; ++j) {
+                if (j < kc) {
+                  const double ui = u[j * a.Ntot + i];
+                  val[j] = fma(ui, phi, val[j]);
+                  g[j][0] = fma(ui, d0, g[j][0]);
+                  g[j][1] = fma(ui, d1, g[j][1]);
+                  g[j][2] = fma(ui, d2, g[j][2]);
+                }
+              }
+            }
+#pragma unroll
+        for (int j = 0; j < KC; ++j) {
+          if (j < kc) {
+            double* cj = ch + j * SC * a.chunk + q;
+            cj[0] = val[j];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) cj[(1 + ax) * a.chunk] = jinv[ax] * g[j][ax];
+          }
+        }
+      } else
       for (int s = 0; s < a.S; ++s) {
         double val[KC], g[KC][D];
 #pragma unroll
@@ -452,12 +503,14 @@ CloudArgs make_cloud_args(const Bucket& b, bool reduced) {
   return a;
 }
 
-// Columns per workgroup of the block kernel: 4 (higher occupancy, measured
-// faster on the finite-cell sphere) or 8 (GMF_CLOUD_COLUMNS=8).
+// Columns per workgroup of the block kernel: 2 by default (measured on the
+// finite-cell sphere: 2 columns 4.46 ms, 4 columns 4.73 ms, 8 columns 8.21 ms
+// per column), 4 or 8 with GMF_CLOUD_COLUMNS.
 int cloud_block_columns() {
   static const int cols = [] {
     const char* e = std::getenv("GMF_CLOUD_COLUMNS");
-    return (e && std::atoi(e) == 8) ? 8 : 4;
+    const int v = e ? std::atoi(e) : 2;
+    return v == 4 || v == 8 ? v : 2;
   }();
   return cols;
 }
@@ -480,10 +533,18 @@ void launch_block(const Bucket& b, const double* X, double* Y, int64_t ld, int k
   pick_chunk(a, block_lds<KC>);
   const dim3 grid(unsigned(b.units), unsigned((k + KC - 1) / KC));
   const size_t lds = block_lds<KC>(a);
-  switch (a.D) {
-    case 1: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 1>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
-    case 2: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 2>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
-    default: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k);
+  // One-sided 3D items with the same mode count on every axis take the
+  // register forward (N = 2..5, degrees 1..4).
+  const bool iso = a.S == 1 && a.D == 3 && a.N[0][0] == a.N[0][1] && a.N[0][0] == a.N[0][2];
+  const int nf = iso && a.N[0][0] >= 2 && a.N[0][0] <= 5 ? a.N[0][0] : 0;
+  switch (a.D * 8 + nf) {
+    case 8: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 1, 0>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 16: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 2, 0>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 26: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3, 2>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 27: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3, 3>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 28: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3, 4>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    case 29: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3, 5>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k); break;
+    default: hipLaunchKernelGGL((cell_apply_cloud_block<KC, 3, 0>), grid, dim3(kCloudLanes), lds, stream, a, X, Y, ld, k);
   }
   GMF_HIP(hipGetLastError());
 }
@@ -493,11 +554,12 @@ void launch_block(const Bucket& b, const double* X, double* Y, int64_t ld, int k
 void launch_cell_cloud_block(const gmf_plan&, const Bucket& b, const double* X, double* Y, int64_t ld, int k,
                              bool reduced, hipStream_t stream) {
   if (b.cells == 0 || k <= 0) return;
-  require(k <= 65535 * 4, GMF_EINVAL, "too many columns for one block launch");
-  if (cloud_block_columns() == 4)
-    launch_block<4>(b, X, Y, ld, k, reduced, stream);
-  else
-    launch_block<8>(b, X, Y, ld, k, reduced, stream);
+  require(k <= 65535 * 2, GMF_EINVAL, "too many columns for one block launch");
+  switch (cloud_block_columns()) {
+    case 4: launch_block<4>(b, X, Y, ld, k, reduced, stream); break;
+    case 8: launch_block<8>(b, X, Y, ld, k, reduced, stream); break;
+    default: launch_block<2>(b, X, Y, ld, k, reduced, stream);
+  }
 }
 
 void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,

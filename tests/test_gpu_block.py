@@ -60,9 +60,24 @@ def case(request, gpu):
 TRUE_RESIDUAL = {"adaptive_hp": 1e-9, "sipg_2d": 1e-9, "fcm_ball_p2": 1e-8, "fcm_ball_p3": 2e-6}
 
 
+# Further point-cloud degrees for the apply only (the register forward of the
+# point-cloud kernels is specialized per mode count, N = 2 .. 5).
+APPLY_ONLY = {
+    "fcm_ball_p1": lambda: P.fcm_ball(n=4, degree=1, depth=2),
+    "fcm_ball_p4": lambda: P.fcm_ball(n=2, degree=4, depth=1, n_theta=12, n_phi=24),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(CASES) + sorted(APPLY_ONLY))
+def apply_case(request, gpu):
+    print("case", request.param, flush=True)
+    plan = g.compile({**CASES, **APPLY_ONLY}[request.param]())
+    return request.param, plan, _Operator(plan)
+
+
 @pytest.mark.parametrize("k", [1, 5, 8, 19])
-def test_block_apply(case, k):
-    name, plan, A = case
+def test_block_apply(apply_case, k):
+    name, plan, A = apply_case
     X = _block(plan.reduced_dof_count, k, 10 + k)
     Y = g.reduced_apply_block(plan, X).cpu().numpy()
     ref = A @ X
