@@ -7,7 +7,8 @@ SPD, TestChannels, TraceTestChannels, add_cell_accumulator,
 add_boundary_accumulator, add_interface_accumulator, jump, average, add_constraint, Dirichlet,
 BoundaryFace, LOWER, UPPER, compile, apply, solve, CGSolver, FGMRESSolver,
 JacobiPreconditioner, GeometricMultigridPreconditioner, l2_error,
-relative_l2_error. Julia's `!` mutators map to `*_into` functions.
+relative_l2_error, AdaptivityPlan, transition, transfer_state. Julia's `!`
+mutators map to `*_into` functions (request_h_refinement! -> request_h_refinement).
 """
 
 from .mesh import Domain, uniform_refine
@@ -26,5 +27,7 @@ from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, 
                     solve_reduced)
 from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid
+from .adaptivity import (AdaptivityLimits, AdaptivityPlan, SpaceTransition, evaluate, transfer_state,
+                         transition)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

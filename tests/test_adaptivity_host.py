@@ -1,0 +1,149 @@
+"""Manual adaptivity, transitions and state transfer on the host side.
+
+Reference: src/adaptivity.jl (request_h_refinement! 922-940: children inherit
+degrees; request_p_refinement!/request_p_derefinement! 459-493, 972-1017:
+max_p / min_p checks), src/transition.jl:349-387, src/transfer.jl:892-1000
+(L2 projection onto the target space). Pins: the vectorized basis against the
+native KAT-pinned basis routine, point evaluation against the plan's own
+sum-factorized interpolation, exact reproduction of the old field when the
+target space contains the source space (nested spaces), and the Legendre
+truncation closed form for p-derefinement on a DG space. The projection is
+solved densely here with the oracle matrix; the device solve is checked in
+test_gpu_transfer.py.
+"""
+
+import numpy as np
+import pytest
+
+import gricoamd as g
+from gricoamd import adaptivity as ad
+from gricoamd.quadrature import fe_basis_values_and_derivatives
+from gricoamd.verification import _field_values
+import pyoracle
+
+
+class _Mass:
+    def cell_accumulate(self, q, trial, test_component):
+        return g.TestChannels(trial.value, tuple(0.0 * x for x in trial.gradient))
+
+
+def hanging_space(degree=2, continuity="cg", dim=2):
+    dom = g.Domain((0.0,) * dim, (1.0,) * dim, (2,) * dim)
+    first = dom.refine(0, 0)
+    dom.refine(first, 1 % dim)
+    return g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree),
+                                         continuity=continuity))
+
+
+def state_on(space, coefficients):
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, _Mass())
+    plan = g.compile(problem)
+    return g.State(plan, coefficients)
+
+
+def dense_projection(transition, state):
+    plan = g.compile(ad.projection_problem(transition, state))
+    A = pyoracle.reduced_matrix(plan)
+    b = plan.rhs_physical()[plan.solve_dofs]
+    full = np.zeros(plan.full_dof_count)
+    full[plan.solve_dofs] = np.linalg.solve(A, b)
+    return g.State(plan, full)
+
+
+def random_points(rng, n, dim):
+    return tuple(rng.uniform(0.0, 1.0, n) for _ in range(dim))
+
+
+def test_integrated_legendre_matches_native_basis():
+    x = np.random.default_rng(3).uniform(-1, 1, 17)
+    for p in range(0, 9):
+        ref, _ = fe_basis_values_and_derivatives(x, p)
+        np.testing.assert_allclose(ad.integrated_legendre(x, p), ref, rtol=0, atol=1e-14)
+
+
+def test_copy_domain_and_locate():
+    space = hanging_space()
+    dom = space.domain
+    copy = ad.copy_domain(dom)
+    assert copy.grid.cell_count() == dom.grid.cell_count()
+    np.testing.assert_array_equal(copy.grid.coords(), dom.grid.coords())
+    np.testing.assert_array_equal(copy.grid.levels(), dom.grid.levels())
+    leaves = dom.active_leaves()
+    centers, halves = dom.cell_centers_and_half_sizes(leaves)
+    for shift in (-0.9, 0.0, 0.9):
+        pts = tuple(centers[:, a] + shift * halves[:, a] for a in range(2))
+        np.testing.assert_array_equal(ad.locate(dom, pts), leaves)
+
+
+def test_plan_requests_and_limits():
+    space = hanging_space()
+    plan = ad.AdaptivityPlan(space, ad.AdaptivityLimits(2, max_h_level=3, max_p=3))
+    assert plan.is_empty()
+    leaf = int(space.leaves[-1])
+    first = plan.request_h_refinement(leaf, 0)
+    assert plan.cell_degrees(first) == plan.cell_degrees(first + 1) == (2, 2)
+    assert not plan.is_empty()
+    with pytest.raises(ValueError):
+        plan.cell_degrees(leaf)  # no longer active
+    plan.request_p_refinement(first, axis=1)
+    assert plan.cell_degrees(first) == (2, 3)
+    with pytest.raises(ValueError):
+        plan.request_p_refinement(first, axis=1)  # max_p
+    plan.request_p_derefinement(first + 1)
+    assert plan.cell_degrees(first + 1) == (1, 1)
+    with pytest.raises(ValueError):
+        plan.request_p_derefinement(first + 1)  # min_p
+    with pytest.raises(ValueError):
+        plan.request_h_refinement(first, 2)  # axis out of range
+    t = ad.transition(plan)
+    assert t.source_leaves(first) == [leaf]
+    assert t.target_space.active_leaf_count() == space.active_leaf_count() + 1
+
+
+def test_evaluate_matches_plan_interpolation():
+    space = hanging_space(degree=3)
+    rng = np.random.default_rng(5)
+    state = state_on(space, rng.standard_normal(space.scalar_dof_count()))
+    for b in state.plan.buckets:
+        ref = _field_values(state.plan, state.coefficients, b)
+        got = ad.evaluate(state, tuple(b.phys_points))
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("continuity", ["cg", "dg"])
+def test_refinement_transfer_reproduces_field(continuity):
+    space = hanging_space(degree=2, continuity=continuity)
+    rng = np.random.default_rng(11)
+    state = state_on(space, rng.standard_normal(space.scalar_dof_count()))
+    plan = ad.AdaptivityPlan(space)
+    leaves = [int(l) for l in space.leaves]
+    plan.request_h_refinement(leaves[0], 1)
+    plan.request_h_refinement(leaves[-1], 0)
+    plan.request_p_refinement(leaves[2], increment=2)
+    t = ad.transition(plan)
+    new = dense_projection(t, state)
+    pts = random_points(rng, 400, 2)
+    np.testing.assert_allclose(ad.evaluate(new, pts), ad.evaluate(state, pts), rtol=0, atol=1e-10)
+
+
+def test_dg_p_derefinement_is_legendre_truncation():
+    """psi_2 = (P_2 - P_0)/sqrt(6) on every cell: its L2 projection on degree 1
+    is the constant -1/sqrt(6) (P_2 is orthogonal to P_0 and P_1)."""
+    space = hanging_space(degree=2, continuity="dg")
+    state = state_on(space, np.zeros(space.scalar_dof_count()))
+    coeffs = np.zeros(space.scalar_dof_count())
+    for b in state.plan.buckets:
+        box = np.zeros((b.cell_count, b.n_modes))
+        box[:, 2] = 1.0  # mode (2, 0): psi_2 along axis 1, psi_0 + psi_1 = 1 needs both
+        box[:, 2 + b.modes[0]] = 1.0  # mode (2, 1)
+        # DG: every (cell, box) slot is one dof with coefficient 1
+        coeffs[b.term_dof] = box[b.term_cell, b.term_box]
+    state = g.State(state.plan, coeffs)
+    plan = ad.AdaptivityPlan(space)
+    for l in space.leaves:
+        plan.request_p_derefinement(int(l))
+    new = dense_projection(ad.transition(plan), state)
+    pts = random_points(np.random.default_rng(2), 300, 2)
+    np.testing.assert_allclose(ad.evaluate(new, pts), -1.0 / np.sqrt(6.0), rtol=0, atol=1e-12)
