@@ -13,9 +13,22 @@ import pytest
 import gricoamd as g
 from gricoamd import adaptivity as ad
 
+import pyoracle
 from test_adaptivity_host import dense_projection, hanging_space, random_points, state_on
 
 pytestmark = pytest.mark.gpu
+
+RTOL = 1e-13
+
+
+def _bound(plan):
+    """Coefficient error allowed by the CG stopping rule ||r|| <= RTOL ||b||:
+    RTOL times the condition number of the Jacobi-scaled mass matrix (up to
+    4.8e5 for the 3D hanging p = 2 space in the integrated Legendre basis)."""
+    A = pyoracle.reduced_matrix(plan)
+    d = 1.0 / np.sqrt(np.diag(A))
+    ev = np.linalg.eigvalsh(d[:, None] * A * d[None, :])
+    return RTOL * ev[-1] / ev[0]
 
 
 def _refined_plan(space):
@@ -34,12 +47,13 @@ def test_device_transfer_refinement(gpu, dim, degree, continuity):
     state = state_on(space, rng.standard_normal(space.scalar_dof_count()))
     t = ad.transition(_refined_plan(space))
     print(f"transfer refinement dim={dim} p={degree} {continuity}", flush=True)
-    new = ad.transfer_state(t, state, relative_tolerance=1e-13)
+    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
     ref = dense_projection(t, state)
     scale = np.max(np.abs(ref.coefficients))
-    assert np.max(np.abs(new.coefficients - ref.coefficients)) <= 1e-10 * scale
+    tol = _bound(ref.plan) * scale
+    assert np.max(np.abs(new.coefficients - ref.coefficients)) <= tol
     pts = random_points(rng, 300, dim)
-    np.testing.assert_allclose(ad.evaluate(new, pts), ad.evaluate(state, pts), rtol=0, atol=1e-9 * scale)
+    np.testing.assert_allclose(ad.evaluate(new, pts), ad.evaluate(state, pts), rtol=0, atol=tol)
 
 
 def test_device_transfer_p_derefinement(gpu):
@@ -51,7 +65,7 @@ def test_device_transfer_p_derefinement(gpu):
         plan.request_p_derefinement(int(l), decrement=2)
     t = ad.transition(plan)
     print("transfer p-derefinement 3 -> 1", flush=True)
-    new = ad.transfer_state(t, state, relative_tolerance=1e-13)
+    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
     ref = dense_projection(t, state)
     scale = np.max(np.abs(ref.coefficients))
-    assert np.max(np.abs(new.coefficients - ref.coefficients)) <= 1e-10 * scale
+    assert np.max(np.abs(new.coefficients - ref.coefficients)) <= _bound(ref.plan) * scale
