@@ -10,7 +10,8 @@ old field on the target space, with the projection's load integrated over
 source/target overlaps).
 
 Here the target grid is a copy of the source grid with the requested splits
-applied, so every target leaf lies inside exactly one source leaf. The
+applied; a merged leaf keeps its children in the grid and is listed as a leaf
+of the target space. The
 projection runs as a device solve: the target mass operator (value channel
 only, no constraints) is compiled like any other cell accumulator, the load
 u_old(x_q) is evaluated on the host at the target quadrature points (located
@@ -20,15 +21,19 @@ max(p_source, p_target) + 1 Gauss points per axis, which integrates the mass
 matrix and the load exactly, so a target space that contains the source space
 reproduces the old field up to the solve tolerance.
 
-Not built: h-derefinement (a target cell covering several source leaves needs
-its load integrated leaf by leaf, src/transfer.jl:22-25) and the automatic
-indicator-driven planning of src/indicators.jl.
+A merged (h-derefined) target leaf covers several source leaves; its cell
+takes a composite rule, one Gauss rule per covered source leaf, so the
+piecewise polynomial old field is integrated leaf by leaf as in
+src/transfer.jl:22-25, and the cell runs on the point-cloud kernel.
+
+Not built: the automatic indicator-driven planning of src/indicators.jl.
 """
 
 import numpy as np
 
 from .mesh import Domain
-from .problem import AffineProblem, ScalarField, SPD, TestChannels
+from .problem import AffineProblem, PointQuadrature, ScalarField, SPD, TestChannels
+from .quadrature import gauss_legendre_rule
 from .space import DegreePlusQuadrature, HpSpace, SpaceOptions
 
 
@@ -102,17 +107,46 @@ class AdaptivityPlan:
 
     def request_h_refinement(self, leaf, axis):
         """Split an active target leaf along `axis`; both children inherit its
-        degrees (src/adaptivity.jl:889-940). Returns the first child."""
+        degrees (src/adaptivity.jl:889-940). Returns the first child. A cell
+        that the grid already splits along `axis` (a leaf merged earlier)
+        gets its existing children back."""
         leaf = self._checked_leaf(leaf)
         axis = self._checked_axis(axis)
-        level = int(self.target_domain.grid.levels()[leaf, axis])
+        grid = self.target_domain.grid
+        level = int(grid.levels()[leaf, axis])
         if level + 1 > self.limits.max_h_level[axis]:
             raise ValueError(f"leaf {leaf} reached max_h_level[{axis}]")
+        first = int(grid.first_child()[leaf])
+        if first >= 0 and int(grid.split_axis()[leaf]) != axis:
+            raise NotImplementedError(f"cell {leaf} is already split along axis {int(grid.split_axis()[leaf])}")
         deg = self.degrees.pop(leaf)
-        first = self.target_domain.refine(leaf, axis)
+        if first < 0:
+            first = self.target_domain.refine(leaf, axis)
         self.degrees[first] = deg
         self.degrees[first + 1] = deg
         return first
+
+    def request_h_derefinement(self, cell, axis):
+        """Merge the two active children of `cell` split along `axis` back into
+        one leaf; it takes the componentwise maximum of the child degrees
+        (src/adaptivity.jl:942-970)."""
+        cell = int(cell)
+        axis = self._checked_axis(axis)
+        grid = self.target_domain.grid
+        if not 0 <= cell < grid.cell_count():
+            raise ValueError(f"cell {cell} out of range")
+        first = int(grid.first_child()[cell])
+        if first < 0 or int(grid.split_axis()[cell]) != axis:
+            raise ValueError(f"cell {cell} is not split along axis {axis}")
+        children = (first, first + 1)
+        for c in children:
+            if c not in self.degrees:
+                raise ValueError(f"child {c} of cell {cell} is not an active target leaf")
+        merged = tuple(max(a, b) for a, b in zip(self.degrees[children[0]], self.degrees[children[1]]))
+        for c in children:
+            del self.degrees[c]
+        self.degrees[cell] = merged
+        return self
 
     def request_p_refinement(self, leaf, axis=None, increment=1):
         """Raise the degree of one leaf along `axis` (all axes if None) by
@@ -140,21 +174,22 @@ class AdaptivityPlan:
         return self
 
     def target_leaves(self):
-        active = self.target_domain.active_leaves()
-        return active[np.isin(active, np.fromiter(self.degrees, dtype=np.int64))]
+        return np.asarray(sorted(self.degrees), dtype=np.int64)
 
 
 class SpaceTransition:
     """Source space, target space and the source leaf holding each target
     leaf (src/transition.jl:349-387, source_leaves 143)."""
 
-    def __init__(self, source_space, target_space, source_leaf_of_target):
+    def __init__(self, source_space, target_space, source_leaves_of_target):
         self.source_space = source_space
         self.target_space = target_space
-        self.source_leaf_of_target = source_leaf_of_target
+        self.source_leaves_of_target = source_leaves_of_target
 
     def source_leaves(self, target_leaf):
-        return [int(self.source_leaf_of_target[int(target_leaf)])]
+        """Source leaves overlapping one target leaf: its ancestor-or-self
+        source leaf, or the source leaves below a merged cell."""
+        return list(self.source_leaves_of_target[int(target_leaf)])
 
 
 def transition(plan):
@@ -166,17 +201,29 @@ def transition(plan):
     options = SpaceOptions(basis=src.options.basis, degree=src.options.degree,
                            quadrature=src.options.quadrature, continuity=src.options.continuity)
     target = HpSpace(plan.target_domain, options, leaves=leaves, degrees=degrees)
-    # Parent walk in the target grid until a source leaf is reached.
+    # Ancestor-or-self walk in the target grid; merged cells collect the
+    # source leaves below them.
     parent = plan.target_domain.grid.parent()
+    first_child = plan.target_domain.grid.first_child()
     source_set = set(int(l) for l in src.leaves)
     owner = {}
     for l in leaves:
         c = int(l)
-        while c not in source_set:
+        while c >= 0 and c not in source_set:
             c = int(parent[c])
-            if c < 0:
-                raise RuntimeError(f"target leaf {int(l)} has no source leaf")
-        owner[int(l)] = c
+        if c >= 0:
+            owner[int(l)] = (c,)
+            continue
+        found, stack = [], [int(l)]
+        while stack:
+            c = stack.pop()
+            if c in source_set:
+                found.append(c)
+            elif first_child[c] >= 0:
+                stack.extend((int(first_child[c]) + 1, int(first_child[c])))
+            else:
+                raise RuntimeError(f"target leaf {int(l)} is not covered by source leaves")
+        owner[int(l)] = tuple(found)
     return SpaceTransition(src, target, owner)
 
 
@@ -202,9 +249,10 @@ def integrated_legendre(x, degree):
     return out
 
 
-def locate(domain, points):
-    """Active leaf of `domain` containing each point (tuple of D arrays of one
-    shape), by descending the refinement tree from the root cell."""
+def locate(domain, points, leaves=None):
+    """Leaf containing each point (tuple of D arrays of one shape), by
+    descending the refinement tree from the root cell; the descent stops at
+    the cells listed in `leaves` (default: the grid's active leaves)."""
     D = domain.dim
     pts = [np.asarray(p, dtype=np.float64).reshape(-1) for p in points]
     grid = domain.grid
@@ -222,11 +270,17 @@ def locate(domain, points):
         i = np.clip(np.floor((pts[a] - domain.origin[a]) / h).astype(np.int64), 0, counts[a] - 1)
         idx = idx * counts[a] + i
     cell = root_of[idx]
+    stop = fc < 0
+    if leaves is not None:
+        stop = np.zeros(fc.size, dtype=bool)
+        stop[np.asarray(leaves, dtype=np.int64)] = True
     while True:
-        inner = np.nonzero(fc[cell] >= 0)[0]
+        inner = np.nonzero(~stop[cell])[0]
         if inner.size == 0:
             return cell
         c = cell[inner]
+        if np.any(fc[c] < 0):
+            raise ValueError("point outside the leaves of the space")
         a = split[c]
         k = np.arange(c.size)
         mid = domain.cell_lowers(c)[k, a] + 0.5 * domain.cell_sizes(c)[k, a]
@@ -247,10 +301,11 @@ def evaluate(state, points):
     D = dom.dim
     shape = np.shape(points[0])
     pts = [np.asarray(p, dtype=np.float64).reshape(-1) for p in points]
-    leaf = locate(dom, pts)
+    leaf = locate(dom, pts, space.leaves)
     out = np.zeros(pts[0].size)
     row_of = np.full(dom.grid.cell_count(), -1, dtype=np.int64)
-    for b in plan.buckets:
+    cells = [cb for cb in getattr(plan, "cloud_buckets", []) if not getattr(cb, "surface", False)]
+    for b in list(plan.buckets) + cells:
         box = np.zeros((b.cell_count, b.n_modes))
         np.add.at(box, (b.term_cell, b.term_box), b.term_coef * state.coefficients[b.term_dof])
         row_of[:] = -1
@@ -269,8 +324,6 @@ def evaluate(state, points):
             # contract the last (axis a) index
             acc = np.einsum("n...i,ni->n...", acc, psi)
         out[sel] = acc
-    if getattr(plan, "cloud_buckets", None):
-        raise NotImplementedError("evaluation of point-cloud cells is not supported")
     return out.reshape(shape)
 
 
@@ -295,9 +348,14 @@ def projection_problem(transition, state):
     if getattr(state.plan, "space", None) is not src:
         raise ValueError("state does not live on the transition's source space")
     extra = 1
+    merged = []
     for l in tgt.leaves:
-        s = transition.source_leaf_of_target[int(l)]
-        drop = np.asarray(src.degrees[src.leaf_index[s]]) - np.asarray(tgt.degrees[tgt.leaf_index[int(l)]])
+        pt = np.asarray(tgt.degrees[tgt.leaf_index[int(l)]])
+        covered = transition.source_leaves(l)
+        if len(covered) > 1 or covered[0] != int(l) and _is_descendant(tgt.domain, covered[0], int(l)):
+            merged.append((int(l), covered))
+            continue
+        drop = np.asarray(src.degrees[src.leaf_index[covered[0]]]) - pt
         extra = max(extra, 1 + int(drop.max()))
     options = SpaceOptions(basis=tgt.options.basis, degree=tgt.options.degree,
                            quadrature=DegreePlusQuadrature(extra), continuity=tgt.options.continuity)
@@ -306,7 +364,45 @@ def projection_problem(transition, state):
     problem = AffineProblem(u, SPD())
     problem.add_cell_accumulator(u, u, _MassOperator())
     problem.add_cell_accumulator(u, _TransferLoad(state))
+    for leaf, covered in merged:
+        problem.add_cell_quadrature(leaf, composite_rule(tgt, src, leaf, covered))
     return problem
+
+
+def _is_descendant(domain, cell, ancestor):
+    parent = domain.grid.parent()
+    while cell >= 0:
+        if cell == ancestor:
+            return True
+        cell = int(parent[cell])
+    return False
+
+
+def composite_rule(target_space, source_space, leaf, covered):
+    """Quadrature on the reference cell of a merged target leaf made of one
+    Gauss rule per covered source leaf, max(p_source, p_target) + 1 points per
+    axis mapped into that leaf's sub-box: exact for the target mass matrix and
+    for the load of the piecewise polynomial old field (src/transfer.jl:22-25)."""
+    dom = target_space.domain
+    D = dom.dim
+    pt = target_space.degrees[target_space.leaf_index[leaf]]
+    lo_t = dom.cell_lowers([leaf])[0]
+    size_t = dom.cell_sizes([leaf])[0]
+    points, weights = [], []
+    for s in covered:
+        ps = source_space.degrees[source_space.leaf_index[s]]
+        lo = (dom.cell_lowers([s])[0] - lo_t) / size_t * 2.0 - 1.0
+        hi = lo + dom.cell_sizes([s])[0] / size_t * 2.0
+        axes = []
+        for a in range(D):
+            x, w = gauss_legendre_rule(max(int(ps[a]), int(pt[a])) + 1)
+            axes.append((lo[a] + 0.5 * (x + 1.0) * (hi[a] - lo[a]), 0.5 * w * (hi[a] - lo[a])))
+        # axis 1 fastest
+        grids = np.meshgrid(*[ax[0] for ax in axes][::-1], indexing="ij")
+        wgrid = np.meshgrid(*[ax[1] for ax in axes][::-1], indexing="ij")
+        points.append(np.stack([g.reshape(-1) for g in grids[::-1]], axis=1))
+        weights.append(np.prod(np.stack([w.reshape(-1) for w in wgrid]), axis=0))
+    return PointQuadrature(np.concatenate(points), np.concatenate(weights))
 
 
 def transfer_state(transition, state, relative_tolerance=1e-13, maxiter=None):

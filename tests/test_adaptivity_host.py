@@ -17,7 +17,7 @@ import pytest
 
 import gricoamd as g
 from gricoamd import adaptivity as ad
-from gricoamd.quadrature import fe_basis_values_and_derivatives
+from gricoamd.quadrature import fe_basis_values_and_derivatives, gauss_legendre_rule
 from gricoamd.verification import _field_values
 import pyoracle
 
@@ -147,3 +147,85 @@ def test_dg_p_derefinement_is_legendre_truncation():
     new = dense_projection(ad.transition(plan), state)
     pts = random_points(np.random.default_rng(2), 300, 2)
     np.testing.assert_allclose(ad.evaluate(new, pts), -1.0 / np.sqrt(6.0), rtol=0, atol=1e-12)
+
+
+def _derefined_plan(space):
+    """Merge the refined corner of hanging_space back to the uniform 2^D roots."""
+    dom = space.domain
+    fc = dom.grid.first_child()
+    first = int(fc[0])
+    plan = ad.AdaptivityPlan(space)
+    plan.request_p_refinement(first + 1, axis=0)
+    plan.request_h_derefinement(first, 1 % space.dim)
+    plan.request_h_derefinement(0, 0)
+    return plan, first
+
+
+def test_h_derefinement_requests():
+    space = hanging_space()
+    plan, first = _derefined_plan(space)
+    assert plan.cell_degrees(0) == (3, 2)  # componentwise maximum of the children
+    with pytest.raises(ValueError):
+        plan.cell_degrees(first)
+    with pytest.raises(ValueError):
+        plan.request_h_derefinement(1, 0)  # root 1 is not split
+    t = ad.transition(plan)
+    assert sorted(t.source_leaves(0)) == sorted(int(l) for l in space.leaves if l not in (1, 2, 3))
+    assert t.target_space.scalar_dof_count() == 25 + 2  # 2x2 roots at p=2 plus the p=3 modes of leaf 0
+    # refining a merged leaf again reuses the grid's children
+    again = ad.AdaptivityPlan(t.target_space)
+    assert again.request_h_refinement(0, 0) == first
+
+
+@pytest.mark.parametrize("continuity", ["cg", "dg"])
+def test_derefine_then_refine_round_trip(continuity):
+    """coarse -> fine is exact (nested spaces); fine -> coarse must then give
+    back the coarse field: the merged cells' composite rules integrate the
+    piecewise field leaf by leaf."""
+    fine = hanging_space(degree=2, continuity=continuity)
+    plan, first = _derefined_plan(fine)
+    coarse = ad.transition(plan).target_space
+    rng = np.random.default_rng(29)
+    c_state = state_on(coarse, rng.standard_normal(coarse.scalar_dof_count()))
+    up = ad.AdaptivityPlan(coarse)
+    up.request_h_refinement(0, 0)
+    up.request_h_refinement(first, 1)
+    to_fine = ad.transition(up)
+    f_state = dense_projection(to_fine, c_state)
+    pts = random_points(rng, 300, 2)
+    np.testing.assert_allclose(ad.evaluate(f_state, pts), ad.evaluate(c_state, pts), rtol=0, atol=1e-10)
+    down = ad.AdaptivityPlan(f_state.plan.space)
+    down.request_h_derefinement(first, 1)
+    down.request_h_derefinement(0, 0)
+    back = dense_projection(ad.transition(down), f_state)
+    np.testing.assert_allclose(ad.evaluate(back, pts), ad.evaluate(c_state, pts), rtol=0, atol=1e-10)
+
+
+def test_derefinement_projection_is_l2_optimal():
+    """The projection of a genuinely piecewise field onto the merged space
+    leaves an error orthogonal to the target space: perturbing the result by
+    any target-space field only increases the L2 distance."""
+    fine = hanging_space(degree=2, continuity="dg")
+    rng = np.random.default_rng(31)
+    f_state = state_on(fine, rng.standard_normal(fine.scalar_dof_count()))
+    plan, _ = _derefined_plan(fine)
+    t = ad.transition(plan)
+    proj = dense_projection(t, f_state)
+    # L2 distance by a 6-point Gauss rule per axis on every fine cell (exact
+    # for the squared difference of the piecewise degree <= 3 fields)
+    x, wx = gauss_legendre_rule(6)
+    dom = fine.domain
+    centers, halves = dom.cell_centers_and_half_sizes(fine.leaves)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    WX = np.outer(wx, wx).reshape(-1)
+    pts = (np.concatenate([c[0] + h[0] * X.reshape(-1) for c, h in zip(centers, halves)]),
+           np.concatenate([c[1] + h[1] * Y.reshape(-1) for c, h in zip(centers, halves)]))
+    w = np.concatenate([WX * h[0] * h[1] for h in halves])
+    u_old = ad.evaluate(f_state, pts)
+
+    def dist(coeffs):
+        return float(np.sum(w * (ad.evaluate(g.State(proj.plan, coeffs), pts) - u_old) ** 2))
+
+    best = dist(proj.coefficients)
+    for _ in range(5):
+        assert dist(proj.coefficients + 1e-3 * rng.standard_normal(proj.coefficients.size)) > best

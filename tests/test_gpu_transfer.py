@@ -69,3 +69,21 @@ def test_device_transfer_p_derefinement(gpu):
     ref = dense_projection(t, state)
     scale = np.max(np.abs(ref.coefficients))
     assert np.max(np.abs(new.coefficients - ref.coefficients)) <= _bound(ref.plan) * scale
+
+
+@pytest.mark.parametrize("continuity", ["cg", "dg"])
+def test_device_transfer_h_derefinement(gpu, continuity):
+    """Merged cells carry composite rules and run on the point-cloud kernel."""
+    from test_adaptivity_host import _derefined_plan
+
+    space = hanging_space(degree=2, continuity=continuity, dim=2)
+    rng = np.random.default_rng(41)
+    state = state_on(space, rng.standard_normal(space.scalar_dof_count()))
+    plan, _ = _derefined_plan(space)
+    t = ad.transition(plan)
+    print(f"transfer h-derefinement {continuity}", flush=True)
+    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
+    ref = dense_projection(t, state)
+    assert new.plan.cloud_buckets, "merged cells must use composite rules"
+    scale = np.max(np.abs(ref.coefficients))
+    assert np.max(np.abs(new.coefficients - ref.coefficients)) <= _bound(ref.plan) * scale
