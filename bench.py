@@ -1089,6 +1089,77 @@ def ctypes_byref(obj):
     return ctypes.byref(obj)
 
 
+def transfer_leg(roots=16, degree=2):
+    """State transfer across an hp transition (3D CG, roots^3 cells at p=2;
+    the corner octant, split once along x in the source, is merged back onto
+    composite rules, the opposite octant is split along x and the middle
+    layer is raised to p=3): host lowering of the L2 projection, device
+    Jacobi-CG of the projection to 1e-12, and the oracle Jacobi-CG of the
+    same system timed beside it."""
+    import numpy as np
+
+    import gricoamd as g
+    import pyoracle
+    from gricoamd import adaptivity as ad
+
+    class Mass:
+        def cell_accumulate(self, q, trial, test_component):
+            return g.TestChannels(trial.value, tuple(0.0 * x for x in trial.gradient))
+
+    dom = g.Domain((0.0,) * 3, (1.0,) * 3, (roots,) * 3)
+    half = roots // 2
+    crd = dom.grid.coords()
+    corner = [int(c) for c in dom.active_leaves() if np.all(crd[c] < half)]
+    for c in corner:
+        dom.refine(c, 0)
+    space = g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    src_problem = g.AffineProblem(u, g.SPD())
+    src_problem.add_cell_accumulator(u, u, Mass())
+    src = g.State(g.compile(src_problem), np.random.default_rng(7).standard_normal(space.scalar_dof_count()))
+    plan = ad.AdaptivityPlan(space)
+    for c in corner:
+        plan.request_h_derefinement(c, 0)
+    crd = plan.target_domain.grid.coords()
+    for leaf in list(plan.degrees):
+        lv = plan.target_domain.grid.levels()[leaf]
+        if np.all(lv == 0) and np.all(crd[leaf] >= half):
+            plan.request_h_refinement(leaf, 0)
+        elif np.all(lv == 0) and crd[leaf][2] == half - 1:
+            plan.request_p_refinement(leaf)
+    t0 = time.perf_counter()
+    tr = ad.transition(plan)
+    t_space = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pplan = g.compile(ad.projection_problem(tr, src))
+    b = pplan.rhs_physical()[pplan.solve_dofs]
+    t_lower = time.perf_counter() - t0
+    solver = g.CGSolver(preconditioner=g.JacobiPreconditioner())
+    g.solve(pplan, solver=solver, relative_tolerance=1e-12)  # warm-up
+    t0 = time.perf_counter()
+    state = g.solve(pplan, solver=solver, relative_tolerance=1e-12)
+    t_dev = time.perf_counter() - t0
+    inv = 1.0 / pyoracle.diagonal_physical(pplan)[pplan.solve_dofs]
+    oracle_iters = 100  # bounded sample: the full oracle solve takes ~35 s
+    t0 = time.perf_counter()
+    _, it_or = pyoracle.cg(pplan, b, inv_diag=inv, rtol=1e-12, maxiter=oracle_iters, lowered=False)
+    t_or = time.perf_counter() - t0
+    xr = np.random.default_rng(3).standard_normal(pplan.full_dof_count)
+    y_dev = g.apply_physical(pplan, xr)
+    y_or = pyoracle.apply(pplan, xr, physical=True)
+    out = {"workload": f"3D CG p={degree}, {roots}^3 roots; corner octant merged (composite rules), opposite "
+                       f"octant split along x, one layer raised to p={degree + 1}; L2 projection of a random field",
+           "source_dofs": space.scalar_dof_count(), "target_dofs": pplan.full_dof_count,
+           "merged_cells": len(corner), "target_space_seconds": t_space, "host_lowering_seconds": t_lower,
+           "device_solve_seconds": t_dev, "device_iterations": state.info.iterations,
+           "device_seconds_per_iteration": t_dev / max(state.info.iterations, 1),
+           "oracle_cg_seconds_per_iteration": t_or / max(it_or, 1), "oracle_sample_iterations": it_or,
+           "projection_apply_max_difference_vs_oracle_scaled": float(np.abs(y_dev - y_or).max() /
+                                                                     np.abs(y_or).max())}
+    log(f"transfer: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1101,6 +1172,8 @@ def main():
     ap.add_argument("--no-lognormal", action="store_true", help="skip the log-normal GMG-CG leg")
     ap.add_argument("--no-sipg", action="store_true", help="skip the interior penalty DG (face kernels) leg")
     ap.add_argument("--no-newton", action="store_true", help="skip the nonlinear Bratu Newton leg")
+    ap.add_argument("--no-transfer", action="store_true", help="skip the hp state-transfer leg")
+    ap.add_argument("--transfer-only", action="store_true", help="run only the hp state-transfer leg and print its JSON")
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
     ap.add_argument("--no-fcm", action="store_true",
@@ -1131,6 +1204,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.sipg_only:
         print(json.dumps({"sipg": sipg_leg()}), flush=True)
+        return
+    if args.transfer_only:
+        print(json.dumps({"transfer": transfer_leg()}), flush=True)
         return
     if args.fcm_only:
         print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)}), flush=True)
@@ -1233,6 +1309,12 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"newton leg failed: {exc!r}")
             result["newton"] = {"error": repr(exc)}
+    if not args.no_transfer:
+        try:
+            result["transfer"] = transfer_leg()
+        except Exception as exc:  # keep the headline line
+            log(f"transfer leg failed: {exc!r}")
+            result["transfer"] = {"error": repr(exc)}
     if not args.no_solve:
         result["solve"] = solve_leg(problem, plan)
     if not args.no_adaptive:
