@@ -1144,6 +1144,7 @@ def transfer_leg(roots=16, degree=2):
     t0 = time.perf_counter()
     _, it_or = pyoracle.cg(pplan, b, inv_diag=inv, rtol=1e-12, maxiter=oracle_iters, lowered=False)
     t_or = time.perf_counter() - t0
+    it_or = oracle_iters if it_or < 0 else it_or  # -1: stopped at maxiter (not converged in the sample)
     xr = np.random.default_rng(3).standard_normal(pplan.full_dof_count)
     y_dev = g.apply_physical(pplan, xr)
     y_or = pyoracle.apply(pplan, xr, physical=True)
