@@ -27,7 +27,8 @@ from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, 
                     solve_reduced)
 from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid
-from .adaptivity import (AdaptivityLimits, AdaptivityPlan, SpaceTransition, evaluate, transfer_state,
+from .adaptivity import (AdaptivityLimits, AdaptivityPlan, SpaceTransition, evaluate, interface_jump_indicators,
+                         transfer_state,
                          transition)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -26,7 +26,9 @@ takes a composite rule, one Gauss rule per covered source leaf, so the
 piecewise polynomial old field is integrated leaf by leaf as in
 src/transfer.jl:22-25, and the cell runs on the point-cloud kernel.
 
-Not built: the automatic indicator-driven planning of src/indicators.jl.
+Interface jump indicators (src/indicators.jl:260-324) are evaluated on the
+host. Not built: the modal and multiresolution indicators and the automatic
+planner of src/indicators.jl.
 """
 
 import numpy as np
@@ -292,16 +294,21 @@ def locate(domain, points, leaves=None):
         cell[inner] = np.where(coord >= mid, upper_child, lower_child)
 
 
-def evaluate(state, points):
+def evaluate(state, points, leaves=None):
     """u_h at arbitrary points of the state's domain (tuple of D arrays of one
-    shape); returns an array of that shape."""
+    shape); returns an array of that shape. `leaves` (same shape) names the
+    leaf whose polynomial to evaluate, e.g. one side of a face; by default the
+    leaf containing each point."""
     plan = state.plan
     space = plan.space
     dom = space.domain
     D = dom.dim
     shape = np.shape(points[0])
     pts = [np.asarray(p, dtype=np.float64).reshape(-1) for p in points]
-    leaf = locate(dom, pts, space.leaves)
+    if leaves is None:
+        leaf = locate(dom, pts, space.leaves)
+    else:
+        leaf = np.asarray(leaves, dtype=np.int64).reshape(-1)
     out = np.zeros(pts[0].size)
     row_of = np.full(dom.grid.cell_count(), -1, dtype=np.int64)
     cells = [cb for cb in getattr(plan, "cloud_buckets", []) if not getattr(cb, "surface", False)]
@@ -415,3 +422,49 @@ def transfer_state(transition, state, relative_tolerance=1e-13, maxiter=None):
     plan = compile_plan(projection_problem(transition, state))
     return solve(plan, solver=CGSolver(preconditioner=JacobiPreconditioner()),
                  relative_tolerance=relative_tolerance, maxiter=maxiter)
+
+
+def interface_jump_indicators(state):
+    """Per-leaf, per-axis interface jump indicators (src/indicators.jl:260-324):
+    for every interface patch normal to axis a, J = int_F |u- - u+|^2 dS is
+    added to both leaves with their own scaling (p_a + 1)^2 / h_a; the result
+    is sqrt of the sums, shape (leaf count, D) in the space's leaf order. The
+    patch is the overlap of the two faces (the smaller face on hanging
+    interfaces), integrated by max(p-, p+) + 1 Gauss points per tangential
+    axis, which is exact for the squared jump."""
+    space = state.plan.space
+    dom = space.domain
+    D = dom.dim
+    out = np.zeros((len(space.leaves), D))
+    # the space's interface lists hold positions in space.leaves
+    im, ip, axes = space.interface_minus, space.interface_plus, space.interface_axis
+    if len(im) == 0:
+        return out
+    minus, plus = space.leaves[im], space.leaves[ip]
+    lo_m, lo_p = dom.cell_lowers(minus), dom.cell_lowers(plus)
+    hi_m, hi_p = lo_m + dom.cell_sizes(minus), lo_p + dom.cell_sizes(plus)
+    for k in range(len(minus)):
+        a = int(axes[k])
+        n = int(max(space.degrees[im[k]].max(), space.degrees[ip[k]].max())) + 1
+        x, w = gauss_legendre_rule(n)
+        lo = np.maximum(lo_m[k], lo_p[k])
+        hi = np.minimum(hi_m[k], hi_p[k])
+        tang = [b for b in range(D) if b != a]
+        grids = np.meshgrid(*[lo[b] + 0.5 * (x + 1.0) * (hi[b] - lo[b]) for b in tang], indexing="ij")
+        wts = np.ones(1)
+        for b in tang:
+            wts = np.multiply.outer(wts, 0.5 * w * (hi[b] - lo[b]))
+        wts = wts.reshape(-1)
+        m = wts.size
+        pts = [None] * D
+        for j, b in enumerate(tang):
+            pts[b] = grids[j].reshape(-1)
+        pts[a] = np.full(m, hi_m[k][a])
+        jump = evaluate(state, tuple(pts), np.full(m, minus[k])) - evaluate(state, tuple(pts), np.full(m, plus[k]))
+        energy = float(np.sum(wts * jump * jump))
+        if not np.isfinite(energy):
+            raise ValueError("interface jump indicators must be finite")
+        for idx, leaf in ((im[k], minus[k]), (ip[k], plus[k])):
+            h = dom.cell_sizes([leaf])[0][a]
+            out[idx, a] += (space.degrees[idx][a] + 1) ** 2 / h * energy
+    return np.sqrt(out)

@@ -229,3 +229,78 @@ def test_derefinement_projection_is_l2_optimal():
     best = dist(proj.coefficients)
     for _ in range(5):
         assert dist(proj.coefficients + 1e-3 * rng.standard_normal(proj.coefficients.size)) > best
+
+
+def _dg_constants(space, values):
+    """DG p=1 field equal to values[i] on leaf i: every vertex mode of a cell
+    carries the cell's constant (psi_0 + psi_1 = 1 per axis)."""
+    st = state_on(space, np.zeros(space.scalar_dof_count()))
+    coeffs = np.zeros(space.scalar_dof_count())
+    for b in st.plan.buckets:
+        vals = np.asarray(values)[b.leaf_idx]
+        box = np.repeat(vals[:, None], b.n_modes, axis=1)
+        coeffs[b.term_dof] = box[b.term_cell, b.term_box]
+    return g.State(st.plan, coeffs)
+
+
+def test_jump_indicators_closed_form():
+    """2x2 DG p=1 cells of size 1/2 with constants c: each leaf meets one
+    interface per axis, J = (c- - c+)^2 * 1/2 and the scale is 2^2 / (1/2),
+    so the indicator is 2 |c- - c+|."""
+    dom = g.Domain((0.0, 0.0), (1.0, 1.0), (2, 2))
+    space = g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(1), continuity="dg"))
+    crd = dom.grid.coords()[space.leaves]
+    c = np.asarray([1.0, 4.0, -2.0, 0.5])
+    vals = c[crd[:, 0] + 2 * crd[:, 1]]
+    ind = ad.interface_jump_indicators(_dg_constants(space, vals))
+    for i in range(4):
+        for a in range(2):
+            other = crd[i].copy()
+            other[a] = 1 - other[a]
+            j = [k for k in range(4) if np.all(crd[k] == other)][0]
+            assert ind[i, a] == pytest.approx(2.0 * abs(vals[i] - vals[j]), abs=1e-13)
+
+
+def test_jump_indicators_vanish_for_continuous_fields():
+    rng = np.random.default_rng(37)
+    cg = hanging_space(degree=3, continuity="cg")
+    ind = ad.interface_jump_indicators(state_on(cg, rng.standard_normal(cg.scalar_dof_count())))
+    assert np.max(ind) < 1e-12
+    # a linear field in the hanging DG space: reproduced exactly on every cell
+    dg = hanging_space(degree=1, continuity="dg")
+    f = lambda x, y: 1.0 + 2.0 * x - 3.0 * y  # noqa: E731
+    st = _dg_constants(dg, np.zeros(len(dg.leaves)))
+    coeffs = np.zeros(dg.scalar_dof_count())
+    for b in st.plan.buckets:
+        # vertex values of the linear field: box (i, j) -> corner (i, j) of the cell
+        lo = dg.domain.cell_lowers(b.cells)
+        hi = lo + dg.domain.cell_sizes(b.cells)
+        box = np.stack([f(lo[:, 0], lo[:, 1]), f(hi[:, 0], lo[:, 1]), f(lo[:, 0], hi[:, 1]), f(hi[:, 0], hi[:, 1])],
+                       axis=1)
+        coeffs[b.term_dof] = box[b.term_cell, b.term_box]
+    st = g.State(st.plan, coeffs)
+    pts = random_points(rng, 50, 2)
+    np.testing.assert_allclose(ad.evaluate(st, pts), f(*pts), atol=1e-13)
+    assert np.max(ad.interface_jump_indicators(st)) < 1e-12
+
+
+def test_jump_indicators_hanging_patches():
+    """Root 0 split in x, its lower-x child split in y (hanging_space); the
+    DG p=1 field is 1 on root 2 (x < 1/2, y > 1/2) and 0 elsewhere. Root 2
+    meets two 1/4-long hanging patches on axis 1 (scale 2^2/(1/2)): sqrt(8 *
+    (1/4 + 1/4)) = 2; the 1/4-tall child below it: sqrt(16 * 1/4) = 2; the
+    1/4-wide, 1/2-tall child: sqrt(8 * 1/4) = sqrt(2); root 3 on axis 0:
+    sqrt(8 * 1/2) = 2."""
+    space = hanging_space(degree=1, continuity="dg")
+    dom = space.domain
+    first = int(dom.grid.first_child()[0])
+    inner = int(dom.grid.first_child()[first])  # children of the lower-x child, split in y
+    vals = np.asarray([1.0 if l == 2 else 0.0 for l in space.leaves])
+    ind = ad.interface_jump_indicators(_dg_constants(space, vals))
+    at = {int(l): ind[i] for i, l in enumerate(space.leaves)}
+    np.testing.assert_allclose(at[2], [2.0, 2.0], atol=1e-13)
+    np.testing.assert_allclose(at[3], [2.0, 0.0], atol=1e-13)
+    np.testing.assert_allclose(at[inner + 1], [0.0, 2.0], atol=1e-13)
+    np.testing.assert_allclose(at[first + 1], [0.0, np.sqrt(2.0)], atol=1e-13)
+    np.testing.assert_allclose(at[inner], [0.0, 0.0], atol=1e-13)
+    np.testing.assert_allclose(at[1], [0.0, 0.0], atol=1e-13)
