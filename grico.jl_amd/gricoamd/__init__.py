@@ -28,7 +28,7 @@ from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, 
 from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid
 from .adaptivity import (AdaptivityLimits, AdaptivityPlan, SpaceTransition, evaluate, interface_jump_indicators,
-                         transfer_state,
+                         coefficient_coarsening_indicators, transfer_state,
                          transition)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -26,7 +26,8 @@ takes a composite rule, one Gauss rule per covered source leaf, so the
 piecewise polynomial old field is integrated leaf by leaf as in
 src/transfer.jl:22-25, and the cell runs on the point-cloud kernel.
 
-Interface jump indicators (src/indicators.jl:260-324) are evaluated on the
+Interface jump and modal coefficient indicators (src/indicators.jl:260-342)
+are evaluated on the
 host. Not built: the modal and multiresolution indicators and the automatic
 planner of src/indicators.jl.
 """
@@ -468,3 +469,47 @@ def interface_jump_indicators(state):
             h = dom.cell_sizes([leaf])[0][a]
             out[idx, a] += (space.degrees[idx][a] + 1) ** 2 / h * energy
     return np.sqrt(out)
+
+
+def coefficient_coarsening_indicators(state):
+    """Per-leaf, per-axis normalized top-layer modal energy sqrt(E_top / E_total)
+    (src/indicators.jl:326-342, layer energies 61-108). E_total sums the squared
+    local mode amplitudes of the leaf; E_top sums those whose index along the
+    axis equals the degree. At degree 1 each (0, 1) endpoint pair along the
+    axis is read as constant + linear content and E_top sums the squared linear
+    parts (upper - lower) / 2; at degree 0 it is zero. Shape (leaf count, D),
+    space leaf order."""
+    plan = state.plan
+    space = plan.space
+    D = space.dim
+    out = np.zeros((len(space.leaves), D))
+    for b in plan.buckets:
+        amp = np.zeros((b.cell_count, b.n_modes))
+        np.add.at(amp, (b.term_cell, b.term_box), b.term_coef * state.coefficients[b.term_dof])
+        active = np.zeros((b.cell_count, b.n_modes), dtype=bool)
+        for r, li in enumerate(b.leaf_idx):
+            active[r, space.mode_box[space.mode_offsets[li]:space.mode_offsets[li + 1]]] = True
+        amp = np.where(active, amp, 0.0)
+        total = np.sum(amp * amp, axis=1)
+        shape = tuple(reversed(b.modes))  # C order: axis D .. axis 1
+        A = amp.reshape((b.cell_count,) + shape)
+        M = active.reshape((b.cell_count,) + shape)
+        for a in range(D):
+            p = b.degrees[a]
+            ax = D - a  # array axis of mesh axis a
+            if p == 0:
+                continue
+            if p == 1:
+                lower = np.take(A, 0, axis=ax)
+                upper = np.take(A, 1, axis=ax)
+                both = np.take(M, 0, axis=ax) & np.take(M, 1, axis=ax)
+                lin = np.where(both, 0.5 * (upper - lower), 0.0)
+                top = np.sum((lin * lin).reshape(b.cell_count, -1), axis=1)
+            else:
+                t = np.take(A, p, axis=ax)
+                top = np.sum((t * t).reshape(b.cell_count, -1), axis=1)
+            with np.errstate(invalid="ignore", divide="ignore"):
+                out[b.leaf_idx, a] = np.where(total > 0, np.sqrt(top / np.where(total > 0, total, 1.0)), 0.0)
+    if not np.all(np.isfinite(out)):
+        raise ValueError("modal indicator energies must be finite")
+    return out

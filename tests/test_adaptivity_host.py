@@ -304,3 +304,30 @@ def test_jump_indicators_hanging_patches():
     np.testing.assert_allclose(at[first + 1], [0.0, np.sqrt(2.0)], atol=1e-13)
     np.testing.assert_allclose(at[inner], [0.0, 0.0], atol=1e-13)
     np.testing.assert_allclose(at[1], [0.0, 0.0], atol=1e-13)
+
+
+def test_coefficient_indicators_closed_forms():
+    """One DG p=2 cell per leaf: the single mode psi_2(x) psi_0(y) is all top
+    layer along x (1) and none along y (0). A p=1 field with vertex values
+    (a-b, a+b) on both x-pairs has linear parts b, so E_top = 2 b^2 against
+    E_total = 2 (a-b)^2 + 2 (a+b)^2 along x, and no linear part along y."""
+    dom = g.Domain((0.0, 0.0), (1.0, 1.0), (2, 1))
+    sp2 = g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(2), continuity="dg"))
+    st = state_on(sp2, np.zeros(sp2.scalar_dof_count()))
+    coeffs = np.zeros(sp2.scalar_dof_count())
+    for b in st.plan.buckets:
+        box = np.zeros((b.cell_count, b.n_modes))
+        box[:, 2] = 1.0  # mode (2, 0)
+        coeffs[b.term_dof] = box[b.term_cell, b.term_box]
+    ind = ad.coefficient_coarsening_indicators(g.State(st.plan, coeffs))
+    np.testing.assert_allclose(ind, [[1.0, 0.0]] * 2, atol=1e-15)
+    sp1 = g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(1), continuity="dg"))
+    st = state_on(sp1, np.zeros(sp1.scalar_dof_count()))
+    a, bb = 3.0, 0.5
+    coeffs = np.zeros(sp1.scalar_dof_count())
+    for b in st.plan.buckets:
+        box = np.tile([a - bb, a + bb, a - bb, a + bb], (b.cell_count, 1))  # box (i, j): i along x
+        coeffs[b.term_dof] = box[b.term_cell, b.term_box]
+    ind = ad.coefficient_coarsening_indicators(g.State(st.plan, coeffs))
+    expect = np.sqrt(2 * bb ** 2 / (2 * (a - bb) ** 2 + 2 * (a + bb) ** 2))
+    np.testing.assert_allclose(ind, [[expect, 0.0]] * 2, atol=1e-15)
