@@ -27,9 +27,8 @@ piecewise polynomial old field is integrated leaf by leaf as in
 src/transfer.jl:22-25, and the cell runs on the point-cloud kernel.
 
 Interface jump and modal coefficient indicators (src/indicators.jl:260-342)
-are evaluated on the
-host. Not built: the modal and multiresolution indicators and the automatic
-planner of src/indicators.jl.
+are evaluated on the host. Not built: the projection and multiresolution
+indicators and the automatic planner of src/indicators.jl.
 """
 
 import numpy as np
