@@ -1003,7 +1003,22 @@ def _boundary_l2_projection(plan, cons, dofs):
     if n == 0:
         return np.zeros(0)
     G = sps.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
-    return np.asarray(spla.spsolve(G, r), dtype=np.float64)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("error", spla.MatrixRankWarning)
+        try:
+            sol = np.asarray(spla.spsolve(G, r), dtype=np.float64)
+        except spla.MatrixRankWarning:
+            sol = None
+    # A singular trace Gram matrix means trace dofs that the selected faces do
+    # not control independently; the reference turns those into affine
+    # relations (src/plans.jl:884-943), which this sparse path does not build.
+    resid = np.inf if sol is None or not np.all(np.isfinite(sol)) else \
+        np.linalg.norm(G @ sol - r) / max(np.linalg.norm(r), np.finfo(np.float64).tiny)
+    if not resid <= 1e-8:
+        raise ValueError("rank-deficient Dirichlet trace system: affine Dirichlet relations "
+                         "(src/plans.jl:884-943) are not supported by this plan compiler")
+    return sol
 
 
 def compile(problem):
