@@ -915,10 +915,26 @@ def learned_leg(repeats=3, pairs=2000, steps=3000):
     return out
 
 
+# Sources of the cell-pass kernels whose HBM traffic profiles/apply_traffic.json records.
+TRAFFIC_SOURCES = ("grico.jl_amd/csrc/device/cell_fast.hip", "grico.jl_amd/csrc/device/eo.hpp",
+                   "grico.jl_amd/csrc/device/common.hpp")
+
+
+def kernel_source_hash():
+    """First 16 hex digits of the SHA-256 over the cell-pass kernel sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _traffic_from_profiles(n, cells):
     """HBM bytes per cell pass measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by a 1 GiB copy), recorded in profiles/apply_traffic.json
-    for this workload; None when absent or for another workload."""
+    for this workload and these kernel sources; None when absent, for another
+    workload, or when the kernel sources changed since the measurement."""
     path = os.path.join(ROOT, "profiles", "apply_traffic.json")
     try:
         with open(path) as f:
@@ -926,6 +942,8 @@ def _traffic_from_profiles(n, cells):
     except (OSError, ValueError):
         return None
     if rec.get("full_dofs") != n or rec.get("cells") != cells:
+        return None
+    if rec.get("kernel_source_sha256_16") != kernel_source_hash():
         return None
     return rec.get("traffic_bytes_per_cell_pass")
 

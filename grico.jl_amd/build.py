@@ -49,9 +49,23 @@ def device_sources():
 
 
 def build_device():
+    """One object per source, compiled in parallel, then one shared-library link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
     out = os.path.join(LIBDIR, "libgrico_mf.so")
-    _run([HIPCC] + HIP_FLAGS + ["-o", out] + device_sources() + ["-lrocsolver", "-lrocblas"])
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    jobs = []
+    for src in device_sources():
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        jobs.append((src, obj))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(workers) as pool:
+        list(pool.map(lambda j: _run([HIPCC] + compile_flags + ["-c", "-o", j[1], j[0]]), jobs))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + [o for _, o in jobs] +
+         ["-lrocsolver", "-lrocblas"])
     return out
 
 

@@ -20,15 +20,17 @@
 //
 // Two drivers share that cell body:
 //  * brick kernel: one workgroup per brick of 2x2x2 cells that are
-//    consecutive in Morton order. The brick's (2N-1)^3 lattice of modes is
-//    gathered into LDS once, the eight cells read their modes from it and
-//    add their results into an LDS accumulator lattice (ds_add_f64), which
-//    is written back once: plain stores where this brick is the first writer
-//    of the dof, plain read-modify-write otherwise, and per-brick corner
-//    slots for dofs that occur only at brick corners (summed afterwards by
-//    k_brick_corners). Bricks are launched one color at a time; colors are
-//    disjoint in every non-corner dof, so no two workgroups of a launch touch
-//    the same dof and no global atomics are needed.
+//    consecutive in Morton order, all bricks in one launch. The brick's
+//    (2N-1)^3 lattice of modes is gathered into LDS once, the eight cells
+//    read their modes from it and add their results into an LDS accumulator
+//    lattice (ds_add_f64), which is written back once: dofs that no other
+//    brick holds with plain stores, every occurrence of a dof shared between
+//    bricks into its own halo slot, and k_brick_halo sums each shared dof's
+//    slots (contiguous, in brick order) into y without global atomics. One
+//    launch keeps neighbouring bricks' shared input lines in L2, which a
+//    colored sequence of launches re-fetches once per color (an LRU model of
+//    the 4 MB L2 per XCD on the 64^3 p=4 mesh: 1.05x instead of 2.17x the
+//    input's cache lines fetched per pass).
 //  * cell kernel: independent cells with a global atomic scatter, used when
 //    the cells do not form verified bricks or the batch is small.
 // Both remap workgroup ids so that consecutive Morton-ordered work stays on
@@ -220,19 +222,20 @@ __global__ void __launch_bounds__(256) cell_apply_fast(CellArgs<R> a, EOTables<N
 
 // ---- brick driver --------------------------------------------------------------
 
-// Lattice entry kinds (low two bits of a brick map entry).
-constexpr int kLatRmw = 0, kLatFirst = 1, kLatCorner = 2;
-
 template <class R>
 struct BrickArgs {
-  const int64_t* order;   // bricks of this launch (one color), Morton order
-  int64_t count;          // bricks in this launch
-  // bricks x L lattice entries: (dof << 2) | kind, -1 for no dof.
+  // bricks x L lattice entries: (dof << 1) | exclusive; -1 for no dof, -2 for
+  // no dof at a shared position (a constrained dof of the reduced layout),
+  // which still takes its halo slot.
   const int32_t* map;
-  const R* jinv;          // cells x 3
-  const R* coef;          // cells x N^3
-  R* slots;               // bricks x 8 corner results
-  int store_first;        // 0: always read-modify-write (y pre-zeroed)
+  // Shared positions of brick b, in ascending lattice position, write halo
+  // slots halo_slot[halo_base[b] .. halo_base[b + 1]).
+  const int32_t* halo_base;
+  const int32_t* halo_slot;
+  const R* jinv;  // cells x 3
+  const R* coef;  // cells x N^3
+  R* halo;        // halo values of the bucket (shared-dof major)
+  int accumulate; // 1: results are added to y (other batches wrote it first)
 };
 
 // Lattice offset of local mode m along one axis of a cell: vertex 0 at 0,
@@ -242,47 +245,43 @@ __host__ __device__ inline int lat(int m) {
   return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1);
 }
 
-// Corner index (x bit 0, y bit 1, z bit 2) of a lattice position of an
-// M^3 brick lattice, -1 if the position is not a brick corner.
-__host__ __device__ inline int lattice_corner(int pos, int M) {
-  const int i = pos % M, j = (pos / M) % M, k = pos / (M * M);
-  const bool c = (i == 0 || i == M - 1) && (j == 0 || j == M - 1) && (k == 0 || k == M - 1);
-  return c ? int(i != 0) | (int(j != 0) << 1) | (int(k != 0) << 2) : -1;
+// Upper bound of the shared positions of one brick (its lattice surface).
+template <int N>
+constexpr int brick_surface() {
+  constexpr int M = 2 * N - 1;
+  return M * M * M - (M - 2) * (M - 2) * (M - 2);
 }
 
-// One workgroup per brick of one color. The coefficient pencils and the
-// lattice entries are loaded first (independent of each other), then the
-// inputs and the current results of the read-modify-write points (dependent
-// on the entries); the y values are stable for the launch because bricks of
-// one color share no dof outside the brick corners, and corners go to the
-// brick's own slots.
+// One workgroup per brick, all bricks in one launch, consecutive bricks on one
+// XCD so that neighbours share its L2. Phase 1 issues the lattice entries, the
+// cells' coefficients (each cell's N^3 values loaded contiguously by its lanes
+// and staged in LDS), the brick's halo slot list and the input gathers; phase
+// 2 runs the eight cell bodies, which add into an LDS result lattice; phase 3
+// stores the positions whose dof no other brick holds to y, and the shared
+// positions (ranked in ascending lattice position by wavefront ballots) to
+// their halo slots, which k_brick_halo sums per dof.
 template <int N, class R>
 __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs<R> a, EOTables<N, R> T,
                                                                         const R* __restrict__ x, R* __restrict__ y) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
   constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC, PER = (L + THREADS - 1) / THREADS;
+  constexpr int WAVES = (THREADS + 63) / 64, CPL = (NNN + LPC - 1) / LPC, SURF = brick_surface<N>();
+  constexpr int SPER = (SURF + THREADS - 1) / THREADS;
   extern __shared__ double smem_words[];
+  __shared__ int shared_count[PER * WAVES];
+  __shared__ int32_t slots[SURF];
   R* xl = reinterpret_cast<R*>(smem_words);  // L input modes
   R* acc = xl + L;                            // L accumulated results
   R* scratch = acc + L;                       // 8 x 2 x N^3 (per cell A, B)
   const int tid = threadIdx.x;
-  const int lc = tid / LPC;
+  const int lc = tid / LPC, lane_in_cell = tid % LPC;
+  const int wave = tid >> 6, lane = tid & 63;
   int t0, t1;
   bool act;
-  lane_coords<N>(tid % LPC, t0, t1, act);
-  const int64_t brick = a.order[xcd_block(blockIdx.x, gridDim.x)];
-  // Cell position inside the brick (Morton child order: x fastest).
-  const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
+  lane_coords<N>(lane_in_cell, t0, t1, act);
+  const int64_t brick = xcd_block(blockIdx.x, gridDim.x);
   const int64_t cell = brick * 8 + lc;
-  const R* cb = a.coef + cell * NNN;
-  R cx[N], cy[N], cz[N];
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
-    cx[q] = cb[q + N * t0 + NN * t1];
-    cy[q] = cb[t0 + N * q + NN * t1];
-    cz[q] = cb[t0 + N * t1 + NN * q];
-  }
-  const R jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
+  // Phase 1: entries, coefficients, slot list, inputs.
   const int32_t* map = a.map + brick * L;
   int32_t e[PER];
 #pragma unroll
@@ -290,56 +289,107 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int p = tid + k * THREADS;
     e[k] = p < L ? map[p] : -1;
   }
-  R yold[PER];
+  const R* cb = a.coef + cell * NNN;
+  R cv[CPL];
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int i = lane_in_cell + k * LPC;
+    cv[k] = i < NNN ? cb[i] : R(0);
+  }
+  const int32_t sbase = a.halo_base[brick], scount = a.halo_base[brick + 1] - sbase;
+  int32_t sv[SPER];
+#pragma unroll
+  for (int k = 0; k < SPER; ++k) {
+    const int i = tid + k * THREADS;
+    sv[k] = i < scount ? a.halo_slot[sbase + i] : 0;
+  }
+  const R jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
+  R xv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int32_t d = e[k] >> 1;
+    xv[k] = d >= 0 ? x[d] : R(0);
+  }
+  R* A = scratch + lc * 2 * NNN;
+  R* Bc = A + NNN;  // coefficients, until the cell body first writes B
+#pragma unroll
+  for (int k = 0; k < CPL; ++k) {
+    const int i = lane_in_cell + k * LPC;
+    if (i < NNN) Bc[i] = cv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < SPER; ++k) {
+    const int i = tid + k * THREADS;
+    if (i < scount) slots[i] = sv[k];
+  }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int p = tid + k * THREADS;
-    const int32_t d = e[k] >> 2, kind = e[k] & 3;
-    const bool read = e[k] >= 0 && kind != kLatCorner && !(a.store_first && kind == kLatFirst);
-    yold[k] = read ? y[d] : R(0);
     if (p < L) {
-      xl[p] = e[k] >= 0 ? x[d] : R(0);
+      xl[p] = xv[k];
       acc[p] = R(0);
     }
   }
   __syncthreads();
+  // Phase 2: cell bodies. The coefficient pencils are read from B before the
+  // body's first wave-ordered write of B.
+  R cx[N], cy[N], cz[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    cx[q] = Bc[q + N * t0 + NN * t1];
+    cy[q] = Bc[t0 + N * q + NN * t1];
+    cz[q] = Bc[t0 + N * t1 + NN * q];
+  }
+  const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
   R u[N], r[N];
   {
     const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
 #pragma unroll
     for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
   }
-  R* A = scratch + lc * 2 * NNN;
-  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, A + NNN, t0, t1, act, r);
+  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, Bc, t0, t1, act, r);
   if (act) {
     const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
 #pragma unroll
     for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[ox + lat<N>(ii) + M * (py + M * pz)], r[ii]);
   }
+  // Ranks of the shared positions, counted per (round, wavefront).
+  uint64_t fm[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    fm[k] = __ballot(!(e[k] & 1));
+    if (lane == 0) shared_count[k * WAVES + wave] = __popcll(fm[k]);
+  }
   __syncthreads();
+  // Phase 3.
+  const uint64_t below = (uint64_t(1) << lane) - 1;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int p = tid + k * THREADS;
-    if (e[k] < 0) continue;
-    if ((e[k] & 3) == kLatCorner)
-      a.slots[brick * 8 + lattice_corner(p, M)] = acc[p];
-    else
-      y[e[k] >> 2] = yold[k] + acc[p];
+    if (e[k] & 1) {
+      const int32_t d = e[k] >> 1;
+      if (d >= 0) y[d] = a.accumulate ? y[d] + acc[p] : acc[p];
+    } else {
+      int off = 0;
+      for (int j = 0; j < k * WAVES + wave; ++j) off += shared_count[j];
+      a.halo[slots[off + __popcll(fm[k] & below)]] = acc[p];
+    }
   }
 }
 
-// Brick-corner dofs: y[d] (+)= sum of the corner slots that hold d, in slot
-// order (deterministic).
+// Shared dofs: y[d] (+)= the sum of the dof's contiguous halo values
+// (the owner's first, then the other bricks in order: deterministic).
+// Consecutive blocks stay on one XCD.
 template <class R>
-__global__ void k_brick_corners(R* __restrict__ y, const R* __restrict__ slots, const int32_t* dof,
-                                const int64_t* ptr, const int32_t* slot, int64_t count, int accumulate) {
-  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+__global__ void k_brick_halo(R* __restrict__ y, const R* __restrict__ halo, const int32_t* __restrict__ dof,
+                             const int32_t* __restrict__ ptr, int64_t count, int accumulate) {
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * int64_t(blockDim.x) + threadIdx.x;
   if (i >= count) return;
   const int32_t d = dof[i];
-  if (d < 0) return;
+  const int32_t lo = ptr[i], hi = ptr[i + 1];
   R s = R(0);
-  for (int64_t j = ptr[i]; j < ptr[i + 1]; ++j) s += slots[slot[j]];
-  y[d] = accumulate ? y[d] + s : s;
+  for (int32_t j = lo; j < hi; ++j) s += halo[j];
+  if (d >= 0) y[d] = accumulate ? y[d] + s : s;
 }
 
 template <int N, class R>
@@ -387,29 +437,23 @@ size_t brick_lds_bytes() {
 }
 
 template <int N, class R>
-void launch_bricks(const Bucket& b, const R* x, R* y, bool reduced, bool store_first, R* slots, hipStream_t stream) {
+void launch_bricks(const Bucket& b, const R* x, R* y, bool reduced, bool accumulate, R* slots, hipStream_t stream) {
   EOTables<N, R> T = tables_of<N, R>(b);
   const BrickPath& bp = b.bricks;
-  require(bp.corner_count == 0 || slots != nullptr, GMF_EINVAL, "workspace has no brick corner slots");
-  R* bslots = slots ? slots + bp.slot_base : nullptr;
-  for (int c = 0; c < bp.colors; ++c) {
-    const int64_t lo = bp.color_offsets[size_t(c)], hi = bp.color_offsets[size_t(c) + 1];
-    if (hi == lo) continue;
-    BrickArgs<R> a;
-    a.order = bp.order + lo;
-    a.count = hi - lo;
-    a.map = reduced ? bp.map_red : bp.map_full;
-    a.jinv = CellArrays<R>::jinv(b);
-    a.coef = CellArrays<R>::coef(b);
-    a.slots = bslots;
-    a.store_first = store_first ? 1 : 0;
-    hipLaunchKernelGGL((brick_apply<N, R>), dim3(unsigned(hi - lo)), dim3(8 * lanes_per_cell<N>()),
-                       (brick_lds_bytes<N, R>()), stream, a, T, x, y);
-  }
-  if (bp.corner_count > 0)
-    hipLaunchKernelGGL(k_brick_corners<R>, dim3(unsigned((bp.corner_count + 255) / 256)), dim3(256), 0, stream, y,
-                       bslots, reduced ? bp.corner_red : bp.corner_full, bp.corner_ptr, bp.corner_slot,
-                       bp.corner_count, store_first ? 0 : 1);
+  require(bp.halo_slots == 0 || slots != nullptr, GMF_EINVAL, "workspace has no brick halo slots");
+  BrickArgs<R> a;
+  a.map = reduced ? bp.map_red : bp.map_full;
+  a.halo_base = bp.halo_base;
+  a.halo_slot = bp.halo_slot;
+  a.jinv = CellArrays<R>::jinv(b);
+  a.coef = CellArrays<R>::coef(b);
+  a.halo = slots ? slots + bp.slot_base : nullptr;
+  a.accumulate = accumulate ? 1 : 0;
+  hipLaunchKernelGGL((brick_apply<N, R>), dim3(unsigned(bp.bricks)), dim3(8 * lanes_per_cell<N>()),
+                     (brick_lds_bytes<N, R>()), stream, a, T, x, y);
+  if (bp.shared_count > 0)
+    hipLaunchKernelGGL(k_brick_halo<R>, dim3(unsigned((bp.shared_count + 255) / 256)), dim3(256), 0, stream, y,
+                       a.halo, reduced ? bp.halo_red : bp.halo_full, bp.halo_ptr, bp.shared_count, a.accumulate);
   GMF_HIP(hipGetLastError());
 }
 
@@ -450,11 +494,9 @@ std::vector<double> collocation_derivative(const std::vector<double>& B, const s
 // dof may appear at most once per brick. Returns false when the cells do not
 // form such bricks.
 //
-// Dofs that sit only at brick corners (a vertex shared by up to eight
-// bricks) are summed through per-brick corner slots by k_brick_corners, so
-// two bricks conflict only when they share another dof. On a brick grid that
-// leaves four colors (each brick with the brick diagonally opposite across a
-// corner) instead of eight, i.e. four dependent launches per pass.
+// Dofs held by one brick only are stored by brick_apply directly; every
+// occurrence of a dof held by several bricks writes its own halo slot, and
+// k_brick_halo sums a dof's slots (contiguous, in brick order).
 bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full) {
   const int N = b.N[0];
   const int M = 2 * N - 1, L = M * M * M;
@@ -493,96 +535,59 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
       map[size_t(br * L + pos)] = g;
     }
   }
-  // Corner-only dofs: every occurrence at a brick corner.
-  std::vector<int> corner_of(static_cast<size_t>(L));
-  for (int pos = 0; pos < L; ++pos) corner_of[size_t(pos)] = lattice_corner(pos, M);
-  std::vector<uint8_t> corner_only(size_t(p.n_full), 1);
-  for (int64_t br = 0; br < nb; ++br)
-    for (int pos = 0; pos < L; ++pos) {
-      const int32_t g = map[size_t(br * L + pos)];
-      if (g >= 0 && corner_of[size_t(pos)] < 0) corner_only[size_t(g)] = 0;
+  // Shared dofs (held by more than one brick): their occurrences take
+  // consecutive halo slots per dof, in brick order; each brick lists the slots
+  // of its shared positions in ascending lattice position.
+  std::vector<int32_t> occ(static_cast<size_t>(p.n_full), 0);
+  for (size_t i = 0; i < map.size(); ++i)
+    if (map[i] >= 0) ++occ[size_t(map[i])];
+  std::vector<int32_t> shared_dofs, ptr(1, 0), next(static_cast<size_t>(p.n_full), -1);
+  for (int64_t g = 0; g < p.n_full; ++g)
+    if (occ[size_t(g)] > 1) {
+      if (int64_t(ptr.back()) + occ[size_t(g)] >= (int64_t(1) << 31)) return false;
+      next[size_t(g)] = ptr.back();
+      shared_dofs.push_back(int32_t(g));
+      ptr.push_back(ptr.back() + occ[size_t(g)]);
     }
-  // Greedy coloring of bricks in Morton order: bricks sharing a dof that is
-  // not corner-only get different colors.
-  std::vector<uint64_t> used(size_t(p.n_full), 0);
-  std::vector<int> color(static_cast<size_t>(nb));
-  int ncolors = 0;
+  const int64_t slots = ptr.back();
+  const int surface = M * M * M - (M - 2) * (M - 2) * (M - 2);
+  std::vector<int32_t> base(size_t(nb) + 1, 0), slot_of;
+  slot_of.reserve(size_t(slots));
+  std::vector<int32_t> enc_full(map.size()), enc_red(red_of_full.empty() ? 0 : map.size());
   for (int64_t br = 0; br < nb; ++br) {
-    uint64_t mask = 0;
+    base[size_t(br)] = int32_t(slot_of.size());
     for (int pos = 0; pos < L; ++pos) {
-      const int32_t g = map[size_t(br * L + pos)];
-      if (g >= 0 && !corner_only[size_t(g)]) mask |= used[size_t(g)];
-    }
-    int c = 0;
-    while (c < 64 && ((mask >> c) & 1u)) ++c;
-    if (c == 64) return false;
-    color[size_t(br)] = c;
-    ncolors = std::max(ncolors, c + 1);
-    for (int pos = 0; pos < L; ++pos) {
-      const int32_t g = map[size_t(br * L + pos)];
-      if (g >= 0 && !corner_only[size_t(g)]) used[size_t(g)] |= uint64_t(1) << c;
-    }
-  }
-  // Entry kinds: corner slot, first writer (the lowest color touching the
-  // dof), or read-modify-write.
-  std::vector<uint8_t> kind(size_t(nb) * size_t(L), uint8_t(kLatRmw));
-  std::vector<std::pair<int32_t, int32_t>> corners;  // (dof, slot)
-  for (int64_t br = 0; br < nb; ++br)
-    for (int pos = 0; pos < L; ++pos) {
-      const int32_t g = map[size_t(br * L + pos)];
-      if (g < 0) continue;
-      if (corner_only[size_t(g)]) {
-        kind[size_t(br * L + pos)] = uint8_t(kLatCorner);
-        corners.emplace_back(g, int32_t(br * 8 + corner_of[size_t(pos)]));
-        continue;
+      const size_t i = size_t(br * L + pos);
+      const int32_t g = map[i];
+      const bool exclusive = g < 0 || occ[size_t(g)] == 1;
+      if (!exclusive) slot_of.push_back(next[size_t(g)]++);
+      enc_full[i] = g >= 0 ? (g << 1) | int32_t(exclusive) : -1;
+      if (!red_of_full.empty()) {
+        const int64_t rg = g >= 0 ? red_of_full[size_t(g)] : -1;
+        enc_red[i] = rg >= 0 ? (int32_t(rg) << 1) | int32_t(exclusive) : (exclusive ? -1 : -2);
       }
-      const uint64_t m = used[size_t(g)];
-      int lowest = 0;
-      while (!((m >> lowest) & 1u)) ++lowest;
-      if (color[size_t(br)] == lowest) kind[size_t(br * L + pos)] = uint8_t(kLatFirst);
     }
+    if (int64_t(slot_of.size()) - base[size_t(br)] > surface) return false;  // the kernel's slot list bound
+  }
+  base[size_t(nb)] = int32_t(slot_of.size());
   BrickPath& bp = b.bricks;
-  bp.colors = ncolors;
-  bp.color_offsets.assign(size_t(ncolors) + 1, 0);
-  for (int64_t br = 0; br < nb; ++br) bp.color_offsets[size_t(color[size_t(br)]) + 1]++;
-  for (int c = 0; c < ncolors; ++c) bp.color_offsets[size_t(c) + 1] += bp.color_offsets[size_t(c)];
-  std::vector<int64_t> order(static_cast<size_t>(nb));
-  std::vector<int64_t> fillp(bp.color_offsets.begin(), bp.color_offsets.end() - 1);
-  for (int64_t br = 0; br < nb; ++br) order[size_t(fillp[size_t(color[size_t(br)])]++)] = br;
   bp.bricks = nb;
-  bp.order = upload_array(p, order.data(), order.size());
-  std::vector<int32_t> enc(map.size());
-  for (size_t i = 0; i < map.size(); ++i) enc[i] = map[i] >= 0 ? (map[i] << 2) | int32_t(kind[i]) : -1;
-  bp.map_full = upload_array(p, enc.data(), enc.size());
-  if (!red_of_full.empty()) {
-    for (size_t i = 0; i < map.size(); ++i)
-      enc[i] = map[i] >= 0 && red_of_full[size_t(map[i])] >= 0
-                   ? (int32_t(red_of_full[size_t(map[i])]) << 2) | int32_t(kind[i])
-                   : -1;
-    bp.map_red = upload_array(p, enc.data(), enc.size());
-  }
-  // Corner CSR, dofs ascending, slots ascending within a dof.
-  std::sort(corners.begin(), corners.end());
-  std::vector<int32_t> cfull, cred, cslot;
-  std::vector<int64_t> cptr(1, 0);
-  for (size_t i = 0; i < corners.size(); ++i) {
-    if (i == 0 || corners[i].first != corners[i - 1].first) {
-      if (i > 0) cptr.push_back(int64_t(i));
-      cfull.push_back(corners[i].first);
-      cred.push_back(red_of_full.empty() ? -1 : int32_t(red_of_full[size_t(corners[i].first)]));
-    }
-    cslot.push_back(corners[i].second);
-  }
-  if (!corners.empty()) cptr.push_back(int64_t(corners.size()));
-  bp.corner_count = int64_t(cfull.size());
-  if (bp.corner_count > 0) {
-    bp.corner_full = upload_array(p, cfull.data(), cfull.size());
-    bp.corner_red = upload_array(p, cred.data(), cred.size());
-    bp.corner_ptr = upload_array(p, cptr.data(), cptr.size());
-    bp.corner_slot = upload_array(p, cslot.data(), cslot.size());
+  bp.map_full = upload_array(p, enc_full.data(), enc_full.size());
+  if (!red_of_full.empty()) bp.map_red = upload_array(p, enc_red.data(), enc_red.size());
+  bp.halo_base = upload_array(p, base.data(), base.size());
+  bp.halo_slots = slots;
+  bp.shared_count = int64_t(shared_dofs.size());
+  if (slots > 0) bp.halo_slot = upload_array(p, slot_of.data(), slot_of.size());
+  if (bp.shared_count > 0) {
+    std::vector<int32_t> red(shared_dofs.size(), -1);
+    if (!red_of_full.empty())
+      for (size_t i = 0; i < shared_dofs.size(); ++i) red[i] = int32_t(red_of_full[size_t(shared_dofs[i])]);
+    bp.halo_full = upload_array(p, shared_dofs.data(), shared_dofs.size());
+    bp.halo_red = upload_array(p, red.data(), red.size());
+    bp.halo_ptr = upload_array(p, ptr.data(), ptr.size());
   }
   bp.slot_base = p.brick_slots;
-  p.brick_slots += nb * 8;
+  p.brick_slots += slots;
   bp.enabled = true;
   return true;
 }
@@ -628,13 +633,13 @@ bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool redu
   if (!b.fast || b.cells == 0) return false;
   if (reduced && !b.ridx) return false;
   if (b.bricks.enabled && (!reduced || b.bricks.map_red)) {
-    const bool store_first = p.buckets.size() == 1;
+    const bool accumulate = p.buckets.size() != 1;
     switch (b.N[0]) {
-      case 2: launch_bricks<2, R>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 3: launch_bricks<3, R>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 4: launch_bricks<4, R>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 5: launch_bricks<5, R>(b, x, y, reduced, store_first, slots, stream); return true;
-      case 6: launch_bricks<6, R>(b, x, y, reduced, store_first, slots, stream); return true;
+      case 2: launch_bricks<2, R>(b, x, y, reduced, accumulate, slots, stream); return true;
+      case 3: launch_bricks<3, R>(b, x, y, reduced, accumulate, slots, stream); return true;
+      case 4: launch_bricks<4, R>(b, x, y, reduced, accumulate, slots, stream); return true;
+      case 5: launch_bricks<5, R>(b, x, y, reduced, accumulate, slots, stream); return true;
+      case 6: launch_bricks<6, R>(b, x, y, reduced, accumulate, slots, stream); return true;
       default: break;
     }
   }
@@ -701,8 +706,8 @@ bool prepare_fp32(gmf_plan& p) {
 }
 
 bool plan_writes_every_dof(const gmf_plan& p) {
-  // The brick store-first path writes every dof of its lattice; with a single
-  // brick bucket covering the plan no zero fill of the result is needed.
+  // The owners of a brick bucket store every dof of its lattices; with a
+  // single brick bucket covering the plan no zero fill of the result is needed.
   return p.buckets.size() == 1 && p.buckets[0].bricks.enabled;
 }
 

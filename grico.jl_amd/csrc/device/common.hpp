@@ -31,26 +31,31 @@ inline void require(bool ok, gmf_status s, const std::string& msg) {
 constexpr int kMaxDim = 3;
 constexpr int kMaxExtent = 16;  // per-axis modes / points handled by the generic kernel
 
-// 2x2x2-brick decomposition of a fast-path bucket (cell_fast.hip).
+// 2x2x2-brick decomposition of a fast-path bucket (cell_fast.hip). All
+// bricks run in one launch. A dof held by one brick only is stored by that
+// brick; every occurrence of a dof held by several bricks (the bricks' shared
+// surfaces) writes its own halo slot, and k_brick_halo sums each such dof's
+// slots, which are contiguous and in brick order.
 struct BrickPath {
   bool enabled = false;
   int64_t bricks = 0;
-  int colors = 0;
-  std::vector<int64_t> color_offsets;  // host: bricks of color c are order[off[c]..off[c+1])
-  int64_t* order = nullptr;            // brick ids sorted by color
-  // bricks x (2N-1)^3 lattice entries (dof << 2) | kind, -1 = none; kind 0:
-  // read-modify-write, 1: first writer (plain store), 2: brick-corner dof
-  // whose result goes to the brick's corner slot.
+  // bricks x (2N-1)^3 lattice entries (dof << 1) | exclusive; -1: no dof;
+  // -2: no dof (a constrained dof of the reduced layout) at a shared position,
+  // which still takes its halo slot.
   int32_t* map_full = nullptr;
-  int32_t* map_red = nullptr;          // same with reduced dofs (fixed dofs -> -1)
-  // Corner dofs: dof i sums slots corner_slot[corner_ptr[i]..corner_ptr[i+1])
-  // (slot = brick * 8 + corner) into corner_full[i] / corner_red[i] (-1: none).
-  int64_t corner_count = 0;
-  int32_t* corner_full = nullptr;
-  int32_t* corner_red = nullptr;
-  int64_t* corner_ptr = nullptr;
-  int32_t* corner_slot = nullptr;
-  int64_t slot_base = 0;               // offset of this bucket's slots in the workspace array
+  int32_t* map_red = nullptr;  // same with reduced dofs
+  // Shared positions of brick b, in ascending lattice position, write the
+  // halo slots halo_slot[halo_base[b] .. halo_base[b + 1]) (bricks + 1 offsets).
+  int32_t* halo_base = nullptr;
+  int32_t* halo_slot = nullptr;
+  int64_t halo_slots = 0;      // halo slots of the bucket
+  // Shared dof i sums slots halo_ptr[i] .. halo_ptr[i+1]-1 into halo_full[i] /
+  // halo_red[i] (-1: no reduced dof).
+  int64_t shared_count = 0;
+  int32_t* halo_full = nullptr;
+  int32_t* halo_red = nullptr;
+  int32_t* halo_ptr = nullptr;
+  int64_t slot_base = 0;       // offset of this bucket's slots in the workspace array
 };
 
 // Sum-factorized path of a point-cloud batch of conforming faces
@@ -184,7 +189,7 @@ struct gmf_plan {
   bool selection_map = false;  // P is a 0/1 selection without shift terms in apply
   bool multi_terms = false;
   int64_t device_bytes = 0;
-  int64_t brick_slots = 0;  // corner slots of all brick buckets (workspace scratch)
+  int64_t brick_slots = 0;  // halo slots of all brick buckets (workspace scratch)
   int fp32_state = 0;       // 0: not prepared, 1: every bucket has FP32 kernels, -1: FP64 with conversion
   std::vector<gmf::GenericBatch> generic_batches;  // one launch per lanes-per-cell tier
   gmf::GenericBatch face_batch;                    // every face-tensor batch in one launch
@@ -202,7 +207,7 @@ struct gmf_workspace {
   double* pipelined_scalars = nullptr;  // pinned mirror of the CG loop's overlapped reads
   hipEvent_t scalar_event = nullptr;    // marks the arrival of pipelined_scalars
   int64_t partial_capacity = 0;
-  double* brick_slots = nullptr;   // plan->brick_slots corner results of the brick kernels
+  double* brick_slots = nullptr;   // plan->brick_slots halo slots of the brick kernels
   double* red64_in = nullptr;      // FP64 staging of FP32 reduced applies on plans without FP32 kernels
   double* red64_out = nullptr;
 };
@@ -256,7 +261,7 @@ void prepare_face_batch(gmf_plan& plan);
 void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
 void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
                        std::vector<char>& done);
-// slots: the workspace's brick corner array (plan.brick_slots entries).
+// slots: the workspace's brick halo array (plan.brick_slots entries).
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream);
 // lidx: host copy of the bucket's full-layout index map; red_of_full: full dof

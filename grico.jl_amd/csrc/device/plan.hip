@@ -839,11 +839,8 @@ gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity) 
                           (b.face.enabled ? " face-tensor nnz=" + std::to_string(b.face.nnz) : std::string())
                     : (b.bricks.enabled ? "brick" : (b.fast ? "fast-cell" : "generic")));
       if (b.bricks.enabled) {
-        s += " bricks=" + std::to_string(b.bricks.bricks) + " colors=" + std::to_string(b.bricks.colors) + " [";
-        for (int c = 0; c < b.bricks.colors; ++c)
-          s += std::to_string(b.bricks.color_offsets[size_t(c) + 1] - b.bricks.color_offsets[size_t(c)]) +
-               (c + 1 < b.bricks.colors ? "," : "");
-        s += "]";
+        s += " bricks=" + std::to_string(b.bricks.bricks) + " halo_slots=" + std::to_string(b.bricks.halo_slots) +
+             " shared_dofs=" + std::to_string(b.bricks.shared_count);
       }
       if (!b.cloud && !b.fast) s += " lanes=" + std::to_string(b.generic_threads);
       s += "\n";
