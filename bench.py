@@ -837,10 +837,13 @@ def lognormal_leg(repeats=3):
     return out
 
 
-def learned_leg(repeats=3, pairs=2000, steps=3000):
+def learned_leg(depth=5, repeats=3, pairs=2000, steps=3000):
     """Learned-correction FGMRES against GMG-CG on the log-normal p = 3
-    problem: time from x0 = 0 to ||r|| <= 1e-8 ||b||, hierarchy and network
-    already built; training reported separately."""
+    problem (32^3 cells at depth 5): time from x0 = 0 to ||r|| <= 1e-8 ||b||,
+    hierarchy and network already built, training reported separately. Both
+    methods use the same FP64 Chebyshev V-cycle (hierarchy cut at the first
+    level of at most 4096 dofs, solved there by the device dense inverse: the
+    faster of the two GMG-CG variants of the log-normal leg)."""
     import ctypes
 
     import numpy as np
@@ -848,16 +851,19 @@ def learned_leg(repeats=3, pairs=2000, steps=3000):
 
     import gricoamd as g
     from gricoamd import _devabi, _native, ops
-    from gricoamd.learned import CellMLP, LearnedBlockCorrection, train_block_correction
-    from gricoamd.multigrid import GeometricMultigridPreconditioner
-    from gricoamd.solve import compile_preconditioner
+    from gricoamd.learned import train_block_correction
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
 
     lib = _native.device()
     t0 = time.perf_counter()
-    problem = build_lognormal_problem()
+    problem = build_lognormal_problem(depth=depth)
     plan = g.compile(problem)
-    log(f"log-normal problem: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
-    net, report, M, V = train_block_correction(problem, plan=plan, pairs=pairs, steps=steps, log=log)
+    log(f"learned leg: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
+    counts = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"),
+                                         fine_plan=plan).reduced_dof_counts()
+    keep = len(counts) - max([i for i, c in enumerate(counts) if c <= 4096] or [0])
+    inner = GeometricMultigridPreconditioner(smoother="chebyshev", max_levels=keep, coarse_direct_dof_limit=4096)
+    net, report, M, V = train_block_correction(problem, plan=plan, pairs=pairs, steps=steps, inner=inner, log=log)
     b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
     x = torch.zeros_like(b)
     dev = ops._ws(plan)
@@ -865,6 +871,7 @@ def learned_leg(repeats=3, pairs=2000, steps=3000):
     def timed(fn):
         fn()
         ts = []
+        info = None
         for _ in range(repeats):
             torch.cuda.synchronize()
             t = time.perf_counter()
@@ -888,6 +895,10 @@ def learned_leg(repeats=3, pairs=2000, steps=3000):
         _devabi.check(status)
         return info
 
+    def l2_norm(xr):
+        st = g.State(plan, ops.expand_reduced(plan, xr, include_shift=True).cpu().numpy())
+        return g.l2_error(st, lambda p: 0.0 * p[0])
+
     t_gmg, i_gmg = timed(lambda: cg(V.handle))
     x_gmg = x.clone()
     t_l, i_l = timed(lambda: fgmres(M.handle))
@@ -895,19 +906,23 @@ def learned_leg(repeats=3, pairs=2000, steps=3000):
     if fallback:
         log("learned FGMRES did not converge: GMG-CG fallback")
         t_l, i_l = t_gmg, i_gmg
+        x.copy_(x_gmg)
     x_l = x.clone()
     t_0, i_0 = timed(lambda: fgmres(V.handle))
-    diff = float(torch.linalg.norm(x_l - x_gmg) / torch.linalg.norm(x_gmg))
+    n_gmg = l2_norm(x_gmg)
     out = {
         "metric": "time-to-1e-8 relative residual, learned-correction FGMRES vs GMG-CG (s)",
-        "workload": "3D Poisson 32^3 cells (uniform refinement of one root), p=3, log-normal kappa "
+        "workload": f"3D Poisson {2 ** depth}^3 cells (uniform refinement of one root), p=3, log-normal kappa "
                     "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
         "reduced_dofs": plan.reduced_dof_count,
+        "vcycle_levels": [int(c) for c in counts[-keep:]],
         "gmg_cg": {"seconds": t_gmg, "iterations": i_gmg.iterations},
         "learned_fgmres": {"seconds": t_l, "iterations": i_l.iterations if i_l else None, "fallback": fallback},
         "fgmres_vcycle_only": {"seconds": t_0, "iterations": i_0.iterations if i_0 else None},
         "speedup_vs_gmg_cg": t_gmg / t_l,
-        "solution_relative_difference": diff,
+        "target_speedup": 2.0,
+        "solution_relative_difference": float(torch.linalg.norm(x_l - x_gmg) / torch.linalg.norm(x_gmg)),
+        "solution_l2_norm_relative_difference": abs(l2_norm(x_l) - n_gmg) / n_gmg,
         "training": report,
         "network": "per-cell MLP 64-256-256-64, BF16 weights on MFMA, FP32 accumulation",
     }
@@ -1202,8 +1217,11 @@ def main():
     ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
     ap.add_argument("--fcm-iterations", type=int, default=30, help="CG iterations timed in the finite-cell leg")
     ap.add_argument("--learned", action="store_true",
-                    help="run the learned-correction FGMRES leg on the 32^3 p=3 log-normal problem (off by default: "
-                         "its 2000-residual collection faulted the GPU once and the cause is not yet found)")
+                    help="run the learned-correction FGMRES leg on the log-normal p=3 problem")
+    ap.add_argument("--learned-only", action="store_true", help="run only the learned leg and print its JSON")
+    ap.add_argument("--learned-depth", type=int, default=5, help="uniform refinements of the learned leg (5 -> 32^3)")
+    ap.add_argument("--learned-pairs", type=int, default=2000, help="training residuals of the learned leg")
+    ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned leg")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="after timing, copy a 1 GiB buffer three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -1226,6 +1244,10 @@ def main():
         return
     if args.transfer_only:
         print(json.dumps({"transfer": transfer_leg()}), flush=True)
+        return
+    if args.learned_only:
+        print(json.dumps({"learned": learned_leg(depth=args.learned_depth, pairs=args.learned_pairs,
+                                                 steps=args.learned_steps)}), flush=True)
         return
     if args.fcm_only:
         print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)}), flush=True)
@@ -1355,7 +1377,7 @@ def main():
             log(f"fcm leg failed: {exc!r}")
             result["fcm"] = {"error": repr(exc)}
     if args.learned:
-        result["learned"] = learned_leg()
+        result["learned"] = learned_leg(depth=args.learned_depth, pairs=args.learned_pairs, steps=args.learned_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

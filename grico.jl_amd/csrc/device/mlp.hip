@@ -286,6 +286,24 @@ __global__ void k_reciprocal(double* __restrict__ a, int64_t n) {
   GMF_STRIDE(i, n) a[i] = a[i] > 0.0 ? 1.0 / a[i] : 0.0;
 }
 
+// x_i ~ N(0, 1), counter-based: SplitMix64 of (seed, i) -> two uniforms -> Box-Muller.
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void k_normal(double* __restrict__ x, int64_t n, uint64_t seed) {
+  GMF_STRIDE(i, n) {
+    const uint64_t a = splitmix64(seed * 0x100000001B3ull + 2 * uint64_t(i));
+    const uint64_t b = splitmix64(seed * 0x100000001B3ull + 2 * uint64_t(i) + 1);
+    const double u1 = (double(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+    const double u2 = double(b >> 11) * 0x1.0p-53;
+    x[i] = sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+  }
+}
+
 __global__ void k_sub(double* __restrict__ out, const double* __restrict__ a, const double* __restrict__ b, int64_t n) {
   GMF_STRIDE(i, n) out[i] = a[i] - b[i];
 }
@@ -431,36 +449,103 @@ struct LearnedPrecond : gmf_preconditioner {
     GMF_HIP(hipGetLastError());
   }
 
-  // Energy-norm training on residuals r'_k left by the inner preconditioner:
-  // minimise ||e' - c||_A^2 = e'^T A e' - 2 c^T r' + c^T A c over the network,
-  // one residual per Adam step. history[s] = ||e' - c||_A^2 / ||e'||_A^2
-  // before the step (energy[k] = e'^T r' = ||e'||_A^2).
-  void train(const double* residuals, const double* energy, int64_t pairs, int64_t steps, float lr, float b1, float b2,
-             float eps, double* history) {
+  // One step of the energy-norm objective on residual rk (energy en = e'^T r'):
+  // the correction c of the network, q = A c, the loss ||e' - c||_A^2 / en =
+  // 1 + (c^T q - 2 c^T r') / en (returned), and its parameter gradient in
+  // net->grad (no parameter update). c and gq: n doubles, dY: cells x width floats.
+  double energy_step(const double* rk, double en, double* c, double* gq, float* dY) {
     const int64_t n = ws->plan->n_red;
     hipStream_t s = ws->stream;
-    DeviceBuffer<double> c(static_cast<size_t>(n)), gq(static_cast<size_t>(n));
-    DeviceBuffer<float> dY(size_t(cells) * size_t(width));
     net->stream = s;
+    fill(c, 0.0, n, s);
+    correction(rk, c);
+    reduced_apply(ws, c, q.p);
+    dot(ws, c, q.p, n, kTmp);
+    dot(ws, c, rk, n, kTmp + 1);
+    read_scalars(ws, kTmp + 2);
+    const double cq = ws->host_scalars[kTmp], cr = ws->host_scalars[kTmp + 1];
+    hipLaunchKernelGGL(k_energy_gradient, dim3(grid1(n)), dim3(256), 0, s, gq, q.p, rk, 2.0 / en, n);
+    hipLaunchKernelGGL(k_patch_gradient, dim3(grid1(cells * width)), dim3(256), 0, s, dY, gq, scale.p, cw.p, pu.p, idx,
+                       cells, width);
+    GMF_HIP(hipGetLastError());
+    net->backward_from(dY, cells);
+    return 1.0 + (cq - 2.0 * cr) / en;
+  }
+
+  // Training pairs from GMG-CG solves with the inner preconditioner V: for
+  // reference solutions x* ~ N(0, 1) (seeded, one per solve) and right-hand
+  // sides b = A x*, at every CG iterate x_k the pair y = x_k + V(b - A x_k),
+  // r' = b - A y (stored as an FP32 row of residuals), e' = x* - y and
+  // energy e'^T r'. Up to `iterations` pairs per solve; a solve also ends when
+  // its residual drops below 1e-12 ||b||.
+  void collect(int64_t pairs, int iterations, uint64_t seed, float* residuals, double* energies) {
+    const int64_t n = ws->plan->n_red;
+    hipStream_t s = ws->stream;
+    require(inner != nullptr, GMF_EINVAL, "collecting training pairs needs an inner preconditioner");
+    inner->ws->stream = s;
+    DeviceBuffer<double> xs(static_cast<size_t>(n)), b(static_cast<size_t>(n)), x(static_cast<size_t>(n)),
+        r(static_cast<size_t>(n)), z(static_cast<size_t>(n)), p(static_cast<size_t>(n)), y(static_cast<size_t>(n)),
+        e1(static_cast<size_t>(n));
+    const double* hs = ws->host_scalars;
+    auto dot1 = [&](const double* u, const double* v) {
+      dot(ws, u, v, n, kTmp);
+      read_scalars(ws, kTmp + 1);
+      return hs[kTmp];
+    };
+    int64_t got = 0;
+    for (uint64_t solve = 0; got < pairs; ++solve) {
+      hipLaunchKernelGGL(k_normal, dim3(grid1(n)), dim3(256), 0, s, xs.p, n, seed + solve);
+      GMF_HIP(hipGetLastError());
+      reduced_apply(ws, xs.p, b.p);
+      fill(x.p, 0.0, n, s);
+      copy(r.p, b.p, n, s);
+      inner->apply(r.p, z.p);
+      copy(p.p, z.p, n, s);
+      double rz = dot1(r.p, z.p);
+      const double bnorm = std::sqrt(dot1(b.p, b.p));
+      for (int it = 0; it < iterations && got < pairs; ++it) {
+        copy(y.p, x.p, n, s);
+        axpby(ws, y.p, 1.0, z.p, 1.0, n);
+        reduced_apply(ws, y.p, q.p);
+        copy(r1.p, b.p, n, s);
+        axpby(ws, r1.p, -1.0, q.p, 1.0, n);
+        copy(e1.p, xs.p, n, s);
+        axpby(ws, e1.p, -1.0, y.p, 1.0, n);
+        narrow(residuals + got * n, r1.p, n, s);
+        energies[got] = dot1(e1.p, r1.p);
+        ++got;
+        reduced_apply(ws, p.p, q.p);
+        const double pq = dot1(p.p, q.p);
+        require(pq > 0.0 && std::isfinite(pq), GMF_EBREAKDOWN, "CG encountered a non-positive operator direction");
+        const double alpha = rz / pq;
+        axpby(ws, x.p, alpha, p.p, 1.0, n);
+        axpby(ws, r.p, -alpha, q.p, 1.0, n);
+        if (std::sqrt(dot1(r.p, r.p)) <= 1e-12 * bnorm) break;
+        inner->apply(r.p, z.p);
+        const double rz_new = dot1(r.p, z.p);
+        axpby(ws, p.p, 1.0, z.p, rz_new / rz, n);
+        rz = rz_new;
+      }
+    }
+    sync(ws);
+  }
+
+  // Energy-norm training on residuals r'_k left by the inner preconditioner
+  // (FP32 rows, widened per step): minimise ||e' - c||_A^2 = e'^T A e' -
+  // 2 c^T r' + c^T A c over the network, one residual per Adam step.
+  // history[s] = ||e' - c||_A^2 / ||e'||_A^2 before the step (energy[k] =
+  // e'^T r' = ||e'||_A^2).
+  void train(const float* residuals, const double* energy, int64_t pairs, int64_t steps, float lr, float b1, float b2,
+             float eps, double* history) {
+    const int64_t n = ws->plan->n_red;
+    DeviceBuffer<double> c(static_cast<size_t>(n)), gq(static_cast<size_t>(n)), rk(static_cast<size_t>(n));
+    DeviceBuffer<float> dY(size_t(cells) * size_t(width));
     for (int64_t st = 0; st < steps; ++st) {
       const int64_t k = st % pairs;
-      const double* rk = residuals + k * n;
-      fill(c.p, 0.0, n, s);
-      correction(rk, c.p);
-      reduced_apply(ws, c.p, q.p);
-      double cq = 0.0, cr = 0.0;
-      dot(ws, c.p, q.p, n, kTmp);
-      dot(ws, c.p, rk, n, kTmp + 1);
-      read_scalars(ws, kTmp + 2);
-      cq = ws->host_scalars[kTmp];
-      cr = ws->host_scalars[kTmp + 1];
       const double en = energy[k] > 0.0 ? energy[k] : 1.0;
-      if (history) history[st] = 1.0 + (cq - 2.0 * cr) / en;
-      hipLaunchKernelGGL(k_energy_gradient, dim3(grid1(n)), dim3(256), 0, s, gq.p, q.p, rk, 2.0 / en, n);
-      hipLaunchKernelGGL(k_patch_gradient, dim3(grid1(cells * width)), dim3(256), 0, s, dY.p, gq.p, scale.p, cw.p,
-                         pu.p, idx, cells, width);
-      GMF_HIP(hipGetLastError());
-      net->backward_from(dY.p, cells);
+      widen(rk.p, residuals + k * n, n, ws->stream);
+      const double loss = energy_step(rk.p, en, c.p, gq.p, dY.p);
+      if (history) history[st] = loss;
       net->adam(lr, b1, b2, eps);
     }
     sync(ws);
@@ -686,7 +771,18 @@ gmf_status gmf_precond_create_learned(gmf_workspace* ws, gmf_preconditioner* inn
   });
 }
 
-gmf_status gmf_learned_train(gmf_preconditioner* learned, const double* residuals, const double* energies,
+gmf_status gmf_learned_collect(gmf_preconditioner* learned, int64_t pairs, int32_t iterations, uint64_t seed,
+                               float* residuals, double* energies) {
+  return guarded_call([&] {
+    auto* L = dynamic_cast<LearnedPrecond*>(learned);
+    require(L != nullptr, GMF_EINVAL, "not a learned-correction preconditioner");
+    require(residuals && energies && pairs > 0 && iterations > 0, GMF_EINVAL, "invalid argument");
+    require_device_pointer(residuals, "residuals");
+    L->collect(pairs, iterations, seed, residuals, energies);
+  });
+}
+
+gmf_status gmf_learned_train(gmf_preconditioner* learned, const float* residuals, const double* energies,
                              int64_t pairs, int64_t steps, float lr, float beta1, float beta2, float eps,
                              double* history) {
   return guarded_call([&] {
@@ -697,6 +793,24 @@ gmf_status gmf_learned_train(gmf_preconditioner* learned, const double* residual
             "invalid Adam parameters");
     require_device_pointer(residuals, "residuals");
     L->train(residuals, energies, pairs, steps, lr, beta1, beta2, eps, history);
+  });
+}
+
+gmf_status gmf_learned_gradient(gmf_preconditioner* learned, const double* residual, double energy, double* loss,
+                                float* grad_host) {
+  return guarded_call([&] {
+    auto* L = dynamic_cast<LearnedPrecond*>(learned);
+    require(L != nullptr, GMF_EINVAL, "not a learned-correction preconditioner");
+    require(residual && std::isfinite(energy) && energy > 0.0, GMF_EINVAL, "invalid argument");
+    require_device_pointer(residual, "residual");
+    const int64_t n = L->ws->plan->n_red;
+    DeviceBuffer<double> c(static_cast<size_t>(n)), gq(static_cast<size_t>(n));
+    DeviceBuffer<float> dY(size_t(L->cells) * size_t(L->width));
+    const double l = L->energy_step(residual, energy, c.p, gq.p, dY.p);
+    sync(L->ws);
+    if (loss) *loss = l;
+    if (grad_host)
+      GMF_HIP(hipMemcpy(grad_host, L->net->grad.p, size_t(L->net->nparam) * sizeof(float), hipMemcpyDeviceToHost));
   });
 }
 

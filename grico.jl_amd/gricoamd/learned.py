@@ -11,14 +11,16 @@ of different conductivity. The preconditioner varies with the data it sees, so
 it is used inside FGMRES (src/solve.jl:80-88 allows that); the seam is the
 preconditioner policy interface (src/solve.jl:32, 641-670, 753-774).
 
-Training data come from the GMG-CG solves themselves: for reference
-solutions x* and the CG iterates x_k, the pair (r', e') = (b - A y, x* - y)
-with y = x_k + V(b - A x_k) is the residual / error the V-cycle leaves. The
-network (BF16 weights, FP32 accumulation on MFMA) is trained on the device
-with Adam to minimise the energy norm of the error left after its correction,
-||e' - c(r')||_A^2 = e'^T A e' - 2 c^T r' + c^T A c, which is what the outer
-Krylov method sees (gmf_learned_train). A supervised per-cell fit of patches
-(gmf_mlp_train on gmf_cell_patches output) is available as well.
+Training data come from the GMG-CG solves themselves, collected on the device
+(gmf_learned_collect): for seeded reference solutions x* and the CG iterates
+x_k, the pair (r', e') = (b - A y, x* - y) with y = x_k + V(b - A x_k) is the
+residual / error the V-cycle leaves; the residuals are kept as FP32 rows with
+the energies e'^T r'. The network (BF16 weights, FP32 accumulation on MFMA) is
+trained on the device with Adam to minimise the energy norm of the error left
+after its correction, ||e' - c(r')||_A^2 = e'^T A e' - 2 c^T r' + c^T A c,
+which is what the outer Krylov method sees (gmf_learned_train). A supervised
+fit of per-cell patches (gmf_mlp_train on gmf_cell_patches output) is
+available for networks used outside the preconditioner.
 """
 
 import ctypes
@@ -52,7 +54,9 @@ def _declare(lib):
     lib.gmf_vector_dot.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, ctypes.POINTER(c_double)]
     lib.gmf_learned_train.argtypes = [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_float, c_float, c_float,
                                       c_float, c_void_p]
-    for name in ("gmf_learned_train", "gmf_mlp_create", "gmf_mlp_destroy", "gmf_mlp_set_stream", "gmf_mlp_parameter_count",
+    lib.gmf_learned_gradient.argtypes = [c_void_p, c_void_p, c_double, ctypes.POINTER(c_double), c_void_p]
+    lib.gmf_learned_collect.argtypes = [c_void_p, c_int64, c_int32, ctypes.c_uint64, c_void_p, c_void_p]
+    for name in ("gmf_learned_train", "gmf_learned_gradient", "gmf_learned_collect", "gmf_mlp_create", "gmf_mlp_destroy", "gmf_mlp_set_stream", "gmf_mlp_parameter_count",
                  "gmf_mlp_get_parameters", "gmf_mlp_forward", "gmf_mlp_gradient", "gmf_mlp_train", "gmf_cell_patches",
                  "gmf_precond_create_learned", "gmf_vector_axpby", "gmf_vector_dot"):
         getattr(lib, name).restype = ctypes.c_int
@@ -166,134 +170,20 @@ def _ws(plan):
     return ops._ws(plan).ws
 
 
-def _axpby(plan, y, a, x, b):
-    _devabi.check(_native.device().gmf_vector_axpby(_ws(plan), y.data_ptr(), float(a), x.data_ptr(), float(b),
-                                                    y.numel()))
-
-
-def _dot(plan, x, y):
-    out = ctypes.c_double()
-    _devabi.check(_native.device().gmf_vector_dot(_ws(plan), x.data_ptr(), y.data_ptr(), x.numel(),
-                                                  ctypes.byref(out)))
-    return out.value
-
-
-def _rapply(plan, x, y):
-    _devabi.check(_native.device().gmf_reduced_apply(_ws(plan), x.data_ptr(), y.data_ptr()))
-
-
-def collect_pairs(plan, vcycle, pairs=2000, cells_per_pair=256, iterations=None, seed=0, rtol=1e-12):
-    """(X, T) training patches from GMG-CG solves (module docstring). Returns
-    device tensors of shape (pairs * cells_per_pair, width)."""
+def collect_residuals(learned, pairs=2000, iterations=40, seed=0):
+    """Training residuals from GMG-CG solves with the learned preconditioner's
+    inner V-cycle, collected on the device (gmf_learned_collect, module
+    docstring): a (pairs x n_red) float32 device tensor of residuals r' and a
+    host array of energies e'^T r'."""
     torch = ops._torch()
     lib = _native.device()
     _declare(lib)
-    n = plan.reduced_dof_count
-    b0 = plan.buckets[0]
-    width = b0.n_modes
-    ncell = b0.cell_count
-    cw = torch.as_tensor(cell_weights(plan), device="cuda")
-    rng = np.random.default_rng(seed)
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    iterations = iterations or 40
-    solves = int(math.ceil(pairs / iterations))
-    X = torch.empty((pairs * cells_per_pair, width), dtype=torch.float32, device="cuda")
-    T = torch.empty_like(X)
-    f64 = dict(dtype=torch.float64, device="cuda")
-    xs, b, x, r, z, p, q, y, r1, e1 = (torch.empty(n, **f64) for _ in range(10))
-    got = 0
-    for s in range(solves):
-        if got >= pairs:
-            break
-        # reference solution: rough random field plus a smooth mode mix
-        xs.normal_(generator=gen)
-        _rapply(plan, xs, b)
-        # GMG-CG from zero, recording the V-cycle's leftover residual/error
-        x.zero_()
-        r.copy_(b)
-        z.copy_(vcycle.apply(r))
-        p.copy_(z)
-        rz = _dot(plan, r, z)
-        bnorm = math.sqrt(_dot(plan, b, b))
-        for k in range(iterations):
-            if got >= pairs:
-                break
-            # pair at the current iterate: y = x + V(r)
-            y.copy_(x)
-            _axpby(plan, y, 1.0, z, 1.0)
-            _rapply(plan, y, q)
-            r1.copy_(b)
-            _axpby(plan, r1, -1.0, q, 1.0)
-            e1.copy_(xs)
-            _axpby(plan, e1, -1.0, y, 1.0)
-            cells = torch.as_tensor(rng.choice(ncell, size=cells_per_pair, replace=cells_per_pair > ncell),
-                                    dtype=torch.int64, device="cuda")
-            rows = slice(got * cells_per_pair, (got + 1) * cells_per_pair)
-            xr, tr = X[rows], T[rows]
-            _devabi.check(lib.gmf_cell_patches(_ws(plan), r1.data_ptr(), e1.data_ptr(), cells.data_ptr(),
-                                               cells_per_pair, cw.data_ptr(), xr.data_ptr(), tr.data_ptr(), None))
-            got += 1
-            # CG step
-            _rapply(plan, p, q)
-            alpha = rz / _dot(plan, p, q)
-            _axpby(plan, x, alpha, p, 1.0)
-            _axpby(plan, r, -alpha, q, 1.0)
-            if math.sqrt(_dot(plan, r, r)) <= rtol * bnorm:
-                break
-            z.copy_(vcycle.apply(r))
-            rz_new = _dot(plan, r, z)
-            _axpby(plan, p, 1.0, z, rz_new / rz)
-            rz = rz_new
-    X, T = X[:got * cells_per_pair], T[:got * cells_per_pair]
-    perm = torch.as_tensor(rng.permutation(X.shape[0]), device="cuda")
-    return X.index_select(0, perm).contiguous(), T.index_select(0, perm).contiguous(), got
-
-
-def collect_residuals(plan, vcycle, pairs=2000, iterations=40, seed=0, rtol=1e-12):
-    """Residuals r' = b - A y and energies e'^T r' from GMG-CG solves (module
-    docstring): a (pairs x n_red) float64 device tensor and a host array."""
-    torch = ops._torch()
-    n = plan.reduced_dof_count
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed)
-    f64 = dict(dtype=torch.float64, device="cuda")
-    R = torch.empty((pairs, n), **f64)
+    n = learned.plan.reduced_dof_count
+    R = torch.empty((pairs, n), dtype=torch.float32, device="cuda")
     energy = np.zeros(pairs)
-    xs, b, x, r, z, p, q, y, e1 = (torch.empty(n, **f64) for _ in range(9))
-    got = 0
-    while got < pairs:
-        xs.normal_(generator=gen)
-        _rapply(plan, xs, b)
-        x.zero_()
-        r.copy_(b)
-        z.copy_(vcycle.apply(r))
-        p.copy_(z)
-        rz = _dot(plan, r, z)
-        bnorm = math.sqrt(_dot(plan, b, b))
-        for _ in range(iterations):
-            if got >= pairs:
-                break
-            r1 = R[got]
-            y.copy_(x)
-            _axpby(plan, y, 1.0, z, 1.0)
-            _rapply(plan, y, q)
-            r1.copy_(b)
-            _axpby(plan, r1, -1.0, q, 1.0)
-            e1.copy_(xs)
-            _axpby(plan, e1, -1.0, y, 1.0)
-            energy[got] = _dot(plan, e1, r1)
-            got += 1
-            _rapply(plan, p, q)
-            alpha = rz / _dot(plan, p, q)
-            _axpby(plan, x, alpha, p, 1.0)
-            _axpby(plan, r, -alpha, q, 1.0)
-            if math.sqrt(_dot(plan, r, r)) <= rtol * bnorm:
-                break
-            z.copy_(vcycle.apply(r))
-            rz_new = _dot(plan, r, z)
-            _axpby(plan, p, 1.0, z, rz_new / rz)
-            rz = rz_new
+    _ws(learned.plan)  # the plan's stream is torch's current stream
+    _devabi.check(lib.gmf_learned_collect(learned.handle, int(pairs), int(iterations), int(seed), R.data_ptr(),
+                                          energy.ctypes.data_as(ctypes.c_void_p)))
     return R, energy
 
 
@@ -327,9 +217,10 @@ class LearnedBlockCorrection:
 
 def train_block_correction(problem, plan=None, widths=None, pairs=2000, steps=3000, learning_rate=1e-3, seed=0,
                            inner=None, iterations=40, log=None):
-    """Build the GMG hierarchy, collect V-cycle residuals from GMG-CG solves
-    and train a CellMLP in the energy norm. Returns (network, report,
-    compiled learned preconditioner, compiled V-cycle)."""
+    """Build the GMG hierarchy (policy `inner`, Chebyshev smoothing by
+    default), collect V-cycle residuals from GMG-CG solves on the device and
+    train a CellMLP in the energy norm. Returns (network, report, compiled
+    learned preconditioner, compiled V-cycle)."""
     from .plan import compile as compile_plan
     torch = ops._torch()
     plan = plan or compile_plan(problem)
@@ -339,12 +230,12 @@ def train_block_correction(problem, plan=None, widths=None, pairs=2000, steps=30
     V = hierarchy.device_preconditioner()
     width = plan.buckets[0].n_modes
     widths = widths or (width, 256, 256, width)
-    t1 = time.perf_counter()
-    R, energy = collect_residuals(plan, V, pairs=pairs, iterations=iterations, seed=seed)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
     net = CellMLP(widths, seed=seed)
     M = LearnedBlockCorrection(net, inner=V).compile(plan)
+    t1 = time.perf_counter()
+    R, energy = collect_residuals(M, pairs=pairs, iterations=iterations, seed=seed)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
     hist = np.zeros(steps)
     _devabi.check(_native.device().gmf_learned_train(M.handle, R.data_ptr(), energy.ctypes.data_as(ctypes.c_void_p),
                                                      pairs, steps, float(learning_rate), 0.9, 0.999, 1e-8,

@@ -387,14 +387,30 @@ gmf_status gmf_cell_patches(gmf_workspace* ws, const double* r, const double* e,
 gmf_status gmf_precond_create_learned(gmf_workspace* ws, gmf_preconditioner* inner, gmf_mlp* net,
                                       const float* cell_weight, gmf_preconditioner** out);
 
+/* Training pairs for the learned preconditioner from GMG-CG solves with its
+ * inner preconditioner V (required): per solve, a seeded reference solution
+ * x* ~ N(0, 1) and b = A x*; at each CG iterate x_k (at most `iterations` per
+ * solve), y = x_k + V(b - A x_k), residual r' = b - A y (row of `residuals`,
+ * DEVICE, pairs x n_red, FP32) and energy e'^T r' with e' = x* - y (HOST,
+ * pairs). Runs on the preconditioner's workspace stream. */
+gmf_status gmf_learned_collect(gmf_preconditioner* learned, int64_t pairs, int32_t iterations, uint64_t seed,
+                               float* residuals, double* energies);
+
 /* Energy-norm training of the learned preconditioner's network: for each
- * residual r'_k (DEVICE, pairs x n_red, row k = pair k) left by the inner
+ * residual r'_k (DEVICE, pairs x n_red FP32, row k = pair k) left by the inner
  * preconditioner, minimise ||e'_k - c(r'_k)||_A^2 = e'^T A e' - 2 c^T r' + c^T A c
  * with one Adam step per residual (cycling). energies (HOST) = e'_k^T r'_k;
  * history (HOST, steps entries, may be NULL) records ||e' - c||_A^2 / ||e'||_A^2. */
-gmf_status gmf_learned_train(gmf_preconditioner* learned, const double* residuals, const double* energies,
+gmf_status gmf_learned_train(gmf_preconditioner* learned, const float* residuals, const double* energies,
                              int64_t pairs, int64_t steps, float learning_rate, float beta1, float beta2, float eps,
                              double* history);
+
+/* The same objective on one residual without a parameter update: loss =
+ * ||e' - c(r')||_A^2 / energy (energy = e'^T r' > 0) and its gradient over the
+ * packed FP32 parameters (HOST array of gmf_mlp_parameter_count entries; may be
+ * NULL). residual: DEVICE, n_red. The gradient checks of the training path use it. */
+gmf_status gmf_learned_gradient(gmf_preconditioner* learned, const double* residual, double energy, double* loss,
+                                float* grad_host);
 
 /* Timed repetitions of gmf_apply on the workspace stream with HIP events
  * around each stage: average milliseconds of the cell-kernel pass and of the

@@ -124,3 +124,52 @@ def test_trained_correction_reduces_iterations(gpu):
     print("iterations with / without the network:", with_net.info.iterations, without.info.iterations, report)
     assert with_net.info.converged == 1 and without.info.converged == 1
     assert with_net.info.iterations < without.info.iterations
+
+
+def test_energy_gradient_matches_oracle(gpu):
+    """One step of the energy-norm training objective (gmf_learned_gradient:
+    patches, network, per-cell rescaling, shared-dof averaging, q = A c, loss
+    and backpropagation) against the oracle restatement with the device's BF16
+    rounding, on a 4^3 p = 3 log-normal plan."""
+    import ctypes
+
+    import pyoracle
+    from gricoamd import _devabi, _native
+    from gricoamd.learned import _declare
+    from gricoamd.solve import compile_preconditioner
+    pr = P.lognormal_poisson(depth=2, degree=3, seed=8)
+    plan = g.compile(pr)
+    widths = (64, 256, 256, 64)
+    rng = np.random.default_rng(3)
+    p = init_parameters(widths, seed=9)
+    p[-64:] = rng.standard_normal(64).astype(np.float32) * 0.05
+    net = CellMLP(widths, parameters=p)
+    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
+    M = LearnedBlockCorrection(net, inner=J).compile(plan)
+    A = pyoracle.reduced_matrix(plan)
+    r = rng.standard_normal(plan.reduced_dof_count)
+    e = np.linalg.solve(A, r)
+    energy = float(e @ A @ e)
+    lib = _native.device()
+    _declare(lib)
+    loss = ctypes.c_double()
+    grad = np.zeros(net.parameter_count, dtype=np.float32)
+    rt = gpu.as_tensor(r, device="cuda")
+    _devabi.check(lib.gmf_learned_gradient(M.handle, rt.data_ptr(), energy, ctypes.byref(loss),
+                                           grad.ctypes.data_as(ctypes.c_void_p)))
+    b = plan.buckets[0]
+    dofs = np.asarray(b.local_index)
+    ridx = np.where(dofs >= 0, plan.reduced_index[np.maximum(dofs, 0)], -1)
+    cw = cell_weights(plan).astype(np.float64)
+    rl, rg = MO.energy_loss_and_gradient(p, widths, A, r, energy, ridx, cw, bf16=True)
+    assert loss.value == pytest.approx(rl, rel=1e-4)
+    o = 0
+    for k, n in zip(widths[:-1], widths[1:]):
+        for size in (k * n, n):
+            a, ref = grad[o:o + size], rg[o:o + size]
+            assert np.linalg.norm(a - ref) <= 2e-2 * np.linalg.norm(ref) + 1e-9
+            o += size
+    # and the exact (float64) objective, to the BF16 network's accuracy
+    l64, g64 = MO.energy_loss_and_gradient(p, widths, A, r, energy, ridx, cw, bf16=False)
+    assert loss.value == pytest.approx(l64, rel=2e-2)
+    assert np.linalg.norm(grad - g64) <= 5e-2 * np.linalg.norm(g64)
