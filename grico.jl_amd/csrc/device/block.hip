@@ -205,9 +205,9 @@ void reduced_apply_block(gmf_workspace* ws, const double* X, double* Y, int k) {
   for (int j = 0; j < k; ++j) {
     const double* x = X + int64_t(j) * n;
     double* y = Y + int64_t(j) * n;
-    std::vector<char> batched;
-    launch_generic_batches(p, x, y, true, ws->stream, batched);
-    launch_face_batch(p, x, y, true, ws->stream, batched);
+    BucketFlags batched(p.buckets.size());
+    launch_generic_batches(p, x, y, true, ws->stream, batched.data);
+    launch_face_batch(p, x, y, true, ws->stream, batched.data);
     for (size_t i = 0; i < p.buckets.size(); ++i) {
       const Bucket& b = p.buckets[i];
       if (b.cloud) {

@@ -529,8 +529,7 @@ void prepare_generic_batch(gmf_plan& p) {
 }
 
 void launch_generic_batches(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream,
-                            std::vector<char>& done) {
-  done.assign(p.buckets.size(), 0);
+                            char* done) {
   for (const GenericBatch& g : p.generic_batches) {
     const void* args = reduced ? g.args_red : g.args_full;
     if (args == nullptr) continue;

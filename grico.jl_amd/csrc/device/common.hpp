@@ -235,10 +235,26 @@ void tune_generic(gmf_plan& plan);
 // Batch the plan's generic-kernel buckets into one launch per lanes-per-cell
 // tier (tiers holding at least two buckets).
 void prepare_generic_batch(gmf_plan& plan);
-// Launch every batch; `done[i]` is set for the buckets they covered. False
-// entries must be launched per bucket (no batch, or no reduced maps).
+// Launch every batch; `done[i]` (one flag per bucket, cleared by the caller)
+// is set for the buckets they covered. Clear entries must be launched per
+// bucket (no batch, or no reduced maps).
 void launch_generic_batches(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
-                            std::vector<char>& done);
+                            char* done);
+// Per-bucket flags for the batch launchers: on the stack for plans of at most
+// kStackBuckets buckets (no host allocation per cell pass), else on the heap.
+struct BucketFlags {
+  static constexpr size_t kStackBuckets = 64;
+  char stack[kStackBuckets] = {};
+  std::vector<char> heap;
+  char* data;
+  explicit BucketFlags(size_t n) : data(stack) {
+    if (n > kStackBuckets) {
+      heap.assign(n, 0);
+      data = heap.data();
+    }
+  }
+  char operator[](size_t i) const { return data[i]; }
+};
 void launch_cell_diagonal_generic(const gmf_plan& plan, const Bucket& b, double* diag, bool reduced,
                                   hipStream_t stream);
 // Exact reduced-diagonal contributions of the bucket's coupled cells (and the
@@ -260,7 +276,7 @@ void prepare_face_batch(gmf_plan& plan);
 // Face loads (items x points x sides x (1 + D)) of one face-tensor batch added into y.
 void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
 void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
-                       std::vector<char>& done);
+                       char* done);
 // slots: the workspace's brick halo array (plan.brick_slots entries).
 bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream);

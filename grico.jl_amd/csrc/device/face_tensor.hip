@@ -490,7 +490,7 @@ void prepare_face_batch(gmf_plan& p) {
 }
 
 void launch_face_batch(const gmf_plan& p, const double* x, double* y, bool reduced, hipStream_t stream,
-                       std::vector<char>& done) {
+                       char* done) {
   const GenericBatch& g = p.face_batch;
   const void* args = reduced ? g.args_red : g.args_full;
   if (g.count == 0 || args == nullptr) return;

@@ -183,9 +183,9 @@ __global__ void k_scale_dev(double* __restrict__ y, const double* __restrict__ x
 void cell_pass(gmf_workspace* ws, const double* x, double* y, bool reduced) {
   const gmf_plan& p = *ws->plan;
   // Generic-kernel buckets of uneven degree share one launch per tier.
-  std::vector<char> batched;
-  launch_generic_batches(p, x, y, reduced, ws->stream, batched);
-  launch_face_batch(p, x, y, reduced, ws->stream, batched);
+  BucketFlags batched(p.buckets.size());
+  launch_generic_batches(p, x, y, reduced, ws->stream, batched.data);
+  launch_face_batch(p, x, y, reduced, ws->stream, batched.data);
   for (size_t i = 0; i < p.buckets.size(); ++i) {
     const Bucket& b = p.buckets[i];
     if (b.cloud) {
