@@ -178,6 +178,18 @@ gmf_status gmf_apply_async(gmf_workspace* ws, const double* x, double* y);
 gmf_status gmf_apply_physical(gmf_workspace* ws, const double* x, double* y);
 /* y_red = P^T A P x_red (src/assembly.jl:905-915 _reduced_apply!). */
 gmf_status gmf_reduced_apply(gmf_workspace* ws, const double* x_red, double* y_red);
+
+/* y_red = P^T A P x_red like gmf_reduced_apply, allocating nothing. No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops). The workspace scratch
+ * comes from a fixed 32 MiB device array that the library holds per device
+ * until the process ends; plans that need more (brick halo slots, plus twice
+ * the full dof count when the reduced map is not a selection) or have more
+ * than 64 cell batches are rejected with GMF_EUNSUPPORTED. The plan's device
+ * must be current. Asynchronous on `stream` (NULL: the null stream); calls that
+ * overlap on one device share the array and race. */
+gmf_status gmf_reduced_apply_noalloc(const gmf_plan* plan, const double* x, double* y, void* stream);
+
 /* The same operator on FP32 vectors, the level operator of the FP32 V-cycle:
  * FP32 cell kernels where every bucket of the plan has them (3D tensor cells
  * with N = Q <= 8 and the diffusion form, selection reduced maps), otherwise
