@@ -779,7 +779,12 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
     DeviceBuffer<double> inv64(n);
     {
       DeviceBuffer<double> d(n);
-      reduced_diagonal(lv.ws, d.p);
+      const double* given = desc->level_diagonals ? desc->level_diagonals[l] : nullptr;
+      if (given != nullptr) {
+        if (n > 0) GMF_HIP(hipMemcpy(d.p, given, n * sizeof(double), hipMemcpyHostToDevice));
+      } else {
+        reduced_diagonal(lv.ws, d.p);
+      }
       invert_diagonal(lv.ws, d.p, inv64.p, lv.n);
     }
     if (l > 0 && cheb && lv.n > 0) {

@@ -4,7 +4,7 @@ The names follow the reference's public surface (src/Grico.jl exports):
 Domain, HpSpace, SpaceOptions, FullTensorBasis, TrunkBasis, UniformDegree,
 AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, ScalarField, AffineProblem,
 SPD, TestChannels, TraceTestChannels, add_cell_accumulator,
-add_boundary_accumulator, add_interface_accumulator, jump, average, add_constraint, Dirichlet,
+add_boundary_accumulator, add_interface_accumulator, jump, average, add_constraint, Dirichlet, MeanValue,
 BoundaryFace, LOWER, UPPER, compile, apply, solve, CGSolver, FGMRESSolver,
 JacobiPreconditioner, GeometricMultigridPreconditioner, l2_error,
 relative_l2_error, AdaptivityPlan, transition, transfer_state. Julia's `!`
@@ -14,7 +14,7 @@ mutators map to `*_into` functions (request_h_refinement! -> request_h_refinemen
 from .mesh import Domain, uniform_refine
 from .space import (AxisDegrees, ByLeafDegrees, DegreePlusQuadrature, FullTensorBasis, HpSpace, SpaceOptions,
                     StoredDegrees, TrunkBasis, UniformDegree, scalar_dof_count)
-from .problem import (LOWER, UPPER, AffineProblem, BoundaryFace, Dirichlet, NonsymmetricOperator, ScalarField, SPD,
+from .problem import (LOWER, UPPER, AffineProblem, BoundaryFace, Dirichlet, MeanValue, NonsymmetricOperator, ScalarField, SPD,
                       TestChannels, TraceTestChannels, TraceTrialChannels, TracePair, TrialChannels,
                       add_boundary_accumulator, add_cell_accumulator, add_constraint, add_interface_accumulator,
                       average, cell_size, component, gradient, inner, jump, minus, normal, normal_component, plus,

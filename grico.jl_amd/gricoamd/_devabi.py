@@ -62,7 +62,7 @@ class MgDesc(ctypes.Structure):
         ("chebyshev_degree", c_int32), ("chebyshev_range", c_double), ("eigen_iterations", c_int32),
         ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
         ("coarse_maxiter", c_int64), ("graph", c_int32), ("precision", c_int32),
-        ("smoother_restart", c_int32), ("spd", c_int32),
+        ("smoother_restart", c_int32), ("spd", c_int32), ("level_diagonals", ctypes.POINTER(P_f64)),
     ]
 
 
@@ -197,8 +197,20 @@ class DevicePlan:
         keep.append(fixed)
         pd.fixed_count = len(fixed)
         pd.fixed_dofs = _p(fixed, c_int64)
-        pd.affine_count = 0
-        pd.mean_count = 0
+        for name, rows in (("affine", getattr(plan, "affine_rows", [])), ("mean", getattr(plan, "mean_rows", []))):
+            setattr(pd, name + "_count", len(rows))
+            if not rows:
+                continue
+            piv = np.asarray([r[0] for r in rows], dtype=np.int64)
+            off = np.zeros(len(rows) + 1, dtype=np.int64)
+            off[1:] = np.cumsum([len(r[1]) for r in rows])
+            idx = np.ascontiguousarray(np.concatenate([r[1] for r in rows]).astype(np.int64))
+            cf = np.ascontiguousarray(np.concatenate([r[2] for r in rows]).astype(np.float64))
+            keep += [piv, off, idx, cf]
+            setattr(pd, name + "_pivots", _p(piv, c_int64))
+            setattr(pd, name + "_offsets", _p(off, c_int64))
+            setattr(pd, name + "_indices", _p(idx, c_int64))
+            setattr(pd, name + "_coefficients", _p(cf, c_double))
         pd.reduced_dof_count = plan.reduced_dof_count
         mo, mi, mc, sh = (_native.as_i64(plan.map_offsets), _native.as_i64(plan.map_indices),
                           _native.as_f64(plan.map_coefficients), _native.as_f64(plan.shift))

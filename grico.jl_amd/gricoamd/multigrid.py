@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _native
 from .plan import AssemblyPlan, compile as compile_plan
-from .problem import SPD, AffineProblem, Dirichlet, ScalarField
+from .problem import SPD, AffineProblem, Dirichlet, MeanValue, ScalarField
 from .space import HpSpace, SpaceOptions, StoredDegrees
 
 _DEFAULT_COARSE_DIRECT_DOF_LIMIT = 512
@@ -214,6 +214,8 @@ def problem_on_space(problem, space):
         out.add_interface_accumulator(field, op)
     for con in problem.constraints:
         out.add_constraint(Dirichlet(field, con.boundary, con.data))
+    for con in getattr(problem, "mean_constraints", []):
+        out.add_constraint(MeanValue(field, con.target))
     return out
 
 
@@ -433,6 +435,16 @@ class MultigridHierarchy:
             d.precision = 32 if pol.precision == "fp32" else 64
             d.smoother_restart = pol.smoother_restart
             d.spd = int(isinstance(self.problem.operator_class, SPD))
+            # Levels whose reduced map is not a selection (affine Dirichlet
+            # relations, mean rows) get their reduced diagonal from the host.
+            if not all(getattr(p, "selection_map", True) for p in self.plans):
+                from .solve import reduced_diagonal_general
+                diags = [None if getattr(p, "selection_map", True) else
+                         np.ascontiguousarray(reduced_diagonal_general(p), dtype=np.float64) for p in self.plans]
+                arr = (_native.P_f64 * len(diags))(*[_native.ptr(a, _native.c_double) if a is not None else
+                                                     _native.P_f64() for a in diags])
+                keep += [diags, arr]
+                d.level_diagonals = arr
             handle = ctypes.c_void_p()
             _devabi.check(lib.gmf_precond_create_mg(ctypes.byref(d), ctypes.byref(handle)))
             self._device = DevicePreconditioner(self.plans[-1], handle, keep=(keep, self))

@@ -130,6 +130,8 @@ def reduced_rhs(plan, as_numpy=True):
     """b_red = P^T (rhs - A_phys shift) on the device (src/assembly.jl:917-933)."""
     torch = _torch()
     dev = _ws(plan)
+    if plan.reduced_dof_count == 0:  # every dof constrained
+        return np.zeros(0) if as_numpy else torch.zeros(0, dtype=torch.float64, device="cuda")
     rhs = rhs_physical(plan, as_numpy=False)
     b = torch.empty(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
     _call("gmf_reduced_rhs", dev.ws, rhs.data_ptr(), b.data_ptr())
@@ -157,6 +159,9 @@ def reduced_apply_into(plan, y, x):
 def expand_reduced(plan, x, include_shift=True):
     dev = _ws(plan)
     xt = to_device(x, plan.reduced_dof_count)
+    if plan.reduced_dof_count == 0:  # no free dof: u = shift
+        y = to_device(plan.shift if include_shift else np.zeros(plan.full_dof_count))
+        return _out_like(x, y)
     y = _torch().empty(plan.full_dof_count, dtype=xt.dtype, device=xt.device)
     _call("gmf_expand_reduced", dev.ws, xt.data_ptr(), y.data_ptr(), int(bool(include_shift)))
     return _out_like(x, y)

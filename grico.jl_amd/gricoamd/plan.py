@@ -635,6 +635,7 @@ class AssemblyPlan:
             self.surface_buckets.append(sb)
         self._compile_faces(problem)
         self._compile_dirichlet()
+        self._compile_mean_constraints()
         self._compile_reduced_map()
         self._rhs = None
         self._device = None
@@ -701,19 +702,70 @@ class AssemblyPlan:
 
     # Dirichlet data (src/plans.jl:486-514, 884-972).
     def _compile_dirichlet(self):
+        """Fixed trace dofs and affine relations. Zero data on every selected
+        face fixes every active trace dof to zero (the trace systems built here
+        are full rank); other data go through the boundary L2 projection."""
         space = self.space
         cons = self.problem.constraints
-        fixed = {}
+        self.affine_rows = []  # (pivot, indices, coefficients, rhs): u_p - sum c_j u_j = rhs
+        fixed_dofs, fixed_values = np.zeros(0, dtype=np.int64), np.zeros(0)
         if cons:
             for con in cons:
                 sel = (space.boundary_axis == con.boundary.axis) & (space.boundary_side == con.boundary.side)
                 if not np.any(sel):
                     raise ValueError("Dirichlet constraint does not match any boundary face")
             dofs = self._dirichlet_trace_dofs(cons)
-            values = self._dirichlet_values(cons, dofs)
-            fixed = dict(zip(dofs.tolist(), values.tolist()))
-        self.fixed_dofs = np.asarray(sorted(fixed), dtype=np.int64)
-        self.fixed_values = np.asarray([fixed[d] for d in self.fixed_dofs.tolist()], dtype=np.float64)
+            if all((not callable(c.data)) and float(c.data) == 0.0 for c in cons):
+                fixed_dofs, fixed_values = dofs, np.zeros(len(dofs))
+            else:
+                fixed_dofs, fixed_values, self.affine_rows = _boundary_l2_projection(self, cons, dofs)
+        order = np.argsort(fixed_dofs, kind="stable")
+        self.fixed_dofs = np.asarray(fixed_dofs, dtype=np.int64)[order]
+        self.fixed_values = np.asarray(fixed_values, dtype=np.float64)[order]
+        self.affine_rows.sort(key=lambda row: row[0])
+
+    # Mean-value rows (src/plans.jl:750-805, 976-1003).
+    def _compile_mean_constraints(self):
+        """One explicit linear row per MeanValue: sum_j c_j u_j = target *
+        measure with c_j = integral of the dof's basis function over the
+        compiled cells; fixed dofs move to the right-hand side and the pivot is
+        the smallest free dof not already eliminated or used."""
+        self.mean_rows = []
+        means = getattr(self.problem, "mean_constraints", [])
+        if not means:
+            return
+        if self.cloud_buckets:
+            raise NotImplementedError("mean-value constraints on plans with cut-cell quadratures are not supported")
+        n = self.full_dof_count
+        coef = np.zeros(n)
+        touched = np.zeros(n, dtype=bool)
+        measure = 0.0
+        for bk in self.buckets:
+            phi = np.ones((1, 1))
+            for a in range(bk.dim - 1, -1, -1):  # mode and point boxes are axis-1 fastest
+                phi = np.kron(phi, bk.B[a])
+            ints = bk.weights @ phi.T  # cells x modes: integral of each mode over each cell
+            measure += float(bk.weights.sum())
+            np.add.at(coef, bk.term_dof, bk.term_coef * ints[bk.term_cell, bk.term_box])
+            touched[bk.term_dof] = True
+        fixed_value = dict(zip(self.fixed_dofs.tolist(), self.fixed_values.tolist()))
+        blocked = set(self.fixed_dofs.tolist()) | {row[0] for row in self.affine_rows}
+        used = set()
+        for con in means:
+            idx = np.nonzero(touched)[0]
+            is_fixed = np.isin(idx, self.fixed_dofs)
+            rhs = con.target * measure - float(sum(coef[j] * fixed_value[j] for j in idx[is_fixed].tolist()))
+            indices = idx[~is_fixed]
+            if len(indices) == 0:
+                raise ValueError("mean-value constraint leaves no free dofs")
+            free = [j for j in indices.tolist() if j not in blocked]
+            if not free:
+                raise ValueError("mean-value constraint leaves no free dofs")
+            pivot = min(free)
+            if pivot in used:
+                raise ValueError("mean-value pivots must be unique")
+            used.add(pivot)
+            self.mean_rows.append((pivot, indices.astype(np.int64), coef[indices].copy(), rhs))
 
     def _trace_terms(self, con):
         """Terms of every mode with nonzero trace on the constraint's boundary
@@ -746,26 +798,74 @@ class AssemblyPlan:
             dofs.append(dof[coef != 0.0])
         return np.unique(np.concatenate(dofs)).astype(np.int64) if dofs else np.zeros(0, dtype=np.int64)
 
-    def _dirichlet_values(self, cons, dofs):
-        if all((not callable(c.data)) and float(c.data) == 0.0 for c in cons):
-            return np.zeros(len(dofs))
-        return _boundary_l2_projection(self, cons, dofs)
-
-    # Reduced map (src/assembly.jl:112-154, 282-314) for fixed rows.
+    # Reduced map (src/assembly.jl:112-154, 156-314).
     def _compile_reduced_map(self):
+        """u_full = shift + P u_red as a CSR over full dofs. Fixed dofs carry
+        their value in the shift; affine-relation and mean-row pivots are
+        expanded into the free dofs (reduced) through their rows, the mean rows
+        solved together (dense LU) after the Dirichlet relations are
+        substituted; the other dofs are the reduced unknowns, in order."""
         n = self.full_dof_count
-        constrained = np.zeros(n, dtype=bool)
-        constrained[self.fixed_dofs] = True
+        fixed = np.zeros(n, dtype=bool)
+        fixed[self.fixed_dofs] = True
+        eliminated = np.zeros(n, dtype=bool)
+        for row in self.affine_rows:
+            eliminated[row[0]] = True
+        mean_pivot = np.zeros(n, dtype=bool)
+        for row in self.mean_rows:
+            mean_pivot[row[0]] = True
+        constrained = fixed | eliminated | mean_pivot
         self.solve_dofs = np.nonzero(~constrained)[0].astype(np.int64)
         self.reduced_index = np.full(n, -1, dtype=np.int64)
         self.reduced_index[self.solve_dofs] = np.arange(len(self.solve_dofs))
         self.reduced_dof_count = len(self.solve_dofs)
-        self.map_offsets = np.zeros(n + 1, dtype=np.int64)
-        self.map_offsets[1:] = np.cumsum(~constrained)
-        self.map_indices = np.arange(self.reduced_dof_count, dtype=np.int64)
-        self.map_coefficients = np.ones(self.reduced_dof_count)
         self.shift = np.zeros(n)
         self.shift[self.fixed_dofs] = self.fixed_values
+        counts = (~constrained).astype(np.int64)
+        pivots = np.nonzero(eliminated | mean_pivot)[0]
+        if len(pivots) == 0:
+            self.map_offsets = np.zeros(n + 1, dtype=np.int64)
+            self.map_offsets[1:] = np.cumsum(counts)
+            self.map_indices = np.arange(self.reduced_dof_count, dtype=np.int64)
+            self.map_coefficients = np.ones(self.reduced_dof_count)
+            self.selection_map = True
+            return
+        expansions = _reduced_expansions(self, fixed)
+        rows = {}
+        for d in pivots.tolist():
+            shift, red = expansions[d]
+            self.shift[d] = shift
+            keys = sorted(k for k, v in red.items() if v != 0.0)
+            rows[d] = (np.asarray(keys, dtype=np.int64), np.asarray([red[k] for k in keys]))
+            counts[d] = len(keys)
+        self.map_offsets = np.zeros(n + 1, dtype=np.int64)
+        self.map_offsets[1:] = np.cumsum(counts)
+        self.map_indices = np.zeros(int(self.map_offsets[-1]), dtype=np.int64)
+        self.map_coefficients = np.zeros(int(self.map_offsets[-1]))
+        start = self.map_offsets[self.solve_dofs]
+        self.map_indices[start] = np.arange(self.reduced_dof_count)
+        self.map_coefficients[start] = 1.0
+        for d, (keys, vals) in rows.items():
+            lo = int(self.map_offsets[d])
+            self.map_indices[lo:lo + len(keys)] = keys
+            self.map_coefficients[lo:lo + len(keys)] = vals
+        self.selection_map = False
+
+    def expand_host(self, x_red, include_shift=True):
+        """shift + P x_red on the host (src/assembly.jl:851-874)."""
+        x_red = np.asarray(x_red, dtype=np.float64)
+        rows = np.repeat(np.arange(self.full_dof_count), np.diff(self.map_offsets))
+        out = self.shift.copy() if include_shift else np.zeros(self.full_dof_count)
+        np.add.at(out, rows, self.map_coefficients * x_red[self.map_indices])
+        return out
+
+    def project_host(self, y_full):
+        """P^T y on the host (src/assembly.jl:876-891)."""
+        y_full = np.asarray(y_full, dtype=np.float64)
+        rows = np.repeat(np.arange(self.full_dof_count), np.diff(self.map_offsets))
+        out = np.zeros(self.reduced_dof_count)
+        np.add.at(out, self.map_indices, self.map_coefficients * y_full[rows])
+        return out
 
     # Host-side right-hand side (src/assembly.jl:488-507 rhs!).
     @property
@@ -909,8 +1009,12 @@ class AssemblyPlan:
         return not (self.cloud_buckets or self.surface_buckets or self.boundary_buckets or self.interface_buckets)
 
     def rhs(self):
+        """Full-layout right-hand side with the constraint rows' values
+        (fixed values, relation and mean right-hand sides)."""
         b = self.rhs_physical().copy()
         b[self.fixed_dofs] = self.fixed_values
+        for row in self.affine_rows + self.mean_rows:
+            b[row[0]] = row[3]
         return b
 
     def reduced_rhs(self):
@@ -919,7 +1023,7 @@ class AssemblyPlan:
         if np.any(self.shift != 0):
             from . import ops
             b -= ops.apply_physical(self, self.shift)
-        return b[self.solve_dofs]
+        return b[self.solve_dofs] if self.selection_map else self.project_host(b)
 
     # Device plan.
     def device(self):
@@ -929,11 +1033,12 @@ class AssemblyPlan:
 
 
 def _boundary_l2_projection(plan, cons, dofs):
-    """Fixed trace values: L2 projection of g onto the active trace space
-    (src/plans.jl:884-972). Per boundary face the basis values are tensor
-    products of per-axis tables at the face's Gauss points, the face's Gram
-    block is Phi W Phi^T; the global Gram matrix is sparse (face blocks) and
-    solved with SciPy's sparse LU."""
+    """L2 projection of g onto the active trace space (src/plans.jl:884-972):
+    (fixed dofs, their values, affine relations). Per boundary face the basis
+    values are tensor products of per-axis tables at the face's Gauss points
+    and the face's Gram block is Phi W Phi^T; the global Gram matrix is sparse
+    (face blocks). Small systems take the reference's dense spectral route
+    (_dirichlet_relations), larger ones SciPy's sparse LU (full rank only)."""
     import scipy.sparse as sps
     import scipy.sparse.linalg as spla
 
@@ -1001,8 +1106,10 @@ def _boundary_l2_projection(plan, cons, dofs):
             vals.append(gl.reshape(-1))
             np.add.at(r, ii, rl)
     if n == 0:
-        return np.zeros(0)
+        return np.zeros(0, dtype=np.int64), np.zeros(0), []
     G = sps.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    if n <= DENSE_DIRICHLET_LIMIT:
+        return _dirichlet_relations(dofs, G.toarray(), r)
     import warnings
     with warnings.catch_warnings():
         warnings.simplefilter("error", spla.MatrixRankWarning)
@@ -1011,14 +1118,158 @@ def _boundary_l2_projection(plan, cons, dofs):
         except spla.MatrixRankWarning:
             sol = None
     # A singular trace Gram matrix means trace dofs that the selected faces do
-    # not control independently; the reference turns those into affine
-    # relations (src/plans.jl:884-943), which this sparse path does not build.
+    # not control independently; above the dense limit the relations the
+    # reference builds for them (src/plans.jl:884-943) are not computed.
     resid = np.inf if sol is None or not np.all(np.isfinite(sol)) else \
         np.linalg.norm(G @ sol - r) / max(np.linalg.norm(r), np.finfo(np.float64).tiny)
     if not resid <= 1e-8:
-        raise ValueError("rank-deficient Dirichlet trace system: affine Dirichlet relations "
-                         "(src/plans.jl:884-943) are not supported by this plan compiler")
-    return sol
+        raise ValueError(f"rank-deficient Dirichlet trace system with {n} dofs: affine Dirichlet relations are "
+                         f"built for at most {DENSE_DIRICHLET_LIMIT} trace dofs")
+    return np.asarray(dofs, dtype=np.int64), sol, []
+
+
+# Trace systems up to this size take the reference's dense spectral route
+# (eigen decomposition, rank cut, column-pivoted QR); larger ones are solved
+# sparsely and must be full rank.
+DENSE_DIRICHLET_LIMIT = 2000
+
+
+def _dirichlet_relations(dofs, G, r):
+    """Fixed dofs and affine relations from the trace Gram system G a = r
+    (src/plans.jl:884-972). Positive eigenmodes above 1000 eps scale n define
+    the independent constraints; the data must be consistent on the others.
+    Column-pivoted QR of the constraint matrix picks one pivot dof per
+    constraint; the pivots become fixed dofs when no free trace dof couples to
+    them above the rank tolerance, else relations u_p - sum c_j u_j = value."""
+    import scipy.linalg as sla
+    dofs = np.asarray(dofs, dtype=np.int64)
+    n = len(dofs)
+    lam, V = np.linalg.eigh(0.5 * (G + G.T))
+    order = np.argsort(lam)[::-1]
+    lam, V = lam[order], V[:, order]
+    scale = float(np.max(np.abs(lam))) if n else 0.0
+    tol = 0.0 if scale == 0.0 else 1000.0 * np.finfo(np.float64).eps * scale * max(n, 1)
+    k = int(np.count_nonzero(lam > tol))
+    tr = V.T @ r
+    if np.any(np.abs(tr[k:]) > tol * max(np.linalg.norm(r), 1.0)):
+        raise ValueError("Dirichlet data is inconsistent on the selected boundary faces")
+    if k == 0:
+        return np.zeros(0, dtype=np.int64), np.zeros(0), []
+    C = V[:, :k].T
+    targets = tr[:k] / lam[:k]
+    _, _, perm = sla.qr(C, mode="economic", pivoting=True)
+    pivot_cols = perm[:k]
+    is_pivot = np.zeros(n, dtype=bool)
+    is_pivot[pivot_cols] = True
+    free_cols = np.nonzero(~is_pivot)[0]
+    Pm = C[:, pivot_cols]
+    values = np.linalg.solve(Pm, targets)
+    if len(free_cols) == 0:
+        return dofs[pivot_cols], values, []
+    coupling = -np.linalg.solve(Pm, C[:, free_cols])
+    fixed_d, fixed_v, rows = [], [], []
+    for i in range(k):
+        keep = np.abs(coupling[i]) > tol
+        if not np.any(keep):
+            fixed_d.append(int(dofs[pivot_cols[i]]))
+            fixed_v.append(float(values[i]))
+        else:
+            rows.append((int(dofs[pivot_cols[i]]), dofs[free_cols[keep]].astype(np.int64),
+                         coupling[i][keep].copy(), float(values[i])))
+    return np.asarray(fixed_d, dtype=np.int64), np.asarray(fixed_v), rows
+
+
+def _reduced_expansions(plan, fixed):
+    """(shift, {reduced index: coefficient}) of every relation and mean pivot
+    (src/assembly.jl:156-270): relation pivots expand through their rows
+    (recursively, cycles rejected); the mean rows, with relation pivots
+    substituted, form a small dense system for the mean pivots, solved once for
+    the shift and once per reduced column that appears."""
+    red_index = plan.reduced_index
+    fixed_value = np.zeros(plan.full_dof_count)
+    fixed_value[plan.fixed_dofs] = plan.fixed_values
+    rows = {row[0]: row for row in plan.affine_rows}
+    mean_of = {row[0]: i for i, row in enumerate(plan.mean_rows)}
+    base_cache, visiting = {}, set()
+
+    def base(d):
+        """(shift, reduced {}, mean {}) with mean pivots kept symbolic."""
+        if d in base_cache:
+            return base_cache[d]
+        if d in visiting:
+            raise ValueError("cyclic Dirichlet constraint relation detected")
+        if fixed[d]:
+            out = (fixed_value[d], {}, {})
+        elif red_index[d] >= 0:
+            out = (0.0, {int(red_index[d]): 1.0}, {})
+        elif d in mean_of:
+            out = (0.0, {}, {mean_of[d]: 1.0})
+        elif d in rows:
+            visiting.add(d)
+            _, idx, cf, rhs = rows[d]
+            shift, red, mean = rhs, {}, {}
+            for j, c in zip(idx.tolist(), cf.tolist()):
+                s2, r2, m2 = base(j)
+                shift += c * s2
+                for key, v in r2.items():
+                    red[key] = red.get(key, 0.0) + c * v
+                for key, v in m2.items():
+                    mean[key] = mean.get(key, 0.0) + c * v
+            visiting.discard(d)
+            out = (shift, red, mean)
+        else:
+            raise ValueError(f"dof {d} has no reduced representation")
+        base_cache[d] = out
+        return out
+
+    m = len(plan.mean_rows)
+    mean_shift = np.zeros(m)
+    mean_red = [{} for _ in range(m)]
+    if m:
+        A = np.zeros((m, m))
+        rhs = np.zeros(m)
+        cols = {}
+        for i, (_, idx, cf, target) in enumerate(plan.mean_rows):
+            rhs[i] = target
+            for j, c in zip(idx.tolist(), cf.tolist()):
+                s2, r2, m2 = base(j)
+                rhs[i] -= c * s2
+                for key, v in m2.items():
+                    A[i, key] += c * v
+                for key, v in r2.items():
+                    cols.setdefault(key, np.zeros(m))[i] -= c * v
+        import scipy.linalg as sla
+        lu = sla.lu_factor(A)
+        mean_shift = sla.lu_solve(lu, rhs)
+        for key, col in cols.items():
+            sol = sla.lu_solve(lu, col)
+            for i in range(m):
+                mean_red[i][key] = mean_red[i].get(key, 0.0) + sol[i]
+    final_cache = {}
+
+    def final(d):
+        if d in final_cache:
+            return final_cache[d]
+        if fixed[d]:
+            out = (fixed_value[d], {})
+        elif red_index[d] >= 0:
+            out = (0.0, {int(red_index[d]): 1.0})
+        elif d in mean_of:
+            i = mean_of[d]
+            out = (float(mean_shift[i]), dict(mean_red[i]))
+        else:
+            _, idx, cf, rhs = rows[d]
+            shift, red = rhs, {}
+            for j, c in zip(idx.tolist(), cf.tolist()):
+                s2, r2 = final(j)
+                shift += c * s2
+                for key, v in r2.items():
+                    red[key] = red.get(key, 0.0) + c * v
+            out = (shift, red)
+        final_cache[d] = out
+        return out
+
+    return {d: final(d) for d in list(rows) + list(mean_of)}
 
 
 def compile(problem):

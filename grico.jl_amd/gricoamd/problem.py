@@ -193,6 +193,20 @@ class Dirichlet:
         self.data = data
 
 
+class MeanValue:
+    """Integral-mean constraint of a field (src/problem.jl:190-195): the
+    integral of u over the compiled cells equals `target` times their measure.
+    Compiled as one explicit linear row on a free pivot dof
+    (src/plans.jl:750-805)."""
+
+    def __init__(self, field, target):
+        t = np.asarray(target, dtype=np.float64)
+        if t.size != 1 or not np.isfinite(t).all():
+            raise ValueError("mean-value target of a scalar field must be one finite number")
+        self.field = field
+        self.target = float(t.reshape(-1)[0])
+
+
 class PointQuadrature:
     """Explicit quadrature on the biunit reference cell [-1, 1]^D (src/quadrature.jl
     PointQuadrature): points (n x D) and weights (n)."""
@@ -237,6 +251,7 @@ class AffineProblem:
         self.cell_operators = []   # (test_field, trial_field, operator)
         self.cell_rhs = []         # (test_field, operator)
         self.constraints = []
+        self.mean_constraints = []
         self.cell_quadratures = {}     # leaf -> PointQuadrature
         self.surface_quadratures = []  # SurfaceQuadrature
         self.surface_operators = []    # (test_field, trial_field, operator)
@@ -324,8 +339,13 @@ class AffineProblem:
         return self
 
     def add_constraint(self, constraint):
+        if isinstance(constraint, MeanValue):
+            if constraint.field is not self.field:
+                raise ValueError("this backend supports a single scalar field")
+            self.mean_constraints.append(constraint)
+            return self
         if not isinstance(constraint, Dirichlet):
-            raise NotImplementedError("only Dirichlet constraints are supported")
+            raise NotImplementedError("only Dirichlet and MeanValue constraints are supported")
         self.constraints.append(constraint)
         return self
 

@@ -82,10 +82,56 @@ class DevicePreconditioner:
         return z
 
 
+def reduced_diagonal_general(plan):
+    """Diagonal of P^T A P when P is not a selection (src/assembly.jl:662-675
+    gating; src/solve.jl:722-738 probing). P = S + sum_p e_p w_p^T with S the
+    selection of the free dofs and w_p the expansion of relation / mean pivot p,
+    so for a symmetric A
+    diag_j = A[s_j, s_j] + 2 sum_p w_p[j] A[p, s_j] + sum_{p,q} w_p[j] A[p, q] w_q[j],
+    which needs the physical diagonal and one physical apply per pivot (the
+    local physical diagonal is exact on single-term tensor-cell maps). Other
+    plans are probed with one reduced apply per reduced dof, as the reference
+    does."""
+    from .problem import SPD
+    torch = ops._torch()
+    n = plan.reduced_dof_count
+    local_exact = all(len(b.multi_offsets) <= 1 for b in plan.buckets) and not (
+        plan.cloud_buckets or plan.surface_buckets or plan.boundary_buckets or plan.interface_buckets)
+    if not (isinstance(plan.problem.operator_class, SPD) and local_exact):
+        d = np.zeros(n)
+        e = torch.zeros(n, dtype=torch.float64, device="cuda")
+        for j in range(n):
+            e.zero_()
+            e[j] = 1.0
+            d[j] = float(ops.reduced_apply(plan, e)[j])
+        return d
+    rows_of = np.diff(plan.map_offsets)
+    sel = plan.solve_dofs
+    pivots = np.nonzero((rows_of > 0) & (plan.reduced_index < 0))[0]
+    d = ops.diagonal_physical(plan)[sel].copy()
+    if len(pivots) == 0:
+        return d
+    W = np.zeros((len(pivots), n))
+    cols = np.zeros((len(pivots), plan.full_dof_count))
+    for i, p in enumerate(pivots.tolist()):
+        lo, hi = plan.map_offsets[p], plan.map_offsets[p + 1]
+        W[i, plan.map_indices[lo:hi]] = plan.map_coefficients[lo:hi]
+        e = np.zeros(plan.full_dof_count)
+        e[p] = 1.0
+        cols[i] = ops.apply_physical(plan, e)
+    App = cols[:, pivots]  # A[pivot_i, pivot_k] = (A e_k)[pivot_i]
+    d += 2.0 * np.einsum("pj,pj->j", W, cols[:, sel])
+    d += np.einsum("pj,pq,qj->j", W, App.T, W)
+    return d
+
+
 def jacobi_inverse_diagonal(plan):
     """Inverse reduced diagonal with the reference's zero / non-finite check."""
     torch = ops._torch()
-    d = ops.reduced_diagonal(plan, as_numpy=False)
+    if getattr(plan, "selection_map", True):
+        d = ops.reduced_diagonal(plan, as_numpy=False)
+    else:
+        d = ops.to_device(reduced_diagonal_general(plan))
     inv = torch.empty_like(d)
     dev = ops._ws(plan)
     _devabi.check(_native.device().gmf_invert_diagonal(dev.ws, d.data_ptr(), inv.data_ptr(), d.numel()))
@@ -132,6 +178,10 @@ def solve_reduced(plan, solver, b_red, relative_tolerance=None, absolute_toleran
     b = ops.to_device(b_red, n)
     x = torch.zeros(n, dtype=torch.float64, device="cuda") if initial_solution is None \
         else ops.to_device(initial_solution, n).clone()
+    if n == 0:  # every dof constrained: the solution is the map's shift
+        info = _devabi.SolveInfo()
+        info.converged = 1
+        return x, info
     M = compile_preconditioner(solver.preconditioner, plan)
     dev = ops._ws(plan)
     info = _devabi.SolveInfo()

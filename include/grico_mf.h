@@ -343,6 +343,12 @@ typedef struct gmf_mg_desc {
    * (LU dense coarse inverse, FGMRES coarse solve with restart min(30, n))
    * (src/multigrid.jl:382-408, 891-924). */
   int32_t spd;
+  /* Optional HOST arrays, one per level (NULL, or NULL entries: computed on
+   * the device): the reduced diagonal of each level's operator. Levels whose
+   * reduced map is not a selection (affine Dirichlet relations, mean rows)
+   * need it, since the device's exact diagonal covers selection maps only;
+   * the reference probes such diagonals (src/solve.jl:722-738). */
+  const double* const* level_diagonals;
 } gmf_mg_desc;
 
 /* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR

@@ -111,9 +111,49 @@ class _Bucket:
 
 
 def _fixed_mask(plan):
+    """Rows the full apply writes from the constraints instead of the
+    operator: fixed dofs, affine-relation pivots and mean-row pivots
+    (src/plans.jl:457-480 masks, src/assembly.jl:1788-1803 scatter skip)."""
     m = np.zeros(plan.full_dof_count, dtype=np.uint8)
     m[plan.fixed_dofs] = 1
+    for row in getattr(plan, "affine_rows", []) + getattr(plan, "mean_rows", []):
+        m[row[0]] = 1
     return m
+
+
+def _constraint_rows(plan, x, y):
+    """src/assembly.jl:1893-1920: fixed rows copy x; relation rows
+    y_p = x_p - sum c_j x_j; mean rows y_p = sum c_j x_j."""
+    for (p, idx, cf, _) in getattr(plan, "affine_rows", []):
+        v = x[p]
+        for j, c in zip(idx.tolist(), cf.tolist()):
+            v -= c * x[j]
+        y[p] = v
+    for (p, idx, cf, _) in getattr(plan, "mean_rows", []):
+        v = 0.0
+        for j, c in zip(idx.tolist(), cf.tolist()):
+            v += c * x[j]
+        y[p] = v
+
+
+def expand(plan, x_red, shift=True):
+    """shift + P x_red from the plan's CSR map (src/assembly.jl:851-874)."""
+    out = np.array(plan.shift, dtype=np.float64) if shift else np.zeros(plan.full_dof_count)
+    off, idx, cf = plan.map_offsets, plan.map_indices, plan.map_coefficients
+    for f in np.nonzero(np.diff(off))[0].tolist():
+        for t in range(off[f], off[f + 1]):
+            out[f] += cf[t] * x_red[idx[t]]
+    return out
+
+
+def project(plan, y):
+    """P^T y from the plan's CSR map (src/assembly.jl:876-891)."""
+    out = np.zeros(plan.reduced_dof_count)
+    off, idx, cf = plan.map_offsets, plan.map_indices, plan.map_coefficients
+    for f in np.nonzero(np.diff(off))[0].tolist():
+        for t in range(off[f], off[f + 1]):
+            out[idx[t]] += cf[t] * y[f]
+    return out
 
 
 def _trial(value, gradient):
@@ -303,6 +343,7 @@ def apply(plan, x, physical=False):
     if not physical:
         fd = np.ascontiguousarray(plan.fixed_dofs, dtype=np.int64)
         L.oracle_fixed_rows(_p(fd, ctypes.c_int64), len(fd), _p(x, ctypes.c_double), _p(y, ctypes.c_double))
+        _constraint_rows(plan, x, y)
     return y
 
 
@@ -478,10 +519,12 @@ def newton_dense(rplan, u0=None, rtol=1e-10, atol=1e-10, maxiter=20):
 
 
 def reduced_apply(plan, x_red):
-    """P^T A P x with the selection map of fixed-row plans."""
-    full = np.zeros(plan.full_dof_count)
-    full[plan.solve_dofs] = x_red
-    return apply(plan, full, physical=True)[plan.solve_dofs]
+    """P^T A P x (src/assembly.jl:905-915); selection maps by indexing."""
+    if getattr(plan, "selection_map", True):
+        full = np.zeros(plan.full_dof_count)
+        full[plan.solve_dofs] = x_red
+        return apply(plan, full, physical=True)[plan.solve_dofs]
+    return project(plan, apply(plan, expand(plan, x_red, shift=False), physical=True))
 
 
 def cg(plan, b_red, inv_diag=None, rtol=1e-10, atol=0.0, maxiter=10000, lowered=True, threads=0):
@@ -494,6 +537,9 @@ def cg(plan, b_red, inv_diag=None, rtol=1e-10, atol=0.0, maxiter=10000, lowered=
     def op(_ctx, xp, yp):
         xr = np.ctypeslib.as_array(xp, shape=(n,))
         yr = np.ctypeslib.as_array(yp, shape=(n,))
+        if not getattr(plan, "selection_map", True):
+            yr[:] = reduced_apply(plan, xr)
+            return
         full = np.zeros(plan.full_dof_count)
         full[plan.solve_dofs] = xr
         if lowered:

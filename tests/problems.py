@@ -50,6 +50,39 @@ class SineLoad:
         return out
 
 
+class CosLoad:
+    """f = D pi^2 prod cos(pi x_a): with zero normal flux and zero mean the
+    solution is prod cos(pi x_a)."""
+
+    def __init__(self, dim):
+        self.dim = dim
+
+    def cell_rhs_accumulate(self, q, test_component):
+        out = self.dim * np.pi ** 2
+        for x in q.point:
+            out = out * np.cos(np.pi * x)
+        return out
+
+
+def cos_exact(p):
+    out = 1.0
+    for x in p:
+        out = out * np.cos(np.pi * x)
+    return out
+
+
+def mean_golden_problem():
+    """One DG p = 0 cell on [0, 1] with an integral-mean constraint 2
+    (test/matrix_free.jl:1047-1053)."""
+    domain = g.Domain((0.0,), (1.0,), (1,))
+    space = g.HpSpace(domain, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(0), continuity="dg"))
+    u = g.ScalarField(space, "u")
+    problem = g.AffineProblem(u, g.SPD())
+    problem.add_cell_accumulator(u, u, Mass())
+    problem.add_constraint(g.MeanValue(u, 2.0))
+    return problem
+
+
 def sine_exact(p):
     out = 1.0
     for x in p:
