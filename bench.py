@@ -1127,8 +1127,10 @@ def transfer_leg(roots=16, degree=2):
     the corner octant, split once along x in the source, is merged back onto
     composite rules, the opposite octant is split along x and the middle
     layer is raised to p=3): host lowering of the L2 projection, device
-    Jacobi-CG of the projection to 1e-12, and the oracle Jacobi-CG of the
-    same system timed beside it."""
+    Jacobi-CG of the global projection to 1e-12 (transfer_state with a
+    linear_solve), the oracle Jacobi-CG of the same system timed beside it,
+    and the default local-projection transfer (per-cell solves reconciled on
+    shared dofs, host) with its distance from the global projection."""
     import numpy as np
 
     import gricoamd as g
@@ -1178,6 +1180,10 @@ def transfer_leg(roots=16, degree=2):
     _, it_or = pyoracle.cg(pplan, b, inv_diag=inv, rtol=1e-12, maxiter=oracle_iters, lowered=False)
     t_or = time.perf_counter() - t0
     it_or = oracle_iters if it_or < 0 else it_or  # -1: stopped at maxiter (not converged in the sample)
+    t0 = time.perf_counter()
+    local = ad.reconcile(pplan, ad.local_projections(pplan, src))  # the default (no linear_solve) transfer
+    t_local = time.perf_counter() - t0
+    glob = state.coefficients  # full layout on the target space
     xr = np.random.default_rng(3).standard_normal(pplan.full_dof_count)
     y_dev = g.apply_physical(pplan, xr)
     y_or = pyoracle.apply(pplan, xr, physical=True)
@@ -1188,6 +1194,9 @@ def transfer_leg(roots=16, degree=2):
            "device_solve_seconds": t_dev, "device_iterations": state.info.iterations,
            "device_seconds_per_iteration": t_dev / max(state.info.iterations, 1),
            "oracle_cg_seconds_per_iteration": t_or / max(it_or, 1), "oracle_sample_iterations": it_or,
+           "local_projection_host_seconds": t_local,
+           "local_vs_global_projection_max_difference_scaled": float(np.abs(local - glob).max() /
+                                                                     np.abs(glob).max()),
            "projection_apply_max_difference_vs_oracle_scaled": float(np.abs(y_dev - y_or).max() /
                                                                      np.abs(y_or).max())}
     log(f"transfer: {out}")

@@ -11,15 +11,20 @@ source/target overlaps).
 
 Here the target grid is a copy of the source grid with the requested splits
 applied; a merged leaf keeps its children in the grid and is listed as a leaf
-of the target space. The
-projection runs as a device solve: the target mass operator (value channel
-only, no constraints) is compiled like any other cell accumulator, the load
+of the target space. The target mass operator (value channel only, no
+constraints) is compiled like any other cell accumulator and the load
 u_old(x_q) is evaluated on the host at the target quadrature points (located
-in the source grid by descending its refinement tree) and the mass system is
-solved by Jacobi-CG through the device library. The target quadrature takes
-max(p_source, p_target) + 1 Gauss points per axis, which integrates the mass
-matrix and the load exactly, so a target space that contains the source space
-reproduces the old field up to the solve tolerance.
+in the source grid by descending its refinement tree). The target quadrature
+takes max(p_source, p_target) + 1 Gauss points per axis, which integrates the
+mass matrix and the load exactly, so a target space that contains the source
+space reproduces the old field up to the solve tolerance.
+
+transfer_state without a linear_solve policy follows the reference's default
+(src/transfer.jl:431-437, 637-804): one local L2 projection per target cell
+over the leaf's basis modes, reconciled on shared dofs through the normal
+equations of the local relations (host). With a linear_solve policy the
+global L2 projection is solved on the device by that solver (the reference's
+variational path).
 
 A merged (h-derefined) target leaf covers several source leaves; its cell
 takes a composite rule, one Gauss rule per covered source leaf, so the
@@ -412,16 +417,145 @@ def composite_rule(target_space, source_space, leaf, covered):
     return PointQuadrature(np.concatenate(points), np.concatenate(weights))
 
 
-def transfer_state(transition, state, relative_tolerance=1e-13, maxiter=None):
-    """L2 projection of `state` onto the transition's target space, solved on
-    the device by Jacobi-CG (src/transfer.jl:892-1000). Returns a State of the
-    projection plan (full layout on the target space)."""
+def _regularized_cholesky_solve(M, r):
+    """Batched dense solves (items x n x n) with the reference's relative
+    diagonal shift 1000 eps max|M_ii| (src/transfer.jl:497-525); items whose
+    matrix is zero get zero."""
+    scale = np.abs(np.diagonal(M, axis1=1, axis2=2)).max(axis=1)
+    out = np.zeros_like(r)
+    live = scale > 0.0
+    if not np.any(live):
+        return out
+    n = M.shape[1]
+    A = M[live] + (1000.0 * np.finfo(np.float64).eps * scale[live])[:, None, None] * np.eye(n)[None]
+    L = np.linalg.cholesky(A)
+    y = np.linalg.solve(L, r[live][..., None])
+    out[live] = np.linalg.solve(np.transpose(L, (0, 2, 1)), y)[..., 0]
+    return out
+
+
+def _cloud_basis(bucket):
+    """psi values of the bucket's full mode box at every point (points x modes,
+    axis-1-fastest boxes)."""
+    D = bucket.dim
+    phi = np.ones((bucket.n_points_total, 1))
+    for a in range(D - 1, -1, -1):
+        va = integrated_legendre(bucket.ref[:, a], bucket.degrees[a])
+        phi = (phi[:, :, None] * va[:, None, :]).reshape(bucket.n_points_total, -1)
+    return phi
+
+
+def _active_modes(space, bucket):
+    """cells x modes mask of the leaves' basis modes."""
+    lo = space.mode_offsets[bucket.leaf_idx]
+    counts = space.mode_offsets[bucket.leaf_idx + 1] - lo
+    from .plan import _expand_ranges
+    modes = _expand_ranges(lo, counts)
+    mask = np.zeros((bucket.cell_count, bucket.n_modes), dtype=bool)
+    mask[np.repeat(np.arange(bucket.cell_count), counts), space.mode_box[modes].astype(np.int64)] = True
+    return mask
+
+
+def local_projections(plan, state):
+    """Per batch of the projection plan, the local L2 projection of the old
+    field on each target cell (cells x modes; src/transfer.jl:778-798): the
+    cell's mass matrix and the old field's load on the cell's own rule (the
+    composite per-source-leaf rule on merged cells), solved per cell."""
+    out = []
+    for bk in list(plan.buckets) + list(plan.cloud_buckets):
+        if getattr(bk, "cloud", False):
+            phi = _cloud_basis(bk)
+            v = evaluate(state, tuple(bk.phys_points))
+            wphi = bk.weights[:, None] * phi
+            M = np.zeros((bk.cell_count, bk.n_modes, bk.n_modes))
+            np.add.at(M, bk.item_of_point, wphi[:, :, None] * phi[:, None, :])
+            rhs = np.zeros((bk.cell_count, bk.n_modes))
+            np.add.at(rhs, bk.item_of_point, wphi * v[:, None])
+        else:
+            phi = np.ones((1, 1))
+            for a in range(bk.dim - 1, -1, -1):
+                phi = np.kron(phi, bk.B[a])
+            v = evaluate(state, tuple(p.reshape(-1) for p in bk.phys_points)).reshape(bk.weights.shape)
+            M = np.einsum("iq,cq,jq->cij", phi, bk.weights, phi)
+            rhs = (bk.weights * v) @ phi.T
+        active = _active_modes(plan.space, bk)
+        if not np.all(active):  # modes outside the basis: identity rows, zero load
+            idx = np.arange(bk.n_modes)
+            M = np.where(active[:, :, None] & active[:, None, :], M, 0.0)
+            M[:, idx, idx] = np.where(active, M[:, idx, idx], 1.0)
+            rhs = np.where(active, rhs, 0.0)
+        out.append((bk, _regularized_cholesky_solve(M, rhs)))
+    return out
+
+
+def reconcile(plan, parts):
+    """Target coefficients from the local projections (src/transfer.jl:637-776):
+    the normal equations of the local relations sum_t c_t u_(g_t) = s_(cell,
+    mode) over every (cell, mode) and its terms; dofs coupled by multi-term
+    relations form connected components solved densely (regularized Cholesky),
+    the others are rhs / diagonal (zero where the diagonal is below 1000 eps)."""
+    import scipy.sparse as sps
+    from scipy.sparse.csgraph import connected_components
+    n = plan.full_dof_count
+    diag = np.zeros(n)
+    rhs = np.zeros(n)
+    rows, cols, vals = [], [], []
+    for bk, s in parts:
+        diag_add = bk.term_coef * bk.term_coef
+        np.add.at(diag, bk.term_dof, diag_add)
+        np.add.at(rhs, bk.term_dof, bk.term_coef * s[bk.term_cell, bk.term_box])
+        key = bk.term_cell * bk.n_modes + bk.term_box
+        order = np.argsort(key, kind="stable")
+        k_sorted = key[order]
+        starts = np.nonzero(np.r_[True, k_sorted[1:] != k_sorted[:-1]])[0]
+        counts = np.diff(np.r_[starts, len(k_sorted)])
+        for st, ct in zip(starts[counts > 1].tolist(), counts[counts > 1].tolist()):
+            terms = order[st:st + ct]
+            for i in range(ct):
+                for j in range(i + 1, ct):
+                    a, b = int(bk.term_dof[terms[i]]), int(bk.term_dof[terms[j]])
+                    if a == b:
+                        continue
+                    v = float(bk.term_coef[terms[i]] * bk.term_coef[terms[j]])
+                    rows += [a, b]
+                    cols += [b, a]
+                    vals += [v, v]
+    u = np.zeros(n)
+    coupled = np.zeros(n, dtype=bool)
+    if rows:
+        C = sps.csr_matrix((vals, (rows, cols)), shape=(n, n))
+        C.sum_duplicates()
+        coupled[np.unique(rows)] = True
+    tol = 1000.0 * np.finfo(np.float64).eps
+    lone = ~coupled & (diag > tol)
+    u[lone] = rhs[lone] / diag[lone]
+    if rows:
+        _, labels = connected_components(C, directed=False)
+        for comp in np.unique(labels[coupled]).tolist():
+            dofs = np.nonzero(coupled & (labels == comp))[0]
+            A = C[dofs][:, dofs].toarray()
+            A[np.arange(len(dofs)), np.arange(len(dofs))] = diag[dofs]
+            u[dofs] = _regularized_cholesky_solve(A[None], rhs[dofs][None])[0]
+    return u
+
+
+def transfer_state(transition, state, linear_solve=None, relative_tolerance=1e-13, maxiter=None):
+    """Old field on the transition's target space (src/transfer.jl:892-1000).
+
+    Default (src/transfer.jl:431-437, 778-804): a local L2 projection on every
+    target cell, reconciled on shared dofs through the normal equations of the
+    local relations (for DG targets this is the cellwise projection). With a
+    linear_solve policy (e.g. CGSolver(preconditioner=JacobiPreconditioner())),
+    the global L2 projection (the reference's variational path) is solved on
+    the device with it. Returns a State of the projection plan (full layout on
+    the target space)."""
     from .plan import compile as compile_plan
-    from .solve import CGSolver, JacobiPreconditioner, solve
+    from .solve import State, solve
 
     plan = compile_plan(projection_problem(transition, state))
-    return solve(plan, solver=CGSolver(preconditioner=JacobiPreconditioner()),
-                 relative_tolerance=relative_tolerance, maxiter=maxiter)
+    if linear_solve is None:
+        return State(plan, reconcile(plan, local_projections(plan, state)))
+    return solve(plan, solver=linear_solve, relative_tolerance=relative_tolerance, maxiter=maxiter)
 
 
 def interface_jump_indicators(state):

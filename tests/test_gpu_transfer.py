@@ -1,7 +1,8 @@
 """State transfer across an hp transition solved on the device.
 
-Reference: src/transfer.jl:892-1000 (transfer_state: L2 projection of the old
-field on the target space). The device Jacobi-CG solve of the projection is
+Reference: src/transfer.jl:892-1000 (transfer_state with a linear_solve policy:
+the global L2 projection of the old field on the target space, the
+reference's variational path 431-437). The device Jacobi-CG solve of the projection is
 compared with the oracle's dense FP64 solve of the same discrete system, and a
 target space that contains the source space must reproduce the old field at
 arbitrary points (nested spaces).
@@ -19,6 +20,7 @@ from test_adaptivity_host import dense_projection, hanging_space, random_points,
 pytestmark = pytest.mark.gpu
 
 RTOL = 1e-13
+JACOBI_CG = g.CGSolver(preconditioner=g.JacobiPreconditioner())
 
 
 def _bound(plan):
@@ -47,7 +49,7 @@ def test_device_transfer_refinement(gpu, dim, degree, continuity):
     state = state_on(space, rng.standard_normal(space.scalar_dof_count()))
     t = ad.transition(_refined_plan(space))
     print(f"transfer refinement dim={dim} p={degree} {continuity}", flush=True)
-    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
+    new = ad.transfer_state(t, state, linear_solve=JACOBI_CG, relative_tolerance=RTOL)
     ref = dense_projection(t, state)
     scale = np.max(np.abs(ref.coefficients))
     tol = _bound(ref.plan) * scale
@@ -65,7 +67,7 @@ def test_device_transfer_p_derefinement(gpu):
         plan.request_p_derefinement(int(l), decrement=2)
     t = ad.transition(plan)
     print("transfer p-derefinement 3 -> 1", flush=True)
-    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
+    new = ad.transfer_state(t, state, linear_solve=JACOBI_CG, relative_tolerance=RTOL)
     ref = dense_projection(t, state)
     scale = np.max(np.abs(ref.coefficients))
     assert np.max(np.abs(new.coefficients - ref.coefficients)) <= _bound(ref.plan) * scale
@@ -82,7 +84,7 @@ def test_device_transfer_h_derefinement(gpu, continuity):
     plan, _ = _derefined_plan(space)
     t = ad.transition(plan)
     print(f"transfer h-derefinement {continuity}", flush=True)
-    new = ad.transfer_state(t, state, relative_tolerance=RTOL)
+    new = ad.transfer_state(t, state, linear_solve=JACOBI_CG, relative_tolerance=RTOL)
     ref = dense_projection(t, state)
     assert new.plan.cloud_buckets, "merged cells must use composite rules"
     scale = np.max(np.abs(ref.coefficients))
