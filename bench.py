@@ -68,8 +68,7 @@ def cpu_baseline(budget_s=12.0):
     import gricoamd as g
     import pyoracle
 
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(int(threads), 64))
+    threads = host_threads()
     plan = g.compile(build_problem(4))
     mats = [pyoracle.action_matrices(plan, bk) for bk in plan.buckets]
     cols = [pyoracle.color_batches(plan, pyoracle._Bucket(bk)) for bk in plan.buckets]
@@ -99,99 +98,141 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
-def solve_leg(problem, plan, repeats=3):
-    """GMG-CG (FP64 and FP32 V-cycles) and Jacobi-CG from x0 = 0 to ||r|| <=
-    1e-8 ||b|| on the device: wall time of the Krylov loop with the hierarchy
-    already compiled."""
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(int(n), 64))
+
+
+def oracle_relative_residuals(plan, solutions, b_red):
+    """{name: ||b - A x|| / ||b||} of device solutions with A the C oracle's
+    FP64 operator (lowered apply, all host threads): each solution checked
+    against the reference algorithm's operator rather than the device's own."""
+    import numpy as np
+
+    import pyoracle
+
+    mats = [pyoracle.action_matrices(plan, bk) for bk in plan.buckets]
+    cols = [pyoracle.color_batches(plan, pyoracle._Bucket(bk)) for bk in plan.buckets]
+    out = {}
+    for name, x_red in solutions.items():
+        full = np.zeros(plan.full_dof_count)
+        full[plan.solve_dofs] = x_red.cpu().numpy()
+        y = pyoracle.apply_lowered(plan, full, threads=host_threads(), matrices=mats, colors=cols,
+                                   physical=True)[plan.solve_dofs]
+        out[name] = float(np.linalg.norm(b_red - y) / np.linalg.norm(b_red))
+    return out
+
+
+def _device_solver_kit(plan, repeats):
+    """(b, x, timed(fn) -> dict, cg(handle) -> info) for device CG solves of
+    the plan's reduced system from x0 = 0 to ||r|| <= 1e-8 ||b||."""
     import ctypes
 
     import torch
 
-    import gricoamd as g
     from gricoamd import _devabi, _native, ops
-    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
 
     lib = _native.device()
-    t0 = time.perf_counter()
-    policy = GeometricMultigridPreconditioner(smoother="chebyshev")
-    hierarchy = compile_geometric_multigrid(problem, policy, fine_plan=plan)
-    t_host = time.perf_counter() - t0
     b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
     x = torch.zeros_like(b)
     dev = ops._ws(plan)
 
-    def run(handle, maxiter):
+    def timed(fn):
+        fn()
+        ts = []
+        info = None
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            info = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        return {"seconds": sorted(ts)[len(ts) // 2], "iterations": info.iterations,
+                "final_relative_residual": info.final_residual_norm / info.rhs_norm}
+
+    def cg(handle, maxiter=20000):
         info = _devabi.SolveInfo()
-        torch.cuda.synchronize()
-        t = time.perf_counter()
         _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, maxiter,
                                        ctypes.byref(info)))
-        torch.cuda.synchronize()
-        return time.perf_counter() - t, info
+        return info
 
-    def gmg(h, tag):
-        t0 = time.perf_counter()
-        M = h.device_preconditioner()
-        torch.cuda.synchronize()
-        t_dev = time.perf_counter() - t0
-        log(f"GMG hierarchy ({tag}): host {t_host:.1f}s, device setup {t_dev:.1f}s, levels {h.reduced_dof_counts()}")
-        run(M.handle, 1000)
-        times = []
-        for _ in range(repeats):
-            dt, info = run(M.handle, 1000)
-            times.append(dt)
-        sol = x.clone()
-        out = {"seconds": sorted(times)[len(times) // 2], "iterations": info.iterations,
-               "operator_applications": info.operator_applications,
-               "final_relative_residual": info.final_residual_norm / info.rhs_norm}
-        r = torch.randn(plan.reduced_dof_count, dtype=torch.float64, device="cuda")
-        z = torch.empty_like(r)
-        for _ in range(2):
-            _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(10):
-            _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
-        torch.cuda.synchronize()
-        out["vcycle_seconds"] = (time.perf_counter() - t) / 10
-        log(f"GMG-CG ({tag}): {out}")
-        return out, sol
+    return b, x, timed, cg
 
-    gmg64, x64 = gmg(hierarchy, "FP64 V-cycle")
-    policy32 = GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32")
-    gmg32, x32 = gmg(hierarchy.with_policy(policy32), "FP32 V-cycle")
-    # The finest levels down to the first one with at most 4096 dofs, solved
-    # there exactly by a device-computed dense inverse (the levels below are
-    # launch-latency bound).
+
+def _vcycle_seconds(M, n, reps=10):
+    import torch
+
+    from gricoamd import _devabi, _native
+
+    lib = _native.device()
+    r = torch.randn(n, dtype=torch.float64, device="cuda")
+    z = torch.empty_like(r)
+    for _ in range(2):
+        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        _devabi.check(lib.gmf_precond_apply(M.handle, r.data_ptr(), z.data_ptr()))
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def gmg_policy_record(pol, levels):
+    return {"smoother": f"Chebyshev-Jacobi degree {pol.chebyshev_degree} on [lambda_max/{pol.chebyshev_range:g}, "
+                        f"lambda_max], V({pol.pre_smoothing_steps},{pol.post_smoothing_steps})",
+            "precision": pol.precision, "p_sequence": pol.p_sequence, "h_steps_per_level": pol.h_steps_per_level,
+            "coarse": f"device dense inverse at <= {pol.coarse_direct_dof_limit} dofs", "levels": levels}
+
+
+def solve_leg(problem, plan, repeats=3):
+    """GMG-CG (FP32 and FP64 Chebyshev V-cycles) and Jacobi-CG from x0 = 0 to
+    ||r|| <= 1e-8 ||b|| on the device, hierarchy already compiled; every
+    solution's residual is re-evaluated with the FP64 oracle operator. The
+    V-cycle (degree 3, hierarchy cut at the first level of at most 4096 dofs,
+    solved there by a device dense inverse) is the fastest of a sweep over
+    degree 2-6, range 20/50, p-sequence and h-steps per level on this
+    problem."""
+    import torch
+
+    import gricoamd as g
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
+    from gricoamd.solve import compile_preconditioner
+
+    t0 = time.perf_counter()
+    hierarchy = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"),
+                                            fine_plan=plan)
+    t_host = time.perf_counter() - t0
     counts = hierarchy.reduced_dof_counts()
     keep = len(counts) - max(i for i, c in enumerate(counts) if c <= 4096)
-    policy_t = GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32", max_levels=keep,
-                                                coarse_direct_dof_limit=4096)
-    gmg32t, x32t = gmg(hierarchy.with_policy(policy_t), "FP32 V-cycle, dense coarse level")
-    from gricoamd.solve import compile_preconditioner
+    b, x, timed, cg = _device_solver_kit(plan, repeats)
+    b_host = b.cpu().numpy()
+    out = {"metric": "time-to-1e-8 relative residual, reduced system (s)", "hierarchy_host_seconds": t_host,
+           "load": "f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)"}
+    sols = {}
+    for tag, prec in (("gmg_cg", "fp32"), ("gmg_cg_fp64_vcycle", "fp64")):
+        pol = GeometricMultigridPreconditioner(smoother="chebyshev", precision=prec, chebyshev_degree=3,
+                                               max_levels=keep, coarse_direct_dof_limit=4096)
+        h = hierarchy.with_policy(pol)
+        M = h.device_preconditioner()
+        res = timed(lambda: cg(M.handle))
+        res["vcycle_seconds"] = _vcycle_seconds(M, plan.reduced_dof_count)
+        res["policy"] = gmg_policy_record(pol, h.reduced_dof_counts())
+        sols[tag] = x.clone()
+        out[tag] = res
+        log(f"C2 {tag}: {res}")
+        del M
     J = compile_preconditioner(g.JacobiPreconditioner(), plan)
-    dt, info = run(J.handle, 20000)
-    jac = {"seconds": dt, "iterations": info.iterations,
-           "final_relative_residual": info.final_residual_norm / info.rhs_norm}
-    nrm = float(torch.linalg.norm(x))
-    diff64 = float(torch.linalg.norm(x - x64)) / nrm
-    diff32 = float(torch.linalg.norm(x - x32)) / nrm
-    diff32t = float(torch.linalg.norm(x - x32t)) / nrm
-    log(f"Jacobi-CG: {jac}; relative difference to GMG-CG solutions: FP64 V-cycle {diff64:.2e}, "
-        f"FP32 V-cycle {diff32:.2e}")
-    return {
-        "metric": "time-to-1e-8 relative residual, reduced system (s)",
-        "gmg_cg": gmg64,
-        "gmg_cg_fp32_vcycle": gmg32,
-        "gmg_cg_fp32_vcycle_dense_coarse": dict(gmg32t, levels=hierarchy.with_policy(policy_t).reduced_dof_counts()),
-        "jacobi_cg": jac,
-        "solution_relative_difference": {"fp64_vcycle": diff64, "fp32_vcycle": diff32,
-                                         "fp32_vcycle_dense_coarse": diff32t},
-        "levels": hierarchy.reduced_dof_counts(),
-        "smoother": f"Chebyshev-Jacobi degree {policy.chebyshev_degree}, V({policy.pre_smoothing_steps},"
-                    f"{policy.post_smoothing_steps}); FP64 CG outside",
-        "load": "f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z)",
-    }
+    out["jacobi_cg"] = timed(lambda: cg(J.handle))
+    sols["jacobi_cg"] = x.clone()
+    out["gmg_cg_speedup_vs_jacobi_cg"] = out["jacobi_cg"]["seconds"] / out["gmg_cg"]["seconds"]
+    nrm = float(torch.linalg.norm(sols["jacobi_cg"]))
+    out["solution_relative_difference_vs_jacobi_cg"] = {
+        k: float(torch.linalg.norm(v - sols["jacobi_cg"])) / nrm for k, v in sols.items() if k != "jacobi_cg"}
+    t0 = time.perf_counter()
+    out["oracle_relative_residual"] = oracle_relative_residuals(plan, sols, b_host)
+    out["oracle_check_seconds"] = time.perf_counter() - t0
+    log(f"C2 solve leg: {out}")
+    return out
 
 
 def build_adaptive_problem(base=16, levels=5, pmax=8):
@@ -762,11 +803,17 @@ def build_lognormal_problem(depth=5, degree=3, seed=2024):
     return problem
 
 
-def lognormal_leg(repeats=3):
-    """GMG-CG (FP64 and FP32 Chebyshev V-cycles) and FGMRES with the FP64
-    V-cycle alone on the log-normal p = 3 problem, from x0 = 0 to
-    ||r|| <= 1e-8 ||b||, hierarchy already built: the baseline side of the
-    learned-correction comparison."""
+def lognormal_leg(repeats=3, depth=5, learned=True, pairs=2000, steps=3000):
+    """The log-normal p = 3 problem (32^3 cells at depth 5) from x0 = 0 to
+    ||r|| <= 1e-8 ||b||, hierarchy and network already built: GMG-CG with FP32
+    and FP64 Chebyshev V-cycles, Jacobi-CG, and FGMRES with the learned block
+    correction around the same FP32 V-cycle (trained on `pairs` GMG-CG
+    residuals, `steps` Adam steps; training reported separately), against
+    GMG-CG. Every solution's residual is re-evaluated with the FP64 oracle
+    operator. The V-cycle (degree 3, range 50, one h-level per full 2x2x2
+    coarsening, hierarchy cut at the first level of at most 4096 dofs and
+    solved there by a device dense inverse) is the fastest of a sweep over
+    degree, range, p-sequence and h-steps per level on this problem."""
     import ctypes
 
     import torch
@@ -774,159 +821,82 @@ def lognormal_leg(repeats=3):
     import gricoamd as g
     from gricoamd import _devabi, _native, ops
     from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
-
-    lib = _native.device()
-    t0 = time.perf_counter()
-    problem = build_lognormal_problem()
-    plan = g.compile(problem)
-    log(f"log-normal problem: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
-    h64 = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"), fine_plan=plan)
-    h32 = h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", precision="fp32"))
-    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
-    x = torch.zeros_like(b)
-    dev = ops._ws(plan)
-
-    def timed(fn):
-        fn()
-        ts = []
-        info = None
-        for _ in range(repeats):
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            info = fn()
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t)
-        return {"seconds": sorted(ts)[len(ts) // 2], "iterations": info.iterations,
-                "final_relative_residual": info.final_residual_norm / info.rhs_norm}
-
-    def cg(handle):
-        info = _devabi.SolveInfo()
-        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 50000,
-                                       ctypes.byref(info)))
-        return info
-
-    def fgmres(handle):
-        info = _devabi.SolveInfo()
-        _devabi.check(lib.gmf_fgmres_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 30, 1e-8, 0.0, 5000,
-                                           ctypes.byref(info)))
-        return info
-
-    M64 = h64.device_preconditioner()
-    out = {"workload": "3D Poisson 32^3 cells (uniform refinement of one root), p=3, log-normal kappa "
-                       "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
-           "reduced_dofs": plan.reduced_dof_count, "levels": h64.reduced_dof_counts()}
-    out["gmg_cg"] = timed(lambda: cg(M64.handle))
-    x64 = x.clone()
-    out["gmg_cg_fp32_vcycle"] = timed(lambda: cg(h32.device_preconditioner().handle))
-    x32 = x.clone()
-    out["fgmres_vcycle"] = timed(lambda: fgmres(M64.handle))
-    # Hierarchy cut at the first level with at most 4096 dofs, solved there by
-    # a device-computed dense inverse (the smaller levels are launch-latency bound).
-    counts = h64.reduced_dof_counts()
-    keep = len(counts) - max(i for i, c in enumerate(counts) if c <= 4096)
-    for tag, prec in (("gmg_cg_dense_coarse", "fp64"), ("gmg_cg_fp32_vcycle_dense_coarse", "fp32")):
-        ht = h64.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", precision=prec, max_levels=keep,
-                                                              coarse_direct_dof_limit=4096))
-        Mt = ht.device_preconditioner()
-        out[tag] = dict(timed(lambda: cg(Mt.handle)), levels=ht.reduced_dof_counts())
     from gricoamd.solve import compile_preconditioner
-    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
-    out["jacobi_cg"] = timed(lambda: cg(J.handle))
-    out["fp32_vs_fp64_solution_relative_difference"] = float(torch.linalg.norm(x32 - x64) / torch.linalg.norm(x64))
-    log(f"log-normal leg: {out}")
-    return out
-
-
-def learned_leg(depth=5, repeats=3, pairs=2000, steps=3000):
-    """Learned-correction FGMRES against GMG-CG on the log-normal p = 3
-    problem (32^3 cells at depth 5): time from x0 = 0 to ||r|| <= 1e-8 ||b||,
-    hierarchy and network already built, training reported separately. Both
-    methods use the same FP64 Chebyshev V-cycle (hierarchy cut at the first
-    level of at most 4096 dofs, solved there by the device dense inverse: the
-    faster of the two GMG-CG variants of the log-normal leg)."""
-    import ctypes
-
-    import numpy as np
-    import torch
-
-    import gricoamd as g
-    from gricoamd import _devabi, _native, ops
-    from gricoamd.learned import train_block_correction
-    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
 
     lib = _native.device()
     t0 = time.perf_counter()
     problem = build_lognormal_problem(depth=depth)
     plan = g.compile(problem)
-    log(f"learned leg: compile {time.perf_counter() - t0:.1f}s, reduced dofs {plan.reduced_dof_count}")
-    counts = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev"),
-                                         fine_plan=plan).reduced_dof_counts()
+    t_compile = time.perf_counter() - t0
+    log(f"log-normal problem: compile {t_compile:.1f}s, reduced dofs {plan.reduced_dof_count}")
+    t0 = time.perf_counter()
+    hierarchy = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(smoother="chebyshev",
+                                                                                      h_steps_per_level=3),
+                                            fine_plan=plan)
+    t_host = time.perf_counter() - t0
+    counts = hierarchy.reduced_dof_counts()
     keep = len(counts) - max([i for i, c in enumerate(counts) if c <= 4096] or [0])
-    inner = GeometricMultigridPreconditioner(smoother="chebyshev", max_levels=keep, coarse_direct_dof_limit=4096)
-    net, report, M, V = train_block_correction(problem, plan=plan, pairs=pairs, steps=steps, inner=inner, log=log)
-    b = ops.to_device(plan.reduced_rhs(), plan.reduced_dof_count)
-    x = torch.zeros_like(b)
-    dev = ops._ws(plan)
+    b, x, timed, cg = _device_solver_kit(plan, repeats)
+    b_host = b.cpu().numpy()
+    out = {"workload": f"3D Poisson {2 ** depth}^3 cells (uniform refinement of one root), p=3, log-normal kappa "
+                       "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
+           "reduced_dofs": plan.reduced_dof_count, "host_compile_seconds": t_compile,
+           "hierarchy_host_seconds": t_host}
+    sols = {}
+    policies = {}
+    for tag, prec in (("gmg_cg", "fp32"), ("gmg_cg_fp64_vcycle", "fp64")):
+        pol = GeometricMultigridPreconditioner(smoother="chebyshev", precision=prec, chebyshev_degree=3,
+                                               chebyshev_range=50.0, h_steps_per_level=3, max_levels=keep,
+                                               coarse_direct_dof_limit=4096)
+        policies[tag] = pol
+        h = hierarchy.with_policy(pol)
+        M = h.device_preconditioner()
+        res = timed(lambda: cg(M.handle))
+        res["vcycle_seconds"] = _vcycle_seconds(M, plan.reduced_dof_count)
+        res["policy"] = gmg_policy_record(pol, h.reduced_dof_counts())
+        sols[tag] = x.clone()
+        out[tag] = res
+        log(f"C4 {tag}: {res}")
+        del M
+    J = compile_preconditioner(g.JacobiPreconditioner(), plan)
+    out["jacobi_cg"] = timed(lambda: cg(J.handle))
+    sols["jacobi_cg"] = x.clone()
+    out["gmg_cg_speedup_vs_jacobi_cg"] = out["jacobi_cg"]["seconds"] / out["gmg_cg"]["seconds"]
+    if learned:
+        from gricoamd.learned import train_block_correction
 
-    def timed(fn):
-        fn()
-        ts = []
-        info = None
-        for _ in range(repeats):
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            info = fn()
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t)
-        return sorted(ts)[len(ts) // 2], info
+        h = hierarchy.with_policy(policies["gmg_cg"])
+        net, report, M, V = train_block_correction(problem, plan=plan, pairs=pairs, steps=steps, hierarchy=h, log=log)
 
-    def cg(handle):
-        info = _devabi.SolveInfo()
-        _devabi.check(lib.gmf_cg_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 1e-8, 0.0, 5000,
-                                       ctypes.byref(info)))
-        return info
+        def fgmres(handle):
+            info = _devabi.SolveInfo()
+            status = lib.gmf_fgmres_solve(ops._ws(plan).ws, handle, b.data_ptr(), x.data_ptr(), 0, 30, 1e-8, 0.0,
+                                          5000, ctypes.byref(info))
+            if status == -3:  # GMF_ENOCONV
+                return None
+            _devabi.check(status)
+            return info
 
-    def fgmres(handle):
-        info = _devabi.SolveInfo()
-        status = lib.gmf_fgmres_solve(dev.ws, handle, b.data_ptr(), x.data_ptr(), 0, 30, 1e-8, 0.0, 5000,
-                                      ctypes.byref(info))
-        if status == -3:
-            return None
-        _devabi.check(status)
-        return info
-
-    def l2_norm(xr):
-        st = g.State(plan, ops.expand_reduced(plan, xr, include_shift=True).cpu().numpy())
-        return g.l2_error(st, lambda p: 0.0 * p[0])
-
-    t_gmg, i_gmg = timed(lambda: cg(V.handle))
-    x_gmg = x.clone()
-    t_l, i_l = timed(lambda: fgmres(M.handle))
-    fallback = i_l is None
-    if fallback:
-        log("learned FGMRES did not converge: GMG-CG fallback")
-        t_l, i_l = t_gmg, i_gmg
-        x.copy_(x_gmg)
-    x_l = x.clone()
-    t_0, i_0 = timed(lambda: fgmres(V.handle))
-    n_gmg = l2_norm(x_gmg)
-    out = {
-        "metric": "time-to-1e-8 relative residual, learned-correction FGMRES vs GMG-CG (s)",
-        "workload": f"3D Poisson {2 ** depth}^3 cells (uniform refinement of one root), p=3, log-normal kappa "
-                    "(corr. length 0.1, sigma 1, seed 2024), unit load, zero Dirichlet",
-        "reduced_dofs": plan.reduced_dof_count,
-        "vcycle_levels": [int(c) for c in counts[-keep:]],
-        "gmg_cg": {"seconds": t_gmg, "iterations": i_gmg.iterations},
-        "learned_fgmres": {"seconds": t_l, "iterations": i_l.iterations if i_l else None, "fallback": fallback},
-        "fgmres_vcycle_only": {"seconds": t_0, "iterations": i_0.iterations if i_0 else None},
-        "speedup_vs_gmg_cg": t_gmg / t_l,
-        "target_speedup": 2.0,
-        "solution_relative_difference": float(torch.linalg.norm(x_l - x_gmg) / torch.linalg.norm(x_gmg)),
-        "solution_l2_norm_relative_difference": abs(l2_norm(x_l) - n_gmg) / n_gmg,
-        "training": report,
-        "network": "per-cell MLP 64-256-256-64, BF16 weights on MFMA, FP32 accumulation",
-    }
-    log(f"learned leg: {out}")
+        first = fgmres(M.handle)
+        if first is None:
+            out["learned_fgmres"] = {"converged": False}
+        else:
+            out["learned_fgmres"] = timed(lambda: fgmres(M.handle))
+            sols["learned_fgmres"] = x.clone()
+            out["learned_speedup_vs_gmg_cg"] = out["gmg_cg"]["seconds"] / out["learned_fgmres"]["seconds"]
+        out["fgmres_vcycle_only"] = timed(lambda: fgmres(V.handle))
+        sols["fgmres_vcycle_only"] = x.clone()
+        out["learned_target_speedup"] = 2.0
+        out["learned_training"] = report
+        out["learned_network"] = "per-cell MLP 64-256-256-64, BF16 weights on MFMA, FP32 accumulation"
+        del M, V, net
+    nrm = float(torch.linalg.norm(sols["jacobi_cg"]))
+    out["solution_relative_difference_vs_jacobi_cg"] = {
+        k: float(torch.linalg.norm(v - sols["jacobi_cg"])) / nrm for k, v in sols.items() if k != "jacobi_cg"}
+    t0 = time.perf_counter()
+    out["oracle_relative_residual"] = oracle_relative_residuals(plan, sols, b_host)
+    out["oracle_check_seconds"] = time.perf_counter() - t0
+    log(f"log-normal leg: {out}")
     return out
 
 
@@ -1225,10 +1195,11 @@ def main():
     ap.add_argument("--fcm-cells", type=int, default=48, help="background cells per axis of the finite-cell leg")
     ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
     ap.add_argument("--fcm-iterations", type=int, default=30, help="CG iterations timed in the finite-cell leg")
-    ap.add_argument("--learned", action="store_true",
-                    help="run the learned-correction FGMRES leg on the log-normal p=3 problem")
-    ap.add_argument("--learned-only", action="store_true", help="run only the learned leg and print its JSON")
-    ap.add_argument("--learned-depth", type=int, default=5, help="uniform refinements of the learned leg (5 -> 32^3)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-correction FGMRES part of the log-normal leg")
+    ap.add_argument("--lognormal-only", action="store_true",
+                    help="run only the log-normal leg (with the learned correction) and print its JSON")
+    ap.add_argument("--lognormal-depth", type=int, default=5, help="uniform refinements of the log-normal leg (5 -> 32^3)")
     ap.add_argument("--learned-pairs", type=int, default=2000, help="training residuals of the learned leg")
     ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps of the learned leg")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -1254,9 +1225,10 @@ def main():
     if args.transfer_only:
         print(json.dumps({"transfer": transfer_leg()}), flush=True)
         return
-    if args.learned_only:
-        print(json.dumps({"learned": learned_leg(depth=args.learned_depth, pairs=args.learned_pairs,
-                                                 steps=args.learned_steps)}), flush=True)
+    if args.lognormal_only:
+        print(json.dumps({"lognormal": lognormal_leg(depth=args.lognormal_depth, learned=not args.no_learned,
+                                                     pairs=args.learned_pairs, steps=args.learned_steps)}),
+              flush=True)
         return
     if args.fcm_only:
         print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)}), flush=True)
@@ -1375,7 +1347,8 @@ def main():
             result["adaptive"] = {"error": repr(exc)}
     if not args.no_lognormal:
         try:
-            result["lognormal"] = lognormal_leg()
+            result["lognormal"] = lognormal_leg(depth=args.lognormal_depth, learned=not args.no_learned,
+                                                pairs=args.learned_pairs, steps=args.learned_steps)
         except Exception as exc:  # keep the headline line
             log(f"log-normal leg failed: {exc!r}")
             result["lognormal"] = {"error": repr(exc)}
@@ -1385,8 +1358,6 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"fcm leg failed: {exc!r}")
             result["fcm"] = {"error": repr(exc)}
-    if args.learned:
-        result["learned"] = learned_leg(depth=args.learned_depth, pairs=args.learned_pairs, steps=args.learned_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

@@ -235,7 +235,14 @@ struct BrickArgs {
   const R* jinv;  // cells x 3
   const R* coef;  // cells x N^3
   R* halo;        // halo values of the bucket (shared-dof major)
-  int accumulate; // 1: results are added to y (other batches wrote it first)
+};
+
+// Result stores of the plain apply: y[i] (+)= v.
+template <class R>
+struct StoreResult {
+  R* y;
+  int accumulate;  // 1: results are added to y (other batches wrote it first)
+  __device__ void operator()(int64_t i, R v, R) const { y[i] = accumulate ? y[i] + v : v; }
 };
 
 // Lattice offset of local mode m along one axis of a cell: vertex 0 at 0,
@@ -259,10 +266,12 @@ constexpr int brick_surface() {
 // 2 runs the eight cell bodies, which add into an LDS result lattice; phase 3
 // stores the positions whose dof no other brick holds to y, and the shared
 // positions (ranked in ascending lattice position by wavefront ballots) to
-// their halo slots, which k_brick_halo sums per dof.
-template <int N, class R>
+// their halo slots, which k_brick_halo sums per dof. The exclusive stores go
+// through `store` (plain result or a fused smoother step, which may update x in
+// place: x is not declared restrict).
+template <int N, class R, class Store>
 __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs<R> a, EOTables<N, R> T,
-                                                                        const R* __restrict__ x, R* __restrict__ y) {
+                                                                        const R* x, Store store) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
   constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC, PER = (L + THREADS - 1) / THREADS;
   constexpr int WAVES = (THREADS + 63) / 64, CPL = (NNN + LPC - 1) / LPC, SURF = brick_surface<N>();
@@ -368,7 +377,7 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int p = tid + k * THREADS;
     if (e[k] & 1) {
       const int32_t d = e[k] >> 1;
-      if (d >= 0) y[d] = a.accumulate ? y[d] + acc[p] : acc[p];
+      if (d >= 0) store(d, acc[p], xl[p]);
     } else {
       int off = 0;
       for (int j = 0; j < k * WAVES + wave; ++j) off += shared_count[j];
@@ -377,19 +386,19 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   }
 }
 
-// Shared dofs: y[d] (+)= the sum of the dof's contiguous halo values
-// (the owner's first, then the other bricks in order: deterministic).
-// Consecutive blocks stay on one XCD.
-template <class R>
-__global__ void k_brick_halo(R* __restrict__ y, const R* __restrict__ halo, const int32_t* __restrict__ dof,
-                             const int32_t* __restrict__ ptr, int64_t count, int accumulate) {
+// Shared dofs: store(d, sum of the dof's contiguous halo values) (the owner's
+// first, then the other bricks in order: deterministic). x is the apply input
+// (read only when the store needs it). Consecutive blocks stay on one XCD.
+template <class R, class Store>
+__global__ void k_brick_halo(const R* x, const R* __restrict__ halo, const int32_t* __restrict__ dof,
+                             const int32_t* __restrict__ ptr, int64_t count, Store store, int needs_input) {
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * int64_t(blockDim.x) + threadIdx.x;
   if (i >= count) return;
   const int32_t d = dof[i];
   const int32_t lo = ptr[i], hi = ptr[i + 1];
   R s = R(0);
   for (int32_t j = lo; j < hi; ++j) s += halo[j];
-  if (d >= 0) y[d] = accumulate ? y[d] + s : s;
+  if (d >= 0) store(d, s, needs_input ? x[d] : R(0));
 }
 
 template <int N, class R>
@@ -436,8 +445,9 @@ size_t brick_lds_bytes() {
   return sizeof(R) * size_t(2 * M * M * M + 8 * 2 * N * N * N);
 }
 
-template <int N, class R>
-void launch_bricks(const Bucket& b, const R* x, R* y, bool reduced, bool accumulate, R* slots, hipStream_t stream) {
+template <int N, class R, class Store>
+void launch_bricks(const Bucket& b, const R* x, bool reduced, R* slots, hipStream_t stream, Store store,
+                   bool needs_input) {
   EOTables<N, R> T = tables_of<N, R>(b);
   const BrickPath& bp = b.bricks;
   require(bp.halo_slots == 0 || slots != nullptr, GMF_EINVAL, "workspace has no brick halo slots");
@@ -448,12 +458,12 @@ void launch_bricks(const Bucket& b, const R* x, R* y, bool reduced, bool accumul
   a.jinv = CellArrays<R>::jinv(b);
   a.coef = CellArrays<R>::coef(b);
   a.halo = slots ? slots + bp.slot_base : nullptr;
-  a.accumulate = accumulate ? 1 : 0;
-  hipLaunchKernelGGL((brick_apply<N, R>), dim3(unsigned(bp.bricks)), dim3(8 * lanes_per_cell<N>()),
-                     (brick_lds_bytes<N, R>()), stream, a, T, x, y);
+  hipLaunchKernelGGL((brick_apply<N, R, Store>), dim3(unsigned(bp.bricks)), dim3(8 * lanes_per_cell<N>()),
+                     (brick_lds_bytes<N, R>()), stream, a, T, x, store);
   if (bp.shared_count > 0)
-    hipLaunchKernelGGL(k_brick_halo<R>, dim3(unsigned((bp.shared_count + 255) / 256)), dim3(256), 0, stream, y,
-                       a.halo, reduced ? bp.halo_red : bp.halo_full, bp.halo_ptr, bp.shared_count, a.accumulate);
+    hipLaunchKernelGGL((k_brick_halo<R, Store>), dim3(unsigned((bp.shared_count + 255) / 256)), dim3(256), 0,
+                       stream, x, a.halo, reduced ? bp.halo_red : bp.halo_full, bp.halo_ptr, bp.shared_count, store,
+                       needs_input ? 1 : 0);
   GMF_HIP(hipGetLastError());
 }
 
@@ -611,18 +621,16 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   std::vector<double> G(b.h_derivs.begin(), b.h_derivs.begin() + n * n);
   b.h_colloc = collocation_derivative(B, G, n);
   b.fast = true;
-  // Small batches (coarse multigrid levels) take the one-launch per-cell
-  // kernel: four colored brick launches and the corner pass cost more than the
-  // atomics they avoid. For p = 1 (8 vertex dofs per cell) that holds up to a
-  // few million cells (measured on 262,144-cell levels: V-cycles 3-10% faster).
-  // GMF_BRICK_MIN_CELLS overrides the threshold, GMF_BRICK_MIN_CELLS_N<n> for
-  // batches of n modes per axis only.
-  int64_t min_cells = n == 2 ? (int64_t(1) << 22) : 32768;
+  // Batches below 32768 cells (coarse multigrid levels) take the one-launch
+  // per-cell kernel; larger ones the brick kernel (also for p = 1: V-cycles of
+  // the 32^3 p = 3 and 64^3 p = 4 Poisson hierarchies measured 3-6% faster
+  // with bricks on their 32768- and 262144-cell p = 1 levels, which can then
+  // also fuse their Chebyshev steps). GMF_BRICK_MIN_CELLS overrides the
+  // threshold, GMF_BRICK_MIN_CELLS_N<n> for batches of n modes per axis only.
+  int64_t min_cells = 32768;
   if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
   const std::string per_n = "GMF_BRICK_MIN_CELLS_N" + std::to_string(n);
   if (const char* env = std::getenv(per_n.c_str())) min_cells = std::strtoll(env, nullptr, 10);
-  if (n == 2)
-    if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS_N2")) min_cells = std::strtoll(env, nullptr, 10);
   if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
 }
 
@@ -633,13 +641,13 @@ bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool redu
   if (!b.fast || b.cells == 0) return false;
   if (reduced && !b.ridx) return false;
   if (b.bricks.enabled && (!reduced || b.bricks.map_red)) {
-    const bool accumulate = p.buckets.size() != 1;
+    const StoreResult<R> st{y, p.buckets.size() != 1 ? 1 : 0};
     switch (b.N[0]) {
-      case 2: launch_bricks<2, R>(b, x, y, reduced, accumulate, slots, stream); return true;
-      case 3: launch_bricks<3, R>(b, x, y, reduced, accumulate, slots, stream); return true;
-      case 4: launch_bricks<4, R>(b, x, y, reduced, accumulate, slots, stream); return true;
-      case 5: launch_bricks<5, R>(b, x, y, reduced, accumulate, slots, stream); return true;
-      case 6: launch_bricks<6, R>(b, x, y, reduced, accumulate, slots, stream); return true;
+      case 2: launch_bricks<2, R>(b, x, reduced, slots, stream, st, false); return true;
+      case 3: launch_bricks<3, R>(b, x, reduced, slots, stream, st, false); return true;
+      case 4: launch_bricks<4, R>(b, x, reduced, slots, stream, st, false); return true;
+      case 5: launch_bricks<5, R>(b, x, reduced, slots, stream, st, false); return true;
+      case 6: launch_bricks<6, R>(b, x, reduced, slots, stream, st, false); return true;
       default: break;
     }
   }
@@ -651,6 +659,22 @@ bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool redu
     case 6: launch_cells<6, R>(b, x, y, reduced, stream); return true;
     case 7: launch_cells<7, R>(b, x, y, reduced, stream); return true;
     case 8: launch_cells<8, R>(b, x, y, reduced, stream); return true;
+    default: return false;
+  }
+}
+
+template <class R>
+bool launch_chebyshev_fused(const gmf_plan& p, const R* u, const ChebFused<R>& c, R* slots, hipStream_t stream) {
+  if (!p.selection_map || p.buckets.size() != 1) return false;
+  const Bucket& b = p.buckets[0];
+  if (b.cloud || !b.fast || b.cells == 0 || !b.bricks.enabled || !b.bricks.map_red) return false;
+  if (std::is_same<R, float>::value && !b.fp32) return false;
+  switch (b.N[0]) {
+    case 2: launch_bricks<2, R>(b, u, true, slots, stream, c, true); return true;
+    case 3: launch_bricks<3, R>(b, u, true, slots, stream, c, true); return true;
+    case 4: launch_bricks<4, R>(b, u, true, slots, stream, c, true); return true;
+    case 5: launch_bricks<5, R>(b, u, true, slots, stream, c, true); return true;
+    case 6: launch_bricks<6, R>(b, u, true, slots, stream, c, true); return true;
     default: return false;
   }
 }
@@ -684,6 +708,16 @@ bool launch_cell_fast32(const gmf_plan& p, const Bucket& b, const float* x, floa
                         hipStream_t stream) {
   if (!b.fp32) return false;
   return launch_fast<float>(p, b, x, y, reduced, slots, stream);
+}
+
+bool chebyshev_fused(const gmf_plan& p, const double* u, const ChebFused<double>& c, double* slots,
+                     hipStream_t stream) {
+  return launch_chebyshev_fused<double>(p, u, c, slots, stream);
+}
+
+bool chebyshev_fused(const gmf_plan& p, const float* u, const ChebFused<float>& c, float* slots,
+                     hipStream_t stream) {
+  return launch_chebyshev_fused<float>(p, u, c, slots, stream);
 }
 
 bool prepare_fp32(gmf_plan& p) {

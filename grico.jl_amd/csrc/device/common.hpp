@@ -291,6 +291,49 @@ bool launch_cell_fast32(const gmf_plan& plan, const Bucket& b, const float* x, f
 // bucket of the plan then has FP32 kernels (and the reduced map is a selection).
 bool prepare_fp32(gmf_plan& plan);
 
+// One Chebyshev smoother step applied where the operator's result of dof i is
+// stored (v = (A u)_i, u_i the apply input at i), with the arithmetic of the
+// separate vector kernels (no contraction of a product into a following sum in
+// either), so fused and unfused steps round alike:
+//   start (u is x):  r = b - v, d = c2 (D^-1 r), x = u + d
+//   step  (u is d):  r = r - v, d = c1 u + c2 (D^-1 r), x = x + d
+// Each reduced dof is stored exactly once per apply (by the brick holding it
+// alone, or by the halo pass after every brick finished), so updating u in
+// place there races with no other read of u.
+template <class R>
+struct ChebFused {
+  R* x;
+  R* d;
+  R* r;
+  const R* b;
+  const R* dinv;
+  R c1, c2;
+  int start;
+  __device__ void operator()(int64_t i, R v, R u) const {
+#pragma clang fp contract(off)
+    if (start) {
+      const R ri = b[i] - v;
+      const R di = c2 * (dinv[i] * ri);
+      r[i] = ri;
+      d[i] = di;
+      x[i] = u + di;
+    } else {
+      const R ri = r[i] - v;
+      const R di = fma(c1, u, c2 * (dinv[i] * ri));
+      r[i] = ri;
+      d[i] = di;
+      x[i] += di;
+    }
+  }
+};
+// The Chebyshev step fused into the reduced apply of u when the plan is one
+// brick bucket on a selection map (u = c.x for start, c.d for step); returns
+// false, launching nothing, for any other plan. slots: the workspace's halo.
+bool chebyshev_fused(const gmf_plan& plan, const double* u, const ChebFused<double>& c, double* slots,
+                     hipStream_t stream);
+bool chebyshev_fused(const gmf_plan& plan, const float* u, const ChebFused<float>& c, float* slots,
+                     hipStream_t stream);
+
 // Copy a host array to a new device allocation owned by the plan.
 template <class T>
 T* upload_array(gmf_plan& p, const T* host, size_t n) {

@@ -12,6 +12,7 @@
 // the coarse solve, the eigenvalue estimates and the diagonals stay FP64.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <memory>
 #include <numeric>
@@ -73,10 +74,13 @@ __global__ void k_prolong_add(R* __restrict__ x, const R* __restrict__ xc, const
   }
 }
 
+// The Chebyshev kernels round every product before a following sum (no
+// contraction), like the fused brick-store variant (ChebFused).
 // Chebyshev start from x = 0: r = b, d = s D^-1 r, x = d.
 template <class R>
 __global__ void k_cheb_start0(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
                               const R* __restrict__ dinv, R s, int64_t n) {
+#pragma clang fp contract(off)
   GMF_GRID_STRIDE(i, n) {
     const R ri = b[i];
     const R di = s * (dinv[i] * ri);
@@ -90,6 +94,7 @@ __global__ void k_cheb_start0(R* __restrict__ x, R* __restrict__ d, R* __restric
 template <class R>
 __global__ void k_cheb_start(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
                              const R* __restrict__ q, const R* __restrict__ dinv, R s, int64_t n) {
+#pragma clang fp contract(off)
   GMF_GRID_STRIDE(i, n) {
     const R ri = b[i] - q[i];
     const R di = s * (dinv[i] * ri);
@@ -103,6 +108,7 @@ __global__ void k_cheb_start(R* __restrict__ x, R* __restrict__ d, R* __restrict
 template <class R>
 __global__ void k_cheb_step(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ q,
                             const R* __restrict__ dinv, R c1, R c2, int64_t n) {
+#pragma clang fp contract(off)
   GMF_GRID_STRIDE(i, n) {
     const R ri = r[i] - q[i];
     const R di = fma(c1, d[i], c2 * (dinv[i] * ri));
@@ -233,6 +239,7 @@ struct MGPrecond : MGBase {
   double cheb_range = 20.0;
   int restart = 8;
   bool spd = true;
+  bool fuse = true;  // Chebyshev steps fused into brick-level applies
   // coarse solver (FP64)
   bool dense = false;
   DeviceBuffer<double> coarse_inverse;
@@ -419,21 +426,36 @@ struct MGPrecond : MGBase {
       if (x_zero && k == 0) {
         hipLaunchKernelGGL(k_cheb_start0<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
                            R(1.0 / theta), n);
-      } else {
+      } else if (!cheb_fused(L, x, ChebFused<R>{x, L.d.p, L.r.p, b, L.invdiag.p, R(0), R(1.0 / theta), 1})) {
         level_apply(L.ws, x, L.q.p);
         hipLaunchKernelGGL(k_cheb_start<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p,
                            L.invdiag.p, R(1.0 / theta), n);
       }
       double rho = 1.0 / sigma;
       for (int i = 1; i < cheb_degree; ++i) {
-        level_apply(L.ws, L.d.p, L.q.p);
         const double rho_new = 1.0 / (2.0 * sigma - rho);
-        hipLaunchKernelGGL(k_cheb_step<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
-                           R(rho_new * rho), R(2.0 * rho_new / delta), n);
+        const R c1 = R(rho_new * rho), c2 = R(2.0 * rho_new / delta);
+        if (!cheb_fused(L, L.d.p, ChebFused<R>{x, L.d.p, L.r.p, nullptr, L.invdiag.p, c1, c2, 0})) {
+          level_apply(L.ws, L.d.p, L.q.p);
+          hipLaunchKernelGGL(k_cheb_step<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
+                             c1, c2, n);
+        }
         rho = rho_new;
       }
       GMF_HIP(hipGetLastError());
     }
+  }
+
+  // Chebyshev start or step fused into the level's apply of u (one brick
+  // bucket on a selection map); false when the level needs the separate
+  // apply and vector kernel. GMF_MG_FUSED=0 turns the fusion off.
+  bool cheb_fused(Level<R>& L, const R* u, const ChebFused<R>& c) {
+    if (!fuse) return false;
+    gmf_workspace* w = L.ws;
+    if constexpr (kDouble)
+      return chebyshev_fused(*w->plan, u, c, w->brick_slots, w->stream);
+    else
+      return chebyshev_fused(*w->plan, u, c, reinterpret_cast<float*>(w->brick_slots), w->stream);
   }
 
   void coarse_solve64(double* x, const double* b) {
@@ -764,6 +786,7 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
   mg->coarse_maxiter = desc->coarse_maxiter;
   mg->restart = desc->smoother_restart;
   mg->spd = desc->spd != 0;
+  if (const char* e = std::getenv("GMF_MG_FUSED")) mg->fuse = std::atoi(e) != 0;
   mg->levels.resize(size_t(L));
   const bool cheb = desc->smoother == GMF_SMOOTHER_CHEBYSHEV;
   const bool gmres = desc->smoother == GMF_SMOOTHER_GMRES;

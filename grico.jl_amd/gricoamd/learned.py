@@ -216,17 +216,21 @@ class LearnedBlockCorrection:
 
 
 def train_block_correction(problem, plan=None, widths=None, pairs=2000, steps=3000, learning_rate=1e-3, seed=0,
-                           inner=None, iterations=40, log=None):
+                           inner=None, iterations=40, log=None, hierarchy=None):
     """Build the GMG hierarchy (policy `inner`, Chebyshev smoothing by
-    default), collect V-cycle residuals from GMG-CG solves on the device and
-    train a CellMLP in the energy norm. Returns (network, report, compiled
-    learned preconditioner, compiled V-cycle)."""
+    default; or take a compiled `hierarchy` on `plan`), collect V-cycle
+    residuals from GMG-CG solves on the device and train a CellMLP in the
+    energy norm. Returns (network, report, compiled learned preconditioner,
+    compiled V-cycle)."""
     from .plan import compile as compile_plan
     torch = ops._torch()
-    plan = plan or compile_plan(problem)
+    plan = plan or (hierarchy.plans[-1] if hierarchy is not None else compile_plan(problem))
     inner = inner or GeometricMultigridPreconditioner(smoother="chebyshev")
     t0 = time.perf_counter()
-    hierarchy = compile_geometric_multigrid(problem, inner, fine_plan=plan)
+    if hierarchy is None:
+        hierarchy = compile_geometric_multigrid(problem, inner, fine_plan=plan)
+    elif hierarchy.plans[-1] is not plan:
+        raise ValueError("the hierarchy's fine level must be the given plan")
     V = hierarchy.device_preconditioner()
     width = plan.buckets[0].n_modes
     widths = widths or (width, 256, 256, width)

@@ -10,6 +10,14 @@ children 563-566). Every level is the problem rediscretized on its space
 (gh_transfer_create, src/multigrid.jl:614-857) and the V-cycle runs on the
 device (gmf_precond_create_mg, src/multigrid.jl:858-1087).
 
+`h_steps_per_level` (no reference counterpart; default 1, the reference's
+hierarchy) merges that many binary refinement frontiers into one h-level: the
+intermediate levels are compiled for their transfers only, and the kept
+levels' prolongation is the product of the intermediate ones (exact, since the
+spaces are nested). With 3 on a uniformly refined 3D mesh each h-level halves
+the mesh along every axis, which removes the small semi-coarsened levels whose
+V-cycle work is launch-latency bound.
+
 Smoothers: "jacobi" is the reference's damped Jacobi V(pre, post) (943-956);
 "chebyshev" is a Chebyshev polynomial in D^-1 A on [lambda_max / range,
 lambda_max] with lambda_max estimated by Lanczos (no reference counterpart).
@@ -63,7 +71,7 @@ class GeometricMultigridPreconditioner:
                  post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
                  coarse_relative_tolerance=1.0e-12, coarse_absolute_tolerance=0.0, coarse_maxiter=10_000,
                  coarse_direct_dof_limit=_DEFAULT_COARSE_DIRECT_DOF_LIMIT, chebyshev_degree=4,
-                 chebyshev_range=20.0, eigen_iterations=12, graph=True, precision="fp64"):
+                 chebyshev_range=20.0, eigen_iterations=12, graph=True, precision="fp64", h_steps_per_level=1):
         self.min_degree = _nonnegative_int(min_degree, "min_degree")
         self.max_levels = _positive_int(max_levels, "max_levels")
         if p_sequence not in ("bisect", "decrease_by_one", "go_to_one"):
@@ -95,6 +103,7 @@ class GeometricMultigridPreconditioner:
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
+        self.h_steps_per_level = _positive_int(h_steps_per_level, "h_steps_per_level")
 
     def resolved_smoother(self, operator_class):
         if self.smoother == "auto":
@@ -162,26 +171,45 @@ def h_coarsened_space(space):
     return _space_with(space, leaves, degrees)
 
 
-def multigrid_spaces(fine_space, policy):
-    """Coarse-to-fine list of level spaces (src/multigrid.jl:443-469)."""
-    spaces = [fine_space]
+def space_chain(fine_space, policy):
+    """Coarse-to-fine list of (space, kept): the p-levels, then one entry per
+    merged binary refinement frontier; with h_steps_per_level = k only every
+    k-th h-entry (and the coarsest) is a level of the hierarchy, the others
+    only carry transfers (src/multigrid.jl:443-469 for k = 1)."""
+    chain = [(fine_space, True)]
+    kept = 1
     current = fine_space
     degrees = np.asarray(fine_space.degrees, dtype=np.int32)
-    while len(spaces) < policy.max_levels:
+    while kept < policy.max_levels:
         nxt = coarsened_degrees(degrees, current.continuity, policy.min_degree, policy.p_sequence)
         if np.array_equal(nxt, degrees):
             break
         current = _space_with(current, current.leaves, nxt)
-        spaces.append(current)
+        chain.append((current, True))
+        kept += 1
         degrees = nxt
-    while len(spaces) < policy.max_levels:
-        nxt = h_coarsened_space(current)
-        if nxt is None:
+    while kept < policy.max_levels:
+        group = []
+        cur = current
+        for _ in range(policy.h_steps_per_level):
+            nxt = h_coarsened_space(cur)
+            if nxt is None:
+                break
+            group.append(nxt)
+            cur = nxt
+        if not group:
             break
-        current = nxt
-        spaces.append(current)
-    spaces.reverse()
-    return spaces
+        chain.extend((sp, False) for sp in group[:-1])
+        chain.append((group[-1], True))
+        kept += 1
+        current = group[-1]
+    chain.reverse()
+    return chain
+
+
+def multigrid_spaces(fine_space, policy):
+    """Coarse-to-fine list of level spaces (src/multigrid.jl:443-469)."""
+    return [sp for sp, kept in space_chain(fine_space, policy) if kept]
 
 
 def has_coarsening_level(space, policy):
@@ -297,6 +325,29 @@ class ReducedTransfer:
         self.coarse_plan = coarse_plan
         self.fine_plan = fine_plan
 
+    @classmethod
+    def compose(cls, transfers):
+        """The prolongation from the first transfer's coarse plan to the last
+        one's fine plan through the intermediate levels (product of the
+        sparse factors), as COO entries sorted by (fine, coarse)."""
+        import scipy.sparse as sps
+        mats = [sps.csr_matrix((t.coefficients, (t.fine_indices, t.coarse_indices)),
+                               shape=(t.fine_plan.reduced_dof_count, t.coarse_plan.reduced_dof_count))
+                for t in transfers]
+        P = mats[0]
+        for m in mats[1:]:
+            P = m @ P
+        P = sps.csr_matrix(P)
+        P.eliminate_zeros()
+        P.sort_indices()
+        out = object.__new__(cls)
+        out.fine_indices = np.repeat(np.arange(P.shape[0], dtype=np.int64), np.diff(P.indptr))
+        out.coarse_indices = P.indices.astype(np.int64)
+        out.coefficients = P.data.astype(np.float64)
+        out.coarse_plan = transfers[0].coarse_plan
+        out.fine_plan = transfers[-1].fine_plan
+        return out
+
     @property
     def nnz(self):
         return len(self.coefficients)
@@ -347,17 +398,21 @@ class MultigridHierarchy:
         self.problem = problem
         self.smoother = policy.resolved_smoother(problem.operator_class)
         _check_smoother(policy, self.smoother)
-        self.spaces = multigrid_spaces(fine_space, policy)
+        chain = space_chain(fine_space, policy)
         plans = []
-        for i, sp in enumerate(self.spaces):
-            if i == len(self.spaces) - 1 and fine_plan is not None:
+        for i, (sp, _) in enumerate(chain):
+            if i == len(chain) - 1 and fine_plan is not None:
                 plans.append(fine_plan)
-            elif i == len(self.spaces) - 1:
+            elif i == len(chain) - 1:
                 plans.append(compile_plan(problem))
             else:
                 plans.append(compile_plan(problem_on_space(problem, sp)))
-        self.plans = plans
-        self.transfers = [ReducedTransfer(plans[i], plans[i + 1]) for i in range(len(plans) - 1)]
+        steps = [ReducedTransfer(plans[i], plans[i + 1]) for i in range(len(plans) - 1)]
+        kept = [i for i, (_, k) in enumerate(chain) if k]
+        self.spaces = [chain[i][0] for i in kept]
+        self.plans = [plans[i] for i in kept]
+        self.transfers = [steps[a] if b == a + 1 else ReducedTransfer.compose(steps[a:b])
+                          for a, b in zip(kept[:-1], kept[1:])]
         self._device = None
 
     def with_policy(self, policy):
@@ -366,7 +421,7 @@ class MultigridHierarchy:
         finest levels (the hierarchy is generated from the fine end, so this is
         the hierarchy that max_levels builds); the other coarsening parameters
         must agree."""
-        for name in ("min_degree", "p_sequence"):
+        for name in ("min_degree", "p_sequence", "h_steps_per_level"):
             if getattr(policy, name) != getattr(self.policy, name):
                 raise ValueError(f"with_policy keeps the hierarchy: {name} must agree")
         if policy.max_levels > self.policy.max_levels and len(self.plans) == self.policy.max_levels:

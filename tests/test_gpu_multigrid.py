@@ -63,6 +63,50 @@ def test_chebyshev_bounds(gpu, name):
         assert 0.9 * true <= lam[l] / 1.2 <= true * (1 + 1e-8)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_chebyshev_equals_unfused(gpu, monkeypatch, precision):
+    """Chebyshev steps fused into the brick levels' applies (the default) give
+    the V-cycle of the separate apply + vector kernels: the fused store performs
+    the same operations on the same values. Not bit for bit, because a brick's
+    eight cells add into its shared lattice points with LDS atomics in no fixed
+    order, so two applies of one vector may differ in the last bit; a wrong
+    coefficient, vector or sign changes the result at O(1). The p = 4, 2, 1
+    levels of 2^3 cells run on bricks (forced in conftest); the h-levels of
+    4 and 2 cells do not and take the unfused path in both runs."""
+    from gricoamd import ops
+    problem = CASES["hp_3d_p4"]()
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", precision=precision)
+    h = compile_geometric_multigrid(problem, pol)
+    assert " bricks=1 " in ops._ws(h.plans[-1]).describe()
+    rs = [P.rng_vector(h.plans[-1].reduced_dof_count, seed) for seed in (3, 4)]
+    fused = [_device_vcycle(h, r) for r in rs]
+    monkeypatch.setenv("GMF_MG_FUSED", "0")
+    plain = h.with_policy(pol)
+    tol = 1e-12 if precision == "fp64" else 1e-5  # last-bit differences of FP64 / FP32 levels
+    for r, z in zip(rs, fused):
+        assert np.abs(_device_vcycle(plain, r) - z).max() <= tol * np.abs(z).max()
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+def test_merged_h_steps_vcycle_matches_oracle(gpu, smoother):
+    """h_steps_per_level = 3 (one h-level per full 2x2x2 coarsening) on a
+    4^3-cell p = 2 mesh: the device V-cycle over composed transfers against
+    the oracle's dense V-cycle with the same level matrices and transfers."""
+    problem = P.poisson(3, None, 2, refine_depth=2, kappa=P.kappa_c2, rhs=False)
+    pol = GeometricMultigridPreconditioner(smoother=smoother, h_steps_per_level=3)
+    h = compile_geometric_multigrid(problem, pol)
+    assert [len(sp.leaves) for sp in h.spaces] == [1, 8, 64, 64]
+    mats = [pyoracle.reduced_matrix(p) for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    lambdas = h.level_lambda_max() if smoother == "chebyshev" else None
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 5)
+    z = _device_vcycle(h, r)
+    ref = pyoracle.vcycle(mats, Ps, r, smoother=smoother, pre=pol.pre_smoothing_steps,
+                          post=pol.post_smoothing_steps, omega=pol.smoother_damping, lambdas=lambdas,
+                          cheb_degree=pol.chebyshev_degree, cheb_range=pol.chebyshev_range)
+    assert np.abs(z - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
 def test_gmg_cg_matches_jacobi_cg_mass(gpu):
     """DG p=3 mass on two cells (test/multigrid.jl:91-101)."""
     d = g.Domain((0.0,), (1.0,), (2,))

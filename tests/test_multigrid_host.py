@@ -203,3 +203,34 @@ def test_gmres_smoother_policy_checks():
         compile_geometric_multigrid(pr, GeometricMultigridPreconditioner(precision="fp32"))
     h = compile_geometric_multigrid(pr, GeometricMultigridPreconditioner())
     assert h.smoother == "gmres"
+
+
+@pytest.mark.parametrize("steps", [2, 3])
+@pytest.mark.parametrize("name", ["cube_p2", "corner_3d"])
+def test_merged_h_steps_galerkin_identity(name, steps):
+    """h_steps_per_level = k keeps every k-th h-level; the kept levels'
+    prolongation (a product of k single-merge transfers) must still satisfy
+    P^T A_fine P = A_coarse exactly, and the kept spaces are the single-step
+    hierarchy's spaces at every k-th h-level."""
+    problem = {"cube_p2": lambda: P.poisson(3, None, 2, refine_depth=2, rhs=False),
+               "corner_3d": lambda: P.corner_refined(3, base=2, levels=1, degree=2)}[name]()
+    h1 = compile_geometric_multigrid(problem)
+    hk = compile_geometric_multigrid(problem, GeometricMultigridPreconditioner(h_steps_per_level=steps))
+    leaves1 = [len(sp.leaves) for sp in h1.spaces]
+    p_levels = sum(1 for sp in h1.spaces if len(sp.leaves) == leaves1[-1]) - 1
+    n_h = len(leaves1) - 1 - p_levels
+    picked = sorted({0} | set(range(n_h % steps, n_h, steps)))
+    assert [len(sp.leaves) for sp in hk.spaces] == [leaves1[i] for i in picked] + leaves1[n_h:]
+    assert hk.level_count == len(picked) + p_levels + 1
+    mats = [pyoracle.reduced_matrix(p) for p in hk.plans]
+    for l, t in enumerate(hk.transfers):
+        Ac, Af = mats[l], mats[l + 1]
+        assert t.coarse_plan is hk.plans[l] and t.fine_plan is hk.plans[l + 1]
+        order = np.lexsort((t.coarse_indices, t.fine_indices))
+        assert np.array_equal(order, np.arange(t.nnz)), "entries sorted by (fine, coarse)"
+        if Ac.size == 0:
+            continue
+        Pm = t.matrix()
+        assert np.abs(Pm.T @ Af @ Pm - Ac).max() <= 1e-11 * np.abs(Ac).max()
+    with pytest.raises(ValueError):
+        h1.with_policy(GeometricMultigridPreconditioner(h_steps_per_level=steps))
