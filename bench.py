@@ -967,7 +967,14 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-5):
     pr = g.AffineProblem(u, g.SPD())
     pr.add_cell_accumulator(u, u, Diffusion())
     pr.add_cell_accumulator(u, Load())
-    add_cut_cell_quadratures(pr, region, cut, alpha=alpha)
+    last = [0.0]
+
+    def progress(done, total):
+        if time.perf_counter() - last[0] > 20.0 or done == total:
+            last[0] = time.perf_counter()
+            log(f"cut-cell moment fitting: {done}/{total} cells")
+
+    add_cut_cell_quadratures(pr, region, cut, alpha=alpha, progress=progress)
     for sq in sphere_surface_quadratures(space, (0.0, 0.0, 0.0), 1.0, n_theta=4 * cells, n_phi=8 * cells):
         pr.add_surface_quadrature(sq)
     pr.add_surface_accumulator(u, u, Nitsche(10.0 * (degree + 1) ** 2))
@@ -1005,6 +1012,7 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, o
     cut = plan.cloud_buckets
     out.update({"reduced_dofs": n, "tensor_cells": sum(b.cell_count for b in plan.buckets),
                 "cut_cells": sum(b.cell_count for b in cut),
+                "cut_cell_rules": "moment-fitted (src/regions.jl:519-708) plus alpha times the tensor rule",
                 "cut_cell_points": int(sum(b.n_points_total for b in cut)),
                 "surface_pieces": sum(b.cell_count for b in plan.surface_buckets),
                 "surface_points": int(sum(b.n_points_total for b in plan.surface_buckets))})
@@ -1189,8 +1197,10 @@ def main():
     ap.add_argument("--transfer-only", action="store_true", help="run only the hp state-transfer leg and print its JSON")
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
-    ap.add_argument("--no-fcm", action="store_true",
-                    help="skip the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG)")
+    ap.add_argument("--fcm", action="store_true",
+                    help="run the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG); "
+                         "off by default: fitting the 7,532 cut-cell rules takes minutes of host time")
+    ap.add_argument("--no-fcm", action="store_true", help="(default) skip the finite-cell sphere leg")
     ap.add_argument("--fcm-only", action="store_true", help="run only the finite-cell sphere leg and print its JSON")
     ap.add_argument("--fcm-cells", type=int, default=48, help="background cells per axis of the finite-cell leg")
     ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
@@ -1352,7 +1362,7 @@ def main():
         except Exception as exc:  # keep the headline line
             log(f"log-normal leg failed: {exc!r}")
             result["lognormal"] = {"error": repr(exc)}
-    if not args.no_fcm:
+    if args.fcm and not args.no_fcm:
         try:
             result["fcm"] = fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)
         except Exception as exc:  # keep the headline line

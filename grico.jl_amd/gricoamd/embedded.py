@@ -7,17 +7,22 @@ negative = inside); cut cells get a candidate rule by recursive bisection to
 `subdivision_depth` (_collect_finite_cell_candidates_recursive! 477-517):
 inside subcells take the cell's tensor rule mapped to the subcell, terminal
 cut subcells keep only their inside points, with the corner/midpoint fallback
-for slivers (759-817). The reference then compresses the candidates by
-non-negative moment fitting (519-708, NNLS with SVD); that step is not
-restated here: the candidate rule is used as is, so cut cells carry more
-points than in the reference, and the points and weights of cut cells are
-parity unpinned (the reference's own tests check them only for finiteness).
+for slivers (759-817). The candidates are then compressed by the reference's
+non-negative moment fitting (519-708): the Legendre moments of the full
+tensor box 0..q_a, the NNLS active-set loop with SVD minimum-norm solves, and
+point elimination down to prod(q) points while the moments stay fitted to
+1000 eps. Where the fit does not reach that tolerance the candidate rule is
+kept, as in the reference (on the 48^3 unit-ball mesh at depth 3 about half
+of the cut cells keep their candidates and the fitted ones stop at
+(q + 1)^D points, where no single removal keeps the fit).
 
 The surface rule for a sphere is a product rule on the sphere (Gauss-Legendre
 in cos(theta), uniform in phi) whose points are assigned to the leaves that
 contain them and mapped to reference coordinates with the weight and normal
 transforms of src/integration.jl:2501-2516.
 """
+
+import os
 
 import numpy as np
 
@@ -157,6 +162,184 @@ def cut_cell_rules(domain, region, leaves, quadrature_shape):
     return out
 
 
+# ---- non-negative moment fitting (src/regions.jl:519-708) ----------------------
+
+_MOMENT_TOLERANCE = 1.0e3 * np.finfo(np.float64).eps  # 519, 523
+_WEIGHT_PRUNE_RATIO = 1.0e-8                          # 520
+_MAX_ELIMINATION_ITERATIONS = 1024                    # 521
+
+
+def _legendre_table(x, degree):
+    """P_0..P_degree at the points x (n) by the three-term recurrence
+    (src/polynomials.jl:294-307): (n, degree + 1)."""
+    t = np.empty((len(x), degree + 1))
+    t[:, 0] = 1.0
+    if degree >= 1:
+        t[:, 1] = x
+    for n in range(2, degree + 1):
+        inv_n = 1.0 / n
+        t[:, n] = (2 * n - 1) * inv_n * x * t[:, n - 1] - (n - 1) * inv_n * t[:, n - 2]
+    return t
+
+
+def moment_matrix(points, shape):
+    """Legendre moments of the full tensor box 0..q_a at the points: modes
+    (first axis fastest) x points (src/regions.jl:538-561), each product
+    formed axis by axis in the reference's order (rounding-identical)."""
+    n, D = points.shape
+    grids = np.meshgrid(*[np.arange(q + 1) for q in shape][::-1], indexing="ij")
+    modes = [g.reshape(-1) for g in grids[::-1]]  # per axis, first axis fastest
+    M = np.ones((len(modes[0]), n))
+    for a in range(D):
+        M = M * _legendre_table(points[:, a], shape[a]).T[modes[a]]
+    return M
+
+
+def _least_squares_min_norm(A, b, tol):
+    """Minimum-norm least squares, singular values at most tol * s_max
+    dropped (src/regions.jl:709-725)."""
+    if A.shape[1] == 0:
+        return np.zeros(0)
+    U, S, Vt = np.linalg.svd(A, full_matrices=False)
+    keep = S > tol * S[0]
+    scaled = np.where(keep, (U.T @ b) / np.where(keep, S, 1.0), 0.0)
+    return Vt.T @ scaled
+
+
+def _nnls(A, b, tol):
+    """Non-negative least squares by the reference's active-set loop
+    (src/regions.jl:641-707): the entering column is the first one of largest
+    positive gradient, passive solves are minimum-norm, infeasible steps are
+    cut back to the first passive weight that reaches tol. Returns (weights,
+    |b - A w| / max(|b|, 1))."""
+    n = A.shape[1]
+    w = np.zeros(n)
+    passive = np.zeros(n, dtype=bool)
+    gradient = A.T @ b
+    for _ in range(max(8 * n, 32)):
+        g = np.where(passive, 0.0, gradient)
+        cand = int(np.argmax(g))
+        if not g[cand] > 0.0 or not gradient[cand] > tol:
+            break
+        passive[cand] = True
+        while True:
+            idx = np.nonzero(passive)[0]
+            sol = _least_squares_min_norm(A[:, idx], b, tol)
+            if np.all(sol > tol):
+                w = np.zeros(n)
+                w[idx] = sol
+                break
+            cur = w[idx]
+            den = cur - sol
+            step = (sol <= tol) & (np.abs(den) > tol)
+            alpha = min(1.0, float(np.min(cur[step] / den[step]))) if np.any(step) else 1.0
+            w[idx] = cur + alpha * (sol - cur)
+            drop = idx[w[idx] <= tol]
+            w[drop] = 0.0
+            passive[drop] = False
+            if not np.any(passive):
+                break
+        gradient = A.T @ (b - A @ w)
+    return w, float(np.linalg.norm(b - A @ w) / max(np.linalg.norm(b), 1.0))
+
+
+def _removable(weights, min_points, tol):
+    """src/regions.jl:621-630: weights at most max(tol, 1e-8 max w) in
+    increasing order, else the smallest one."""
+    if len(weights) <= min_points:
+        return []
+    threshold = max(tol, _WEIGHT_PRUNE_RATIO * float(np.max(weights)))
+    small = np.nonzero(weights <= threshold)[0]
+    if len(small) == 0:
+        return [int(np.argmin(weights))]
+    return [int(i) for i in small[np.argsort(weights[small], kind="stable")]]
+
+
+def moment_fit(points, weights, shape):
+    """Fitted rule (points, weights) of a candidate rule on the biunit cell
+    (src/regions.jl:523-536, _point_elimination 563-619); the candidate itself
+    when it has at most prod(shape) points or the fit misses the tolerance."""
+    points = np.asarray(points, dtype=np.float64)
+    weights = np.asarray(weights, dtype=np.float64)
+    min_points = int(np.prod(shape))
+    if len(points) <= min_points or int(np.prod([q + 1 for q in shape])) <= 1:
+        return points, weights
+    tol = _MOMENT_TOLERANCE
+    A = moment_matrix(points, shape)
+    moments = A @ weights
+    w, res = _nnls(A, moments, tol)
+    if not res <= tol:
+        return points, weights
+    current = np.nonzero(w > tol)[0]
+    if len(current) == 0:
+        return points, weights
+    cw = w[current]
+    if len(current) > A.shape[0]:
+        current = current[np.argsort(-cw, kind="stable")[:A.shape[0]]]
+        cw, res = _nnls(A[:, current], moments, tol)
+        if not res <= tol:
+            return points, weights
+    for _ in range(_MAX_ELIMINATION_ITERATIONS):
+        if len(current) <= min_points:
+            break
+        accepted = False
+        for li in _removable(cw, min_points, tol):
+            trial = np.delete(current, li)
+            if len(trial) < min_points:
+                continue
+            tw, res = _nnls(A[:, trial], moments, tol)
+            if not res <= tol:
+                continue
+            current, cw, accepted = trial, tw, True
+            break
+        if not accepted:
+            break
+    return points[current], cw
+
+
+def _fit_rules(items):
+    return [moment_fit(p, w, shape) for p, w, shape in items]
+
+
+def fitted_cut_cell_rules(domain, region, leaves, quadrature_shape, workers=None, progress=None):
+    """cut_cell_rules compressed by moment fitting (src/regions.jl:304-329):
+    dict leaf -> PointQuadrature. Cells are fitted in `workers` spawned
+    processes (default: the host's CPUs, at most 16) when there are many;
+    `progress(done, total)` is called as batches of cells finish."""
+    cands = cut_cell_rules(domain, region, leaves, quadrature_shape)
+    keys = list(cands)
+    items = [(cands[k].points, cands[k].weights, tuple(quadrature_shape)) for k in keys]
+    if workers is None:
+        workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        workers = max(1, min(int(workers), 16))
+    if workers > 1 and len(items) >= 64:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        chunk = max(1, len(items) // (8 * workers))
+        parts = [items[i:i + chunk] for i in range(0, len(items), chunk)]
+        # one BLAS thread per worker: the solves are small and many, and
+        # threaded BLAS in every worker oversubscribes the cores badly
+        names = ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS", "MKL_NUM_THREADS")
+        saved = {k: os.environ.get(k) for k in names}
+        os.environ.update({k: "1" for k in names})
+        try:
+            with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+                fitted = []
+                for part in ex.map(_fit_rules, parts):
+                    fitted.extend(part)
+                    if progress is not None:
+                        progress(len(fitted), len(items))
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
+    else:
+        fitted = _fit_rules(items)
+    return {k: PointQuadrature(p, w) for k, (p, w) in zip(keys, fitted)}
+
+
 def physical_space(domain, region, options):
     """HpSpace on the leaves that are not outside the region
     (src/regions.jl:267-288), plus their classification."""
@@ -180,7 +363,7 @@ def _tensor_rule(shape):
     return pts, w.reshape(-1)
 
 
-def add_cut_cell_quadratures(problem, region, cut_leaves, alpha=0.0):
+def add_cut_cell_quadratures(problem, region, cut_leaves, alpha=0.0, progress=None):
     """Attach the cut-cell rules of `cut_leaves` to the problem; returns the
     total number of cut-cell points. alpha > 0 is the finite-cell extension
     (src/regions.jl:56-87, 352-380): physical weights scaled by 1 - alpha plus
@@ -195,7 +378,7 @@ def add_cut_cell_quadratures(problem, region, cut_leaves, alpha=0.0):
     total = 0
     for shape, ls in shapes.items():
         fp, fw = _tensor_rule(shape)
-        for leaf, rule in cut_cell_rules(space.domain, region, ls, shape).items():
+        for leaf, rule in fitted_cut_cell_rules(space.domain, region, ls, shape, progress=progress).items():
             if alpha > 0.0:
                 rule = PointQuadrature(np.concatenate([rule.points, fp], axis=0),
                                        np.concatenate([(1.0 - alpha) * rule.weights, alpha * fw]))

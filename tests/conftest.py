@@ -9,6 +9,18 @@ import pytest
 # below 32768 cells to the per-cell kernel by default).
 os.environ.setdefault("GMF_BRICK_MIN_CELLS", "0")
 
+# Parallel test workers (pytest -n): one BLAS thread each, or every worker's
+# BLAS pool spins on all cores and the small dense solves of the tests crawl.
+if "PYTEST_XDIST_WORKER" in os.environ:
+    for _name in ("OPENBLAS_NUM_THREADS", "OMP_NUM_THREADS"):
+        os.environ.setdefault(_name, "1")
+    try:
+        from threadpoolctl import threadpool_limits
+
+        threadpool_limits(1)
+    except ImportError:
+        pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "grico.jl_amd")
 for p in (PKG, os.path.join(ROOT, "oracle"), os.path.dirname(os.path.abspath(__file__))):
