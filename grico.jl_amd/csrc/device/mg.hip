@@ -699,8 +699,16 @@ DeviceBuffer<double> device_dense_inverse(gmf_workspace* ws, int64_t n, bool spd
 // _DEFAULT_COARSE_DIRECT_DOF_LIMIT); larger coarse levels are inverted on the device.
 constexpr int64_t kHostInverseLimit = 512;
 
+// Largest dense coarse level: rocSOLVER's potrf / potri / getrf / getri take
+// 32-bit sizes and leading dimensions, and n x n elements must stay below 2^31
+// for their element offsets (46340^2 < 2^31 <= 46341^2).
+constexpr int64_t kDenseCoarseMax = 46340;
+
 // Dense coarse inverse from the probed SPD coarse matrix (src/multigrid.jl:382-424).
 DeviceBuffer<double> dense_coarse_inverse(gmf_workspace* ws, int64_t n, bool spd) {
+  require(n <= kDenseCoarseMax, GMF_EUNSUPPORTED,
+          "dense coarse multigrid level has " + std::to_string(n) + " dofs; the device dense inverse supports at most " +
+              std::to_string(kDenseCoarseMax) + " (lower coarse_direct_dof_limit or keep more levels)");
   hipStream_t s = ws->stream;
   DeviceBuffer<double> e(static_cast<size_t>(n)), col(static_cast<size_t>(n * n));
   for (int64_t j = 0; j < n; ++j) {

@@ -34,6 +34,7 @@ from .problem import SPD, AffineProblem, Dirichlet, MeanValue, ScalarField
 from .space import HpSpace, SpaceOptions, StoredDegrees
 
 _DEFAULT_COARSE_DIRECT_DOF_LIMIT = 512
+DENSE_COARSE_MAX = 46340  # largest n with n^2 < 2^31 (rocSOLVER 32-bit sizes)
 
 SMOOTHERS = {"jacobi": 0, "chebyshev": 1, "gmres": 2}
 
@@ -451,9 +452,18 @@ class MultigridHierarchy:
         n = self.plans[0].reduced_dof_count
         return "dense_cholesky" if 0 < n <= self.policy.coarse_direct_dof_limit else "krylov"
 
+    def _check_dense_coarse(self):
+        """The device dense inverse holds n^2 < 2^31 entries (rocSOLVER's
+        32-bit sizes): larger dense coarse levels are rejected up front."""
+        n = self.plans[0].reduced_dof_count
+        if self.coarse_solver_kind() == "dense_cholesky" and n > DENSE_COARSE_MAX:
+            raise ValueError(f"dense coarse multigrid level has {n} dofs; the device dense inverse supports at most "
+                             f"{DENSE_COARSE_MAX} (lower coarse_direct_dof_limit or keep more levels)")
+
     def device_preconditioner(self):
         """Compile the V-cycle on the device (gmf_precond_create_mg)."""
         if self._device is None:
+            self._check_dense_coarse()
             from . import _devabi, ops
             from .solve import DevicePreconditioner
             lib = _native.device()

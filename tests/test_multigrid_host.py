@@ -234,3 +234,22 @@ def test_merged_h_steps_galerkin_identity(name, steps):
         assert np.abs(Pm.T @ Af @ Pm - Ac).max() <= 1e-11 * np.abs(Ac).max()
     with pytest.raises(ValueError):
         h1.with_policy(GeometricMultigridPreconditioner(h_steps_per_level=steps))
+
+
+def test_dense_coarse_level_size_is_checked():
+    """A dense coarse level above 46340 dofs (n^2 >= 2^31, beyond rocSOLVER's
+    32-bit sizes) is rejected before anything is built on the device."""
+    from gricoamd import multigrid as mgm
+    h = compile_geometric_multigrid(P.poisson(2, None, 2, refine_depth=2, rhs=False))
+    h = h.with_policy(GeometricMultigridPreconditioner(max_levels=2))
+    n = h.plans[0].reduced_dof_count
+    assert n > 0 and h.coarse_solver_kind() == "dense_cholesky"
+    assert mgm.DENSE_COARSE_MAX ** 2 < 2 ** 31 <= (mgm.DENSE_COARSE_MAX + 1) ** 2
+    h._check_dense_coarse()  # a small dense coarse level passes
+    saved = mgm.DENSE_COARSE_MAX
+    try:
+        mgm.DENSE_COARSE_MAX = n - 1
+        with pytest.raises(ValueError, match="dense coarse multigrid level"):
+            h._check_dense_coarse()
+    finally:
+        mgm.DENSE_COARSE_MAX = saved
