@@ -342,6 +342,7 @@ def adaptive_leg(repeats=3, steps=20):
                        "anisotropic dyadic refinement at (0,0,0), degree 1..8 rising with distance, u = r^(2/3)",
            "full_dofs": n, "reduced_dofs": plan.reduced_dof_count,
            "buckets": [list(bk.degrees) + [bk.cell_count] for bk in plan.buckets],
+           "kernel_tuning": dev.tuning(),
            "apply_gdofs": n / apply_s / 1e9, "apply_seconds": apply_s,
            "levels": h64.reduced_dof_counts(), "host_compile_seconds": t_plan, "hierarchy_seconds": t_h}
     counts = h64.reduced_dof_counts()
@@ -961,7 +962,13 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-5):
                                   tuple(-trial.value * nc for nc in n))
 
     region = ImplicitRegion(lambda p: p[0] ** 2 + p[1] ** 2 + p[2] ** 2 - 1.0, subdivision_depth=depth)
-    dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (cells, cells, cells))
+    # cells = roots * 2^k with roots odd: roots^3 root cells refined k times, so
+    # that multigrid finds h-levels (the same leaves as cells^3 root cells).
+    roots, k = cells, 0
+    while roots % 2 == 0:
+        roots, k = roots // 2, k + 1
+    dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (roots, roots, roots))
+    g.uniform_refine(dom, k)
     space, cut = physical_space(dom, region, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
     u = g.ScalarField(space, "u")
     pr = g.AffineProblem(u, g.SPD())
@@ -1014,6 +1021,7 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, o
                 "cut_cells": sum(b.cell_count for b in cut),
                 "cut_cell_rules": "moment-fitted (src/regions.jl:519-708) plus alpha times the tensor rule",
                 "cut_cell_points": int(sum(b.n_points_total for b in cut)),
+                "kernel_tuning": [t for t in dev.tuning() if t["kernel"] == "point-cloud"],
                 "surface_pieces": sum(b.cell_count for b in plan.surface_buckets),
                 "surface_points": int(sum(b.n_points_total for b in plan.surface_buckets))})
     log(f"fcm: {out}")
@@ -1304,6 +1312,7 @@ def main():
                         f"full-layout matrix-free apply",
             "full_dofs": n,
             "cells": cells,
+            "kernel_tuning": dev.tuning(),
         },
         "roofline": {
             "bound": "hbm",

@@ -25,6 +25,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "internal.hpp"
 #include "termmap.hpp"
 
 namespace gmf {
@@ -48,6 +49,8 @@ struct CloudArgs {
   TermMap map;                 // items x Ntot
   const int32_t* unit_item;    // work unit -> item
   const int64_t* unit_start;   // work unit -> first point
+  int unit_points;             // points per work unit
+  int lds_kb;                  // LDS budget of one workgroup in KB
 };
 
 // Item and point range [pb, pe) of the apply kernels' work unit `u`.
@@ -55,7 +58,7 @@ __device__ inline int64_t unit_range(const CloudArgs& a, int64_t u, int64_t& pb,
   const int64_t item = a.unit_item[u];
   pb = a.unit_start[u];
   const int64_t end = a.poff[item + 1];
-  pe = end - pb < kCloudUnitPoints ? end : pb + kCloudUnitPoints;
+  pe = end - pb < a.unit_points ? end : pb + a.unit_points;
   return item;
 }
 
@@ -451,14 +454,15 @@ size_t block_lds(const CloudArgs& a) {
                            size_t(2 * a.S * a.D * a.Nmax * a.ts));
 }
 
-// LDS budget per workgroup in KB (GMF_CLOUD_LDS_KB, 8..64; default 64).
-size_t cloud_lds_budget() {
-  static const size_t kb = [] {
+// LDS budget per workgroup in KB: GMF_CLOUD_LDS_KB (8..64) when set, else the
+// bucket's tuned budget.
+size_t cloud_lds_budget(const CloudArgs& a) {
+  static const int forced = [] {
     const char* e = std::getenv("GMF_CLOUD_LDS_KB");
-    const int v = e ? std::atoi(e) : 64;
-    return size_t(v >= 8 && v <= 64 ? v : 64);
+    const int v = e ? std::atoi(e) : 0;
+    return v >= 8 && v <= 64 ? v : 0;
   }();
-  return kb * 1024;
+  return size_t(forced ? forced : a.lds_kb) * 1024;
 }
 
 // Largest chunk (64, 32, .., 8 points) whose LDS footprint fits the budget
@@ -467,7 +471,7 @@ template <class F>
 void pick_chunk(CloudArgs& a, F lds_of) {
   a.chunk = kCloudLanes;
   a.ts = a.chunk + 1;
-  while (a.chunk > 8 && lds_of(a) > cloud_lds_budget()) {
+  while (a.chunk > 8 && lds_of(a) > cloud_lds_budget(a)) {
     a.chunk /= 2;
     a.ts = a.chunk + 1;
   }
@@ -500,6 +504,8 @@ CloudArgs make_cloud_args(const Bucket& b, bool reduced) {
   a.map = TermMap{reduced ? b.ridx : b.lidx, b.lcoef, b.moff, reduced ? b.mridx : b.midx, b.mcoef, b.Ntot};
   a.unit_item = b.unit_item;
   a.unit_start = b.unit_start;
+  a.unit_points = b.unit_points;
+  a.lds_kb = b.lds_kb;
   return a;
 }
 
@@ -574,6 +580,91 @@ void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double
   pick_chunk(a, apply_lds);
   hipLaunchKernelGGL(cell_apply_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), apply_lds(a), stream, a, x, y);
   GMF_HIP(hipGetLastError());
+}
+
+int64_t cloud_units(const Bucket& b, int unit_points, std::vector<int32_t>& item, std::vector<int64_t>& start) {
+  require(unit_points > 0 && int64_t(b.h_poff.size()) == b.cells + 1, GMF_EINVAL, "invalid point-cloud work units");
+  item.clear();
+  start.clear();
+  const int64_t* po = b.h_poff.data();
+  for (int64_t c = 0; c < b.cells; ++c)
+    for (int64_t s0 = po[c]; s0 == po[c] || s0 < po[c + 1]; s0 += unit_points) {
+      item.push_back(int32_t(c));
+      start.push_back(s0);
+    }
+  return int64_t(item.size());
+}
+
+// Work-unit size (256 .. 2048 points) and LDS budget (64 or 32 KB, which
+// halves the chunk of large boxes) of every point-cloud batch that runs the
+// point-cloud kernel, timed on its one-column apply. Candidates that give the
+// same work decomposition and chunk as one already timed are skipped;
+// GMF_CLOUD_UNIT_POINTS restricts the work-unit size to one value.
+void tune_cloud(gmf_plan& p, TuneTimer& t) {
+  int forced = 0;  // GMF_CLOUD_UNIT_POINTS: time only this work-unit size
+  if (const char* e = std::getenv("GMF_CLOUD_UNIT_POINTS")) forced = std::atoi(e);
+  for (Bucket& b : p.buckets) {
+    if (!b.cloud || b.cells == 0 || b.face.enabled) continue;
+    TuneInfo& ti = b.tune;
+    ti.kernel = kTuneCloud;
+    const int64_t def_units = b.units;
+    int32_t* def_item = b.unit_item;
+    int64_t* def_start = b.unit_start;
+    const int def_points = b.unit_points, def_kb = b.lds_kb;
+    int best_points = def_points, best_kb = def_kb;
+    float best = -1.0f;
+    std::vector<int32_t> best_item;
+    std::vector<int64_t> best_start;
+    int64_t last_units = -1;
+    for (int up : {256, 512, 1024, 2048}) {
+      if (forced > 0 && up != forced) continue;
+      std::vector<int32_t> item;
+      std::vector<int64_t> start;
+      const int64_t units = cloud_units(b, up, item, start);
+      if (units == last_units) continue;  // no item is split differently
+      last_units = units;
+      DeviceBuffer<int32_t> di(item.size());
+      DeviceBuffer<int64_t> ds(start.size());
+      GMF_HIP(hipMemcpy(di.p, item.data(), item.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      GMF_HIP(hipMemcpy(ds.p, start.data(), start.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+      b.units = units;
+      b.unit_item = di.p;
+      b.unit_start = ds.p;
+      b.unit_points = up;
+      int last_chunk = -1;
+      for (int kb : {64, 32}) {
+        b.lds_kb = kb;
+        CloudArgs a = make_cloud_args(b, false);
+        pick_chunk(a, block_lds<1>);
+        if (a.chunk == last_chunk) continue;  // the budget changes nothing
+        last_chunk = a.chunk;
+        const float us = t.time_us([&] { launch_cell_cloud(p, b, t.x, t.y, false, nullptr); });
+        ++ti.candidates;
+        if (up == def_points && kb == def_kb) ti.default_us = us;
+        if (best < 0.0f || us < best) {
+          best = us;
+          best_points = up;
+          best_kb = kb;
+          best_item = item;
+          best_start = start;
+        }
+      }
+    }
+    b.units = def_units;
+    b.unit_item = def_item;
+    b.unit_start = def_start;
+    b.unit_points = def_points;
+    b.lds_kb = best_kb;
+    if (best_points != def_points) {
+      b.units = int64_t(best_item.size());
+      b.unit_item = upload_array(p, best_item.data(), best_item.size());
+      b.unit_start = upload_array(p, best_start.data(), best_start.size());
+      b.unit_points = best_points;
+    }
+    ti.param0 = b.unit_points;
+    ti.param1 = b.lds_kb;
+    ti.best_us = best;
+  }
 }
 
 void launch_cell_cloud_diagonal(const gmf_plan&, const Bucket& b, double* diag, bool reduced, hipStream_t stream) {

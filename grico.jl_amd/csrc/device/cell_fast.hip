@@ -179,10 +179,13 @@ struct CellArgs {
   const R* lcoef;       // cells x N^3 or null
 };
 
-template <int N, bool COEF, class R>
-__global__ void __launch_bounds__(256) cell_apply_fast(CellArgs<R> a, EOTables<N, R> T, const R* __restrict__ x,
-                                                       R* __restrict__ y) {
-  constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>(), CPB = 256 / LPC;
+// THREADS (64, 128 or 256 lanes per workgroup, at least one cell) is picked
+// per bucket by the plan-time tuner.
+template <int N, bool COEF, class R, int THREADS>
+__global__ void __launch_bounds__(THREADS) cell_apply_fast(CellArgs<R> a, EOTables<N, R> T, const R* __restrict__ x,
+                                                           R* __restrict__ y) {
+  constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>(), CPB = THREADS / LPC;
+  static_assert(CPB >= 1, "a workgroup holds at least one cell");
   __shared__ R S1[CPB * NNN];
   __shared__ R S2[CPB * NNN];
   const int lc = threadIdx.x / LPC;
@@ -421,9 +424,9 @@ struct CellArrays32 {
 template <class R>
 using CellArrays = typename std::conditional<std::is_same<R, float>::value, CellArrays32, CellArrays64>::type;
 
-template <int N, class R>
-void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, hipStream_t stream) {
-  constexpr int CPB = 256 / lanes_per_cell<N>();
+template <int N, class R, int THREADS>
+void launch_cells_shape(const Bucket& b, const R* x, R* y, bool reduced, hipStream_t stream) {
+  constexpr int CPB = THREADS / lanes_per_cell<N>();
   EOTables<N, R> T = tables_of<N, R>(b);
   CellArgs<R> a;
   a.cells = b.cells;
@@ -433,10 +436,25 @@ void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, hipStream_t s
   a.lcoef = CellArrays<R>::lcoef(b);
   const int64_t blocks = (b.cells + CPB - 1) / CPB;
   if (b.lcoef)
-    hipLaunchKernelGGL((cell_apply_fast<N, true, R>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, true, R, THREADS>), dim3(unsigned(blocks)), dim3(THREADS), 0, stream, a,
+                       T, x, y);
   else
-    hipLaunchKernelGGL((cell_apply_fast<N, false, R>), dim3(unsigned(blocks)), dim3(256), 0, stream, a, T, x, y);
+    hipLaunchKernelGGL((cell_apply_fast<N, false, R, THREADS>), dim3(unsigned(blocks)), dim3(THREADS), 0, stream, a,
+                       T, x, y);
   GMF_HIP(hipGetLastError());
+}
+
+// Per-cell launch with `threads` lanes per workgroup (64, 128, 256; raised to
+// the lanes of one cell where a cell needs more).
+template <int N, class R>
+void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, int threads, hipStream_t stream) {
+  constexpr int LPC = lanes_per_cell<N>();
+  if (threads <= 64 && LPC <= 64)
+    launch_cells_shape<N, R, 64>(b, x, y, reduced, stream);
+  else if (threads <= 128)
+    launch_cells_shape<N, R, 128>(b, x, y, reduced, stream);
+  else
+    launch_cells_shape<N, R, 256>(b, x, y, reduced, stream);
 }
 
 template <int N, class R>
@@ -621,13 +639,11 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   std::vector<double> G(b.h_derivs.begin(), b.h_derivs.begin() + n * n);
   b.h_colloc = collocation_derivative(B, G, n);
   b.fast = true;
-  // Batches below 32768 cells (coarse multigrid levels) take the one-launch
-  // per-cell kernel; larger ones the brick kernel (also for p = 1: V-cycles of
-  // the 32^3 p = 3 and 64^3 p = 4 Poisson hierarchies measured 3-6% faster
-  // with bricks on their 32768- and 262144-cell p = 1 levels, which can then
-  // also fuse their Chebyshev steps). GMF_BRICK_MIN_CELLS overrides the
-  // threshold, GMF_BRICK_MIN_CELLS_N<n> for batches of n modes per axis only.
-  int64_t min_cells = 32768;
+  // Batches of at least 4096 cells get the brick decomposition; the plan-time
+  // tuner (tune_fast) then keeps it unless the per-cell kernel is clearly
+  // faster on the batch. GMF_BRICK_MIN_CELLS overrides the threshold,
+  // GMF_BRICK_MIN_CELLS_N<n> for batches of n modes per axis only.
+  int64_t min_cells = 4096;
   if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
   const std::string per_n = "GMF_BRICK_MIN_CELLS_N" + std::to_string(n);
   if (const char* env = std::getenv(per_n.c_str())) min_cells = std::strtoll(env, nullptr, 10);
@@ -635,6 +651,20 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
 }
 
 namespace {
+
+template <class R>
+bool launch_cells_n(const Bucket& b, const R* x, R* y, bool reduced, int threads, hipStream_t stream) {
+  switch (b.N[0]) {
+    case 2: launch_cells<2, R>(b, x, y, reduced, threads, stream); return true;
+    case 3: launch_cells<3, R>(b, x, y, reduced, threads, stream); return true;
+    case 4: launch_cells<4, R>(b, x, y, reduced, threads, stream); return true;
+    case 5: launch_cells<5, R>(b, x, y, reduced, threads, stream); return true;
+    case 6: launch_cells<6, R>(b, x, y, reduced, threads, stream); return true;
+    case 7: launch_cells<7, R>(b, x, y, reduced, threads, stream); return true;
+    case 8: launch_cells<8, R>(b, x, y, reduced, threads, stream); return true;
+    default: return false;
+  }
+}
 
 template <class R>
 bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool reduced, R* slots, hipStream_t stream) {
@@ -651,16 +681,7 @@ bool launch_fast(const gmf_plan& p, const Bucket& b, const R* x, R* y, bool redu
       default: break;
     }
   }
-  switch (b.N[0]) {
-    case 2: launch_cells<2, R>(b, x, y, reduced, stream); return true;
-    case 3: launch_cells<3, R>(b, x, y, reduced, stream); return true;
-    case 4: launch_cells<4, R>(b, x, y, reduced, stream); return true;
-    case 5: launch_cells<5, R>(b, x, y, reduced, stream); return true;
-    case 6: launch_cells<6, R>(b, x, y, reduced, stream); return true;
-    case 7: launch_cells<7, R>(b, x, y, reduced, stream); return true;
-    case 8: launch_cells<8, R>(b, x, y, reduced, stream); return true;
-    default: return false;
-  }
+  return launch_cells_n<R>(b, x, y, reduced, b.cell_threads, stream);
 }
 
 template <class R>
@@ -698,6 +719,55 @@ float* float_copy(gmf_plan& p, const double* d, size_t n) {
 }
 
 }  // namespace
+
+// Bricks stay unless the best per-cell shape beats them by this factor: a
+// brick batch also runs the multigrid smoother steps fused into its stores.
+constexpr float kBrickKeepFactor = 0.9f;
+
+void tune_fast(gmf_plan& p, TuneTimer& t) {
+  int forced = 0;
+  if (const char* env = std::getenv("GMF_CELL_THREADS")) forced = std::atoi(env);
+  // GMF_KEEP_BRICKS=1: brick batches stay on the brick kernel.
+  const char* keep_env = std::getenv("GMF_KEEP_BRICKS");
+  const bool keep_bricks = keep_env && std::string(keep_env) == "1";
+  for (Bucket& b : p.buckets) {
+    if (!b.fast || b.cloud || b.cells == 0) continue;
+    TuneInfo& ti = b.tune;
+    float brick_us = -1.0f;
+    if (b.bricks.enabled) {
+      const StoreResult<double> st{t.y, 0};
+      brick_us = t.time_us([&] {
+        switch (b.N[0]) {
+          case 2: launch_bricks<2, double>(b, t.x, false, t.slots, nullptr, st, false); break;
+          case 3: launch_bricks<3, double>(b, t.x, false, t.slots, nullptr, st, false); break;
+          case 4: launch_bricks<4, double>(b, t.x, false, t.slots, nullptr, st, false); break;
+          case 5: launch_bricks<5, double>(b, t.x, false, t.slots, nullptr, st, false); break;
+          default: launch_bricks<6, double>(b, t.x, false, t.slots, nullptr, st, false); break;
+        }
+      });
+      ++ti.candidates;
+    }
+    float cell_us = -1.0f;
+    int best_threads = 256;
+    for (int threads : {64, 128, 256}) {
+      if (forced && threads != forced) continue;
+      const float us = t.time_us([&] { launch_cells_n<double>(b, t.x, t.y, false, threads, nullptr); });
+      ++ti.candidates;
+      if (threads == 256) ti.default_us = us;
+      if (cell_us < 0.0f || us < cell_us) {
+        cell_us = us;
+        best_threads = threads;
+      }
+    }
+    if (b.bricks.enabled) ti.default_us = brick_us;
+    b.cell_threads = best_threads;
+    if (b.bricks.enabled && !keep_bricks && cell_us >= 0.0f && cell_us < kBrickKeepFactor * brick_us)
+      b.bricks.enabled = false;
+    ti.kernel = b.bricks.enabled ? kTuneBrick : kTuneCell;
+    ti.param0 = b.bricks.enabled ? 0 : b.cell_threads;
+    ti.best_us = b.bricks.enabled ? brick_us : cell_us;
+  }
+}
 
 bool launch_cell_fast(const gmf_plan& p, const Bucket& b, const double* x, double* y, bool reduced, double* slots,
                       hipStream_t stream) {

@@ -447,41 +447,30 @@ void launch_cell_generic(const gmf_plan&, const Bucket& b, const double* x, doub
   launch_generic_threads(b, x, y, reduced, b.generic_threads, stream);
 }
 
-void tune_generic(gmf_plan& p) {
+void tune_generic(gmf_plan& p, TuneTimer& t) {
   int forced = 0;
   if (const char* env = std::getenv("GMF_GENERIC_THREADS")) forced = std::atoi(env);
-  std::vector<Bucket*> todo;
-  for (Bucket& b : p.buckets)
-    if (!b.cloud && !b.fast && b.cells > 0) todo.push_back(&b);
-  if (todo.empty()) return;
-  if (forced == 64 || forced == 128 || forced == 256) {
-    for (Bucket* b : todo) b->generic_threads = forced;
-    return;
-  }
-  DeviceBuffer<double> x(size_t(std::max<int64_t>(p.n_full, 1))), y(size_t(std::max<int64_t>(p.n_full, 1)));
-  fill(x.p, 1.0, p.n_full, nullptr);
-  fill(y.p, 0.0, p.n_full, nullptr);
-  hipEvent_t e0, e1;
-  GMF_HIP(hipEventCreate(&e0));
-  GMF_HIP(hipEventCreate(&e1));
-  for (Bucket* b : todo) {
+  for (Bucket& b : p.buckets) {
+    if (b.cloud || b.fast || b.cells == 0) continue;
+    b.tune.kernel = kTuneGeneric;
+    if (forced == 64 || forced == 128 || forced == 256) {
+      b.generic_threads = forced;
+      b.tune.param0 = forced;
+      continue;
+    }
     float best = -1.0f;
     for (int threads : {64, 128, 256}) {
-      launch_generic_threads(*b, x.p, y.p, false, threads, nullptr);  // warm-up
-      GMF_HIP(hipEventRecord(e0, nullptr));
-      for (int rep = 0; rep < 3; ++rep) launch_generic_threads(*b, x.p, y.p, false, threads, nullptr);
-      GMF_HIP(hipEventRecord(e1, nullptr));
-      GMF_HIP(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      GMF_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (best < 0.0f || ms < best) {
-        best = ms;
-        b->generic_threads = threads;
+      const float us = t.time_us([&] { launch_generic_threads(b, t.x, t.y, false, threads, nullptr); });
+      if (threads == 64) b.tune.default_us = us;
+      ++b.tune.candidates;
+      if (best < 0.0f || us < best) {
+        best = us;
+        b.generic_threads = threads;
       }
     }
+    b.tune.param0 = b.generic_threads;
+    b.tune.best_us = best;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
 }
 
 void prepare_generic_batch(gmf_plan& p) {

@@ -78,7 +78,18 @@ struct FacePath {
 };
 
 // Device-resident cell batch.
-constexpr int kCloudUnitPoints = 512;
+constexpr int kCloudUnitPoints = 512;  // default points per work unit of the point-cloud kernels
+
+// Kernel family and launch shape a bucket runs with, as picked by the
+// plan-time tuner (tune.hip); mirrors gmf_bucket_tuning.
+enum TuneKernel { kTuneNone = 0, kTuneBrick = 1, kTuneCell = 2, kTuneGeneric = 3, kTuneCloud = 4, kTuneFace = 5 };
+struct TuneInfo {
+  int kernel = kTuneNone;
+  int param0 = 0, param1 = 0;  // brick: -, -; cell / generic: lanes per workgroup / cell; cloud: unit points, LDS KB
+  int candidates = 0;          // shapes timed (0: not tuned)
+  float best_us = 0.0f;        // time per launch of the chosen shape
+  float default_us = 0.0f;     // time per launch of the default shape
+};
 
 struct Bucket {
   int dim = 0;
@@ -107,8 +118,11 @@ struct Bucket {
   std::vector<double> h_colloc;
   bool fast = false;
   BrickPath bricks;
-  // Lanes per cell of the generic kernel, picked by tune_generic at plan creation.
+  // Lanes per cell of the generic kernel and lanes per workgroup of the
+  // per-cell fast kernel, picked by the plan-time tuner.
   int generic_threads = 64;
+  int cell_threads = 256;
+  TuneInfo tune;
   // FP32 copies of the per-cell arrays for FP32 multigrid levels (prepare_fp32).
   bool fp32 = false;
   float* jinv32 = nullptr;
@@ -134,6 +148,9 @@ struct Bucket {
   int64_t units = 0;
   int32_t* unit_item = nullptr;
   int64_t* unit_start = nullptr;
+  int unit_points = kCloudUnitPoints;  // points per work unit (tuned)
+  int lds_kb = 64;                     // LDS budget of one workgroup (tuned)
+  std::vector<int64_t> h_poff;         // host copy of poff (work-unit rebuilds)
   // Exact reduced diagonal on cells whose modes couple through multi-term
   // expansions or repeat a dof: `coupled` flags those cells (null if none) and
   // each (cell, reduced dof) pair lists the dof's local coefficient vector.
@@ -228,10 +245,38 @@ void launch_cell_l2(const Bucket& b, const double* u, const double* w, const dou
 void launch_cell_interp(const Bucket& b, const double* u, int channels, double* out, hipStream_t stream);
 // out[0..1] = column sums of partial (cells x 2) in a fixed order.
 void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream);
-// Autotune the generic kernel: for every generic-kernel bucket time 64, 128
-// and 256 lanes per cell on scratch vectors and keep the fastest
-// (GMF_GENERIC_THREADS=64|128|256 forces one width).
-void tune_generic(gmf_plan& plan);
+// Plan-time autotuner (tune.hip): at plan creation every bucket's kernel
+// launch shape is timed over its candidates on scratch vectors (HIP events,
+// one warm-up and `reps` timed launches each) and the fastest is kept.
+struct TuneTimer {
+  double* x = nullptr;  // scratch input (n_full, filled with 1)
+  double* y = nullptr;  // scratch output (n_full)
+  double* slots = nullptr;  // scratch brick halo slots (plan.brick_slots)
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int reps = 3;
+  template <class F>
+  float time_us(F&& launch) {
+    launch();
+    GMF_HIP(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) launch();
+    GMF_HIP(hipEventRecord(e1, nullptr));
+    GMF_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    GMF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return 1000.0f * ms / float(reps);
+  }
+};
+// Tune every bucket of the plan (GMF_TUNE=0: keep the default shapes).
+void tune_plan(gmf_plan& plan);
+// Per-family tuners: generic kernel lanes per cell (64, 128, 256;
+// GMF_GENERIC_THREADS forces one), fast kernels (brick vs per-cell and the
+// per-cell workgroup size), point-cloud work-unit size and LDS budget.
+void tune_generic(gmf_plan& plan, TuneTimer& t);
+void tune_fast(gmf_plan& plan, TuneTimer& t);
+void tune_cloud(gmf_plan& plan, TuneTimer& t);
+// Work units of a point-cloud bucket at `unit_points` points per unit into
+// device arrays owned by the caller (item, first point); returns the count.
+int64_t cloud_units(const Bucket& b, int unit_points, std::vector<int32_t>& item, std::vector<int64_t>& start);
 // Batch the plan's generic-kernel buckets into one launch per lanes-per-cell
 // tier (tiers holding at least two buckets).
 void prepare_generic_batch(gmf_plan& plan);

@@ -729,15 +729,11 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
                     "reference points must lie in the biunit cell");
           b.poff = upload(p, po, size_t(b.cells) + 1);
           b.rpts = upload(p, bd.ref_points, size_t(b.total_points) * size_t(b.sides) * size_t(b.dim));
+          require(b.cells <= INT32_MAX, GMF_EINVAL, "too many point-cloud items in one batch");
+          b.h_poff.assign(po, po + b.cells + 1);
           std::vector<int32_t> uitem;
           std::vector<int64_t> ustart;
-          for (int64_t c = 0; c < b.cells; ++c)
-            for (int64_t s0 = po[c]; s0 == po[c] || s0 < po[c + 1]; s0 += kCloudUnitPoints) {
-              uitem.push_back(int32_t(c));
-              ustart.push_back(s0);
-            }
-          require(b.cells <= INT32_MAX, GMF_EINVAL, "too many point-cloud items in one batch");
-          b.units = int64_t(uitem.size());
+          b.units = cloud_units(b, b.unit_points, uitem, ustart);
           b.unit_item = upload(p, uitem.data(), uitem.size());
           b.unit_start = upload(p, ustart.data(), ustart.size());
         } else {
@@ -795,7 +791,7 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
         p.buckets.push_back(std::move(b));
       }
       GMF_HIP(hipDeviceSynchronize());
-      tune_generic(p);
+      tune_plan(p);
       prepare_generic_batch(p);
       prepare_face_batch(p);
       GMF_HIP(hipDeviceSynchronize());

@@ -51,6 +51,15 @@ class SolveInfo(ctypes.Structure):
     ]
 
 
+class BucketTuning(ctypes.Structure):
+    _fields_ = [("kernel", c_int32), ("param0", c_int32), ("param1", c_int32), ("candidates", c_int32),
+                ("best_us", c_double), ("default_us", c_double)]
+
+
+# gmf_bucket_tuning.kernel
+TUNED_KERNELS = {0: "empty", 1: "brick", 2: "cell", 3: "generic", 4: "point-cloud", 5: "face-tensor"}
+
+
 class TransferDesc(ctypes.Structure):
     _fields_ = [("nnz", c_int64), ("coarse_indices", P_i64), ("fine_indices", P_i64), ("coefficients", P_f64)]
 
@@ -112,7 +121,8 @@ def declare(lib):
     lib.gmf_cg_solve_block.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_double,
                                        c_double, c_int64, ctypes.POINTER(SolveInfo)]
     lib.gmf_precond_create_jacobi_f32.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
-    for name in ("gmf_plan_create", "gmf_plan_destroy", "gmf_plan_sizes", "gmf_workspace_create",
+    lib.gmf_plan_tuning.argtypes = [c_void_p, c_int32, ctypes.POINTER(BucketTuning)]
+    for name in ("gmf_plan_tuning", "gmf_plan_create", "gmf_plan_destroy", "gmf_plan_sizes", "gmf_workspace_create",
                  "gmf_workspace_destroy", "gmf_workspace_set_stream", "gmf_apply", "gmf_apply_async",
                  "gmf_apply_physical", "gmf_reduced_apply", "gmf_reduced_apply_f32", "gmf_project_reduced",
                  "gmf_expand_reduced",
@@ -222,6 +232,7 @@ class DevicePlan:
         handle = c_void_p()
         check(lib.gmf_plan_create(ctypes.byref(pd), ctypes.byref(handle)))
         self.handle = handle
+        self.bucket_count = len(buckets)
         ws = c_void_p()
         check(lib.gmf_workspace_create(handle, ctypes.byref(ws)))
         self.ws = ws
@@ -248,6 +259,18 @@ class DevicePlan:
         buf = ctypes.create_string_buffer(1 << 16)
         check(_native.device().gmf_plan_describe(self.handle, buf, len(buf)))
         return buf.value.decode()
+
+    def tuning(self):
+        """Kernel family and launch shape of every device bucket as picked by
+        the plan-time tuner (gmf_plan_tuning), in bucket order."""
+        out = []
+        lib = _native.device()
+        for i in range(self.bucket_count):
+            t = BucketTuning()
+            check(lib.gmf_plan_tuning(self.handle, i, ctypes.byref(t)))
+            out.append({"kernel": TUNED_KERNELS.get(t.kernel, t.kernel), "param0": t.param0, "param1": t.param1,
+                        "candidates": t.candidates, "best_us": t.best_us, "default_us": t.default_us})
+        return out
 
     def device_bytes(self):
         a, b, c = c_int64(), c_int64(), c_int64()

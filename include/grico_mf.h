@@ -163,6 +163,24 @@ gmf_status gmf_plan_sizes(const gmf_plan* plan, int64_t* full_dof_count, int64_t
  * counterpart). */
 gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity);
 
+/* Kernel family and launch shape one bucket runs with, picked at plan
+ * creation by timing the candidates (no reference counterpart: the reference
+ * runs every batch through one CPU loop). kernel: 0 empty bucket, 1 brick
+ * kernel, 2 per-cell tensor kernel (param0 = lanes per workgroup), 3 generic
+ * tensor kernel (param0 = lanes per cell), 4 point-cloud kernel (param0 =
+ * points per work unit, param1 = LDS budget in KB), 5 face-tensor kernel.
+ * candidates = shapes timed (0: defaults kept, GMF_TUNE=0); best_us /
+ * default_us = time per launch of the chosen / the default shape. */
+typedef struct gmf_bucket_tuning {
+  int32_t kernel;
+  int32_t param0;
+  int32_t param1;
+  int32_t candidates;
+  double best_us;
+  double default_us;
+} gmf_bucket_tuning;
+gmf_status gmf_plan_tuning(const gmf_plan* plan, int32_t bucket, gmf_bucket_tuning* out);
+
 /* Workspace: per-caller scratch tied to one plan (src/assembly.jl:388-441,
  * 566-584 OperatorWorkspace / _ReducedOperatorWorkspace). `stream` is a
  * hipStream_t (NULL = null stream). */

@@ -5,9 +5,11 @@ import sys
 import numpy as np
 import pytest
 
-# Keep the brick path under test on small meshes (the library routes batches
-# below 32768 cells to the per-cell kernel by default).
+# Keep the brick path under test on small meshes: bricks for batches of any
+# size (the library builds them from 4096 cells), and the plan-time tuner may
+# not trade them for the per-cell kernel.
 os.environ.setdefault("GMF_BRICK_MIN_CELLS", "0")
+os.environ.setdefault("GMF_KEEP_BRICKS", "1")
 
 # Parallel test workers (pytest -n): one BLAS thread each, or every worker's
 # BLAS pool spins on all cores and the small dense solves of the tests crawl.

@@ -24,6 +24,7 @@ PAIRS = {
     "TransferDesc": ("gmf_transfer_desc", _devabi.TransferDesc),
     "MgDesc": ("gmf_mg_desc", _devabi.MgDesc),
     "MlpDesc": ("gmf_mlp_desc", learned.MlpDesc),
+    "BucketTuning": ("gmf_bucket_tuning", _devabi.BucketTuning),
 }
 
 C_SCALARS = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64", "float": "Float32"}
