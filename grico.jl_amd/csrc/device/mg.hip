@@ -23,6 +23,7 @@
 #include "blas.hpp"
 #include <rocsolver/rocsolver.h>
 
+#include "blocks.hpp"
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -193,7 +194,8 @@ template <class R>
 struct Level {
   gmf_workspace* ws = nullptr;
   int64_t n = 0;
-  DeviceBuffer<R> invdiag, x, b, q, r, d;
+  DeviceBuffer<R> invdiag, x, b, q, r, d;  // invdiag: zero on block dofs when the level has blocks
+  BlockSet<R> blocks;                      // cell blocks of the block smoother (count 0: point smoothing)
   DeviceBuffer<double> gV, gZ, gy;  // GMRES smoother Krylov basis, preconditioned basis, coefficients
   double lambda_max = 0.0;
 };
@@ -410,10 +412,13 @@ struct MGPrecond : MGBase {
       for (int k = 0; k < steps; ++k) {
         if (x_zero && k == 0) {
           hipLaunchKernelGGL(k_jacobi_first<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.invdiag.p, R(omega), n);
+          block_add<R>(L.blocks, x, nullptr, b, nullptr, R(omega), s);
         } else {
           level_apply(L.ws, x, L.q.p);
           hipLaunchKernelGGL(k_jacobi_update<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.q.p, L.invdiag.p,
                              R(omega), n);
+          // The block part uses the same residual b - A x_old (q is unchanged).
+          block_add<R>(L.blocks, x, nullptr, b, L.q.p, R(omega), s);
         }
         GMF_HIP(hipGetLastError());
       }
@@ -422,6 +427,8 @@ struct MGPrecond : MGBase {
     const double lmax = L.lambda_max, lmin = L.lambda_max / cheb_range;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
+    // With blocks, each start / step kernel forms the point part of S r and
+    // block_add then adds the blocks' part to d and x (r is final by then).
     for (int k = 0; k < steps; ++k) {
       if (x_zero && k == 0) {
         hipLaunchKernelGGL(k_cheb_start0<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
@@ -431,6 +438,7 @@ struct MGPrecond : MGBase {
         hipLaunchKernelGGL(k_cheb_start<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p,
                            L.invdiag.p, R(1.0 / theta), n);
       }
+      block_add<R>(L.blocks, x, L.d.p, L.r.p, nullptr, R(1.0 / theta), s);
       double rho = 1.0 / sigma;
       for (int i = 1; i < cheb_degree; ++i) {
         const double rho_new = 1.0 / (2.0 * sigma - rho);
@@ -440,6 +448,7 @@ struct MGPrecond : MGBase {
           hipLaunchKernelGGL(k_cheb_step<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
                              c1, c2, n);
         }
+        block_add<R>(L.blocks, x, L.d.p, L.r.p, nullptr, c2, s);
         rho = rho_new;
       }
       GMF_HIP(hipGetLastError());
@@ -598,6 +607,55 @@ double estimate_lambda_max(gmf_workspace* ws, int64_t n, const double* invdiag, 
   beta.resize(alpha.size() > 0 ? alpha.size() - 1 : 0);
   sync(ws);
   return tridiagonal_max_eigenvalue(alpha, beta);
+}
+
+// lambda_max(S A) for the block smoother S = D_u^-1 + sum_b R_b^T B_b^-1 R_b
+// from the Lanczos tridiagonal of `iterations` preconditioned CG steps
+// (T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1}, T_j,j+1 = sqrt(beta_j)/alpha_j).
+// FP64 throughout (invdiag zero on block dofs).
+double estimate_lambda_max_blocks(gmf_workspace* ws, int64_t n, const double* invdiag, const BlockSet<double>& bs,
+                                  int iterations) {
+  hipStream_t s = ws->stream;
+  const int m = int(std::min<int64_t>(iterations, n));
+  if (m <= 0) return 0.0;
+  DeviceBuffer<double> r(static_cast<size_t>(n)), z(static_cast<size_t>(n)), p(static_cast<size_t>(n)),
+      q(static_cast<size_t>(n));
+  auto precondition = [&] {
+    hipLaunchKernelGGL(k_mul, dim3(grid1(n)), dim3(256), 0, s, z.p, invdiag, r.p, n);
+    block_add<double>(bs, z.p, nullptr, r.p, nullptr, 1.0, s);
+  };
+  hipLaunchKernelGGL(k_start_vector, dim3(grid1(n)), dim3(256), 0, s, r.p, n);
+  precondition();
+  copy(p.p, z.p, n, s);
+  dot(ws, r.p, z.p, n, kTmp);
+  read_scalars(ws, kTmp + 1);
+  double rz = ws->host_scalars[kTmp];
+  require(rz > 0.0 && std::isfinite(rz), GMF_ENOTSPD, "block smoother is not positive definite");
+  std::vector<double> alpha, beta, diag, off;
+  for (int j = 0; j < m; ++j) {
+    reduced_apply(ws, p.p, q.p);
+    dot(ws, p.p, q.p, n, kTmp);
+    read_scalars(ws, kTmp + 1);
+    const double pq = ws->host_scalars[kTmp];
+    require(pq > 0.0 && std::isfinite(pq), GMF_ENOTSPD, "Chebyshev smoother needs a positive spectrum of S A");
+    const double a = rz / pq;
+    diag.push_back(1.0 / a + (j > 0 ? beta.back() / alpha.back() : 0.0));
+    alpha.push_back(a);
+    axpby(ws, r.p, -a, q.p, 1.0, n);
+    precondition();
+    dot(ws, r.p, z.p, n, kTmp);
+    read_scalars(ws, kTmp + 1);
+    const double rz_new = ws->host_scalars[kTmp];
+    if (j + 1 == m || !(rz_new > 1e-28 * rz)) break;
+    const double bt = rz_new / rz;
+    off.push_back(std::sqrt(bt) / a);
+    beta.push_back(bt);
+    axpby(ws, p.p, 1.0, z.p, bt, n);  // p = z + beta p
+    rz = rz_new;
+  }
+  off.resize(diag.size() > 0 ? diag.size() - 1 : 0);
+  sync(ws);
+  return tridiagonal_max_eigenvalue(diag, off);
 }
 
 // Inverse of a general dense matrix by LU with partial pivoting (the
@@ -818,8 +876,28 @@ std::unique_ptr<MGBase> build_mg(const gmf_mg_desc* desc) {
       }
       invert_diagonal(lv.ws, d.p, inv64.p, lv.n);
     }
+    const gmf_block_desc* bd = (desc->level_blocks && l > 0) ? &desc->level_blocks[l] : nullptr;
+    BlockSet<double> blocks64;
+    if (bd != nullptr && bd->count > 0) {
+      require(!gmres, GMF_EUNSUPPORTED, "block smoothing runs with the Jacobi and Chebyshev smoothers");
+      require(bd->inverse != nullptr, GMF_EINVAL, "block inverses missing");
+      blocks64 = upload_blocks<double>(*bd, lv.n);
+      const int64_t nd = bd->offsets[bd->count];
+      hipLaunchKernelGGL(k_zero_at<double>, dim3(grid1(nd)), dim3(256), 0, lv.ws->stream, inv64.p, blocks64.dofs.p,
+                         nd);
+      GMF_HIP(hipGetLastError());
+      if constexpr (MGPrecond<R>::kDouble)
+        lv.blocks = std::move(blocks64);
+      else
+        lv.blocks = upload_blocks<R>(*bd, lv.n);
+    }
     if (l > 0 && cheb && lv.n > 0) {
-      lv.lambda_max = 1.2 * estimate_lambda_max(lv.ws, lv.n, inv64.p, desc->eigen_iterations);
+      const BlockSet<double>* b64 = &blocks64;
+      if constexpr (MGPrecond<R>::kDouble) b64 = &lv.blocks;
+      if (bd != nullptr && bd->count > 0)
+        lv.lambda_max = 1.2 * estimate_lambda_max_blocks(lv.ws, lv.n, inv64.p, *b64, desc->eigen_iterations);
+      else
+        lv.lambda_max = 1.2 * estimate_lambda_max(lv.ws, lv.n, inv64.p, desc->eigen_iterations);
       require(std::isfinite(lv.lambda_max) && lv.lambda_max > 0.0, GMF_ENOTSPD,
               "Chebyshev smoother needs a positive spectrum of D^-1 A");
     }

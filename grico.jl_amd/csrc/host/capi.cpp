@@ -213,4 +213,57 @@ int gh_transfer_export(const gh_transfer* transfer, int64_t* coarse_indices, int
   });
 }
 
+int gh_conflict_coloring(int64_t block_count, const int64_t* block_ptr, const int64_t* block_dofs,
+                         int64_t item_count, const int64_t* item_ptr, const int64_t* item_dofs, int64_t dof_count,
+                         int32_t* colors, int32_t* color_count) {
+  return guarded([&] {
+    if (block_count < 0 || item_count < 0 || dof_count < 0 || !colors || !color_count ||
+        (block_count > 0 && (!block_ptr || !block_dofs)) || (item_count > 0 && (!item_ptr || !item_dofs)))
+      throw std::invalid_argument("invalid coloring arguments");
+    // dof -> items touching it (CSR)
+    std::vector<int64_t> dptr(size_t(dof_count) + 1, 0);
+    for (int64_t i = 0; i < item_count; ++i)
+      for (int64_t t = item_ptr[i]; t < item_ptr[i + 1]; ++t) {
+        const int64_t d = item_dofs[t];
+        if (d < 0 || d >= dof_count) throw std::invalid_argument("item dof out of range");
+        ++dptr[size_t(d) + 1];
+      }
+    for (int64_t d = 0; d < dof_count; ++d) dptr[size_t(d) + 1] += dptr[size_t(d)];
+    std::vector<int64_t> ditems(size_t(dptr.back())), fillp(dptr.begin(), dptr.end() - 1);
+    for (int64_t i = 0; i < item_count; ++i)
+      for (int64_t t = item_ptr[i]; t < item_ptr[i + 1]; ++t) ditems[size_t(fillp[size_t(item_dofs[t])]++)] = i;
+    // item -> blocks colored so far that touch it
+    std::vector<std::vector<int32_t>> item_blocks(static_cast<size_t>(item_count));
+    std::vector<int64_t> stamp(static_cast<size_t>(item_count), -1);
+    std::vector<int64_t> used;  // color -> last block that saw it forbidden
+    int32_t ncol = 0;
+    std::vector<int64_t> touched;
+    for (int64_t b = 0; b < block_count; ++b) {
+      touched.clear();
+      for (int64_t t = block_ptr[b]; t < block_ptr[b + 1]; ++t) {
+        const int64_t d = block_dofs[t];
+        if (d < 0 || d >= dof_count) throw std::invalid_argument("block dof out of range");
+        for (int64_t k = dptr[size_t(d)]; k < dptr[size_t(d) + 1]; ++k) {
+          const int64_t i = ditems[size_t(k)];
+          if (stamp[size_t(i)] != b) {
+            stamp[size_t(i)] = b;
+            touched.push_back(i);
+          }
+        }
+      }
+      for (int64_t i : touched)
+        for (int32_t ob : item_blocks[size_t(i)]) used[size_t(colors[ob])] = b;
+      int32_t c = 0;
+      while (c < ncol && used[size_t(c)] == b) ++c;
+      if (c == ncol) {
+        ++ncol;
+        used.push_back(-1);
+      }
+      colors[b] = c;
+      for (int64_t i : touched) item_blocks[size_t(i)].push_back(int32_t(b));
+    }
+    *color_count = ncol;
+  });
+}
+
 }  // extern "C"

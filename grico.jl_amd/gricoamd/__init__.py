@@ -27,6 +27,7 @@ from .solve import (AutoLinearSolver, CGSolver, ConvergenceError, FGMRESSolver, 
                     solve_reduced)
 from .verification import device_l2_error, device_relative_l2_error, l2_error, relative_l2_error
 from .multigrid import GeometricMultigridPreconditioner, MultigridHierarchy, compile_geometric_multigrid
+from .embedded import FiniteCellExtension, ImplicitRegion, PhysicalDomain, PhysicalMeasure, SampledSurface
 from .adaptivity import (AdaptivityLimits, AdaptivityPlan, SpaceTransition, evaluate, interface_jump_indicators,
                          coefficient_coarsening_indicators, transfer_state,
                          transition)

@@ -64,6 +64,10 @@ class TransferDesc(ctypes.Structure):
     _fields_ = [("nnz", c_int64), ("coarse_indices", P_i64), ("fine_indices", P_i64), ("coefficients", P_f64)]
 
 
+class BlockDesc(ctypes.Structure):
+    _fields_ = [("count", c_int64), ("offsets", P_i64), ("dofs", P_i32), ("inverse", P_f64)]
+
+
 class MgDesc(ctypes.Structure):
     _fields_ = [
         ("level_count", c_int32), ("levels", ctypes.POINTER(c_void_p)), ("transfers", ctypes.POINTER(TransferDesc)),
@@ -72,6 +76,7 @@ class MgDesc(ctypes.Structure):
         ("coarse_direct_limit", c_int64), ("coarse_rtol", c_double), ("coarse_atol", c_double),
         ("coarse_maxiter", c_int64), ("graph", c_int32), ("precision", c_int32),
         ("smoother_restart", c_int32), ("spd", c_int32), ("level_diagonals", ctypes.POINTER(P_f64)),
+        ("level_blocks", ctypes.POINTER(BlockDesc)),
     ]
 
 
@@ -82,6 +87,8 @@ def declare_mg(lib):
     lib.gmf_precond_create_mg.restype = ctypes.c_int
     lib.gmf_mg_level_info.argtypes = [c_void_p, c_int32, P_i64, P_f64]
     lib.gmf_mg_level_info.restype = ctypes.c_int
+    lib.gmf_reduced_blocks.argtypes = [c_void_p, ctypes.POINTER(BlockDesc), P_i32, c_int32, P_f64]
+    lib.gmf_reduced_blocks.restype = ctypes.c_int
     lib._mg_declared = True
 
 

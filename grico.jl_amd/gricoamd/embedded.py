@@ -340,6 +340,193 @@ def fitted_cut_cell_rules(domain, region, leaves, quadrature_shape, workers=None
     return {k: PointQuadrature(p, w) for k, (p, w) in zip(keys, fitted)}
 
 
+# ---- physical domains (src/regions.jl:40-184, 255-395) ---------------------------
+
+class PhysicalMeasure:
+    """Cut-cell integrals over the physical part of the cell only
+    (src/regions.jl:45-53)."""
+
+
+class FiniteCellExtension:
+    """Cut-cell measure (1 - alpha) Omega_phys + alpha Omega_full
+    (src/regions.jl:55-80): the fictitious part of a cut cell keeps weight
+    alpha, which bounds the conditioning of small cuts."""
+
+    def __init__(self, alpha):
+        alpha = float(alpha)
+        if not np.isfinite(alpha):
+            raise ValueError("finite-cell extension alpha must be finite")
+        if not 0.0 <= alpha <= 1.0:
+            raise ValueError("finite-cell extension alpha must lie in [0, 1]")
+        self.alpha = alpha
+
+
+class PhysicalDomain:
+    """A background Domain together with an implicit region and a cut-cell
+    measure (src/regions.jl:125-184). Spaces built on it drop the leaves
+    classified outside (255-300); compile() gives every cut leaf of the space
+    its moment-fitted rule for the leaf's quadrature shape (304-329, cached
+    per leaf and shape like the reference's signature cache) with the cell
+    measure applied (338-380). The background's geometry and refinement tree
+    are used as they are."""
+
+    def __init__(self, background, region, cell_measure=None):
+        if isinstance(background, PhysicalDomain):
+            raise TypeError("PhysicalDomain wraps a background Domain")
+        if not isinstance(region, ImplicitRegion):
+            raise TypeError("region must be an ImplicitRegion")
+        measure = PhysicalMeasure() if cell_measure is None else cell_measure
+        if not isinstance(measure, (PhysicalMeasure, FiniteCellExtension)):
+            raise TypeError("cell_measure must be PhysicalMeasure() or FiniteCellExtension(alpha)")
+        self.background = background
+        self.region = region
+        self.cell_measure = measure
+        self._fitted = {}  # (leaf, shape) -> fitted physical rule, or None
+
+    def __getattr__(self, name):
+        if name.startswith("__") or name == "background":
+            raise AttributeError(name)
+        return getattr(self.background, name)
+
+    def classify(self, leaves):
+        return classify_leaves(self.background, self.region, leaves)
+
+    def active_leaves(self):
+        leaves = np.asarray(self.background.active_leaves(), dtype=np.int64)
+        keep = leaves[self.classify(leaves) != OUTSIDE]
+        if len(keep) == 0:
+            raise ValueError("physical domains must contain at least one active physical leaf")
+        return keep
+
+    def physical_rules(self, leaves, shape, progress=None):
+        """Moment-fitted physical rules of cut `leaves` for one quadrature
+        shape (dict leaf -> PointQuadrature; leaves whose candidate rule is
+        empty are absent)."""
+        shape = tuple(int(q) for q in shape)
+        todo = [int(l) for l in leaves if (int(l), shape) not in self._fitted]
+        if todo:
+            got = fitted_cut_cell_rules(self.background, self.region, todo, shape, progress=progress)
+            for l in todo:
+                self._fitted[(l, shape)] = got.get(l)
+        return {int(l): self._fitted[(int(l), shape)] for l in leaves if self._fitted[(int(l), shape)] is not None}
+
+    def assembly_cell_quadratures(self, space, progress=None):
+        """leaf -> PointQuadrature of every cut leaf of `space` under the cell
+        measure (src/integration.jl:2406-2418, src/regions.jl:338-380)."""
+        measure = self.cell_measure
+        if isinstance(measure, FiniteCellExtension) and measure.alpha == 1.0:
+            return {}
+        cls = self.classify(space.leaves)
+        shapes = {}
+        for li in np.nonzero(cls == CUT)[0]:
+            shapes.setdefault(tuple(int(q) for q in space.quadrature_shapes[li]), []).append(int(space.leaves[li]))
+        out = {}
+        for shape, ls in shapes.items():
+            rules = self.physical_rules(ls, shape, progress=progress)
+            if isinstance(measure, FiniteCellExtension) and measure.alpha > 0.0:
+                fp, fw = _tensor_rule(shape)
+                a = measure.alpha
+                rules = {l: PointQuadrature(np.concatenate([r.points, fp], axis=0),
+                                            np.concatenate([(1.0 - a) * r.weights, a * fw]))
+                         for l, r in rules.items()}
+            out.update(rules)
+        return out
+
+
+class SampledSurface:
+    """An embedded surface given by physical quadrature points, weights and
+    outward normals, converted at compile time into SurfaceQuadrature pieces
+    on whichever leaves of the space contain the points (the reference's
+    EmbeddedSurface role, src/embedded.jl:140-187, for 3D surfaces the
+    reference only takes as explicit per-leaf pieces: the pieces then follow
+    the leaves of every multigrid level). Reference weights and normals use
+    the transforms of src/integration.jl:2501-2516."""
+
+    def __init__(self, points, weights, normals):
+        self.points = np.ascontiguousarray(points, dtype=np.float64)
+        self.weights = np.ascontiguousarray(weights, dtype=np.float64).reshape(-1)
+        self.normals = np.ascontiguousarray(normals, dtype=np.float64)
+        n = len(self.weights)
+        if self.points.ndim != 2 or self.points.shape[0] != n or self.normals.shape != self.points.shape:
+            raise ValueError("one point, weight and normal per surface sample is required")
+
+    def surface_quadratures(self, space):
+        dom = space.domain
+        D = space.dim
+        if self.points.shape[1] != D:
+            raise ValueError("surface dimension must match the space")
+        owner = locate_points(dom, space.leaves, self.points)
+        found = np.nonzero(owner >= 0)[0]
+        # pieces in the order of their first sample
+        first = {}
+        for p_ in found:
+            first.setdefault(int(owner[p_]), []).append(p_)
+        out = []
+        lowers = dom.cell_lowers(space.leaves)
+        halves = 0.5 * dom.cell_sizes(space.leaves)
+        for li, ps in first.items():
+            ps = np.asarray(ps)
+            lo, half = lowers[li], halves[li]
+            xi = np.clip((self.points[ps] - lo) / half - 1.0, -1.0, 1.0)
+            nr = self.normals[ps] * half
+            nr = nr / np.linalg.norm(nr, axis=1)[:, None]
+            scale = np.prod(half) / np.linalg.norm(half * nr, axis=1)
+            out.append(SurfaceQuadrature(int(space.leaves[li]), PointQuadrature(xi, self.weights[ps] / scale), nr))
+        return out
+
+
+def locate_points(domain, leaves, x):
+    """Index into `leaves` of the leaf containing each point of x (n, D), or -1:
+    leaves are grouped by size, and each group is looked up by the integer
+    coordinates of the points on that group's lattice."""
+    leaves = np.asarray(leaves, dtype=np.int64)
+    x = np.asarray(x, dtype=np.float64)
+    out = np.full(len(x), -1, dtype=np.int64)
+    if len(leaves) == 0 or len(x) == 0:
+        return out
+    origin = np.asarray(domain.origin, dtype=np.float64)
+    sizes = domain.cell_sizes(leaves)
+    lowers = domain.cell_lowers(leaves)
+    groups = {}
+    for i, h in enumerate(map(tuple, sizes)):
+        groups.setdefault(h, []).append(i)
+    for h, idx in groups.items():
+        idx = np.asarray(idx)
+        h = np.asarray(h)
+        k = np.rint((lowers[idx] - origin) / h).astype(np.int64)
+        ext = k.max(axis=0) + 2
+        code = np.zeros(len(idx), dtype=np.int64)
+        for a in range(len(h)):
+            code = code * ext[a] + k[:, a]
+        order = np.argsort(code)
+        codes = code[order]
+        kp = np.floor((x - origin) / h).astype(np.int64)
+        # a point on the upper face of the last leaf of a row belongs to that leaf
+        kp = np.minimum(kp, k.max(axis=0))
+        ok = np.all(kp >= 0, axis=1) & (out < 0)
+        pc = np.zeros(len(x), dtype=np.int64)
+        for a in range(len(h)):
+            pc = pc * ext[a] + np.clip(kp[:, a], 0, ext[a] - 1)
+        pos = np.searchsorted(codes, pc)
+        pos = np.minimum(pos, len(codes) - 1)
+        hit = ok & (codes[pos] == pc)
+        out[hit] = idx[order[pos[hit]]]
+    return out
+
+
+def sphere_surface(center, radius, n_theta=64, n_phi=128):
+    """Samples of the sphere |x - center| = radius: Gauss-Legendre in
+    cos(theta) times uniform in phi, outward normals."""
+    mu, wmu = gauss_legendre_rule(n_theta)
+    phi = (np.arange(n_phi) + 0.5) * (2.0 * np.pi / n_phi)
+    M, PH = np.meshgrid(mu, phi, indexing="ij")
+    W = np.outer(wmu, np.full(n_phi, 2.0 * np.pi / n_phi)) * radius ** 2
+    st = np.sqrt(1.0 - M ** 2)
+    nrm = np.stack([st * np.cos(PH), st * np.sin(PH), M], axis=-1).reshape(-1, 3)
+    x = np.asarray(center, dtype=np.float64)[None, :] + radius * nrm
+    return SampledSurface(x, W.reshape(-1), nrm)
+
+
 def physical_space(domain, region, options):
     """HpSpace on the leaves that are not outside the region
     (src/regions.jl:267-288), plus their classification."""
@@ -389,42 +576,7 @@ def add_cut_cell_quadratures(problem, region, cut_leaves, alpha=0.0, progress=No
 
 def sphere_surface_quadratures(space, center, radius, n_theta=64, n_phi=128):
     """SurfaceQuadrature pieces of the sphere |x - center| = radius on the
-    leaves of a space whose leaves all have the same size (uniform meshes);
-    outward normals."""
+    leaves of `space` (outward normals)."""
     if space.dim != 3:
         raise ValueError("the sphere rule is three-dimensional")
-    dom = space.domain
-    mu, wmu = gauss_legendre_rule(n_theta)
-    phi = (np.arange(n_phi) + 0.5) * (2.0 * np.pi / n_phi)
-    M, PH = np.meshgrid(mu, phi, indexing="ij")
-    W = np.outer(wmu, np.full(n_phi, 2.0 * np.pi / n_phi)) * radius ** 2
-    st = np.sqrt(1.0 - M ** 2)
-    nrm = np.stack([st * np.cos(PH), st * np.sin(PH), M], axis=-1).reshape(-1, 3)
-    x = np.asarray(center, dtype=np.float64)[None, :] + radius * nrm
-    w = W.reshape(-1)
-    sizes = dom.cell_sizes(space.leaves)
-    if not np.allclose(sizes, sizes[0]):
-        raise ValueError("sphere_surface_quadratures needs leaves of one size")
-    h = sizes[0]
-    lowers = dom.cell_lowers(space.leaves)
-    origin = np.asarray(dom.origin, dtype=np.float64)
-    key = {tuple(np.round((lo - origin) / h).astype(np.int64)): int(l) for lo, l in zip(lowers, space.leaves)}
-    ijk = np.floor((x - origin) / h).astype(np.int64)
-    groups = {}
-    for p in range(x.shape[0]):
-        leaf = key.get(tuple(ijk[p]))
-        if leaf is not None:
-            groups.setdefault(leaf, []).append(p)
-    out = []
-    half = 0.5 * h
-    for leaf, ps in groups.items():
-        ps = np.asarray(ps)
-        lo = dom.cell_lowers([leaf])[0]
-        xi = np.clip((x[ps] - lo) / half - 1.0, -1.0, 1.0)
-        n_phys = nrm[ps]
-        # reference normal: J n normalized; reference weight: physical / (det J / |J n_ref|)
-        nr = n_phys * half
-        nr = nr / np.linalg.norm(nr, axis=1)[:, None]
-        scale = np.prod(half) / np.linalg.norm(half * nr, axis=1)
-        out.append(SurfaceQuadrature(leaf, PointQuadrature(xi, w[ps] / scale), nr))
-    return out
+    return sphere_surface(center, radius, n_theta, n_phi).surface_quadratures(space)

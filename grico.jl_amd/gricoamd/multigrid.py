@@ -63,16 +63,19 @@ class GeometricMultigridPreconditioner:
 
     Additions without a reference counterpart: the Chebyshev-Jacobi smoother
     (`chebyshev_degree`, `chebyshev_range`, `eigen_iterations`), `graph`
-    (replay the V-cycle as a captured HIP graph) and `precision`: "fp64", or
+    (replay the V-cycle as a captured HIP graph), `precision`: "fp64", or
     "fp32" for smoothers, transfers and level operators in FP32 (coarse solve,
     diagonals and eigenvalue estimates in FP64) inside the FP64 outer Krylov
-    method."""
+    method, `h_steps_per_level` and `cell_blocks`: "cut" smooths with one
+    dense block per cell that carries its own rule (finite cells cut by the
+    boundary) on every smoothing level, point Jacobi elsewhere."""
 
     def __init__(self, min_degree=1, max_levels=16, p_sequence="bisect", pre_smoothing_steps=None,
                  post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
                  coarse_relative_tolerance=1.0e-12, coarse_absolute_tolerance=0.0, coarse_maxiter=10_000,
                  coarse_direct_dof_limit=_DEFAULT_COARSE_DIRECT_DOF_LIMIT, chebyshev_degree=4,
-                 chebyshev_range=20.0, eigen_iterations=12, graph=True, precision="fp64", h_steps_per_level=1):
+                 chebyshev_range=20.0, eigen_iterations=12, graph=True, precision="fp64", h_steps_per_level=1,
+                 cell_blocks=None):
         self.min_degree = _nonnegative_int(min_degree, "min_degree")
         self.max_levels = _positive_int(max_levels, "max_levels")
         if p_sequence not in ("bisect", "decrease_by_one", "go_to_one"):
@@ -105,6 +108,9 @@ class GeometricMultigridPreconditioner:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
         self.h_steps_per_level = _positive_int(h_steps_per_level, "h_steps_per_level")
+        if cell_blocks not in (None, "cut"):
+            raise ValueError("cell_blocks must be None or 'cut'")
+        self.cell_blocks = cell_blocks
 
     def resolved_smoother(self, operator_class):
         if self.smoother == "auto":
@@ -169,6 +175,12 @@ def h_coarsened_space(space):
     drop[child1] = True
     leaves = np.concatenate([space.leaves[~drop], parents.astype(np.int64)])
     degrees = np.concatenate([deg[~drop], parent_degrees]).astype(np.int32)
+    # Physical domains keep only domain-active leaves (src/adaptivity.jl:526-551).
+    classify = getattr(space.domain, "classify", None)
+    if classify is not None:
+        from .embedded import OUTSIDE
+        keep = classify(leaves) != OUTSIDE
+        leaves, degrees = leaves[keep], degrees[keep]
     return _space_with(space, leaves, degrees)
 
 
@@ -225,7 +237,8 @@ def has_coarsening_level(space, policy):
 
 def problem_on_space(problem, space):
     """The same operators and constraints on another space (src/multigrid.jl:596-612):
-    cell, boundary-face and interface operators carry over."""
+    cell, boundary-face, interface and surface operators, surface pieces,
+    embedded surfaces and cell rules carry over."""
     old = problem.field
     field = ScalarField(space, old.name)
     out = AffineProblem(field, problem.operator_class)
@@ -241,6 +254,19 @@ def problem_on_space(problem, space):
         out.add_interface_accumulator(field, field, op)
     for (_, op) in getattr(problem, "interface_rhs", []):
         out.add_interface_accumulator(field, op)
+    for (_, _, op) in getattr(problem, "surface_operators", []):
+        out.add_surface_accumulator(field, field, op)
+    for (_, op) in getattr(problem, "surface_rhs", []):
+        out.add_surface_accumulator(field, op)
+    # Explicit pieces and rules are tied to leaves: compile() rejects them on a
+    # level where their leaf is not active (as the reference does); embedded
+    # surfaces and a physical domain's cut-cell rules follow every level.
+    for sq in getattr(problem, "surface_quadratures", []):
+        out.add_surface_quadrature(sq)
+    for es in getattr(problem, "embedded_surfaces", []):
+        out.add_embedded_surface(es)
+    for leaf, rule in getattr(problem, "cell_quadratures", {}).items():
+        out.add_cell_quadrature(leaf, rule)
     for con in problem.constraints:
         out.add_constraint(Dirichlet(field, con.boundary, con.data))
     for con in getattr(problem, "mean_constraints", []):
@@ -502,10 +528,29 @@ class MultigridHierarchy:
             d.spd = int(isinstance(self.problem.operator_class, SPD))
             # Levels whose reduced map is not a selection (affine Dirichlet
             # relations, mean rows) get their reduced diagonal from the host.
-            if not all(getattr(p, "selection_map", True) for p in self.plans):
+            if pol.cell_blocks == "cut":
+                bds = (_devabi.BlockDesc * len(self.plans))()
+                self.block_colors = [0] * len(self.plans)
+                for lvl in range(1, len(self.plans)):
+                    blocks = cut_cell_blocks(self.plans[lvl])
+                    if not blocks:
+                        continue
+                    off, dofs, inv, ncol = block_inverses(self.plans[lvl], blocks)
+                    keep += [off, dofs, inv]
+                    bds[lvl].count = len(blocks)
+                    bds[lvl].offsets = _native.ptr(off, _native.c_int64)
+                    bds[lvl].dofs = _native.ptr(dofs, _native.c_int32)
+                    bds[lvl].inverse = _native.ptr(inv, _native.c_double)
+                    self.block_colors[lvl] = ncol
+                keep.append(bds)
+                d.level_blocks = bds
+            probe = [getattr(p, "selection_map", True) and _device_diagonal_unsupported(p) for p in self.plans]
+            if not all(getattr(p, "selection_map", True) for p in self.plans) or any(probe):
                 from .solve import reduced_diagonal_general
-                diags = [None if getattr(p, "selection_map", True) else
-                         np.ascontiguousarray(reduced_diagonal_general(p), dtype=np.float64) for p in self.plans]
+                diags = [np.ascontiguousarray(probed_diagonal(p)) if pr_ else
+                         None if getattr(p, "selection_map", True) else
+                         np.ascontiguousarray(reduced_diagonal_general(p), dtype=np.float64)
+                         for p, pr_ in zip(self.plans, probe)]
                 arr = (_native.P_f64 * len(diags))(*[_native.ptr(a, _native.c_double) if a is not None else
                                                      _native.P_f64() for a in diags])
                 keep += [diags, arr]
@@ -525,6 +570,146 @@ class MultigridHierarchy:
             _devabi.check(lib.gmf_mg_level_info(M.handle, lvl, ctypes.byref(n), ctypes.byref(lam)))
             out.append(lam.value)
         return out
+
+
+# ---- cell blocks of the block smoother --------------------------------------------
+
+def cut_cell_blocks(plan):
+    """Reduced dofs of every cell that carries its own rule (the plan's
+    point-cloud cell batches), one sorted int32 array per cell."""
+    if not getattr(plan, "selection_map", True):
+        raise ValueError("cell blocks need a reduced map that selects dofs")
+    out = []
+    for cb in getattr(plan, "cloud_buckets", []):
+        red = plan.reduced_index[cb.term_dof]
+        keep = red >= 0
+        cells, dofs = cb.term_cell[keep], red[keep]
+        order = np.lexsort((dofs, cells))
+        cells, dofs = cells[order], dofs[order]
+        bounds = np.searchsorted(cells, np.arange(cb.cell_count + 1))
+        for c in range(cb.cell_count):
+            d = np.unique(dofs[bounds[c]:bounds[c + 1]])
+            if len(d):
+                out.append(d.astype(np.int32))
+    return out
+
+
+def _item_dofs(plan):
+    """(item_ptr, item_dofs): the reduced dofs every item of the plan (cell,
+    surface piece, face) touches."""
+    ptr, dofs = [np.zeros(1, dtype=np.int64)], []
+    base = 0
+    for bk in plan.all_buckets:
+        if bk.cell_count == 0:
+            continue
+        red = plan.reduced_index[bk.term_dof]
+        keep = red >= 0
+        cells, d = bk.term_cell[keep], red[keep]
+        order = np.argsort(cells, kind="stable")
+        counts = np.bincount(cells, minlength=bk.cell_count)
+        ptr.append(base + np.cumsum(counts))
+        dofs.append(d[order])
+        base += int(counts.sum())
+    return np.ascontiguousarray(np.concatenate(ptr)), np.ascontiguousarray(np.concatenate(dofs).astype(np.int64))
+
+
+def block_colors(plan, blocks):
+    """Colors of the blocks such that no item of the plan touches dofs of two
+    blocks of one color (gh_conflict_coloring): the blocks of one color can be
+    probed with one apply per local position."""
+    lib = _native.host()
+    lib.gh_conflict_coloring.argtypes = [ctypes.c_int64, _native.P_i64, _native.P_i64, ctypes.c_int64,
+                                         _native.P_i64, _native.P_i64, ctypes.c_int64, _native.P_i32,
+                                         ctypes.POINTER(ctypes.c_int32)]
+    iptr, idofs = _item_dofs(plan)
+    lens = np.array([len(b) for b in blocks], dtype=np.int64)
+    bptr = np.ascontiguousarray(np.concatenate([[0], np.cumsum(lens)]).astype(np.int64))
+    bdofs = np.ascontiguousarray(np.concatenate(blocks).astype(np.int64)) if len(blocks) else np.zeros(0, np.int64)
+    colors = np.zeros(len(blocks), dtype=np.int32)
+    count = ctypes.c_int32()
+    _native.check_host(lib.gh_conflict_coloring(len(blocks), _native.ptr(bptr, _native.c_int64),
+                                                _native.ptr(bdofs, _native.c_int64), len(iptr) - 1,
+                                                _native.ptr(iptr, _native.c_int64), _native.ptr(idofs, _native.c_int64),
+                                                plan.reduced_dof_count, _native.ptr(colors, _native.c_int32),
+                                                ctypes.byref(count)))
+    return colors
+
+
+def _device_diagonal_unsupported(plan):
+    """True when the device's exact reduced diagonal refuses the plan: point-
+    cloud items whose modes expand into several dofs (hanging faces of cut
+    cells on coarse finite-cell levels) or repeat a dof."""
+    for cb in list(getattr(plan, "cloud_buckets", [])) + list(getattr(plan, "surface_buckets", [])):
+        if len(cb.multi_offsets) > 1:
+            return True
+        li = cb.local_index
+        for c in range(cb.cell_count):
+            row = li[c][li[c] >= 0]
+            if len(np.unique(row)) != len(row):
+                return True
+    return False
+
+
+def probed_diagonal(plan):
+    """Reduced operator diagonal by colored probing (gmf_reduced_blocks on
+    one-dof blocks): one apply per color of dofs no item couples; for plans
+    whose local diagonal is not exact and that the device diagonal does not
+    cover (point-cloud items with multi-term expansions). The reference
+    probes one unit vector per dof in that case (src/solve.jl:722-738)."""
+    from . import _devabi, ops
+    lib = _native.device()
+    _devabi.declare_mg(lib)
+    n = plan.reduced_dof_count
+    blocks = [np.array([j], dtype=np.int32) for j in range(n)]
+    colors = block_colors(plan, blocks)
+    off = np.arange(n + 1, dtype=np.int64)
+    dofs = np.arange(n, dtype=np.int32)
+    desc = _devabi.BlockDesc()
+    desc.count = n
+    desc.offsets = _native.ptr(off, _native.c_int64)
+    desc.dofs = _native.ptr(dofs, _native.c_int32)
+    out = np.zeros(n)
+    _devabi.check(lib.gmf_reduced_blocks(ops._ws(plan).ws, ctypes.byref(desc), _native.ptr(colors, _native.c_int32),
+                                         int(colors.max()) + 1, _native.ptr(out, _native.c_double)))
+    return out
+
+
+def block_inverses(plan, blocks):
+    """Inverses of B_b = R_b A R_b^T (row-major, one after another) from the
+    device operator by colored probing (gmf_reduced_blocks); each block must be
+    symmetric positive definite."""
+    from . import _devabi, ops
+    lib = _native.device()
+    _devabi.declare_mg(lib)
+    lens = np.array([len(b) for b in blocks], dtype=np.int64)
+    off = np.zeros(len(blocks) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(lens)
+    dofs = np.ascontiguousarray(np.concatenate(blocks).astype(np.int32))
+    colors = block_colors(plan, blocks)
+    desc = _devabi.BlockDesc()
+    desc.count = len(blocks)
+    desc.offsets = _native.ptr(off, _native.c_int64)
+    desc.dofs = _native.ptr(dofs, _native.c_int32)
+    vals = np.zeros(int((lens ** 2).sum()))
+    _devabi.check(lib.gmf_reduced_blocks(ops._ws(plan).ws, ctypes.byref(desc),
+                                         _native.ptr(colors, _native.c_int32), int(colors.max()) + 1,
+                                         _native.ptr(vals, _native.c_double)))
+    voff = np.zeros(len(blocks) + 1, dtype=np.int64)
+    voff[1:] = np.cumsum(lens ** 2)
+    inv = np.empty_like(vals)
+    for m in np.unique(lens):
+        idx = np.nonzero(lens == m)[0]
+        B = np.stack([vals[voff[b]:voff[b + 1]].reshape(m, m) for b in idx])
+        B = 0.5 * (B + B.transpose(0, 2, 1))
+        try:
+            L = np.linalg.cholesky(B)
+        except np.linalg.LinAlgError:
+            raise ValueError("a cell block of the multigrid smoother is not positive definite") from None
+        Li = np.linalg.inv(L)
+        Binv = Li.transpose(0, 2, 1) @ Li
+        for k, b in enumerate(idx):
+            inv[voff[b]:voff[b + 1]] = Binv[k].reshape(-1)
+    return off, dofs, inv, int(colors.max()) + 1
 
 
 def compile_geometric_multigrid(problem, policy=None, fine_plan=None):

@@ -596,7 +596,15 @@ class AssemblyPlan:
         self.full_dof_count = space.dof_count
         keys = {}
         custom = {}
-        cq = getattr(problem, "cell_quadratures", {})
+        explicit = getattr(problem, "cell_quadratures", {})
+        for leaf in explicit:
+            if leaf not in space.leaf_index:
+                raise ValueError(f"cell quadrature leaf {leaf} is not an active leaf of the space")
+        # Physical domains: automatic cut-cell rules, explicit rules override
+        # them (src/integration.jl:2420-2453).
+        auto = getattr(space.domain, "assembly_cell_quadratures", None)
+        cq = dict(auto(space)) if auto is not None else {}
+        cq.update(explicit)
         for li in range(len(space.leaves)):
             leaf = int(space.leaves[li])
             if leaf in cq:
@@ -622,7 +630,10 @@ class AssemblyPlan:
             self.cloud_buckets.append(cb)
         sops = [op for (_, _, op) in getattr(problem, "surface_operators", [])]
         surf = {}
-        for sq in getattr(problem, "surface_quadratures", []):
+        pieces = list(getattr(problem, "surface_quadratures", []))
+        for es in getattr(problem, "embedded_surfaces", []):
+            pieces.extend(es.surface_quadratures(space))
+        for sq in pieces:
             if sq.leaf not in space.leaf_index:
                 raise ValueError(f"surface quadrature leaf {sq.leaf} is not an active leaf of the space")
             li = space.leaf_index[sq.leaf]

@@ -254,6 +254,7 @@ class AffineProblem:
         self.mean_constraints = []
         self.cell_quadratures = {}     # leaf -> PointQuadrature
         self.surface_quadratures = []  # SurfaceQuadrature
+        self.embedded_surfaces = []    # geometry converted to SurfaceQuadrature pieces at compile time
         self.surface_operators = []    # (test_field, trial_field, operator)
         self.surface_rhs = []          # (test_field, operator)
         self.boundary_operators = []   # (BoundaryFace, test_field, trial_field, operator)
@@ -304,6 +305,15 @@ class AffineProblem:
         if surface.quadrature.points.shape[1] != self.field.space.dim:
             raise ValueError("surface quadrature dimension must match the space")
         self.surface_quadratures.append(surface)
+        return self
+
+    def add_embedded_surface(self, surface):
+        """Attach surface geometry that compile() turns into SurfaceQuadrature
+        pieces on the active leaves of the problem's space
+        (src/embedded.jl:140-163); it must provide surface_quadratures(space)."""
+        if not hasattr(surface, "surface_quadratures"):
+            raise TypeError("embedded surfaces must provide surface_quadratures(space)")
+        self.embedded_surfaces.append(surface)
         return self
 
     def add_surface_accumulator(self, *args):
@@ -364,6 +374,10 @@ def add_cell_quadrature(problem, leaf, quadrature):
 
 def add_surface_quadrature(problem, surface):
     return problem.add_surface_quadrature(surface)
+
+
+def add_embedded_surface(problem, surface):
+    return problem.add_embedded_surface(surface)
 
 
 def add_surface_accumulator(problem, *args):

@@ -121,6 +121,18 @@ int64_t gh_transfer_nnz(const gh_transfer* transfer);
 int gh_transfer_export(const gh_transfer* transfer, int64_t* coarse_indices, int64_t* fine_indices,
                        double* coefficients);
 
+/* Greedy coloring of blocks of dofs such that no item touches dofs of two
+ * blocks of one color: block b holds dofs block_dofs[block_ptr[b] ..
+ * block_ptr[b+1]), item i touches item_dofs[item_ptr[i] .. item_ptr[i+1])
+ * (dofs in 0 .. dof_count - 1). Blocks are colored in order with the smallest
+ * color no conflicting earlier block holds. Used to probe cell blocks and
+ * operator diagonals with one apply per color and position (no reference
+ * counterpart: the reference probes one unit vector per dof,
+ * src/solve.jl:722-738). */
+int gh_conflict_coloring(int64_t block_count, const int64_t* block_ptr, const int64_t* block_dofs,
+                         int64_t item_count, const int64_t* item_ptr, const int64_t* item_dofs, int64_t dof_count,
+                         int32_t* colors, int32_t* color_count);
+
 #ifdef __cplusplus
 }
 #endif

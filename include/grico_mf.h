@@ -327,6 +327,30 @@ typedef struct gmf_transfer_desc {
   const double* coefficients;
 } gmf_transfer_desc;
 
+/* Cell blocks of one multigrid level for block smoothing (no reference
+ * counterpart; the reference smooths with the point diagonal only): block b
+ * holds the reduced dofs dofs[offsets[b] .. offsets[b+1]) (distinct, at most
+ * 512) and `inverse` the row-major inverses of the blocks B_b = R_b A R_b^T
+ * one after another (m_b^2 values each; symmetric). The level's smoother then
+ * preconditions with S = D_u^-1 + sum_b R_b^T B_b^-1 R_b, D_u^-1 the Jacobi
+ * inverse diagonal on the dofs no block covers. HOST arrays, copied. */
+typedef struct gmf_block_desc {
+  int64_t count;
+  const int64_t* offsets;
+  const int32_t* dofs;
+  const double* inverse;
+} gmf_block_desc;
+
+/* The blocks B_b = R_b A R_b^T of a workspace's reduced operator (row-major,
+ * m_b^2 values per block one after another into `out`, a HOST array) by
+ * probing: for every color and every local position j, one reduced apply of
+ * the sum of the unit vectors of the j-th dofs of that color's blocks. Blocks
+ * of one color must be independent: no cell of the plan may touch dofs of two
+ * of them (otherwise the probes mix; the result is then wrong, not an error).
+ * colors[b] in 0 .. color_count - 1. `blocks.inverse` is not read. */
+gmf_status gmf_reduced_blocks(gmf_workspace* ws, const gmf_block_desc* blocks, const int32_t* colors,
+                              int32_t color_count, double* out);
+
 /* V-cycle hierarchy (src/multigrid.jl:51-96 GeometricMultigridPreconditioner,
  * 309-345 _compile_geometric_multigrid). levels[0] is the coarsest level and
  * levels[level_count-1] the workspace of the operator being preconditioned; the
@@ -367,6 +391,11 @@ typedef struct gmf_mg_desc {
    * need it, since the device's exact diagonal covers selection maps only;
    * the reference probes such diagonals (src/solve.jl:722-738). */
   const double* const* level_diagonals;
+  /* Optional, level_count entries (NULL, or entries with count 0: point
+   * smoothing): cell blocks of each level's block smoother (see
+   * gmf_block_desc; Jacobi and Chebyshev smoothers; the coarsest level's entry
+   * is not used). The Chebyshev bounds are then estimated for S A. */
+  const gmf_block_desc* level_blocks;
 } gmf_mg_desc;
 
 /* Compile the V-cycle: level Jacobi diagonals, Chebyshev bounds, transfer CSR

@@ -570,21 +570,123 @@ def reduced_matrix(plan):
     return A
 
 
+def _probe_points(ops, callback, q, P, D):
+    """W-less (D+1)x(D+1) action per point of `ops` probed with unit trial
+    channels through `callback` (linear in the trial channels)."""
+    M = np.zeros((P, D + 1, D + 1))
+    ones, zeros = np.ones(P), np.zeros(P)
+    for op in ops:
+        call = getattr(op, callback)
+        for j in range(D + 1):
+            v, g = _test_channels(call(q, _trial(ones if j == 0 else zeros,
+                                                 [ones if j == a + 1 else zeros for a in range(D)]), 0), (P,), D)
+            M[:, 0, j] += v
+            for a in range(D):
+                M[:, a + 1, j] += g[a]
+    return M
+
+
+def assembled_reduced_matrix(plan):
+    """Sparse P^T A P from dense element matrices E_c = sum_q chi_q^T (W_q M_q)
+    chi_q, chi = (value, physical gradient) of the full basis at the points and
+    M_q the accumulators probed at the points (the reference's basis path,
+    src/accumulators.jl:1022-1054, 1133-1161), scattered through the term maps.
+    Tensor cells, point-cloud cells and embedded-surface pieces; selection
+    reduced maps (the reduced dofs of a plan without constraint rows)."""
+    import scipy.sparse as sps
+    if not getattr(plan, "selection_map", True) or len(plan.fixed_dofs) or getattr(plan, "boundary_buckets", []) \
+            or getattr(plan, "interface_buckets", []):
+        raise ValueError("assembled_reduced_matrix covers cells and surface pieces on selection maps")
+    n = plan.full_dof_count
+    D = plan.dim
+    A = sps.csr_matrix((n, n))
+    cell_ops = [op for (_, _, op) in plan.problem.cell_operators]
+    surf_ops = [op for (_, _, op) in getattr(plan.problem, "surface_operators", [])]
+    groups = [(b, False) for b in plan.buckets] + [(b, True) for b in getattr(plan, "cloud_buckets", [])] + \
+             [(b, True) for b in getattr(plan, "surface_buckets", [])]
+    for bk, cloud in groups:
+        if bk.cell_count == 0:
+            continue
+        if cloud:
+            phi, grads = bk.tables()
+            chi = np.concatenate([phi[None], np.asarray(grads)], axis=0)  # (D+1, P, N)
+            ops, cb = (surf_ops, "surface_accumulate") if bk.surface else (cell_ops, "cell_accumulate")
+            M = _probe_points(ops, cb, bk.quadrature(), bk.n_points_total, D) * bk.weights[:, None, None]
+            E = np.zeros((bk.cell_count, bk.n_modes, bk.n_modes))
+            for c in range(bk.cell_count):
+                s0, s1 = bk.point_offsets[c], bk.point_offsets[c + 1]
+                ch = chi[:, s0:s1, :]
+                E[c] = np.einsum("cpn,pck,kpm->nm", ch, M[s0:s1], ch)
+        else:
+            Phi, dPhi = _dense_tables(bk)
+            M = action_matrices(plan, bk).reshape(bk.cell_count, bk.n_points, D + 1, D + 1)
+            M = M * bk.weights[:, :, None, None]
+            chi = [np.broadcast_to(Phi, (bk.cell_count,) + Phi.shape)] + \
+                  [dPhi[a][None] * bk.inverse_jacobian[:, a][:, None, None] for a in range(D)]
+            E = np.zeros((bk.cell_count, bk.n_modes, bk.n_modes))
+            for c1 in range(D + 1):
+                for c2 in range(D + 1):
+                    E += np.einsum("cqn,cq,cqm->cnm", chi[c1], M[:, :, c1, c2], chi[c2])
+        S = sps.csr_matrix((bk.term_coef, (bk.term_cell * bk.n_modes + bk.term_box, bk.term_dof)),
+                           shape=(bk.cell_count * bk.n_modes, n))
+        A = A + S.T @ sps.block_diag(list(E), format="csr") @ S
+    sd = plan.solve_dofs
+    return sps.csr_matrix(A)[sd][:, sd]
+
+
 def _chebyshev(A, dinv, x, b, x_zero, lam_max, degree, rng):
+    """Chebyshev smoothing in S A, S = diag(dinv), or S = dinv(r) when dinv
+    is callable (the block smoother)."""
+    S = dinv if callable(dinv) else (lambda v: dinv * v)
     lam_min = lam_max / rng
     theta, delta = 0.5 * (lam_max + lam_min), 0.5 * (lam_max - lam_min)
     sigma = theta / delta
     r = b.copy() if x_zero else b - A @ x
-    d = dinv * r / theta
+    d = S(r) / theta
     x = d.copy() if x_zero else x + d
     rho = 1.0 / sigma
     for _ in range(1, degree):
         r = r - A @ d
         rho_new = 1.0 / (2.0 * sigma - rho)
-        d = rho_new * rho * d + (2.0 * rho_new / delta) * (dinv * r)
+        d = rho_new * rho * d + (2.0 * rho_new / delta) * S(r)
         x = x + d
         rho = rho_new
     return x
+
+
+def cell_block_dofs(plan):
+    """Reduced dofs touched by each cell that carries its own rule (the
+    plan's point-cloud cells), in cell order: the blocks of the cell-block
+    smoother."""
+    out = []
+    for cb in getattr(plan, "cloud_buckets", []):
+        red = plan.reduced_index[cb.term_dof]
+        for c in range(cb.cell_count):
+            d = np.unique(red[(cb.term_cell == c) & (red >= 0)])
+            if len(d):
+                out.append(d)
+    return out
+
+
+def block_smoother(A, blocks):
+    """S r = D_u^-1 r + sum_b R_b^T (R_b A R_b^T)^-1 R_b r: dense inverses of the
+    level matrix restricted to each block's dofs, point Jacobi on the dofs no
+    block covers."""
+    d = np.diag(A)
+    covered = np.zeros(len(d), dtype=bool)
+    inv = []
+    for dofs in blocks:
+        dofs = np.asarray(dofs, dtype=np.int64)
+        inv.append((dofs, np.linalg.inv(A[np.ix_(dofs, dofs)])))
+        covered[dofs] = True
+    dinv_u = np.where(covered, 0.0, 1.0 / d)
+
+    def S(r):
+        z = dinv_u * r
+        for dofs, Bi in inv:
+            z[dofs] += Bi @ r[dofs]
+        return z
+    return S
 
 
 def gmres_smooth(A, dinv, x, b, steps, restart, omega):
@@ -622,13 +724,19 @@ def gmres_smooth(A, dinv, x, b, steps, restart, omega):
 
 
 def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, lambdas=None,
-           cheb_degree=4, cheb_range=20.0, restart=8):
+           cheb_degree=4, cheb_range=20.0, restart=8, blocks=None):
     """z = M^-1 r for the V-cycle: damped Jacobi sweeps (src/multigrid.jl:943-956)
     or Chebyshev applications, residual restriction with P^T, recursive coarse
     correction, exact dense coarse solve (891-899), prolongation (1060-1072).
     mats[l]: dense reduced operator of level l (0 = coarsest); prolongations[l]:
-    dense (n_{l+1} x n_l) matrix."""
+    dense (n_{l+1} x n_l) matrix. blocks[l] (optional): dof lists of level l's
+    block smoother (block_smoother), which replaces D^-1 in the Jacobi and
+    Chebyshev smoothers."""
     dinvs = [1.0 / np.diag(A) if A.size else np.zeros(0) for A in mats]
+    if blocks is not None:
+        for l, bl in enumerate(blocks):
+            if l > 0 and bl:
+                dinvs[l] = block_smoother(mats[l], bl)
 
     def smooth(l, x, b, steps, x_zero):
         A, dinv = mats[l], dinvs[l]
@@ -640,7 +748,7 @@ def vcycle(mats, prolongations, r, smoother="jacobi", pre=2, post=2, omega=0.8, 
             first = x_zero and k == 0
             if smoother == "jacobi":
                 res = b if first else b - A @ x
-                x = (0.0 if first else x) + omega * dinv * res
+                x = (0.0 if first else x) + omega * (dinv(res) if callable(dinv) else dinv * res)
             else:
                 x = _chebyshev(A, dinv, x, b, first, lambdas[l], cheb_degree, cheb_range)
         return x

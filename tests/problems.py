@@ -234,6 +234,46 @@ def fcm_ball(n=8, degree=2, depth=2, gamma=None, n_theta=48, n_phi=96, alpha=1e-
     return pr
 
 
+def _ball_background(n):
+    """[-1.2, 1.2]^3 with n^3 leaves built as roots^3 root cells refined k
+    times (n = roots 2^k, roots odd), so that multigrid finds h-levels."""
+    roots, k = n, 0
+    while roots % 2 == 0:
+        roots, k = roots // 2, k + 1
+    dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (roots,) * 3)
+    g.uniform_refine(dom, k)
+    return dom
+
+
+def fcm_ball_physical(n=4, degree=2, depth=2, gamma=None, n_theta=48, n_phi=96, alpha=1e-3, explicit=False):
+    """fcm_ball on a PhysicalDomain: the cut-cell rules are generated by
+    compile() for whatever leaves the space has (moment fitted, finite-cell
+    extension alpha) and the sphere is a SampledSurface; explicit=True attaches
+    the same rules and pieces per leaf instead (add_cell_quadrature /
+    add_surface_quadrature)."""
+    from gricoamd.embedded import (FiniteCellExtension, ImplicitRegion, PhysicalDomain, add_cut_cell_quadratures,
+                                   physical_space, sphere_surface)
+    region = ImplicitRegion(lambda p: p[0] ** 2 + p[1] ** 2 + p[2] ** 2 - 1.0, subdivision_depth=depth)
+    opts = g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree))
+    surface = sphere_surface((0.0, 0.0, 0.0), 1.0, n_theta=n_theta, n_phi=n_phi)
+    if explicit:
+        space, cut = physical_space(_ball_background(n), region, opts)
+    else:
+        space = g.HpSpace(PhysicalDomain(_ball_background(n), region, FiniteCellExtension(alpha)), opts)
+    u = g.ScalarField(space, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, Diffusion())
+    pr.add_cell_accumulator(u, BallLoad(3))
+    if explicit:
+        add_cut_cell_quadratures(pr, region, cut, alpha=alpha)
+        for sq in surface.surface_quadratures(space):
+            pr.add_surface_quadrature(sq)
+    else:
+        pr.add_embedded_surface(surface)
+    pr.add_surface_accumulator(u, u, NitscheDirichlet(gamma if gamma is not None else 10.0 * (degree + 1) ** 2))
+    return pr
+
+
 # ---- adaptive hp corner problem ---------------------------------------------------
 
 def corner_exact(p):

@@ -25,10 +25,11 @@ PAIRS = {
     "MgDesc": ("gmf_mg_desc", _devabi.MgDesc),
     "MlpDesc": ("gmf_mlp_desc", learned.MlpDesc),
     "BucketTuning": ("gmf_bucket_tuning", _devabi.BucketTuning),
+    "BlockDesc": ("gmf_block_desc", _devabi.BlockDesc),
 }
 
 C_SCALARS = {"int32_t": "Int32", "int64_t": "Int64", "double": "Float64", "float": "Float32"}
-C_STRUCTS = {"gmf_bucket_desc": "BucketDesc", "gmf_transfer_desc": "TransferDesc"}
+C_STRUCTS = {"gmf_bucket_desc": "BucketDesc", "gmf_transfer_desc": "TransferDesc", "gmf_block_desc": "BlockDesc"}
 SIZES = {"Int32": 4, "Int64": 8, "Float64": 8, "Float32": 4}
 
 

@@ -253,3 +253,94 @@ def test_dense_coarse_level_size_is_checked():
             h._check_dense_coarse()
     finally:
         mgm.DENSE_COARSE_MAX = saved
+
+
+# ---- block smoother (oracle) and physical-domain hierarchies ----------------------
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n))
+    return M @ M.T + n * np.eye(n)
+
+
+def test_oracle_block_smoother_closed_forms():
+    A = _spd(12, 1)
+    b = np.random.default_rng(2).standard_normal(12)
+    # one block holding every dof: S = A^-1
+    S = pyoracle.block_smoother(A, [np.arange(12)])
+    assert np.allclose(S(b), np.linalg.solve(A, b), rtol=1e-12, atol=1e-12)
+    # single-dof blocks: S = D^-1
+    S = pyoracle.block_smoother(A, [[i] for i in range(12)])
+    assert np.allclose(S(b), b / np.diag(A), rtol=1e-14)
+    # no blocks: point Jacobi
+    S = pyoracle.block_smoother(A, [])
+    assert np.allclose(S(b), b / np.diag(A), rtol=1e-14)
+
+
+def test_oracle_block_smoother_is_spd_with_overlap():
+    A = _spd(15, 3)
+    blocks = [np.arange(0, 6), np.arange(4, 10), np.array([9, 11, 12])]
+    S = pyoracle.block_smoother(A, blocks)
+    M = np.stack([S(e) for e in np.eye(15)], axis=1)
+    assert np.allclose(M, M.T, atol=1e-13)
+    assert np.linalg.eigvalsh(M).min() > 0.0
+    # a dof no block covers keeps its Jacobi entry
+    assert np.isclose(M[14, 14], 1.0 / A[14, 14]) and np.allclose(M[14, :14], 0.0)
+
+
+def test_oracle_chebyshev_with_point_blocks_equals_jacobi_chebyshev():
+    A = _spd(10, 4)
+    b = np.random.default_rng(5).standard_normal(10)
+    dinv = 1.0 / np.diag(A)
+    S = pyoracle.block_smoother(A, [[i] for i in range(10)])
+    x1 = pyoracle._chebyshev(A, dinv, np.zeros(10), b, True, 3.0, 4, 20.0)
+    x2 = pyoracle._chebyshev(A, S, np.zeros(10), b, True, 3.0, 4, 20.0)
+    assert np.allclose(x1, x2, rtol=1e-14, atol=1e-14)
+
+
+def test_physical_domain_rules_equal_explicit_rules():
+    """compile() on a PhysicalDomain generates the rules and surface pieces
+    that the explicit attachments give: the same operator."""
+    auto = g.compile(P.fcm_ball_physical(n=4, degree=2, depth=2))
+    expl = g.compile(P.fcm_ball_physical(n=4, degree=2, depth=2, explicit=True))
+    assert auto.full_dof_count == expl.full_dof_count
+    assert len(auto.cloud_buckets) and len(auto.surface_buckets)
+    x = P.rng_vector(auto.full_dof_count, 11)
+    ya, ye = pyoracle.apply(auto, x), pyoracle.apply(expl, x)
+    assert np.abs(ya - ye).max() <= 1e-14 * np.abs(ye).max()
+
+
+def test_physical_domain_hierarchy_levels():
+    from gricoamd.embedded import OUTSIDE
+    from gricoamd.multigrid import MultigridHierarchy, block_colors, cut_cell_blocks
+    pr = P.fcm_ball_physical(n=8, degree=2, depth=1)
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", h_steps_per_level=3, cell_blocks="cut")
+    h = MultigridHierarchy(pr, pol)
+    dom = pr.field.space.domain
+    counts = h.reduced_dof_counts()
+    assert counts == sorted(counts) and len(counts) >= 3
+    for plan in h.plans:
+        cls = dom.classify(plan.space.leaves)
+        assert not np.any(cls == OUTSIDE)
+        # every cut leaf of the level carries its own rule, and only those
+        ruled = sorted(int(c) for cb in plan.cloud_buckets for c in cb.cells)
+        assert ruled == sorted(int(l) for l in plan.space.leaves[cls == 2])
+        assert plan.surface_buckets
+        blocks = cut_cell_blocks(plan)
+        colors = block_colors(plan, blocks)
+        # no item of the plan touches dofs of two blocks of one color
+        for bk in plan.all_buckets:
+            red = plan.reduced_index[bk.term_dof]
+            for c in range(bk.cell_count):
+                touched = set(red[(bk.term_cell == c) & (red >= 0)].tolist())
+                seen = {}
+                for i, bl in enumerate(blocks):
+                    if touched & set(bl.tolist()):
+                        assert seen.setdefault(colors[i], i) == i, (c, i)
+
+
+def test_explicit_rules_on_h_levels_are_rejected():
+    pr = P.fcm_ball_physical(n=4, degree=2, depth=1, explicit=True)
+    from gricoamd.multigrid import MultigridHierarchy
+    with pytest.raises(ValueError, match="not an active leaf"):
+        MultigridHierarchy(pr, GeometricMultigridPreconditioner(h_steps_per_level=1))

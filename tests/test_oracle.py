@@ -358,3 +358,27 @@ def test_oracle_rhs_gradient_channel_and_l2_moment():
     assert s0 < 1e-24
     s0, _ = pyoracle.l2_sums(plan_x, cx, w, plan_x.point_values(lambda p: 0.0 * p[0]))
     assert s0 == pytest.approx(1.0 / 3.0, rel=1e-12)
+
+
+def test_assembled_matrix_equals_reduced_apply():
+    """The oracle's element-matrix assembly against its own apply: tensor
+    cells with a general point action, cut cells with random rules and
+    surface pieces, and the finite-cell ball."""
+    from test_cloud_host import Nitsche
+    from gricoamd.problem import PointQuadrature, SurfaceQuadrature
+    rng = np.random.default_rng(4)
+    pr = P.poisson(3, 2, 2, operator=P.MassDiffusion(0.7, 1.3, b=(0.2, -0.1, 0.3)), rhs=False, dirichlet=False)
+    for leaf in pr.field.space.leaves[::3]:
+        n = int(rng.integers(3, 40))
+        pr.add_cell_quadrature(int(leaf), PointQuadrature(rng.uniform(-1, 1, (n, 3)), rng.uniform(0.01, 0.2, n)))
+    for leaf in pr.field.space.leaves[::2]:
+        pr.add_surface_quadrature(SurfaceQuadrature(int(leaf), PointQuadrature(rng.uniform(-1, 1, (5, 3)),
+                                                                               rng.uniform(0.05, 0.3, 5)),
+                                                    rng.standard_normal((5, 3))))
+    pr.add_surface_accumulator(pr.field, pr.field, Nitsche(6.0))
+    for problem in (pr, P.fcm_ball(n=3, degree=2, depth=1)):
+        plan = g.compile(problem)
+        A = pyoracle.assembled_reduced_matrix(plan)
+        x = P.rng_vector(plan.reduced_dof_count, 8)
+        y = pyoracle.reduced_apply(plan, x)
+        assert np.abs(A @ x - y).max() <= 1e-13 * np.abs(y).max()
