@@ -934,14 +934,16 @@ def _traffic_from_profiles(n, cells):
     return rec.get("traffic_bytes_per_cell_pass")
 
 
-def build_fcm_problem(cells, degree, depth, alpha=1e-5):
+def build_fcm_problem(cells, degree, depth, alpha=1e-3):
     """Finite-cell Poisson on the unit ball embedded in [-1.2, 1.2]^3: cells^3
-    background cells, cut cells with subdivision rules to `depth` and the
-    finite-cell extension alpha, Nitsche (symmetric, gamma = 10 (p + 1)^2 / h)
-    on explicit sphere surface pieces, load 6 (exact solution 1 - r^2)."""
+    background leaves of a PhysicalDomain (cut-cell rules moment fitted by
+    compile() on every leaf set it sees, subdivision depth `depth`, finite-cell
+    extension alpha), Nitsche (symmetric, gamma = 10 (p + 1)^2 / h) on a sampled
+    sphere, load 6 (exact solution 1 - r^2). alpha = 1e-3 keeps the Nitsche form
+    positive definite (with 1e-5 the 8^3 ball already has negative eigenvalues
+    on small cuts, so CG has no guarantee)."""
     import gricoamd as g
-    from gricoamd.embedded import ImplicitRegion, add_cut_cell_quadratures, physical_space, \
-        sphere_surface_quadratures
+    from gricoamd.embedded import FiniteCellExtension, ImplicitRegion, PhysicalDomain, sphere_surface
 
     class Diffusion:
         def cell_accumulate(self, q, trial, test_component):
@@ -969,11 +971,7 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-5):
         roots, k = roots // 2, k + 1
     dom = g.Domain((-1.2, -1.2, -1.2), (2.4, 2.4, 2.4), (roots, roots, roots))
     g.uniform_refine(dom, k)
-    space, cut = physical_space(dom, region, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
-    u = g.ScalarField(space, "u")
-    pr = g.AffineProblem(u, g.SPD())
-    pr.add_cell_accumulator(u, u, Diffusion())
-    pr.add_cell_accumulator(u, Load())
+    pdom = PhysicalDomain(dom, region, FiniteCellExtension(alpha))
     last = [0.0]
 
     def progress(done, total):
@@ -981,34 +979,38 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-5):
             last[0] = time.perf_counter()
             log(f"cut-cell moment fitting: {done}/{total} cells")
 
-    add_cut_cell_quadratures(pr, region, cut, alpha=alpha, progress=progress)
-    for sq in sphere_surface_quadratures(space, (0.0, 0.0, 0.0), 1.0, n_theta=4 * cells, n_phi=8 * cells):
-        pr.add_surface_quadrature(sq)
+    pdom.fit_progress = progress
+    space = g.HpSpace(pdom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(degree)))
+    u = g.ScalarField(space, "u")
+    pr = g.AffineProblem(u, g.SPD())
+    pr.add_cell_accumulator(u, u, Diffusion())
+    pr.add_cell_accumulator(u, Load())
+    pr.add_embedded_surface(sphere_surface((0.0, 0.0, 0.0), 1.0, n_theta=4 * cells, n_phi=8 * cells))
     pr.add_surface_accumulator(u, u, Nitsche(10.0 * (degree + 1) ** 2))
     return pr
 
 
-def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, oracle_cells=4):
+def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, oracle_cells=4, precision="fp64"):
     """Finite-cell sphere workload with nrhs seeded N(0,1) reduced right-hand
-    sides: the block apply (point-cloud items take every column in one launch)
-    against the one-column apply, and CG with the FP32 Jacobi preconditioner
-    (FP64 Krylov) run for a fixed number of iterations on the whole block and
-    on `sequential` columns one at a time (time per iteration and right-hand
-    side; Jacobi-CG needs tens of thousands of iterations to reach 1e-8 on this
-    operator, so convergence is checked by the small-ball tests); the block
-    apply against the oracle on a small ball."""
+    sides solved to 1e-8 by block CG (FP64 Krylov, lockstep columns) with the
+    hp-multigrid preconditioner on physical-domain levels (cut-cell rules
+    rediscretized per level, Chebyshev smoothing over cell blocks on the cut
+    cells); the block apply against the one-column apply; `sequential` columns
+    solved one at a time; the physical load's solution against 1 - r^2; the
+    block apply against the oracle on a small ball."""
     import numpy as np
     import torch
 
     import gricoamd as g
     import pyoracle
     from gricoamd import _devabi, _native, ops
-    from gricoamd.solve import compile_preconditioner
+    from gricoamd.multigrid import GeometricMultigridPreconditioner, compile_geometric_multigrid
 
     lib = _native.device()
     out = {"workload": f"finite-cell Poisson, unit ball in [-1.2,1.2]^3, {cells}^3 background cells, p={degree}, "
-                       f"cut cells subdivided to depth {depth} (alpha 1e-5), Nitsche on explicit sphere pieces; "
-                       f"{nrhs} seeded N(0,1) right-hand sides, CG + FP32 Jacobi (FP64 Krylov)"}
+                       f"PhysicalDomain cut cells subdivided to depth {depth} and moment fitted, finite-cell "
+                       f"extension alpha 1e-3, Nitsche on a sampled sphere; {nrhs} seeded N(0,1) right-hand sides, "
+                       f"block CG to 1e-8 with the hp-multigrid preconditioner ({precision} V-cycle)"}
     t0 = time.perf_counter()
     problem = build_fcm_problem(cells, degree, depth)
     plan = g.compile(problem)
@@ -1027,8 +1029,8 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, o
     log(f"fcm: {out}")
     gen = torch.Generator(device="cuda")
     gen.manual_seed(5)
-    X = torch.randn((nrhs, n), dtype=torch.float64, device="cuda", generator=gen)  # column j = X[j]
-    Y = torch.empty_like(X)
+    B = torch.randn((nrhs, n), dtype=torch.float64, device="cuda", generator=gen)  # column j = B[j]
+    Y = torch.empty_like(B)
     y1 = torch.empty(n, dtype=torch.float64, device="cuda")
 
     def clock(fn, reps):
@@ -1040,9 +1042,9 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, o
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps
 
-    t_one = clock(lambda: _devabi.check(lib.gmf_reduced_apply(dev.ws, X[0].data_ptr(), y1.data_ptr())), 5)
-    t_block = clock(lambda: _devabi.check(lib.gmf_reduced_apply_block(dev.ws, X.data_ptr(), Y.data_ptr(), nrhs)), 2)
-    _devabi.check(lib.gmf_reduced_apply(dev.ws, X[nrhs - 1].data_ptr(), y1.data_ptr()))
+    t_one = clock(lambda: _devabi.check(lib.gmf_reduced_apply(dev.ws, B[0].data_ptr(), y1.data_ptr())), 5)
+    t_block = clock(lambda: _devabi.check(lib.gmf_reduced_apply_block(dev.ws, B.data_ptr(), Y.data_ptr(), nrhs)), 2)
+    _devabi.check(lib.gmf_reduced_apply(dev.ws, B[nrhs - 1].data_ptr(), y1.data_ptr()))
     torch.cuda.synchronize()
     out.update({"apply_one_column_seconds": t_one, "apply_block_seconds": t_block,
                 "apply_block_seconds_per_column": t_block / nrhs, "apply_block_speedup_per_column": t_one / (t_block / nrhs),
@@ -1050,49 +1052,69 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, iterations=30, sequential=2, o
                     float(torch.max(torch.abs(Y[nrhs - 1] - y1)) / torch.max(torch.abs(y1)))})
     log(f"fcm apply: one column {t_one * 1e3:.2f} ms, block {t_block / nrhs * 1e3:.2f} ms per column")
     del Y
-    # CG with a fixed iteration budget: the whole block, then columns one at a time.
-    solver = g.CGSolver(preconditioner=g.JacobiPreconditioner(precision="fp32"))
-    M = compile_preconditioner(solver.preconditioner, plan)
-    B = X
-    Xs = torch.zeros_like(B)
+    # hp-multigrid on physical-domain levels with cell-block smoothing.
+    t0 = time.perf_counter()
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", cell_blocks="cut", h_steps_per_level=3,
+                                           chebyshev_degree=3, precision=precision, coarse_direct_dof_limit=40000)
+    h = compile_geometric_multigrid(problem, pol, fine_plan=plan)
+    M = h.device_preconditioner()
+    torch.cuda.synchronize()
+    out.update({"hierarchy_seconds": time.perf_counter() - t0, "levels": h.reduced_dof_counts(),
+                "cut_blocks_per_level": [sum(b.cell_count for b in p.cloud_buckets) for p in h.plans],
+                "block_colors_per_level": getattr(h, "block_colors", None), "coarse_solver": h.coarse_solver_kind()})
+    log(f"fcm hierarchy: {h.reduced_dof_counts()} in {out['hierarchy_seconds']:.1f}s")
+    X = torch.zeros_like(B)
     infos = (_devabi.SolveInfo * nrhs)()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    status = lib.gmf_cg_solve_block(dev.ws, M.handle, B.data_ptr(), Xs.data_ptr(), nrhs, 0, 1e-8, 0.0, iterations,
-                                    infos)
+    status = lib.gmf_cg_solve_block(dev.ws, M.handle, B.data_ptr(), X.data_ptr(), nrhs, 0, 1e-8, 0.0, maxiter, infos)
     torch.cuda.synchronize()
     t_bsolve = time.perf_counter() - t0
-    # -3: the budget ended before 1e-8 (expected); -4: a column met a
-    # non-positive direction (Nitsche on small cuts can make the operator
-    # indefinite), reported with the iterations done.
-    if status not in (0, -3, -4):
-        _devabi.check(status)
+    _devabi.check(status)
     its = [int(i.iterations) for i in infos]
-    out.update({"cg_iterations": iterations, "block_cg_status": int(status), "block_cg_seconds": t_bsolve,
-                "block_cg_seconds_per_iteration_and_rhs": t_bsolve / (max(1, max(its)) * nrhs),
-                "block_relative_residual_after_budget_max": max(i.final_residual_norm / i.rhs_norm for i in infos)})
-    log(f"fcm block cg: {t_bsolve:.2f} s for {max(its)} iterations on {nrhs} rhs")
+    # true residuals of the block solution
+    R = torch.empty_like(B)
+    _devabi.check(lib.gmf_reduced_apply_block(dev.ws, X.data_ptr(), R.data_ptr(), nrhs))
+    rel_true = (torch.linalg.norm(B - R, dim=1) / torch.linalg.norm(B, dim=1)).cpu().numpy()
+    out.update({"gmg_block_cg_status": int(status), "gmg_block_cg_seconds": t_bsolve,
+                "gmg_block_cg_seconds_per_rhs": t_bsolve / nrhs,
+                "gmg_block_cg_iterations_max": max(its), "gmg_block_cg_iterations_mean": float(np.mean(its)),
+                "all_rhs_converged": bool(all(int(i.converged) == 1 for i in infos)),
+                "true_relative_residual_max": float(rel_true.max())})
+    log(f"fcm GMG block CG: {t_bsolve:.2f} s for {nrhs} rhs, iterations max {max(its)}, "
+        f"true residual max {rel_true.max():.2e}")
+    del R
     seq = []
     for j in range(sequential):
         info = _devabi.SolveInfo()
         x1 = torch.zeros(n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        status = lib.gmf_cg_solve(dev.ws, M.handle, B[j].data_ptr(), x1.data_ptr(), 0, 1e-8, 0.0, iterations,
-                                  ctypes_byref(info))
+        _devabi.check(lib.gmf_cg_solve(dev.ws, M.handle, B[j].data_ptr(), x1.data_ptr(), 0, 1e-8, 0.0, maxiter,
+                                       ctypes_byref(info)))
         torch.cuda.synchronize()
-        if status not in (0, -3, -4):
-            _devabi.check(status)
-        seq.append((time.perf_counter() - t0, max(1, int(info.iterations)), info.final_residual_norm / info.rhs_norm,
-                    infos[j].final_residual_norm / infos[j].rhs_norm,
-                    float(torch.max(torch.abs(x1 - Xs[j])) / torch.max(torch.abs(x1)))))
-    t_seq = sum(s_[0] / s_[1] for s_ in seq) / len(seq)
-    out.update({"one_column_cg_seconds_per_iteration": t_seq,
-                "block_speedup_per_iteration_and_rhs": t_seq / out["block_cg_seconds_per_iteration_and_rhs"],
-                "one_column_vs_block_relative_residual": [[s_[2], s_[3]] for s_ in seq],
-                "one_column_vs_block_iterate_max_relative_difference": max(s_[4] for s_ in seq)})
-    log(f"fcm one-column cg: {t_seq * 1e3:.2f} ms per iteration")
-    del Xs, X
+        seq.append((time.perf_counter() - t0, int(info.iterations),
+                    float(torch.max(torch.abs(x1 - X[j])) / torch.max(torch.abs(x1)))))
+    out.update({"one_column_gmg_cg_seconds": float(np.mean([s_[0] for s_ in seq])),
+                "one_column_gmg_cg_iterations": [s_[1] for s_ in seq],
+                "one_column_vs_block_solution_max_relative_difference": max(s_[2] for s_ in seq)})
+    del X, B
+    # the physical load (exact solution 1 - r^2)
+    b = torch.as_tensor(plan.reduced_rhs(), device="cuda")
+    xs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    info = _devabi.SolveInfo()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _devabi.check(lib.gmf_cg_solve(dev.ws, M.handle, b.data_ptr(), xs.data_ptr(), 0, 1e-8, 0.0, maxiter,
+                                   ctypes_byref(info)))
+    torch.cuda.synchronize()
+    t_phys = time.perf_counter() - t0
+    state = g.State(plan, ops.expand_reduced(plan, xs, include_shift=True).cpu().numpy())
+    out.update({"physical_load_seconds": t_phys, "physical_load_iterations": int(info.iterations),
+                "physical_load_relative_l2_error_vs_1_minus_r2":
+                    g.relative_l2_error(state, lambda p: 1.0 - (p[0] ** 2 + p[1] ** 2 + p[2] ** 2))})
+    log(f"fcm physical load: {int(info.iterations)} iterations, {t_phys:.2f} s, "
+        f"L2 {out['physical_load_relative_l2_error_vs_1_minus_r2']:.2e}")
     # The block apply against the oracle on a small ball of the same kind.
     small = g.compile(build_fcm_problem(oracle_cells, degree, depth))
     Xo = np.random.default_rng(9).standard_normal((small.reduced_dof_count, 3))
@@ -1206,13 +1228,15 @@ def main():
     ap.add_argument("--sipg-only", action="store_true",
                     help="run only the interior penalty DG leg (face kernel profiling) and print its JSON")
     ap.add_argument("--fcm", action="store_true",
-                    help="run the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG); "
-                         "off by default: fitting the 7,532 cut-cell rules takes minutes of host time")
+                    help="run the finite-cell sphere leg (48^3 background cells, 64 right-hand sides, block CG "
+                         "with hp-multigrid to 1e-8); off by default: fitting the cut-cell rules of every level "
+                         "takes minutes of host time")
     ap.add_argument("--no-fcm", action="store_true", help="(default) skip the finite-cell sphere leg")
     ap.add_argument("--fcm-only", action="store_true", help="run only the finite-cell sphere leg and print its JSON")
     ap.add_argument("--fcm-cells", type=int, default=48, help="background cells per axis of the finite-cell leg")
     ap.add_argument("--fcm-rhs", type=int, default=64, help="right-hand sides of the finite-cell leg")
-    ap.add_argument("--fcm-iterations", type=int, default=30, help="CG iterations timed in the finite-cell leg")
+    ap.add_argument("--fcm-iterations", type=int, default=2000,
+                    help="CG iteration limit of the finite-cell leg's solves (1e-8)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-correction FGMRES part of the log-normal leg")
     ap.add_argument("--lognormal-only", action="store_true",
@@ -1249,7 +1273,7 @@ def main():
               flush=True)
         return
     if args.fcm_only:
-        print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)}), flush=True)
+        print(json.dumps({"fcm": fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, maxiter=args.fcm_iterations)}), flush=True)
         return
     t0 = time.perf_counter()
     problem = build_problem(args.depth)
@@ -1373,7 +1397,7 @@ def main():
             result["lognormal"] = {"error": repr(exc)}
     if args.fcm and not args.no_fcm:
         try:
-            result["fcm"] = fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, iterations=args.fcm_iterations)
+            result["fcm"] = fcm_leg(cells=args.fcm_cells, nrhs=args.fcm_rhs, maxiter=args.fcm_iterations)
         except Exception as exc:  # keep the headline line
             log(f"fcm leg failed: {exc!r}")
             result["fcm"] = {"error": repr(exc)}

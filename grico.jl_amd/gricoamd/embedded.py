@@ -382,6 +382,7 @@ class PhysicalDomain:
         self.region = region
         self.cell_measure = measure
         self._fitted = {}  # (leaf, shape) -> fitted physical rule, or None
+        self.fit_progress = None  # optional progress(done, total) of the moment fits
 
     def __getattr__(self, name):
         if name.startswith("__") or name == "background":
@@ -405,7 +406,8 @@ class PhysicalDomain:
         shape = tuple(int(q) for q in shape)
         todo = [int(l) for l in leaves if (int(l), shape) not in self._fitted]
         if todo:
-            got = fitted_cut_cell_rules(self.background, self.region, todo, shape, progress=progress)
+            got = fitted_cut_cell_rules(self.background, self.region, todo, shape,
+                                        progress=progress if progress is not None else self.fit_progress)
             for l in todo:
                 self._fitted[(l, shape)] = got.get(l)
         return {int(l): self._fitted[(int(l), shape)] for l in leaves if self._fitted[(int(l), shape)] is not None}

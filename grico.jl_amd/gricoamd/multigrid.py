@@ -161,8 +161,52 @@ def h_coarsening_candidates(space):
     return cells[ok], fc[ok]
 
 
+def _balanced(space, parents, firsts):
+    """Mask of the candidates whose merge keeps every neighbour within a factor
+    two per axis: all leaves touching the parent's box have the children's
+    size. Physical domains need it: leaves whose sibling is outside never
+    merge, so without the check size ratios across a face grow by a factor two
+    per level, and the continuity constraints of such faces (a leaf hanging on
+    a neighbour four or more times its size) no longer give nested spaces.
+    Parent boxes are compared on a lattice of the finest leaf size."""
+    dom = space.domain
+    leaves = space.leaves
+    origin = np.asarray(dom.origin, dtype=np.float64)
+    sizes = dom.cell_sizes(leaves)
+    unit = sizes.min(axis=0)
+    lo = np.rint((dom.cell_lowers(leaves) - origin) / unit).astype(np.int64)
+    ext = np.rint(sizes / unit).astype(np.int64)
+    shape = (lo + ext).max(axis=0) + 2
+    owner = np.full(tuple(shape), -1, dtype=np.int64)
+    D = space.dim
+    for e in np.unique(ext, axis=0):
+        sel = np.nonzero(np.all(ext == e, axis=1))[0]
+        offs = np.stack(np.meshgrid(*[np.arange(int(v)) for v in e], indexing="ij"), axis=-1).reshape(-1, D)
+        cells = lo[sel][:, None, :] + offs[None, :, :]
+        owner[tuple(cells.reshape(-1, D).T + 1)] = np.repeat(sel, len(offs))
+    ok = np.zeros(len(parents), dtype=bool)
+    psize = dom.cell_sizes(parents)
+    plo = np.rint((dom.cell_lowers(parents) - origin) / unit).astype(np.int64)
+    pext = np.rint(psize / unit).astype(np.int64)
+    csize = dom.cell_sizes(firsts)
+    for e in np.unique(pext, axis=0):
+        sel = np.nonzero(np.all(pext == e, axis=1))[0]
+        box = np.stack(np.meshgrid(*[np.arange(-1, int(v) + 1) for v in e], indexing="ij"), axis=-1).reshape(-1, D)
+        inner = np.all((box >= 0) & (box < e), axis=1)
+        shell = box[~inner]
+        cells = plo[sel][:, None, :] + shell[None, :, :] + 1
+        cells = np.clip(cells, 0, shape - 1)
+        nb = owner[tuple(cells.reshape(-1, D).T)].reshape(len(sel), len(shell))
+        nb_size = np.where(nb[..., None] >= 0, sizes[np.maximum(nb, 0)], csize[sel][:, None, :])
+        ok[sel] = np.all(np.isclose(nb_size, csize[sel][:, None, :], rtol=1e-12, atol=0.0), axis=(1, 2))
+    return ok
+
+
 def h_coarsened_space(space):
     parents, firsts = h_coarsening_candidates(space)
+    if parents.size and getattr(space.domain, "classify", None) is not None:
+        keep = _balanced(space, parents, firsts)
+        parents, firsts = parents[keep], firsts[keep]
     if parents.size == 0:
         return None
     index = space.leaf_index
