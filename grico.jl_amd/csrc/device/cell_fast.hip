@@ -255,6 +255,31 @@ __host__ __device__ inline int lat(int m) {
   return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1);
 }
 
+// LDS placement of the brick lattice: position (x, y, z) at x + PY y + PZ z.
+// Row and plane strides may exceed 2N - 1 (padding) so that the lanes of one
+// wavefront, which gather rows of different cells and planes, spread over more
+// LDS banks.
+#ifndef GMF_BRICK_PAD_Y
+#define GMF_BRICK_PAD_Y 0
+#endif
+#ifndef GMF_BRICK_PAD_Z
+#define GMF_BRICK_PAD_Z 0
+#endif
+template <int N>
+struct BrickLds {
+  static constexpr int M = 2 * N - 1;
+  static constexpr int PY = M + GMF_BRICK_PAD_Y;
+  static constexpr int PZ = PY * M + GMF_BRICK_PAD_Z;
+  static constexpr int SIZE = PZ * (M - 1) + PY * (M - 1) + M;  // elements spanned
+  __device__ static inline int at(int x, int y, int z) { return x + PY * y + PZ * z; }
+  // LDS offset of flattened lattice position p = x + M (y + M z).
+  __device__ static inline int of(int p) {
+    if (PY == M && PZ == M * M) return p;
+    const int x = p % M, yz = p / M;
+    return x + PY * (yz % M) + PZ * (yz / M);
+  }
+};
+
 // Upper bound of the shared positions of one brick (its lattice surface).
 template <int N>
 constexpr int brick_surface() {
@@ -282,9 +307,10 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   extern __shared__ double smem_words[];
   __shared__ int shared_count[PER * WAVES];
   __shared__ int32_t slots[SURF];
-  R* xl = reinterpret_cast<R*>(smem_words);  // L input modes
-  R* acc = xl + L;                            // L accumulated results
-  R* scratch = acc + L;                       // 8 x 2 x N^3 (per cell A, B)
+  using LD = BrickLds<N>;
+  R* xl = reinterpret_cast<R*>(smem_words);  // lattice of input modes (LD layout)
+  R* acc = xl + LD::SIZE;                     // lattice of accumulated results
+  R* scratch = acc + LD::SIZE;                // 8 x 2 x N^3 (per cell A, B)
   const int tid = threadIdx.x;
   const int lc = tid / LPC, lane_in_cell = tid % LPC;
   const int wave = tid >> 6, lane = tid & 63;
@@ -338,8 +364,9 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   for (int k = 0; k < PER; ++k) {
     const int p = tid + k * THREADS;
     if (p < L) {
-      xl[p] = xv[k];
-      acc[p] = R(0);
+      const int o = LD::of(p);
+      xl[o] = xv[k];
+      acc[o] = R(0);
     }
   }
   __syncthreads();
@@ -357,13 +384,13 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   {
     const int lx = ox + lat<N>(t0), ly = oy + lat<N>(t1);
 #pragma unroll
-    for (int k = 0; k < N; ++k) u[k] = xl[lx + M * (ly + M * (oz + lat<N>(k)))];
+    for (int k = 0; k < N; ++k) u[k] = xl[LD::at(lx, ly, oz + lat<N>(k))];
   }
   cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, Bc, t0, t1, act, r);
   if (act) {
     const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
 #pragma unroll
-    for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[ox + lat<N>(ii) + M * (py + M * pz)], r[ii]);
+    for (int ii = 0; ii < N; ++ii) atomicAdd(&acc[LD::at(ox + lat<N>(ii), py, pz)], r[ii]);
   }
   // Ranks of the shared positions, counted per (round, wavefront).
   uint64_t fm[PER];
@@ -378,13 +405,14 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int p = tid + k * THREADS;
+    const int o = LD::of(p < L ? p : 0);
     if (e[k] & 1) {
       const int32_t d = e[k] >> 1;
-      if (d >= 0) store(d, acc[p], xl[p]);
+      if (d >= 0) store(d, acc[o], xl[o]);
     } else {
       int off = 0;
       for (int j = 0; j < k * WAVES + wave; ++j) off += shared_count[j];
-      a.halo[slots[off + __popcll(fm[k] & below)]] = acc[p];
+      a.halo[slots[off + __popcll(fm[k] & below)]] = acc[o];
     }
   }
 }
@@ -459,8 +487,7 @@ void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, int threads, 
 
 template <int N, class R>
 size_t brick_lds_bytes() {
-  constexpr int M = 2 * N - 1;
-  return sizeof(R) * size_t(2 * M * M * M + 8 * 2 * N * N * N);
+  return sizeof(R) * size_t(2 * BrickLds<N>::SIZE + 8 * 2 * N * N * N);
 }
 
 template <int N, class R, class Store>

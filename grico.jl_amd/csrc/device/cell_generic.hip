@@ -305,12 +305,16 @@ __global__ void __launch_bounds__(256) cell_interp_generic(GenericArgs a, const 
   }
 }
 
-// out[k] = sum_c partial[2 c + k] in a fixed order (one workgroup).
-__global__ void __launch_bounds__(256) k_sum_pairs(const double* __restrict__ partial, int64_t cells,
-                                                   double* __restrict__ out) {
+// Column sums of partial (cells x 2) in a fixed order, in two stages: stage 1
+// sums workgroup b's contiguous chunk of `chunk` cells and writes its two sums
+// over the chunk's first pair (read before, by the same workgroup); stage 2
+// (one workgroup, step = chunk) sums those chunk results in chunk order.
+__global__ void __launch_bounds__(256) k_sum_pairs(double* __restrict__ partial, int64_t cells, int64_t chunk,
+                                                   int64_t step, double* __restrict__ out) {
   __shared__ double red[2 * 256];
+  const int64_t lo = blockIdx.x * chunk, hi = lo + chunk < cells ? lo + chunk : cells;
   double s0 = 0.0, s1 = 0.0;
-  for (int64_t c = threadIdx.x; c < cells; c += blockDim.x) {
+  for (int64_t c = lo + threadIdx.x * step; c < hi; c += int64_t(blockDim.x) * step) {
     s0 += partial[2 * c];
     s1 += partial[2 * c + 1];
   }
@@ -325,8 +329,13 @@ __global__ void __launch_bounds__(256) k_sum_pairs(const double* __restrict__ pa
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = red[0];
-    out[1] = red[256];
+    if (out) {
+      out[0] = red[0];
+      out[1] = red[256];
+    } else if (lo < hi) {
+      partial[2 * lo] = red[0];
+      partial[2 * lo + 1] = red[256];
+    }
   }
 }
 
@@ -560,8 +569,17 @@ void launch_cell_interp(const Bucket& b, const double* u, int channels, double* 
   GMF_HIP(hipGetLastError());
 }
 
-void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream) {
-  hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, stream, partial, cells, out);
+void launch_sum_pairs(double* partial, int64_t cells, double* out, hipStream_t stream) {
+  // chunks of at least 4096 cells, at most 1024 of them
+  const int64_t chunk = std::max<int64_t>(4096, (cells + 1023) / 1024);
+  const int64_t blocks = std::max<int64_t>(1, (cells + chunk - 1) / chunk);
+  if (blocks > 1) {
+    hipLaunchKernelGGL(k_sum_pairs, dim3(unsigned(blocks)), dim3(256), 0, stream, partial, cells, chunk, int64_t(1),
+                       static_cast<double*>(nullptr));
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, stream, partial, cells, cells, chunk, out);
+  } else {
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, stream, partial, cells, cells, int64_t(1), out);
+  }
   GMF_HIP(hipGetLastError());
 }
 

@@ -243,8 +243,9 @@ void launch_cell_l2(const Bucket& b, const double* u, const double* w, const dou
 // Value (+ physical gradient) of full-layout u at every point of one tensor
 // bucket: cells x Q x channels (channels 1 or 1 + D).
 void launch_cell_interp(const Bucket& b, const double* u, int channels, double* out, hipStream_t stream);
-// out[0..1] = column sums of partial (cells x 2) in a fixed order.
-void launch_sum_pairs(const double* partial, int64_t cells, double* out, hipStream_t stream);
+// out[0..1] = column sums of partial (cells x 2) in a fixed order; partial is
+// overwritten (its chunk sums).
+void launch_sum_pairs(double* partial, int64_t cells, double* out, hipStream_t stream);
 // Plan-time autotuner (tune.hip): at plan creation every bucket's kernel
 // launch shape is timed over its candidates on scratch vectors (HIP events,
 // one warm-up and `reps` timed launches each) and the fastest is kept.

@@ -591,12 +591,12 @@ def assembled_reduced_matrix(plan):
     chi_q, chi = (value, physical gradient) of the full basis at the points and
     M_q the accumulators probed at the points (the reference's basis path,
     src/accumulators.jl:1022-1054, 1133-1161), scattered through the term maps.
-    Tensor cells, point-cloud cells and embedded-surface pieces; selection
-    reduced maps (the reduced dofs of a plan without constraint rows)."""
+    Tensor cells, point-cloud cells and embedded-surface pieces; P is the
+    plan's reduced map (a selection: fixed dofs drop out with their rows and
+    columns; else the coefficient CSR of the affine and mean relations)."""
     import scipy.sparse as sps
-    if not getattr(plan, "selection_map", True) or len(plan.fixed_dofs) or getattr(plan, "boundary_buckets", []) \
-            or getattr(plan, "interface_buckets", []):
-        raise ValueError("assembled_reduced_matrix covers cells and surface pieces on selection maps")
+    if getattr(plan, "boundary_buckets", []) or getattr(plan, "interface_buckets", []):
+        raise ValueError("assembled_reduced_matrix covers cells and surface pieces")
     n = plan.full_dof_count
     D = plan.dim
     A = sps.csr_matrix((n, n))
@@ -630,8 +630,12 @@ def assembled_reduced_matrix(plan):
         S = sps.csr_matrix((bk.term_coef, (bk.term_cell * bk.n_modes + bk.term_box, bk.term_dof)),
                            shape=(bk.cell_count * bk.n_modes, n))
         A = A + S.T @ sps.block_diag(list(E), format="csr") @ S
-    sd = plan.solve_dofs
-    return sps.csr_matrix(A)[sd][:, sd]
+    if getattr(plan, "selection_map", True):
+        sd = plan.solve_dofs
+        return sps.csr_matrix(A)[sd][:, sd]
+    rows = np.repeat(np.arange(n), np.diff(plan.map_offsets))
+    Pm = sps.csr_matrix((plan.map_coefficients, (rows, plan.map_indices)), shape=(n, plan.reduced_dof_count))
+    return sps.csr_matrix(Pm.T @ A @ Pm)
 
 
 def _chebyshev(A, dinv, x, b, x_zero, lam_max, degree, rng):
