@@ -223,6 +223,74 @@ __global__ void __launch_bounds__(THREADS) cell_apply_fast(CellArgs<R> a, EOTabl
   }
 }
 
+// ---- verification sums on fast buckets ------------------------------------------
+
+// Verification sums of one cell (src/verification.jl:201-230): partial[2 c] =
+// sum_q w_q (u_h(x_q) - e_q)^2, partial[2 c + 1] = sum_q w_q e_q^2, u_h
+// interpolated from the full-layout coefficients (3 passes); the cell's lanes
+// combine their sums by a fixed butterfly (deterministic).
+template <int N, bool COEF>
+__global__ void __launch_bounds__(256) cell_l2_fast(CellArgs<double> a, EOTables<N, double> T,
+                                                    const double* __restrict__ u, const double* __restrict__ w,
+                                                    const double* __restrict__ ex, double* __restrict__ partial) {
+  constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>(), CPB = 256 / LPC;
+  __shared__ double S1[CPB * NNN];
+  __shared__ double S2[CPB * NNN];
+  const int lc = threadIdx.x / LPC, lane_in_cell = threadIdx.x % LPC;
+  int t0, t1;
+  bool act;
+  lane_coords<N>(lane_in_cell, t0, t1, act);
+  const int64_t cell = xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
+  const bool live = cell < a.cells;
+  act = act && live;
+  const int64_t c = live ? cell : 0;
+  const int64_t base = c * NNN;
+  double* A = S1 + lc * NNN;
+  double* B = S2 + lc * NNN;
+  double v[N], t[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int e = t0 + N * t1 + NN * k;
+    const int32_t gd = a.idx[base + e];
+    v[k] = gd >= 0 ? (COEF ? a.lcoef[base + e] * u[gd] : u[gd]) : 0.0;
+  }
+  eo_interp<N>(T, v, t);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (act) A[t0 + N * t1 + NN * q] = t[q];
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = A[t0 + N * j + NN * t1];
+  eo_interp<N>(T, v, t);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if (act) B[t0 + N * q + NN * t1] = t[q];
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = B[i + N * t0 + NN * t1];
+  eo_interp<N>(T, v, t);
+  double s0 = 0.0, s1 = 0.0;
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const int64_t pq = base + q + N * t0 + NN * t1;
+      const double ev = ex[pq], wq = w[pq];
+      const double d = t[q] - ev;
+      s0 = fma(wq * d, d, s0);
+      s1 = fma(wq * ev, ev, s1);
+    }
+  }
+#pragma unroll
+  for (int off = LPC / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, LPC);
+    s1 += __shfl_xor(s1, off, LPC);
+  }
+  if (live && lane_in_cell == 0) {
+    partial[2 * c] = s0;
+    partial[2 * c + 1] = s1;
+  }
+}
+
 // ---- brick driver --------------------------------------------------------------
 
 template <class R>
@@ -834,6 +902,34 @@ bool prepare_fp32(gmf_plan& p) {
   GMF_HIP(hipDeviceSynchronize());
   p.fp32_state = all ? 1 : -1;
   return all;
+}
+
+bool launch_cell_l2_fast(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
+                         hipStream_t stream) {
+  if (!b.fast || b.cells == 0) return false;
+  auto go = [&](auto nc) {
+    constexpr int N = decltype(nc)::value;
+    constexpr int CPB = 256 / lanes_per_cell<N>();
+    EOTables<N, double> T = tables_of<N, double>(b);
+    CellArgs<double> a{b.cells, b.jinv, nullptr, b.lidx, b.lcoef};
+    const dim3 grid(unsigned((b.cells + CPB - 1) / CPB));
+    if (b.lcoef)
+      hipLaunchKernelGGL((cell_l2_fast<N, true>), grid, dim3(256), 0, stream, a, T, u, w, e, partial);
+    else
+      hipLaunchKernelGGL((cell_l2_fast<N, false>), grid, dim3(256), 0, stream, a, T, u, w, e, partial);
+    GMF_HIP(hipGetLastError());
+    return true;
+  };
+  switch (b.N[0]) {
+    case 2: return go(std::integral_constant<int, 2>());
+    case 3: return go(std::integral_constant<int, 3>());
+    case 4: return go(std::integral_constant<int, 4>());
+    case 5: return go(std::integral_constant<int, 5>());
+    case 6: return go(std::integral_constant<int, 6>());
+    case 7: return go(std::integral_constant<int, 7>());
+    case 8: return go(std::integral_constant<int, 8>());
+    default: return false;
+  }
 }
 
 bool plan_writes_every_dof(const gmf_plan& p) {

@@ -551,6 +551,7 @@ void launch_cell_rhs(const Bucket& b, const double* loads, int channels, double*
 void launch_cell_l2(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
                     hipStream_t stream) {
   if (b.cells == 0) return;
+  if (launch_cell_l2_fast(b, u, w, e, partial, stream)) return;
   GenericArgs a = make_args(b, false);
   const int threads = 64;
   const size_t lds = sizeof(double) * size_t(3 * a.M + a.Qtot + 2 * threads);

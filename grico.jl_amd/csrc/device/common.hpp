@@ -243,6 +243,11 @@ void launch_cell_l2(const Bucket& b, const double* u, const double* w, const dou
 // Value (+ physical gradient) of full-layout u at every point of one tensor
 // bucket: cells x Q x channels (channels 1 or 1 + D).
 void launch_cell_interp(const Bucket& b, const double* u, int channels, double* out, hipStream_t stream);
+// Sum-factorized verification sums on fast buckets (3D, N = Q, isotropic
+// tables): false when the bucket has no fast path (the caller then runs the
+// generic kernel).
+bool launch_cell_l2_fast(const Bucket& b, const double* u, const double* w, const double* e, double* partial,
+                         hipStream_t stream);
 // out[0..1] = column sums of partial (cells x 2) in a fixed order; partial is
 // overwritten (its chunk sums).
 void launch_sum_pairs(double* partial, int64_t cells, double* out, hipStream_t stream);
