@@ -59,7 +59,7 @@ __device__ inline int64_t xcd_block(int64_t b, int64_t n) {
 // Lanes per cell: N*N rounded up to a power of two (<= 64).
 template <int N>
 constexpr int lanes_per_cell() {
-  return N * N <= 4 ? 4 : N * N <= 16 ? 16 : N * N <= 32 ? 32 : 64;
+  return N * N <= 4 ? 4 : N * N <= 16 ? 16 : N * N <= 32 ? 32 : N * N <= 64 ? 64 : 128;
 }
 
 // Order LDS traffic of this wavefront: no compiler motion across, and the
@@ -68,6 +68,17 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ordering of a cell's LDS transposes: wavefront order when the cell lives in
+// one wavefront, a workgroup barrier when it spans two (N = 9: 81 pencils;
+// every lane of the workgroup then reaches each barrier).
+template <int N>
+__device__ inline void cell_sync() {
+  if constexpr (lanes_per_cell<N>() > 64)
+    __syncthreads();
+  else
+    wave_sync();
 }
 
 // Per-cell body. In: u = z-pencil of modes (i=t0, j=t1, k); cx, cy, cz =
@@ -89,7 +100,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
 #pragma unroll
   for (int q = 0; q < N; ++q)
     if (act) A[t0 + N * t1 + NN * q] = a[q];
-  wave_sync();
+  cell_sync<N>();
   // y: lane (i=t0, qz=t1), B = (i, qy, qz).
 #pragma unroll
   for (int j = 0; j < N; ++j) a[j] = A[t0 + N * j + NN * t1];
@@ -97,7 +108,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
 #pragma unroll
   for (int q = 0; q < N; ++q)
     if (act) B[t0 + N * q + NN * t1] = b[q];
-  wave_sync();
+  cell_sync<N>();
   // x: lane (qy=t0, qz=t1): point values to A; x channel f = W c J^-2 dU/dxi
   // projected back with D^T, kept in B (x layout).
 #pragma unroll
@@ -113,7 +124,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
 #pragma unroll
   for (int s = 0; s < N; ++s)
     if (act) B[s + N * t0 + NN * t1] = b[s];
-  wave_sync();
+  cell_sync<N>();
   // y channel on the y-pencil (qx=t0, qz=t1), added into B in place.
   {
 #pragma unroll
@@ -136,7 +147,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
     for (int q = 0; q < N; ++q) b[q] *= cz[q] * jz2;
     eo_deriv_t<N>(T, b, h);
   }
-  wave_sync();
+  cell_sync<N>();
 #pragma unroll
   for (int s = 0; s < N; ++s) h[s] += B[t0 + N * t1 + NN * s];
   // Transposed z: modes k at (qx=t0, qy=t1) into A.
@@ -144,7 +155,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
 #pragma unroll
   for (int k = 0; k < N; ++k)
     if (act) A[t0 + N * t1 + NN * k] = a[k];
-  wave_sync();
+  cell_sync<N>();
   // Transposed y: lane (qx=t0, k=t1) -> modes j into B.
 #pragma unroll
   for (int q = 0; q < N; ++q) a[q] = A[t0 + N * q + NN * t1];
@@ -152,7 +163,7 @@ __device__ inline void cell_body(const EOTables<N, R>& T, const R (&u)[N], const
 #pragma unroll
   for (int j = 0; j < N; ++j)
     if (act) B[t0 + N * j + NN * t1] = b[j];
-  wave_sync();
+  cell_sync<N>();
   // Transposed x: lane (j=t0, k=t1) -> modes i.
 #pragma unroll
   for (int q = 0; q < N; ++q) a[q] = B[q + N * t0 + NN * t1];
@@ -258,14 +269,14 @@ __global__ void __launch_bounds__(256) cell_l2_fast(CellArgs<double> a, EOTables
 #pragma unroll
   for (int q = 0; q < N; ++q)
     if (act) A[t0 + N * t1 + NN * q] = t[q];
-  wave_sync();
+  cell_sync<N>();
 #pragma unroll
   for (int j = 0; j < N; ++j) v[j] = A[t0 + N * j + NN * t1];
   eo_interp<N>(T, v, t);
 #pragma unroll
   for (int q = 0; q < N; ++q)
     if (act) B[t0 + N * q + NN * t1] = t[q];
-  wave_sync();
+  cell_sync<N>();
 #pragma unroll
   for (int i = 0; i < N; ++i) v[i] = B[i + N * t0 + NN * t1];
   eo_interp<N>(T, v, t);
@@ -280,14 +291,34 @@ __global__ void __launch_bounds__(256) cell_l2_fast(CellArgs<double> a, EOTables
       s1 = fma(wq * ev, ev, s1);
     }
   }
+  if constexpr (LPC <= 64) {
 #pragma unroll
-  for (int off = LPC / 2; off > 0; off >>= 1) {
-    s0 += __shfl_xor(s0, off, LPC);
-    s1 += __shfl_xor(s1, off, LPC);
-  }
-  if (live && lane_in_cell == 0) {
-    partial[2 * c] = s0;
-    partial[2 * c + 1] = s1;
+    for (int off = LPC / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, LPC);
+      s1 += __shfl_xor(s1, off, LPC);
+    }
+    if (live && lane_in_cell == 0) {
+      partial[2 * c] = s0;
+      partial[2 * c + 1] = s1;
+    }
+  } else {
+    // cell over two wavefronts: butterfly in each, then the two halves in order
+    __shared__ double half[CPB][2][2];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, 64);
+      s1 += __shfl_xor(s1, off, 64);
+    }
+    const int hw = lane_in_cell / 64;
+    if (lane_in_cell % 64 == 0) {
+      half[lc][hw][0] = s0;
+      half[lc][hw][1] = s1;
+    }
+    __syncthreads();
+    if (live && lane_in_cell == 0) {
+      partial[2 * c] = half[lc][0][0] + half[lc][1][0];
+      partial[2 * c + 1] = half[lc][0][1] + half[lc][1][1];
+    }
   }
 }
 
@@ -545,9 +576,13 @@ void launch_cells_shape(const Bucket& b, const R* x, R* y, bool reduced, hipStre
 template <int N, class R>
 void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, int threads, hipStream_t stream) {
   constexpr int LPC = lanes_per_cell<N>();
-  if (threads <= 64 && LPC <= 64)
-    launch_cells_shape<N, R, 64>(b, x, y, reduced, stream);
-  else if (threads <= 128)
+  if constexpr (LPC <= 64) {
+    if (threads <= 64) {
+      launch_cells_shape<N, R, 64>(b, x, y, reduced, stream);
+      return;
+    }
+  }
+  if (threads <= 128)
     launch_cells_shape<N, R, 128>(b, x, y, reduced, stream);
   else
     launch_cells_shape<N, R, 256>(b, x, y, reduced, stream);
@@ -721,7 +756,7 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   b.fast = false;
   if (b.dim != 3 || b.form != GMF_FORM_DIFFUSION || b.multi_count != 0) return;
   const int n = b.N[0];
-  if (n < 2 || n > 8) return;
+  if (n < 2 || n > 9) return;
   for (int a = 0; a < 3; ++a)
     if (b.N[a] != n || b.Q[a] != n) return;
   // The three axes must share one table (isotropic degree and rule).
@@ -757,6 +792,7 @@ bool launch_cells_n(const Bucket& b, const R* x, R* y, bool reduced, int threads
     case 6: launch_cells<6, R>(b, x, y, reduced, threads, stream); return true;
     case 7: launch_cells<7, R>(b, x, y, reduced, threads, stream); return true;
     case 8: launch_cells<8, R>(b, x, y, reduced, threads, stream); return true;
+    case 9: launch_cells<9, R>(b, x, y, reduced, threads, stream); return true;
     default: return false;
   }
 }
@@ -928,6 +964,7 @@ bool launch_cell_l2_fast(const Bucket& b, const double* u, const double* w, cons
     case 6: return go(std::integral_constant<int, 6>());
     case 7: return go(std::integral_constant<int, 7>());
     case 8: return go(std::integral_constant<int, 8>());
+    case 9: return go(std::integral_constant<int, 9>());
     default: return false;
   }
 }
