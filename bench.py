@@ -636,6 +636,32 @@ def sipg_leg(cells=32, degree=3, reps=20, cpu_cells=8):
     out.update({"gmg_cg_dense_coarse_seconds": time.perf_counter() - t0,
                 "gmg_cg_dense_coarse_iterations": int(info.iterations),
                 "gmg_dense_coarse_levels": ht.reduced_dof_counts()})
+    # One level per 2x2x2 coarsening (h_steps_per_level = 3) instead of one per
+    # binary merge, dense coarse level: far fewer latency-bound DG levels.
+    t0 = time.perf_counter()
+    pol3 = GeometricMultigridPreconditioner(smoother="chebyshev", h_steps_per_level=3, chebyshev_degree=3,
+                                            coarse_direct_dof_limit=4096)
+    h3 = compile_geometric_multigrid(build_sipg_problem(cells, degree), pol3, fine_plan=plan)
+    counts3 = h3.reduced_dof_counts()
+    keep3 = len(counts3) - max(i for i, c in enumerate(counts3) if c <= 4096)
+    h3 = h3.with_policy(GeometricMultigridPreconditioner(smoother="chebyshev", h_steps_per_level=3,
+                                                         chebyshev_degree=3, max_levels=keep3,
+                                                         coarse_direct_dof_limit=4096))
+    M3 = h3.device_preconditioner()
+    t_setup3 = time.perf_counter() - t0
+    solve_reduced(plan, g.CGSolver(preconditioner=M3), b, 1e-8, 0.0, 2000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x3, info = solve_reduced(plan, g.CGSolver(preconditioner=M3), b, 1e-8, 0.0, 2000)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter() - t0
+    full = ops.expand_reduced(plan, x3, include_shift=True).cpu().numpy()
+    out.update({"gmg_cg_merged_levels_seconds": t3, "gmg_cg_merged_levels_iterations": int(info.iterations),
+                "gmg_merged_levels": h3.reduced_dof_counts(), "gmg_merged_setup_seconds": t_setup3,
+                "gmg_cg_merged_levels_relative_l2_error": g.relative_l2_error(g.State(plan, full), _SIPGFaces.exact),
+                "gmg_cg_merged_levels_speedup_vs_jacobi_cg": out["solve_seconds"] / t3})
+    log(f"sipg GMG-CG (one level per 2x2x2 coarsening): {int(info.iterations)} iterations, {t3:.3f} s "
+        f"(Jacobi-CG {out['solve_seconds']:.3f} s)")
     # Hanging interfaces: the same operator on 16^3 cells with the cells of the
     # octant [0, 1/2]^3 refined once in every axis.
     hang = build_sipg_problem(16, degree, interfaces_only=True)
