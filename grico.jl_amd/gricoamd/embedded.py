@@ -487,6 +487,7 @@ def locate_points(domain, leaves, x):
     if len(leaves) == 0 or len(x) == 0:
         return out
     origin = np.asarray(domain.origin, dtype=np.float64)
+    extent = np.asarray(domain.extent, dtype=np.float64)
     sizes = domain.cell_sizes(leaves)
     lowers = domain.cell_lowers(leaves)
     groups = {}
@@ -503,8 +504,9 @@ def locate_points(domain, leaves, x):
         order = np.argsort(code)
         codes = code[order]
         kp = np.floor((x - origin) / h).astype(np.int64)
-        # a point on the upper face of the last leaf of a row belongs to that leaf
-        kp = np.minimum(kp, k.max(axis=0))
+        # a point on the domain's upper face belongs to the last cell of the
+        # lattice at this size (interior faces belong to the upper cell)
+        kp = np.minimum(kp, np.rint(extent / h).astype(np.int64) - 1)
         ok = np.all(kp >= 0, axis=1) & (out < 0)
         pc = np.zeros(len(x), dtype=np.int64)
         for a in range(len(h)):

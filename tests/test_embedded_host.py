@@ -71,3 +71,114 @@ def test_cut_rules_attach():
     assert n > 0 and len(pr.cell_quadratures) == len(cut)
     plan = g.compile(pr)
     assert sum(b.cell_count for b in plan.cloud_buckets) == len(cut)
+
+
+# ---- physical domains, sampled surfaces, point location --------------------------
+
+def test_locate_points_on_mixed_leaf_sizes():
+    from gricoamd.embedded import locate_points
+    dom = g.Domain((0.0, 0.0, 0.0), (2.0, 1.0, 1.0), (2, 1, 1))
+    first = dom.refine(1, 0)
+    dom.refine(first, 1)
+    leaves = np.asarray(dom.active_leaves())
+    rng = np.random.default_rng(3)
+    x = rng.uniform([0, 0, 0], [2, 1, 1], size=(400, 3))
+    owner = locate_points(dom, leaves, x)
+    assert np.all(owner >= 0)
+    lo = dom.cell_lowers(leaves)[owner]
+    hi = lo + dom.cell_sizes(leaves)[owner]
+    assert np.all((x >= lo - 1e-12) & (x <= hi + 1e-12))
+    # a brute-force check of ownership
+    for p_, o in zip(x[:50], owner[:50]):
+        inside = np.all((p_ >= dom.cell_lowers(leaves)) & (p_ < dom.cell_lowers(leaves) + dom.cell_sizes(leaves)), axis=1)
+        assert inside[o]
+    # points outside the leaf set are not located; the domain's upper faces are
+    out = locate_points(dom, leaves[:1], np.array([[1.5, 0.5, 0.5], [0.2, 0.2, 0.2]]))
+    assert out[0] == -1 and out[1] == 0
+    assert locate_points(dom, leaves, np.array([[2.0, 1.0, 1.0]]))[0] >= 0
+
+
+def test_sampled_surface_pieces_on_mixed_leaf_sizes():
+    # every sample lands inside its piece's leaf (nothing is clipped onto a
+    # face), and the pieces keep every sample once
+    from gricoamd.embedded import sphere_surface
+    dom = g.Domain((-1.2,) * 3, (2.4,) * 3, (3, 3, 3))
+    for axis in range(3):
+        dom.refine(int(dom.active_leaves()[0]), axis)
+    dom.refine(int(dom.active_leaves()[13]), 0)
+    sp = g.HpSpace(dom, g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(1)))
+    surface = sphere_surface((0.05, -0.1, 0.0), 1.0, n_theta=12, n_phi=24)
+    pieces = surface.surface_quadratures(sp)
+    leaves = list(np.asarray(sp.leaves))
+    got = []
+    for piece in pieces:
+        li = leaves.index(piece.leaf)
+        lo = dom.cell_lowers([piece.leaf])[0]
+        half = 0.5 * dom.cell_sizes([piece.leaf])[0]
+        got.append(lo + half * (piece.quadrature.points + 1.0))
+    got = np.concatenate(got)
+    assert len(got) == len(surface.points)
+    key = lambda a: np.lexsort(a.T[::-1])
+    assert np.allclose(got[key(got)], surface.points[key(surface.points)], atol=1e-12)
+
+
+def test_finite_cell_extension_rules_and_checks():
+    from gricoamd.embedded import (FiniteCellExtension, ImplicitRegion, PhysicalDomain, PhysicalMeasure, CUT,
+                                   _tensor_rule)
+    with pytest.raises(ValueError):
+        FiniteCellExtension(1.5)
+    with pytest.raises(ValueError):
+        FiniteCellExtension(float("nan"))
+    region = ImplicitRegion(lambda p: p[0] ** 2 + p[1] ** 2 + p[2] ** 2 - 1.0, subdivision_depth=1)
+    dom = g.Domain((-1.2,) * 3, (2.4,) * 3, (3, 3, 3))
+    opts = g.SpaceOptions(basis=g.FullTensorBasis(), degree=g.UniformDegree(1))
+    phys = PhysicalDomain(dom, region, PhysicalMeasure())
+    ext = PhysicalDomain(dom, region, FiniteCellExtension(0.25))
+    full = PhysicalDomain(dom, region, FiniteCellExtension(1.0))
+    sp = g.HpSpace(phys, opts)
+    # outside leaves are dropped, and the space's leaves agree across measures
+    assert np.all(phys.classify(sp.leaves) != 1)
+    rp, re_ = phys.assembly_cell_quadratures(sp), ext.assembly_cell_quadratures(g.HpSpace(ext, opts))
+    assert set(rp) == set(re_) and len(rp) > 0
+    assert not full.assembly_cell_quadratures(g.HpSpace(full, opts))  # alpha = 1: the tensor rule everywhere
+    fp, fw = _tensor_rule((2, 2, 2))
+    for leaf, r in rp.items():
+        e = re_[leaf]
+        npnt = r.point_count()
+        assert np.allclose(e.weights[:npnt], 0.75 * r.weights) and np.allclose(e.weights[npnt:], 0.25 * fw)
+        assert np.allclose(e.points[npnt:], fp)
+    # rules are cached per leaf and shape
+    leaf = next(iter(rp))
+    assert phys.physical_rules([leaf], (2, 2, 2))[leaf] is phys.physical_rules([leaf], (2, 2, 2))[leaf]
+
+
+def test_conflict_coloring_is_valid():
+    import ctypes
+    from gricoamd import _native
+    lib = _native.host()
+    lib.gh_conflict_coloring.argtypes = [ctypes.c_int64, _native.P_i64, _native.P_i64, ctypes.c_int64,
+                                         _native.P_i64, _native.P_i64, ctypes.c_int64, _native.P_i32,
+                                         ctypes.POINTER(ctypes.c_int32)]
+    rng = np.random.default_rng(5)
+    ndof = 300
+    items = [np.unique(rng.integers(0, ndof, size=int(rng.integers(1, 9)))) for _ in range(120)]
+    blocks = [np.unique(rng.integers(0, ndof, size=int(rng.integers(1, 6)))) for _ in range(80)]
+    iptr = np.concatenate([[0], np.cumsum([len(i) for i in items])]).astype(np.int64)
+    idofs = np.concatenate(items).astype(np.int64)
+    bptr = np.concatenate([[0], np.cumsum([len(b) for b in blocks])]).astype(np.int64)
+    bdofs = np.concatenate(blocks).astype(np.int64)
+    colors = np.zeros(len(blocks), dtype=np.int32)
+    count = ctypes.c_int32()
+    _native.check_host(lib.gh_conflict_coloring(len(blocks), _native.ptr(bptr, _native.c_int64),
+                                                _native.ptr(bdofs, _native.c_int64), len(items),
+                                                _native.ptr(iptr, _native.c_int64), _native.ptr(idofs, _native.c_int64),
+                                                ndof, _native.ptr(colors, _native.c_int32), ctypes.byref(count)))
+    assert colors.max() + 1 == count.value
+    for it in items:
+        touched = [b for b, bl in enumerate(blocks) if np.intersect1d(it, bl).size]
+        assert len({int(colors[b]) for b in touched}) == len(touched)
+    # greedy: every block's color is the smallest one its conflicting predecessors leave free
+    for b in range(len(blocks)):
+        conflicting = {int(colors[o]) for o in range(b) for it in items
+                       if np.intersect1d(it, blocks[b]).size and np.intersect1d(it, blocks[o]).size}
+        assert colors[b] == min(set(range(len(blocks) + 1)) - conflicting)
