@@ -379,21 +379,15 @@ struct BrickLds {
   }
 };
 
-// Upper bound of the shared positions of one brick (its lattice surface).
-template <int N>
-constexpr int brick_surface() {
-  constexpr int M = 2 * N - 1;
-  return M * M * M - (M - 2) * (M - 2) * (M - 2);
-}
-
 // One workgroup per brick, all bricks in one launch, consecutive bricks on one
 // XCD so that neighbours share its L2. Phase 1 issues the lattice entries, the
 // cells' coefficients (each cell's N^3 values loaded contiguously by its lanes
-// and staged in LDS), the brick's halo slot list and the input gathers; phase
-// 2 runs the eight cell bodies, which add into an LDS result lattice; phase 3
-// stores the positions whose dof no other brick holds to y, and the shared
-// positions (ranked in ascending lattice position by wavefront ballots) to
-// their halo slots, which k_brick_halo sums per dof. The exclusive stores go
+// and staged in the cells' A scratch) and the input gathers (staged over the
+// cells' B scratch); phase 2 reads each lane's pencils, then runs the eight
+// cell bodies, which add into an LDS result lattice; phase 3 stores the
+// positions whose dof no other brick holds to y, and the shared positions
+// (ranked in ascending lattice position by wavefront ballots) to their halo
+// slots, which k_brick_halo sums per dof. The exclusive stores go
 // through `store` (plain result or a fused smoother step, which may update x in
 // place: x is not declared restrict).
 template <int N, class R, class Store>
@@ -401,15 +395,14 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
                                                                         const R* x, Store store) {
   constexpr int NN = N * N, NNN = N * N * N, LPC = lanes_per_cell<N>();
   constexpr int M = 2 * N - 1, L = M * M * M, THREADS = 8 * LPC, PER = (L + THREADS - 1) / THREADS;
-  constexpr int WAVES = (THREADS + 63) / 64, CPL = (NNN + LPC - 1) / LPC, SURF = brick_surface<N>();
-  constexpr int SPER = (SURF + THREADS - 1) / THREADS;
+  constexpr int WAVES = (THREADS + 63) / 64, CPL = (NNN + LPC - 1) / LPC;
   extern __shared__ double smem_words[];
   __shared__ int shared_count[PER * WAVES];
-  __shared__ int32_t slots[SURF];
   using LD = BrickLds<N>;
-  R* xl = reinterpret_cast<R*>(smem_words);  // lattice of input modes (LD layout)
-  R* acc = xl + LD::SIZE;                     // lattice of accumulated results
-  R* scratch = acc + LD::SIZE;                // 8 x 2 x N^3 (per cell A, B)
+  static_assert(LD::SIZE <= 8 * NNN, "brick input lattice must fit the cells' B scratch");
+  R* acc = reinterpret_cast<R*>(smem_words);  // lattice of accumulated results (LD layout)
+  R* scratch = acc + LD::SIZE;                // cells' A (8 x N^3), then cells' B (8 x N^3)
+  R* xl = scratch + 8 * NNN;                  // lattice of input modes over the B scratch, until phase 2 starts
   const int tid = threadIdx.x;
   const int lc = tid / LPC, lane_in_cell = tid % LPC;
   const int wave = tid >> 6, lane = tid & 63;
@@ -433,13 +426,7 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int i = lane_in_cell + k * LPC;
     cv[k] = i < NNN ? cb[i] : R(0);
   }
-  const int32_t sbase = a.halo_base[brick], scount = a.halo_base[brick + 1] - sbase;
-  int32_t sv[SPER];
-#pragma unroll
-  for (int k = 0; k < SPER; ++k) {
-    const int i = tid + k * THREADS;
-    sv[k] = i < scount ? a.halo_slot[sbase + i] : 0;
-  }
+  const int32_t* brick_slots = a.halo_slot + a.halo_base[brick];
   const R jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
   R xv[PER];
 #pragma unroll
@@ -447,17 +434,12 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int32_t d = e[k] >> 1;
     xv[k] = d >= 0 ? x[d] : R(0);
   }
-  R* A = scratch + lc * 2 * NNN;
-  R* Bc = A + NNN;  // coefficients, until the cell body first writes B
+  R* A = scratch + lc * NNN;  // coefficients, until the cell body first writes A
+  R* B = scratch + 8 * NNN + lc * NNN;
 #pragma unroll
   for (int k = 0; k < CPL; ++k) {
     const int i = lane_in_cell + k * LPC;
-    if (i < NNN) Bc[i] = cv[k];
-  }
-#pragma unroll
-  for (int k = 0; k < SPER; ++k) {
-    const int i = tid + k * THREADS;
-    if (i < scount) slots[i] = sv[k];
+    if (i < NNN) A[i] = cv[k];
   }
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
@@ -469,14 +451,14 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     }
   }
   __syncthreads();
-  // Phase 2: cell bodies. The coefficient pencils are read from B before the
-  // body's first wave-ordered write of B.
+  // Phase 2: cell bodies. Every lane reads its coefficient pencils (A) and
+  // its input pencil (lattice over B) before any cell body writes A or B.
   R cx[N], cy[N], cz[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) {
-    cx[q] = Bc[q + N * t0 + NN * t1];
-    cy[q] = Bc[t0 + N * q + NN * t1];
-    cz[q] = Bc[t0 + N * t1 + NN * q];
+    cx[q] = A[q + N * t0 + NN * t1];
+    cy[q] = A[t0 + N * q + NN * t1];
+    cz[q] = A[t0 + N * t1 + NN * q];
   }
   const int ox = (lc & 1) * (N - 1), oy = ((lc >> 1) & 1) * (N - 1), oz = ((lc >> 2) & 1) * (N - 1);
   R u[N], r[N];
@@ -485,7 +467,8 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
 #pragma unroll
     for (int k = 0; k < N; ++k) u[k] = xl[LD::at(lx, ly, oz + lat<N>(k))];
   }
-  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, Bc, t0, t1, act, r);
+  __syncthreads();
+  cell_body<N>(T, u, cx, cy, cz, jx * jx, jy * jy, jz * jz, A, B, t0, t1, act, r);
   if (act) {
     const int py = oy + lat<N>(t0), pz = oz + lat<N>(t1);
 #pragma unroll
@@ -507,11 +490,11 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
     const int o = LD::of(p < L ? p : 0);
     if (e[k] & 1) {
       const int32_t d = e[k] >> 1;
-      if (d >= 0) store(d, acc[o], xl[o]);
+      if (d >= 0) store(d, acc[o], xv[k]);
     } else {
       int off = 0;
       for (int j = 0; j < k * WAVES + wave; ++j) off += shared_count[j];
-      a.halo[slots[off + __popcll(fm[k] & below)]] = acc[o];
+      a.halo[brick_slots[off + __popcll(fm[k] & below)]] = acc[o];
     }
   }
 }
@@ -590,7 +573,7 @@ void launch_cells(const Bucket& b, const R* x, R* y, bool reduced, int threads, 
 
 template <int N, class R>
 size_t brick_lds_bytes() {
-  return sizeof(R) * size_t(2 * BrickLds<N>::SIZE + 8 * 2 * N * N * N);
+  return sizeof(R) * size_t(BrickLds<N>::SIZE + 8 * 2 * N * N * N);
 }
 
 template <int N, class R, class Store>
