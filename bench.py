@@ -1125,8 +1125,35 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, or
                 "one_column_gmg_cg_iterations": [s_[1] for s_ in seq],
                 "one_column_vs_block_solution_max_relative_difference": max(s_[2] for s_ in seq)})
     del X, B
-    # the physical load (exact solution 1 - r^2)
-    b = torch.as_tensor(plan.reduced_rhs(), device="cuda")
+    # the physical load (exact solution 1 - r^2): right-hand side on the host
+    # (lowering + projection) and on the device (host lowering of the point
+    # loads, gmf_rhs_physical + gmf_rhs_faces)
+    plan._rhs = None
+    t0 = time.perf_counter()
+    host_rhs = plan.rhs_physical()
+    t_host_rhs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dev_rhs = ops.rhs_physical(plan, as_numpy=False)
+    torch.cuda.synchronize()
+    t_dev_rhs = time.perf_counter() - t0
+    tl, ch = plan.lowered_loads()
+    fl = torch.as_tensor(plan.face_loads(), device="cuda")
+    tl = torch.as_tensor(tl, device="cuda")
+    y = torch.zeros(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if tl.numel():
+        _devabi.check(lib.gmf_rhs_physical(dev.ws, tl.data_ptr(), int(ch), y.data_ptr()))
+    _devabi.check(lib.gmf_rhs_faces(dev.ws, fl.data_ptr(), y.data_ptr()))
+    torch.cuda.synchronize()
+    t_dev_kernels = time.perf_counter() - t0
+    out.update({"rhs_host_seconds": t_host_rhs, "rhs_device_seconds_with_host_lowering": t_dev_rhs,
+                "rhs_device_kernel_seconds": t_dev_kernels,
+                "rhs_device_vs_host_max_relative_difference":
+                    float(np.abs(dev_rhs.cpu().numpy() - host_rhs).max() / np.abs(host_rhs).max())})
+    log(f"fcm right-hand side: host {t_host_rhs:.2f} s, device {t_dev_rhs:.2f} s with lowering, "
+        f"kernels {t_dev_kernels * 1e3:.2f} ms")
+    b = ops.reduced_rhs(plan, as_numpy=False)
     xs = torch.zeros(n, dtype=torch.float64, device="cuda")
     info = _devabi.SolveInfo()
     torch.cuda.synchronize()

@@ -200,6 +200,56 @@ __global__ void __launch_bounds__(kCloudLanes) cell_apply_cloud(CloudArgs a, con
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], y, false);
 }
 
+// Point loads projected onto the items' modes (src/accumulators.jl:1022-1054
+// with the load channels in place of the point action's output, 1230-1274
+// _accumulate_one_sided_rhs!, 1666-1698 interface_rhs!): for every point and
+// side the channels (W v, W g_1, .., W g_D) of the load callbacks, gradient
+// channels mapped by J^-1 onto the reference derivatives, then the
+// projection and the atomic term scatter of cell_apply_cloud. Work units as
+// in the apply.
+__global__ void __launch_bounds__(kCloudLanes) cell_rhs_cloud(CloudArgs a, const double* __restrict__ loads,
+                                                              double* __restrict__ y) {
+  extern __shared__ double smem[];
+  int64_t pb, pe;
+  const int64_t item = unit_range(a, blockIdx.x, pb, pe);
+  const int C = a.D + 1;
+  double* r = smem;
+  double* tv = r + a.Ntot;
+  double* td = tv + a.S * a.D * a.Nmax * a.ts;
+  double* ch = td + a.S * a.D * a.Nmax * a.ts;
+  const double* jinv = a.jinv + item * a.S * a.D;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) r[i] = 0.0;
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
+    const int q = threadIdx.x;
+    if (q < cnt) {
+      point_tables(a, p0 + q, q, tv, td);
+      const double* l = loads + (p0 + q) * a.S * C;
+      for (int s = 0; s < a.S; ++s) {
+        ch[(s * C) * a.chunk + q] = l[s * C];
+        for (int ax = 0; ax < a.D; ++ax)
+          ch[(s * C + 1 + ax) * a.chunk + q] = l[s * C + 1 + ax] * jinv[s * a.D + ax];
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+      int m[kMaxDim];
+      const int s = unflatten(a, i, m);
+      const double* chs = ch + s * C * a.chunk;
+      double acc = 0.0;
+      for (int qq = 0; qq < cnt; ++qq) {
+        double phi, dphi[kMaxDim];
+        basis_at(a, tv, td, s, m, qq, phi, dphi);
+        acc = fma(chs[qq], phi, acc);
+        for (int ax = 0; ax < a.D; ++ax) acc = fma(chs[(ax + 1) * a.chunk + qq], dphi[ax], acc);
+      }
+      r[i] += acc;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], y, false);
+}
+
 // Basis value and reference derivatives with the dimension known at compile
 // time (the products of basis_at in the same order, kept in registers).
 template <int D>
@@ -443,6 +493,11 @@ size_t apply_lds(const CloudArgs& a) {
                            size_t(a.S * (a.D + 1) * a.chunk));
 }
 
+size_t rhs_lds(const CloudArgs& a) {
+  return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.ts) +
+                           size_t(a.S * (a.D + 1) * a.chunk));
+}
+
 size_t diagonal_lds(const CloudArgs& a) {
   return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.ts) +
                            size_t(a.S * (a.D + 1) * (a.D + 1) * a.chunk));
@@ -665,6 +720,14 @@ void tune_cloud(gmf_plan& p, TuneTimer& t) {
     ti.param1 = b.lds_kb;
     ti.best_us = best;
   }
+}
+
+void launch_cloud_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream) {
+  if (b.cells == 0) return;
+  CloudArgs a = make_cloud_args(b, false);
+  pick_chunk(a, rhs_lds);
+  hipLaunchKernelGGL(cell_rhs_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), rhs_lds(a), stream, a, loads, y);
+  GMF_HIP(hipGetLastError());
 }
 
 void launch_cell_cloud_diagonal(const gmf_plan&, const Bucket& b, double* diag, bool reduced, hipStream_t stream) {

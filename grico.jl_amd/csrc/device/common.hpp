@@ -326,6 +326,9 @@ bool launch_face_tensor(const gmf_plan& plan, const Bucket& b, const double* x, 
 void prepare_face_batch(gmf_plan& plan);
 // Face loads (items x points x sides x (1 + D)) of one face-tensor batch added into y.
 void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
+// Point loads of a point-cloud batch off the face path projected and added
+// into y (full layout): per item, point and side (W v, W g_1, .., W g_D).
+void launch_cloud_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
 void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
                        char* done);
 // slots: the workspace's brick halo array (plan.brick_slots entries).

@@ -916,15 +916,12 @@ gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t chan
   return guarded([&] {
     require(ws && loads && y, GMF_EINVAL, "null argument");
     const gmf_plan& p = *ws->plan;
-    for (const Bucket& b : p.buckets)
-      require(!b.cloud || b.face.enabled, GMF_EUNSUPPORTED,
-              "the device right-hand side covers tensor-product cells and face-path batches (gmf_rhs_faces)");
     require_device_pointer(loads, "loads");
     require_device_pointer(y, "y");
     fill(y, 0.0, p.n_full, ws->stream);
     int64_t off = 0;
     for (const Bucket& b : p.buckets) {
-      if (b.cloud) continue;  // face loads: gmf_rhs_faces
+      if (b.cloud) continue;  // point-cloud loads: gmf_rhs_faces
       launch_cell_rhs(b, loads + off, channels, y, ws->stream);
       off += b.cells * int64_t(b.Qtot) * channels;
     }
@@ -936,14 +933,15 @@ gmf_status gmf_rhs_faces(gmf_workspace* ws, const double* loads, double* y) {
   return guarded([&] {
     require(ws && loads && y, GMF_EINVAL, "null argument");
     const gmf_plan& p = *ws->plan;
-    for (const Bucket& b : p.buckets)
-      if (b.cloud) require(b.face.enabled, GMF_EUNSUPPORTED, "device face loads cover batches on the face path");
     require_device_pointer(loads, "loads");
     require_device_pointer(y, "y");
     int64_t off = 0;
     for (const Bucket& b : p.buckets) {
       if (!b.cloud) continue;
-      launch_face_rhs(b, loads + off, y, ws->stream);
+      if (b.face.enabled)
+        launch_face_rhs(b, loads + off, y, ws->stream);
+      else
+        launch_cloud_rhs(b, loads + off, y, ws->stream);
       off += b.total_points * int64_t(b.sides) * (b.dim + 1);
     }
     sync(ws);

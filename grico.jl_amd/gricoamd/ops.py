@@ -106,19 +106,20 @@ def reduced_apply_f32(plan, x):
 
 def rhs_physical(plan, as_numpy=True):
     """Physical right-hand side on the device from the plan's lowered point
-    loads: tensor-product cells (src/assembly.jl:488-507), then boundary-face
-    and interface loads on the face path (src/accumulators.jl:1230-1274,
-    1666-1698). Plans whose loads fall outside (plan.device_rhs is False)
-    raise."""
+    loads: tensor-product cells (src/assembly.jl:488-507), then the point-cloud
+    batches' loads: cut cells, embedded surfaces, boundary faces and interfaces
+    (src/accumulators.jl:1022-1054, 1230-1274, 1666-1698). Plans with loads on
+    batches the device plan does not hold (plan.device_rhs is False) raise."""
     torch = _torch()
     if not plan.device_rhs:
-        raise ValueError("this plan's loads are not all on the device path (embedded surfaces, cut cells, or "
-                         "faces without an operator); use plan.rhs_physical()")
+        raise ValueError("this plan's loads are not all on the device path (cell loads without a cell operator "
+                         "next to face or surface batches, or faces without an operator); use plan.rhs_physical()")
     dev = _ws(plan)
     loads, channels = plan.lowered_loads()
-    lt = to_device(loads)
-    y = torch.empty(plan.full_dof_count, dtype=torch.float64, device="cuda")
-    _call("gmf_rhs_physical", dev.ws, lt.data_ptr(), int(channels), y.data_ptr())
+    y = torch.zeros(plan.full_dof_count, dtype=torch.float64, device="cuda")
+    if loads.size:  # plans without tensor-product cells have no tensor loads
+        lt = to_device(loads)
+        _call("gmf_rhs_physical", dev.ws, lt.data_ptr(), int(channels), y.data_ptr())
     faces = plan.face_loads()
     if faces.size:
         ft = to_device(faces)

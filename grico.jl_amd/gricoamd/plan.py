@@ -970,13 +970,11 @@ class AssemblyPlan:
 
     @property
     def device_rhs(self):
-        """True when every load lives where the device right-hand side reaches:
-        tensor-product cells (with the cell batches on the device) and
-        boundary faces / interfaces of batches the device plan holds."""
-        if self.cloud_buckets or self.surface_buckets:
-            return False
-        if self.problem.cell_rhs and not self.problem.cell_operators and (self.boundary_buckets or
-                                                                          self.interface_buckets):
+        """True when every load lives on a batch the device plan holds: tensor
+        and cut cells (the cell batches are on the device), embedded-surface
+        pieces, and boundary faces / interfaces with an operator."""
+        if self.problem.cell_rhs and not self.problem.cell_operators and (
+                self.boundary_buckets or self.interface_buckets or self.surface_buckets):
             return False  # the device plan then holds no cell batches
         if any(rops and not ops for (_, ops, rops) in self.boundary_buckets):
             return False
@@ -985,11 +983,14 @@ class AssemblyPlan:
         return True
 
     def face_loads(self):
-        """Point loads of the boundary-face and interface batches of the device
-        plan, in its bucket order: per item, point and side (W v, W g_1, ..,
-        W g_D) (gmf_rhs_faces)."""
+        """Point loads of the device plan's point-cloud batches, in its bucket
+        order: per item, point and side (W v, W g_1, .., W g_D) (gmf_rhs_faces):
+        cut-cell loads (cell_rhs_accumulate), embedded-surface loads
+        (surface_rhs_accumulate), boundary-face and interface loads."""
         D = self.dim
         rops_of = {id(bk): rops for (bk, _, rops) in self.boundary_buckets}
+        cut = {id(bk) for bk in self.cloud_buckets}
+        surface = {id(bk) for bk in self.surface_buckets}
         out = []
         for bk in self.all_buckets:
             if not getattr(bk, "cloud", False):
@@ -1005,8 +1006,14 @@ class AssemblyPlan:
                             arr[:, s, a + 1] += bk.weights * g[a]
             else:
                 arr = np.zeros((P, 1, D + 1))
-                for op in rops_of.get(id(bk), []):
-                    v, g = _channels(op.boundary_rhs_accumulate(q, 0), (P,), D)
+                if id(bk) in cut:
+                    calls = [op.cell_rhs_accumulate for (_, op) in self.problem.cell_rhs]
+                elif id(bk) in surface:
+                    calls = [op.surface_rhs_accumulate for (_, op) in getattr(self.problem, "surface_rhs", [])]
+                else:
+                    calls = [op.boundary_rhs_accumulate for op in rops_of.get(id(bk), [])]
+                for call in calls:
+                    v, g = _channels(call(q, 0), (P,), D)
                     arr[:, 0, 0] += bk.weights * v
                     for a in range(D):
                         arr[:, 0, a + 1] += bk.weights * g[a]

@@ -295,15 +295,11 @@ def solve(problem_or_plan, solver=None, relative_tolerance=None, absolute_tolera
         plan = problem_or_plan if is_plan else compile_plan(problem_or_plan)
         solver = _default_solver(plan, solver)
     # Device-resident right-hand side (point loads lowered on the host) for
-    # tensor cells, boundary faces and interfaces; the host path covers
-    # embedded surfaces and cut cells.
+    # every batch the device plan holds; loads on batches it does not hold
+    # (plan.device_rhs is False) take the host path.
     b = None
     if plan.device_rhs:
-        try:
-            b = ops.reduced_rhs(plan, as_numpy=False)
-        except _devabi.DeviceError as exc:
-            if exc.status != -8:  # GMF_EUNSUPPORTED: a face batch outside the face path
-                raise
+        b = ops.reduced_rhs(plan, as_numpy=False)
     if b is None:
         b = plan.reduced_rhs()
     x, info = solve_reduced(plan, solver, b, relative_tolerance, absolute_tolerance, maxiter, initial_solution)

@@ -222,17 +222,18 @@ gmf_status gmf_reduced_apply_f32(gmf_workspace* ws, const float* x_red, float* y
  * tensor-product cells in bucket order and each cell's points, the channels
  * (W v) or (W v, W g_1, .., W g_D) (channels = 1 or 1 + D), W the quadrature
  * weight: the load callbacks evaluated on the host, as the operator action is
- * lowered to point tables. Batches on the sum-factorized face path are
- * skipped (their loads: gmf_rhs_faces); other point-cloud items are not
- * covered (GMF_EUNSUPPORTED). Device pointers; synchronous. */
+ * lowered to point tables. Point-cloud batches are skipped (their loads:
+ * gmf_rhs_faces). Device pointers; synchronous. */
 gmf_status gmf_rhs_physical(gmf_workspace* ws, const double* loads, int32_t channels, double* y);
-/* Boundary-face and interface loads (src/accumulators.jl:1230-1274
- * _accumulate_one_sided_rhs!, 1666-1698 interface_rhs!) added into y (full
- * layout): for the plan's point-cloud batches in bucket order, per item, point
- * and side (minus then plus for interfaces) the channels (W v, W g_1, .., W g_D)
- * of the load callbacks evaluated on the host. Every point-cloud batch must run
- * the sum-factorized face path (GMF_EUNSUPPORTED otherwise: embedded-surface
- * and cut-cell loads stay on the host). Device pointers; synchronous. */
+/* Point-cloud loads added into y (full layout): cut-cell loads
+ * (src/accumulators.jl:1022-1054 basis path of cell_rhs), embedded-surface and
+ * boundary-face loads (1230-1274 _accumulate_one_sided_rhs!) and interface
+ * loads (1666-1698 interface_rhs!). For the plan's point-cloud batches in
+ * bucket order, per item, point and side (minus then plus for interfaces) the
+ * channels (W v, W g_1, .., W g_D) of the load callbacks evaluated on the
+ * host; batches on the sum-factorized face path project them with the face
+ * kernel, the others with the point-cloud kernel. Device pointers;
+ * synchronous. */
 gmf_status gmf_rhs_faces(gmf_workspace* ws, const double* loads, double* y);
 /* State channels at the points of the plan's tensor-product cells (bucket
  * order, cells x Q x channels): the value of the full-layout u and, for

@@ -196,6 +196,20 @@ class BallLoad:
         return 2.0 * self.dim + 0.0 * q.point[0]
 
 
+class NitscheData:
+    """Data terms of symmetric Nitsche for u = g on the embedded surface:
+    g (gamma / h v - dv/dn), with g a smooth function of the point (value and
+    gradient test channels both nonzero)."""
+
+    def __init__(self, gamma):
+        self.gamma = gamma
+
+    def surface_rhs_accumulate(self, q, test_component):
+        x = q.point
+        gval = 1.0 + 0.5 * x[0] - 0.25 * x[1] * x[2]
+        return g.TestChannels(self.gamma / q.cell_size[0] * gval, tuple(-gval * nc for nc in q.normal))
+
+
 class NitscheDirichlet:
     """Symmetric Nitsche terms for u = 0 on the embedded surface:
     -(du/dn) v - u (dv/dn) + gamma / h u v."""

@@ -182,3 +182,45 @@ def test_conflict_coloring_is_valid():
         conflicting = {int(colors[o]) for o in range(b) for it in items
                        if np.intersect1d(it, blocks[b]).size and np.intersect1d(it, blocks[o]).size}
         assert colors[b] == min(set(range(len(blocks) + 1)) - conflicting)
+
+
+def test_point_cloud_load_layout_reproduces_host_rhs():
+    # The loads handed to gmf_rhs_faces (per point-cloud batch, item, point:
+    # W v, W g_1..g_D) projected with the host's own bucket projection and
+    # scattered give plan.rhs_physical(): the layout the device kernel reads.
+    import problems as P
+    pr = P.fcm_ball(n=4, degree=2, depth=2, n_theta=12, n_phi=24, alpha=1e-3)
+    pr.add_surface_accumulator(pr.cell_operators[0][0], P.NitscheData(90.0))
+    plan = g.compile(pr)
+    assert plan.device_rhs and plan.cloud_buckets and plan.surface_buckets
+    D = plan.dim
+    b = np.zeros(plan.full_dof_count)
+    loads, ch = plan.lowered_loads()
+    off = 0
+    for bk in plan.buckets:
+        n = bk.cell_count * bk.n_points * ch
+        arr = loads[off:off + n].reshape(bk.cell_count, bk.n_points, ch)
+        off += n
+        local = bk.project(arr[..., 0])
+        for a in range(ch - 1):
+            local = local + bk.project(arr[..., 1 + a], derivative_axis=a)
+        b += np.bincount(bk.term_dof, weights=bk.term_coef * local[bk.term_cell, bk.term_box],
+                         minlength=plan.full_dof_count)
+    fl = plan.face_loads()
+    off = 0
+    for bk in plan.all_buckets:
+        if not getattr(bk, "cloud", False):
+            continue
+        n = bk.n_points_total * (D + 1)
+        arr = fl[off:off + n].reshape(bk.n_points_total, D + 1)
+        off += n
+        local = bk.project(arr[:, 0])
+        for a in range(D):
+            local = local + bk.project(arr[:, 1 + a], derivative_axis=a)
+        b += np.bincount(bk.term_dof, weights=bk.term_coef * local[bk.term_cell, bk.term_box],
+                         minlength=plan.full_dof_count)
+    assert off == fl.size
+    ref = plan.rhs_physical()
+    assert np.abs(b - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the surface data terms contribute (value and gradient channels)
+    assert np.abs(fl.reshape(-1, D + 1)[:, 1:]).max() > 0
