@@ -250,6 +250,52 @@ __global__ void __launch_bounds__(kCloudLanes) cell_rhs_cloud(CloudArgs a, const
   for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) scatter_term(a.map, item, i, r[i], y, false);
 }
 
+// Verification sums of one work unit of a one-sided batch
+// (src/verification.jl:201-230 _l2_error_components on cells with their own
+// points): partial[2u] = sum_q w_q (u_h(x_q) - e_q)^2, partial[2u + 1] =
+// sum_q w_q e_q^2, u_h from the item's gathered coefficients. Each lane reads
+// only its own point's column of the 1D factor tables.
+__global__ void __launch_bounds__(kCloudLanes) cell_l2_cloud(CloudArgs a, const double* __restrict__ u_full,
+                                                             const double* __restrict__ w,
+                                                             const double* __restrict__ e,
+                                                             double* __restrict__ partial) {
+  extern __shared__ double smem[];
+  int64_t pb, pe;
+  const int64_t item = unit_range(a, blockIdx.x, pb, pe);
+  double* u = smem;
+  double* tv = u + a.Ntot;
+  double* td = tv + a.D * a.Nmax * a.ts;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) u[i] = gather_term(a.map, item, i, u_full);
+  __syncthreads();
+  double se = 0.0, sr = 0.0;
+  const int q = threadIdx.x;
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
+    if (q < cnt) {
+      point_tables(a, p0 + q, q, tv, td);
+      double val = 0.0;
+      for (int i = 0; i < a.Ntot; ++i) {
+        int m[kMaxDim];
+        unflatten(a, i, m);
+        double phi, dphi[kMaxDim];
+        basis_at(a, tv, td, 0, m, q, phi, dphi);
+        val = fma(u[i], phi, val);
+      }
+      const double wq = w[p0 + q], eq = e[p0 + q], d = val - eq;
+      se = fma(wq * d, d, se);
+      sr = fma(wq * eq, eq, sr);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    se += __shfl_down(se, off);
+    sr += __shfl_down(sr, off);
+  }
+  if (q == 0) {
+    partial[2 * blockIdx.x] = se;
+    partial[2 * blockIdx.x + 1] = sr;
+  }
+}
+
 // Basis value and reference derivatives with the dimension known at compile
 // time (the products of basis_at in the same order, kept in registers).
 template <int D>
@@ -498,6 +544,10 @@ size_t rhs_lds(const CloudArgs& a) {
                            size_t(a.S * (a.D + 1) * a.chunk));
 }
 
+size_t l2_lds(const CloudArgs& a) {
+  return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.D * a.Nmax * a.ts));
+}
+
 size_t diagonal_lds(const CloudArgs& a) {
   return sizeof(double) * (size_t(a.Ntot) + size_t(2 * a.S * a.D * a.Nmax * a.ts) +
                            size_t(a.S * (a.D + 1) * (a.D + 1) * a.chunk));
@@ -727,6 +777,17 @@ void launch_cloud_rhs(const Bucket& b, const double* loads, double* y, hipStream
   CloudArgs a = make_cloud_args(b, false);
   pick_chunk(a, rhs_lds);
   hipLaunchKernelGGL(cell_rhs_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), rhs_lds(a), stream, a, loads, y);
+  GMF_HIP(hipGetLastError());
+}
+
+void launch_cloud_l2(const Bucket& b, const double* u_full, const double* w, const double* e, double* partial,
+                     hipStream_t stream) {
+  if (b.cells == 0) return;
+  CloudArgs a = make_cloud_args(b, false);
+  require(a.S == 1, GMF_EINVAL, "verification sums cover one-sided items");
+  pick_chunk(a, l2_lds);
+  hipLaunchKernelGGL(cell_l2_cloud, dim3(unsigned(b.units)), dim3(kCloudLanes), l2_lds(a), stream, a, u_full, w, e,
+                     partial);
   GMF_HIP(hipGetLastError());
 }
 

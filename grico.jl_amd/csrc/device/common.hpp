@@ -329,6 +329,10 @@ void launch_face_rhs(const Bucket& b, const double* loads, double* y, hipStream_
 // Point loads of a point-cloud batch off the face path projected and added
 // into y (full layout): per item, point and side (W v, W g_1, .., W g_D).
 void launch_cloud_rhs(const Bucket& b, const double* loads, double* y, hipStream_t stream);
+// Verification sums of a one-sided point-cloud batch off the face path: one
+// pair (sum w (u_h - e)^2, sum w e^2) per work unit into partial.
+void launch_cloud_l2(const Bucket& b, const double* u_full, const double* w, const double* e, double* partial,
+                     hipStream_t stream);
 void launch_face_batch(const gmf_plan& plan, const double* x, double* y, bool reduced, hipStream_t stream,
                        char* done);
 // slots: the workspace's brick halo array (plan.brick_slots entries).

@@ -1007,23 +1007,35 @@ gmf_status gmf_l2_error(gmf_workspace* ws, const double* u_full, const double* w
   return guarded([&] {
     require(ws && u_full && weights && exact && sums, GMF_EINVAL, "null argument");
     const gmf_plan& p = *ws->plan;
-    int64_t cells = 0;
+    // One partial pair per tensor cell and per work unit of a one-sided
+    // point-cloud batch off the face path; two-sided batches are not summed.
+    int64_t pairs = 0;
     for (const Bucket& b : p.buckets) {
-      require(!b.cloud, GMF_EUNSUPPORTED, "the device L2 error covers tensor-product cells only");
-      cells += b.cells;
+      if (!b.cloud)
+        pairs += b.cells;
+      else if (b.sides == 1 && !b.face.enabled)
+        pairs += b.units;
     }
     require_device_pointer(u_full, "u_full");
     require_device_pointer(weights, "weights");
     require_device_pointer(exact, "exact");
-    DeviceBuffer<double> partial(size_t(std::max<int64_t>(2 * cells, 2))), out(2);
-    fill(partial.p, 0.0, 2 * std::max<int64_t>(cells, 1), ws->stream);
+    DeviceBuffer<double> partial(size_t(std::max<int64_t>(2 * pairs, 2))), out(2);
+    fill(partial.p, 0.0, 2 * std::max<int64_t>(pairs, 1), ws->stream);
     int64_t off = 0, coff = 0;
     for (const Bucket& b : p.buckets) {
-      launch_cell_l2(b, u_full, weights + off, exact + off, partial.p + 2 * coff, ws->stream);
-      off += b.cells * int64_t(b.Qtot);
-      coff += b.cells;
+      if (!b.cloud) {
+        launch_cell_l2(b, u_full, weights + off, exact + off, partial.p + 2 * coff, ws->stream);
+        off += b.cells * int64_t(b.Qtot);
+        coff += b.cells;
+      } else if (b.sides == 1) {
+        if (!b.face.enabled) {
+          launch_cloud_l2(b, u_full, weights + off, exact + off, partial.p + 2 * coff, ws->stream);
+          coff += b.units;
+        }
+        off += b.total_points;
+      }
     }
-    launch_sum_pairs(partial.p, cells, out.p, ws->stream);
+    launch_sum_pairs(partial.p, pairs, out.p, ws->stream);
     GMF_HIP(hipMemcpyAsync(sums, out.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
     sync(ws);
   });

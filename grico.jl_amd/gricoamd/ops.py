@@ -140,8 +140,8 @@ def reduced_rhs(plan, as_numpy=True):
 
 
 def l2_sums(plan, u_full, exact):
-    """(sum w (u_h - u)^2, sum w u^2) over the tensor-product cells' points on
-    the device (src/verification.jl:201-230)."""
+    """(sum w (u_h - u)^2, sum w u^2) over the points of the plan's tensor and
+    cut cells on the device (src/verification.jl:201-230)."""
     dev = _ws(plan)
     ut = to_device(u_full, plan.full_dof_count)
     w = to_device(plan.point_weights())

@@ -885,9 +885,9 @@ class AssemblyPlan:
         embedded surfaces, boundary faces and interfaces with an operator."""
         faces = [bk for (bk, ops, _) in self.boundary_buckets if ops]
         faces += list(self.interface_buckets) if self.interface_ops else []
-        cells = list(self.buckets) + list(self.cloud_buckets)
-        if not self.problem.cell_operators and (faces or self.surface_buckets):
-            cells = []  # no cell action: the cell batches would apply zeros
+        # no cell action next to face or surface batches: the cell batches
+        # would apply zeros and are left out
+        cells = list(self.buckets) + list(self.cloud_buckets) if self.device_cells else []
         return cells + list(self.surface_buckets) + faces
 
     def rhs_physical(self):
@@ -959,23 +959,60 @@ class AssemblyPlan:
             self._loads = (np.concatenate(flat) if flat else np.zeros(0), channels)
         return self._loads
 
+    def _l2_layout(self):
+        """Batches of the device verification sums (gmf_l2_error) in device
+        bucket order: (bucket, counted) for the tensor-product cells and every
+        one-sided point-cloud batch; only cells (tensor and cut) are counted."""
+        if not self.device_cells:
+            raise ValueError("the device plan holds no cell batches (no cell operator next to face or surface "
+                             "batches); use the host l2_error")
+        cut = {id(bk) for bk in self.cloud_buckets}
+        out = []
+        for bk in self.all_buckets:
+            if not getattr(bk, "cloud", False):
+                out.append((bk, True))
+            elif getattr(bk, "sides", 1) == 1:
+                out.append((bk, id(bk) in cut))
+        return out
+
     def point_weights(self):
-        """Quadrature weights of the tensor-product cells' points (bucket order)."""
-        return np.concatenate([bk.weights.reshape(-1) for bk in self.buckets])
+        """Quadrature weights per point in the gmf_l2_error layout: tensor
+        cells, then one-sided point-cloud batches (zeros for surface pieces and
+        boundary faces)."""
+        parts = [bk.weights.reshape(-1) if counted else np.zeros(bk.n_points_total)
+                 for bk, counted in self._l2_layout()]
+        return np.concatenate(parts) if parts else np.zeros(0)
 
     def point_values(self, fn):
-        """fn(point tuple) at the tensor-product cells' points (bucket order)."""
-        return np.concatenate([np.broadcast_to(np.asarray(fn(tuple(bk.phys_points)), dtype=np.float64),
-                                               (bk.cell_count, bk.n_points)).reshape(-1) for bk in self.buckets])
+        """fn(point tuple) per point in the gmf_l2_error layout (zeros where
+        point_weights is zero)."""
+        parts = []
+        for bk, counted in self._l2_layout():
+            if not counted:
+                parts.append(np.zeros(bk.n_points_total))
+                continue
+            shape = (bk.n_points_total,) if getattr(bk, "cloud", False) else (bk.cell_count, bk.n_points)
+            parts.append(np.broadcast_to(np.asarray(fn(tuple(bk.phys_points)), dtype=np.float64), shape).reshape(-1))
+        return np.concatenate(parts) if parts else np.zeros(0)
+
+    @property
+    def device_cells(self):
+        """True when the device plan holds the cell batches (a cell operator,
+        or no face and surface batches beside them)."""
+        faces = [bk for (bk, ops, _) in self.boundary_buckets if ops]
+        faces += list(self.interface_buckets) if self.interface_ops else []
+        return bool(self.problem.cell_operators) or not (faces or self.surface_buckets)
 
     @property
     def device_rhs(self):
         """True when every load lives on a batch the device plan holds: tensor
         and cut cells (the cell batches are on the device), embedded-surface
         pieces, and boundary faces / interfaces with an operator."""
-        if self.problem.cell_rhs and not self.problem.cell_operators and (
-                self.boundary_buckets or self.interface_buckets or self.surface_buckets):
-            return False  # the device plan then holds no cell batches
+        if self.problem.cell_rhs and not self.device_cells:
+            return False  # the device plan holds no cell batches
+        if self.problem.cell_rhs and not self.problem.cell_operators and (self.boundary_buckets or
+                                                                          self.interface_buckets):
+            return False  # load-only faces make the host and device plans differ
         if any(rops and not ops for (_, ops, rops) in self.boundary_buckets):
             return False
         if self.interface_rhs_ops and not self.interface_ops:

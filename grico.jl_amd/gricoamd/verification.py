@@ -41,7 +41,7 @@ def relative_l2_error(state, exact):
 
 def device_l2_error(state, exact):
     """l2_error with the interpolation and the weighted sum on the device
-    (tensor-product plans)."""
+    (tensor and cut cells)."""
     from . import ops
     err, _ = ops.l2_sums(state.plan, state.coefficients, exact)
     return float(np.sqrt(err))

@@ -252,10 +252,14 @@ gmf_status gmf_interpolate_channels(gmf_workspace* ws, const double* u_full, int
 gmf_status gmf_plan_set_point_data(gmf_plan* plan, int32_t bucket, const double* point_data);
 /* b_red = P^T (rhs_full - A_phys shift) (src/assembly.jl:917-933 _reduced_rhs!). */
 gmf_status gmf_reduced_rhs(gmf_workspace* ws, const double* rhs_full, double* b_red);
-/* Verification sums over the tensor-product cells' points (src/verification.jl:201-230
- * _l2_error_components): sums[0] = sum_q w_q (u_h(x_q) - e_q)^2, sums[1] = sum_q w_q e_q^2,
- * u_h from the full-layout coefficients u_full; weights and exact values per point in the
- * `loads` order. sums is a host array of two doubles. */
+/* Verification sums (src/verification.jl:201-230 _l2_error_components):
+ * sums[0] = sum_q w_q (u_h(x_q) - e_q)^2, sums[1] = sum_q w_q e_q^2, u_h from the
+ * full-layout coefficients u_full. Weights and exact values per point: the
+ * tensor-product cells' points in the gmf_rhs_physical `loads` order, then, in
+ * bucket order, the points of every one-sided point-cloud batch (cut cells;
+ * zero weights leave out surface pieces and boundary faces, whose points are
+ * skipped on the face path). Two-sided batches take no values. sums is a host
+ * array of two doubles. */
 gmf_status gmf_l2_error(gmf_workspace* ws, const double* u_full, const double* weights, const double* exact,
                         double* sums);
 /* x_full = [shift] + P x_red (src/assembly.jl:851-874 _expand_reduced!). */

@@ -224,3 +224,29 @@ def test_point_cloud_load_layout_reproduces_host_rhs():
     assert np.abs(b - ref).max() <= 1e-12 * np.abs(ref).max()
     # the surface data terms contribute (value and gradient channels)
     assert np.abs(fl.reshape(-1, D + 1)[:, 1:]).max() > 0
+
+
+def test_l2_layout_reproduces_host_error():
+    # gmf_l2_error's per-point weights and exact values (tensor cells, then
+    # one-sided point-cloud batches, zero weights off the cells) summed with
+    # the host's own field values give the host relative L2 error.
+    import problems as P
+    from gricoamd.verification import _field_values, relative_l2_error
+    pr = P.fcm_ball(n=4, degree=2, depth=2, n_theta=12, n_phi=24, alpha=1e-3)
+    plan = g.compile(pr)
+    rng = np.random.default_rng(2)
+    st = g.State(plan, rng.standard_normal(plan.full_dof_count))
+    w = plan.point_weights()
+    e = plan.point_values(P.ball_exact)
+    off, num, den = 0, 0.0, 0.0
+    for bk, counted in plan._l2_layout():
+        n = bk.n_points_total if getattr(bk, "cloud", False) else bk.cell_count * bk.n_points
+        uh = _field_values(plan, st.coefficients, bk).reshape(-1) if counted else np.zeros(n)
+        ww, ee = w[off:off + n], e[off:off + n]
+        num += float(np.sum(ww * (uh - ee) ** 2))
+        den += float(np.sum(ww * ee ** 2))
+        off += n
+    assert off == w.size == e.size
+    assert any(not c for _, c in plan._l2_layout())  # the surface pieces take zero weights
+    ref = relative_l2_error(st, P.ball_exact)
+    assert abs(np.sqrt(num / den) - ref) <= 1e-12 * ref

@@ -73,3 +73,22 @@ def test_solve_uses_device_cloud_rhs(gpu):
     assert np.abs(got - xh).max() <= 1e-6 * np.abs(xh).max()
     A = pyoracle.reduced_matrix(plan)
     assert np.linalg.norm(A @ got - b) <= 1e-9 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("case", ["explicit", "physical", "split"])
+def test_device_l2_on_cut_cells(gpu, case, monkeypatch):
+    # gmf_l2_error over tensor and cut cells against the host verification
+    # sums (src/verification.jl:201-230) on a random coefficient vector.
+    from gricoamd.verification import device_relative_l2_error, relative_l2_error, device_l2_error, l2_error
+    if case == "split":
+        monkeypatch.setenv("GMF_CLOUD_UNIT_POINTS", "256")
+        pr = P.fcm_ball(n=4, degree=2, depth=3, n_theta=16, n_phi=32, alpha=1e-3)
+    elif case == "explicit":
+        pr = P.fcm_ball(n=4, degree=2, depth=2, n_theta=16, n_phi=32, alpha=1e-3)
+    else:
+        pr = P.fcm_ball_physical(n=4, degree=2, depth=2, n_theta=16, n_phi=32)
+    plan = g.compile(pr)
+    st = g.State(plan, np.random.default_rng(4).standard_normal(plan.full_dof_count))
+    assert abs(device_relative_l2_error(st, P.ball_exact) - relative_l2_error(st, P.ball_exact)) <= \
+        1e-12 * relative_l2_error(st, P.ball_exact)
+    assert abs(device_l2_error(st, P.ball_exact) - l2_error(st, P.ball_exact)) <= 1e-12 * l2_error(st, P.ball_exact)
