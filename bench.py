@@ -1163,9 +1163,18 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, or
     torch.cuda.synchronize()
     t_phys = time.perf_counter() - t0
     state = g.State(plan, ops.expand_reduced(plan, xs, include_shift=True).cpu().numpy())
+    from gricoamd.verification import device_relative_l2_error
+    exact = lambda p: 1.0 - (p[0] ** 2 + p[1] ** 2 + p[2] ** 2)  # noqa: E731
+    t0 = time.perf_counter()
+    l2_host = g.relative_l2_error(state, exact)
+    t_l2_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    l2_dev = device_relative_l2_error(state, exact)
+    t_l2_dev = time.perf_counter() - t0
     out.update({"physical_load_seconds": t_phys, "physical_load_iterations": int(info.iterations),
-                "physical_load_relative_l2_error_vs_1_minus_r2":
-                    g.relative_l2_error(state, lambda p: 1.0 - (p[0] ** 2 + p[1] ** 2 + p[2] ** 2))})
+                "physical_load_relative_l2_error_vs_1_minus_r2": l2_dev,
+                "l2_host_seconds": t_l2_host, "l2_device_seconds_with_host_point_values": t_l2_dev,
+                "l2_device_vs_host_relative_difference": abs(l2_dev - l2_host) / l2_host})
     log(f"fcm physical load: {int(info.iterations)} iterations, {t_phys:.2f} s, "
         f"L2 {out['physical_load_relative_l2_error_vs_1_minus_r2']:.2e}")
     # The block apply against the oracle on a small ball of the same kind.
