@@ -68,7 +68,9 @@ class GeometricMultigridPreconditioner:
     diagonals and eigenvalue estimates in FP64) inside the FP64 outer Krylov
     method, `h_steps_per_level` and `cell_blocks`: "cut" smooths with one
     dense block per cell that carries its own rule (finite cells cut by the
-    boundary) on every smoothing level, point Jacobi elsewhere."""
+    boundary) on every smoothing level, point Jacobi elsewhere; "cells" with
+    one block per cell of at most 512 reduced dofs (larger cells keep point
+    Jacobi)."""
 
     def __init__(self, min_degree=1, max_levels=16, p_sequence="bisect", pre_smoothing_steps=None,
                  post_smoothing_steps=None, smoother_damping=0.8, smoother="auto", smoother_restart=8,
@@ -108,8 +110,8 @@ class GeometricMultigridPreconditioner:
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
         self.h_steps_per_level = _positive_int(h_steps_per_level, "h_steps_per_level")
-        if cell_blocks not in (None, "cut"):
-            raise ValueError("cell_blocks must be None or 'cut'")
+        if cell_blocks not in (None, "cut", "cells"):
+            raise ValueError("cell_blocks must be None, 'cut' or 'cells'")
         self.cell_blocks = cell_blocks
 
     def resolved_smoother(self, operator_class):
@@ -572,11 +574,12 @@ class MultigridHierarchy:
             d.spd = int(isinstance(self.problem.operator_class, SPD))
             # Levels whose reduced map is not a selection (affine Dirichlet
             # relations, mean rows) get their reduced diagonal from the host.
-            if pol.cell_blocks == "cut":
+            if pol.cell_blocks is not None:
                 bds = (_devabi.BlockDesc * len(self.plans))()
                 self.block_colors = [0] * len(self.plans)
                 for lvl in range(1, len(self.plans)):
-                    blocks = cut_cell_blocks(self.plans[lvl])
+                    blocks = (cut_cell_blocks(self.plans[lvl]) if pol.cell_blocks == "cut" else
+                              cell_blocks_of(self.plans[lvl]))
                     if not blocks:
                         continue
                     off, dofs, inv, ncol = block_inverses(self.plans[lvl], blocks)
@@ -634,6 +637,26 @@ def cut_cell_blocks(plan):
         for c in range(cb.cell_count):
             d = np.unique(dofs[bounds[c]:bounds[c + 1]])
             if len(d):
+                out.append(d.astype(np.int32))
+    return out
+
+
+def cell_blocks_of(plan, max_dofs=512):
+    """Reduced dofs of every cell of the plan's tensor and point-cloud cell
+    batches with at most max_dofs of them, one sorted int32 array per cell."""
+    if not getattr(plan, "selection_map", True):
+        raise ValueError("cell blocks need a reduced map that selects dofs")
+    out = []
+    for bk in list(plan.buckets) + list(getattr(plan, "cloud_buckets", [])):
+        red = plan.reduced_index[bk.term_dof]
+        keep = red >= 0
+        cells, dofs = bk.term_cell[keep], red[keep]
+        order = np.lexsort((dofs, cells))
+        cells, dofs = cells[order], dofs[order]
+        bounds = np.searchsorted(cells, np.arange(bk.cell_count + 1))
+        for c in range(bk.cell_count):
+            d = np.unique(dofs[bounds[c]:bounds[c + 1]])
+            if 0 < len(d) <= max_dofs:
                 out.append(d.astype(np.int32))
     return out
 

@@ -658,16 +658,18 @@ def _chebyshev(A, dinv, x, b, x_zero, lam_max, degree, rng):
     return x
 
 
-def cell_block_dofs(plan):
+def cell_block_dofs(plan, all_cells=False, max_dofs=512):
     """Reduced dofs touched by each cell that carries its own rule (the
     plan's point-cloud cells), in cell order: the blocks of the cell-block
-    smoother."""
+    smoother; all_cells: every tensor and point-cloud cell with at most
+    max_dofs of them."""
     out = []
-    for cb in getattr(plan, "cloud_buckets", []):
+    buckets = (list(plan.buckets) if all_cells else []) + list(getattr(plan, "cloud_buckets", []))
+    for cb in buckets:
         red = plan.reduced_index[cb.term_dof]
         for c in range(cb.cell_count):
             d = np.unique(red[(cb.term_cell == c) & (red >= 0)])
-            if len(d):
+            if len(d) and (not all_cells or len(d) <= max_dofs):
                 out.append(d)
     return out
 

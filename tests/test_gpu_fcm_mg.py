@@ -110,3 +110,24 @@ def test_fcm_ball_gmg_cg(gpu):
     jac = g.solve(g.compile(pr), solver=g.CGSolver(preconditioner=g.JacobiPreconditioner()),
                   relative_tolerance=1e-8, maxiter=200000)
     assert st.info.iterations < jac.info.iterations / 5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_whole_cell_block_vcycle_matches_oracle(gpu, precision):
+    # cell_blocks="cells": one block per cell (tensor cells of an adaptive hp
+    # mesh with hanging nodes and degrees 1..3) on every smoothing level.
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", cell_blocks="cells", precision=precision,
+                                           pre_smoothing_steps=1, post_smoothing_steps=1, chebyshev_degree=2)
+    problem = P.adaptive_hp(base=4, levels=2, pmax=3)
+    h = compile_geometric_multigrid(problem, pol)
+    M = h.device_preconditioner()
+    assert h.level_count >= 2 and max(h.block_colors) > 0
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 6)
+    z = M.apply(r).cpu().numpy()
+    mats = [pyoracle.assembled_reduced_matrix(p).toarray() for p in h.plans]
+    Ps = [t.matrix() for t in h.transfers]
+    blocks = [pyoracle.cell_block_dofs(p, all_cells=True) for p in h.plans]
+    ref = pyoracle.vcycle(mats, Ps, r, smoother="chebyshev", pre=1, post=1, omega=pol.smoother_damping,
+                          lambdas=h.level_lambda_max(), cheb_degree=2, cheb_range=pol.chebyshev_range, blocks=blocks)
+    tol = 1e-10 if precision == "fp64" else 5e-5
+    assert np.abs(z - ref).max() <= tol * np.abs(ref).max()
