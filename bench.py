@@ -927,9 +927,9 @@ def lognormal_leg(repeats=3, depth=5, learned=True, pairs=2000, steps=3000):
     return out
 
 
-# Sources of the cell-pass kernels whose HBM traffic profiles/apply_traffic.json records.
-TRAFFIC_SOURCES = ("grico.jl_amd/csrc/device/cell_fast.hip", "grico.jl_amd/csrc/device/eo.hpp",
-                   "grico.jl_amd/csrc/device/common.hpp")
+# Sources of the cell-pass kernels (brick kernel, halo pass, their stores and
+# one-dimensional sweeps) whose HBM traffic profiles/apply_traffic.json records.
+TRAFFIC_SOURCES = ("grico.jl_amd/csrc/device/cell_fast.hip", "grico.jl_amd/csrc/device/eo.hpp")
 
 
 def kernel_source_hash():
