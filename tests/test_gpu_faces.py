@@ -197,6 +197,20 @@ def test_device_face_rhs_matches_host(gpu, name):
     assert rel(dev, host) <= 1e-12, name
 
 
+@pytest.mark.parametrize("name", ["2d_sipg_hanging", "3d_sipg_hanging", "2d_interface_loads"])
+def test_device_face_rhs_on_point_cloud_kernel(gpu, monkeypatch, name):
+    """With GMF_FACE_TENSOR=0 every face and interface batch (one- and
+    two-sided) runs the point-cloud kernel, whose load projection
+    (cell_rhs_cloud) then gives the host projection too."""
+    from gricoamd import ops
+    monkeypatch.setenv("GMF_FACE_TENSOR", "0")
+    plan = g.compile(_rhs_cases()[name])
+    assert plan.device_rhs
+    dev = ops.rhs_physical(plan)
+    assert "face-tensor" not in _describe(plan)
+    assert rel(dev, plan.rhs_physical()) <= 1e-12, name
+
+
 def test_device_rhs_falls_back_for_load_only_faces(gpu):
     """A boundary side with a load but no operator is not in the device plan:
     the host path takes over, and solve() still reproduces the solution."""
