@@ -296,6 +296,94 @@ __global__ void __launch_bounds__(kCloudLanes) cell_l2_cloud(CloudArgs a, const 
   }
 }
 
+// Local matrices of point-cloud items: column j of item c's matrix is the
+// item's local action on the unit vector of local mode j, computed with the
+// arithmetic of cell_apply_cloud over all of the item's points (one
+// workgroup per item and column). Stored column-major per item:
+// E[c][j][i] at emat[(c * Ntot + j) * Ntot + i].
+__global__ void __launch_bounds__(kCloudLanes) cell_local_matrix_cloud(CloudArgs a, double* __restrict__ emat) {
+  extern __shared__ double smem[];
+  const int64_t item = blockIdx.x;
+  const int j = int(blockIdx.y);
+  const int C = a.D + 1;
+  double* u = smem;
+  double* r = u + a.Ntot;
+  double* tv = r + a.Ntot;
+  double* td = tv + a.S * a.D * a.Nmax * a.ts;
+  double* ch = td + a.S * a.D * a.Nmax * a.ts;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+    u[i] = i == j ? 1.0 : 0.0;
+    r[i] = 0.0;
+  }
+  __syncthreads();
+  const int64_t pb = a.poff[item], pe = a.poff[item + 1];
+  for (int64_t p0 = pb; p0 < pe; p0 += a.chunk) {
+    const int cnt = int(pe - p0 < a.chunk ? pe - p0 : a.chunk);
+    chunk_forward(a, item, p0, cnt, u, tv, td, ch);
+    __syncthreads();
+    for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) {
+      int m[kMaxDim];
+      const int s = unflatten(a, i, m);
+      const double* chs = ch + s * C * a.chunk;
+      double acc = 0.0;
+      for (int q = 0; q < cnt; ++q) {
+        double phi, dphi[kMaxDim];
+        basis_at(a, tv, td, s, m, q, phi, dphi);
+        acc = fma(chs[q], phi, acc);
+        for (int ax = 0; ax < a.D; ++ax) acc = fma(chs[(ax + 1) * a.chunk + q], dphi[ax], acc);
+      }
+      r[i] += acc;
+    }
+    __syncthreads();
+  }
+  double* col = emat + (item * a.Ntot + j) * a.Ntot;
+  for (int i = threadIdx.x; i < a.Ntot; i += blockDim.x) col[i] = r[i];
+}
+
+constexpr int kMatrixItemsPerGroup = 4;  // one wavefront per item
+// Largest local-matrix store of one point-cloud batch, and the largest local
+// box (the apply stages 4 items x 8 columns of local coefficients in LDS).
+constexpr int64_t kMaxLocalMatrixBytes = int64_t(2) << 30;
+constexpr int kMaxLocalMatrixModes = 128;
+
+// Y[:, j0 + c] += P_c E_c P_c^T X[:, j0 + c] for the KC columns of one column
+// group: one wavefront per item gathers its local coefficients of every
+// column into LDS; lane i forms rows i, i + 64, .. of E x (columns of E are
+// contiguous, so the lanes read consecutive addresses), each summed over the
+// local modes in order, and scatters them through the term map.
+template <int KC>
+__global__ void __launch_bounds__(64 * kMatrixItemsPerGroup)
+    cell_apply_local_matrix(TermMap map, const double* __restrict__ emat, int64_t items, const double* __restrict__ X,
+                            double* __restrict__ Y, int64_t ld, int k) {
+  extern __shared__ double smem[];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t item = int64_t(blockIdx.x) * kMatrixItemsPerGroup + w;
+  const int j0 = int(blockIdx.y) * KC;
+  const int kc = k - j0 < KC ? k - j0 : KC;
+  const int n = map.Ntot;
+  if (item >= items) return;  // no workgroup barrier below: each wavefront owns its LDS slice
+  double* xs = smem + int64_t(w) * KC * n;  // KC x n
+  for (int i = lane; i < n; i += 64)
+    for (int c = 0; c < kc; ++c) xs[c * n + i] = gather_term(map, item, i, X + int64_t(j0 + c) * ld);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double* E = emat + item * int64_t(n) * n;
+  for (int i = lane; i < n; i += 64) {
+    double t[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) t[c] = 0.0;
+    for (int jj = 0; jj < n; ++jj) {
+      const double e = E[int64_t(jj) * n + i];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) t[c] = fma(e, xs[c * n + jj], t[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+      if (c < kc) scatter_term(map, item, i, t[c], Y + int64_t(j0 + c) * ld, false);
+  }
+}
+
 // Basis value and reference derivatives with the dimension known at compile
 // time (the products of basis_at in the same order, kept in registers).
 template <int D>
@@ -662,10 +750,36 @@ void launch_block(const Bucket& b, const double* X, double* Y, int64_t ld, int k
 
 }  // namespace
 
+// Local-matrix apply of KC-column groups.
+template <int KC>
+void launch_local_matrix(const Bucket& b, const double* X, double* Y, int64_t ld, int k, bool reduced,
+                         hipStream_t stream) {
+  const TermMap map{reduced ? b.ridx : b.lidx, b.lcoef, b.moff, reduced ? b.mridx : b.midx, b.mcoef, b.Ntot};
+  require(map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
+  require(b.emat != nullptr && b.Ntot <= kMaxLocalMatrixModes, GMF_EINVAL, "point-cloud batch has no local matrices");
+  const dim3 grid(unsigned((b.cells + kMatrixItemsPerGroup - 1) / kMatrixItemsPerGroup), unsigned((k + KC - 1) / KC));
+  const size_t lds = sizeof(double) * size_t(kMatrixItemsPerGroup) * KC * size_t(b.Ntot);
+  hipLaunchKernelGGL((cell_apply_local_matrix<KC>), grid, dim3(64 * kMatrixItemsPerGroup), lds, stream, map, b.emat,
+                     b.cells, X, Y, ld, k);
+  GMF_HIP(hipGetLastError());
+}
+
+void apply_local_matrix(const Bucket& b, const double* X, double* Y, int64_t ld, int k, bool reduced,
+                        hipStream_t stream) {
+  if (k == 1)
+    launch_local_matrix<1>(b, X, Y, ld, k, reduced, stream);
+  else
+    launch_local_matrix<8>(b, X, Y, ld, k, reduced, stream);
+}
+
 void launch_cell_cloud_block(const gmf_plan&, const Bucket& b, const double* X, double* Y, int64_t ld, int k,
                              bool reduced, hipStream_t stream) {
   if (b.cells == 0 || k <= 0) return;
   require(k <= 65535 * 2, GMF_EINVAL, "too many columns for one block launch");
+  if (b.emat) {
+    apply_local_matrix(b, X, Y, ld, k, reduced, stream);
+    return;
+  }
   switch (cloud_block_columns()) {
     case 4: launch_block<4>(b, X, Y, ld, k, reduced, stream); break;
     case 8: launch_block<8>(b, X, Y, ld, k, reduced, stream); break;
@@ -676,6 +790,10 @@ void launch_cell_cloud_block(const gmf_plan&, const Bucket& b, const double* X, 
 void launch_cell_cloud(const gmf_plan&, const Bucket& b, const double* x, double* y, bool reduced,
                        hipStream_t stream) {
   if (b.cells == 0) return;
+  if (b.emat) {
+    apply_local_matrix(b, x, y, 0, 1, reduced, stream);
+    return;
+  }
   CloudArgs a = make_cloud_args(b, reduced);
   require(a.map.idx != nullptr, GMF_EUNSUPPORTED, "reduced index maps are not available for this plan");
   if (!cloud_generic_kernel()) {
@@ -705,9 +823,33 @@ int64_t cloud_units(const Bucket& b, int unit_points, std::vector<int32_t>& item
 // point-cloud kernel, timed on its one-column apply. Candidates that give the
 // same work decomposition and chunk as one already timed are skipped;
 // GMF_CLOUD_UNIT_POINTS restricts the work-unit size to one value.
+
+// Local matrices of every item of a point-cloud batch (a new device array of
+// cells x Ntot x Ntot doubles, column-major per item).
+double* build_local_matrices(Bucket& b) {
+  CloudArgs a = make_cloud_args(b, false);
+  pick_chunk(a, apply_lds);
+  void* e = nullptr;
+  GMF_HIP(hipMalloc(&e, size_t(b.cells) * size_t(b.Ntot) * size_t(b.Ntot) * sizeof(double)));
+  for (int64_t c0 = 0; c0 < b.cells; c0 += 65535) {
+    // grid.x is limited: items in slices, the kernel indexes items from the slice start
+    const int64_t nc = b.cells - c0 < 65535 ? b.cells - c0 : 65535;
+    CloudArgs as = a;
+    as.jinv = a.jinv + c0 * a.S * a.D;
+    as.poff = a.poff + c0;
+    hipLaunchKernelGGL(cell_local_matrix_cloud, dim3(unsigned(nc), unsigned(b.Ntot)), dim3(kCloudLanes),
+                       apply_lds(a), nullptr, as, static_cast<double*>(e) + c0 * int64_t(b.Ntot) * b.Ntot);
+    GMF_HIP(hipGetLastError());
+  }
+  GMF_HIP(hipDeviceSynchronize());
+  return static_cast<double*>(e);
+}
+
 void tune_cloud(gmf_plan& p, TuneTimer& t) {
   int forced = 0;  // GMF_CLOUD_UNIT_POINTS: time only this work-unit size
   if (const char* e = std::getenv("GMF_CLOUD_UNIT_POINTS")) forced = std::atoi(e);
+  int matrix_mode = -1;  // GMF_CLOUD_MATRIX: 0 never, 1 always (when they fit), unset: timed
+  if (const char* e = std::getenv("GMF_CLOUD_MATRIX")) matrix_mode = std::atoi(e) ? 1 : 0;
   for (Bucket& b : p.buckets) {
     if (!b.cloud || b.cells == 0 || b.face.enabled) continue;
     TuneInfo& ti = b.tune;
@@ -769,6 +911,26 @@ void tune_cloud(gmf_plan& p, TuneTimer& t) {
     ti.param0 = b.unit_points;
     ti.param1 = b.lds_kb;
     ti.best_us = best;
+    // Local matrices: kept when their apply is faster than the best point
+    // shape (GMF_CLOUD_MATRIX=0 never builds them, =1 keeps them whenever
+    // they fit); at most kMaxLocalMatrixBytes per batch.
+    const int64_t bytes = b.cells * int64_t(b.Ntot) * b.Ntot * int64_t(sizeof(double));
+    if (matrix_mode == 0 || b.Ntot > kMaxLocalMatrixModes || bytes > kMaxLocalMatrixBytes) continue;
+    double* e = build_local_matrices(b);
+    b.emat = e;  // the timed apply reads the matrices through the bucket
+    const float us = t.time_us([&] { apply_local_matrix(b, t.x, t.y, 0, 1, false, nullptr); });
+    ++ti.candidates;
+    if (matrix_mode == 1 || us < best) {
+      p.allocations.push_back(e);
+      p.device_bytes += bytes;
+      ti.kernel = kTuneCloudMatrix;
+      ti.param0 = b.Ntot;
+      ti.param1 = 0;
+      ti.best_us = us;
+    } else {
+      b.emat = nullptr;
+      GMF_HIP(hipFree(e));
+    }
   }
 }
 

@@ -82,7 +82,15 @@ constexpr int kCloudUnitPoints = 512;  // default points per work unit of the po
 
 // Kernel family and launch shape a bucket runs with, as picked by the
 // plan-time tuner (tune.hip); mirrors gmf_bucket_tuning.
-enum TuneKernel { kTuneNone = 0, kTuneBrick = 1, kTuneCell = 2, kTuneGeneric = 3, kTuneCloud = 4, kTuneFace = 5 };
+enum TuneKernel {
+  kTuneNone = 0,
+  kTuneBrick = 1,
+  kTuneCell = 2,
+  kTuneGeneric = 3,
+  kTuneCloud = 4,
+  kTuneFace = 5,
+  kTuneCloudMatrix = 6
+};
 struct TuneInfo {
   int kernel = kTuneNone;
   int param0 = 0, param1 = 0;  // brick: -, -; cell / generic: lanes per workgroup / cell; cloud: unit points, LDS KB
@@ -150,6 +158,10 @@ struct Bucket {
   int64_t* unit_start = nullptr;
   int unit_points = kCloudUnitPoints;  // points per work unit (tuned)
   int lds_kb = 64;                     // LDS budget of one workgroup (tuned)
+  // Local matrices (items x Ntot x Ntot, column-major per item), built from
+  // the point data at plan creation when the tuner finds their apply faster;
+  // null: the apply runs over the points.
+  double* emat = nullptr;
   std::vector<int64_t> h_poff;         // host copy of poff (work-unit rebuilds)
   // Exact reduced diagonal on cells whose modes couple through multi-term
   // expansions or repeat a dof: `coupled` flags those cells (null if none) and

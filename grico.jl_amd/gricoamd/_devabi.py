@@ -57,7 +57,8 @@ class BucketTuning(ctypes.Structure):
 
 
 # gmf_bucket_tuning.kernel
-TUNED_KERNELS = {0: "empty", 1: "brick", 2: "cell", 3: "generic", 4: "point-cloud", 5: "face-tensor"}
+TUNED_KERNELS = {0: "empty", 1: "brick", 2: "cell", 3: "generic", 4: "point-cloud", 5: "face-tensor",
+                 6: "point-cloud-matrix"}
 
 
 class TransferDesc(ctypes.Structure):

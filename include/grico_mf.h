@@ -168,7 +168,9 @@ gmf_status gmf_plan_describe(const gmf_plan* plan, char* buf, int64_t capacity);
  * runs every batch through one CPU loop). kernel: 0 empty bucket, 1 brick
  * kernel, 2 per-cell tensor kernel (param0 = lanes per workgroup), 3 generic
  * tensor kernel (param0 = lanes per cell), 4 point-cloud kernel (param0 =
- * points per work unit, param1 = LDS budget in KB), 5 face-tensor kernel.
+ * points per work unit, param1 = LDS budget in KB), 5 face-tensor kernel,
+ * 6 point-cloud items through local matrices built from the point data at
+ * plan creation (param0 = local modes per item).
  * candidates = shapes timed (0: defaults kept, GMF_TUNE=0); best_us /
  * default_us = time per launch of the chosen / the default shape. */
 typedef struct gmf_bucket_tuning {

@@ -82,11 +82,17 @@ def test_report_of_every_bucket_family(gpu):
             assert t["param0"] in (64, 128, 256)
         if t["kernel"] == "cell":
             assert t["param0"] in (64, 128, 256)
-    _, dev = device_plan(cut_problem())
+    _, dev = device_plan(cut_problem(), GMF_CLOUD_MATRIX=0)
     rep = check_report(dev, {"cell", "generic", "brick", "point-cloud"})
     cloud = [t for t in rep if t["kernel"] == "point-cloud"]
     assert cloud and all(t["param0"] in (256, 512, 1024, 2048) and t["param1"] in (32, 64) for t in cloud)
     assert any(t["candidates"] >= 2 for t in cloud)
+    # timed choice between the point shapes and local matrices
+    _, dev = device_plan(cut_problem())
+    rep = check_report(dev, {"cell", "generic", "brick", "point-cloud", "point-cloud-matrix"})
+    for t in rep:
+        if t["kernel"] == "point-cloud-matrix":
+            assert t["param0"] == 27 and t["best_us"] <= t["default_us"] * (1.0 + 1e-6)
 
 
 def test_brick_or_cell_choice(gpu):
@@ -112,13 +118,32 @@ def test_forced_cell_workgroups_match_oracle(gpu, threads):
 
 @pytest.mark.parametrize("points", [256, 2048])
 def test_forced_cloud_units_match_oracle(gpu, points):
-    plan, dev = device_plan(cut_problem(), GMF_CLOUD_UNIT_POINTS=points)
+    plan, dev = device_plan(cut_problem(), GMF_CLOUD_UNIT_POINTS=points, GMF_CLOUD_MATRIX=0)
     rep = dev.tuning()
     assert all(t["param0"] == points for t in rep if t["kernel"] == "point-cloud")
     x = P.rng_vector(plan.full_dof_count, seed=2)
     assert rel(g.apply(plan, x), pyoracle.apply(plan, x)) < TOL
     xr = P.rng_vector(plan.reduced_dof_count, seed=3)
     assert rel(g.reduced_apply(plan, xr), pyoracle.reduced_apply(plan, xr)) < TOL
+
+
+def test_forced_cloud_matrices_match_oracle(gpu):
+    # every point-cloud batch through its local matrices (built from the point
+    # data by the point-cloud arithmetic): full, reduced and block applies
+    plan, dev = device_plan(cut_problem(), GMF_CLOUD_MATRIX=1)
+    rep = dev.tuning()
+    assert any(t["kernel"] == "point-cloud-matrix" for t in rep)
+    x = P.rng_vector(plan.full_dof_count, seed=6)
+    assert rel(g.apply(plan, x), pyoracle.apply(plan, x)) < TOL
+    xr = P.rng_vector(plan.reduced_dof_count, seed=7)
+    yr = g.reduced_apply(plan, xr)
+    assert rel(yr, pyoracle.reduced_apply(plan, xr)) < TOL
+    X = np.stack([P.rng_vector(plan.reduced_dof_count, seed=10 + j) for j in range(11)], axis=1)
+    Y = g.reduced_apply_block(plan, X).cpu().numpy()
+    ref = np.stack([pyoracle.reduced_apply(plan, X[:, j]) for j in range(X.shape[1])], axis=1)
+    assert rel(Y, ref) < TOL
+    # the block apply gives each column the one-column result
+    assert np.abs(Y[:, 0] - np.asarray(g.reduced_apply(plan, X[:, 0]))).max() <= 1e-14 * np.abs(Y).max()
 
 
 def test_tuner_off_keeps_defaults(gpu):
