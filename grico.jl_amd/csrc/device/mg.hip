@@ -119,14 +119,25 @@ __global__ void k_cheb_step(R* __restrict__ x, R* __restrict__ d, R* __restrict_
   }
 }
 
-// x = Ainv b for a small dense row-major inverse (one wavefront per row).
+// x = Ainv b for a dense row-major inverse (one wavefront per row). Each lane
+// keeps four partial sums over columns lane + 64 (4 t + u), so that four
+// loads of the row are in flight per step; fixed order, deterministic.
 __global__ void k_dense_matvec(double* __restrict__ x, const double* __restrict__ A, const double* __restrict__ b,
                                int64_t n) {
   const int64_t row = blockIdx.x * int64_t(blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  double s = 0.0;
-  for (int64_t j = lane; j < n; j += 64) s = fma(A[row * n + j], b[j], s);
+  const double* a = A + row * n;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t j = lane;
+  for (; j + 192 < n; j += 256) {
+    s0 = fma(a[j], b[j], s0);
+    s1 = fma(a[j + 64], b[j + 64], s1);
+    s2 = fma(a[j + 128], b[j + 128], s2);
+    s3 = fma(a[j + 192], b[j + 192], s3);
+  }
+  for (; j < n; j += 64) s0 = fma(a[j], b[j], s0);
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (lane == 0) x[row] = s;
 }
