@@ -1311,7 +1311,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (cells would shard by Morton "
-              "range with an interface-dof exchange; see DESIGN.md)", file=sys.stderr)
+              "range with an interface-dof exchange per apply)", file=sys.stderr)
         sys.exit(2)
 
     import ctypes
