@@ -1310,7 +1310,7 @@ def main():
                     help="after timing, copy a 1 GiB buffer three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (cells would shard by Morton "
+        print("bench.py measures one GPU: there is no multi-GPU path (cells would shard by Morton "
               "range with an interface-dof exchange per apply)", file=sys.stderr)
         sys.exit(2)
 
