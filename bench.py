@@ -1016,7 +1016,7 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-3):
     return pr
 
 
-def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, oracle_cells=4, precision="fp32"):
+def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, oracle_cells=4, precision="fp64"):
     """Finite-cell sphere workload with nrhs seeded N(0,1) reduced right-hand
     sides solved to 1e-8 by block CG (FP64 Krylov, lockstep columns) with the
     hp-multigrid preconditioner on physical-domain levels (cut-cell rules

@@ -266,8 +266,10 @@ void block_precond(gmf_workspace* ws, gmf_preconditioner* M, BlockVectors& v, in
     GMF_HIP(hipGetLastError());
     return;
   }
-  for (int j = 0; j < k; ++j)
-    if (known_live[size_t(j)]) M->apply(v.r.p + int64_t(j) * n, v.z.p + int64_t(j) * n);
+  if (!M->apply_block(v.r.p, v.z.p, n, k)) {
+    for (int j = 0; j < k; ++j)
+      if (known_live[size_t(j)]) M->apply(v.r.p + int64_t(j) * n, v.z.p + int64_t(j) * n);
+  }
   block_dot(ws, v.r.p, v.z.p, n, k, v.s.p, slot, gated, v.parts.p);
 }
 

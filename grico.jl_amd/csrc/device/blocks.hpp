@@ -44,8 +44,14 @@ template <class R>
 __global__ void __launch_bounds__(64 * kBlocksPerGroup)
     k_block_add(R* __restrict__ x, R* __restrict__ d, const R* __restrict__ r, const R* __restrict__ q, R s,
                 const int64_t* __restrict__ off, const int32_t* __restrict__ dofs, const int64_t* __restrict__ voff,
-                const R* __restrict__ inv, int64_t count) {
+                const R* __restrict__ inv, int64_t count, int64_t ld) {
   __shared__ R rs[kBlocksPerGroup][kMaxBlockDofs];
+  // column blockIdx.y of a block of columns (leading dimension ld)
+  const int64_t co = int64_t(blockIdx.y) * ld;
+  x += co;
+  if (d) d += co;
+  r += co;
+  if (q) q += co;
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t b = int64_t(blockIdx.x) * kBlocksPerGroup + w;
   const bool live = b < count;
@@ -67,12 +73,14 @@ __global__ void __launch_bounds__(64 * kBlocksPerGroup)
   }
 }
 
+// k columns at leading dimension ld (k = 1: one vector).
 template <class R>
-inline void block_add(const BlockSet<R>& bs, R* x, R* d, const R* r, const R* q, R s, hipStream_t stream) {
+inline void block_add(const BlockSet<R>& bs, R* x, R* d, const R* r, const R* q, R s, hipStream_t stream, int k = 1,
+                      int64_t ld = 0) {
   if (bs.count == 0) return;
   const unsigned groups = unsigned((bs.count + kBlocksPerGroup - 1) / kBlocksPerGroup);
-  hipLaunchKernelGGL(k_block_add<R>, dim3(groups), dim3(64 * kBlocksPerGroup), 0, stream, x, d, r, q, s,
-                     bs.offsets.p, bs.dofs.p, bs.value_offsets.p, bs.inverse.p, bs.count);
+  hipLaunchKernelGGL(k_block_add<R>, dim3(groups, unsigned(k)), dim3(64 * kBlocksPerGroup), 0, stream, x, d, r, q, s,
+                     bs.offsets.p, bs.dofs.p, bs.value_offsets.p, bs.inverse.p, bs.count, ld);
   GMF_HIP(hipGetLastError());
 }
 

@@ -14,6 +14,9 @@ struct gmf_preconditioner {
   virtual void apply(const double* r, double* z) = 0;
   // z = M^-1 r and scalars[slot] = r.z (fused where the preconditioner allows).
   virtual void apply_dot(const double* r, double* z, int64_t n, int slot);
+  // Z = M^-1 R for k columns of n rows (column-major) in one pass where the
+  // preconditioner supports it; false: apply column by column.
+  virtual bool apply_block(const double*, double*, int64_t, int) { return false; }
 };
 
 namespace gmf {
@@ -91,6 +94,8 @@ void sync(gmf_workspace* ws);
 int progress_interval();
 // y_red = P^T A P x_red on ws->stream (src/assembly.jl:905-915).
 void reduced_apply(gmf_workspace* ws, const double* xr, double* yr);
+// Y = A_red X for k columns of n_red rows (column-major; block.hip).
+void reduced_apply_block(gmf_workspace* ws, const double* X, double* Y, int k);
 // The same operator on FP32 vectors: FP32 kernels on plans whose buckets all
 // have them, otherwise the FP64 apply of the widened input, rounded back.
 // The workspace must have been prepared with prepare_fp32_workspace (which

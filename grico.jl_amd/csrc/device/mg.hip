@@ -37,14 +37,21 @@ namespace {
 
 template <class R>
 __global__ void k_jacobi_first(R* __restrict__ x, const R* __restrict__ b, const R* __restrict__ dinv, R omega,
-                               int64_t n) {
+                               int64_t n, int64_t ld) {
+  const int64_t co = int64_t(blockIdx.y) * ld;  // column blockIdx.y of a block of columns
+  x += co;
+  b += co;
   GMF_GRID_STRIDE(i, n) x[i] = omega * (dinv[i] * b[i]);
 }
 
 // x += omega D^-1 (b - q), q = A x.
 template <class R>
 __global__ void k_jacobi_update(R* __restrict__ x, const R* __restrict__ b, const R* __restrict__ q,
-                                const R* __restrict__ dinv, R omega, int64_t n) {
+                                const R* __restrict__ dinv, R omega, int64_t n, int64_t ld) {
+  const int64_t co = int64_t(blockIdx.y) * ld;
+  x += co;
+  b += co;
+  q += co;
   GMF_GRID_STRIDE(i, n) x[i] = fma(omega, dinv[i] * (b[i] - q[i]), x[i]);
 }
 
@@ -52,7 +59,10 @@ __global__ void k_jacobi_update(R* __restrict__ x, const R* __restrict__ b, cons
 template <class R>
 __global__ void k_restrict(R* __restrict__ bc, const R* __restrict__ b, const R* __restrict__ q,
                            const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
-                           const R* __restrict__ val, int64_t nc) {
+                           const R* __restrict__ val, int64_t nc, int64_t ldc, int64_t ldf) {
+  bc += int64_t(blockIdx.y) * ldc;
+  b += int64_t(blockIdx.y) * ldf;
+  if (q) q += int64_t(blockIdx.y) * ldf;
   GMF_GRID_STRIDE(c, nc) {
     R s = R(0);
     for (int64_t t = off[c]; t < off[c + 1]; ++t) {
@@ -67,7 +77,10 @@ __global__ void k_restrict(R* __restrict__ bc, const R* __restrict__ b, const R*
 // x[f] += sum_c P[f,c] xc[c].
 template <class R>
 __global__ void k_prolong_add(R* __restrict__ x, const R* __restrict__ xc, const int64_t* __restrict__ off,
-                              const int32_t* __restrict__ idx, const R* __restrict__ val, int64_t nf) {
+                              const int32_t* __restrict__ idx, const R* __restrict__ val, int64_t nf, int64_t ldf,
+                              int64_t ldc) {
+  x += int64_t(blockIdx.y) * ldf;
+  xc += int64_t(blockIdx.y) * ldc;
   GMF_GRID_STRIDE(f, nf) {
     R s = R(0);
     for (int64_t t = off[f]; t < off[f + 1]; ++t) s = fma(val[t], xc[idx[t]], s);
@@ -80,8 +93,13 @@ __global__ void k_prolong_add(R* __restrict__ x, const R* __restrict__ xc, const
 // Chebyshev start from x = 0: r = b, d = s D^-1 r, x = d.
 template <class R>
 __global__ void k_cheb_start0(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
-                              const R* __restrict__ dinv, R s, int64_t n) {
+                              const R* __restrict__ dinv, R s, int64_t n, int64_t ld) {
 #pragma clang fp contract(off)
+  const int64_t co = int64_t(blockIdx.y) * ld;
+  x += co;
+  d += co;
+  r += co;
+  b += co;
   GMF_GRID_STRIDE(i, n) {
     const R ri = b[i];
     const R di = s * (dinv[i] * ri);
@@ -94,8 +112,14 @@ __global__ void k_cheb_start0(R* __restrict__ x, R* __restrict__ d, R* __restric
 // Chebyshev start from x with q = A x: r = b - q, d = s D^-1 r, x += d.
 template <class R>
 __global__ void k_cheb_start(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ b,
-                             const R* __restrict__ q, const R* __restrict__ dinv, R s, int64_t n) {
+                             const R* __restrict__ q, const R* __restrict__ dinv, R s, int64_t n, int64_t ld) {
 #pragma clang fp contract(off)
+  const int64_t co = int64_t(blockIdx.y) * ld;
+  x += co;
+  d += co;
+  r += co;
+  b += co;
+  q += co;
   GMF_GRID_STRIDE(i, n) {
     const R ri = b[i] - q[i];
     const R di = s * (dinv[i] * ri);
@@ -108,8 +132,13 @@ __global__ void k_cheb_start(R* __restrict__ x, R* __restrict__ d, R* __restrict
 // Chebyshev step with q = A d: r -= q, d = c1 d + c2 D^-1 r, x += d.
 template <class R>
 __global__ void k_cheb_step(R* __restrict__ x, R* __restrict__ d, R* __restrict__ r, const R* __restrict__ q,
-                            const R* __restrict__ dinv, R c1, R c2, int64_t n) {
+                            const R* __restrict__ dinv, R c1, R c2, int64_t n, int64_t ld) {
 #pragma clang fp contract(off)
+  const int64_t co = int64_t(blockIdx.y) * ld;
+  x += co;
+  d += co;
+  r += co;
+  q += co;
   GMF_GRID_STRIDE(i, n) {
     const R ri = r[i] - q[i];
     const R di = fma(c1, d[i], c2 * (dinv[i] * ri));
@@ -206,6 +235,10 @@ struct Level {
   gmf_workspace* ws = nullptr;
   int64_t n = 0;
   DeviceBuffer<R> invdiag, x, b, q, r, d;  // invdiag: zero on block dofs when the level has blocks
+  // The same vectors for a block of columns (column-major, n x kb), allocated
+  // on the first block V-cycle with kb columns or more.
+  int kb = 0;
+  DeviceBuffer<R> xb, bb, qb, rb, db;
   BlockSet<R> blocks;                      // cell blocks of the block smoother (count 0: point smoothing)
   DeviceBuffer<double> gV, gZ, gy;  // GMRES smoother Krylov basis, preconditioned basis, coefficients
   double lambda_max = 0.0;
@@ -270,10 +303,12 @@ struct MGPrecond : MGBase {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   DeviceBuffer<R> top_b, top_x;
+  rocblas_handle blas = nullptr;  // dense coarse products of block V-cycles
 
   static constexpr bool kDouble = std::is_same<R, double>::value;
 
   ~MGPrecond() override {
+    if (blas) (void)rocblas_destroy_handle(blas);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -410,7 +445,29 @@ struct MGPrecond : MGBase {
     sync(lw);
   }
 
-  void smooth(int l, R* x, const R* b, int steps, bool x_zero) {
+  // Level work vectors for k columns (k = 1: the level's own vectors).
+  struct Work {
+    R *x, *b, *q, *r, *d;
+  };
+  Work work(Level<R>& L, int k) {
+    if (k == 1) return Work{L.x.p, L.b.p, L.q.p, L.r.p, L.d.p};
+    return Work{L.xb.p, L.bb.p, L.qb.p, L.rb.p, L.db.p};
+  }
+
+  void apply_level(Level<R>& L, const R* x, R* q, int k) {
+    if (k == 1) {
+      level_apply(L.ws, x, q);
+      return;
+    }
+    if constexpr (kDouble) {
+      reduced_apply_block(L.ws, x, q, k);
+    } else {
+      require(false, GMF_EINVAL, "block V-cycles run in FP64");
+    }
+  }
+
+  // steps smoothing sweeps on k columns (x, b: n x k, column-major).
+  void smooth(int l, R* x, const R* b, int steps, bool x_zero, int k = 1) {
     Level<R>& L = levels[size_t(l)];
     const int64_t n = L.n;
     hipStream_t s = ws->stream;
@@ -419,17 +476,18 @@ struct MGPrecond : MGBase {
       if constexpr (kDouble) smooth_gmres(l, x, b, steps, x_zero);
       return;
     }
+    const Work w = work(L, k);
+    const dim3 grid(grid1(n), unsigned(k));
     if (smoother == GMF_SMOOTHER_JACOBI) {
-      for (int k = 0; k < steps; ++k) {
-        if (x_zero && k == 0) {
-          hipLaunchKernelGGL(k_jacobi_first<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.invdiag.p, R(omega), n);
-          block_add<R>(L.blocks, x, nullptr, b, nullptr, R(omega), s);
+      for (int it = 0; it < steps; ++it) {
+        if (x_zero && it == 0) {
+          hipLaunchKernelGGL(k_jacobi_first<R>, grid, dim3(256), 0, s, x, b, L.invdiag.p, R(omega), n, n);
+          block_add<R>(L.blocks, x, nullptr, b, nullptr, R(omega), s, k, n);
         } else {
-          level_apply(L.ws, x, L.q.p);
-          hipLaunchKernelGGL(k_jacobi_update<R>, dim3(grid1(n)), dim3(256), 0, s, x, b, L.q.p, L.invdiag.p,
-                             R(omega), n);
+          apply_level(L, x, w.q, k);
+          hipLaunchKernelGGL(k_jacobi_update<R>, grid, dim3(256), 0, s, x, b, w.q, L.invdiag.p, R(omega), n, n);
           // The block part uses the same residual b - A x_old (q is unchanged).
-          block_add<R>(L.blocks, x, nullptr, b, L.q.p, R(omega), s);
+          block_add<R>(L.blocks, x, nullptr, b, w.q, R(omega), s, k, n);
         }
         GMF_HIP(hipGetLastError());
       }
@@ -440,26 +498,26 @@ struct MGPrecond : MGBase {
     const double sigma = theta / delta;
     // With blocks, each start / step kernel forms the point part of S r and
     // block_add then adds the blocks' part to d and x (r is final by then).
-    for (int k = 0; k < steps; ++k) {
-      if (x_zero && k == 0) {
-        hipLaunchKernelGGL(k_cheb_start0<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.invdiag.p,
-                           R(1.0 / theta), n);
-      } else if (!cheb_fused(L, x, ChebFused<R>{x, L.d.p, L.r.p, b, L.invdiag.p, R(0), R(1.0 / theta), 1})) {
-        level_apply(L.ws, x, L.q.p);
-        hipLaunchKernelGGL(k_cheb_start<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, b, L.q.p,
-                           L.invdiag.p, R(1.0 / theta), n);
+    // The fused brick steps serve one column.
+    for (int it = 0; it < steps; ++it) {
+      if (x_zero && it == 0) {
+        hipLaunchKernelGGL(k_cheb_start0<R>, grid, dim3(256), 0, s, x, w.d, w.r, b, L.invdiag.p, R(1.0 / theta), n,
+                           n);
+      } else if (k > 1 || !cheb_fused(L, x, ChebFused<R>{x, L.d.p, L.r.p, b, L.invdiag.p, R(0), R(1.0 / theta), 1})) {
+        apply_level(L, x, w.q, k);
+        hipLaunchKernelGGL(k_cheb_start<R>, grid, dim3(256), 0, s, x, w.d, w.r, b, w.q, L.invdiag.p, R(1.0 / theta),
+                           n, n);
       }
-      block_add<R>(L.blocks, x, L.d.p, L.r.p, nullptr, R(1.0 / theta), s);
+      block_add<R>(L.blocks, x, w.d, w.r, nullptr, R(1.0 / theta), s, k, n);
       double rho = 1.0 / sigma;
       for (int i = 1; i < cheb_degree; ++i) {
         const double rho_new = 1.0 / (2.0 * sigma - rho);
         const R c1 = R(rho_new * rho), c2 = R(2.0 * rho_new / delta);
-        if (!cheb_fused(L, L.d.p, ChebFused<R>{x, L.d.p, L.r.p, nullptr, L.invdiag.p, c1, c2, 0})) {
-          level_apply(L.ws, L.d.p, L.q.p);
-          hipLaunchKernelGGL(k_cheb_step<R>, dim3(grid1(n)), dim3(256), 0, s, x, L.d.p, L.r.p, L.q.p, L.invdiag.p,
-                             c1, c2, n);
+        if (k > 1 || !cheb_fused(L, L.d.p, ChebFused<R>{x, L.d.p, L.r.p, nullptr, L.invdiag.p, c1, c2, 0})) {
+          apply_level(L, w.d, w.q, k);
+          hipLaunchKernelGGL(k_cheb_step<R>, grid, dim3(256), 0, s, x, w.d, w.r, w.q, L.invdiag.p, c1, c2, n, n);
         }
-        block_add<R>(L.blocks, x, L.d.p, L.r.p, nullptr, c2, s);
+        block_add<R>(L.blocks, x, w.d, w.r, nullptr, c2, s, k, n);
         rho = rho_new;
       }
       GMF_HIP(hipGetLastError());
@@ -507,32 +565,85 @@ struct MGPrecond : MGBase {
     }
   }
 
-  // x is treated as zero on entry.
-  void vcycle(int l, R* x, const R* b) {
+  // Coarse solve of k columns: one product with the dense inverse (rocBLAS
+  // GEMM), or column by column.
+  void coarse_solve_block(double* x, const double* b, int k) {
+    Level<R>& L = levels[0];
+    if (L.n == 0) return;
+    if (dense) {
+      if (!blas) {
+        require(rocblas_create_handle(&blas) == rocblas_status_success, GMF_EDEVICE, "rocblas_create_handle failed");
+      }
+      require(rocblas_set_stream(blas, ws->stream) == rocblas_status_success, GMF_EDEVICE, "rocblas_set_stream failed");
+      // coarse_inverse is row-major: as a column-major matrix it is the
+      // transpose, so x = A^-1 b is op(A^T) with op = transpose.
+      const double one = 1.0, zero = 0.0;
+      const rocblas_int m = rocblas_int(L.n);
+      require(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, m, rocblas_int(k), m, &one,
+                            coarse_inverse.p, m, b, m, &zero, x, m) == rocblas_status_success,
+              GMF_EDEVICE, "rocblas_dgemm failed");
+      return;
+    }
+    for (int j = 0; j < k; ++j) coarse_solve64(x + int64_t(j) * L.n, b + int64_t(j) * L.n);
+  }
+
+  // x is treated as zero on entry; k columns (column-major, the level's n rows).
+  void vcycle(int l, R* x, const R* b, int k = 1) {
     if (l == 0) {
-      coarse_solve(x, b);
+      if (k == 1) {
+        coarse_solve(x, b);
+      } else if constexpr (kDouble) {
+        coarse_solve_block(x, b, k);
+      }
       return;
     }
     Level<R>& L = levels[size_t(l)];
     Level<R>& C = levels[size_t(l - 1)];
     TransferCsr<R>& T = transfers[size_t(l - 1)];
     hipStream_t s = ws->stream;
+    const Work w = work(L, k), wc = work(C, k);
     if (pre > 0) {
-      smooth(l, x, b, pre, true);
-      level_apply(L.ws, x, L.q.p);
-      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, L.q.p, T.roff.p, T.ridx.p,
-                         T.rval.p, T.nc);
+      smooth(l, x, b, pre, true, k);
+      apply_level(L, x, w.q, k);
+      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc), unsigned(k)), dim3(256), 0, s, wc.b, b, w.q, T.roff.p,
+                         T.ridx.p, T.rval.p, T.nc, C.n, L.n);
     } else {
-      fill(x, R(0), L.n, s);
-      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc)), dim3(256), 0, s, C.b.p, b, nullptr, T.roff.p, T.ridx.p,
-                         T.rval.p, T.nc);
+      fill(x, R(0), L.n * k, s);
+      hipLaunchKernelGGL(k_restrict<R>, dim3(grid1(T.nc), unsigned(k)), dim3(256), 0, s, wc.b, b, nullptr, T.roff.p,
+                         T.ridx.p, T.rval.p, T.nc, C.n, L.n);
     }
     GMF_HIP(hipGetLastError());
-    vcycle(l - 1, C.x.p, C.b.p);
-    hipLaunchKernelGGL(k_prolong_add<R>, dim3(grid1(T.nf)), dim3(256), 0, s, x, C.x.p, T.poff.p, T.pidx.p,
-                       T.pval.p, T.nf);
+    vcycle(l - 1, wc.x, wc.b, k);
+    hipLaunchKernelGGL(k_prolong_add<R>, dim3(grid1(T.nf), unsigned(k)), dim3(256), 0, s, x, wc.x, T.poff.p,
+                       T.pidx.p, T.pval.p, T.nf, L.n, C.n);
     GMF_HIP(hipGetLastError());
-    smooth(l, x, b, post, false);
+    smooth(l, x, b, post, false, k);
+  }
+
+  // Z = M^-1 R for k columns at once (FP64 V-cycles with Jacobi or Chebyshev
+  // smoothing): every level applies, smooths and transfers all columns per
+  // launch, and a dense coarse level solves them with one matrix product.
+  bool apply_block(const double* r, double* z, int64_t n, int k) override {
+    if constexpr (!kDouble) {
+      return false;
+    } else {
+      if (k < 2 || smoother == GMF_SMOOTHER_GMRES) return false;
+      const int top = int(levels.size()) - 1;
+      require(levels[size_t(top)].n == n, GMF_EINVAL, "block size does not match the multigrid top level");
+      for (Level<R>& L : levels) {
+        if (L.kb >= k) continue;
+        const size_t m = size_t(L.n) * size_t(k);
+        L.xb = DeviceBuffer<R>(m);
+        L.bb = DeviceBuffer<R>(m);
+        L.qb = DeviceBuffer<R>(m);
+        L.rb = DeviceBuffer<R>(m);
+        L.db = DeviceBuffer<R>(m);
+        L.kb = k;
+      }
+      set_streams(ws->stream);
+      vcycle(top, z, r, k);
+      return true;
+    }
   }
 };
 

@@ -1191,6 +1191,17 @@ gmf_status gmf_precond_apply(gmf_preconditioner* M, const double* r, double* z) 
   });
 }
 
+gmf_status gmf_precond_apply_block(gmf_preconditioner* M, const double* R, double* Z, int32_t k) {
+  return guarded([&] {
+    require(M && R && Z && k >= 1, GMF_EINVAL, "invalid argument");
+    require(R != Z, GMF_EALIAS, "result and input blocks must not alias");
+    const int64_t n = M->ws->plan->n_red;
+    if (!M->apply_block(R, Z, n, k))
+      for (int j = 0; j < k; ++j) M->apply(R + int64_t(j) * n, Z + int64_t(j) * n);
+    sync(M->ws);
+  });
+}
+
 gmf_status gmf_cg_solve(gmf_workspace* ws, gmf_preconditioner* M, const double* b, double* x, int32_t use_initial,
                         double rtol, double atol, int64_t maxiter, gmf_solve_info* info) {
   gmf_solve_info local{};

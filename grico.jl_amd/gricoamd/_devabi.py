@@ -111,6 +111,7 @@ def declare(lib):
     lib.gmf_precond_create_jacobi.argtypes = [c_void_p, c_void_p, ctypes.POINTER(c_void_p)]
     lib.gmf_precond_destroy.argtypes = [c_void_p]
     lib.gmf_precond_apply.argtypes = [c_void_p, c_void_p, c_void_p]
+    lib.gmf_precond_apply_block.argtypes = [c_void_p, c_void_p, c_void_p, c_int32]
     lib.gmf_expand_reduced.argtypes = [c_void_p, c_void_p, c_void_p, c_int32]
     lib.gmf_rhs_physical.argtypes = [c_void_p, c_void_p, c_int32, c_void_p]
     lib.gmf_reduced_rhs.argtypes = [c_void_p, c_void_p, c_void_p]
@@ -137,7 +138,7 @@ def declare(lib):
                  "gmf_diagonal_physical", "gmf_reduced_diagonal", "gmf_invert_diagonal", "gmf_cg_solve",
                  "gmf_fgmres_solve", "gmf_rhs_physical", "gmf_reduced_rhs", "gmf_l2_error",
                  "gmf_interpolate_channels", "gmf_plan_set_point_data", "gmf_rhs_faces", "gmf_device_info", "gmf_apply_timed", "gmf_precond_create_jacobi",
-                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_plan_describe", "gmf_vector_copy",
+                 "gmf_precond_destroy", "gmf_precond_apply", "gmf_precond_apply_block", "gmf_plan_describe", "gmf_vector_copy",
                  "gmf_reduced_apply_block", "gmf_cg_solve_block", "gmf_precond_create_jacobi_f32"):
         getattr(lib, name).restype = ctypes.c_int
 

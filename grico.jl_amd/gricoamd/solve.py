@@ -81,6 +81,21 @@ class DevicePreconditioner:
         _devabi.check(_native.device().gmf_precond_apply(self.handle, rt.data_ptr(), z.data_ptr()))
         return z
 
+    def apply_block(self, R):
+        """Z = M^-1 R for the columns of R (n x k): one block V-cycle for FP64
+        multigrid with Jacobi or Chebyshev smoothing, else column by column
+        (gmf_precond_apply_block)."""
+        torch = ops._torch()
+        n = self.plan.reduced_dof_count
+        Rt = torch.as_tensor(R, dtype=torch.float64).to("cuda")
+        if Rt.ndim != 2 or Rt.shape[0] != n:
+            raise ValueError(f"expected a block of shape ({n}, k)")
+        Rc = Rt.t().contiguous()  # column-major storage: one column after another
+        Z = torch.empty_like(Rc)
+        _devabi.check(_native.device().gmf_precond_apply_block(self.handle, Rc.data_ptr(), Z.data_ptr(),
+                                                               int(Rt.shape[1])))
+        return Z.t()
+
 
 def reduced_diagonal_general(plan):
     """Diagonal of P^T A P when P is not a selection (src/assembly.jl:662-675

@@ -288,6 +288,12 @@ gmf_status gmf_precond_create_jacobi_f32(gmf_workspace* ws, const double* invers
 gmf_status gmf_precond_destroy(gmf_preconditioner* precond);
 /* z = M^-1 r on the workspace stream. */
 gmf_status gmf_precond_apply(gmf_preconditioner* precond, const double* r, double* z);
+/* Z = M^-1 R for k columns of the preconditioner's reduced size (column-major
+ * device blocks), what the block solves apply: FP64 multigrid V-cycles with
+ * Jacobi or Chebyshev smoothing run all columns per launch (a dense coarse
+ * level as one matrix product), other preconditioners column by column
+ * (src/solve.jl:753-774 _apply_preconditioner! per right-hand side). */
+gmf_status gmf_precond_apply_block(gmf_preconditioner* precond, const double* R, double* Z, int32_t k);
 
 /* Preconditioned CG on the reduced system from x (in: initial guess if
  * use_initial, else zero), stopping at ||r|| <= max(atol, rtol ||b||)

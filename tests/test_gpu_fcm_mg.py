@@ -131,3 +131,45 @@ def test_whole_cell_block_vcycle_matches_oracle(gpu, precision):
                           lambdas=h.level_lambda_max(), cheb_degree=2, cheb_range=pol.chebyshev_range, blocks=blocks)
     tol = 1e-10 if precision == "fp64" else 5e-5
     assert np.abs(z - ref).max() <= tol * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+@pytest.mark.parametrize("blocks", [None, "cut"])
+@pytest.mark.parametrize("coarse", ["dense", "cg"])
+def test_block_vcycle_matches_columns(gpu, smoother, blocks, coarse):
+    # gmf_precond_apply_block: one V-cycle over 5 columns (all levels, a
+    # dense coarse level as one matrix product) gives every column the
+    # one-column V-cycle's result
+    pol = GeometricMultigridPreconditioner(smoother=smoother, cell_blocks=blocks, pre_smoothing_steps=1,
+                                           post_smoothing_steps=1,
+                                           coarse_direct_dof_limit=4096 if coarse == "dense" else 0)
+    h = compile_geometric_multigrid(_cut_problem(), pol)
+    M = h.device_preconditioner()
+    n = h.plans[-1].reduced_dof_count
+    R = np.stack([P.rng_vector(n, 20 + j) for j in range(5)], axis=1)
+    Z = M.apply_block(R).cpu().numpy()
+    tol = 1e-12 if coarse == "dense" else 1e-9  # the coarse CG stops at its own tolerance
+    for j in range(R.shape[1]):
+        zj = M.apply(R[:, j]).cpu().numpy()
+        assert np.abs(Z[:, j] - zj).max() <= tol * np.abs(zj).max(), j
+
+
+def test_block_vcycle_physical_hierarchy_and_fp32_fallback(gpu):
+    # h- and p-levels of a finite-cell ball with cut-cell blocks (FP64: one
+    # block V-cycle); the FP32 V-cycle applies column by column
+    pr = P.fcm_ball_physical(n=4, degree=2, depth=2, n_theta=16, n_phi=32)
+    for precision in ("fp64", "fp32"):
+        pol = GeometricMultigridPreconditioner(smoother="chebyshev", cell_blocks="cut", h_steps_per_level=3,
+                                               chebyshev_degree=3, precision=precision,
+                                               coarse_direct_dof_limit=40000)
+        h = compile_geometric_multigrid(pr, pol)
+        M = h.device_preconditioner()
+        n = h.plans[-1].reduced_dof_count
+        R = np.stack([P.rng_vector(n, 40 + j) for j in range(3)], axis=1)
+        Z = M.apply_block(R).cpu().numpy()
+        # FP32 V-cycles sum some contributions with atomics: two applies of
+        # one column agree to FP32 rounding, not bit for bit
+        tol = 1e-12 if precision == "fp64" else 1e-5
+        for j in range(R.shape[1]):
+            zj = M.apply(R[:, j]).cpu().numpy()
+            assert np.abs(Z[:, j] - zj).max() <= tol * np.abs(zj).max(), (precision, j)
