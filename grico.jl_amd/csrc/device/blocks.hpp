@@ -73,12 +73,72 @@ __global__ void __launch_bounds__(64 * kBlocksPerGroup)
   }
 }
 
-// k columns at leading dimension ld (k = 1: one vector).
+// Columns per wavefront of k_block_add_cols.
+constexpr int kBlockCols = 8;
+
+// k_block_add on up to kBlockCols columns per wavefront (column group
+// blockIdx.y, leading dimension ld): each block's inverse is read once for
+// the group; per column the arithmetic and summation order of k_block_add.
+// Dynamic LDS: kBlocksPerGroup x kBlockCols x mmax residual values.
+template <class R>
+__global__ void __launch_bounds__(64 * kBlocksPerGroup)
+    k_block_add_cols(R* __restrict__ x, R* __restrict__ d, const R* __restrict__ r, const R* __restrict__ q, R s,
+                     const int64_t* __restrict__ off, const int32_t* __restrict__ dofs,
+                     const int64_t* __restrict__ voff, const R* __restrict__ inv, int64_t count, int64_t ld, int k,
+                     int mmax) {
+  extern __shared__ unsigned char block_smem[];
+  R* rs = reinterpret_cast<R*>(block_smem);
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t b = int64_t(blockIdx.x) * kBlocksPerGroup + w;
+  const int c0 = int(blockIdx.y) * kBlockCols;
+  const int kc = k - c0 < kBlockCols ? k - c0 : kBlockCols;
+  const bool live = b < count;
+  const int64_t o = live ? off[b] : 0;
+  const int m = live ? int(off[b + 1] - o) : 0;
+  R* rw = rs + int64_t(w) * kBlockCols * mmax;  // this wavefront's kc x m residuals
+  for (int i = lane; i < m; i += 64) {
+    const int32_t g = dofs[o + i];
+    for (int c = 0; c < kc; ++c) {
+      const int64_t cg = int64_t(c0 + c) * ld + g;
+      rw[c * mmax + i] = q ? r[cg] - q[cg] : r[cg];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const R* B = inv + voff[b];
+  for (int i = lane; i < m; i += 64) {
+    R t[kBlockCols];
+#pragma unroll
+    for (int c = 0; c < kBlockCols; ++c) t[c] = R(0);
+    for (int j = 0; j < m; ++j) {
+      const R e = B[int64_t(j) * m + i];
+#pragma unroll
+      for (int c = 0; c < kBlockCols; ++c) t[c] = fma(e, rw[c * mmax + j], t[c]);
+    }
+    const int32_t g = dofs[o + i];
+    for (int c = 0; c < kc; ++c) {
+      const int64_t cg = int64_t(c0 + c) * ld + g;
+      unsafeAtomicAdd(&x[cg], s * t[c]);
+      if (d) unsafeAtomicAdd(&d[cg], s * t[c]);
+    }
+  }
+}
+
+// k columns at leading dimension ld (k = 1: one vector). Several columns go
+// through k_block_add_cols when its residual staging fits 64 KB of LDS.
 template <class R>
 inline void block_add(const BlockSet<R>& bs, R* x, R* d, const R* r, const R* q, R s, hipStream_t stream, int k = 1,
                       int64_t ld = 0) {
   if (bs.count == 0) return;
   const unsigned groups = unsigned((bs.count + kBlocksPerGroup - 1) / kBlocksPerGroup);
+  const size_t cols_lds = sizeof(R) * size_t(kBlocksPerGroup) * kBlockCols * size_t(bs.max_dofs);
+  if (k > 1 && cols_lds <= 64 * 1024) {
+    hipLaunchKernelGGL(k_block_add_cols<R>, dim3(groups, unsigned((k + kBlockCols - 1) / kBlockCols)),
+                       dim3(64 * kBlocksPerGroup), cols_lds, stream, x, d, r, q, s, bs.offsets.p, bs.dofs.p,
+                       bs.value_offsets.p, bs.inverse.p, bs.count, ld, k, bs.max_dofs);
+    GMF_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(k_block_add<R>, dim3(groups, unsigned(k)), dim3(64 * kBlocksPerGroup), 0, stream, x, d, r, q, s,
                      bs.offsets.p, bs.dofs.p, bs.value_offsets.p, bs.inverse.p, bs.count, ld);
   GMF_HIP(hipGetLastError());
