@@ -1016,7 +1016,8 @@ def build_fcm_problem(cells, degree, depth, alpha=1e-3):
     return pr
 
 
-def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, oracle_cells=4, precision="fp64"):
+def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, oracle_cells=4, precision="fp64",
+            max_levels=3):
     """Finite-cell sphere workload with nrhs seeded N(0,1) reduced right-hand
     sides solved to 1e-8 by block CG (FP64 Krylov, lockstep columns) with the
     hp-multigrid preconditioner on physical-domain levels (cut-cell rules
@@ -1081,7 +1082,8 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, or
     # hp-multigrid on physical-domain levels with cell-block smoothing.
     t0 = time.perf_counter()
     pol = GeometricMultigridPreconditioner(smoother="chebyshev", cell_blocks="cut", h_steps_per_level=3,
-                                           chebyshev_degree=3, precision=precision, coarse_direct_dof_limit=40000)
+                                           chebyshev_degree=3, precision=precision, coarse_direct_dof_limit=40000,
+                                           max_levels=max_levels)
     h = compile_geometric_multigrid(problem, pol, fine_plan=plan)
     M = h.device_preconditioner()
     torch.cuda.synchronize()
