@@ -1050,7 +1050,7 @@ def fcm_leg(cells=48, degree=3, depth=3, nrhs=64, maxiter=2000, sequential=2, or
                 "cut_cells": sum(b.cell_count for b in cut),
                 "cut_cell_rules": "moment-fitted (src/regions.jl:519-708) plus alpha times the tensor rule",
                 "cut_cell_points": int(sum(b.n_points_total for b in cut)),
-                "kernel_tuning": [t for t in dev.tuning() if t["kernel"].startswith("point-cloud")],
+                "kernel_tuning": [t for t in dev.tuning() if t["kernel"] != "empty"],
                 "surface_pieces": sum(b.cell_count for b in plan.surface_buckets),
                 "surface_points": int(sum(b.n_points_total for b in plan.surface_buckets))})
     log(f"fcm: {out}")

@@ -337,6 +337,9 @@ struct BrickArgs {
   const R* jinv;  // cells x 3
   const R* coef;  // cells x N^3
   R* halo;        // halo values of the bucket (shared-dof major)
+  // bricks x 8 cells, -1 for an empty position (zero coefficients: the
+  // position adds nothing); null: brick b holds cells 8 b .. 8 b + 7.
+  const int32_t* cell_of;
 };
 
 // Result stores of the plain apply: y[i] (+)= v.
@@ -410,7 +413,9 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
   bool act;
   lane_coords<N>(lane_in_cell, t0, t1, act);
   const int64_t brick = xcd_block(blockIdx.x, gridDim.x);
-  const int64_t cell = brick * 8 + lc;
+  const int64_t slot_cell = a.cell_of ? int64_t(a.cell_of[brick * 8 + lc]) : brick * 8 + lc;
+  const bool present = slot_cell >= 0;
+  const int64_t cell = present ? slot_cell : 0;
   // Phase 1: entries, coefficients, slot list, inputs.
   const int32_t* map = a.map + brick * L;
   int32_t e[PER];
@@ -424,7 +429,7 @@ __global__ void __launch_bounds__(8 * lanes_per_cell<N>()) brick_apply(BrickArgs
 #pragma unroll
   for (int k = 0; k < CPL; ++k) {
     const int i = lane_in_cell + k * LPC;
-    cv[k] = i < NNN ? cb[i] : R(0);
+    cv[k] = i < NNN && present ? cb[i] : R(0);
   }
   const int32_t* brick_slots = a.halo_slot + a.halo_base[brick];
   const R jx = a.jinv[cell * 3], jy = a.jinv[cell * 3 + 1], jz = a.jinv[cell * 3 + 2];
@@ -589,6 +594,7 @@ void launch_bricks(const Bucket& b, const R* x, bool reduced, R* slots, hipStrea
   a.jinv = CellArrays<R>::jinv(b);
   a.coef = CellArrays<R>::coef(b);
   a.halo = slots ? slots + bp.slot_base : nullptr;
+  a.cell_of = bp.cell_of;
   hipLaunchKernelGGL((brick_apply<N, R, Store>), dim3(unsigned(bp.bricks)), dim3(8 * lanes_per_cell<N>()),
                      (brick_lds_bytes<N, R>()), stream, a, T, x, store);
   if (bp.shared_count > 0)
@@ -630,20 +636,38 @@ std::vector<double> collocation_derivative(const std::vector<double>& B, const s
 }
 
 // Build and verify the 2x2x2 brick decomposition of a bucket: every group of
-// eight consecutive cells must place its modes on a common (2N-1)^3 lattice
+// eight consecutive cells (or the cells brick_cells assigns to a brick, with
+// empty positions) must place its modes on a common (2N-1)^3 lattice
 // consistently (cells sharing a lattice point agree on its dof) and every
-// dof may appear at most once per brick. Returns false when the cells do not
-// form such bricks.
+// dof may appear at most once per brick. Lattice points no cell of a grouped
+// brick covers hold no dof. Returns false when the cells do not form such
+// bricks.
 //
 // Dofs held by one brick only are stored by brick_apply directly; every
 // occurrence of a dof held by several bricks writes its own halo slot, and
 // k_brick_halo sums a dof's slots (contiguous, in brick order).
-bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full) {
+bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full,
+                  const int32_t* brick_cells, int64_t brick_count) {
   const int N = b.N[0];
   const int M = 2 * N - 1, L = M * M * M;
   const int NNN = N * N * N;
-  if (b.cells % 8 != 0 || b.cells == 0) return false;
-  const int64_t nb = b.cells / 8;
+  if (b.cells == 0) return false;
+  if (brick_cells) {
+    // every cell of the batch exactly once
+    require(brick_count >= 1 && brick_count <= (int64_t(1) << 27), GMF_EINVAL, "invalid brick count");
+    std::vector<char> placed(size_t(b.cells), 0);
+    for (int64_t i = 0; i < brick_count * 8; ++i) {
+      const int32_t c = brick_cells[i];
+      require(c >= -1 && c < b.cells, GMF_EINVAL, "brick cell out of range");
+      if (c < 0) continue;
+      require(!placed[size_t(c)], GMF_EINVAL, "a cell appears in two brick positions");
+      placed[size_t(c)] = 1;
+    }
+    for (int64_t c = 0; c < b.cells; ++c) require(placed[size_t(c)], GMF_EINVAL, "a cell belongs to no brick");
+  } else if (b.cells % 8 != 0) {
+    return false;
+  }
+  const int64_t nb = brick_cells ? brick_count : b.cells / 8;
   if (nb > (int64_t(1) << 27) || p.n_full >= (int64_t(1) << 29)) return false;
   std::vector<int32_t> map(size_t(nb) * size_t(L));
   auto latp = [N](int m) { return m == 0 ? 0 : (m == 1 ? N - 1 : m - 1); };
@@ -653,7 +677,9 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
     std::fill(lattice.begin(), lattice.end(), -2);
     for (int c = 0; c < 8; ++c) {
       const int ox = (c & 1) * (N - 1), oy = ((c >> 1) & 1) * (N - 1), oz = ((c >> 2) & 1) * (N - 1);
-      const int32_t* li = lidx + (br * 8 + c) * NNN;
+      const int64_t cell = brick_cells ? int64_t(brick_cells[br * 8 + c]) : br * 8 + c;
+      if (cell < 0) continue;
+      const int32_t* li = lidx + cell * NNN;
       for (int k = 0; k < N; ++k)
         for (int j = 0; j < N; ++j)
           for (int i = 0; i < N; ++i) {
@@ -667,8 +693,11 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
           }
     }
     for (int pos = 0; pos < L; ++pos) {
-      const int32_t g = lattice[size_t(pos)];
-      if (g == -2) return false;
+      int32_t g = lattice[size_t(pos)];
+      if (g == -2) {
+        if (!brick_cells) return false;
+        g = -1;  // covered by no cell of the brick
+      }
       if (g >= 0) {
         if (seen_brick[size_t(g)] == br) return false;  // dof twice in one brick
         seen_brick[size_t(g)] = br;
@@ -729,13 +758,20 @@ bool build_bricks(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector
   }
   bp.slot_base = p.brick_slots;
   p.brick_slots += slots;
+  bp.cell_of = nullptr;
+  if (brick_cells) {
+    bool consecutive = nb * 8 == b.cells;
+    for (int64_t i = 0; consecutive && i < nb * 8; ++i) consecutive = brick_cells[i] == i;
+    if (!consecutive) bp.cell_of = upload_array(p, brick_cells, size_t(nb) * 8);
+  }
   bp.enabled = true;
   return true;
 }
 
 }  // namespace
 
-void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full) {
+void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full,
+                       const int32_t* brick_cells, int64_t brick_count) {
   b.fast = false;
   if (b.dim != 3 || b.form != GMF_FORM_DIFFUSION || b.multi_count != 0) return;
   const int n = b.N[0];
@@ -760,7 +796,8 @@ void prepare_fast_path(gmf_plan& p, Bucket& b, const int32_t* lidx, const std::v
   if (const char* env = std::getenv("GMF_BRICK_MIN_CELLS")) min_cells = std::strtoll(env, nullptr, 10);
   const std::string per_n = "GMF_BRICK_MIN_CELLS_N" + std::to_string(n);
   if (const char* env = std::getenv(per_n.c_str())) min_cells = std::strtoll(env, nullptr, 10);
-  if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells) build_bricks(p, b, lidx, red_of_full);
+  if (b.lcoef == nullptr && n <= 6 && b.cells >= min_cells)
+    build_bricks(p, b, lidx, red_of_full, brick_cells, brick_count);
 }
 
 namespace {

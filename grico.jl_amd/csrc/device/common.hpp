@@ -56,6 +56,9 @@ struct BrickPath {
   int32_t* halo_red = nullptr;
   int32_t* halo_ptr = nullptr;
   int64_t slot_base = 0;       // offset of this bucket's slots in the workspace array
+  // bricks x 8 batch cells (-1: empty position) when the bricks come from an
+  // explicit grouping; null: brick b holds cells 8 b .. 8 b + 7.
+  int32_t* cell_of = nullptr;
 };
 
 // Sum-factorized path of a point-cloud batch of conforming faces
@@ -352,7 +355,8 @@ bool launch_cell_fast(const gmf_plan& plan, const Bucket& b, const double* x, do
                       hipStream_t stream);
 // lidx: host copy of the bucket's full-layout index map; red_of_full: full dof
 // -> reduced dof (empty unless the reduced map is a selection).
-void prepare_fast_path(gmf_plan& plan, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full);
+void prepare_fast_path(gmf_plan& plan, Bucket& b, const int32_t* lidx, const std::vector<int64_t>& red_of_full,
+                       const int32_t* brick_cells = nullptr, int64_t brick_count = 0);
 bool plan_writes_every_dof(const gmf_plan& plan);
 // FP32 variant of launch_cell_fast for buckets prepared by prepare_fp32.
 bool launch_cell_fast32(const gmf_plan& plan, const Bucket& b, const float* x, float* y, bool reduced, float* slots,

@@ -786,7 +786,9 @@ gmf_status gmf_plan_create(const gmf_plan_desc* desc, gmf_plan** out) {
           }
           build_diagonal_pairs(p, b, r, bd, mr);
         }
-        if (!b.cloud) prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>());
+        if (!b.cloud)
+          prepare_fast_path(p, b, bd.local_index, p.selection_map ? red_of_full : std::vector<int64_t>(),
+                            bd.brick_cells, bd.brick_count);
         else prepare_face_path(p, b, bd);
         p.buckets.push_back(std::move(b));
       }

@@ -26,6 +26,7 @@ class BucketDesc(ctypes.Structure):
         ("multi_count", c_int64), ("multi_offsets", P_i64), ("multi_indices", P_i32),
         ("multi_coefficients", P_f64), ("point_offsets", P_i64), ("ref_points", P_f64),
         ("sides", c_int32), ("plus_modes", c_int32 * 3),
+        ("brick_cells", P_i32), ("brick_count", c_int64),
     ]
 
 
@@ -207,6 +208,12 @@ class DevicePlan:
             d.multi_offsets = _p(b.multi_offsets, c_int64)
             d.multi_indices = _p(b.multi_indices, c_int32)
             d.multi_coefficients = _p(b.multi_coefficients, c_double)
+            bricks = None if cloud else getattr(b, "brick_cells", None)
+            if bricks is not None:
+                bricks = np.ascontiguousarray(bricks, dtype=np.int32)
+                keep.append(bricks)
+                d.brick_cells = _p(bricks, c_int32)
+                d.brick_count = len(bricks)
         pd = PlanDesc()
         pd.dim = plan.dim
         pd.full_dof_count = plan.full_dof_count

@@ -66,6 +66,32 @@ class _TermLists:
         raise AttributeError(name)
 
 
+def brick_grouping(grid, cells):
+    """2x2x2 groups of cells by lattice position for the device brick kernel:
+    cells of one refinement level (per axis) with coordinates 2i, 2i+1 along
+    every axis form a brick, position x + 2y + 4z inside it. Returns a
+    (bricks x 8) int32 array of indices into `cells` (-1: no cell of the batch
+    at that position), bricks in the order of their first cell, or None when
+    the batch already consists of consecutive complete bricks (the device then
+    groups cells 8b..8b+7 by itself)."""
+    cells = np.asarray(cells, dtype=np.int64)
+    n = len(cells)
+    if n == 0:
+        return None
+    lev = grid.levels()[cells].astype(np.int64)
+    crd = grid.coords()[cells].astype(np.int64)
+    key = np.concatenate([lev, crd // 2], axis=1)
+    pos = crd[:, 0] % 2 + 2 * (crd[:, 1] % 2) + 4 * (crd[:, 2] % 2)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    rank = np.empty(len(first), dtype=np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(len(first))
+    out = np.full((len(first), 8), -1, dtype=np.int32)
+    out[rank[inv.reshape(-1)], pos] = np.arange(n, dtype=np.int32)
+    if len(first) * 8 == n and np.array_equal(out.reshape(-1), np.arange(n)):
+        return None
+    return out
+
+
 class Bucket(_TermLists):
     """Cells sharing degree tuple and quadrature shape (one device kernel batch)."""
 
@@ -103,6 +129,7 @@ class Bucket(_TermLists):
                             for a in range(D)]
         self.inverse_jacobian = np.ascontiguousarray(1.0 / halves)
         self._gather_maps(space)
+        self.brick_cells = brick_grouping(dom.grid, self.cells) if D == 3 else None
 
     def _gather_maps(self, space):
         """local_index / local_coefficient / multi-term CSR over the full mode box."""

@@ -101,6 +101,16 @@ typedef struct gmf_bucket_desc {
    * sides = 0 or 1: one-sided items (plus_modes ignored). */
   int32_t sides;
   int32_t plus_modes[3];
+  /* Optional 2x2x2 brick grouping of a tensor-product batch (NULL: consecutive
+   * groups of eight cells, or none). Brick b holds the batch cells
+   * brick_cells[8 b + c] (-1: no cell there) at position c = x + 2 y + 4 z
+   * within the brick; each cell of the batch appears once. With it, batches
+   * whose cells leave gaps in the 2x2x2 groups (cells cut by an embedded
+   * boundary taken out, cells outside the domain dropped) still run on the
+   * brick kernel. A traversal hint of the device plan with no counterpart in
+   * the reference, whose cell traversal is src/plans.jl:368-424. */
+  const int32_t* brick_cells;
+  int64_t brick_count;
 } gmf_bucket_desc;
 
 /* Compiled plan: cell batches + algebraic constraint rows + reduced map

@@ -250,3 +250,32 @@ def test_l2_layout_reproduces_host_error():
     assert any(not c for _, c in plan._l2_layout())  # the surface pieces take zero weights
     ref = relative_l2_error(st, P.ball_exact)
     assert abs(np.sqrt(num / den) - ref) <= 1e-12 * ref
+
+
+def test_brick_grouping_of_ball_tensor_cells():
+    # host grouping for the device brick kernel against a dictionary built
+    # cell by cell: (level, coordinate // 2) per axis names the brick, the
+    # coordinate parities the position; a complete Morton-ordered mesh needs none
+    import problems as P
+    from gricoamd.plan import brick_grouping
+    pr = P.fcm_ball(n=8, degree=1, depth=1, n_theta=12, n_phi=24)
+    plan = g.compile(pr)
+    b = next(b for b in plan.buckets if not getattr(b, "cloud", False))
+    grid = pr.field.space.domain.grid
+    lev, crd = grid.levels()[b.cells], grid.coords()[b.cells]
+    want = {}
+    for i in range(b.cell_count):
+        key = tuple(lev[i]) + tuple(crd[i] // 2)
+        want.setdefault(key, [-1] * 8)[int(crd[i][0] % 2 + 2 * (crd[i][1] % 2) + 4 * (crd[i][2] % 2))] = i
+    got = b.brick_cells
+    assert got is not None and got.shape == (len(want), 8)
+    assert sorted(map(tuple, got.tolist())) == sorted(map(tuple, want.values()))
+    firsts = [min(c for c in row if c >= 0) for row in got.tolist()]
+    assert firsts == sorted(firsts)  # bricks in the order of their first cell
+    assert (got < 0).any() and sorted(got[got >= 0].tolist()) == list(range(b.cell_count))
+    pr2 = P.poisson(3, 4, 1, rhs=False)
+    full = g.compile(pr2)
+    assert full.buckets[0].brick_cells is None
+    # the same cells in reverse order need an explicit grouping
+    rev = brick_grouping(pr2.field.space.domain.grid, full.buckets[0].cells[::-1])
+    assert rev is not None and rev.shape == (8, 8) and (rev >= 0).all()

@@ -154,3 +154,25 @@ def test_tuner_off_keeps_defaults(gpu):
             assert (t["param0"], t["param1"]) == (512, 64)
     x = P.rng_vector(plan.full_dof_count, seed=4)
     assert rel(g.apply(plan, x), pyoracle.apply(plan, x)) < TOL
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_grouped_bricks_on_ball_match_oracle(gpu, degree):
+    # the tensor cells of a finite-cell ball leave gaps in their 2x2x2 groups
+    # (cut and outside cells are not in the batch): the host grouping with
+    # empty positions puts them on the brick kernel; full, reduced and block
+    # applies against the oracle
+    pr = P.fcm_ball(n=8, degree=degree, depth=1, n_theta=12, n_phi=24)
+    plan, dev = device_plan(pr, GMF_KEEP_BRICKS=1, GMF_BRICK_MIN_CELLS=8)
+    tensor = [i for i, b in enumerate(plan.buckets) if not getattr(b, "cloud", False)]
+    assert tensor and plan.buckets[tensor[0]].brick_cells is not None
+    assert (plan.buckets[tensor[0]].brick_cells < 0).any()
+    assert dev.tuning()[tensor[0]]["kernel"] == "brick"
+    x = P.rng_vector(plan.full_dof_count, seed=21)
+    assert rel(g.apply(plan, x), pyoracle.apply(plan, x)) < TOL
+    xr = P.rng_vector(plan.reduced_dof_count, seed=22)
+    assert rel(g.reduced_apply(plan, xr), pyoracle.reduced_apply(plan, xr)) < TOL
+    X = np.stack([P.rng_vector(plan.reduced_dof_count, seed=30 + j) for j in range(5)], axis=1)
+    Y = g.reduced_apply_block(plan, X).cpu().numpy()
+    ref = np.stack([pyoracle.reduced_apply(plan, X[:, j]) for j in range(X.shape[1])], axis=1)
+    assert rel(Y, ref) < TOL
