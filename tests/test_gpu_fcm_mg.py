@@ -173,3 +173,26 @@ def test_block_vcycle_physical_hierarchy_and_fp32_fallback(gpu):
         for j in range(R.shape[1]):
             zj = M.apply(R[:, j]).cpu().numpy()
             assert np.abs(Z[:, j] - zj).max() <= tol * np.abs(zj).max(), (precision, j)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_vcycle_on_grouped_bricks_matches_oracle(gpu, monkeypatch, precision):
+    # 3^3 cells form eight partial bricks (host grouping with empty positions);
+    # bricks forced on every level, FP64 and FP32 instantiations in the V-cycle
+    monkeypatch.setenv("GMF_BRICK_MIN_CELLS", "8")
+    monkeypatch.setenv("GMF_KEEP_BRICKS", "1")
+    pr = P.poisson(3, 3, 2, kappa=P.kappa_c2, rhs=False)
+    pol = GeometricMultigridPreconditioner(smoother="chebyshev", precision=precision, pre_smoothing_steps=1,
+                                           post_smoothing_steps=1)
+    h = compile_geometric_multigrid(pr, pol)
+    assert h.level_count == 2
+    for p in h.plans:
+        assert p.buckets[0].brick_cells is not None and (p.buckets[0].brick_cells < 0).any()
+    from gricoamd import ops
+    assert ops._ws(h.plans[-1]).tuning()[0]["kernel"] == "brick"
+    M = h.device_preconditioner()
+    r = P.rng_vector(h.plans[-1].reduced_dof_count, 8)
+    z = M.apply(r).cpu().numpy()
+    ref = _oracle_vcycle(h, pol, r, use_blocks=False)
+    tol = 1e-10 if precision == "fp64" else 5e-5
+    assert np.abs(z - ref).max() <= tol * np.abs(ref).max()
